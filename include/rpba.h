@@ -1,0 +1,167 @@
+/*
+ * rpba.h -- C ABI of the MI355X robust Levenberg-Marquardt Schur-complement
+ * bundle adjustment (global "rbacore" solve of RPBA on one GPU).
+ *
+ * Every entry point takes plain pointers and sizes; the caller owns all host
+ * buffers, the library owns device memory for the duration of a call (or of a
+ * context).  Outputs are written in place, as the reference overwrites its
+ * arguments.
+ *
+ * Reference interface replaced (helmayer/RPBA):
+ *   rpba_calc_plsa   <- void CalcPLSA(MatrixXf& XX, vector<MatrixXf>& xg,
+ *                        wx, wy, wxy, vector<vector<IMAPNR>>& pointXXvio,
+ *                        vector<PMat>& PMatrices, vector<Camera>& cameras,
+ *                        iwidth, iheight, bool robustflag,
+ *                        vector<bool>& addflagsin, ostream& os, int mt,
+ *                        bool extevalflag)
+ *                       (rpba.h:693-703, rpbacore.cpp:28-531)
+ *   rpba_ctx_*       <- the same solve split into create / run / download so a
+ *                       caller can time a fixed number of LM iterations
+ *                       (LSA::LSA rbacore.cpp:37-123, LSA::Adjust 333-616,
+ *                       LSA::OutputLSAc 644-666)
+ *   rpba_read_bal / rpba_write_bal / rpba_read_parameters
+ *                    <- InputBlock / OutputBlock (rpba.cpp:180-338) and the
+ *                       `parameters` INI reader of main (rpba.cpp:53-80)
+ */
+#ifndef RPBA_MI355X_H
+#define RPBA_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPBA_ABI_VERSION 1
+
+/* Return codes.  The reference returns void and prints these conditions; the
+ * same text lines go to the log callback and the last stored state is still
+ * written back, as rpbacore.cpp:523 does after an early return from Adjust. */
+enum rpba_status {
+    RPBA_OK = 0,
+    RPBA_E_NO_OBS_IN_IMAGE = -1, /* "BA-Error: No observations in image" rbacore.cpp:414-425 */
+    RPBA_E_LOW_REDUNDANCY = -2,  /* "BA-Error: Low redundancy"           rbacore.cpp:468-476 */
+    RPBA_E_NAN_RESULT = -3,      /* "BA-Error: NANs in result vector"    rbacore.cpp:541-546 */
+    RPBA_E_INVALID = -10,        /* malformed arguments (bad index, duplicate camera in a track) */
+    RPBA_E_UNSUPPORTED = -11,    /* partitioned path (rpbacore.cpp:100-510) requested; not built */
+    RPBA_E_HIP = -20,            /* HIP runtime error */
+    RPBA_E_NOMEM = -21           /* host or device allocation failed */
+};
+
+/* One bundle block in flat structure-of-arrays form.
+ *
+ * Observations are listed in file order.  The reference's per-image lists
+ * (xg[image], wx[image] ...) and tracks pointXXvio[point] = {(image, pointnr)}
+ * are exactly what InputBlock builds from such a list (rpba.cpp:219-238):
+ * pointnr is the rank of the observation among the observations of its image,
+ * and a track lists its observations in file order. */
+typedef struct rpba_block {
+    int32_t n_cams;          /* images (= cameras; one camera per image)       */
+    int32_t n_pts;           /* points                                         */
+    int32_t n_obs;           /* observations                                   */
+    const int32_t *obs_cam;  /* [n_obs] image index; < 0 = absent (IMAPNR.image = -1) */
+    const int32_t *obs_pt;   /* [n_obs] point index                            */
+    const float *obs_xy;     /* [2*n_obs] xg: (-x_file, -y_file) per observation */
+    const float *obs_w;      /* [3*n_obs] (wx, wy, wxy) or NULL = unit weights */
+    float *cam_P;            /* [12*n_cams] row-major 3x4 [R|t] (PMat), in/out */
+    float *cam_f;            /* [n_cams] K(0,0) = K(1,1), in/out               */
+    float *cam_d;            /* [2*n_cams] distpara[0..1] (k24out-scaled), in/out */
+    float *pts;              /* [3*n_pts] point XYZ (XX rows 0..2), in/out     */
+    const int32_t *img_w;    /* [n_cams] image widths or NULL (= 1, as InputBlock sets) */
+    const int32_t *img_h;    /* [n_cams] image heights or NULL (= 1)           */
+    uint8_t *obs_kept;       /* [n_obs] out (may be NULL): 1 = still in its track */
+    uint8_t *pt_kept;        /* [n_pts] out (may be NULL): 1 = point kept      */
+} rpba_block;
+
+/* The `parameters` controls (rpba.cpp:56-64) plus the form selectors. */
+typedef struct rpba_flags {
+    int32_t robust;          /* Robust  */
+    int32_t exteval;         /* Exteval */
+    int32_t threads;         /* Threads (< 1: all host cores, rpba.cpp:88-89)   */
+    int32_t est_k2;          /* k2 -> addflags[0] */
+    int32_t est_k4;          /* k4 -> addflags[1] */
+    int32_t est_f;           /* f  -> addflags[2] */
+    int32_t k24out;          /* k24out (I/O scaling only, rpba.cpp:255-262)     */
+    int32_t precision;       /* 0 = plain FP64 (the only form built so far)     */
+    int32_t preconditioner;  /* 0 = block-Jacobi (rbacore.cpp:1838-1846)        */
+    int32_t force_global;    /* 1 = run the global LSA even where the reference
+                                would dispatch to the partitioned path          */
+} rpba_flags;
+
+/* Optional per-solve report.  sigma values are sqrt(sig0 * ihw2) exactly as the
+ * "Sigma0:" log line prints them (rbacore.cpp:583-597). */
+typedef struct rpba_stats {
+    double sigma_in[2];      /* per pass: first value of the "Sigma0:" line    */
+    double sigma_out[2];     /* per pass: second value (best stored)           */
+    int32_t redundancy[2];
+    int32_t nobs[2];
+    int32_t nunknowns[2];
+    int32_t passes_done;     /* 0, 1 or 2 */
+    int32_t itt;             /* LM iterations (the reference's itt counter)    */
+    int64_t pcg_iters;       /* total PCG iterations                           */
+    double sig0_final;       /* sig0 of the stored state (sigma0^2 / ihw2)     */
+    double rms_px;           /* sqrt(sum |r|^2 / n) over kept observations, final state */
+    int32_t n_obs_kept;
+    int32_t n_pts_kept;
+    double exteval_sigma;    /* ITTTT value (rpbacore.cpp:938)                 */
+    int64_t exteval_nobs;
+    int64_t exteval_dof;
+    /* per-iteration trace (caller-owned arrays of trace_cap entries, or NULL) */
+    int32_t trace_cap;
+    int32_t trace_len;
+    double *trace_sig0;      /* sig0 of each trial (after the wsumg/redundancy scaling) */
+    int32_t *trace_accept;   /* 1 = StoreResultsIteration ran                  */
+    int32_t *trace_pcg;      /* PCG iterations of that LM iteration            */
+    int32_t *trace_nrobs;    /* nrobs used for that iteration's redundancy     */
+} rpba_stats;
+
+typedef void (*rpba_log_fn)(const char *line, void *user);
+
+/* Whole solve (Exteval, or the global robust LM), device resident.
+ * Mirrors CalcPLSA: rpbacore.cpp:28-99 (setup, dispatch, Exteval) and
+ * 511-527 (single LSA: ctor, Adjust(robust, os, false, -1), OutputLSAc). */
+int rpba_calc_plsa(rpba_block *io, const rpba_flags *flags,
+                   rpba_log_fn log, void *user, rpba_stats *stats);
+
+/* Resumable form of the same global solve. */
+typedef struct rpba_ctx rpba_ctx;
+/* LSA::LSA + uploads (rbacore.cpp:37-123). The block must stay valid until
+ * rpba_ctx_destroy (download writes back into it). */
+int rpba_ctx_create(rpba_ctx **ctx, rpba_block *io, const rpba_flags *flags,
+                    rpba_log_fn log, void *user);
+/* Continue LSA::Adjust for at most max_iters LM iterations (max_iters < 0: to
+ * the end).  *iters_done receives the iterations run, *finished 1 when both
+ * passes are complete (or the solve stopped on an error, returned as status). */
+int rpba_ctx_run(rpba_ctx *ctx, int32_t max_iters, int32_t *iters_done, int32_t *finished);
+/* Restore the state right after rpba_ctx_create (device-to-device copies). */
+int rpba_ctx_reset(rpba_ctx *ctx);
+/* OutputLSAc (rbacore.cpp:644-666): write the stored state back to the block. */
+int rpba_ctx_download(rpba_ctx *ctx, rpba_stats *stats);
+/* Kernel timing of the last rpba_ctx_run with timing on: name -> total ms and
+ * launches, measured with HIP events on the context's stream. */
+int rpba_ctx_set_timing(rpba_ctx *ctx, int32_t on);
+int rpba_ctx_kernel_time(rpba_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
+/* Problem sizes after the point filter of PointXXvIn (rbacore.cpp:1954-1955)
+ * and the current covisibility structure: out[0..5] = cameras, points,
+ * observations, point pairs, off-diagonal S blocks, S dimension (nrioff). */
+int rpba_ctx_sizes(rpba_ctx *ctx, int64_t out[6]);
+void rpba_ctx_destroy(rpba_ctx *ctx);
+
+/* BAL text I/O (rpba.cpp:180-338), host only.
+ * rpba_read_bal allocates the arrays of *io with malloc; free with
+ * rpba_free_block.  iwidth = iheight = 1 as InputBlock sets them. */
+int rpba_read_bal(const char *path, int32_t k24out, rpba_block *io);
+int rpba_write_bal(const char *path, const rpba_block *io, int32_t k24out);
+void rpba_free_block(rpba_block *io);
+/* `parameters` INI (rpba.cpp:53-80): fills flags, input dir and file name. */
+int rpba_read_parameters(const char *path, rpba_flags *flags,
+                         char *input_dir, int32_t dir_cap,
+                         char *file, int32_t file_cap);
+
+/* Library version and device info for logs: returns RPBA_ABI_VERSION. */
+int rpba_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RPBA_MI355X_H */

@@ -1,0 +1,1447 @@
+/*
+ * rpba_oracle.c -- CPU oracle for the global robust LM bundle adjustment of
+ * helmayer/RPBA.  TEST INFRASTRUCTURE ONLY: the product (rpba_amd/, the C ABI
+ * in include/rpba.h) never links, loads or calls this file.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the
+ * checker and as the timed CPU baseline.
+ *
+ * A from-scratch C restatement of the reference's algorithm (the reference is
+ * C++ on Eigen 3, Boost and METIS, none of which exist in this image, so the
+ * reference cannot be built).  It keeps the reference's data
+ * model: compact camera blocks of the sizes camparv gives (intrinsics only for
+ * image 0, 8 for the gauge camera `normcamera`, 6+a for the others), the
+ * camera-major observation numbering `nro`, the profile-style upper block
+ * store of S with touched flags, and the reference's operation order in the
+ * per-observation formulas.
+ *
+ * Two arithmetic modes:
+ *   fsites = 1  the reference's float sites: the cost sig0 and the weight sum
+ *               wsum accumulate in float in point order and the sigma algebra
+ *               and LM comparisons run in float (rbacore.cpp:451, 487-494,
+ *               526, 555-563, 727, 776, 881, 1206, 1312-1327);
+ *   fsites = 0  plain FP64: the same sums in double, per-observation cost terms
+ *               from the double residuals.  This is the GPU's plain form.
+ * Per-observation float quantities that feed the robust labels (the key
+ * sig0rmv, the weights Wij, the medians) are float in both modes.
+ *
+ * Parity pins (tests/test_oracle_*.py): Jacobians by central finite
+ * differences; U/V/W/Schur/b by a dense numpy normal-equation build on tiny
+ * scenes; PCG against a dense Cholesky solve and a textbook replay; Cayley,
+ * rectification and angle-axis by closed forms; robust labels on constructed
+ * outliers; the LM control flow by scripted cases.  Nothing pins the exact
+ * Eigen summation order or GCC's FMA contraction of the reference build
+ * (compiled here with -ffp-contract=off).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/rpba.h"
+
+#define MAXB 9
+
+/* ------------------------------------------------------------------------- */
+/* small math                                                                */
+/* ------------------------------------------------------------------------- */
+
+/* Cayley rotation from (a,b,c); ComputeRotationMatrixfromRodrigues,
+ * rbacore.cpp:2014-2043.  Row-major R. */
+void oracle_cayley(double a, double b, double c, double *R)
+{
+    const double a2 = a * a, b2 = b * b, c2 = c * c;
+    const double ab = a * b, ac = a * c, bc = b * c;
+    double s = 4 + a2 + b2 + c2;
+    if (fabs(s) < 1.e-20)
+        s = (s < 0.) ? -1.e20 : 1.e20;
+    else
+        s = 1. / s;
+    R[0] = s * (4 + a2 - b2 - c2);
+    R[1] = s * (2 * ab - 4 * c);
+    R[2] = s * (2 * ac + 4 * b);
+    R[3] = s * (2 * ab + 4 * c);
+    R[4] = s * (4 - a2 + b2 - c2);
+    R[5] = s * (2 * bc - 4 * a);
+    R[6] = s * (2 * ac - 4 * b);
+    R[7] = s * (2 * bc + 4 * a);
+    R[8] = s * (4 - a2 - b2 + c2);
+}
+
+/* Quaternion re-orthonormalisation of a rotation; RectifyRotationMatrix3/4,
+ * rpba.h:876-960.  R row-major 3x3 with row stride `ld` (3 or 4). */
+void oracle_rectify(double *R, int ld)
+{
+#define E(r, c) R[(r) * ld + (c)]
+    double a0 = E(2, 1) - E(1, 2), a1 = E(0, 2) - E(2, 0), a2 = E(1, 0) - E(0, 1);
+    if (fabs(a0) < 1.e-20) a0 = 1.e-20;
+    if (fabs(a1) < 1.e-20) a1 = 1.e-20;
+    if (fabs(a2) < 1.e-20) a2 = 1.e-20;
+    const double al = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    a0 /= al; a1 /= al; a2 /= al;
+    const double phi = atan2(al, E(0, 0) + E(1, 1) + E(2, 2) - 1.);
+    const double q0 = cos(phi * 0.5), sh = sin(phi * 0.5);
+    const double q1 = a0 * sh, q2 = a1 * sh, q3 = a2 * sh;
+    E(0, 0) = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
+    E(0, 1) = 2 * (q1 * q2 - q0 * q3);
+    E(0, 2) = 2 * (q1 * q3 + q0 * q2);
+    E(1, 0) = 2 * (q2 * q1 + q0 * q3);
+    E(1, 1) = q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3;
+    E(1, 2) = 2 * (q2 * q3 - q0 * q1);
+    E(2, 0) = 2 * (q3 * q1 - q0 * q2);
+    E(2, 1) = 2 * (q3 * q2 + q0 * q1);
+    E(2, 2) = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+#undef E
+}
+
+/* Angle-axis -> float rotation as InputBlock does (ComputeRotationMatrixfrom-
+ * RodriguesAngle, rpba.cpp:343-366): double math, float result. */
+void oracle_aa_to_R(double a, double b, double c, float *R)
+{
+    const double a2 = a * a, b2 = b * b, c2 = c * c;
+    const double ab = a * b, ac = a * c, bc = b * c;
+    const double aa = sqrt(a2 + b2 + c2);
+    const double ct = aa == 0. ? 0.5 : (1. - cos(aa)) / (aa * aa);
+    const double st = aa == 0. ? 1 : sin(aa) / aa;
+    R[0] = (float)(1. - (b2 + c2) * ct);
+    R[1] = (float)(ab * ct - c * st);
+    R[2] = (float)(ac * ct + b * st);
+    R[3] = (float)(ab * ct + c * st);
+    R[4] = (float)(1. - (c2 + a2) * ct);
+    R[5] = (float)(bc * ct - a * st);
+    R[6] = (float)(ac * ct - b * st);
+    R[7] = (float)(bc * ct + a * st);
+    R[8] = (float)(1. - (a2 + b2) * ct);
+}
+
+/* Float rotation -> angle-axis as OutputBlock does
+ * (ComputeRodriguesAnglefromRotationMatrix, rpba.cpp:371-439). */
+void oracle_R_to_aa(const float *R, double *out)
+{
+    const double e0 = 0.01, e1 = 0.1, pi = 3.14159265358979323846;
+#define E(r, c) ((double)R[(r) * 3 + (c)])
+    const double tr = (E(0, 0) + E(1, 1) + E(2, 2) - 1.) * 0.5;
+    if (fabs(E(0, 1) - E(1, 0)) < e0 && fabs(E(1, 2) - E(2, 1)) < e0 && fabs(E(0, 2) - E(2, 0)) < e0) {
+        if (fabs(E(0, 1) + E(1, 0)) < e1 && fabs(E(1, 2) + E(2, 1)) < e1 && fabs(E(0, 2) + E(2, 0)) < e1 && tr > 0.9) {
+            out[0] = out[1] = out[2] = 0.;
+            return;
+        }
+        const double xx = (E(0, 0) + 1.) * 0.5, yy = (E(1, 1) + 1.) * 0.5, zz = (E(2, 2) + 1.) * 0.5;
+        const double xy = (E(0, 1) + E(1, 0)) * 0.25, xz = (E(0, 2) + E(2, 0)) * 0.25, yz = (E(1, 2) + E(2, 1)) * 0.25;
+        if (xx > yy && xx > zz) {
+            if (xx < e0) { out[0] = 0.; out[1] = out[2] = sqrt(0.5) * pi; return; }
+            const double t = sqrt(xx);
+            out[0] = t * pi; out[1] = xy / t * pi; out[2] = xz / t * pi;
+            return;
+        }
+        if (yy > zz) {
+            if (yy < e0) { out[0] = out[2] = sqrt(0.5) * pi; out[1] = 0.; return; }
+            const double t = sqrt(yy);
+            out[0] = xy / t * pi; out[1] = t * pi; out[2] = yz / t * pi;
+            return;
+        }
+        if (zz < e0) { out[0] = out[1] = sqrt(0.5) * pi; out[2] = 0.; return; }
+        const double t = sqrt(zz);
+        out[0] = xz / t * pi; out[1] = yz / t * pi; out[2] = t * pi;
+        return;
+    }
+    const double aa = acos(tr);
+    const double bb = 0.5 * aa / sin(aa);
+    out[0] = bb * (E(2, 1) - E(1, 2));
+    out[1] = bb * (E(0, 2) - E(2, 0));
+    out[2] = bb * (E(1, 0) - E(0, 1));
+#undef E
+}
+
+/* 3x3 inverse by cofactors (the reference calls Eigen's Matrix3d::inverse,
+ * rbacore.cpp:1344, a cofactor/adjugate formula).  Row-major. */
+void oracle_inv3(const double *m, double *r)
+{
+    /* cofactor c(i,j) = m(i1,j1) m(i2,j2) - m(i1,j2) m(i2,j1), i1=(i+1)%3 ... */
+    double c[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            c[i * 3 + j] = m[i1 * 3 + j1] * m[i2 * 3 + j2] - m[i1 * 3 + j2] * m[i2 * 3 + j1];
+        }
+    const double det = c[0] * m[0] + c[3] * m[3] + c[6] * m[6];
+    const double id = 1. / det;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r[i * 3 + j] = c[j * 3 + i] * id; /* adjugate = cofactor transpose */
+}
+
+/* n x n inverse by LU with partial (row) pivoting, the algorithm Eigen's
+ * MatrixXd::inverse() uses for dynamic sizes (rbacore.cpp:1841).
+ * Row-major, n <= MAXB. */
+void oracle_invlu(int n, const double *a_in, double *inv)
+{
+    double a[MAXB * MAXB];
+    int perm[MAXB];
+    memcpy(a, a_in, sizeof(double) * n * n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = fabs(a[k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(a[i * n + k]) > best) { best = fabs(a[i * n + k]); p = i; }
+        if (p != k) {
+            for (int j = 0; j < n; ++j) { double t = a[k * n + j]; a[k * n + j] = a[p * n + j]; a[p * n + j] = t; }
+            int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
+        }
+        const double piv = a[k * n + k];
+        if (piv != 0.)
+            for (int i = k + 1; i < n; ++i) {
+                const double l = a[i * n + k] / piv;
+                a[i * n + k] = l;
+                for (int j = k + 1; j < n; ++j) a[i * n + j] -= l * a[k * n + j];
+            }
+    }
+    /* solve L U X = P I column by column */
+    for (int col = 0; col < n; ++col) {
+        double y[MAXB];
+        for (int i = 0; i < n; ++i) {
+            double s = (perm[i] == col) ? 1. : 0.;
+            for (int j = 0; j < i; ++j) s -= a[i * n + j] * y[j];
+            y[i] = s;
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double s = y[i];
+            for (int j = i + 1; j < n; ++j) s -= a[i * n + j] * y[j];
+            y[i] = s / a[i * n + i];
+        }
+        for (int i = 0; i < n; ++i) inv[i * n + col] = y[i];
+    }
+}
+
+/* upper median of n floats: sorted[n/2] (rbacore.cpp:844-845) */
+static int cmp_float(const void *a, const void *b)
+{
+    const float x = *(const float *)a, y = *(const float *)b;
+    return (x < y) ? -1 : (x > y) ? 1 : 0;
+}
+float oracle_upper_median(const float *v, int n)
+{
+    float *t = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    memcpy(t, v, sizeof(float) * (size_t)n);
+    qsort(t, (size_t)n, sizeof(float), cmp_float);
+    const float m = t[n / 2];
+    free(t);
+    return m;
+}
+
+/* ------------------------------------------------------------------------- */
+/* solver state                                                              */
+/* ------------------------------------------------------------------------- */
+
+typedef struct orc {
+    int fsites, robust, nraddpi, addf[3];
+    int nimg, XXnr;
+    float ihw2;
+    double defdiff;
+    const float *obs_xy, *obs_w;
+    /* tracks (pointXXv after PointXXvIn) */
+    int npts, ntr;
+    int *nrpv, *toff, *timg, *tfo;
+    float *wold;             /* Wijoldv, point-major [3*ntr] */
+    /* camera-major numbering (nrobv / nrobvo) */
+    int *nrobvo, *nro;
+    float *Wij, *sig0rmv;
+    unsigned char *ISflag, *ISflagnew;
+    /* cameras: stored (PMatrices, cameras) and trial (eRMnew, ePCnew, camerasnew) */
+    double *PM;              /* [12*nimg] row-major [R|t] */
+    double *cf, *cd0, *cd1;
+    double *nf, *nd0, *nd1;
+    double *Rb, *Cb;         /* eRM / ePC, indexed by image (reference index k-1) */
+    double *Rn, *Cn;         /* eRMnew / ePCnew */
+    int normcamera, esvm[2], esvmaxm, lflflag, minimagenrdel;
+    int nrpari, nrparir, nrioff, nrioff3;
+    int *camparv;
+    double *XP, *XPn, *XX, *XXn, *x;
+    size_t xpsize;
+    /* linear system */
+    double **Sb;             /* [nimg*nimg] upper blocks k<=l, row-major bs_k x bs_l */
+    unsigned char *Sf;
+    double *ex;              /* exiv, flat [nrioff] */
+    int begini;
+    int maxtrack;
+    /* scalars */
+    double sig0, wsum, wsumg, sig0old, sig0in;
+    int imagepointnr, nrobs, nrunknowns, redundancy, nrobsnew, nrunknownsnew, redundancynew;
+    int64_t pcg_total;
+    int pcg_last;
+    int lasterr;
+} orc;
+
+#define FS(o, v) ((o)->fsites ? (double)(float)(v) : (double)(v))
+
+static int bsz(const orc *o, int i) { return o->camparv[i + 1] - o->camparv[i]; }
+
+static void orc_log(rpba_log_fn log, void *user, const char *s)
+{
+    if (log) log(s, user);
+}
+
+static void *xcalloc(size_t n, size_t s)
+{
+    void *p = calloc(n ? n : 1, s);
+    if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    return p;
+}
+
+void orc_destroy(orc *o)
+{
+    if (!o) return;
+    free(o->nrpv); free(o->toff); free(o->timg); free(o->tfo); free(o->wold);
+    free(o->nrobvo); free(o->nro); free(o->Wij); free(o->sig0rmv); free(o->ISflag); free(o->ISflagnew);
+    free(o->PM); free(o->cf); free(o->cd0); free(o->cd1); free(o->nf); free(o->nd0); free(o->nd1);
+    free(o->Rb); free(o->Cb); free(o->Rn); free(o->Cn); free(o->camparv);
+    free(o->XP); free(o->XPn); free(o->XX); free(o->XXn); free(o->x); free(o->ex);
+    if (o->Sb) {
+        for (size_t i = 0; i < (size_t)o->nimg * o->nimg; ++i) free(o->Sb[i]);
+        free(o->Sb);
+    }
+    free(o->Sf);
+    free(o);
+}
+
+/* LSA::LSA + PointXXvIn + PrepareCameras + PrepareIteration
+ * (rbacore.cpp:37-123, 1918-1974, 670-700, 127-229). */
+orc *orc_create(const rpba_block *io, const rpba_flags *f, int fsites)
+{
+    orc *o = (orc *)xcalloc(1, sizeof(orc));
+    const int nc = io->n_cams, np = io->n_pts, no = io->n_obs;
+    o->fsites = fsites;
+    o->robust = f->robust != 0;
+    o->addf[0] = f->est_k2 != 0; o->addf[1] = f->est_k4 != 0; o->addf[2] = f->est_f != 0;
+    o->nraddpi = o->addf[0] + o->addf[1] + o->addf[2];
+    o->nimg = nc;
+    o->XXnr = np;
+    o->obs_xy = io->obs_xy;
+    o->obs_w = io->obs_w;
+    /* ihw2 (rpbacore.cpp:87-90) */
+    float ihw2 = 0.f;
+    for (int i = 0; i < nc; ++i) {
+        const int w = io->img_w ? io->img_w[i] : 1, h = io->img_h ? io->img_h[i] : 1;
+        ihw2 += (float)((h + w) * (h + w)) * 0.25f;
+    }
+    ihw2 /= (float)nc;
+    o->ihw2 = ihw2;
+    o->defdiff = 5000. / sqrtf(ihw2); /* rbacore.cpp:48 */
+    o->minimagenrdel = nc > 2 ? 3 : 2; /* rbacore.cpp:73-76 */
+
+    /* tracks in file order (InputBlock rpba.cpp:219-238), then PointXXvIn */
+    int *cnt = (int *)xcalloc((size_t)np + 1, sizeof(int));
+    for (int i = 0; i < no; ++i)
+        if (io->obs_cam[i] >= 0) cnt[io->obs_pt[i] + 1]++;
+    for (int i = 0; i < np; ++i) cnt[i + 1] += cnt[i];
+    int *all = (int *)xcalloc((size_t)cnt[np] + 1, sizeof(int));
+    int *pos = (int *)xcalloc((size_t)np + 1, sizeof(int));
+    memcpy(pos, cnt, sizeof(int) * (size_t)(np + 1));
+    for (int i = 0; i < no; ++i)
+        if (io->obs_cam[i] >= 0) all[pos[io->obs_pt[i]]++] = i;
+    o->nrpv = (int *)xcalloc((size_t)np, sizeof(int));
+    o->toff = (int *)xcalloc((size_t)np + 1, sizeof(int));
+    o->timg = (int *)xcalloc((size_t)cnt[np] + 1, sizeof(int));
+    o->tfo = (int *)xcalloc((size_t)cnt[np] + 1, sizeof(int));
+    int npk = 0, nt = 0;
+    for (int i = 0; i < np; ++i) {
+        const int t = cnt[i + 1] - cnt[i];
+        const double Z = (double)io->pts[3 * i + 2];
+        if (fabs(Z) > 1.e-8 && t >= 2) { /* rbacore.cpp:1954-1955 */
+            o->nrpv[npk] = i;
+            o->toff[npk] = nt;
+            for (int j = cnt[i]; j < cnt[i + 1]; ++j) {
+                o->timg[nt] = io->obs_cam[all[j]];
+                o->tfo[nt] = all[j];
+                ++nt;
+            }
+            ++npk;
+        }
+    }
+    o->toff[npk] = nt;
+    o->npts = npk;
+    o->ntr = nt;
+    free(cnt); free(all); free(pos);
+
+    o->XX = (double *)xcalloc((size_t)3 * npk + 3, sizeof(double));
+    o->XXn = (double *)xcalloc((size_t)3 * npk + 3, sizeof(double));
+    for (int i = 0; i < npk; ++i)
+        for (int j = 0; j < 3; ++j)
+            o->XX[3 * i + j] = o->XXn[3 * i + j] = (double)io->pts[3 * o->nrpv[i] + j];
+
+    o->PM = (double *)xcalloc((size_t)12 * nc, sizeof(double));
+    for (int i = 0; i < 12 * nc; ++i) o->PM[i] = (double)io->cam_P[i];
+
+    /* PrepareCameras: gauge camera by |R_i t_i - R_0 t_0|^2 before any
+     * rectification (rbacore.cpp:670-700) */
+    {
+        double T0[3];
+        for (int r = 0; r < 3; ++r)
+            T0[r] = o->PM[r * 4 + 0] * o->PM[3] + o->PM[r * 4 + 1] * o->PM[7] + o->PM[r * 4 + 2] * o->PM[11];
+        double nd = 0.;
+        o->normcamera = (nc > 1) ? 1 : -1; /* reference leaves it unset if no distance is > 0 */
+        for (int i = 1; i < nc; ++i) {
+            const double *P = o->PM + 12 * i;
+            double d = 0.;
+            for (int r = 0; r < 3; ++r) {
+                const double T = P[r * 4 + 0] * P[3] + P[r * 4 + 1] * P[7] + P[r * 4 + 2] * P[11];
+                d += (T - T0[r]) * (T - T0[r]);
+            }
+            if (d > nd) { nd = d; o->normcamera = i; }
+        }
+    }
+
+    /* PrepareIteration (rbacore.cpp:127-229) */
+    o->camparv = (int *)xcalloc((size_t)nc + 2, sizeof(int));
+    o->Rb = (double *)xcalloc((size_t)9 * nc, sizeof(double));
+    o->Cb = (double *)xcalloc((size_t)3 * nc, sizeof(double));
+    o->Rn = (double *)xcalloc((size_t)9 * nc, sizeof(double));
+    o->Cn = (double *)xcalloc((size_t)3 * nc, sizeof(double));
+    oracle_rectify(o->PM, 4);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) o->Rn[r * 3 + c] = o->PM[r * 4 + c];
+    for (int c = 0; c < 3; ++c)
+        o->Cn[c] = o->Rn[0 * 3 + c] * o->PM[3] + o->Rn[1 * 3 + c] * o->PM[7] + o->Rn[2 * 3 + c] * o->PM[11];
+    o->nrparir = 5 + o->nraddpi;
+    o->nrpari = 6 + o->nraddpi;
+    o->cf = (double *)xcalloc((size_t)nc, sizeof(double));
+    o->cd0 = (double *)xcalloc((size_t)nc, sizeof(double));
+    o->cd1 = (double *)xcalloc((size_t)nc, sizeof(double));
+    o->nf = (double *)xcalloc((size_t)nc, sizeof(double));
+    o->nd0 = (double *)xcalloc((size_t)nc, sizeof(double));
+    o->nd1 = (double *)xcalloc((size_t)nc, sizeof(double));
+    o->lflflag = 0;
+    for (int i = 0; i < nc; ++i) {
+        o->cf[i] = o->nf[i] = (double)io->cam_f[i];
+        o->cd0[i] = o->nd0[i] = (double)io->cam_d[2 * i];
+        o->cd1[i] = o->nd1[i] = (double)io->cam_d[2 * i + 1];
+        if (io->cam_f[i] > 2.f) o->lflflag = 1; /* rbacore.cpp:213-220 */
+    }
+    o->nrioff = o->nraddpi + o->nrparir + o->nrpari * (nc - 2);
+
+    o->xpsize = (size_t)nc * 6 + (size_t)3 * np + (size_t)7 * nc + 16;
+    o->XP = (double *)xcalloc(o->xpsize, sizeof(double));
+    o->XPn = (double *)xcalloc(o->xpsize, sizeof(double));
+    o->x = (double *)xcalloc(o->xpsize, sizeof(double));
+    o->ex = (double *)xcalloc((size_t)o->nrioff + 16, sizeof(double));
+    o->Sb = (double **)xcalloc((size_t)nc * nc, sizeof(double *));
+    o->Sf = (unsigned char *)xcalloc((size_t)nc * nc, 1);
+    o->nrobvo = (int *)xcalloc((size_t)nc + 2, sizeof(int));
+    o->nro = (int *)xcalloc((size_t)o->ntr + 1, sizeof(int));
+    o->Wij = (float *)xcalloc((size_t)3 * o->ntr + 3, sizeof(float));
+    o->sig0rmv = (float *)xcalloc((size_t)o->ntr + 1, sizeof(float));
+    o->ISflag = (unsigned char *)xcalloc((size_t)o->ntr + 1, 1);
+    o->ISflagnew = (unsigned char *)xcalloc((size_t)o->ntr + 1, 1);
+    o->wold = (float *)xcalloc((size_t)3 * o->ntr + 3, sizeof(float));
+    for (int q = 0; q < o->ntr; ++q) {
+        const int fo = o->tfo[q];
+        o->wold[3 * q + 0] = o->obs_w ? o->obs_w[3 * fo + 0] : 1.f;
+        o->wold[3 * q + 1] = o->obs_w ? o->obs_w[3 * fo + 1] : 1.f;
+        o->wold[3 * q + 2] = o->obs_w ? o->obs_w[3 * fo + 2] : 0.f;
+    }
+    return o;
+}
+
+/* nrobv counts and the camera-major index of every track entry
+ * (rbacore.cpp:401-430 and the running counters of 745-757). */
+static int count_images(orc *o, rpba_log_fn log, void *user)
+{
+    const int nc = o->nimg;
+    int *cnt = o->nrobvo;
+    memset(cnt, 0, sizeof(int) * (size_t)(nc + 2));
+    o->maxtrack = 0;
+    for (int i = 0; i < o->npts; ++i) {
+        const int t = o->toff[i + 1] - o->toff[i];
+        if (t > o->maxtrack) o->maxtrack = t;
+        for (int q = o->toff[i]; q < o->toff[i + 1]; ++q) cnt[o->timg[q] + 1]++;
+    }
+    for (int i = 1; i < nc + 1; ++i)
+        if (cnt[i] == 0) {
+            if (nc > 3) {
+                char buf[128];
+                snprintf(buf, sizeof buf, "BA-Error: No observations in image %d (image index)\n", i - 1);
+                orc_log(log, user, buf);
+            }
+            return RPBA_E_NO_OBS_IN_IMAGE;
+        }
+    for (int i = 0; i < nc; ++i) cnt[i + 1] += cnt[i];
+    int *run = (int *)xcalloc((size_t)nc + 1, sizeof(int));
+    memcpy(run, cnt, sizeof(int) * (size_t)nc);
+    for (int q = 0; q < o->ntr; ++q) o->nro[q] = run[o->timg[q]]++;
+    free(run);
+    return 0;
+}
+
+/* PrepareWeights2D (rbacore.cpp:704-783). Wij in camera-major order. */
+static void prepare_weights(orc *o)
+{
+    double ws = 0.;
+    for (int q = 0; q < o->ntr; ++q) {
+        const float *w = o->wold + 3 * q; /* input weights (Wijoldv or wx/wy/wxy) */
+        float *W = o->Wij + 3 * o->nro[q];
+        W[0] = w[0]; W[1] = w[1]; W[2] = w[2];
+        ws = o->fsites ? (double)((float)ws + (w[0] + w[1])) : ws + ((double)w[0] + (double)w[1]);
+    }
+    o->wsum = ws;
+}
+
+/* EncodeCameras (rbacore.cpp:999-1087), non-GCP branch. */
+static void encode_cameras(orc *o)
+{
+    const int nc = o->nimg;
+    for (int k = 0; k < nc + 1; ++k) o->camparv[k] = 0;
+    o->camparv[1] = o->nraddpi;
+    int addp = -1;
+    if (o->addf[0]) o->XP[++addp] = o->cd0[0];
+    if (o->addf[1]) o->XP[++addp] = o->cd1[0];
+    if (o->addf[2]) o->XP[++addp] = o->cf[0];
+    for (int k = 1; k < nc; ++k) {
+        const int cp = o->camparv[k];
+        double *P = o->PM + 12 * k;
+        oracle_rectify(P, 4);
+        double *R = o->Rb + 9 * k, *C = o->Cb + 3 * k;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) R[r * 3 + c] = P[r * 4 + c];
+        for (int c = 0; c < 3; ++c) C[c] = R[0 * 3 + c] * P[3] + R[1 * 3 + c] * P[7] + R[2 * 3 + c] * P[11];
+        for (int i = 0; i < 3; ++i) o->XP[cp + i] = 0.;
+        if (k == o->normcamera) {
+            float maxv = 0.f;
+            int maxi = 0;
+            for (int i = 0; i < 3; ++i)
+                if (fabs(C[i]) > maxv) { maxv = (float)fabs(C[i]); maxi = i; }
+            int ii = -1;
+            for (int i = 0; i < 3; ++i)
+                if (maxi != i) o->esvm[++ii] = i;
+            o->esvmaxm = maxi;
+            o->XP[cp + 3] = o->XP[cp + 4] = 0.;
+            int a = 4 + cp;
+            if (o->addf[0]) o->XP[++a] = o->cd0[k];
+            if (o->addf[1]) o->XP[++a] = o->cd1[k];
+            if (o->addf[2]) o->XP[++a] = o->cf[k];
+            o->camparv[k + 1] = cp + o->nrparir;
+        } else {
+            for (int i = 0; i < 3; ++i) o->XP[cp + 3 + i] = 0.;
+            int a = 5 + cp;
+            if (o->addf[0]) o->XP[++a] = o->cd0[k];
+            if (o->addf[1]) o->XP[++a] = o->cd1[k];
+            if (o->addf[2]) o->XP[++a] = o->cf[k];
+            o->camparv[k + 1] = cp + o->nrpari;
+        }
+    }
+    o->nrioff3 = o->nrioff + 3 * o->npts;
+}
+
+/* DecodeCameras (rbacore.cpp:1733-1800), non-GCP branch: image 0 keeps its
+ * pose; every other image gets Cay(theta_rot) * base, rectified. */
+static void decode_cameras(orc *o, const double *XM)
+{
+    for (int k = 1; k < o->nimg; ++k) {
+        const int cp = o->camparv[k];
+        double Ra[9], *Rn = o->Rn + 9 * k;
+        const double *Rb = o->Rb + 9 * k;
+        oracle_cayley(XM[cp], XM[cp + 1], XM[cp + 2], Ra);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                Rn[r * 3 + c] = Ra[r * 3 + 0] * Rb[0 * 3 + c] + Ra[r * 3 + 1] * Rb[1 * 3 + c] + Ra[r * 3 + 2] * Rb[2 * 3 + c];
+        oracle_rectify(Rn, 3);
+        double e[3];
+        int a;
+        if (k == o->normcamera) {
+            e[o->esvm[0]] = XM[cp + 3];
+            e[o->esvm[1]] = XM[cp + 4];
+            e[o->esvmaxm] = 0.;
+            a = 4;
+        } else {
+            for (int i = 0; i < 3; ++i) e[i] = XM[cp + 3 + i];
+            a = 5;
+        }
+        for (int i = 0; i < 3; ++i) o->Cn[3 * k + i] = e[i] + o->Cb[3 * k + i];
+        if (o->addf[0]) o->nd0[k] = XM[++a + cp];
+        if (o->addf[1]) o->nd1[k] = XM[++a + cp];
+        if (o->addf[2]) o->nf[k] = XM[++a + cp];
+    }
+    int a = -1;
+    if (o->addf[0]) o->nd0[0] = XM[++a];
+    if (o->addf[1]) o->nd1[0] = XM[++a];
+    if (o->addf[2]) o->nf[0] = XM[++a];
+}
+
+/* One observation's projection and residual at the trial cameras and the point
+ * X (CalcCorrections rbacore.cpp:1238-1307, RadialReconstructionImage
+ * rpba.h:850-857).  Returns the nan flag. */
+typedef struct {
+    double u, v;       /* ex3(0), ex3(1) after the 1/z scaling */
+    double xs3;        /* guarded 1/e_z */
+    double su, sv;     /* ex3u: distorted normalised coordinates */
+    double diff1, diff2;
+    int nan;
+} obs_eval_t;
+
+static void eval_obs(const orc *o, int k, const double *X, const float *xy, obs_eval_t *e)
+{
+    const double *R = o->Rn + 9 * k, *C = o->Cn + 3 * k;
+    const double p0 = X[0] + C[0], p1 = X[1] + C[1], p2 = X[2] + C[2];
+    double ex0 = R[0] * p0 + R[1] * p1 + R[2] * p2;
+    double ex1 = R[3] * p0 + R[4] * p1 + R[5] * p2;
+    double xs3 = R[6] * p0 + R[7] * p1 + R[8] * p2;
+    if (fabs(xs3) < 1.e-3)
+        xs3 = (xs3 < 0.) ? -1.e3 : 1.e3;
+    else
+        xs3 = 1. / xs3;
+    ex0 *= xs3;
+    ex1 *= xs3;
+    e->u = ex0; e->v = ex1; e->xs3 = xs3;
+    const double dist = ex0 * ex0 + ex1 * ex1;
+    const double z = 1. + (o->nd0[k] * dist + o->nd1[k] * dist * dist);
+    e->su = ex0 * z;
+    e->sv = ex1 * z;
+    const double f = o->nf[k];
+    double d1 = (double)xy[0] - e->su * f;
+    double d2 = (double)xy[1] - e->sv * f;
+    if (d1 > o->defdiff) d1 = o->defdiff;
+    if (d1 < -o->defdiff) d1 = -o->defdiff;
+    if (d2 > o->defdiff) d2 = o->defdiff;
+    if (d2 < -o->defdiff) d2 = -o->defdiff;
+    e->nan = isnan(d1) || isinf(d1) || isnan(d2) || isinf(d2);
+    if (e->nan) d1 = d2 = 0.;
+    e->diff1 = d1; e->diff2 = d2;
+}
+
+/* Calibrated derivatives (CalcDerivatives rbacore.cpp:1420-1616 with
+ * eCalcCalibratedDerivativesX/3, 1620-1673).  Fills Bc (2 x n, row-major with
+ * row stride MAXB; n = the camera's compact block size) and Bp (2x3). */
+static int derivs(const orc *o, int k, const obs_eval_t *e, double *Bc, double *Bp)
+{
+    const double *R = o->Rn + 9 * k;
+    const double u = e->u, v = e->v, xs3 = e->xs3;
+    const double x1 = u * xs3, x2 = v * xs3;
+    const double xs31 = 1. / xs3, xs32 = xs31 * xs31;
+    const double k2 = o->nd0[k], k4 = o->nd1[k], f = o->nf[k];
+    const double xy2 = u * u + v * v, xy4 = xy2 * xy2;
+    const double xxpc = k4 * xy4 + u * (2 * k2 * u + 4 * k4 * u * xy2) + k2 * xy2 + 1.;
+    const double xypc = u * (4 * k4 * xy2 * v + 2 * k2 * v);
+    const double yxpc = (4 * k4 * u * xy2 + 2 * k2 * u) * v;
+    const double yypc = v * (2 * k2 * v + 4 * k4 * xy2 * v) + k4 * xy4 + k2 * xy2 + 1.;
+#define CAL(c0, c1, o0, o1)                                  \
+    do {                                                     \
+        const double xp_ = xxpc * (c0) + xypc * (c1);        \
+        const double yp_ = yypc * (c1) + yxpc * (c0);        \
+        (o0) = xp_ * f + yp_ * 0.;                           \
+        (o1) = yp_ * f;                                      \
+    } while (0)
+    /* point columns: (-R0j xs3 + R2j x1, -R1j xs3 + R2j x2), calibrated */
+    for (int j = 0; j < 3; ++j) {
+        const double c0 = -R[0 * 3 + j] * xs3 + R[2 * 3 + j] * x1;
+        const double c1 = -R[1 * 3 + j] * xs3 + R[2 * 3 + j] * x2;
+        CAL(c0, c1, Bp[j], Bp[3 + j]);
+    }
+    const double xyp2 = u * u + v * v, xyp4 = xyp2 * xyp2;
+    double id0[2], id1[2], iff[2];
+    id0[0] = -u * xyp2 * f - v * xyp2 * 0.; id0[1] = -v * xyp2 * f;
+    id1[0] = -u * xyp4 * f - v * xyp4 * 0.; id1[1] = -v * xyp4 * f;
+    iff[0] = -e->su; iff[1] = -e->sv;
+    int n = 0;
+    if (k > 0) {
+        double rc[2][3];
+        rc[0][0] = x2 * x1 * xs32;       rc[1][0] = 1. + x2 * x2 * xs32;
+        rc[0][1] = -1. - x1 * x1 * xs32; rc[1][1] = -x2 * x1 * xs32;
+        rc[0][2] = x2 * xs31;            rc[1][2] = -x1 * xs31;
+        for (int j = 0; j < 3; ++j) { CAL(rc[0][j], rc[1][j], Bc[j], Bc[MAXB + j]); }
+        n = 3;
+        if (k == o->normcamera) {
+            for (int jj = 0; jj < 2; ++jj) { Bc[n] = Bp[o->esvm[jj]]; Bc[MAXB + n] = Bp[3 + o->esvm[jj]]; ++n; }
+        } else {
+            for (int j = 0; j < 3; ++j) { Bc[n] = Bp[j]; Bc[MAXB + n] = Bp[3 + j]; ++n; }
+        }
+    }
+    if (o->addf[0]) { Bc[n] = id0[0]; Bc[MAXB + n] = id0[1]; ++n; }
+    if (o->addf[1]) { Bc[n] = id1[0]; Bc[MAXB + n] = id1[1]; ++n; }
+    if (o->addf[2]) { Bc[n] = iff[0]; Bc[MAXB + n] = iff[1]; ++n; }
+#undef CAL
+    return n;
+}
+
+/* S block (k,l), k<=l, allocated on first touch like the profile store of
+ * rbacore.cpp:1364-1389 */
+static double *sblock(orc *o, int k, int l)
+{
+    const size_t id = (size_t)k * o->nimg + l;
+    if (!o->Sb[id]) o->Sb[id] = (double *)xcalloc((size_t)MAXB * MAXB, sizeof(double));
+    o->Sf[id] = 1;
+    return o->Sb[id];
+}
+
+/* CalcCorrections (rbacore.cpp:1201-1414).  mode: 0 = cost only,
+ * 1 = linearise + Schur, 2 = back-substitution into x. */
+static void corrections(orc *o, int mode, double mult1, const double *XM, int srmv)
+{
+    double sig0 = 0.;
+    int kkk = o->nrioff - 1;
+    const int mt = o->maxtrack > 0 ? o->maxtrack : 1;
+    double *Wl = (double *)xcalloc((size_t)mt * MAXB * 3, sizeof(double)); /* eNlij */
+    int *nl = (int *)xcalloc((size_t)mt, sizeof(int));
+    for (int i = 0; i < o->npts; ++i) {
+        double V[9] = {0}, g[3] = {0};
+        double *X = o->XXn + 3 * i;
+        for (int j = 0; j < 3; ++j) X[j] = XM[o->nrioff + 3 * i + j];
+        int ks = 0;
+        for (int q = o->toff[i]; q < o->toff[i + 1]; ++q, ++ks) {
+            const int k = o->timg[q], nro = o->nro[q], fo = o->tfo[q];
+            obs_eval_t e;
+            eval_obs(o, k, X, o->obs_xy + 2 * fo, &e);
+            const float *W = o->Wij + 3 * nro;
+            if (mode > 0) {
+                double Bc[2 * MAXB], Bp[6];
+                const int n = derivs(o, k, &e, Bc, Bp);
+                nl[ks] = n;
+                double Bct[MAXB][2], Bpt[3][2];
+                for (int j = 0; j < n; ++j) {
+                    const double xp = Bc[j], yp = Bc[MAXB + j];
+                    Bct[j][0] = xp * W[0] + yp * W[2];
+                    Bct[j][1] = xp * W[2] + yp * W[1];
+                }
+                if (mode == 1 && n > 0) {
+                    double *U = sblock(o, k, k);
+                    double *gc = o->ex + o->camparv[k];
+                    for (int r = 0; r < n; ++r) {
+                        for (int c = 0; c < n; ++c) U[r * MAXB + c] += Bct[r][0] * Bc[c] + Bct[r][1] * Bc[MAXB + c];
+                        gc[r] += Bct[r][0] * e.diff1 + Bct[r][1] * e.diff2;
+                    }
+                }
+                double *Wq = Wl + (size_t)ks * MAXB * 3;
+                for (int r = 0; r < n; ++r)
+                    for (int c = 0; c < 3; ++c) Wq[r * 3 + c] = Bct[r][0] * Bp[c] + Bct[r][1] * Bp[3 + c];
+                for (int j = 0; j < 3; ++j) {
+                    const double xp = Bp[j], yp = Bp[3 + j];
+                    Bpt[j][0] = xp * W[0] + yp * W[2];
+                    Bpt[j][1] = xp * W[2] + yp * W[1];
+                }
+                for (int r = 0; r < 3; ++r) {
+                    for (int c = 0; c < 3; ++c) V[r * 3 + c] += Bpt[r][0] * Bp[c] + Bpt[r][1] * Bp[3 + c];
+                    g[r] += Bpt[r][0] * e.diff1 + Bpt[r][1] * e.diff2;
+                }
+            }
+            if (e.nan) {
+                o->sig0rmv[nro] = 1.e20f;
+            } else {
+                const float d1f = (float)e.diff1, d2f = (float)e.diff2;
+                const float d11 = d1f * d1f, d12 = 2.f * d1f * d2f, d22 = d2f * d2f;
+                const float sv = d11 * W[0] + d12 * W[2] + d22 * W[1];
+                float srv = sv;
+                if (o->robust) {
+                    const float *w0 = o->wold + 3 * q;
+                    srv = d11 * w0[0] + d12 * w0[2] + d22 * w0[1];
+                }
+                if (srmv) o->sig0rmv[nro] = srv;
+                if (o->fsites)
+                    sig0 = (double)((float)sig0 + sv);
+                else
+                    sig0 += e.diff1 * e.diff1 * (double)W[0] + 2. * e.diff1 * e.diff2 * (double)W[2] +
+                            e.diff2 * e.diff2 * (double)W[1];
+            }
+        }
+        if (mode == 0) continue;
+        /* V damping and inverse (rbacore.cpp:1341-1348) */
+        for (int j = 0; j < 3; ++j) V[j * 3 + j] *= mult1;
+        double Vi[9];
+        oracle_inv3(V, Vi);
+        const int t = o->toff[i + 1] - o->toff[i];
+        if (mode == 1) {
+            /* Schur pairs (rbacore.cpp:1350-1392) */
+            for (int is = 0; is < t; ++is) {
+                const int ij = o->timg[o->toff[i] + is];
+                const int ni = nl[is];
+                if (ni == 0) continue; /* image 0 without intrinsics: ij < begini */
+                const double *Wa = Wl + (size_t)is * MAXB * 3;
+                double N[MAXB * 3]; /* eNlijN = eNlij * V^-1 */
+                for (int r = 0; r < ni; ++r)
+                    for (int c = 0; c < 3; ++c)
+                        N[r * 3 + c] = Wa[r * 3 + 0] * Vi[0 * 3 + c] + Wa[r * 3 + 1] * Vi[1 * 3 + c] + Wa[r * 3 + 2] * Vi[2 * 3 + c];
+                double *D = sblock(o, ij, ij);
+                double *gc = o->ex + o->camparv[ij];
+                for (int r = 0; r < ni; ++r) {
+                    for (int c = 0; c < ni; ++c)
+                        D[r * MAXB + c] -= N[r * 3 + 0] * Wa[c * 3 + 0] + N[r * 3 + 1] * Wa[c * 3 + 1] + N[r * 3 + 2] * Wa[c * 3 + 2];
+                    gc[r] -= N[r * 3 + 0] * g[0] + N[r * 3 + 1] * g[1] + N[r * 3 + 2] * g[2];
+                }
+                for (int iis = is + 1; iis < t; ++iis) {
+                    const int ii = o->timg[o->toff[i] + iis];
+                    const int nb = nl[iis];
+                    if (nb == 0) continue;
+                    const double *Wb = Wl + (size_t)iis * MAXB * 3;
+                    if (ij < ii) {
+                        double *B = sblock(o, ij, ii);
+                        for (int r = 0; r < ni; ++r)
+                            for (int c = 0; c < nb; ++c)
+                                B[r * MAXB + c] -= N[r * 3 + 0] * Wb[c * 3 + 0] + N[r * 3 + 1] * Wb[c * 3 + 1] + N[r * 3 + 2] * Wb[c * 3 + 2];
+                    } else {
+                        /* ij > ii: block (ii, ij) -= eNlij[iis] * eNlijN^T; ij == ii
+                         * cannot occur (duplicate cameras in a track are rejected) */
+                        double *B = sblock(o, ii, ij);
+                        for (int r = 0; r < nb; ++r)
+                            for (int c = 0; c < ni; ++c)
+                                B[r * MAXB + c] -= Wb[r * 3 + 0] * N[c * 3 + 0] + Wb[r * 3 + 1] * N[c * 3 + 1] + Wb[r * 3 + 2] * N[c * 3 + 2];
+                    }
+                }
+            }
+        } else {
+            /* back-substitution (rbacore.cpp:1395-1410) */
+            double xp[3];
+            for (int r = 0; r < 3; ++r) xp[r] = Vi[r * 3 + 0] * g[0] + Vi[r * 3 + 1] * g[1] + Vi[r * 3 + 2] * g[2];
+            for (int is = 0; is < t; ++is) {
+                const int j = o->timg[o->toff[i] + is];
+                const int n = nl[is];
+                if (n == 0) continue;
+                const double *Wa = Wl + (size_t)is * MAXB * 3;
+                const double *xc = o->ex + o->camparv[j];
+                double T[3 * MAXB]; /* V^-1 * eNlij^T */
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < n; ++c)
+                        T[r * MAXB + c] = Vi[r * 3 + 0] * Wa[c * 3 + 0] + Vi[r * 3 + 1] * Wa[c * 3 + 1] + Vi[r * 3 + 2] * Wa[c * 3 + 2];
+                for (int r = 0; r < 3; ++r) {
+                    double s = 0.;
+                    for (int c = 0; c < n; ++c) s += T[r * MAXB + c] * xc[c];
+                    xp[r] -= s;
+                }
+            }
+            o->x[++kkk] = xp[0];
+            o->x[++kkk] = xp[1];
+            o->x[++kkk] = xp[2];
+        }
+    }
+    free(Wl);
+    free(nl);
+    o->sig0 = sig0;
+}
+
+/* y = S d over the touched blocks, both triangles (eAAdv rbacore.cpp:2048-2068) */
+static void spmv(int nb, int begini, const int *off, double *const *Sb, const unsigned char *Sf,
+                 const double *d, double *y)
+{
+    for (int i = begini; i < nb; ++i)
+        for (int j = off[i]; j < off[i + 1]; ++j) y[j] = 0.;
+    for (int l = begini; l < nb; ++l) {
+        const int nl_ = off[l + 1] - off[l];
+        for (int k = l; k >= begini; --k) {
+            const size_t id = (size_t)k * nb + l;
+            if (!Sf[id]) continue;
+            const double *B = Sb[id];
+            const int nk = off[k + 1] - off[k];
+            for (int r = 0; r < nk; ++r) {
+                double s = 0.;
+                for (int c = 0; c < nl_; ++c) s += B[r * MAXB + c] * d[off[l] + c];
+                y[off[k] + r] += s;
+            }
+            if (k != l)
+                for (int c = 0; c < nl_; ++c) {
+                    double s = 0.;
+                    for (int r = 0; r < nk; ++r) s += B[r * MAXB + c] * d[off[k] + r];
+                    y[off[l] + c] += s;
+                }
+        }
+    }
+}
+
+/* sum over blocks of the block dot products (evvdv rbacore.cpp:2072-2084) */
+static double dotb(int nb, int begini, const int *off, const double *a, const double *b)
+{
+    double c = 0.;
+    for (int i = begini; i < nb; ++i) {
+        double cc = 0.;
+        for (int j = off[i]; j < off[i + 1]; ++j) cc += b[j] * a[j];
+        c += cc;
+    }
+    return c;
+}
+
+/* Block-Jacobi PCG with the reference's step-norm stopping rule and
+ * best-iterate return (SparsePCG rbacore.cpp:1807-1914).  b in, x out.
+ * Returns the iteration count. */
+int oracle_block_pcg(int nb, int begini, const int *off, double *const *Sb, const unsigned char *Sf,
+                     const double *b, double *xout)
+{
+    const int n = off[nb];
+    double *x = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *xo = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *r = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *r1 = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *d = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *Ad = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *h = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *h1 = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *hd = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *Cinv = (double *)xcalloc((size_t)nb * MAXB * MAXB, sizeof(double));
+    /* every vector starts as a copy of b; x = 0 (rbacore.cpp:1826-1836) */
+    for (int j = 0; j < n; ++j) r[j] = r1[j] = d[j] = Ad[j] = h[j] = h1[j] = xo[j] = b[j];
+    int msize = 0;
+    for (int i = begini; i < nb; ++i) {
+        const int m = off[i + 1] - off[i];
+        double a[MAXB * MAXB];
+        for (int rr = 0; rr < m; ++rr)
+            for (int cc = 0; cc < m; ++cc) a[rr * m + cc] = Sb[(size_t)i * nb + i][rr * MAXB + cc];
+        if (m > 0) oracle_invlu(m, a, Cinv + (size_t)i * MAXB * MAXB);
+        msize += m;
+        const double *Ci = Cinv + (size_t)i * MAXB * MAXB;
+        for (int rr = 0; rr < m; ++rr) {
+            double s = 0.;
+            for (int cc = 0; cc < m; ++cc) s += Ci[rr * m + cc] * r[off[i] + cc];
+            h[off[i] + rr] = s;
+        }
+        for (int rr = 0; rr < m; ++rr) d[off[i] + rr] = h[off[i] + rr];
+    }
+    if (msize < 1000) msize = 1000;
+    double mold = 1.e50, m0 = 1.e50;
+    int k;
+    for (k = 0; k < msize; ++k) {
+        spmv(nb, begini, off, Sb, Sf, d, Ad);
+        const double rth = dotb(nb, begini, off, r, h);
+        const double dAd = dotb(nb, begini, off, d, Ad);
+        const double alpha = (fabs(rth) < 1.e-20 || fabs(rth) > 1.e20) ? 0. : rth / dAd;
+        double m = 0.;
+        for (int i = begini; i < nb; ++i) {
+            for (int j = off[i]; j < off[i + 1]; ++j) {
+                x[j] += alpha * d[j];
+                const double dr = alpha * Ad[j];
+                r1[j] = r[j] - dr;
+                m += dr * dr;
+            }
+            const int mm = off[i + 1] - off[i];
+            const double *Ci = Cinv + (size_t)i * MAXB * MAXB;
+            for (int rr = 0; rr < mm; ++rr) {
+                double s = 0.;
+                for (int cc = 0; cc < mm; ++cc) s += Ci[rr * mm + cc] * r1[off[i] + cc];
+                h1[off[i] + rr] = s;
+            }
+        }
+        if (k == 0) m0 = m;
+        if (m < mold) {
+            memcpy(xo, x, sizeof(double) * (size_t)n);
+            mold = m;
+        }
+        if (m * 1.e5 < m0) break;
+        for (int i = begini; i < nb; ++i)
+            for (int j = off[i]; j < off[i + 1]; ++j) hd[j] = h1[j] - h[j];
+        const double num = dotb(nb, begini, off, r1, hd);
+        const double den = dotb(nb, begini, off, r, h);
+        double beta = (fabs(num) < 1.e-10) ? 0. : num / den;
+        if (beta < 0.) beta = 0.;
+        for (int i = begini; i < nb; ++i)
+            for (int j = off[i]; j < off[i + 1]; ++j) {
+                d[j] = h1[j] + beta * d[j];
+                r[j] = r1[j];
+                h[j] = h1[j];
+            }
+    }
+    /* entries of blocks below begini keep the copy of b (exivold = exiv) */
+    memcpy(xout, xo, sizeof(double) * (size_t)n);
+    free(x); free(xo); free(r); free(r1); free(d); free(Ad); free(h); free(h1); free(hd); free(Cinv);
+    return (k < msize) ? k + 1 : k;
+}
+
+/* CalcBundleSep (rbacore.cpp:1094-1195) */
+static int bundle_sep(orc *o, double mult)
+{
+    const double mult1 = 1. + mult;
+    const int nc = o->nimg;
+    o->begini = (o->nraddpi > 0) ? 0 : 1;
+    for (size_t id = 0; id < (size_t)nc * nc; ++id) {
+        o->Sf[id] = 0;
+        if (o->Sb[id]) memset(o->Sb[id], 0, sizeof(double) * MAXB * MAXB);
+    }
+    memset(o->ex, 0, sizeof(double) * (size_t)o->nrioff);
+    for (int i = o->begini; i < nc; ++i) sblock(o, i, i); /* diagonal always present */
+    corrections(o, 1, mult1, o->XP, 0);
+    for (int i = o->begini; i < nc; ++i) {
+        double *D = o->Sb[(size_t)i * nc + i];
+        const int m = bsz(o, i);
+        for (int j = 0; j < m; ++j) D[j * MAXB + j] *= mult1;
+        for (int j = 0; j < m; ++j) D[j * MAXB + j] += 1.e-2;
+    }
+    double *sol = (double *)xcalloc((size_t)o->nrioff + 1, sizeof(double));
+    o->pcg_last = oracle_block_pcg(nc, o->begini, o->camparv, o->Sb, o->Sf, o->ex, sol);
+    o->pcg_total += o->pcg_last;
+    memcpy(o->ex, sol, sizeof(double) * (size_t)o->nrioff);
+    free(sol);
+    for (int j = o->begini; j < nc; ++j)
+        for (int k = o->camparv[j]; k < o->camparv[j + 1]; ++k) o->x[k] = (float)o->ex[k]; /* H3 */
+    corrections(o, 2, mult1, o->XP, 0);
+    return 0;
+}
+
+/* Reweight2D (rbacore.cpp:823-887) */
+static void reweight(orc *o, int iflag)
+{
+    const int nc = o->nimg;
+    float *key = (float *)xcalloc((size_t)o->ntr + 1, sizeof(float));
+    /* keys in camera-major order: image k occupies [nrobvo[k], nrobvo[k+1]) */
+    for (int i = 0; i < o->npts; ++i) {
+        int pxs = (o->toff[i + 1] - o->toff[i]) * 2;
+        if (pxs < 4) pxs = 4;
+        const float rf = (float)pxs / (float)(pxs - 3);
+        const float redfac = rf * rf;
+        for (int q = o->toff[i]; q < o->toff[i + 1]; ++q) key[o->nro[q]] = o->sig0rmv[o->nro[q]] * redfac;
+    }
+    float *s00 = (float *)xcalloc((size_t)nc, sizeof(float));
+    for (int k = 0; k < nc; ++k) {
+        const int b = o->nrobvo[k], n = o->nrobvo[k + 1] - b;
+        s00[k] = 1.f / (oracle_upper_median(key + b, n) * 2.1981f);
+        if (iflag > 1) s00[k] *= 0.001f;
+    }
+    double ws = 0.;
+    for (int q = 0; q < o->ntr; ++q) {
+        const int k = o->timg[q], nro = o->nro[q];
+        float ww = o->sig0rmv[nro] * s00[k];
+        o->ISflag[nro] = ww > 9.f; /* kf, rpba.h:529 */
+        if (o->ISflag[nro]) {
+            ww = 1.e-4f;
+            o->nrobs -= 2;
+        } else
+            ww = 1.f;
+        float *W = o->Wij + 3 * nro;
+        const float *w0 = o->wold + 3 * q;
+        W[0] = ww * w0[0]; W[1] = ww * w0[1]; W[2] = ww * w0[2];
+        if (W[0] + W[1] < 2.e-8f) { W[0] = W[1] = 1.e-8f; W[2] = 0.f; }
+        ws = o->fsites ? (double)((float)ws + (W[0] + W[1])) : ws + ((double)W[0] + (double)W[1]);
+    }
+    o->wsum = ws;
+    free(key);
+    free(s00);
+}
+
+/* StoreResultsIteration (rbacore.cpp:891-918) */
+static void store(orc *o)
+{
+    memcpy(o->XP, o->XPn, sizeof(double) * o->xpsize);
+    for (int k = 0; k < o->nimg; ++k) {
+        double *P = o->PM + 12 * k;
+        const double *R = o->Rn + 9 * k, *C = o->Cn + 3 * k;
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) P[r * 4 + c] = R[r * 3 + c];
+            P[r * 4 + 3] = R[r * 3 + 0] * C[0] + R[r * 3 + 1] * C[1] + R[r * 3 + 2] * C[2];
+        }
+        o->cf[k] = o->nf[k]; o->cd0[k] = o->nd0[k]; o->cd1[k] = o->nd1[k];
+    }
+    memcpy(o->XX, o->XXn, sizeof(double) * 3 * (size_t)o->npts);
+    memcpy(o->ISflagnew, o->ISflag, (size_t)o->ntr);
+    o->nrobsnew = o->nrobs;
+    o->nrunknownsnew = o->nrunknowns;
+    o->redundancynew = o->redundancy;
+}
+
+/* RobustDelete (rbacore.cpp:922-995): drop flagged observations; drop a point
+ * whose remaining track is shorter than minimagenrdel */
+static void robust_delete(orc *o)
+{
+    int *ntoff = (int *)xcalloc((size_t)o->npts + 1, sizeof(int));
+    int np2 = 0, nt2 = 0;
+    for (int i = 0; i < o->npts; ++i) {
+        int nf = 0;
+        const int t = o->toff[i + 1] - o->toff[i];
+        for (int q = o->toff[i]; q < o->toff[i + 1]; ++q) nf += o->ISflagnew[o->nro[q]] ? 1 : 0;
+        if (nf > 0 && t - nf < o->minimagenrdel) continue; /* point erased */
+        const int b = nt2;
+        for (int q = o->toff[i]; q < o->toff[i + 1]; ++q) {
+            if (nf > 0 && o->ISflagnew[o->nro[q]]) continue;
+            o->timg[nt2] = o->timg[q];
+            o->tfo[nt2] = o->tfo[q];
+            o->wold[3 * nt2 + 0] = o->wold[3 * q + 0];
+            o->wold[3 * nt2 + 1] = o->wold[3 * q + 1];
+            o->wold[3 * nt2 + 2] = o->wold[3 * q + 2];
+            ++nt2;
+        }
+        ntoff[np2] = b;
+        o->nrpv[np2] = o->nrpv[i];
+        for (int j = 0; j < 3; ++j) o->XX[3 * np2 + j] = o->XX[3 * i + j];
+        ++np2;
+    }
+    ntoff[np2] = nt2;
+    memcpy(o->toff, ntoff, sizeof(int) * (size_t)(np2 + 1));
+    free(ntoff);
+    o->npts = np2;
+    o->ntr = nt2;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Adjust (rbacore.cpp:333-616), non-GCP, itbreakflag = false                */
+/* ------------------------------------------------------------------------- */
+
+static double fsq(const orc *o, double v) { return o->fsites ? (double)sqrtf((float)v) : sqrt(v); }
+
+int orc_adjust(orc *o, rpba_log_fn log, void *user, rpba_stats *st)
+{
+    const int robust = o->robust;
+    int itt = 0, ittflag = 1;
+    const double frac = o->lflflag ? (double)1.01f : (double)1.05f;
+    double mult = 1.e-3;
+    int rc = 0;
+    char buf[512];
+    if (st) { st->passes_done = 0; st->trace_len = 0; }
+    for (int it = 0; it < 2; ++it) {
+        o->sig0old = 1.e20f;
+        if (it > 0) mult = 1.e-6;
+        o->imagepointnr = o->ntr;
+        o->nrobs = 2 * o->imagepointnr;
+        rc = count_images(o, log, user);
+        if (rc) break;
+        if (ittflag) {
+            memset(o->ISflag, 0, (size_t)o->ntr);
+            memset(o->ISflagnew, 0, (size_t)o->ntr);
+            prepare_weights(o);
+            ittflag = 0;
+        }
+        o->wsum = o->fsites ? (double)((float)(2 * o->imagepointnr) / (float)o->wsum) : (double)(2 * o->imagepointnr) / o->wsum;
+        o->wsumg = o->wsum;
+        for (int i = 0; i < o->npts; ++i)
+            for (int j = 0; j < 3; ++j) o->XP[o->nrioff + 3 * i + j] = o->XX[3 * i + j];
+        encode_cameras(o);
+        o->nrunknowns = o->nrioff3 + o->nraddpi * o->nimg; /* H7 */
+        memcpy(o->XXn, o->XX, sizeof(double) * 3 * (size_t)o->npts);
+        memset(o->x, 0, sizeof(double) * o->xpsize);
+        o->redundancy = o->nrobs - o->nrunknowns;
+        if (o->redundancy < 0) {
+            if (o->nimg > 3) orc_log(log, user, "BA-Error: Low redundancy\n");
+            rc = RPBA_E_LOW_REDUNDANCY;
+            break;
+        }
+        if (it == 0) {
+            decode_cameras(o, o->XP);
+            corrections(o, 0, 1. + mult, o->XP, 1);
+            o->sig0 = FS(o, FS(o, o->sig0 * o->wsumg) / (double)(float)o->redundancy);
+            o->sig0in = o->sig0old = o->sig0;
+        } else {
+            o->sig0in = o->sig0old;
+        }
+        int iflag = robust ? 2 : 1;
+        int iter = 1;
+        const int nrunknownsgiven = o->nrunknowns, nrobsgiven = o->nrobs;
+        while (iflag > 0) {
+            if (it == 0 && robust) {
+                o->nrunknowns = nrunknownsgiven;
+                o->nrobs = nrobsgiven;
+                reweight(o, iflag);
+                o->redundancy = o->nrobs - o->nrunknowns;
+                o->wsum = o->fsites ? (double)((float)(2 * o->imagepointnr) / (float)o->wsum) : (double)(2 * o->imagepointnr) / o->wsum;
+                o->wsumg = o->wsum;
+            }
+            bundle_sep(o, mult);
+            for (int i = 0; i < o->nrunknowns; ++i) {
+                if (isnan(o->x[i]) || isinf(o->x[i])) {
+                    snprintf(buf, sizeof buf, "BA-Error: NANs in result vector %d %g %d %d\n", i, o->x[i], o->nrioff, o->nrioff3);
+                    orc_log(log, user, buf);
+                    rc = RPBA_E_NAN_RESULT;
+                    break;
+                }
+                o->XPn[i] = o->XP[i] - o->x[i];
+            }
+            if (rc) break;
+            decode_cameras(o, o->XPn);
+            corrections(o, 0, 1. + mult, o->XPn, 1);
+            o->sig0 = FS(o, FS(o, o->sig0 * o->wsumg) / (double)(float)o->redundancy);
+            int acc = 0;
+            if (o->sig0old < FS(o, o->sig0 * frac)) --iflag;
+            if (o->sig0 <= o->sig0old || iter == 1) {
+                store(o);
+                o->sig0old = o->sig0;
+                acc = 1;
+            }
+            if (st && st->trace_sig0 && st->trace_len < st->trace_cap) {
+                st->trace_sig0[st->trace_len] = o->sig0;
+                if (st->trace_accept) st->trace_accept[st->trace_len] = acc;
+                if (st->trace_pcg) st->trace_pcg[st->trace_len] = o->pcg_last;
+                if (st->trace_nrobs) st->trace_nrobs[st->trace_len] = o->nrobs;
+                st->trace_len++;
+            }
+            if (itt > 200) break;
+            ++itt;
+            ++iter;
+        }
+        if (rc) break;
+        const double s_in = fsq(o, FS(o, o->sig0in * (double)o->ihw2));
+        const double s_out = fsq(o, FS(o, o->sig0old * (double)o->ihw2));
+        int red, nob, nun;
+        if (it == 0 && robust) { red = o->redundancynew; nob = o->nrobsnew; nun = o->nrunknownsnew; }
+        else { red = o->redundancy; nob = o->nrobs; nun = o->nrunknowns; }
+        snprintf(buf, sizeof buf, "Sigma0: %g %g  %d %d %d  IT: %d  ITT: %d\n", s_in, s_out, red, nob, nun, it, itt);
+        orc_log(log, user, buf);
+        if (st) {
+            st->sigma_in[it] = s_in;
+            st->sigma_out[it] = s_out;
+            st->redundancy[it] = red;
+            st->nobs[it] = nob;
+            st->nunknowns[it] = nun;
+            st->passes_done = it + 1;
+        }
+        if (robust && it == 0) robust_delete(o);
+        ittflag = 1;
+    }
+    if (st) {
+        st->itt = itt;
+        st->pcg_iters = o->pcg_total;
+        st->sig0_final = o->sig0old;
+    }
+    o->lasterr = rc;
+    return rc;
+}
+
+/* OutputLSAc + PointXXvOut (rbacore.cpp:644-666, 1978-2010).  Camera
+ * intrinsics come from the last trial (camerasnew), R|t and X from the stored
+ * state. */
+void orc_output(orc *o, rpba_block *io)
+{
+    if (io->obs_kept) memset(io->obs_kept, 0, (size_t)io->n_obs);
+    if (io->pt_kept) memset(io->pt_kept, 0, (size_t)io->n_pts);
+    for (int i = 0; i < o->npts; ++i) {
+        const int p = o->nrpv[i];
+        for (int j = 0; j < 3; ++j) io->pts[3 * p + j] = (float)o->XX[3 * i + j];
+        if (io->pt_kept) io->pt_kept[p] = 1;
+        for (int q = o->toff[i]; q < o->toff[i + 1]; ++q)
+            if (io->obs_kept) io->obs_kept[o->tfo[q]] = 1;
+    }
+    for (int k = 0; k < o->nimg; ++k) {
+        for (int j = 0; j < 12; ++j) io->cam_P[12 * k + j] = (float)o->PM[12 * k + j];
+        io->cam_f[k] = (float)o->nf[k];
+        io->cam_d[2 * k] = (float)o->nd0[k];
+        io->cam_d[2 * k + 1] = (float)o->nd1[k];
+    }
+}
+
+/* EvalLSA + CalcCorrPointMulti (rpbacore.cpp:882-941, 778-837) on the given
+ * parameters; returns sqrt(sig0g*ihw2) ("ITTTT:" line). */
+double oracle_exteval(const rpba_block *io, const rpba_flags *f, int fsites, int64_t *nobs_out, int64_t *dof_out)
+{
+    const int nc = io->n_cams, np = io->n_pts, no = io->n_obs;
+    const int nraddp = (f->est_k2 != 0) + (f->est_k4 != 0) + (f->est_f != 0);
+    float ihw2 = 0.f;
+    for (int i = 0; i < nc; ++i) {
+        const int w = io->img_w ? io->img_w[i] : 1, h = io->img_h ? io->img_h[i] : 1;
+        ihw2 += (float)((h + w) * (h + w)) * 0.25f;
+    }
+    ihw2 /= (float)nc;
+    int *cnt = (int *)xcalloc((size_t)np + 1, sizeof(int));
+    for (int i = 0; i < no; ++i)
+        if (io->obs_cam[i] >= 0) cnt[io->obs_pt[i] + 1]++;
+    for (int i = 0; i < np; ++i) cnt[i + 1] += cnt[i];
+    int *all = (int *)xcalloc((size_t)cnt[np] + 1, sizeof(int));
+    int *pos = (int *)xcalloc((size_t)np + 1, sizeof(int));
+    memcpy(pos, cnt, sizeof(int) * (size_t)(np + 1));
+    for (int i = 0; i < no; ++i)
+        if (io->obs_cam[i] >= 0) all[pos[io->obs_pt[i]]++] = i;
+    long red = 0, nobs = 0;
+    double sg = 0.;
+    for (int i = 0; i < np; ++i) {
+        const int t = cnt[i + 1] - cnt[i];
+        if (t <= 1) continue;
+        double ws = 0.;
+        for (int j = cnt[i]; j < cnt[i + 1]; ++j) {
+            const int q = all[j];
+            const float wx = io->obs_w ? io->obs_w[3 * q] : 1.f, wy = io->obs_w ? io->obs_w[3 * q + 1] : 1.f;
+            ws = fsites ? (double)((float)ws + (wx + wy)) : ws + ((double)wx + (double)wy);
+        }
+        ws = fsites ? (double)((float)ws / (float)(2 * t)) : ws / (double)(2 * t);
+        const double X0 = io->pts[3 * i], X1 = io->pts[3 * i + 1], X2 = io->pts[3 * i + 2];
+        double s = 0.;
+        for (int j = cnt[i]; j < cnt[i + 1]; ++j) {
+            const int q = all[j], c = io->obs_cam[q];
+            const float *P = io->cam_P + 12 * c;
+            double p1 = (double)P[0] * X0 + (double)P[1] * X1 + (double)P[2] * X2 + (double)P[3];
+            double p2 = (double)P[4] * X0 + (double)P[5] * X1 + (double)P[6] * X2 + (double)P[7];
+            double p3 = (double)P[8] * X0 + (double)P[9] * X1 + (double)P[10] * X2 + (double)P[11];
+            if (fabs(p3) < 1.e-3)
+                p3 = (p3 > 0.) ? 1.e3 : -1.e3;
+            else
+                p3 = 1. / p3;
+            p1 *= p3;
+            p2 *= p3;
+            const double dist = p1 * p1 + p2 * p2;
+            const double z = 1. + ((double)io->cam_d[2 * c] * dist + (double)io->cam_d[2 * c + 1] * dist * dist);
+            p1 *= z;
+            p2 *= z;
+            p1 *= (double)io->cam_f[c];
+            p2 *= (double)io->cam_f[c];
+            const double dx = (double)io->obs_xy[2 * q] - p1, dy = (double)io->obs_xy[2 * q + 1] - p2;
+            const float wx = io->obs_w ? io->obs_w[3 * q] : 1.f, wy = io->obs_w ? io->obs_w[3 * q + 1] : 1.f;
+            const float wxy = io->obs_w ? io->obs_w[3 * q + 2] : 0.f;
+            const double e = dx * dx * wx + dy * dy * wy + 2 * dx * dy * wxy;
+            s = fsites ? (double)((float)s + (float)e) : s + e;
+        }
+        sg = fsites ? (double)((float)sg + (float)s / (float)ws) : sg + s / ws;
+        red += 2 * t - 3;
+        nobs += 2 * t;
+    }
+    const long dof = red - (long)(nc - 2) * 6 - 5 - (long)nc * nraddp;
+    sg = fsites ? (double)((float)sg * (1.f / (float)dof)) : sg * (1. / (double)dof);
+    free(cnt); free(all); free(pos);
+    if (nobs_out) *nobs_out = nobs;
+    if (dof_out) *dof_out = dof;
+    return fsites ? (double)sqrtf((float)sg * ihw2) : sqrt(sg * (double)ihw2);
+}
+
+/* CalcPLSA restated for the paths this oracle covers (rpbacore.cpp:28-99,
+ * 511-527). */
+int oracle_calc_plsa(rpba_block *io, const rpba_flags *f, int fsites, rpba_log_fn log, void *user, rpba_stats *st)
+{
+    if (f->exteval) {
+        int64_t nobs = 0, dof = 0;
+        const double s = oracle_exteval(io, f, fsites, &nobs, &dof);
+        char buf[160];
+        snprintf(buf, sizeof buf, "ITTTT: %g %lld %lld\n", s, (long long)nobs, (long long)dof);
+        orc_log(log, user, buf);
+        if (st) { st->exteval_sigma = s; st->exteval_nobs = nobs; st->exteval_dof = dof; }
+        return 0;
+    }
+    orc *o = orc_create(io, f, fsites);
+    const int rc = orc_adjust(o, log, user, st);
+    orc_output(o, io);
+    if (st) {
+        st->n_pts_kept = o->npts;
+        st->n_obs_kept = o->ntr;
+    }
+    orc_destroy(o);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* test hooks                                                                */
+/* ------------------------------------------------------------------------- */
+
+/* Pass-0 setup of Adjust up to the initial cost (rbacore.cpp:370-499) so the
+ * tests can probe one linearisation. */
+int orc_setup_pass0(orc *o)
+{
+    o->imagepointnr = o->ntr;
+    o->nrobs = 2 * o->ntr;
+    int rc = count_images(o, NULL, NULL);
+    if (rc) return rc;
+    memset(o->ISflag, 0, (size_t)o->ntr);
+    memset(o->ISflagnew, 0, (size_t)o->ntr);
+    prepare_weights(o);
+    o->wsum = (double)(2 * o->imagepointnr) / o->wsum;
+    o->wsumg = o->wsum;
+    for (int i = 0; i < o->npts; ++i)
+        for (int j = 0; j < 3; ++j) o->XP[o->nrioff + 3 * i + j] = o->XX[3 * i + j];
+    encode_cameras(o);
+    o->nrunknowns = o->nrioff3 + o->nraddpi * o->nimg;
+    memcpy(o->XXn, o->XX, sizeof(double) * 3 * (size_t)o->npts);
+    decode_cameras(o, o->XP);
+    corrections(o, 0, 1.001, o->XP, 1);
+    return 0;
+}
+
+/* dims: nimg, npts, ntr, nrioff, nrioff3, normcamera, esvmaxm, nraddpi */
+void orc_dims(const orc *o, int *out)
+{
+    out[0] = o->nimg; out[1] = o->npts; out[2] = o->ntr; out[3] = o->nrioff;
+    out[4] = o->nrioff3; out[5] = o->normcamera; out[6] = o->esvmaxm; out[7] = o->nraddpi;
+}
+
+void orc_camparv(const orc *o, int *out)
+{
+    for (int i = 0; i <= o->nimg; ++i) out[i] = o->camparv[i];
+}
+
+void orc_get_theta(const orc *o, double *theta) { memcpy(theta, o->XP, sizeof(double) * (size_t)o->nrioff3); }
+
+/* residuals (point-major track order) and float keys at parameters theta;
+ * the trial cameras are left at theta (like the reference's DecodeCameras) */
+void orc_residuals(orc *o, const double *theta, double *r, float *key)
+{
+    decode_cameras(o, theta);
+    corrections(o, 0, 1.001, theta, 1);
+    for (int i = 0; i < o->npts; ++i) {
+        const double *X = theta + o->nrioff + 3 * i;
+        for (int q = o->toff[i]; q < o->toff[i + 1]; ++q) {
+            obs_eval_t e;
+            eval_obs(o, o->timg[q], X, o->obs_xy + 2 * o->tfo[q], &e);
+            r[2 * q] = e.diff1;
+            r[2 * q + 1] = e.diff2;
+            if (key) key[q] = o->sig0rmv[o->nro[q]];
+        }
+    }
+    o->sig0 = o->sig0;
+}
+
+/* per-track-entry Jacobians at the current trial cameras and points XXn:
+ * Bc [ntr][2][MAXB] (compact camera columns), Bp [ntr][2][3] */
+void orc_jacobians(orc *o, double *Bc, double *Bp, int *ncols)
+{
+    for (int i = 0; i < o->npts; ++i)
+        for (int q = o->toff[i]; q < o->toff[i + 1]; ++q) {
+            obs_eval_t e;
+            eval_obs(o, o->timg[q], o->XXn + 3 * i, o->obs_xy + 2 * o->tfo[q], &e);
+            double bc[2 * MAXB] = {0}, bp[6];
+            ncols[q] = derivs(o, o->timg[q], &e, bc, bp);
+            memcpy(Bc + (size_t)q * 2 * MAXB, bc, sizeof bc);
+            memcpy(Bp + (size_t)q * 6, bp, sizeof bp);
+        }
+}
+
+/* One linearisation + Schur at the current state (first half of CalcBundleSep
+ * with the S damping): dense S [nrioff^2], b [nrioff]. */
+void orc_linearize(orc *o, double mult, double *Sd, double *b)
+{
+    const int nc = o->nimg, n = o->nrioff;
+    const double mult1 = 1. + mult;
+    o->begini = (o->nraddpi > 0) ? 0 : 1;
+    for (size_t id = 0; id < (size_t)nc * nc; ++id) {
+        o->Sf[id] = 0;
+        if (o->Sb[id]) memset(o->Sb[id], 0, sizeof(double) * MAXB * MAXB);
+    }
+    memset(o->ex, 0, sizeof(double) * (size_t)n);
+    for (int i = o->begini; i < nc; ++i) sblock(o, i, i);
+    corrections(o, 1, mult1, o->XP, 0);
+    for (int i = o->begini; i < nc; ++i) {
+        double *D = o->Sb[(size_t)i * nc + i];
+        const int m = bsz(o, i);
+        for (int j = 0; j < m; ++j) D[j * MAXB + j] *= mult1;
+        for (int j = 0; j < m; ++j) D[j * MAXB + j] += 1.e-2;
+    }
+    memset(Sd, 0, sizeof(double) * (size_t)n * n);
+    for (int k = 0; k < nc; ++k)
+        for (int l = k; l < nc; ++l) {
+            const size_t id = (size_t)k * nc + l;
+            if (!o->Sf[id]) continue;
+            for (int r = 0; r < bsz(o, k); ++r)
+                for (int c = 0; c < bsz(o, l); ++c) {
+                    const double v = o->Sb[id][r * MAXB + c];
+                    Sd[(size_t)(o->camparv[k] + r) * n + o->camparv[l] + c] = v;
+                    Sd[(size_t)(o->camparv[l] + c) * n + o->camparv[k] + r] = v;
+                }
+        }
+    memcpy(b, o->ex, sizeof(double) * (size_t)n);
+}
+
+/* One full CalcBundleSep at the current state: x [nrioff3]; returns PCG its */
+int orc_step(orc *o, double mult, double *x)
+{
+    bundle_sep(o, mult);
+    memcpy(x, o->x, sizeof(double) * (size_t)o->nrioff3);
+    return o->pcg_last;
+}
+
+/* PCG on a dense symmetric matrix cut into blocks of the given sizes */
+int oracle_pcg_dense(int nb, const int *bs, const double *Sd, const double *b, double *x)
+{
+    int *off = (int *)xcalloc((size_t)nb + 1, sizeof(int));
+    for (int i = 0; i < nb; ++i) off[i + 1] = off[i] + bs[i];
+    const int n = off[nb];
+    double **Sb = (double **)xcalloc((size_t)nb * nb, sizeof(double *));
+    unsigned char *Sf = (unsigned char *)xcalloc((size_t)nb * nb, 1);
+    for (int k = 0; k < nb; ++k)
+        for (int l = k; l < nb; ++l) {
+            int nz = (k == l);
+            for (int r = 0; r < bs[k]; ++r)
+                for (int c = 0; c < bs[l]; ++c)
+                    if (Sd[(size_t)(off[k] + r) * n + off[l] + c] != 0.) nz = 1;
+            if (!nz) continue;
+            double *B = (double *)xcalloc(MAXB * MAXB, sizeof(double));
+            for (int r = 0; r < bs[k]; ++r)
+                for (int c = 0; c < bs[l]; ++c) B[r * MAXB + c] = Sd[(size_t)(off[k] + r) * n + off[l] + c];
+            Sb[(size_t)k * nb + l] = B;
+            Sf[(size_t)k * nb + l] = 1;
+        }
+    const int it = oracle_block_pcg(nb, 0, off, Sb, Sf, b, x);
+    for (size_t i = 0; i < (size_t)nb * nb; ++i) free(Sb[i]);
+    free(Sb); free(Sf); free(off);
+    return it;
+}
