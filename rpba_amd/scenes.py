@@ -50,14 +50,11 @@ def rodrigues(aa):
 
 
 def rot_to_aa(R):
-    """Rotation matrices -> angle-axis (no angles near pi in these scenes)."""
+    """Rotation matrices -> angle-axis, robust at every angle (quaternion
+    based, via scipy)."""
+    from scipy.spatial.transform import Rotation
     R = np.asarray(R, dtype=np.float64).reshape(-1, 3, 3)
-    tr = np.clip((np.trace(R, axis1=1, axis2=2) - 1.0) * 0.5, -1.0, 1.0)
-    th = np.arccos(tr)
-    w = np.stack([R[:, 2, 1] - R[:, 1, 2], R[:, 0, 2] - R[:, 2, 0], R[:, 1, 0] - R[:, 0, 1]], axis=1)
-    s = np.sin(th)
-    k = np.where(th < 1e-12, 0.5, th / (2.0 * np.where(s == 0.0, 1.0, s)))
-    return w * k[:, None]
+    return Rotation.from_matrix(R).as_rotvec()
 
 
 def _look_at_origin(centers):
