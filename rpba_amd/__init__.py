@@ -1,0 +1,58 @@
+"""MI355X-native robust LM Schur-complement bundle adjustment (the global
+"rbacore" solve of helmayer/RPBA) behind the C ABI of include/rpba.h.
+
+``lib()`` loads the in-tree HIP library and fails loudly when it is missing:
+there is no CPU fallback in the product path.
+"""
+import ctypes as C
+import os
+
+from . import capi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librpba_hip.so")
+
+# every symbol include/rpba.h declares
+EXPORTS = [
+    "rpba_calc_plsa", "rpba_ctx_create", "rpba_ctx_run", "rpba_ctx_reset", "rpba_ctx_download",
+    "rpba_ctx_set_timing", "rpba_ctx_kernel_time", "rpba_ctx_sizes", "rpba_ctx_destroy",
+    "rpba_read_bal", "rpba_write_bal", "rpba_free_block", "rpba_read_parameters", "rpba_version",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library {LIB_PATH} is not built; run `python -m rpba_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    L.rpba_calc_plsa.argtypes = [P(capi.RpbaBlock), P(capi.RpbaFlags), capi.LOG_FN, C.c_void_p, P(capi.RpbaStats)]
+    L.rpba_calc_plsa.restype = C.c_int
+    L.rpba_ctx_create.argtypes = [P(C.c_void_p), P(capi.RpbaBlock), P(capi.RpbaFlags), capi.LOG_FN, C.c_void_p]
+    L.rpba_ctx_create.restype = C.c_int
+    L.rpba_ctx_run.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32)]
+    L.rpba_ctx_run.restype = C.c_int
+    L.rpba_ctx_reset.argtypes = [C.c_void_p]
+    L.rpba_ctx_reset.restype = C.c_int
+    L.rpba_ctx_download.argtypes = [C.c_void_p, P(capi.RpbaStats)]
+    L.rpba_ctx_download.restype = C.c_int
+    L.rpba_ctx_set_timing.argtypes = [C.c_void_p, C.c_int32]
+    L.rpba_ctx_kernel_time.argtypes = [C.c_void_p, C.c_char_p, P(C.c_double), P(C.c_int64)]
+    L.rpba_ctx_sizes.argtypes = [C.c_void_p, P(C.c_int64)]
+    L.rpba_ctx_destroy.argtypes = [C.c_void_p]
+    L.rpba_ctx_destroy.restype = None
+    L.rpba_read_bal.argtypes = [C.c_char_p, C.c_int32, P(capi.RpbaBlock)]
+    L.rpba_read_bal.restype = C.c_int
+    L.rpba_write_bal.argtypes = [C.c_char_p, P(capi.RpbaBlock), C.c_int32]
+    L.rpba_write_bal.restype = C.c_int
+    L.rpba_free_block.argtypes = [P(capi.RpbaBlock)]
+    L.rpba_free_block.restype = None
+    L.rpba_read_parameters.argtypes = [C.c_char_p, P(capi.RpbaFlags), C.c_char_p, C.c_int32, C.c_char_p, C.c_int32]
+    L.rpba_read_parameters.restype = C.c_int
+    L.rpba_version.restype = C.c_int
+    _lib = L
+    return L

@@ -1,0 +1,72 @@
+"""Python mirror of the reference's solve entry point over the C ABI.
+
+``calc_plsa`` follows ``CalcPLSA`` (rpba.h:693-703, rpbacore.cpp:28-531): it
+takes a block (cameras, points, observations in file order), overwrites the
+cameras and points with the adjusted values, marks the observations and
+points that survive the robust deletion, and returns the status together
+with the log lines the reference writes to its ``os`` stream.
+"""
+import ctypes as C
+
+from . import capi, lib
+
+
+def calc_plsa(scene, flags=None, trace_cap=512):
+    """Returns (status, outputs, stats, log lines)."""
+    flags = flags or capi.make_flags()
+    blk = capi.Block(scene)
+    tr = capi.Trace(trace_cap)
+    logc = capi.LogCollector()
+    rc = lib().rpba_calc_plsa(C.byref(blk.c), C.byref(flags), logc.fn, None, C.byref(tr.stats))
+    return rc, blk.result(), tr.as_dict(), logc.lines
+
+
+class Context:
+    """Resumable solve (rpba_ctx_*): create uploads the block, run advances
+    the LM iterations, download writes the stored state back."""
+
+    def __init__(self, scene, flags=None):
+        self.flags = flags or capi.make_flags()
+        self.blk = capi.Block(scene)
+        self.logc = capi.LogCollector()
+        self.h = C.c_void_p()
+        rc = lib().rpba_ctx_create(C.byref(self.h), C.byref(self.blk.c), C.byref(self.flags), self.logc.fn, None)
+        if rc != 0:
+            raise RuntimeError(f"rpba_ctx_create failed with status {rc}")
+
+    def run(self, max_iters=-1):
+        done, fin = C.c_int32(), C.c_int32()
+        rc = lib().rpba_ctx_run(self.h, max_iters, C.byref(done), C.byref(fin))
+        return rc, done.value, bool(fin.value)
+
+    def reset(self):
+        return lib().rpba_ctx_reset(self.h)
+
+    def set_timing(self, on):
+        return lib().rpba_ctx_set_timing(self.h, 1 if on else 0)
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(), C.c_int64()
+        lib().rpba_ctx_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def sizes(self):
+        out = (C.c_int64 * 6)()
+        lib().rpba_ctx_sizes(self.h, out)
+        return dict(zip(["cams", "pts", "obs", "pairs", "blocks", "sdim"], list(out)))
+
+    def download(self, trace_cap=512):
+        tr = capi.Trace(trace_cap)
+        rc = lib().rpba_ctx_download(self.h, C.byref(tr.stats))
+        return rc, self.blk.result(), tr.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().rpba_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

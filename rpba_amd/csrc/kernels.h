@@ -1,0 +1,103 @@
+// Kernel argument blocks and launchers of kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace rpba {
+
+// device-resident camera state, all double, indexed by image
+struct CamPtrs {
+    double *Rb, *Cb;    // base rotation / centre offset of the pass (eRM, ePC)
+    double *Rn, *Cn;    // trial rotation / C (eRMnew, ePCnew)
+    double *In;         // trial intrinsics d0, d1, f (camerasnew)
+    double *th, *thn;   // accepted / trial parameters, 9 slots (XPMatrices, XPMatricesnew)
+    double *Pst;        // stored [R|t] (PMatrices)
+    double *Ist;        // stored intrinsics (cameras)
+    unsigned *mask;     // bit j set: slot j is estimated
+};
+
+struct PcgScalars {
+    double rho, m0, mold;
+    int k, done, iters, nan_flag;
+};
+
+struct LinArgs {
+    int nc, npts, ntr;
+    double defdiff, mult1;
+    CamPtrs cam;
+    const int *toff, *tcam, *tpt, *camoff, *camq;
+    const float *ox, *oy, *wx, *wy, *wxy;
+    const double *X;
+    double *Wo, *Yo, *Vinv, *gp, *Sd, *bvec;
+};
+
+struct SchurArgs {
+    int nblk;
+    const int *poff, *pa, *pb;
+    const double *Yo, *Wo;
+    double *Su;
+};
+
+struct PcgArgs {
+    int nc, msize;
+    const double *Sd, *Su, *Cinv, *b;
+    const int *row_off, *col_off, *col_blk, *blk_k, *blk_l;
+    double *x, *best, *r, *d, *Ad, *h, *h1, *rowdot;
+    PcgScalars *S;
+};
+
+struct StepArgs {
+    int nc, npts;
+    CamPtrs cam;
+    const int *toff, *tcam;
+    const double *Wo, *Vinv, *gp, *sol, *X;
+    double *Xn;
+    PcgScalars *S;
+};
+
+struct CostArgs {
+    int ntr, robust, write_key;
+    double defdiff;
+    CamPtrs cam;
+    const int *tcam, *tpt;
+    const float *ox, *oy, *wx, *wy, *wxy, *w0x, *w0y, *w0xy;
+    const double *X;
+    float *key;
+    double *partial;
+};
+
+struct RobustArgs {
+    int nc, ntr, iflag;
+    const int *camoff, *camq, *tpt, *toff, *tcam;
+    const float *key, *w0x, *w0y, *w0xy;
+    float *s00, *wx, *wy, *wxy;
+    unsigned char *flag;
+    double *partial;
+    int *ipartial;
+};
+
+struct ExtArgs {
+    int npts;
+    const int *toff, *cam;
+    const float *ox, *oy, *wx, *wy, *wxy, *P, *cf, *cd, *pts;
+    double *partial;
+    int *ipartial;
+};
+
+void launch_lin(const LinArgs &a, const SchurArgs &s, hipStream_t st);
+void launch_lin_points(const LinArgs &a, hipStream_t st);
+void launch_schur_pairs(const SchurArgs &s, hipStream_t st);
+void launch_lin_cameras(const LinArgs &a, hipStream_t st);
+void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st);
+void launch_pcg_init(const PcgArgs &a, hipStream_t st);
+void launch_spmv(const PcgArgs &a, hipStream_t st);
+void launch_pcg_update(const PcgArgs &a, hipStream_t st);
+void launch_backsub(const StepArgs &a, hipStream_t st);
+void launch_decode(const StepArgs &a, hipStream_t st);
+int cost_blocks(int ntr);
+void launch_cost(const CostArgs &a, double *out, hipStream_t st);
+void launch_reweight(const RobustArgs &a, double *wsum_out, int *nflag_out, hipStream_t st);
+void launch_store_cams(const CamPtrs &cp, int nc, hipStream_t st);
+void launch_exteval(const ExtArgs &a, double *sg_out, int *t_out, hipStream_t st);
+
+}  // namespace rpba

@@ -1,0 +1,975 @@
+// HIP kernels (gfx950) of the plain-FP64 robust LM Schur solve.
+//
+// Layout in HBM (one bundle block):
+//   track entries q (point-major, the order of the reference's pointXXv):
+//     tcam[q], tpt[q] int32; ox[q], oy[q] float (xg); weights as SoA floats
+//     (wx, wy, wxy) for the current Wij and the original Wijoldv; key[q] the
+//     float robust key sig0rmv; flag[q] ISflag.
+//   points: X[3*i] accepted (XPMatrices point part), Xn trial.
+//   per entry 9x3 W = Bc^T W Bp and Y = W V^-1 as 27 contiguous doubles.
+//   cameras: uniform 9-slot parameter blocks (rot a,b,c | C x,y,z | d0, d1, f)
+//     with a per-camera mask of the estimated slots (image 0 pose and one C
+//     component of the gauge camera are fixed, rbacore.cpp:1042-1069).
+//   S: diagonal blocks Sd[k] (81 doubles, row-major) and the upper
+//     off-diagonal blocks Su[b] of the covisibility list.
+// Every reduction runs in a fixed order: results are bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace rpba {
+
+// ---------------------------------------------------------------------------
+// per-observation math
+// ---------------------------------------------------------------------------
+
+struct CamV {
+    double R[9], C[3], d0, d1, f;
+};
+
+__device__ __forceinline__ void load_cam(const CamPtrs &cp, int k, CamV &c)
+{
+#pragma unroll
+    for (int j = 0; j < 9; ++j) c.R[j] = cp.Rn[9 * k + j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) c.C[j] = cp.Cn[3 * k + j];
+    c.d0 = cp.In[3 * k + 0];
+    c.d1 = cp.In[3 * k + 1];
+    c.f = cp.In[3 * k + 2];
+}
+
+struct Ev {
+    double u, v, xs3, su, sv, d1, d2;
+    bool nan;
+};
+
+// projection and residual (CalcCorrections rbacore.cpp:1238-1307,
+// RadialReconstructionImage rpba.h:850-857)
+__device__ __forceinline__ void eval_obs(const CamV &c, double X0, double X1, double X2, float ox, float oy,
+                                         double defdiff, Ev &e)
+{
+    const double p0 = X0 + c.C[0], p1 = X1 + c.C[1], p2 = X2 + c.C[2];
+    double ex0 = c.R[0] * p0 + c.R[1] * p1 + c.R[2] * p2;
+    double ex1 = c.R[3] * p0 + c.R[4] * p1 + c.R[5] * p2;
+    double xs3 = c.R[6] * p0 + c.R[7] * p1 + c.R[8] * p2;
+    if (fabs(xs3) < 1.e-3)
+        xs3 = (xs3 < 0.) ? -1.e3 : 1.e3;
+    else
+        xs3 = 1. / xs3;
+    ex0 *= xs3;
+    ex1 *= xs3;
+    e.u = ex0;
+    e.v = ex1;
+    e.xs3 = xs3;
+    const double dist = ex0 * ex0 + ex1 * ex1;
+    const double z = 1. + (c.d0 * dist + c.d1 * dist * dist);
+    e.su = ex0 * z;
+    e.sv = ex1 * z;
+    double d1 = (double)ox - e.su * c.f;
+    double d2 = (double)oy - e.sv * c.f;
+    if (d1 > defdiff) d1 = defdiff;
+    if (d1 < -defdiff) d1 = -defdiff;
+    if (d2 > defdiff) d2 = defdiff;
+    if (d2 < -defdiff) d2 = -defdiff;
+    e.nan = isnan(d1) || isnan(d2) || isinf(d1) || isinf(d2);
+    if (e.nan) d1 = d2 = 0.;
+    e.d1 = d1;
+    e.d2 = d2;
+}
+
+// calibrated Jacobian dr/dtheta (CalcDerivatives rbacore.cpp:1420-1616,
+// eCalcCalibratedDerivativesX/3 1620-1673) in the uniform 9-slot layout;
+// unestimated slots give zero columns
+__device__ __forceinline__ void jac_obs(const CamV &c, const Ev &e, unsigned mask, double Bc[2][9], double Bp[2][3])
+{
+    const double u = e.u, v = e.v, xs3 = e.xs3;
+    const double x1 = u * xs3, x2 = v * xs3;
+    const double xs31 = 1. / xs3, xs32 = xs31 * xs31;
+    const double k2 = c.d0, k4 = c.d1, f = c.f;
+    const double xy2 = u * u + v * v, xy4 = xy2 * xy2;
+    const double xxpc = k4 * xy4 + u * (2 * k2 * u + 4 * k4 * u * xy2) + k2 * xy2 + 1.;
+    const double xypc = u * (4 * k4 * xy2 * v + 2 * k2 * v);
+    const double yxpc = (4 * k4 * u * xy2 + 2 * k2 * u) * v;
+    const double yypc = v * (2 * k2 * v + 4 * k4 * xy2 * v) + k4 * xy4 + k2 * xy2 + 1.;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const double c0 = -c.R[0 * 3 + j] * xs3 + c.R[2 * 3 + j] * x1;
+        const double c1 = -c.R[1 * 3 + j] * xs3 + c.R[2 * 3 + j] * x2;
+        const double xp = xxpc * c0 + xypc * c1;
+        const double yp = yypc * c1 + yxpc * c0;
+        Bp[0][j] = xp * f + yp * 0.;
+        Bp[1][j] = yp * f;
+    }
+    double rc[2][3];
+    rc[0][0] = x2 * x1 * xs32;
+    rc[1][0] = 1. + x2 * x2 * xs32;
+    rc[0][1] = -1. - x1 * x1 * xs32;
+    rc[1][1] = -x2 * x1 * xs32;
+    rc[0][2] = x2 * xs31;
+    rc[1][2] = -x1 * xs31;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const double xp = xxpc * rc[0][j] + xypc * rc[1][j];
+        const double yp = yypc * rc[1][j] + yxpc * rc[0][j];
+        Bc[0][j] = xp * f + yp * 0.;
+        Bc[1][j] = yp * f;
+        Bc[0][3 + j] = Bp[0][j];
+        Bc[1][3 + j] = Bp[1][j];
+    }
+    const double xyp2 = u * u + v * v, xyp4 = xyp2 * xyp2;
+    Bc[0][6] = -u * xyp2 * f - v * xyp2 * 0.;
+    Bc[1][6] = -v * xyp2 * f;
+    Bc[0][7] = -u * xyp4 * f - v * xyp4 * 0.;
+    Bc[1][7] = -v * xyp4 * f;
+    Bc[0][8] = -e.su;
+    Bc[1][8] = -e.sv;
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+        if (!(mask & (1u << j))) Bc[0][j] = Bc[1][j] = 0.;
+}
+
+// 3x3 inverse by cofactors (adjugate / determinant)
+__device__ __forceinline__ void inv3(const double m[9], double r[9])
+{
+    double cf[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            cf[i * 3 + j] = m[i1 * 3 + j1] * m[i2 * 3 + j2] - m[i1 * 3 + j2] * m[i2 * 3 + j1];
+        }
+    const double det = cf[0] * m[0] + cf[3] * m[3] + cf[6] * m[6];
+    const double id = 1. / det;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r[i * 3 + j] = cf[j * 3 + i] * id;
+}
+
+// ---------------------------------------------------------------------------
+// deterministic reductions
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// block-wide sum, fixed order; every thread gets the result.  sh holds
+// blockDim/64 doubles.
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.;
+    for (int i = 0; i < nw; ++i) s += sh[i];
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// linearisation: points (V, g_p, W, Y), one thread per point
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.npts) return;
+    const double X0 = a.X[3 * i], X1 = a.X[3 * i + 1], X2 = a.X[3 * i + 2];
+    double V[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+    const int b = a.toff[i], e = a.toff[i + 1];
+    for (int q = b; q < e; ++q) {
+        const int k = a.tcam[q];
+        CamV c;
+        load_cam(a.cam, k, c);
+        Ev ev;
+        eval_obs(c, X0, X1, X2, a.ox[q], a.oy[q], a.defdiff, ev);
+        double Bc[2][9], Bp[2][3];
+        jac_obs(c, ev, a.cam.mask[k], Bc, Bp);
+        const double W0 = a.wx[q], W1 = a.wy[q], W2 = a.wxy[q];
+        double *Wo = a.Wo + (size_t)q * 27;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const double t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
+            const double t1 = Bc[0][r] * W2 + Bc[1][r] * W1;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) Wo[r * 3 + cc] = t0 * Bp[0][cc] + t1 * Bp[1][cc];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const double t0 = Bp[0][r] * W0 + Bp[1][r] * W2;
+            const double t1 = Bp[0][r] * W2 + Bp[1][r] * W1;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) V[r * 3 + cc] += t0 * Bp[0][cc] + t1 * Bp[1][cc];
+            g[r] += t0 * ev.d1 + t1 * ev.d2;
+        }
+    }
+    V[0] *= a.mult1;
+    V[4] *= a.mult1;
+    V[8] *= a.mult1;
+    double Vi[9];
+    inv3(V, Vi);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) a.Vinv[9 * i + j] = Vi[j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) a.gp[3 * i + j] = g[j];
+    for (int q = b; q < e; ++q) {
+        const double *Wo = a.Wo + (size_t)q * 27;
+        double *Yo = a.Yo + (size_t)q * 27;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const double w0 = Wo[r * 3], w1 = Wo[r * 3 + 1], w2 = Wo[r * 3 + 2];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) Yo[r * 3 + cc] = w0 * Vi[0 * 3 + cc] + w1 * Vi[1 * 3 + cc] + w2 * Vi[2 * 3 + cc];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// linearisation: cameras.  One workgroup (256) per camera:
+//   phase 1: U = sum Bc^T W Bc (upper 45), g_c = sum Bc^T W r
+//   phase 2: diagonal Schur term sum Y W^T (81) and sum Y g_p (9)
+// then S_kk = U - sum Y W^T with the diagonal damping (x(1+lambda), +1e-2,
+// rbacore.cpp:1164-1170) and b_k = g_c - sum Y g_p; fixed slots become an
+// identity row/column with zero right-hand side.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
+{
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    __shared__ double red[4][54];
+    __shared__ double red2[2][90];
+    __shared__ double Ssh[81];
+    __shared__ double bsh[9];
+    const unsigned mask = a.cam.mask[k];
+    CamV c;
+    load_cam(a.cam, k, c);
+    const int b = a.camoff[k], e = a.camoff[k + 1];
+
+    double acc[54];
+#pragma unroll
+    for (int j = 0; j < 54; ++j) acc[j] = 0.;
+    for (int j = b + t; j < e; j += 256) {
+        const int q = a.camq[j];
+        const int p = a.tpt[q];
+        Ev ev;
+        eval_obs(c, a.X[3 * p], a.X[3 * p + 1], a.X[3 * p + 2], a.ox[q], a.oy[q], a.defdiff, ev);
+        double Bc[2][9], Bp[2][3];
+        jac_obs(c, ev, mask, Bc, Bp);
+        const double W0 = a.wx[q], W1 = a.wy[q], W2 = a.wxy[q];
+        double Bt[9][2];
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            Bt[r][0] = Bc[0][r] * W0 + Bc[1][r] * W2;
+            Bt[r][1] = Bc[0][r] * W2 + Bc[1][r] * W1;
+        }
+        int n = 0;
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+#pragma unroll
+            for (int cc = r; cc < 9; ++cc) acc[n++] += Bt[r][0] * Bc[0][cc] + Bt[r][1] * Bc[1][cc];
+#pragma unroll
+        for (int r = 0; r < 9; ++r) acc[45 + r] += Bt[r][0] * ev.d1 + Bt[r][1] * ev.d2;
+    }
+    const int w = t >> 6, lane = t & 63;
+#pragma unroll
+    for (int j = 0; j < 54; ++j) {
+        const double s = wave_sum(acc[j]);
+        if (lane == 0) red[w][j] = s;
+    }
+    // phase 2: two groups of 128; thread o < 81 owns S(o/9, o%9), 81..89 b
+    const int grp = t >> 7, o = t & 127;
+    double s2 = 0.;
+    if (o < 90) {
+        const int r = (o < 81) ? o / 9 : o - 81;
+        const int cc = (o < 81) ? o % 9 : 0;
+        for (int j = b + grp; j < e; j += 2) {
+            const int q = a.camq[j];
+            const double *Y = a.Yo + (size_t)q * 27 + 3 * r;
+            if (o < 81) {
+                const double *W = a.Wo + (size_t)q * 27 + 3 * cc;
+                s2 += Y[0] * W[0] + Y[1] * W[1] + Y[2] * W[2];
+            } else {
+                const double *gp = a.gp + 3 * (size_t)a.tpt[q];
+                s2 += Y[0] * gp[0] + Y[1] * gp[1] + Y[2] * gp[2];
+            }
+        }
+        red2[grp][o] = s2;
+    }
+    __syncthreads();
+    if (t < 81) {
+        const int r = t / 9, cc = t % 9;
+        const int rr = r < cc ? r : cc, c2 = r < cc ? cc : r;
+        const int n = rr * 9 - rr * (rr - 1) / 2 + (c2 - rr);  // index in the packed upper triangle
+        const double U = red[0][n] + red[1][n] + red[2][n] + red[3][n];
+        double s = U - (red2[0][t] + red2[1][t]);
+        if (r == cc) {
+            s *= a.mult1;
+            s += 1.e-2;
+        }
+        const bool fr = (mask >> r) & 1u, fc = (mask >> cc) & 1u;
+        if (!fr || !fc) s = (r == cc) ? 1. : 0.;
+        Ssh[t] = s;
+    } else if (t >= 128 && t < 137) {
+        const int r = t - 128;
+        const double gc = red[0][45 + r] + red[1][45 + r] + red[2][45 + r] + red[3][45 + r];
+        double s = gc - (red2[0][81 + r] + red2[1][81 + r]);
+        if (!((mask >> r) & 1u)) s = 0.;
+        bsh[r] = s;
+    }
+    __syncthreads();
+    if (t < 81) a.Sd[81 * (size_t)k + t] = Ssh[t];
+    if (t < 9) a.bvec[9 * (size_t)k + t] = bsh[t];
+}
+
+// ---------------------------------------------------------------------------
+// Schur off-diagonal blocks, atomic-free: one wavefront per upper block (k,l)
+// walks its covisibility pairs (grouped by block, point order inside) and
+// accumulates S_kl = -sum_pairs Y_a W_b^T.  Pairs are staged through LDS in
+// batches of 8; lane o owns outputs o and o + 64.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
+{
+    const int bid = blockIdx.x;
+    if (bid >= a.nblk) return;
+    const int lane = threadIdx.x;
+    __shared__ double st[8 * 54];
+    const int pb = a.poff[bid], pe = a.poff[bid + 1];
+    const int o0 = lane, o1 = lane + 64;
+    const int r0 = o0 / 9, c0 = o0 % 9;
+    const int r1 = (o1 < 81) ? o1 / 9 : 0, c1 = (o1 < 81) ? o1 % 9 : 0;
+    double acc0 = 0., acc1 = 0.;
+    for (int p = pb; p < pe; p += 8) {
+        const int np = (pe - p < 8) ? pe - p : 8;
+        for (int x = lane; x < np * 54; x += 64) {
+            const int j = x / 54, idx = x % 54;
+            st[x] = (idx < 27) ? a.Yo[(size_t)a.pa[p + j] * 27 + idx] : a.Wo[(size_t)a.pb[p + j] * 27 + (idx - 27)];
+        }
+        __syncthreads();
+        for (int j = 0; j < np; ++j) {
+            const double *Y = st + j * 54, *W = st + j * 54 + 27;
+            acc0 += Y[3 * r0] * W[3 * c0] + Y[3 * r0 + 1] * W[3 * c0 + 1] + Y[3 * r0 + 2] * W[3 * c0 + 2];
+            acc1 += Y[3 * r1] * W[3 * c1] + Y[3 * r1 + 1] * W[3 * c1 + 1] + Y[3 * r1 + 2] * W[3 * c1 + 2];
+        }
+        __syncthreads();
+    }
+    double *S = a.Su + (size_t)bid * 81;
+    S[o0] = -acc0;
+    if (o1 < 81) S[o1] = -acc1;
+}
+
+// ---------------------------------------------------------------------------
+// PCG (SparsePCG rbacore.cpp:1807-1914) with device-held scalars
+// ---------------------------------------------------------------------------
+
+// block-Jacobi inverse by LU with partial pivoting, one thread per camera
+__global__ __launch_bounds__(64) void k_bj_inverse(int nc, const double *Sd, double *Cinv)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    double A[81];
+    int perm[9];
+    for (int j = 0; j < 81; ++j) A[j] = Sd[81 * (size_t)k + j];
+    for (int j = 0; j < 9; ++j) perm[j] = j;
+    for (int kk = 0; kk < 9; ++kk) {
+        int p = kk;
+        double best = fabs(A[kk * 9 + kk]);
+        for (int i = kk + 1; i < 9; ++i)
+            if (fabs(A[i * 9 + kk]) > best) { best = fabs(A[i * 9 + kk]); p = i; }
+        if (p != kk) {
+            for (int j = 0; j < 9; ++j) { const double tt = A[kk * 9 + j]; A[kk * 9 + j] = A[p * 9 + j]; A[p * 9 + j] = tt; }
+            const int tt = perm[kk]; perm[kk] = perm[p]; perm[p] = tt;
+        }
+        const double piv = A[kk * 9 + kk];
+        if (piv != 0.)
+            for (int i = kk + 1; i < 9; ++i) {
+                const double l = A[i * 9 + kk] / piv;
+                A[i * 9 + kk] = l;
+                for (int j = kk + 1; j < 9; ++j) A[i * 9 + j] -= l * A[kk * 9 + j];
+            }
+    }
+    for (int col = 0; col < 9; ++col) {
+        double y[9];
+        for (int i = 0; i < 9; ++i) {
+            double s = (perm[i] == col) ? 1. : 0.;
+            for (int j = 0; j < i; ++j) s -= A[i * 9 + j] * y[j];
+            y[i] = s;
+        }
+        for (int i = 8; i >= 0; --i) {
+            double s = y[i];
+            for (int j = i + 1; j < 9; ++j) s -= A[i * 9 + j] * y[j];
+            y[i] = s / A[i * 9 + i];
+        }
+        for (int i = 0; i < 9; ++i) Cinv[81 * (size_t)k + i * 9 + col] = y[i];
+    }
+}
+
+// h = C r for camera k (thread-local 9x9 mat-vec)
+__device__ __forceinline__ void apply_cinv(const double *Cinv, const double *r, double *h, int k)
+{
+    const double *C = Cinv + 81 * (size_t)k;
+    double rr[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) rr[j] = r[9 * (size_t)k + j];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double s = 0.;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) s += C[i * 9 + j] * rr[j];
+        h[9 * (size_t)k + i] = s;
+    }
+}
+
+// x = 0, r = b, h = C b, d = h, best = b (exivold starts as a copy of b),
+// rho = r.h.  Single workgroup of 1024.
+__global__ __launch_bounds__(1024) void k_pcg_init(PcgArgs a)
+{
+    __shared__ double sh[16];
+    const int n = 9 * a.nc;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        a.x[j] = 0.;
+        a.r[j] = a.b[j];
+        a.best[j] = a.b[j];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < a.nc; k += blockDim.x) apply_cinv(a.Cinv, a.r, a.h, k);
+    __syncthreads();
+    double s = 0.;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        a.d[j] = a.h[j];
+        s += a.r[j] * a.h[j];
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        a.S->rho = s;
+        a.S->m0 = 1.e50;
+        a.S->mold = 1.e50;
+        a.S->k = 0;
+        a.S->done = 0;
+        a.S->iters = 0;
+    }
+}
+
+// Ad = S d, one workgroup (256) per block row k: 28 groups of 9 lanes walk the
+// row's items (diagonal, upper blocks of row k, upper blocks of column k used
+// transposed); also writes the row's share of d.Ad.
+__global__ __launch_bounds__(256) void k_spmv(PcgArgs a)
+{
+    if (a.S->done) return;
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    const int g = t / 9, r = t % 9;
+    __shared__ double red[28][9];
+    const int nu = a.row_off[k + 1] - a.row_off[k];
+    const int nl = a.col_off[k + 1] - a.col_off[k];
+    const int nitems = 1 + nu + nl;
+    double acc = 0.;
+    if (g < 28) {
+        for (int it = g; it < nitems; it += 28) {
+            const double *B;
+            const double *dv;
+            bool tr = false;
+            if (it == 0) {
+                B = a.Sd + 81 * (size_t)k;
+                dv = a.d + 9 * (size_t)k;
+            } else if (it <= nu) {
+                const int bb = a.row_off[k] + it - 1;
+                B = a.Su + 81 * (size_t)bb;
+                dv = a.d + 9 * (size_t)a.blk_l[bb];
+            } else {
+                const int bb = a.col_blk[a.col_off[k] + it - 1 - nu];
+                B = a.Su + 81 * (size_t)bb;
+                dv = a.d + 9 * (size_t)a.blk_k[bb];
+                tr = true;
+            }
+            if (!tr) {
+#pragma unroll
+                for (int cc = 0; cc < 9; ++cc) acc += B[r * 9 + cc] * dv[cc];
+            } else {
+#pragma unroll
+                for (int cc = 0; cc < 9; ++cc) acc += B[cc * 9 + r] * dv[cc];
+            }
+        }
+        red[g][r] = acc;
+    }
+    __syncthreads();
+    if (t < 9) {
+        double s = 0.;
+        for (int gg = 0; gg < 28; ++gg) s += red[gg][t];
+        a.Ad[9 * (size_t)k + t] = s;
+        red[0][t] = s * a.d[9 * (size_t)k + t];
+    }
+    __syncthreads();
+    if (t == 0) {
+        double s = 0.;
+        for (int j = 0; j < 9; ++j) s += red[0][j];
+        a.rowdot[k] = s;
+    }
+}
+
+// one PCG update after the SpMV (single workgroup of 1024)
+__global__ __launch_bounds__(1024) void k_pcg_update(PcgArgs a)
+{
+    __shared__ double sh[16];
+    PcgScalars *S = a.S;
+    if (S->done) return;
+    const int n = 9 * a.nc;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double dAd = 0.;
+    for (int k = tid; k < a.nc; k += nt) dAd += a.rowdot[k];
+    dAd = block_sum(dAd, sh);
+    const double rho = S->rho;
+    const double alpha = (fabs(rho) < 1.e-20 || fabs(rho) > 1.e20) ? 0. : rho / dAd;
+    double m = 0.;
+    for (int j = tid; j < n; j += nt) {
+        a.x[j] += alpha * a.d[j];
+        const double dr = alpha * a.Ad[j];
+        a.r[j] -= dr;  // r becomes r_{k+1}
+        m += dr * dr;
+    }
+    m = block_sum(m, sh);
+    for (int k = tid; k < a.nc; k += nt) apply_cinv(a.Cinv, a.r, a.h1, k);
+    const int it = S->k;
+    const double m0 = (it == 0) ? m : S->m0;
+    const bool better = m < S->mold;
+    if (better)
+        for (int j = tid; j < n; j += nt) a.best[j] = a.x[j];
+    __syncthreads();
+    if (tid == 0) {
+        S->m0 = m0;
+        if (better) S->mold = m;
+    }
+    if (m * 1.e5 < m0) {
+        if (tid == 0) {
+            S->done = 1;
+            S->iters = it + 1;
+        }
+        return;
+    }
+    double num = 0.;
+    for (int j = tid; j < n; j += nt) num += a.r[j] * (a.h1[j] - a.h[j]);
+    num = block_sum(num, sh);
+    double beta = (fabs(num) < 1.e-10) ? 0. : num / rho;
+    if (beta < 0.) beta = 0.;
+    double rh = 0.;
+    for (int j = tid; j < n; j += nt) {
+        const double h1 = a.h1[j];
+        a.d[j] = h1 + beta * a.d[j];
+        a.h[j] = h1;
+        rh += a.r[j] * h1;
+    }
+    rh = block_sum(rh, sh);
+    if (tid == 0) {
+        S->rho = rh;
+        S->k = it + 1;
+        if (it + 1 >= a.msize) {
+            S->done = 1;
+            S->iters = a.msize;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// back-substitution, decode, cost
+// ---------------------------------------------------------------------------
+
+// dp_i = V_i^-1 (g_i - sum_q W_q^T dc_k) with the double camera solution
+// (rbacore.cpp:1395-1410); Xn = X - dp (rbacore.cpp:547)
+__global__ __launch_bounds__(256) void k_backsub(StepArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.npts) return;
+    double t[3] = {a.gp[3 * i], a.gp[3 * i + 1], a.gp[3 * i + 2]};
+    for (int q = a.toff[i]; q < a.toff[i + 1]; ++q) {
+        const double *W = a.Wo + (size_t)q * 27;
+        const double *xc = a.sol + 9 * (size_t)a.tcam[q];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) s += W[r * 3 + j] * xc[r];
+            t[j] -= s;
+        }
+    }
+    const double *Vi = a.Vinv + 9 * (size_t)i;
+    bool bad = false;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const double dp = Vi[r * 3] * t[0] + Vi[r * 3 + 1] * t[1] + Vi[r * 3 + 2] * t[2];
+        if (isnan(dp) || isinf(dp)) bad = true;
+        a.Xn[3 * i + r] = a.X[3 * i + r] - dp;
+    }
+    if (bad) a.S->nan_flag = 1;
+}
+
+__device__ __forceinline__ void cayley(double a, double b, double c, double R[9])
+{
+    const double a2 = a * a, b2 = b * b, c2 = c * c;
+    const double ab = a * b, ac = a * c, bc = b * c;
+    double s = 4 + a2 + b2 + c2;
+    if (fabs(s) < 1.e-20)
+        s = (s < 0.) ? -1.e20 : 1.e20;
+    else
+        s = 1. / s;
+    R[0] = s * (4 + a2 - b2 - c2);
+    R[1] = s * (2 * ab - 4 * c);
+    R[2] = s * (2 * ac + 4 * b);
+    R[3] = s * (2 * ab + 4 * c);
+    R[4] = s * (4 - a2 + b2 - c2);
+    R[5] = s * (2 * bc - 4 * a);
+    R[6] = s * (2 * ac - 4 * b);
+    R[7] = s * (2 * bc + 4 * a);
+    R[8] = s * (4 - a2 - b2 + c2);
+}
+
+// quaternion re-orthonormalisation (RectifyRotationMatrix3, rpba.h:876-913)
+__device__ __forceinline__ void rectify(double R[9])
+{
+    double a0 = R[7] - R[5], a1 = R[2] - R[6], a2 = R[3] - R[1];
+    if (fabs(a0) < 1.e-20) a0 = 1.e-20;
+    if (fabs(a1) < 1.e-20) a1 = 1.e-20;
+    if (fabs(a2) < 1.e-20) a2 = 1.e-20;
+    const double al = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    a0 /= al;
+    a1 /= al;
+    a2 /= al;
+    const double phi = atan2(al, R[0] + R[4] + R[8] - 1.);
+    const double q0 = cos(phi * 0.5), sh = sin(phi * 0.5);
+    const double q1 = a0 * sh, q2 = a1 * sh, q3 = a2 * sh;
+    R[0] = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
+    R[1] = 2 * (q1 * q2 - q0 * q3);
+    R[2] = 2 * (q1 * q3 + q0 * q2);
+    R[3] = 2 * (q2 * q1 + q0 * q3);
+    R[4] = q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3;
+    R[5] = 2 * (q2 * q3 - q0 * q1);
+    R[6] = 2 * (q3 * q1 - q0 * q2);
+    R[7] = 2 * (q3 * q2 + q0 * q1);
+    R[8] = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+}
+
+// theta_new = theta - (float) x_c; trial cameras (DecodeCameras,
+// rbacore.cpp:1733-1800).  sol == nullptr decodes theta itself.
+__global__ __launch_bounds__(64) void k_decode(StepArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nc) return;
+    double th[9];
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+        double xs = 0.;
+        if (a.sol) {
+            const float xf = (float)a.sol[9 * (size_t)k + s];  // camera step cast to float (rbacore.cpp:1185)
+            if (isnan(xf) || isinf(xf)) bad = true;
+            xs = (double)xf;
+        }
+        th[s] = a.cam.th[9 * (size_t)k + s] - xs;
+        a.cam.thn[9 * (size_t)k + s] = th[s];
+    }
+    if (bad) a.S->nan_flag = 1;
+    const unsigned mask = a.cam.mask[k];
+    if (k > 0) {
+        double Ra[9], R[9];
+        cayley(th[0], th[1], th[2], Ra);
+        const double *Rb = a.cam.Rb + 9 * (size_t)k;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) R[r * 3 + c] = Ra[r * 3 + 0] * Rb[0 * 3 + c] + Ra[r * 3 + 1] * Rb[1 * 3 + c] + Ra[r * 3 + 2] * Rb[2 * 3 + c];
+        rectify(R);
+#pragma unroll
+        for (int j = 0; j < 9; ++j) a.cam.Rn[9 * (size_t)k + j] = R[j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double e = ((mask >> (3 + j)) & 1u) ? th[3 + j] : 0.;
+            a.cam.Cn[3 * (size_t)k + j] = e + a.cam.Cb[3 * (size_t)k + j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        if ((mask >> (6 + j)) & 1u) a.cam.In[3 * (size_t)k + j] = th[6 + j];
+}
+
+// per-entry residual at the trial state: float robust key (sig0rmv) and the
+// FP64 cost term; per-workgroup partial sums
+__global__ __launch_bounds__(256) void k_cost(CostArgs a)
+{
+    __shared__ double sh[4];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    double cst = 0.;
+    if (q < a.ntr) {
+        const int k = a.tcam[q], p = a.tpt[q];
+        CamV c;
+        load_cam(a.cam, k, c);
+        Ev ev;
+        eval_obs(c, a.X[3 * p], a.X[3 * p + 1], a.X[3 * p + 2], a.ox[q], a.oy[q], a.defdiff, ev);
+        if (ev.nan) {
+            a.key[q] = 1.e20f;
+        } else {
+            const float W0 = a.wx[q], W1 = a.wy[q], W2 = a.wxy[q];
+            const float d1f = (float)ev.d1, d2f = (float)ev.d2;
+            const float d11 = d1f * d1f, d12 = 2.f * d1f * d2f, d22 = d2f * d2f;
+            float key;
+            if (a.robust)
+                key = d11 * a.w0x[q] + d12 * a.w0xy[q] + d22 * a.w0y[q];
+            else
+                key = d11 * W0 + d12 * W2 + d22 * W1;
+            if (a.write_key) a.key[q] = key;
+            cst = ev.d1 * ev.d1 * (double)W0 + 2. * ev.d1 * ev.d2 * (double)W2 + ev.d2 * ev.d2 * (double)W1;
+        }
+    }
+    const double s = block_sum(cst, sh);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
+}
+
+// sum of n partials in index order (single workgroup of 1024); out[0] = sum,
+// optionally the integer partials too
+__global__ __launch_bounds__(1024) void k_final_sum(const double *partial, int n, const int *ipartial, double *out,
+                                                    int *iout)
+{
+    __shared__ double sh[16];
+    double s = 0.;
+    long long is = 0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        s += partial[j];
+        if (ipartial) is += ipartial[j];
+    }
+    s = block_sum(s, sh);
+    double si = block_sum((double)is, sh);
+    if (threadIdx.x == 0) {
+        out[0] = s;
+        if (iout) iout[0] = (int)si;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// robust reweighting (Reweight2D rbacore.cpp:823-887)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float redfac_of(int t)
+{
+    int pxs = t * 2;
+    if (pxs < 4) pxs = 4;
+    const float rf = (float)pxs / (float)(pxs - 3);
+    return rf * rf;
+}
+
+__device__ __forceinline__ unsigned ord_bits(float f)
+{
+    const unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// exact upper median (sorted[n/2]) of key*redfac per image by 4-pass radix
+// select; s00[k] = 1/(median * 2.1981f), x0.001 while iflag > 1
+__global__ __launch_bounds__(256) void k_median(RobustArgs a)
+{
+    const int k = blockIdx.x;
+    const int b = a.camoff[k], e = a.camoff[k + 1], n = e - b;
+    __shared__ unsigned hist[256];
+    __shared__ unsigned s_prefix, s_rank;
+    if (threadIdx.x == 0) {
+        s_prefix = 0;
+        s_rank = (unsigned)(n / 2);
+    }
+    unsigned maskbits = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        const unsigned prefix = s_prefix;
+        for (int j = b + threadIdx.x; j < e; j += blockDim.x) {
+            const int q = a.camq[j];
+            const int p = a.tpt[q];
+            const float kv = a.key[q] * redfac_of(a.toff[p + 1] - a.toff[p]);
+            const unsigned u = ord_bits(kv);
+            if ((u & maskbits) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned cum = 0, rank = s_rank, sel = 255;
+            for (unsigned d = 0; d < 256; ++d) {
+                if (cum + hist[d] > rank) {
+                    sel = d;
+                    break;
+                }
+                cum += hist[d];
+            }
+            s_rank = rank - cum;
+            s_prefix = prefix | (sel << shift);
+        }
+        maskbits |= 255u << shift;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const unsigned u = s_prefix;
+        const unsigned raw = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+        const float med = __uint_as_float(raw);
+        float s = 1.f / (med * 2.1981f);
+        if (a.iflag > 1) s *= 0.001f;
+        a.s00[k] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reweight(RobustArgs a)
+{
+    __shared__ double sh[4];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    double ws = 0.;
+    int nf = 0;
+    if (q < a.ntr) {
+        float ww = a.key[q] * a.s00[a.tcam[q]];
+        const bool fl = ww > 9.f;  // kf, rpba.h:529
+        a.flag[q] = fl ? 1 : 0;
+        if (fl) {
+            ww = 1.e-4f;
+            nf = 1;
+        } else
+            ww = 1.f;
+        float W0 = ww * a.w0x[q], W1 = ww * a.w0y[q], W2 = ww * a.w0xy[q];
+        if (W0 + W1 < 2.e-8f) {
+            W0 = W1 = 1.e-8f;
+            W2 = 0.f;
+        }
+        a.wx[q] = W0;
+        a.wy[q] = W1;
+        a.wxy[q] = W2;
+        ws = (double)W0 + (double)W1;
+    }
+    const double s = block_sum(ws, sh);
+    const double fsum = block_sum((double)nf, sh);
+    if (threadIdx.x == 0) {
+        a.partial[blockIdx.x] = s;
+        a.ipartial[blockIdx.x] = (int)fsum;
+    }
+}
+
+// StoreResultsIteration for the cameras (rbacore.cpp:895-906): P = [R | R C]
+__global__ __launch_bounds__(64) void k_store_cams(CamPtrs cp, int nc)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    const double *R = cp.Rn + 9 * (size_t)k, *C = cp.Cn + 3 * (size_t)k;
+    double *P = cp.Pst + 12 * (size_t)k;
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) P[r * 4 + c] = R[r * 3 + c];
+        P[r * 4 + 3] = R[r * 3 + 0] * C[0] + R[r * 3 + 1] * C[1] + R[r * 3 + 2] * C[2];
+    }
+    for (int j = 0; j < 3; ++j) cp.Ist[3 * (size_t)k + j] = cp.In[3 * (size_t)k + j];
+    for (int j = 0; j < 9; ++j) cp.th[9 * (size_t)k + j] = cp.thn[9 * (size_t)k + j];
+}
+
+// ---------------------------------------------------------------------------
+// Exteval (EvalLSA + CalcCorrPointMulti, rpbacore.cpp:882-941, 778-837)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_exteval(ExtArgs a)
+{
+    __shared__ double sh[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double contrib = 0.;
+    int t = 0;
+    if (i < a.npts) {
+        const int b = a.toff[i], e = a.toff[i + 1];
+        t = e - b;
+        if (t > 1) {
+            double ws = 0.;
+            for (int q = b; q < e; ++q) ws += (double)a.wx[q] + (double)a.wy[q];
+            ws = ws / (double)(2 * t);
+            const double X0 = a.pts[3 * i], X1 = a.pts[3 * i + 1], X2 = a.pts[3 * i + 2];
+            double s = 0.;
+            for (int q = b; q < e; ++q) {
+                const int c = a.cam[q];
+                const float *P = a.P + 12 * (size_t)c;
+                double p1 = (double)P[0] * X0 + (double)P[1] * X1 + (double)P[2] * X2 + (double)P[3];
+                double p2 = (double)P[4] * X0 + (double)P[5] * X1 + (double)P[6] * X2 + (double)P[7];
+                double p3 = (double)P[8] * X0 + (double)P[9] * X1 + (double)P[10] * X2 + (double)P[11];
+                if (fabs(p3) < 1.e-3)
+                    p3 = (p3 > 0.) ? 1.e3 : -1.e3;
+                else
+                    p3 = 1. / p3;
+                p1 *= p3;
+                p2 *= p3;
+                const double dist = p1 * p1 + p2 * p2;
+                const double z = 1. + ((double)a.cd[2 * c] * dist + (double)a.cd[2 * c + 1] * dist * dist);
+                p1 *= z;
+                p2 *= z;
+                p1 *= (double)a.cf[c];
+                p2 *= (double)a.cf[c];
+                const double dx = (double)a.ox[q] - p1, dy = (double)a.oy[q] - p2;
+                s += dx * dx * a.wx[q] + dy * dy * a.wy[q] + 2 * dx * dy * a.wxy[q];
+            }
+            contrib = s / ws;
+        } else
+            t = 0;
+    }
+    const double sg = block_sum(contrib, sh);
+    const double tt = block_sum((double)t, sh);
+    if (threadIdx.x == 0) {
+        a.partial[blockIdx.x] = sg;
+        a.ipartial[blockIdx.x] = (int)tt;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_lin(const LinArgs &a, const SchurArgs &s, hipStream_t st)
+{
+    if (a.npts > 0) k_lin_points<<<nblk(a.npts, 128), 128, 0, st>>>(a);
+    if (s.nblk > 0) k_schur_pairs<<<s.nblk, 64, 0, st>>>(s);
+    k_lin_cameras<<<a.nc, 256, 0, st>>>(a);
+}
+void launch_lin_points(const LinArgs &a, hipStream_t st)
+{
+    if (a.npts > 0) k_lin_points<<<nblk(a.npts, 128), 128, 0, st>>>(a);
+}
+void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
+{
+    if (s.nblk > 0) k_schur_pairs<<<s.nblk, 64, 0, st>>>(s);
+}
+void launch_lin_cameras(const LinArgs &a, hipStream_t st) { k_lin_cameras<<<a.nc, 256, 0, st>>>(a); }
+void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
+{
+    k_bj_inverse<<<nblk(nc, 64), 64, 0, st>>>(nc, Sd, Cinv);
+}
+void launch_pcg_init(const PcgArgs &a, hipStream_t st) { k_pcg_init<<<1, 1024, 0, st>>>(a); }
+void launch_spmv(const PcgArgs &a, hipStream_t st) { k_spmv<<<a.nc, 256, 0, st>>>(a); }
+void launch_pcg_update(const PcgArgs &a, hipStream_t st) { k_pcg_update<<<1, 1024, 0, st>>>(a); }
+void launch_backsub(const StepArgs &a, hipStream_t st)
+{
+    if (a.npts > 0) k_backsub<<<nblk(a.npts, 256), 256, 0, st>>>(a);
+}
+void launch_decode(const StepArgs &a, hipStream_t st) { k_decode<<<nblk(a.nc, 64), 64, 0, st>>>(a); }
+int cost_blocks(int ntr) { return (int)nblk(ntr > 0 ? ntr : 1, 256); }
+void launch_cost(const CostArgs &a, double *out, hipStream_t st)
+{
+    const int nb = cost_blocks(a.ntr);
+    k_cost<<<nb, 256, 0, st>>>(a);
+    k_final_sum<<<1, 1024, 0, st>>>(a.partial, nb, nullptr, out, nullptr);
+}
+void launch_reweight(const RobustArgs &a, double *wsum_out, int *nflag_out, hipStream_t st)
+{
+    k_median<<<a.nc, 256, 0, st>>>(a);
+    const int nb = cost_blocks(a.ntr);
+    k_reweight<<<nb, 256, 0, st>>>(a);
+    k_final_sum<<<1, 1024, 0, st>>>(a.partial, nb, a.ipartial, wsum_out, nflag_out);
+}
+void launch_store_cams(const CamPtrs &cp, int nc, hipStream_t st) { k_store_cams<<<nblk(nc, 64), 64, 0, st>>>(cp, nc); }
+void launch_exteval(const ExtArgs &a, double *sg_out, int *t_out, hipStream_t st)
+{
+    const int nb = (int)nblk(a.npts > 0 ? a.npts : 1, 256);
+    k_exteval<<<nb, 256, 0, st>>>(a);
+    k_final_sum<<<1, 1024, 0, st>>>(a.partial, nb, a.ipartial, sg_out, t_out);
+}
+
+}  // namespace rpba

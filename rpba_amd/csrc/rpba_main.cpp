@@ -1,0 +1,85 @@
+// `rpba` host driver: the reference's main (rpba.cpp:45-176) on the device
+// solve.  Reads ./parameters, the BAL file InputDir/File, runs the solve and
+// prints the same lines.  An optional `Output = 1` key writes InputDir/out-File
+// in the input format (the reference's OutputBlock call, rpba.cpp:144-150).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+
+#include "../../include/rpba.h"
+
+static void to_stdout(const char *line, void *user)
+{
+    std::string *os = (std::string *)user;
+    os->append(line);
+}
+
+static int read_output_key(const char *path)
+{
+    FILE *fh = std::fopen(path, "r");
+    if (!fh) return 0;
+    char line[1024];
+    int out = 0;
+    while (std::fgets(line, sizeof line, fh)) {
+        char k[256];
+        int v = 0;
+        if (std::sscanf(line, " %255[^= \t] = %d", k, &v) == 2 && std::strcmp(k, "Output") == 0) out = v;
+    }
+    std::fclose(fh);
+    return out;
+}
+
+int main(int argc, char **argv)
+{
+    (void)argv;
+    if (argc > 1) {
+        std::printf("Parameter file not used\n");
+        return 1;
+    }
+    rpba_flags fl;
+    char dir[2048], file[2048];
+    if (rpba_read_parameters("parameters", &fl, dir, sizeof dir, file, sizeof file) != RPBA_OK) {
+        std::fprintf(stderr, "rpba: cannot read ./parameters (all of InputDir, File, Robust, Exteval, k2, k4, f, k24out, Threads are required)\n");
+        return 1;
+    }
+    const std::string inputfile = std::string(dir) + "/" + file;
+    std::printf("Inputfile: %s\n\n", inputfile.c_str());
+    std::printf("Robustflag: %d\n\n", fl.robust);
+    int mt = fl.threads;
+    if (mt < 1) mt = (int)std::thread::hardware_concurrency();
+    if (mt < 1) mt = 1;
+    std::printf("Threads: %d\n\n", mt);
+    std::printf("K2: %d\n", fl.est_k2);
+    std::printf("K4: %d\n", fl.est_k4);
+    std::printf("Focal length: %d\n\n", fl.est_f);
+    std::printf("K24out: %d\n\n", fl.k24out);
+    fl.threads = mt;
+
+    rpba_block io;
+    if (rpba_read_bal(inputfile.c_str(), fl.k24out, &io) != RPBA_OK) {
+        std::fprintf(stderr, "rpba: cannot read %s\n", inputfile.c_str());
+        return 1;
+    }
+    const int nraddpar = fl.est_k2 + fl.est_k4 + fl.est_f;
+    std::printf("Cameras: %d  Points: %d Observations %d\n", io.n_cams, io.n_pts, io.n_obs);
+    const long nrunknowns = 3L * io.n_pts + (long)(io.n_cams - 2) * 6 + 5 + (long)io.n_cams * nraddpar;
+    std::printf("Unknowns %ld  Redundancy %ld\n", nrunknowns, 2L * io.n_obs - nrunknowns);
+
+    std::string os;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = rpba_calc_plsa(&io, &fl, to_stdout, &os, nullptr);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (io.n_cams < 50 || mt == 1) std::fputs(os.c_str(), stdout);
+    std::printf("\n\nBundle adjustment runtime: %g s\n\n", secs);
+    if (rc != RPBA_OK && rc != RPBA_E_NO_OBS_IN_IMAGE && rc != RPBA_E_LOW_REDUNDANCY && rc != RPBA_E_NAN_RESULT)
+        std::fprintf(stderr, "rpba: solve failed with status %d\n", rc);
+    if (!fl.exteval && read_output_key("parameters")) {
+        const std::string out = std::string(dir) + "/out-" + file;
+        rpba_write_bal(out.c_str(), &io, fl.k24out);
+    }
+    rpba_free_block(&io);
+    return rc == RPBA_OK ? 0 : 2;
+}

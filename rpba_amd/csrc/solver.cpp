@@ -1,0 +1,1078 @@
+// Host side of the device solve: the LM state machine of LSA::Adjust
+// (rbacore.cpp:333-616), pass set-up (EncodeCameras 999-1087, PrepareWeights2D
+// 704-783), RobustDelete (922-995), OutputLSAc (644-666) and the C ABI of
+// include/rpba.h.  Every per-observation, per-point and per-camera step runs
+// in kernels.hip; the host keeps the scalars of the state machine and reads
+// back a few of them per LM iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/rpba.h"
+#include "kernels.h"
+#include "structure.h"
+
+using namespace rpba;
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            std::fprintf(stderr, "rpba: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+            return RPBA_E_HIP;                                                             \
+        }                                                                                  \
+    } while (0)
+
+namespace {
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t cnt)
+    {
+        if (cnt <= n && p) return RPBA_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc((void **)&p, sizeof(T) * std::max<size_t>(cnt, 1)) != hipSuccess) return RPBA_E_NOMEM;
+        n = cnt;
+        return RPBA_OK;
+    }
+    int put(const T *h, size_t cnt, hipStream_t st)
+    {
+        int rc = alloc(cnt);
+        if (rc) return rc;
+        if (cnt) HIPCHK(hipMemcpyAsync(p, h, sizeof(T) * cnt, hipMemcpyHostToDevice, st));
+        return RPBA_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct Timed {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+double cpu_seconds()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// quaternion re-orthonormalisation (RectifyRotationMatrix4, rpba.h:923-960) of
+// the 3x3 part of a row-major 3x4 matrix
+void rectify34(double *P)
+{
+#define E(r, c) P[(r)*4 + (c)]
+    double a0 = E(2, 1) - E(1, 2), a1 = E(0, 2) - E(2, 0), a2 = E(1, 0) - E(0, 1);
+    if (std::fabs(a0) < 1.e-20) a0 = 1.e-20;
+    if (std::fabs(a1) < 1.e-20) a1 = 1.e-20;
+    if (std::fabs(a2) < 1.e-20) a2 = 1.e-20;
+    const double al = std::sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    a0 /= al;
+    a1 /= al;
+    a2 /= al;
+    const double phi = std::atan2(al, E(0, 0) + E(1, 1) + E(2, 2) - 1.);
+    const double q0 = std::cos(phi * 0.5), sh = std::sin(phi * 0.5);
+    const double q1 = a0 * sh, q2 = a1 * sh, q3 = a2 * sh;
+    E(0, 0) = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
+    E(0, 1) = 2 * (q1 * q2 - q0 * q3);
+    E(0, 2) = 2 * (q1 * q3 + q0 * q2);
+    E(1, 0) = 2 * (q2 * q1 + q0 * q3);
+    E(1, 1) = q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3;
+    E(1, 2) = 2 * (q2 * q3 - q0 * q1);
+    E(2, 0) = 2 * (q3 * q1 - q0 * q2);
+    E(2, 1) = 2 * (q3 * q2 + q0 * q1);
+    E(2, 2) = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+#undef E
+}
+
+}  // namespace
+
+enum Stage { ST_SETUP = 0, ST_ITER = 1, ST_DONE = 2 };
+
+struct rpba_ctx {
+    rpba_block *io = nullptr;
+    rpba_flags fl{};
+    rpba_log_fn log = nullptr;
+    void *user = nullptr;
+    hipStream_t st = nullptr;
+    int dev = 0;
+
+    // problem
+    int nc = 0, nraddpi = 0, addf[3] = {0, 0, 0};
+    int normcamera = -1, esvmaxm = 0, lflflag = 0, minimagenrdel = 2;
+    int nrioff = 0, nrioff3 = 0;
+    float ihw2 = 1.f;
+    double defdiff = 5000.;
+    Structure s, s_init;
+    std::vector<float> w0, w0_init;  // Wijoldv, 3 per track entry
+    std::vector<double> X, X_init;   // accepted points (host copy at pass boundaries)
+    std::vector<double> Pst, Ist;    // stored cameras (host copy at pass boundaries)
+    std::vector<double> Pst_init, Ist_init, R0, C0;
+    bool structure_dirty = true;
+
+    // device buffers
+    DBuf<int> d_toff, d_tcam, d_tpt, d_camoff, d_camq, d_pa, d_pb, d_poff, d_bk, d_bl, d_roff, d_coff, d_cblk;
+    DBuf<float> d_ox, d_oy, d_wx, d_wy, d_wxy, d_w0x, d_w0y, d_w0xy, d_key, d_s00;
+    DBuf<unsigned char> d_flag, d_flagst;
+    DBuf<double> d_X, d_Xn, d_Wo, d_Yo, d_Vinv, d_gp, d_Sd, d_Su, d_b, d_Cinv;
+    DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_rowdot, d_partial, d_scal;
+    DBuf<int> d_ipartial, d_iscal;
+    DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
+    DBuf<unsigned> c_mask;
+    DBuf<PcgScalars> d_pcg;
+
+    // LM state
+    int stage = ST_SETUP, it = 0, iflag = 0, iter = 0, itt = 0;
+    bool ittflag = true;
+    double sig0 = 0, sig0old = 0, sig0in = 0, wsum = 0, wsumg = 0, mult = 1.e-3, frac = 1.05;
+    int imagepointnr = 0, nrobs = 0, nrunknowns = 0, redundancy = 0;
+    int nrobsnew = 0, nrunknownsnew = 0, redundancynew = 0, nrunknownsgiven = 0, nrobsgiven = 0;
+    int status = RPBA_OK;
+    int64_t pcg_total = 0;
+    int pcg_last = 0;
+    rpba_stats stats{};
+    std::vector<double> tr_sig0;
+    std::vector<int> tr_acc, tr_pcg, tr_nrobs;
+
+    // timing
+    bool timing = false;
+    std::vector<Timed> pending;
+    std::vector<hipEvent_t> evpool;
+    std::map<std::string, std::pair<double, int64_t>> ktime;
+
+    CamPtrs cam() const
+    {
+        CamPtrs c;
+        c.Rb = c_Rb.p; c.Cb = c_Cb.p; c.Rn = c_Rn.p; c.Cn = c_Cn.p; c.In = c_In.p;
+        c.th = c_th.p; c.thn = c_thn.p; c.Pst = c_Pst.p; c.Ist = c_Ist.p; c.mask = c_mask.p;
+        return c;
+    }
+
+    void logs(const char *line)
+    {
+        if (log) log(line, user);
+    }
+
+    hipEvent_t ev()
+    {
+        if (!evpool.empty()) {
+            hipEvent_t e = evpool.back();
+            evpool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    template <class F>
+    void timed(const char *name, F f)
+    {
+        if (!timing) {
+            f();
+            return;
+        }
+        Timed t{name, ev(), ev()};
+        (void)hipEventRecord(t.a, st);
+        f();
+        (void)hipEventRecord(t.b, st);
+        pending.push_back(t);
+    }
+    void resolve_timing()
+    {
+        for (auto &t : pending) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+                auto &e = ktime[t.name];
+                e.first += ms;
+                e.second += 1;
+            }
+            evpool.push_back(t.a);
+            evpool.push_back(t.b);
+        }
+        pending.clear();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// set-up
+// ---------------------------------------------------------------------------
+
+static int upload_structure(rpba_ctx *c)
+{
+    Structure &s = c->s;
+    hipStream_t st = c->st;
+    const int empty = build_camera_major(s);
+    if (empty >= 0) return -1000 - empty;
+    build_pairs(s);
+    const size_t ntr = (size_t)s.ntr, np = (size_t)s.npts;
+    std::vector<float> ox(ntr), oy(ntr), w0x(ntr), w0y(ntr), w0xy(ntr);
+    for (size_t q = 0; q < ntr; ++q) {
+        const int fo = s.tfo[q];
+        ox[q] = c->io->obs_xy[2 * (size_t)fo];
+        oy[q] = c->io->obs_xy[2 * (size_t)fo + 1];
+        w0x[q] = c->w0[3 * q];
+        w0y[q] = c->w0[3 * q + 1];
+        w0xy[q] = c->w0[3 * q + 2];
+    }
+    int rc = 0;
+    rc |= c->d_toff.put(s.toff.data(), s.toff.size(), st);
+    rc |= c->d_tcam.put(s.tcam.data(), ntr, st);
+    rc |= c->d_tpt.put(s.tpt.data(), ntr, st);
+    rc |= c->d_camoff.put(s.camoff.data(), s.camoff.size(), st);
+    rc |= c->d_camq.put(s.camq.data(), ntr, st);
+    rc |= c->d_pa.put(s.pa.data(), s.pa.size(), st);
+    rc |= c->d_pb.put(s.pb.data(), s.pb.size(), st);
+    rc |= c->d_poff.put(s.blk_poff.data(), s.blk_poff.size(), st);
+    rc |= c->d_bk.put(s.blk_k.data(), s.blk_k.size(), st);
+    rc |= c->d_bl.put(s.blk_l.data(), s.blk_l.size(), st);
+    rc |= c->d_roff.put(s.row_off.data(), s.row_off.size(), st);
+    rc |= c->d_coff.put(s.col_off.data(), s.col_off.size(), st);
+    rc |= c->d_cblk.put(s.col_blk.data(), s.col_blk.size(), st);
+    rc |= c->d_ox.put(ox.data(), ntr, st);
+    rc |= c->d_oy.put(oy.data(), ntr, st);
+    rc |= c->d_w0x.put(w0x.data(), ntr, st);
+    rc |= c->d_w0y.put(w0y.data(), ntr, st);
+    rc |= c->d_w0xy.put(w0xy.data(), ntr, st);
+    rc |= c->d_wx.put(w0x.data(), ntr, st);
+    rc |= c->d_wy.put(w0y.data(), ntr, st);
+    rc |= c->d_wxy.put(w0xy.data(), ntr, st);
+    rc |= c->d_key.alloc(ntr);
+    rc |= c->d_flag.alloc(ntr);
+    rc |= c->d_flagst.alloc(ntr);
+    rc |= c->d_X.put(c->X.data(), 3 * np, st);
+    rc |= c->d_Xn.put(c->X.data(), 3 * np, st);
+    rc |= c->d_Wo.alloc(27 * ntr);
+    rc |= c->d_Yo.alloc(27 * ntr);
+    rc |= c->d_Vinv.alloc(9 * np);
+    rc |= c->d_gp.alloc(3 * np);
+    rc |= c->d_Su.alloc(81 * (size_t)s.nblocks());
+    const int nb = std::max(cost_blocks(s.ntr), (int)((np + 255) / 256 + 1));
+    rc |= c->d_partial.alloc((size_t)nb);
+    rc |= c->d_ipartial.alloc((size_t)nb);
+    if (rc) return RPBA_E_NOMEM;
+    HIPCHK(hipMemsetAsync(c->d_flag.p, 0, ntr, st));
+    HIPCHK(hipMemsetAsync(c->d_flagst.p, 0, ntr, st));
+    HIPCHK(hipStreamSynchronize(st));  // the staging vectors above go out of scope
+    c->structure_dirty = false;
+    return RPBA_OK;
+}
+
+// EncodeCameras on the host: rectify the stored rotations (k >= 1), base
+// R and C = R^T t, zero pose increments, intrinsics into theta, the gauge
+// camera's fixed C component (rbacore.cpp:999-1087)
+static int encode_cameras(rpba_ctx *c)
+{
+    const int nc = c->nc;
+    std::vector<double> Rb(9 * (size_t)nc), Cb(3 * (size_t)nc), th(9 * (size_t)nc, 0.);
+    std::vector<unsigned> mask((size_t)nc, 0);
+    for (int k = 0; k < nc; ++k) {
+        double *P = c->Pst.data() + 12 * (size_t)k;
+        if (k > 0) rectify34(P);
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 3; ++cc) Rb[9 * (size_t)k + r * 3 + cc] = P[r * 4 + cc];
+        const double *R = Rb.data() + 9 * (size_t)k;
+        for (int cc = 0; cc < 3; ++cc)
+            Cb[3 * (size_t)k + cc] = R[0 * 3 + cc] * P[3] + R[1 * 3 + cc] * P[7] + R[2 * 3 + cc] * P[11];
+        unsigned m = 0;
+        if (k > 0) m |= 0x3fu;
+        if (c->addf[0]) m |= 1u << 6;
+        if (c->addf[1]) m |= 1u << 7;
+        if (c->addf[2]) m |= 1u << 8;
+        if (k > 0 && k == c->normcamera) {
+            const double *C = Cb.data() + 3 * (size_t)k;
+            float maxv = 0.f;
+            int maxi = 0;
+            for (int i = 0; i < 3; ++i)
+                if (std::fabs(C[i]) > maxv) {
+                    maxv = (float)std::fabs(C[i]);
+                    maxi = i;
+                }
+            c->esvmaxm = maxi;
+            m &= ~(1u << (3 + maxi));
+        }
+        mask[k] = m;
+        th[9 * (size_t)k + 6] = c->Ist[3 * (size_t)k + 0];
+        th[9 * (size_t)k + 7] = c->Ist[3 * (size_t)k + 1];
+        th[9 * (size_t)k + 8] = c->Ist[3 * (size_t)k + 2];
+    }
+    c->nrioff3 = c->nrioff + 3 * c->s.npts;
+    int rc = 0;
+    rc |= c->c_Rb.put(Rb.data(), Rb.size(), c->st);
+    rc |= c->c_Cb.put(Cb.data(), Cb.size(), c->st);
+    rc |= c->c_th.put(th.data(), th.size(), c->st);
+    rc |= c->c_thn.put(th.data(), th.size(), c->st);
+    rc |= c->c_mask.put(mask.data(), mask.size(), c->st);
+    rc |= c->c_Pst.put(c->Pst.data(), c->Pst.size(), c->st);
+    rc |= c->c_Ist.put(c->Ist.data(), c->Ist.size(), c->st);
+    if (rc) return RPBA_E_NOMEM;
+    HIPCHK(hipStreamSynchronize(c->st));  // host vectors above go out of scope
+    return RPBA_OK;
+}
+
+static void reset_state(rpba_ctx *c)
+{
+    c->stage = ST_SETUP;
+    c->it = 0;
+    c->itt = 0;
+    c->iter = 0;
+    c->iflag = 0;
+    c->ittflag = true;
+    c->mult = 1.e-3;
+    c->frac = c->lflflag ? (double)1.01f : (double)1.05f;
+    c->status = RPBA_OK;
+    c->pcg_total = 0;
+    c->pcg_last = 0;
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    c->tr_sig0.clear();
+    c->tr_acc.clear();
+    c->tr_pcg.clear();
+    c->tr_nrobs.clear();
+}
+
+// trial pose of image 0 (PrepareIteration rbacore.cpp:151-155): never decoded
+static int upload_initial_cameras(rpba_ctx *c)
+{
+    const int nc = c->nc;
+    std::vector<double> Rn(9 * (size_t)nc, 0.), Cn(3 * (size_t)nc, 0.), In(3 * (size_t)nc);
+    for (int j = 0; j < 9; ++j) Rn[j] = c->R0[j];
+    for (int j = 0; j < 3; ++j) Cn[j] = c->C0[j];
+    for (size_t j = 0; j < In.size(); ++j) In[j] = c->Ist[j];
+    int rc = 0;
+    rc |= c->c_Rn.put(Rn.data(), Rn.size(), c->st);
+    rc |= c->c_Cn.put(Cn.data(), Cn.size(), c->st);
+    rc |= c->c_In.put(In.data(), In.size(), c->st);
+    if (rc) return RPBA_E_NOMEM;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return RPBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device steps
+// ---------------------------------------------------------------------------
+
+static LinArgs lin_args(rpba_ctx *c)
+{
+    LinArgs a;
+    a.nc = c->nc; a.npts = c->s.npts; a.ntr = c->s.ntr;
+    a.defdiff = c->defdiff; a.mult1 = 1. + c->mult;
+    a.cam = c->cam();
+    a.toff = c->d_toff.p; a.tcam = c->d_tcam.p; a.tpt = c->d_tpt.p; a.camoff = c->d_camoff.p; a.camq = c->d_camq.p;
+    a.ox = c->d_ox.p; a.oy = c->d_oy.p; a.wx = c->d_wx.p; a.wy = c->d_wy.p; a.wxy = c->d_wxy.p;
+    a.X = c->d_X.p;
+    a.Wo = c->d_Wo.p; a.Yo = c->d_Yo.p; a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.Sd = c->d_Sd.p; a.bvec = c->d_b.p;
+    return a;
+}
+
+static PcgArgs pcg_args(rpba_ctx *c)
+{
+    PcgArgs a;
+    a.nc = c->nc;
+    a.msize = std::max(1000, c->nrioff);  // rbacore.cpp:1848-1849
+    a.Sd = c->d_Sd.p; a.Su = c->d_Su.p; a.Cinv = c->d_Cinv.p; a.b = c->d_b.p;
+    a.row_off = c->d_roff.p; a.col_off = c->d_coff.p; a.col_blk = c->d_cblk.p; a.blk_k = c->d_bk.p; a.blk_l = c->d_bl.p;
+    a.x = c->d_x.p; a.best = c->d_best.p; a.r = c->d_r.p; a.d = c->d_d.p; a.Ad = c->d_Ad.p;
+    a.h = c->d_h.p; a.h1 = c->d_h1.p; a.rowdot = c->d_rowdot.p; a.S = c->d_pcg.p;
+    return a;
+}
+
+static StepArgs step_args(rpba_ctx *c, const double *sol)
+{
+    StepArgs a;
+    a.nc = c->nc; a.npts = c->s.npts; a.cam = c->cam();
+    a.toff = c->d_toff.p; a.tcam = c->d_tcam.p;
+    a.Wo = c->d_Wo.p; a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.sol = sol; a.X = c->d_X.p; a.Xn = c->d_Xn.p;
+    a.S = c->d_pcg.p;
+    return a;
+}
+
+static CostArgs cost_args(rpba_ctx *c)
+{
+    CostArgs a;
+    a.ntr = c->s.ntr; a.robust = c->fl.robust != 0; a.write_key = 1; a.defdiff = c->defdiff;
+    a.cam = c->cam(); a.tcam = c->d_tcam.p; a.tpt = c->d_tpt.p;
+    a.ox = c->d_ox.p; a.oy = c->d_oy.p; a.wx = c->d_wx.p; a.wy = c->d_wy.p; a.wxy = c->d_wxy.p;
+    a.w0x = c->d_w0x.p; a.w0y = c->d_w0y.p; a.w0xy = c->d_w0xy.p;
+    a.X = c->d_Xn.p; a.key = c->d_key.p; a.partial = c->d_partial.p;
+    return a;
+}
+
+// trial cost at (trial cameras, Xn); sets the float keys (srmvflag = true)
+static int trial_cost(rpba_ctx *c, double &cost)
+{
+    CostArgs a = cost_args(c);
+    c->timed("cost", [&] { launch_cost(a, c->d_scal.p, c->st); });
+    HIPCHK(hipMemcpyAsync(&cost, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return RPBA_OK;
+}
+
+// CalcBundleSep (rbacore.cpp:1094-1195) + theta_new = theta - x + decode
+static int bundle_step(rpba_ctx *c)
+{
+    LinArgs la = lin_args(c);
+    SchurArgs sa;
+    sa.nblk = c->s.nblocks(); sa.poff = c->d_poff.p; sa.pa = c->d_pa.p; sa.pb = c->d_pb.p;
+    sa.Yo = c->d_Yo.p; sa.Wo = c->d_Wo.p; sa.Su = c->d_Su.p;
+    c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
+    c->timed("schur_pairs", [&] { launch_schur_pairs(sa, c->st); });
+    c->timed("lin_cameras", [&] { launch_lin_cameras(la, c->st); });
+    c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
+    PcgArgs pa = pcg_args(c);
+    c->timed("pcg_init", [&] { launch_pcg_init(pa, c->st); });
+    PcgScalars hs{};
+    int chunk = 8, launched = 0;
+    for (;;) {
+        for (int j = 0; j < chunk; ++j) {
+            c->timed("spmv", [&] { launch_spmv(pa, c->st); });
+            c->timed("pcg_update", [&] { launch_pcg_update(pa, c->st); });
+        }
+        launched += chunk;
+        HIPCHK(hipMemcpyAsync(&hs, c->d_pcg.p, sizeof(hs), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (hs.done) break;
+        if (launched > pa.msize + 8) return RPBA_E_HIP;  // cannot happen: the kernel stops at msize
+        chunk = std::min(chunk * 2, 64);
+    }
+    c->pcg_last = hs.iters;
+    c->pcg_total += hs.iters;
+    StepArgs sa2 = step_args(c, c->d_best.p);
+    HIPCHK(hipMemsetAsync(&c->d_pcg.p->nan_flag, 0, sizeof(int), c->st));
+    c->timed("backsub", [&] { launch_backsub(sa2, c->st); });
+    // NaN / inf check of the whole step before anything is decoded
+    // (rbacore.cpp:541-546): camera part checked in double as in the reference
+    int nanf = 0;
+    HIPCHK(hipMemcpyAsync(&nanf, &c->d_pcg.p->nan_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    std::vector<double> xc(9 * (size_t)c->nc);
+    HIPCHK(hipMemcpyAsync(xc.data(), c->d_best.p, sizeof(double) * xc.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (double v : xc) {
+        const float f = (float)v;
+        if (std::isnan(f) || std::isinf(f)) nanf = 1;
+    }
+    if (nanf) return RPBA_E_NAN_RESULT;
+    c->timed("decode", [&] { launch_decode(sa2, c->st); });
+    return RPBA_OK;
+}
+
+static int device_store(rpba_ctx *c)
+{
+    launch_store_cams(c->cam(), c->nc, c->st);
+    HIPCHK(hipMemcpyAsync(c->d_X.p, c->d_Xn.p, sizeof(double) * 3 * (size_t)c->s.npts, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_flagst.p, c->d_flag.p, (size_t)c->s.ntr, hipMemcpyDeviceToDevice, c->st));
+    c->nrobsnew = c->nrobs;
+    c->nrunknownsnew = c->nrunknowns;
+    c->redundancynew = c->redundancy;
+    return RPBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the state machine
+// ---------------------------------------------------------------------------
+
+static int pass_setup(rpba_ctx *c)
+{
+    char buf[256];
+    c->sig0old = (double)1.e20f;
+    if (c->it > 0) c->mult = 1.e-6;
+    c->imagepointnr = c->s.ntr;
+    c->nrobs = 2 * c->imagepointnr;
+    if (c->structure_dirty) {
+        const int rc = upload_structure(c);
+        if (rc <= -1000) {
+            if (c->nc > 3) {
+                std::snprintf(buf, sizeof buf, "BA-Error: No observations in image %d (image index)\n", -1000 - rc);
+                c->logs(buf);
+            }
+            return RPBA_E_NO_OBS_IN_IMAGE;
+        }
+        if (rc) return rc;
+    }
+    if (c->ittflag) {
+        // PrepareWeights2D: Wij = original weights, wsum in point order
+        double ws = 0.;
+        for (int q = 0; q < c->s.ntr; ++q) ws += (double)c->w0[3 * (size_t)q] + (double)c->w0[3 * (size_t)q + 1];
+        c->wsum = ws;
+        HIPCHK(hipMemcpyAsync(c->d_wx.p, c->d_w0x.p, sizeof(float) * c->s.ntr, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->d_wy.p, c->d_w0y.p, sizeof(float) * c->s.ntr, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->d_wxy.p, c->d_w0xy.p, sizeof(float) * c->s.ntr, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemsetAsync(c->d_flag.p, 0, (size_t)c->s.ntr, c->st));
+        HIPCHK(hipMemsetAsync(c->d_flagst.p, 0, (size_t)c->s.ntr, c->st));
+        c->ittflag = false;
+    }
+    c->wsum = (double)(2 * c->imagepointnr) / c->wsum;
+    c->wsumg = c->wsum;
+    int rc = encode_cameras(c);
+    if (rc) return rc;
+    c->nrunknowns = c->nrioff3 + c->nraddpi * c->nc;  // counts the intrinsics twice (rbacore.cpp:462)
+    c->redundancy = c->nrobs - c->nrunknowns;
+    if (c->redundancy < 0) {
+        if (c->nc > 3) c->logs("BA-Error: Low redundancy\n");
+        return RPBA_E_LOW_REDUNDANCY;
+    }
+    if (c->it == 0) {
+        StepArgs sa = step_args(c, nullptr);
+        launch_decode(sa, c->st);
+        HIPCHK(hipMemcpyAsync(c->d_Xn.p, c->d_X.p, sizeof(double) * 3 * (size_t)c->s.npts, hipMemcpyDeviceToDevice, c->st));
+        double cost = 0.;
+        rc = trial_cost(c, cost);
+        if (rc) return rc;
+        c->sig0 = cost * c->wsumg / (double)c->redundancy;
+        c->sig0in = c->sig0old = c->sig0;
+    } else {
+        c->sig0in = c->sig0old;
+    }
+    c->iflag = c->fl.robust ? 2 : 1;
+    c->iter = 1;
+    c->nrunknownsgiven = c->nrunknowns;
+    c->nrobsgiven = c->nrobs;
+    c->stage = ST_ITER;
+    return RPBA_OK;
+}
+
+static int pass_end(rpba_ctx *c)
+{
+    char buf[256];
+    const double s_in = std::sqrt(c->sig0in * (double)c->ihw2);
+    const double s_out = std::sqrt(c->sig0old * (double)c->ihw2);
+    int red, nob, nun;
+    if (c->it == 0 && c->fl.robust) {
+        red = c->redundancynew; nob = c->nrobsnew; nun = c->nrunknownsnew;
+    } else {
+        red = c->redundancy; nob = c->nrobs; nun = c->nrunknowns;
+    }
+    std::snprintf(buf, sizeof buf, "Sigma0: %g %g  %d %d %d  IT: %d  ITT: %d\n", s_in, s_out, red, nob, nun, c->it, c->itt);
+    c->logs(buf);
+    c->stats.sigma_in[c->it] = s_in;
+    c->stats.sigma_out[c->it] = s_out;
+    c->stats.redundancy[c->it] = red;
+    c->stats.nobs[c->it] = nob;
+    c->stats.nunknowns[c->it] = nun;
+    c->stats.passes_done = c->it + 1;
+    // host copies of the stored state
+    c->X.resize(3 * (size_t)c->s.npts);
+    HIPCHK(hipMemcpyAsync(c->X.data(), c->d_X.p, sizeof(double) * c->X.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->Pst.data(), c->c_Pst.p, sizeof(double) * c->Pst.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->Ist.data(), c->c_Ist.p, sizeof(double) * c->Ist.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->fl.robust && c->it == 0) {
+        std::vector<uint8_t> fl((size_t)c->s.ntr);
+        HIPCHK(hipMemcpy(fl.data(), c->d_flagst.p, fl.size(), hipMemcpyDeviceToHost));
+        robust_delete(c->s, fl, c->minimagenrdel, c->w0, c->X);
+        c->structure_dirty = true;
+    }
+    c->ittflag = true;
+    c->it++;
+    c->stage = (c->it >= 2) ? ST_DONE : ST_SETUP;
+    return RPBA_OK;
+}
+
+static int lm_iteration(rpba_ctx *c)
+{
+    if (c->it == 0 && c->fl.robust) {
+        c->nrunknowns = c->nrunknownsgiven;
+        c->nrobs = c->nrobsgiven;
+        RobustArgs ra;
+        ra.nc = c->nc; ra.ntr = c->s.ntr; ra.iflag = c->iflag;
+        ra.camoff = c->d_camoff.p; ra.camq = c->d_camq.p; ra.tpt = c->d_tpt.p; ra.toff = c->d_toff.p; ra.tcam = c->d_tcam.p;
+        ra.key = c->d_key.p; ra.w0x = c->d_w0x.p; ra.w0y = c->d_w0y.p; ra.w0xy = c->d_w0xy.p;
+        ra.s00 = c->d_s00.p; ra.wx = c->d_wx.p; ra.wy = c->d_wy.p; ra.wxy = c->d_wxy.p;
+        ra.flag = c->d_flag.p; ra.partial = c->d_partial.p; ra.ipartial = c->d_ipartial.p;
+        c->timed("reweight", [&] { launch_reweight(ra, c->d_scal.p, c->d_iscal.p, c->st); });
+        double ws = 0.;
+        int nflag = 0;
+        HIPCHK(hipMemcpyAsync(&ws, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(&nflag, c->d_iscal.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->nrobs -= 2 * nflag;
+        c->redundancy = c->nrobs - c->nrunknowns;
+        c->wsum = (double)(2 * c->imagepointnr) / ws;
+        c->wsumg = c->wsum;
+    }
+    int rc = bundle_step(c);
+    if (rc == RPBA_E_NAN_RESULT) {
+        c->logs("BA-Error: NANs in result vector\n");
+        return rc;
+    }
+    if (rc) return rc;
+    double cost = 0.;
+    rc = trial_cost(c, cost);
+    if (rc) return rc;
+    c->sig0 = cost * c->wsumg / (double)c->redundancy;
+    if (c->sig0old < c->sig0 * c->frac) --c->iflag;
+    int acc = 0;
+    if (c->sig0 <= c->sig0old || c->iter == 1) {
+        rc = device_store(c);
+        if (rc) return rc;
+        c->sig0old = c->sig0;
+        acc = 1;
+    }
+    c->tr_sig0.push_back(c->sig0);
+    c->tr_acc.push_back(acc);
+    c->tr_pcg.push_back(c->pcg_last);
+    c->tr_nrobs.push_back(c->nrobs);
+    if (c->timing) {
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->resolve_timing();
+    }
+    if (c->itt > 200) return 1;  // break out of the pass (rbacore.cpp:575-576)
+    ++c->itt;
+    ++c->iter;
+    return (c->iflag > 0) ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+static int ctx_init(rpba_ctx *c)
+{
+    rpba_block *io = c->io;
+    const int nc = io->n_cams;
+    c->nc = nc;
+    c->addf[0] = c->fl.est_k2 != 0;
+    c->addf[1] = c->fl.est_k4 != 0;
+    c->addf[2] = c->fl.est_f != 0;
+    c->nraddpi = c->addf[0] + c->addf[1] + c->addf[2];
+    float ihw2 = 0.f;
+    for (int i = 0; i < nc; ++i) {
+        const int w = io->img_w ? io->img_w[i] : 1, h = io->img_h ? io->img_h[i] : 1;
+        ihw2 += (float)((h + w) * (h + w)) * 0.25f;
+    }
+    c->ihw2 = ihw2 / (float)nc;
+    c->defdiff = 5000. / (double)sqrtf(c->ihw2);  // rbacore.cpp:48
+    c->minimagenrdel = nc > 2 ? 3 : 2;
+    int rc = build_tracks(io, c->s);
+    if (rc) return rc;
+    c->X.assign(3 * (size_t)c->s.npts, 0.);
+    for (int i = 0; i < c->s.npts; ++i)
+        for (int j = 0; j < 3; ++j) c->X[3 * (size_t)i + j] = (double)io->pts[3 * (size_t)c->s.nrpv[i] + j];
+    c->w0.resize(3 * (size_t)c->s.ntr);
+    for (int q = 0; q < c->s.ntr; ++q) {
+        const int fo = c->s.tfo[q];
+        c->w0[3 * (size_t)q] = io->obs_w ? io->obs_w[3 * (size_t)fo] : 1.f;
+        c->w0[3 * (size_t)q + 1] = io->obs_w ? io->obs_w[3 * (size_t)fo + 1] : 1.f;
+        c->w0[3 * (size_t)q + 2] = io->obs_w ? io->obs_w[3 * (size_t)fo + 2] : 0.f;
+    }
+    c->Pst.resize(12 * (size_t)nc);
+    c->Ist.resize(3 * (size_t)nc);
+    for (int k = 0; k < nc; ++k) {
+        for (int j = 0; j < 12; ++j) c->Pst[12 * (size_t)k + j] = (double)io->cam_P[12 * (size_t)k + j];
+        c->Ist[3 * (size_t)k] = (double)io->cam_d[2 * (size_t)k];
+        c->Ist[3 * (size_t)k + 1] = (double)io->cam_d[2 * (size_t)k + 1];
+        c->Ist[3 * (size_t)k + 2] = (double)io->cam_f[k];
+    }
+    // PrepareCameras: gauge camera from the unrectified matrices (rbacore.cpp:670-700)
+    {
+        const double *P0 = c->Pst.data();
+        double T0[3];
+        for (int r = 0; r < 3; ++r) T0[r] = P0[r * 4] * P0[3] + P0[r * 4 + 1] * P0[7] + P0[r * 4 + 2] * P0[11];
+        double nd = 0.;
+        c->normcamera = nc > 1 ? 1 : -1;
+        for (int i = 1; i < nc; ++i) {
+            const double *P = c->Pst.data() + 12 * (size_t)i;
+            double d = 0.;
+            for (int r = 0; r < 3; ++r) {
+                const double T = P[r * 4] * P[3] + P[r * 4 + 1] * P[7] + P[r * 4 + 2] * P[11];
+                d += (T - T0[r]) * (T - T0[r]);
+            }
+            if (d > nd) {
+                nd = d;
+                c->normcamera = i;
+            }
+        }
+    }
+    // PrepareIteration: rectify image 0, its trial pose, lflflag, nrioff
+    rectify34(c->Pst.data());
+    c->R0.assign(9, 0.);
+    c->C0.assign(3, 0.);
+    for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc) c->R0[r * 3 + cc] = c->Pst[r * 4 + cc];
+    for (int cc = 0; cc < 3; ++cc)
+        c->C0[cc] = c->R0[0 * 3 + cc] * c->Pst[3] + c->R0[1 * 3 + cc] * c->Pst[7] + c->R0[2 * 3 + cc] * c->Pst[11];
+    c->lflflag = 0;
+    for (int k = 0; k < nc; ++k)
+        if (io->cam_f[k] > 2.f) c->lflflag = 1;
+    c->nrioff = c->nraddpi + (5 + c->nraddpi) + (6 + c->nraddpi) * (nc - 2);
+    c->s_init = c->s;
+    c->w0_init = c->w0;
+    c->X_init = c->X;
+    c->Pst_init = c->Pst;
+    c->Ist_init = c->Ist;
+
+    HIPCHK(hipGetDevice(&c->dev));
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    const size_t n9 = 9 * (size_t)nc;
+    rc = 0;
+    rc |= c->d_Sd.alloc(81 * (size_t)nc);
+    rc |= c->d_Cinv.alloc(81 * (size_t)nc);
+    rc |= c->d_b.alloc(n9);
+    rc |= c->d_x.alloc(n9);
+    rc |= c->d_best.alloc(n9);
+    rc |= c->d_r.alloc(n9);
+    rc |= c->d_d.alloc(n9);
+    rc |= c->d_Ad.alloc(n9);
+    rc |= c->d_h.alloc(n9);
+    rc |= c->d_h1.alloc(n9);
+    rc |= c->d_rowdot.alloc((size_t)nc);
+    rc |= c->d_s00.alloc((size_t)nc);
+    rc |= c->d_scal.alloc(4);
+    rc |= c->d_iscal.alloc(4);
+    rc |= c->d_pcg.alloc(1);
+    rc |= c->c_Rb.alloc(n9);
+    rc |= c->c_Cb.alloc(3 * (size_t)nc);
+    if (rc) return RPBA_E_NOMEM;
+    reset_state(c);
+    c->structure_dirty = true;
+    return upload_initial_cameras(c);
+}
+
+extern "C" int rpba_version(void) { return RPBA_ABI_VERSION; }
+
+static bool partition_dispatch(const rpba_flags *f, int nc)
+{
+    int mt = f->threads;
+    if (mt < 1) mt = (int)std::thread::hardware_concurrency();
+    if (mt < 1) mt = 1;
+    const int minnrthreads = (int)std::lround((double)((float)nc / 70.f));  // rpbacore.cpp:79-85
+    const int nrthreads = std::min(mt, minnrthreads);
+    return nrthreads > 1 && nc > 5;  // rpbacore.cpp:92
+}
+
+extern "C" int rpba_ctx_create(rpba_ctx **out, rpba_block *io, const rpba_flags *flags, rpba_log_fn log, void *user)
+{
+    if (!out || !io || !flags || io->n_cams < 1 || io->n_pts < 0 || io->n_obs < 0) return RPBA_E_INVALID;
+    if (!io->obs_cam || !io->obs_pt || !io->obs_xy || !io->cam_P || !io->cam_f || !io->cam_d || !io->pts)
+        return RPBA_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return RPBA_E_HIP;
+    rpba_ctx *c = new rpba_ctx();
+    c->io = io;
+    c->fl = *flags;
+    c->log = log;
+    c->user = user;
+    const int rc = ctx_init(c);
+    if (rc) {
+        rpba_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return RPBA_OK;
+}
+
+extern "C" int rpba_ctx_run(rpba_ctx *c, int32_t max_iters, int32_t *iters_done, int32_t *finished)
+{
+    if (!c) return RPBA_E_INVALID;
+    int done = 0;
+    while (c->stage != ST_DONE && c->status == RPBA_OK && (max_iters < 0 || done < max_iters)) {
+        if (c->stage == ST_SETUP) {
+            const int rc = pass_setup(c);
+            if (rc) {
+                c->status = rc;
+                break;
+            }
+            continue;
+        }
+        const int rc = lm_iteration(c);
+        ++done;
+        if (rc < 0) {
+            c->status = rc;
+            break;
+        }
+        if (rc == 1) {
+            const int r2 = pass_end(c);
+            if (r2) {
+                c->status = r2;
+                break;
+            }
+        }
+    }
+    if (iters_done) *iters_done = done;
+    if (finished) *finished = (c->stage == ST_DONE || c->status != RPBA_OK) ? 1 : 0;
+    return c->status;
+}
+
+extern "C" int rpba_ctx_reset(rpba_ctx *c)
+{
+    if (!c) return RPBA_E_INVALID;
+    c->s = c->s_init;
+    c->w0 = c->w0_init;
+    c->X = c->X_init;
+    c->Pst = c->Pst_init;
+    c->Ist = c->Ist_init;
+    reset_state(c);
+    c->structure_dirty = true;
+    return upload_initial_cameras(c);
+}
+
+extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
+{
+    if (!c) return RPBA_E_INVALID;
+    rpba_block *io = c->io;
+    const int nc = c->nc;
+    std::vector<double> X(3 * (size_t)c->s.npts), P(12 * (size_t)nc), In(3 * (size_t)nc);
+    // the device holds the stored state once a pass has been set up; an error
+    // before that leaves the host copies (inputs, image 0 rectified) current
+    const bool dev_pts = !c->structure_dirty && c->d_X.p && c->d_X.n >= X.size();
+    const bool dev_cams = c->c_Pst.p && c->c_Pst.n >= P.size();
+    if (dev_pts && !X.empty())
+        HIPCHK(hipMemcpyAsync(X.data(), c->d_X.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, c->st));
+    else
+        X.assign(c->X.begin(), c->X.begin() + X.size());
+    if (dev_cams)
+        HIPCHK(hipMemcpyAsync(P.data(), c->c_Pst.p, sizeof(double) * P.size(), hipMemcpyDeviceToHost, c->st));
+    else
+        P = c->Pst;
+    HIPCHK(hipMemcpyAsync(In.data(), c->c_In.p, sizeof(double) * In.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (io->obs_kept) std::memset(io->obs_kept, 0, (size_t)io->n_obs);
+    if (io->pt_kept) std::memset(io->pt_kept, 0, (size_t)io->n_pts);
+    for (int i = 0; i < c->s.npts; ++i) {
+        const int p = c->s.nrpv[i];
+        for (int j = 0; j < 3; ++j) io->pts[3 * (size_t)p + j] = (float)X[3 * (size_t)i + j];
+        if (io->pt_kept) io->pt_kept[p] = 1;
+        for (int q = c->s.toff[i]; q < c->s.toff[(size_t)i + 1]; ++q)
+            if (io->obs_kept) io->obs_kept[c->s.tfo[q]] = 1;
+    }
+    // OutputLSAc: R|t from the stored state, intrinsics from the last trial
+    for (int k = 0; k < nc; ++k) {
+        for (int j = 0; j < 12; ++j) io->cam_P[12 * (size_t)k + j] = (float)P[12 * (size_t)k + j];
+        io->cam_d[2 * (size_t)k] = (float)In[3 * (size_t)k];
+        io->cam_d[2 * (size_t)k + 1] = (float)In[3 * (size_t)k + 1];
+        io->cam_f[k] = (float)In[3 * (size_t)k + 2];
+    }
+    if (st) {
+        // inlier RMS of the returned solution, double
+        double ss = 0.;
+        int64_t n = 0;
+        for (int i = 0; i < c->s.npts; ++i)
+            for (int q = c->s.toff[i]; q < c->s.toff[(size_t)i + 1]; ++q) {
+                const int k = c->s.tcam[q], fo = c->s.tfo[q];
+                const double *Pk = P.data() + 12 * (size_t)k;
+                const double *x = X.data() + 3 * (size_t)i;
+                const double e0 = Pk[0] * x[0] + Pk[1] * x[1] + Pk[2] * x[2] + Pk[3];
+                const double e1 = Pk[4] * x[0] + Pk[5] * x[1] + Pk[6] * x[2] + Pk[7];
+                const double e2 = Pk[8] * x[0] + Pk[9] * x[1] + Pk[10] * x[2] + Pk[11];
+                const double u = e0 / e2, v = e1 / e2, rho = u * u + v * v;
+                const double sc = 1. + In[3 * (size_t)k] * rho + In[3 * (size_t)k + 1] * rho * rho;
+                const double r0 = io->obs_xy[2 * (size_t)fo] - In[3 * (size_t)k + 2] * sc * u;
+                const double r1 = io->obs_xy[2 * (size_t)fo + 1] - In[3 * (size_t)k + 2] * sc * v;
+                ss += r0 * r0 + r1 * r1;
+                n += 1;
+            }
+        rpba_stats keep = *st;
+        *st = c->stats;
+        st->itt = c->itt;
+        st->pcg_iters = c->pcg_total;
+        st->sig0_final = c->sig0old;
+        st->rms_px = n ? std::sqrt(ss / (double)(2 * n)) : 0.;
+        st->n_obs_kept = c->s.ntr;
+        st->n_pts_kept = c->s.npts;
+        st->trace_cap = keep.trace_cap;
+        st->trace_sig0 = keep.trace_sig0;
+        st->trace_accept = keep.trace_accept;
+        st->trace_pcg = keep.trace_pcg;
+        st->trace_nrobs = keep.trace_nrobs;
+        const int nt = std::min<int>((int)c->tr_sig0.size(), keep.trace_cap);
+        st->trace_len = keep.trace_sig0 ? nt : 0;
+        for (int j = 0; j < st->trace_len; ++j) {
+            st->trace_sig0[j] = c->tr_sig0[j];
+            if (st->trace_accept) st->trace_accept[j] = c->tr_acc[j];
+            if (st->trace_pcg) st->trace_pcg[j] = c->tr_pcg[j];
+            if (st->trace_nrobs) st->trace_nrobs[j] = c->tr_nrobs[j];
+        }
+    }
+    return c->status;
+}
+
+extern "C" int rpba_ctx_set_timing(rpba_ctx *c, int32_t on)
+{
+    if (!c) return RPBA_E_INVALID;
+    c->timing = on != 0;
+    c->ktime.clear();
+    return RPBA_OK;
+}
+
+extern "C" int rpba_ctx_kernel_time(rpba_ctx *c, const char *kernel, double *total_ms, int64_t *launches)
+{
+    if (!c || !kernel) return RPBA_E_INVALID;
+    auto it = c->ktime.find(kernel);
+    if (it == c->ktime.end()) {
+        if (total_ms) *total_ms = 0.;
+        if (launches) *launches = 0;
+        return RPBA_OK;
+    }
+    if (total_ms) *total_ms = it->second.first;
+    if (launches) *launches = it->second.second;
+    return RPBA_OK;
+}
+
+extern "C" int rpba_ctx_sizes(rpba_ctx *c, int64_t out[6])
+{
+    if (!c || !out) return RPBA_E_INVALID;
+    out[0] = c->nc;
+    out[1] = c->s.npts;
+    out[2] = c->s.ntr;
+    int64_t P = 0;
+    for (int i = 0; i < c->s.npts; ++i) {
+        const int64_t t = c->s.toff[(size_t)i + 1] - c->s.toff[i];
+        P += t * (t - 1) / 2;
+    }
+    out[3] = P;
+    out[4] = c->structure_dirty ? -1 : c->s.nblocks();
+    out[5] = c->nrioff;
+    return RPBA_OK;
+}
+
+extern "C" void rpba_ctx_destroy(rpba_ctx *c)
+{
+    if (!c) return;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (auto &t : c->pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto e : c->evpool) (void)hipEventDestroy(e);
+    c->d_toff.release(); c->d_tcam.release(); c->d_tpt.release(); c->d_camoff.release(); c->d_camq.release();
+    c->d_pa.release(); c->d_pb.release(); c->d_poff.release(); c->d_bk.release(); c->d_bl.release();
+    c->d_roff.release(); c->d_coff.release(); c->d_cblk.release();
+    c->d_ox.release(); c->d_oy.release(); c->d_wx.release(); c->d_wy.release(); c->d_wxy.release();
+    c->d_w0x.release(); c->d_w0y.release(); c->d_w0xy.release(); c->d_key.release(); c->d_s00.release();
+    c->d_flag.release(); c->d_flagst.release();
+    c->d_X.release(); c->d_Xn.release(); c->d_Wo.release(); c->d_Yo.release(); c->d_Vinv.release(); c->d_gp.release();
+    c->d_Sd.release(); c->d_Su.release(); c->d_b.release(); c->d_Cinv.release();
+    c->d_x.release(); c->d_best.release(); c->d_r.release(); c->d_d.release(); c->d_Ad.release();
+    c->d_h.release(); c->d_h1.release(); c->d_rowdot.release(); c->d_partial.release(); c->d_scal.release();
+    c->d_ipartial.release(); c->d_iscal.release();
+    c->c_Rb.release(); c->c_Cb.release(); c->c_Rn.release(); c->c_Cn.release(); c->c_In.release();
+    c->c_th.release(); c->c_thn.release(); c->c_Pst.release(); c->c_Ist.release(); c->c_mask.release();
+    c->d_pcg.release();
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+// Exteval on the device (EvalLSA rpbacore.cpp:882-941)
+static int exteval(rpba_block *io, const rpba_flags *f, rpba_log_fn log, void *user, rpba_stats *stats)
+{
+    const int nc = io->n_cams, np = io->n_pts, no = io->n_obs;
+    std::vector<int> cnt((size_t)np + 1, 0);
+    for (int i = 0; i < no; ++i) {
+        const int cc = io->obs_cam[i], p = io->obs_pt[i];
+        if (cc < 0) continue;
+        if (cc >= nc || p < 0 || p >= np) return RPBA_E_INVALID;
+        cnt[(size_t)p + 1]++;
+    }
+    for (int i = 0; i < np; ++i) cnt[(size_t)i + 1] += cnt[i];
+    const size_t nt = (size_t)cnt[np];
+    std::vector<int> cam(nt);
+    std::vector<float> ox(nt), oy(nt), wx(nt), wy(nt), wxy(nt);
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < no; ++i) {
+        if (io->obs_cam[i] < 0) continue;
+        const size_t q = (size_t)pos[io->obs_pt[i]]++;
+        cam[q] = io->obs_cam[i];
+        ox[q] = io->obs_xy[2 * (size_t)i];
+        oy[q] = io->obs_xy[2 * (size_t)i + 1];
+        wx[q] = io->obs_w ? io->obs_w[3 * (size_t)i] : 1.f;
+        wy[q] = io->obs_w ? io->obs_w[3 * (size_t)i + 1] : 1.f;
+        wxy[q] = io->obs_w ? io->obs_w[3 * (size_t)i + 2] : 0.f;
+    }
+    int64_t red = 0, nobs = 0;
+    for (int i = 0; i < np; ++i) {
+        const int t = cnt[(size_t)i + 1] - cnt[i];
+        if (t > 1) {
+            red += 2 * t - 3;
+            nobs += 2 * t;
+        }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return RPBA_E_HIP;
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    DBuf<int> d_off, d_cam, d_ip;
+    DBuf<float> d_ox, d_oy, d_wx, d_wy, d_wxy, d_P, d_f, d_d, d_pts;
+    DBuf<double> d_part, d_out;
+    int rc = 0;
+    rc |= d_off.put(cnt.data(), cnt.size(), st);
+    rc |= d_cam.put(cam.data(), nt, st);
+    rc |= d_ox.put(ox.data(), nt, st);
+    rc |= d_oy.put(oy.data(), nt, st);
+    rc |= d_wx.put(wx.data(), nt, st);
+    rc |= d_wy.put(wy.data(), nt, st);
+    rc |= d_wxy.put(wxy.data(), nt, st);
+    rc |= d_P.put(io->cam_P, 12 * (size_t)nc, st);
+    rc |= d_f.put(io->cam_f, (size_t)nc, st);
+    rc |= d_d.put(io->cam_d, 2 * (size_t)nc, st);
+    rc |= d_pts.put(io->pts, 3 * (size_t)np, st);
+    const int nb = (np + 255) / 256 + 1;
+    rc |= d_part.alloc((size_t)nb);
+    rc |= d_ip.alloc((size_t)nb);
+    rc |= d_out.alloc(2);
+    double sg = 0.;
+    int tsum = 0;
+    if (!rc) {
+        ExtArgs a;
+        a.npts = np; a.toff = d_off.p; a.cam = d_cam.p; a.ox = d_ox.p; a.oy = d_oy.p;
+        a.wx = d_wx.p; a.wy = d_wy.p; a.wxy = d_wxy.p; a.P = d_P.p; a.cf = d_f.p; a.cd = d_d.p; a.pts = d_pts.p;
+        a.partial = d_part.p; a.ipartial = d_ip.p;
+        launch_exteval(a, d_out.p, d_ip.p + (nb - 1), st);
+        if (hipMemcpyAsync(&sg, d_out.p, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) rc = RPBA_E_HIP;
+        if (hipStreamSynchronize(st) != hipSuccess) rc = RPBA_E_HIP;
+    }
+    (void)tsum;
+    d_off.release(); d_cam.release(); d_ip.release(); d_ox.release(); d_oy.release(); d_wx.release(); d_wy.release();
+    d_wxy.release(); d_P.release(); d_f.release(); d_d.release(); d_pts.release(); d_part.release(); d_out.release();
+    (void)hipStreamDestroy(st);
+    if (rc) return rc;
+    const int nraddp = (f->est_k2 != 0) + (f->est_k4 != 0) + (f->est_f != 0);
+    const int64_t dof = red - (int64_t)(nc - 2) * 6 - 5 - (int64_t)nc * nraddp;
+    float ihw2 = 0.f;
+    for (int i = 0; i < nc; ++i) {
+        const int w = io->img_w ? io->img_w[i] : 1, h = io->img_h ? io->img_h[i] : 1;
+        ihw2 += (float)((h + w) * (h + w)) * 0.25f;
+    }
+    ihw2 /= (float)nc;
+    sg *= 1. / (double)dof;
+    const double s = std::sqrt(sg * (double)ihw2);
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "ITTTT: %g %lld %lld\n", s, (long long)nobs, (long long)dof);
+    if (log) log(buf, user);
+    if (stats) {
+        stats->exteval_sigma = s;
+        stats->exteval_nobs = nobs;
+        stats->exteval_dof = dof;
+    }
+    return RPBA_OK;
+}
+
+extern "C" int rpba_calc_plsa(rpba_block *io, const rpba_flags *flags, rpba_log_fn log, void *user, rpba_stats *stats)
+{
+    if (!io || !flags) return RPBA_E_INVALID;
+    if (flags->exteval) return exteval(io, flags, log, user, stats);  // rpbacore.cpp:94-97
+    if (partition_dispatch(flags, io->n_cams) && !flags->force_global) {
+        if (log) log("rpba: the partitioned solve (rpbacore.cpp:100-510) is not available; set Threads = 1\n", user);
+        return RPBA_E_UNSUPPORTED;
+    }
+    rpba_ctx *c = nullptr;
+    int rc = rpba_ctx_create(&c, io, flags, log, user);
+    if (rc) return rc;
+    int done = 0, fin = 0;
+    rc = rpba_ctx_run(c, -1, &done, &fin);
+    const int rc2 = rpba_ctx_download(c, stats);
+    rpba_ctx_destroy(c);
+    return rc ? rc : rc2;
+}

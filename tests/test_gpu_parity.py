@@ -1,0 +1,134 @@
+"""HIP path vs the CPU oracle (plain FP64 mode) on the same seeded inputs,
+through the C ABI.
+
+Tolerances: per-iteration sig0 and the final sigma within 1e-6 relative (the
+FP64 bar); accept decisions, PCG iteration counts, nrobs and the robust
+labels (kept observations / points) exactly; parameters within 1e-6 of their
+scale.  The two implementations differ only in summation order."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from rpba_amd import api, capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _cmp_solves(sc, flags, tag):
+    _say(f"[{tag}] oracle solve")
+    rc0, out0, st0, log0 = oracle.solve(sc, flags, fsites=0)
+    _say(f"[{tag}] GPU solve via rpba_calc_plsa n_cams={sc['n_cams']} n_obs={sc['n_obs']}")
+    rc1, out1, st1, log1 = api.calc_plsa(sc, flags)
+    assert rc1 == rc0
+    assert st1["passes_done"] == st0["passes_done"]
+    assert st1["itt"] == st0["itt"], (st0["trace_sig0"], st1["trace_sig0"])
+    n = st0["trace_sig0"].size
+    assert st1["trace_sig0"].size == n
+    rel = np.abs(st1["trace_sig0"] - st0["trace_sig0"]) / np.abs(st0["trace_sig0"])
+    assert rel.max() < 1e-6, rel
+    assert np.array_equal(st1["trace_accept"], st0["trace_accept"])
+    assert np.array_equal(st1["trace_nrobs"], st0["trace_nrobs"])
+    assert np.abs(st1["trace_pcg"] - st0["trace_pcg"]).max() <= 1, (st0["trace_pcg"], st1["trace_pcg"])
+    for p in range(st0["passes_done"]):
+        for key in ("sigma_in", "sigma_out"):
+            assert abs(st1[key][p] - st0[key][p]) <= 1e-6 * abs(st0[key][p])
+        for key in ("redundancy", "nobs", "nunknowns"):
+            assert st1[key][p] == st0[key][p]
+    assert np.array_equal(out1["obs_kept"], out0["obs_kept"])
+    assert np.array_equal(out1["pt_kept"], out0["pt_kept"])
+    for key in ("cam_P", "cam_f", "cam_d", "pts"):
+        a, b = out1[key].astype(np.float64), out0[key].astype(np.float64)
+        scale = max(np.abs(b).max(), 1e-30)
+        assert np.abs(a - b).max() <= 1e-6 * scale + 1e-6 * np.abs(b).max(), key
+    assert len(log1) == len(log0)
+    for l0, l1 in zip(log0, log1):
+        t0, t1 = l0.split(), l1.split()
+        assert t0[0] == t1[0] and t0[-4:] == t1[-4:]
+    return st0, st1, out1
+
+
+def test_exteval_matches_oracle(scene_c1):
+    _say("[exteval c1]")
+    fl = capi.make_flags(exteval=1)
+    s0, n0, d0 = oracle.exteval(scene_c1, fl, fsites=0)
+    rc, out, st, log = api.calc_plsa(scene_c1, fl)
+    assert rc == 0
+    assert st["exteval_nobs"] == n0 and st["exteval_dof"] == d0
+    assert abs(st["exteval_sigma"] - s0) <= 1e-12 * s0
+    assert log[0].startswith("ITTTT: ")
+    # Exteval does not change the block
+    assert np.array_equal(out["pts"], scene_c1["pts"])
+
+
+def test_robust_solve_c1_matches_oracle(scene_c1):
+    st0, st1, out1 = _cmp_solves(scene_c1, capi.make_flags(robust=1), "c1 robust")
+    assert abs(st1["sigma_out"][1] - 0.5) < 0.02
+
+
+@pytest.mark.parametrize("robust,k2,k4,f", [(0, 1, 1, 1), (1, 1, 0, 1), (0, 0, 0, 0), (1, 0, 0, 1)])
+def test_solve_variants_match_oracle(robust, k2, k4, f):
+    sc = scenes.generate(None, seed=31, n_cams=12, n_pts=600, track="uniform2_8", outlier_frac=0.03)
+    _cmp_solves(sc, capi.make_flags(robust=robust, k2=k2, k4=k4, f=f), f"r{robust} k2{k2} k4{k4} f{f}")
+
+
+def test_tiny_scene(tiny_scene):
+    _cmp_solves(tiny_scene, capi.make_flags(robust=1), "tiny")
+
+
+def test_deterministic(scene_c1):
+    _say("[determinism c1]")
+    r1 = api.calc_plsa(scene_c1, capi.make_flags(robust=1))
+    r2 = api.calc_plsa(scene_c1, capi.make_flags(robust=1))
+    for key in ("cam_P", "cam_f", "cam_d", "pts", "obs_kept"):
+        assert np.array_equal(r1[1][key], r2[1][key])
+    assert np.array_equal(r1[2]["trace_sig0"], r2[2]["trace_sig0"])
+
+
+def test_context_chunks_and_reset(scene_c1):
+    _say("[ctx chunks c1]")
+    ref = api.calc_plsa(scene_c1, capi.make_flags(robust=1))
+    ctx = api.Context(scene_c1, capi.make_flags(robust=1))
+    total = 0
+    while True:
+        rc, done, fin = ctx.run(2)
+        assert rc == 0
+        total += done
+        if fin:
+            break
+    rc, out, st = ctx.download()
+    assert st["itt"] == ref[2]["itt"]
+    assert np.array_equal(st["trace_sig0"], ref[2]["trace_sig0"])
+    assert np.array_equal(out["pts"], ref[1]["pts"])
+    assert ctx.reset() == 0
+    rc, done, fin = ctx.run(-1)
+    assert rc == 0 and fin
+    rc, out2, st2 = ctx.download()
+    assert np.array_equal(st2["trace_sig0"], ref[2]["trace_sig0"])
+    ctx.close()
+
+
+def test_error_no_observations_in_image():
+    _say("[error empty image]")
+    sc = scenes.generate(None, seed=8, n_cams=6, n_pts=80, track="uniform2_8", outlier_frac=0.0)
+    m = sc["obs_cam"] != 5
+    bad = dict(sc, obs_cam=sc["obs_cam"][m], obs_pt=sc["obs_pt"][m], obs_xy=sc["obs_xy"][m], n_obs=int(m.sum()))
+    rc0, out0, _, log0 = oracle.solve(bad, capi.make_flags(robust=0))
+    rc1, out1, _, log1 = api.calc_plsa(bad, capi.make_flags(robust=0))
+    assert rc1 == rc0 == capi.RPBA_E_NO_OBS_IN_IMAGE
+    assert log1 == log0
+    assert np.array_equal(out1["pts"], out0["pts"])
+
+
+def test_partition_path_is_refused():
+    sc = scenes.generate(None, seed=9, n_cams=150, n_pts=300, track="uniform2_8", outlier_frac=0.0)
+    rc, _, _, log = api.calc_plsa(sc, capi.make_flags(threads=0))
+    assert rc == capi.RPBA_E_UNSUPPORTED
+    rc, _, st, _ = api.calc_plsa(sc, capi.make_flags(threads=0, force_global=1))
+    assert rc == 0 and st["passes_done"] == 2
