@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark: plain-FP64 robust LM bundle adjustment on one MI355X.
+
+Workload: the 961-camera synthetic scene (187,103 points, ~1.69M
+observations, track length 2 + geometric with mean 9, 0.5 px noise, 3%
+outliers, k2/k4/f estimated, Robust = 1, Threads = 1 semantics), generated
+with seed 2 and resident in HBM before timing starts.
+
+A step is one LM iteration of the reference's Adjust loop (rbacore.cpp:515-581:
+reweighting, linearisation + Schur + PCG + back-substitution, trial decode and
+cost, accept/reject).  K steps run through rpba_ctx_run, continuing across the
+pass boundary (RobustDelete + set-up of pass 1); if the solve finishes inside
+the timed region it restarts from the initial state (rpba_ctx_reset) and the
+restart is timed too.  value = K / elapsed (LM iterations per second).
+
+Also reported: time to converged (one full solve: both passes, from the
+uploaded block to the stored state on the device), the final sigma0 and RMS,
+the per-kernel HIP-event roofline of the dominant kernel, and the CPU oracle
+timed on one LM iteration of the same scene (cpu_baseline).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def device_sync():
+    """Device-wide synchronisation (hipDeviceSynchronize), the barrier around
+    the timed region."""
+    global _hip
+    try:
+        _hip
+    except NameError:
+        _hip = C.CDLL("libamdhip64.so")
+    rc = _hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def algorithmic_bytes(kernel, sz):
+    """Compulsory HBM bytes of one launch: every array the kernel must touch,
+    once.  sz: cams, pts, obs, pairs, blocks."""
+    Nc, Np, No, P, B = sz["cams"], sz["pts"], sz["obs"], sz["pairs"], sz["blocks"]
+    if kernel == "spmv":  # S blocks once (upper + diagonal), d in, Ad out, row partials
+        return 648 * (B + Nc) + 72 * Nc + 72 * Nc + 8 * Nc
+    if kernel == "schur_pairs":  # Y and W of every entry, pair lists, block offsets, S blocks out
+        return 432 * No + 8 * P + 4 * (B + 1) + 648 * B
+    if kernel == "lin_points":  # entry inputs, points, W and Y out, V^-1 and g_p out
+        return (4 + 8 + 12) * No + 432 * No + (24 + 4) * Np + (72 + 24) * Np
+    if kernel == "schur_diag":  # Y, W and g_p of every entry, diagonal terms out
+        return (432 + 4 + 4) * No + 24 * Np + 720 * Nc
+    if kernel == "lin_cameras":  # entry inputs and points in; S_kk and b out
+        return (8 + 8 + 12) * No + 24 * Np + (720 + 648 + 72) * Nc
+    if kernel == "cost":  # entry inputs, weights both sets, trial points, keys out
+        return (8 + 8 + 12 + 12 + 4) * No + 24 * Np
+    if kernel == "backsub":  # W, camera of each entry, V^-1, g_p, X in, Xn out, camera step
+        return (216 + 4) * No + (4 + 72 + 24 + 24 + 24) * Np + 72 * Nc
+    if kernel == "pcg_update":  # block-Jacobi inverse and about twelve vector passes
+        return 648 * Nc + 12 * 72 * Nc + 8 * Nc
+    return None
+
+
+def run_bench(args):
+    import numpy as np
+
+    import rpba_amd
+    from rpba_amd import api, capi, scenes
+
+    rpba_amd.lib()  # loud failure if the HIP library is missing
+    t0 = time.time()
+    sc = scenes.generate(args.config)
+    gen_s = time.time() - t0
+    flags = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1)
+    t0 = time.time()
+    ctx = api.Context(sc, flags)
+    create_s = time.time() - t0
+
+    def run_steps(n):
+        left = n
+        while left > 0:
+            rc, done, fin = ctx.run(left)
+            if rc != 0:
+                raise RuntimeError(f"rpba_ctx_run failed with status {rc}")
+            left -= done
+            if fin:
+                if ctx.reset() != 0:
+                    raise RuntimeError("rpba_ctx_reset failed")
+
+    # warmup
+    run_steps(args.warmup)
+    device_sync()
+    # timed region: exactly K LM iterations
+    device_sync()
+    t0 = time.perf_counter()
+    run_steps(args.steps)
+    device_sync()
+    elapsed = time.perf_counter() - t0
+
+    # time to converged: one whole solve from the uploaded block
+    ctx.reset()
+    device_sync()
+    t1 = time.perf_counter()
+    rc, done, fin = ctx.run(-1)
+    device_sync()
+    ttc = time.perf_counter() - t1
+    rc_d, out, st = ctx.download()
+    sizes = ctx.sizes()
+
+    # roofline: the same steps again with HIP events around every launch
+    ctx.reset()
+    ctx.set_timing(True)
+    run_steps(args.steps)
+    device_sync()
+    kernels = ["lin_points", "schur_pairs", "schur_diag", "lin_cameras", "bj_inverse", "pcg_init", "spmv", "pcg_update",
+               "backsub", "decode", "cost", "reweight"]
+    ktimes = {k: ctx.kernel_time(k) for k in kernels}
+    ctx.set_timing(False)
+    # sizes of pass 0 (the structure the dominant launches ran on)
+    ctx.reset()
+    ctx.run(1)
+    sz0 = ctx.sizes()
+    ctx.close()
+
+    dom = max((k for k in kernels if algorithmic_bytes(k, sz0) is not None and ktimes[k][1] > 0),
+              key=lambda k: ktimes[k][0])
+    tot_ms, nl = ktimes[dom]
+    avg_ms = tot_ms / nl
+    alg = algorithmic_bytes(dom, sz0)
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.config}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    line = {
+        "metric": "LM iters/s, plain FP64 robust LM Schur solve (time-to-converged reported beside)",
+        "value": round(args.steps / elapsed, 3),
+        "unit": "LM iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, BAL camera model)",
+        "config": {"workload": f"{args.config}: robust LM, {sc['n_cams']} cameras, {sc['n_pts']} points, "
+                               f"{sc['n_obs']} observations, k2 k4 f estimated",
+                   "cameras": sc["n_cams"], "points": sc["n_pts"], "observations": sc["n_obs"],
+                   "pairs": sz0["pairs"], "s_blocks": sz0["blocks"], "s_dim": sz0["sdim"],
+                   "parallelism": "single GPU"},
+        "time_to_converged_s": round(ttc, 4),
+        "itt": st["itt"],
+        "pcg_iters": st["pcg_iters"],
+        "sigma0_final": st["sigma_out"][1],
+        "rms_px": st["rms_px"],
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktimes.items() if v[1] > 0},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": round(avg_ms, 5), "launches": nl},
+        "setup_s": {"generate": round(gen_s, 2), "create": round(create_s, 2)},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(sc)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(sc):
+    """The CPU oracle (single-threaded C restatement of the reference's LSA)
+    on a bounded sample: one LM iteration of pass 0 of the same scene
+    (linearisation + Schur + PCG + back-substitution, then the trial cost)."""
+    from oracle import oracle
+    from rpba_amd import capi
+
+    h = oracle.Handle(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1))
+    h.setup_pass0()
+    t0 = time.perf_counter()
+    x, its = h.step(1e-3)
+    th = h.theta() - x
+    h.residuals(th)
+    dt = time.perf_counter() - t0
+    h.close()
+    return {"value": round(1.0 / dt, 5), "unit": "LM iters/s", "cores": 1, "kind": "port",
+            "sample": f"1 LM iteration of pass 0 ({its} PCG iterations), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2_961")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the solve is not sharded); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    run_bench(args)
+
+
+if __name__ == "__main__":
+    main()

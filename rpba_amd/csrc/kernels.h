@@ -29,11 +29,13 @@ struct LinArgs {
     const float *ox, *oy, *wx, *wy, *wxy;
     const double *X;
     double *Wo, *Yo, *Vinv, *gp, *Sd, *bvec;
+    double *Dd;  // per camera: diagonal Schur term (81) and Y g_p (9)
 };
 
 struct SchurArgs {
     int nblk;
-    const int *poff, *pa, *pb;
+    const int *poff;
+    const unsigned long long *pab;  // pair (a << 32 | b), cam(a) < cam(b)
     const double *Yo, *Wo;
     double *Su;
 };
@@ -42,8 +44,9 @@ struct PcgArgs {
     int nc, msize;
     const double *Sd, *Su, *Cinv, *b;
     const int *row_off, *col_off, *col_blk, *blk_k, *blk_l;
-    double *x, *best, *r, *d, *Ad, *h, *h1, *rowdot;
-    PcgScalars *S;
+    double *x, *best, *r, *d, *Ad, *h, *h1, *rowdot, *part;
+    PcgScalars *S;  // two slots: launch j reads S[j & 1] and writes S[(j + 1) & 1]
+    int j;          // launch index inside one PCG solve
 };
 
 struct StepArgs {
@@ -84,10 +87,11 @@ struct ExtArgs {
     int *ipartial;
 };
 
-void launch_lin(const LinArgs &a, const SchurArgs &s, hipStream_t st);
 void launch_lin_points(const LinArgs &a, hipStream_t st);
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st);
+void launch_schur_diag(const LinArgs &a, hipStream_t st);
 void launch_lin_cameras(const LinArgs &a, hipStream_t st);
+int pcg_groups(int nc);
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st);
 void launch_pcg_init(const PcgArgs &a, hipStream_t st);
 void launch_spmv(const PcgArgs &a, hipStream_t st);
@@ -97,6 +101,7 @@ void launch_decode(const StepArgs &a, hipStream_t st);
 int cost_blocks(int ntr);
 void launch_cost(const CostArgs &a, double *out, hipStream_t st);
 void launch_reweight(const RobustArgs &a, double *wsum_out, int *nflag_out, hipStream_t st);
+void launch_wsum(int n, const float *wx, const float *wy, double *partial, double *out, hipStream_t st);
 void launch_store_cams(const CamPtrs &cp, int nc, hipStream_t st);
 void launch_exteval(const ExtArgs &a, double *sg_out, int *t_out, hipStream_t st);
 

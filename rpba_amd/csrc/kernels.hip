@@ -231,12 +231,57 @@ __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// linearisation: cameras.  One workgroup (256) per camera:
-//   phase 1: U = sum Bc^T W Bc (upper 45), g_c = sum Bc^T W r
-//   phase 2: diagonal Schur term sum Y W^T (81) and sum Y g_p (9)
-// then S_kk = U - sum Y W^T with the diagonal damping (x(1+lambda), +1e-2,
-// rbacore.cpp:1164-1170) and b_k = g_c - sum Y g_p; fixed slots become an
-// identity row/column with zero right-hand side.
+// diagonal Schur term per camera, atomic-free: one wavefront per image k walks
+// its entries (camera-major order) and accumulates D = sum Y W^T (81) and
+// e = sum Y g_p (9), staging each entry's Y, W and g_p through LDS.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
+{
+    const int k = blockIdx.x;
+    const int lane = threadIdx.x;
+    __shared__ double st[8 * 57];
+    const int b = a.camoff[k], e = a.camoff[k + 1];
+    const int o0 = lane, o1 = lane + 64;
+    // output o < 81: D(o/9, o%9); 81 <= o < 90: e(o-81)
+    const int r0 = o0 / 9, c0 = o0 % 9;
+    const int r1 = (o1 < 81) ? o1 / 9 : (o1 < 90 ? o1 - 81 : 0), c1 = (o1 < 81) ? o1 % 9 : 0;
+    const bool g1 = (o1 >= 81);
+    double acc0 = 0., acc1 = 0.;
+    for (int j0 = b; j0 < e; j0 += 8) {
+        const int np = (e - j0 < 8) ? e - j0 : 8;
+        for (int x = lane; x < np * 57; x += 64) {
+            const int j = x / 57, idx = x % 57;
+            const int q = a.camq[j0 + j];
+            double v;
+            if (idx < 27)
+                v = a.Yo[(size_t)q * 27 + idx];
+            else if (idx < 54)
+                v = a.Wo[(size_t)q * 27 + (idx - 27)];
+            else
+                v = a.gp[3 * (size_t)a.tpt[q] + (idx - 54)];
+            st[x] = v;
+        }
+        __syncthreads();
+        for (int j = 0; j < np; ++j) {
+            const double *Y = st + j * 57, *W = st + j * 57 + 27, *g = st + j * 57 + 54;
+            acc0 += Y[3 * r0] * W[3 * c0] + Y[3 * r0 + 1] * W[3 * c0 + 1] + Y[3 * r0 + 2] * W[3 * c0 + 2];
+            const double *w1 = g1 ? g : W + 3 * c1;
+            acc1 += Y[3 * r1] * w1[0] + Y[3 * r1 + 1] * w1[1] + Y[3 * r1 + 2] * w1[2];
+        }
+        __syncthreads();
+    }
+    double *D = a.Dd + (size_t)k * 90;
+    D[o0] = acc0;
+    if (o1 < 90) D[o1] = acc1;
+}
+
+// ---------------------------------------------------------------------------
+// linearisation: cameras.  One workgroup (256) per camera accumulates
+// U = sum Bc^T W Bc (upper 45) and g_c = sum Bc^T W r, then forms
+// S_kk = U - D with the diagonal damping (x(1+lambda), +1e-2,
+// rbacore.cpp:1164-1170) and b_k = g_c - e; fixed slots become an identity
+// row/column with zero right-hand side.
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
@@ -244,7 +289,6 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
     const int k = blockIdx.x;
     const int t = threadIdx.x;
     __shared__ double red[4][54];
-    __shared__ double red2[2][90];
     __shared__ double Ssh[81];
     __shared__ double bsh[9];
     const unsigned mask = a.cam.mask[k];
@@ -283,32 +327,14 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
         const double s = wave_sum(acc[j]);
         if (lane == 0) red[w][j] = s;
     }
-    // phase 2: two groups of 128; thread o < 81 owns S(o/9, o%9), 81..89 b
-    const int grp = t >> 7, o = t & 127;
-    double s2 = 0.;
-    if (o < 90) {
-        const int r = (o < 81) ? o / 9 : o - 81;
-        const int cc = (o < 81) ? o % 9 : 0;
-        for (int j = b + grp; j < e; j += 2) {
-            const int q = a.camq[j];
-            const double *Y = a.Yo + (size_t)q * 27 + 3 * r;
-            if (o < 81) {
-                const double *W = a.Wo + (size_t)q * 27 + 3 * cc;
-                s2 += Y[0] * W[0] + Y[1] * W[1] + Y[2] * W[2];
-            } else {
-                const double *gp = a.gp + 3 * (size_t)a.tpt[q];
-                s2 += Y[0] * gp[0] + Y[1] * gp[1] + Y[2] * gp[2];
-            }
-        }
-        red2[grp][o] = s2;
-    }
     __syncthreads();
+    const double *D = a.Dd + (size_t)k * 90;
     if (t < 81) {
         const int r = t / 9, cc = t % 9;
         const int rr = r < cc ? r : cc, c2 = r < cc ? cc : r;
         const int n = rr * 9 - rr * (rr - 1) / 2 + (c2 - rr);  // index in the packed upper triangle
         const double U = red[0][n] + red[1][n] + red[2][n] + red[3][n];
-        double s = U - (red2[0][t] + red2[1][t]);
+        double s = U - D[t];
         if (r == cc) {
             s *= a.mult1;
             s += 1.e-2;
@@ -319,7 +345,7 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
     } else if (t >= 128 && t < 137) {
         const int r = t - 128;
         const double gc = red[0][45 + r] + red[1][45 + r] + red[2][45 + r] + red[3][45 + r];
-        double s = gc - (red2[0][81 + r] + red2[1][81 + r]);
+        double s = gc - D[81 + r];
         if (!((mask >> r) & 1u)) s = 0.;
         bsh[r] = s;
     }
@@ -350,7 +376,8 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
         const int np = (pe - p < 8) ? pe - p : 8;
         for (int x = lane; x < np * 54; x += 64) {
             const int j = x / 54, idx = x % 54;
-            st[x] = (idx < 27) ? a.Yo[(size_t)a.pa[p + j] * 27 + idx] : a.Wo[(size_t)a.pb[p + j] * 27 + (idx - 27)];
+            const unsigned long long ab = a.pab[p + j];
+            st[x] = (idx < 27) ? a.Yo[(size_t)(ab >> 32) * 27 + idx] : a.Wo[(size_t)(ab & 0xffffffffull) * 27 + (idx - 27)];
         }
         __syncthreads();
         for (int j = 0; j < np; ++j) {
@@ -428,7 +455,7 @@ __device__ __forceinline__ void apply_cinv(const double *Cinv, const double *r, 
 }
 
 // x = 0, r = b, h = C b, d = h, best = b (exivold starts as a copy of b),
-// rho = r.h.  Single workgroup of 1024.
+// rho = r.h.  Single workgroup of 1024.  Scalars live in S[0] (launch 0).
 __global__ __launch_bounds__(1024) void k_pcg_init(PcgArgs a)
 {
     __shared__ double sh[16];
@@ -448,12 +475,16 @@ __global__ __launch_bounds__(1024) void k_pcg_init(PcgArgs a)
     }
     s = block_sum(s, sh);
     if (threadIdx.x == 0) {
-        a.S->rho = s;
-        a.S->m0 = 1.e50;
-        a.S->mold = 1.e50;
-        a.S->k = 0;
-        a.S->done = 0;
-        a.S->iters = 0;
+        PcgScalars z;
+        z.rho = s;
+        z.m0 = 1.e50;
+        z.mold = 1.e50;
+        z.k = 0;
+        z.done = 0;
+        z.iters = 0;
+        z.nan_flag = 0;
+        a.S[0] = z;
+        a.S[1] = z;
     }
 }
 
@@ -462,7 +493,7 @@ __global__ __launch_bounds__(1024) void k_pcg_init(PcgArgs a)
 // transposed); also writes the row's share of d.Ad.
 __global__ __launch_bounds__(256) void k_spmv(PcgArgs a)
 {
-    if (a.S->done) return;
+    if (a.S[a.j & 1].done) return;
     const int k = blockIdx.x;
     const int t = threadIdx.x;
     const int g = t / 9, r = t % 9;
@@ -514,66 +545,111 @@ __global__ __launch_bounds__(256) void k_spmv(PcgArgs a)
     }
 }
 
-// one PCG update after the SpMV (single workgroup of 1024)
-__global__ __launch_bounds__(1024) void k_pcg_update(PcgArgs a)
+// sum of n values in a fixed order; every thread of the block gets it
+__device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
 {
-    __shared__ double sh[16];
-    PcgScalars *S = a.S;
-    if (S->done) return;
-    const int n = 9 * a.nc;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    double dAd = 0.;
-    for (int k = tid; k < a.nc; k += nt) dAd += a.rowdot[k];
-    dAd = block_sum(dAd, sh);
-    const double rho = S->rho;
+    double s = 0.;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) s += v[j];
+    return block_sum(s, sh);
+}
+
+// PCG step, part 1 (one workgroup per 28 cameras): alpha from d.Ad, x += alpha
+// d, r -= alpha Ad, h1 = C r; partial sums of |alpha Ad|^2, r.(h1 - h), r.h1
+__global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
+{
+    __shared__ double sh[4];
+    __shared__ double rs[256];
+    const PcgScalars S = a.S[a.j & 1];
+    if (S.done) return;
+    const double dAd = sum_fixed(a.rowdot, a.nc, sh);
+    const double rho = S.rho;
     const double alpha = (fabs(rho) < 1.e-20 || fabs(rho) > 1.e20) ? 0. : rho / dAd;
-    double m = 0.;
-    for (int j = tid; j < n; j += nt) {
+    const int t = threadIdx.x;
+    const int k = blockIdx.x * 28 + t / 9, r = t % 9;
+    const bool on = (t < 252) && (k < a.nc);
+    const size_t j = 9 * (size_t)k + r;
+    double m = 0., num = 0., rh = 0.;
+    if (on) {
         a.x[j] += alpha * a.d[j];
         const double dr = alpha * a.Ad[j];
-        a.r[j] -= dr;  // r becomes r_{k+1}
-        m += dr * dr;
+        const double rn = a.r[j] - dr;
+        a.r[j] = rn;
+        rs[t] = rn;
+        m = dr * dr;
+    }
+    __syncthreads();
+    if (on) {
+        const double *C = a.Cinv + 81 * (size_t)k + 9 * r;
+        const double *rv = rs + (t - r);
+        double h1 = 0.;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) h1 += C[c] * rv[c];
+        a.h1[j] = h1;
+        num = rs[t] * (h1 - a.h[j]);
+        rh = rs[t] * h1;
     }
     m = block_sum(m, sh);
-    for (int k = tid; k < a.nc; k += nt) apply_cinv(a.Cinv, a.r, a.h1, k);
-    const int it = S->k;
-    const double m0 = (it == 0) ? m : S->m0;
-    const bool better = m < S->mold;
-    if (better)
-        for (int j = tid; j < n; j += nt) a.best[j] = a.x[j];
-    __syncthreads();
-    if (tid == 0) {
-        S->m0 = m0;
-        if (better) S->mold = m;
+    num = block_sum(num, sh);
+    rh = block_sum(rh, sh);
+    if (t == 0) {
+        a.part[3 * blockIdx.x] = m;
+        a.part[3 * blockIdx.x + 1] = num;
+        a.part[3 * blockIdx.x + 2] = rh;
     }
-    if (m * 1.e5 < m0) {
-        if (tid == 0) {
-            S->done = 1;
-            S->iters = it + 1;
-        }
+}
+
+// PCG step, part 2: best iterate, stopping rule, beta, d = h1 + beta d, h = h1;
+// workgroup 0 writes the scalars of the next launch
+__global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
+{
+    __shared__ double sh[4];
+    const PcgScalars S = a.S[a.j & 1];
+    PcgScalars *Sn = a.S + ((a.j + 1) & 1);
+    if (S.done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *Sn = S;
         return;
     }
-    double num = 0.;
-    for (int j = tid; j < n; j += nt) num += a.r[j] * (a.h1[j] - a.h[j]);
+    const int G = gridDim.x;
+    double m = 0., num = 0., rh = 0.;
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        m += a.part[3 * g];
+        num += a.part[3 * g + 1];
+        rh += a.part[3 * g + 2];
+    }
+    m = block_sum(m, sh);
     num = block_sum(num, sh);
-    double beta = (fabs(num) < 1.e-10) ? 0. : num / rho;
+    rh = block_sum(rh, sh);
+    const int it = S.k;
+    const double m0 = (it == 0) ? m : S.m0;
+    const bool better = m < S.mold;
+    const int t = threadIdx.x;
+    const int k = blockIdx.x * 28 + t / 9;
+    const bool on = (t < 252) && (k < a.nc);
+    const size_t j = 9 * (size_t)k + t % 9;
+    if (better && on) a.best[j] = a.x[j];
+    PcgScalars z = S;
+    z.m0 = m0;
+    if (better) z.mold = m;
+    if (m * 1.e5 < m0) {
+        z.done = 1;
+        z.iters = it + 1;
+        if (blockIdx.x == 0 && t == 0) *Sn = z;
+        return;
+    }
+    double beta = (fabs(num) < 1.e-10) ? 0. : num / S.rho;
     if (beta < 0.) beta = 0.;
-    double rh = 0.;
-    for (int j = tid; j < n; j += nt) {
+    if (on) {
         const double h1 = a.h1[j];
         a.d[j] = h1 + beta * a.d[j];
         a.h[j] = h1;
-        rh += a.r[j] * h1;
     }
-    rh = block_sum(rh, sh);
-    if (tid == 0) {
-        S->rho = rh;
-        S->k = it + 1;
-        if (it + 1 >= a.msize) {
-            S->done = 1;
-            S->iters = a.msize;
-        }
+    z.rho = rh;
+    z.k = it + 1;
+    if (it + 1 >= a.msize) {
+        z.done = 1;
+        z.iters = a.msize;
     }
+    if (blockIdx.x == 0 && t == 0) *Sn = z;
 }
 
 // ---------------------------------------------------------------------------
@@ -850,6 +926,16 @@ __global__ __launch_bounds__(256) void k_reweight(RobustArgs a)
     }
 }
 
+// sum of wx + wy over the entries (PrepareWeights2D's wsum, FP64)
+__global__ __launch_bounds__(256) void k_wsum(int n, const float *wx, const float *wy, double *partial)
+{
+    __shared__ double sh[4];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const double v = (q < n) ? (double)wx[q] + (double)wy[q] : 0.;
+    const double s = block_sum(v, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // StoreResultsIteration for the cameras (rbacore.cpp:895-906): P = [R | R C]
 __global__ __launch_bounds__(64) void k_store_cams(CamPtrs cp, int nc)
 {
@@ -923,12 +1009,6 @@ __global__ __launch_bounds__(256) void k_exteval(ExtArgs a)
 
 static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-void launch_lin(const LinArgs &a, const SchurArgs &s, hipStream_t st)
-{
-    if (a.npts > 0) k_lin_points<<<nblk(a.npts, 128), 128, 0, st>>>(a);
-    if (s.nblk > 0) k_schur_pairs<<<s.nblk, 64, 0, st>>>(s);
-    k_lin_cameras<<<a.nc, 256, 0, st>>>(a);
-}
 void launch_lin_points(const LinArgs &a, hipStream_t st)
 {
     if (a.npts > 0) k_lin_points<<<nblk(a.npts, 128), 128, 0, st>>>(a);
@@ -937,6 +1017,7 @@ void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
 {
     if (s.nblk > 0) k_schur_pairs<<<s.nblk, 64, 0, st>>>(s);
 }
+void launch_schur_diag(const LinArgs &a, hipStream_t st) { k_schur_diag<<<a.nc, 64, 0, st>>>(a); }
 void launch_lin_cameras(const LinArgs &a, hipStream_t st) { k_lin_cameras<<<a.nc, 256, 0, st>>>(a); }
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
 {
@@ -944,7 +1025,13 @@ void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
 }
 void launch_pcg_init(const PcgArgs &a, hipStream_t st) { k_pcg_init<<<1, 1024, 0, st>>>(a); }
 void launch_spmv(const PcgArgs &a, hipStream_t st) { k_spmv<<<a.nc, 256, 0, st>>>(a); }
-void launch_pcg_update(const PcgArgs &a, hipStream_t st) { k_pcg_update<<<1, 1024, 0, st>>>(a); }
+int pcg_groups(int nc) { return (nc + 27) / 28; }
+void launch_pcg_update(const PcgArgs &a, hipStream_t st)
+{
+    const int G = pcg_groups(a.nc);
+    k_pcg_a<<<G, 256, 0, st>>>(a);
+    k_pcg_b<<<G, 256, 0, st>>>(a);
+}
 void launch_backsub(const StepArgs &a, hipStream_t st)
 {
     if (a.npts > 0) k_backsub<<<nblk(a.npts, 256), 256, 0, st>>>(a);
@@ -963,6 +1050,12 @@ void launch_reweight(const RobustArgs &a, double *wsum_out, int *nflag_out, hipS
     const int nb = cost_blocks(a.ntr);
     k_reweight<<<nb, 256, 0, st>>>(a);
     k_final_sum<<<1, 1024, 0, st>>>(a.partial, nb, a.ipartial, wsum_out, nflag_out);
+}
+void launch_wsum(int n, const float *wx, const float *wy, double *partial, double *out, hipStream_t st)
+{
+    const int nb = cost_blocks(n);
+    k_wsum<<<nb, 256, 0, st>>>(n, wx, wy, partial);
+    k_final_sum<<<1, 1024, 0, st>>>(partial, nb, nullptr, out, nullptr);
 }
 void launch_store_cams(const CamPtrs &cp, int nc, hipStream_t st) { k_store_cams<<<nblk(nc, 64), 64, 0, st>>>(cp, nc); }
 void launch_exteval(const ExtArgs &a, double *sg_out, int *t_out, hipStream_t st)

@@ -19,6 +19,7 @@
 #include "../../include/rpba.h"
 #include "kernels.h"
 #include "structure.h"
+#include "structure_dev.h"
 
 using namespace rpba;
 
@@ -118,19 +119,19 @@ struct rpba_ctx {
     int nrioff = 0, nrioff3 = 0;
     float ihw2 = 1.f;
     double defdiff = 5000.;
-    Structure s, s_init;
-    std::vector<float> w0, w0_init;  // Wijoldv, 3 per track entry
-    std::vector<double> X, X_init;   // accepted points (host copy at pass boundaries)
+    Structure s_init;                // tracks after PointXXvIn (host, for resets)
+    std::vector<float> w0_init;      // Wijoldv at the start, 3 per track entry
+    std::vector<double> X_init;      // points at the start
     std::vector<double> Pst, Ist;    // stored cameras (host copy at pass boundaries)
     std::vector<double> Pst_init, Ist_init, R0, C0;
-    bool structure_dirty = true;
+    bool structure_dirty = true;     // pairs/blocks must be rebuilt
+    DevStruct ds;                    // device tracks, camera-major order, pairs, blocks
 
     // device buffers
-    DBuf<int> d_toff, d_tcam, d_tpt, d_camoff, d_camq, d_pa, d_pb, d_poff, d_bk, d_bl, d_roff, d_coff, d_cblk;
-    DBuf<float> d_ox, d_oy, d_wx, d_wy, d_wxy, d_w0x, d_w0y, d_w0xy, d_key, d_s00;
+    DBuf<float> d_wx, d_wy, d_wxy, d_key, d_s00;
     DBuf<unsigned char> d_flag, d_flagst;
     DBuf<double> d_X, d_Xn, d_Wo, d_Yo, d_Vinv, d_gp, d_Sd, d_Su, d_b, d_Cinv;
-    DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_rowdot, d_partial, d_scal;
+    DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_rowdot, d_partial, d_scal, d_Dd, d_ppart;
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -212,62 +213,45 @@ struct rpba_ctx {
 // set-up
 // ---------------------------------------------------------------------------
 
-static int upload_structure(rpba_ctx *c)
+// tracks, observation values and original weights of the initial block, and
+// the accepted points (LSA construction, rbacore.cpp:37-123)
+static int upload_tracks(rpba_ctx *c)
 {
-    Structure &s = c->s;
-    hipStream_t st = c->st;
-    const int empty = build_camera_major(s);
+    int rc = c->ds.upload_tracks(c->s_init, c->io->obs_xy, c->w0_init, c->st);
+    if (rc) return rc;
+    rc |= c->d_X.put(c->X_init.data(), c->X_init.size(), c->st);
+    rc |= c->d_Xn.put(c->X_init.data(), c->X_init.size(), c->st);
+    if (rc) return RPBA_E_NOMEM;
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->structure_dirty = true;
+    return RPBA_OK;
+}
+
+// camera-major order, pairs and blocks on the device, and the per-entry /
+// per-block work arrays sized for them.  Returns -1000 - k when image k has
+// no observations.
+static int build_structure(rpba_ctx *c)
+{
+    int empty = -1;
+    int rc = c->ds.build(c->st, &empty);
+    if (rc) return rc;
     if (empty >= 0) return -1000 - empty;
-    build_pairs(s);
-    const size_t ntr = (size_t)s.ntr, np = (size_t)s.npts;
-    std::vector<float> ox(ntr), oy(ntr), w0x(ntr), w0y(ntr), w0xy(ntr);
-    for (size_t q = 0; q < ntr; ++q) {
-        const int fo = s.tfo[q];
-        ox[q] = c->io->obs_xy[2 * (size_t)fo];
-        oy[q] = c->io->obs_xy[2 * (size_t)fo + 1];
-        w0x[q] = c->w0[3 * q];
-        w0y[q] = c->w0[3 * q + 1];
-        w0xy[q] = c->w0[3 * q + 2];
-    }
-    int rc = 0;
-    rc |= c->d_toff.put(s.toff.data(), s.toff.size(), st);
-    rc |= c->d_tcam.put(s.tcam.data(), ntr, st);
-    rc |= c->d_tpt.put(s.tpt.data(), ntr, st);
-    rc |= c->d_camoff.put(s.camoff.data(), s.camoff.size(), st);
-    rc |= c->d_camq.put(s.camq.data(), ntr, st);
-    rc |= c->d_pa.put(s.pa.data(), s.pa.size(), st);
-    rc |= c->d_pb.put(s.pb.data(), s.pb.size(), st);
-    rc |= c->d_poff.put(s.blk_poff.data(), s.blk_poff.size(), st);
-    rc |= c->d_bk.put(s.blk_k.data(), s.blk_k.size(), st);
-    rc |= c->d_bl.put(s.blk_l.data(), s.blk_l.size(), st);
-    rc |= c->d_roff.put(s.row_off.data(), s.row_off.size(), st);
-    rc |= c->d_coff.put(s.col_off.data(), s.col_off.size(), st);
-    rc |= c->d_cblk.put(s.col_blk.data(), s.col_blk.size(), st);
-    rc |= c->d_ox.put(ox.data(), ntr, st);
-    rc |= c->d_oy.put(oy.data(), ntr, st);
-    rc |= c->d_w0x.put(w0x.data(), ntr, st);
-    rc |= c->d_w0y.put(w0y.data(), ntr, st);
-    rc |= c->d_w0xy.put(w0xy.data(), ntr, st);
-    rc |= c->d_wx.put(w0x.data(), ntr, st);
-    rc |= c->d_wy.put(w0y.data(), ntr, st);
-    rc |= c->d_wxy.put(w0xy.data(), ntr, st);
+    const size_t ntr = (size_t)c->ds.ntr, np = (size_t)c->ds.npts;
+    rc |= c->d_wx.alloc(ntr);
+    rc |= c->d_wy.alloc(ntr);
+    rc |= c->d_wxy.alloc(ntr);
     rc |= c->d_key.alloc(ntr);
     rc |= c->d_flag.alloc(ntr);
     rc |= c->d_flagst.alloc(ntr);
-    rc |= c->d_X.put(c->X.data(), 3 * np, st);
-    rc |= c->d_Xn.put(c->X.data(), 3 * np, st);
     rc |= c->d_Wo.alloc(27 * ntr);
     rc |= c->d_Yo.alloc(27 * ntr);
     rc |= c->d_Vinv.alloc(9 * np);
     rc |= c->d_gp.alloc(3 * np);
-    rc |= c->d_Su.alloc(81 * (size_t)s.nblocks());
-    const int nb = std::max(cost_blocks(s.ntr), (int)((np + 255) / 256 + 1));
+    rc |= c->d_Su.alloc(81 * (size_t)c->ds.nblk);
+    const int nb = std::max(cost_blocks(c->ds.ntr), (int)((np + 255) / 256 + 1));
     rc |= c->d_partial.alloc((size_t)nb);
     rc |= c->d_ipartial.alloc((size_t)nb);
     if (rc) return RPBA_E_NOMEM;
-    HIPCHK(hipMemsetAsync(c->d_flag.p, 0, ntr, st));
-    HIPCHK(hipMemsetAsync(c->d_flagst.p, 0, ntr, st));
-    HIPCHK(hipStreamSynchronize(st));  // the staging vectors above go out of scope
     c->structure_dirty = false;
     return RPBA_OK;
 }
@@ -310,7 +294,7 @@ static int encode_cameras(rpba_ctx *c)
         th[9 * (size_t)k + 7] = c->Ist[3 * (size_t)k + 1];
         th[9 * (size_t)k + 8] = c->Ist[3 * (size_t)k + 2];
     }
-    c->nrioff3 = c->nrioff + 3 * c->s.npts;
+    c->nrioff3 = c->nrioff + 3 * c->ds.npts;
     int rc = 0;
     rc |= c->c_Rb.put(Rb.data(), Rb.size(), c->st);
     rc |= c->c_Cb.put(Cb.data(), Cb.size(), c->st);
@@ -368,13 +352,14 @@ static int upload_initial_cameras(rpba_ctx *c)
 static LinArgs lin_args(rpba_ctx *c)
 {
     LinArgs a;
-    a.nc = c->nc; a.npts = c->s.npts; a.ntr = c->s.ntr;
+    a.nc = c->nc; a.npts = c->ds.npts; a.ntr = c->ds.ntr;
     a.defdiff = c->defdiff; a.mult1 = 1. + c->mult;
     a.cam = c->cam();
-    a.toff = c->d_toff.p; a.tcam = c->d_tcam.p; a.tpt = c->d_tpt.p; a.camoff = c->d_camoff.p; a.camq = c->d_camq.p;
-    a.ox = c->d_ox.p; a.oy = c->d_oy.p; a.wx = c->d_wx.p; a.wy = c->d_wy.p; a.wxy = c->d_wxy.p;
+    a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.tpt = c->ds.tpt; a.camoff = c->ds.camoff; a.camq = c->ds.camq;
+    a.ox = c->ds.ox; a.oy = c->ds.oy; a.wx = c->d_wx.p; a.wy = c->d_wy.p; a.wxy = c->d_wxy.p;
     a.X = c->d_X.p;
     a.Wo = c->d_Wo.p; a.Yo = c->d_Yo.p; a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.Sd = c->d_Sd.p; a.bvec = c->d_b.p;
+    a.Dd = c->d_Dd.p;
     return a;
 }
 
@@ -384,17 +369,18 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.nc = c->nc;
     a.msize = std::max(1000, c->nrioff);  // rbacore.cpp:1848-1849
     a.Sd = c->d_Sd.p; a.Su = c->d_Su.p; a.Cinv = c->d_Cinv.p; a.b = c->d_b.p;
-    a.row_off = c->d_roff.p; a.col_off = c->d_coff.p; a.col_blk = c->d_cblk.p; a.blk_k = c->d_bk.p; a.blk_l = c->d_bl.p;
+    a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
     a.x = c->d_x.p; a.best = c->d_best.p; a.r = c->d_r.p; a.d = c->d_d.p; a.Ad = c->d_Ad.p;
-    a.h = c->d_h.p; a.h1 = c->d_h1.p; a.rowdot = c->d_rowdot.p; a.S = c->d_pcg.p;
+    a.h = c->d_h.p; a.h1 = c->d_h1.p; a.rowdot = c->d_rowdot.p; a.part = c->d_ppart.p; a.S = c->d_pcg.p;
+    a.j = 0;
     return a;
 }
 
 static StepArgs step_args(rpba_ctx *c, const double *sol)
 {
     StepArgs a;
-    a.nc = c->nc; a.npts = c->s.npts; a.cam = c->cam();
-    a.toff = c->d_toff.p; a.tcam = c->d_tcam.p;
+    a.nc = c->nc; a.npts = c->ds.npts; a.cam = c->cam();
+    a.toff = c->ds.toff; a.tcam = c->ds.tcam;
     a.Wo = c->d_Wo.p; a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.sol = sol; a.X = c->d_X.p; a.Xn = c->d_Xn.p;
     a.S = c->d_pcg.p;
     return a;
@@ -403,10 +389,10 @@ static StepArgs step_args(rpba_ctx *c, const double *sol)
 static CostArgs cost_args(rpba_ctx *c)
 {
     CostArgs a;
-    a.ntr = c->s.ntr; a.robust = c->fl.robust != 0; a.write_key = 1; a.defdiff = c->defdiff;
-    a.cam = c->cam(); a.tcam = c->d_tcam.p; a.tpt = c->d_tpt.p;
-    a.ox = c->d_ox.p; a.oy = c->d_oy.p; a.wx = c->d_wx.p; a.wy = c->d_wy.p; a.wxy = c->d_wxy.p;
-    a.w0x = c->d_w0x.p; a.w0y = c->d_w0y.p; a.w0xy = c->d_w0xy.p;
+    a.ntr = c->ds.ntr; a.robust = c->fl.robust != 0; a.write_key = 1; a.defdiff = c->defdiff;
+    a.cam = c->cam(); a.tcam = c->ds.tcam; a.tpt = c->ds.tpt;
+    a.ox = c->ds.ox; a.oy = c->ds.oy; a.wx = c->d_wx.p; a.wy = c->d_wy.p; a.wxy = c->d_wxy.p;
+    a.w0x = c->ds.w0x; a.w0y = c->ds.w0y; a.w0xy = c->ds.w0xy;
     a.X = c->d_Xn.p; a.key = c->d_key.p; a.partial = c->d_partial.p;
     return a;
 }
@@ -426,23 +412,27 @@ static int bundle_step(rpba_ctx *c)
 {
     LinArgs la = lin_args(c);
     SchurArgs sa;
-    sa.nblk = c->s.nblocks(); sa.poff = c->d_poff.p; sa.pa = c->d_pa.p; sa.pb = c->d_pb.p;
+    sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
     sa.Yo = c->d_Yo.p; sa.Wo = c->d_Wo.p; sa.Su = c->d_Su.p;
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
     c->timed("schur_pairs", [&] { launch_schur_pairs(sa, c->st); });
+    c->timed("schur_diag", [&] { launch_schur_diag(la, c->st); });
     c->timed("lin_cameras", [&] { launch_lin_cameras(la, c->st); });
     c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
     PcgArgs pa = pcg_args(c);
     c->timed("pcg_init", [&] { launch_pcg_init(pa, c->st); });
+    // launch j reads the scalars of slot j & 1 and writes slot (j + 1) & 1;
+    // once the stopping rule fired, later launches only pass the state on
     PcgScalars hs{};
     int chunk = 8, launched = 0;
     for (;;) {
         for (int j = 0; j < chunk; ++j) {
+            pa.j = launched + j;
             c->timed("spmv", [&] { launch_spmv(pa, c->st); });
             c->timed("pcg_update", [&] { launch_pcg_update(pa, c->st); });
         }
         launched += chunk;
-        HIPCHK(hipMemcpyAsync(&hs, c->d_pcg.p, sizeof(hs), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(&hs, c->d_pcg.p + (launched & 1), sizeof(hs), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         if (hs.done) break;
         if (launched > pa.msize + 8) return RPBA_E_HIP;  // cannot happen: the kernel stops at msize
@@ -472,8 +462,8 @@ static int bundle_step(rpba_ctx *c)
 static int device_store(rpba_ctx *c)
 {
     launch_store_cams(c->cam(), c->nc, c->st);
-    HIPCHK(hipMemcpyAsync(c->d_X.p, c->d_Xn.p, sizeof(double) * 3 * (size_t)c->s.npts, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->d_flagst.p, c->d_flag.p, (size_t)c->s.ntr, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_X.p, c->d_Xn.p, sizeof(double) * 3 * (size_t)c->ds.npts, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_flagst.p, c->d_flag.p, (size_t)c->ds.ntr, hipMemcpyDeviceToDevice, c->st));
     c->nrobsnew = c->nrobs;
     c->nrunknownsnew = c->nrunknowns;
     c->redundancynew = c->redundancy;
@@ -489,10 +479,10 @@ static int pass_setup(rpba_ctx *c)
     char buf[256];
     c->sig0old = (double)1.e20f;
     if (c->it > 0) c->mult = 1.e-6;
-    c->imagepointnr = c->s.ntr;
+    c->imagepointnr = c->ds.ntr;
     c->nrobs = 2 * c->imagepointnr;
     if (c->structure_dirty) {
-        const int rc = upload_structure(c);
+        const int rc = build_structure(c);
         if (rc <= -1000) {
             if (c->nc > 3) {
                 std::snprintf(buf, sizeof buf, "BA-Error: No observations in image %d (image index)\n", -1000 - rc);
@@ -502,16 +492,19 @@ static int pass_setup(rpba_ctx *c)
         }
         if (rc) return rc;
     }
+    const size_t ntr = (size_t)c->ds.ntr;
     if (c->ittflag) {
-        // PrepareWeights2D: Wij = original weights, wsum in point order
+        // PrepareWeights2D: Wij = original weights; wsum over the entries
+        HIPCHK(hipMemcpyAsync(c->d_wx.p, c->ds.w0x, sizeof(float) * ntr, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->d_wy.p, c->ds.w0y, sizeof(float) * ntr, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->d_wxy.p, c->ds.w0xy, sizeof(float) * ntr, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemsetAsync(c->d_flag.p, 0, ntr, c->st));
+        HIPCHK(hipMemsetAsync(c->d_flagst.p, 0, ntr, c->st));
+        launch_wsum(c->ds.ntr, c->ds.w0x, c->ds.w0y, c->d_partial.p, c->d_scal.p, c->st);
         double ws = 0.;
-        for (int q = 0; q < c->s.ntr; ++q) ws += (double)c->w0[3 * (size_t)q] + (double)c->w0[3 * (size_t)q + 1];
+        HIPCHK(hipMemcpyAsync(&ws, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
         c->wsum = ws;
-        HIPCHK(hipMemcpyAsync(c->d_wx.p, c->d_w0x.p, sizeof(float) * c->s.ntr, hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(hipMemcpyAsync(c->d_wy.p, c->d_w0y.p, sizeof(float) * c->s.ntr, hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(hipMemcpyAsync(c->d_wxy.p, c->d_w0xy.p, sizeof(float) * c->s.ntr, hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(hipMemsetAsync(c->d_flag.p, 0, (size_t)c->s.ntr, c->st));
-        HIPCHK(hipMemsetAsync(c->d_flagst.p, 0, (size_t)c->s.ntr, c->st));
         c->ittflag = false;
     }
     c->wsum = (double)(2 * c->imagepointnr) / c->wsum;
@@ -527,7 +520,7 @@ static int pass_setup(rpba_ctx *c)
     if (c->it == 0) {
         StepArgs sa = step_args(c, nullptr);
         launch_decode(sa, c->st);
-        HIPCHK(hipMemcpyAsync(c->d_Xn.p, c->d_X.p, sizeof(double) * 3 * (size_t)c->s.npts, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->d_Xn.p, c->d_X.p, sizeof(double) * 3 * (size_t)c->ds.npts, hipMemcpyDeviceToDevice, c->st));
         double cost = 0.;
         rc = trial_cost(c, cost);
         if (rc) return rc;
@@ -563,16 +556,13 @@ static int pass_end(rpba_ctx *c)
     c->stats.nobs[c->it] = nob;
     c->stats.nunknowns[c->it] = nun;
     c->stats.passes_done = c->it + 1;
-    // host copies of the stored state
-    c->X.resize(3 * (size_t)c->s.npts);
-    HIPCHK(hipMemcpyAsync(c->X.data(), c->d_X.p, sizeof(double) * c->X.size(), hipMemcpyDeviceToHost, c->st));
+    // host copies of the stored cameras for the next EncodeCameras
     HIPCHK(hipMemcpyAsync(c->Pst.data(), c->c_Pst.p, sizeof(double) * c->Pst.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(c->Ist.data(), c->c_Ist.p, sizeof(double) * c->Ist.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (c->fl.robust && c->it == 0) {
-        std::vector<uint8_t> fl((size_t)c->s.ntr);
-        HIPCHK(hipMemcpy(fl.data(), c->d_flagst.p, fl.size(), hipMemcpyDeviceToHost));
-        robust_delete(c->s, fl, c->minimagenrdel, c->w0, c->X);
+        const int rc = c->ds.robust_delete(c->d_flagst.p, c->minimagenrdel, c->d_X.p, c->st);
+        if (rc) return rc;
         c->structure_dirty = true;
     }
     c->ittflag = true;
@@ -587,9 +577,9 @@ static int lm_iteration(rpba_ctx *c)
         c->nrunknowns = c->nrunknownsgiven;
         c->nrobs = c->nrobsgiven;
         RobustArgs ra;
-        ra.nc = c->nc; ra.ntr = c->s.ntr; ra.iflag = c->iflag;
-        ra.camoff = c->d_camoff.p; ra.camq = c->d_camq.p; ra.tpt = c->d_tpt.p; ra.toff = c->d_toff.p; ra.tcam = c->d_tcam.p;
-        ra.key = c->d_key.p; ra.w0x = c->d_w0x.p; ra.w0y = c->d_w0y.p; ra.w0xy = c->d_w0xy.p;
+        ra.nc = c->nc; ra.ntr = c->ds.ntr; ra.iflag = c->iflag;
+        ra.camoff = c->ds.camoff; ra.camq = c->ds.camq; ra.tpt = c->ds.tpt; ra.toff = c->ds.toff; ra.tcam = c->ds.tcam;
+        ra.key = c->d_key.p; ra.w0x = c->ds.w0x; ra.w0y = c->ds.w0y; ra.w0xy = c->ds.w0xy;
         ra.s00 = c->d_s00.p; ra.wx = c->d_wx.p; ra.wy = c->d_wy.p; ra.wxy = c->d_wxy.p;
         ra.flag = c->d_flag.p; ra.partial = c->d_partial.p; ra.ipartial = c->d_ipartial.p;
         c->timed("reweight", [&] { launch_reweight(ra, c->d_scal.p, c->d_iscal.p, c->st); });
@@ -656,17 +646,18 @@ static int ctx_init(rpba_ctx *c)
     c->ihw2 = ihw2 / (float)nc;
     c->defdiff = 5000. / (double)sqrtf(c->ihw2);  // rbacore.cpp:48
     c->minimagenrdel = nc > 2 ? 3 : 2;
-    int rc = build_tracks(io, c->s);
+    Structure &s = c->s_init;
+    int rc = build_tracks(io, s);
     if (rc) return rc;
-    c->X.assign(3 * (size_t)c->s.npts, 0.);
-    for (int i = 0; i < c->s.npts; ++i)
-        for (int j = 0; j < 3; ++j) c->X[3 * (size_t)i + j] = (double)io->pts[3 * (size_t)c->s.nrpv[i] + j];
-    c->w0.resize(3 * (size_t)c->s.ntr);
-    for (int q = 0; q < c->s.ntr; ++q) {
-        const int fo = c->s.tfo[q];
-        c->w0[3 * (size_t)q] = io->obs_w ? io->obs_w[3 * (size_t)fo] : 1.f;
-        c->w0[3 * (size_t)q + 1] = io->obs_w ? io->obs_w[3 * (size_t)fo + 1] : 1.f;
-        c->w0[3 * (size_t)q + 2] = io->obs_w ? io->obs_w[3 * (size_t)fo + 2] : 0.f;
+    c->X_init.assign(3 * (size_t)s.npts, 0.);
+    for (int i = 0; i < s.npts; ++i)
+        for (int j = 0; j < 3; ++j) c->X_init[3 * (size_t)i + j] = (double)io->pts[3 * (size_t)s.nrpv[i] + j];
+    c->w0_init.resize(3 * (size_t)s.ntr);
+    for (int q = 0; q < s.ntr; ++q) {
+        const int fo = s.tfo[q];
+        c->w0_init[3 * (size_t)q] = io->obs_w ? io->obs_w[3 * (size_t)fo] : 1.f;
+        c->w0_init[3 * (size_t)q + 1] = io->obs_w ? io->obs_w[3 * (size_t)fo + 1] : 1.f;
+        c->w0_init[3 * (size_t)q + 2] = io->obs_w ? io->obs_w[3 * (size_t)fo + 2] : 0.f;
     }
     c->Pst.resize(12 * (size_t)nc);
     c->Ist.resize(3 * (size_t)nc);
@@ -708,9 +699,6 @@ static int ctx_init(rpba_ctx *c)
     for (int k = 0; k < nc; ++k)
         if (io->cam_f[k] > 2.f) c->lflflag = 1;
     c->nrioff = c->nraddpi + (5 + c->nraddpi) + (6 + c->nraddpi) * (nc - 2);
-    c->s_init = c->s;
-    c->w0_init = c->w0;
-    c->X_init = c->X;
     c->Pst_init = c->Pst;
     c->Ist_init = c->Ist;
 
@@ -732,12 +720,15 @@ static int ctx_init(rpba_ctx *c)
     rc |= c->d_s00.alloc((size_t)nc);
     rc |= c->d_scal.alloc(4);
     rc |= c->d_iscal.alloc(4);
-    rc |= c->d_pcg.alloc(1);
+    rc |= c->d_pcg.alloc(2);
+    rc |= c->d_Dd.alloc(90 * (size_t)nc);
+    rc |= c->d_ppart.alloc(3 * (size_t)pcg_groups(nc));
     rc |= c->c_Rb.alloc(n9);
     rc |= c->c_Cb.alloc(3 * (size_t)nc);
     if (rc) return RPBA_E_NOMEM;
     reset_state(c);
-    c->structure_dirty = true;
+    rc = upload_tracks(c);
+    if (rc) return rc;
     return upload_initial_cameras(c);
 }
 
@@ -809,13 +800,11 @@ extern "C" int rpba_ctx_run(rpba_ctx *c, int32_t max_iters, int32_t *iters_done,
 extern "C" int rpba_ctx_reset(rpba_ctx *c)
 {
     if (!c) return RPBA_E_INVALID;
-    c->s = c->s_init;
-    c->w0 = c->w0_init;
-    c->X = c->X_init;
     c->Pst = c->Pst_init;
     c->Ist = c->Ist_init;
     reset_state(c);
-    c->structure_dirty = true;
+    const int rc = upload_tracks(c);
+    if (rc) return rc;
     return upload_initial_cameras(c);
 }
 
@@ -824,15 +813,15 @@ extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
     if (!c) return RPBA_E_INVALID;
     rpba_block *io = c->io;
     const int nc = c->nc;
-    std::vector<double> X(3 * (size_t)c->s.npts), P(12 * (size_t)nc), In(3 * (size_t)nc);
-    // the device holds the stored state once a pass has been set up; an error
-    // before that leaves the host copies (inputs, image 0 rectified) current
-    const bool dev_pts = !c->structure_dirty && c->d_X.p && c->d_X.n >= X.size();
+    std::vector<int> toff, nrpv, tfo, tcam;
+    int rc = c->ds.download_tracks(toff, nrpv, tfo, tcam, c->st);
+    if (rc) return rc;
+    const int npts = c->ds.npts;
+    std::vector<double> X(3 * (size_t)npts), P(12 * (size_t)nc), In(3 * (size_t)nc);
+    // the device holds the stored state once pass 0 has been set up; an error
+    // before that leaves the initial host values (image 0 rectified) current
+    if (npts) HIPCHK(hipMemcpyAsync(X.data(), c->d_X.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, c->st));
     const bool dev_cams = c->c_Pst.p && c->c_Pst.n >= P.size();
-    if (dev_pts && !X.empty())
-        HIPCHK(hipMemcpyAsync(X.data(), c->d_X.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, c->st));
-    else
-        X.assign(c->X.begin(), c->X.begin() + X.size());
     if (dev_cams)
         HIPCHK(hipMemcpyAsync(P.data(), c->c_Pst.p, sizeof(double) * P.size(), hipMemcpyDeviceToHost, c->st));
     else
@@ -841,12 +830,12 @@ extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
     HIPCHK(hipStreamSynchronize(c->st));
     if (io->obs_kept) std::memset(io->obs_kept, 0, (size_t)io->n_obs);
     if (io->pt_kept) std::memset(io->pt_kept, 0, (size_t)io->n_pts);
-    for (int i = 0; i < c->s.npts; ++i) {
-        const int p = c->s.nrpv[i];
+    for (int i = 0; i < npts; ++i) {
+        const int p = nrpv[i];
         for (int j = 0; j < 3; ++j) io->pts[3 * (size_t)p + j] = (float)X[3 * (size_t)i + j];
         if (io->pt_kept) io->pt_kept[p] = 1;
-        for (int q = c->s.toff[i]; q < c->s.toff[(size_t)i + 1]; ++q)
-            if (io->obs_kept) io->obs_kept[c->s.tfo[q]] = 1;
+        for (int q = toff[i]; q < toff[(size_t)i + 1]; ++q)
+            if (io->obs_kept) io->obs_kept[tfo[q]] = 1;
     }
     // OutputLSAc: R|t from the stored state, intrinsics from the last trial
     for (int k = 0; k < nc; ++k) {
@@ -859,9 +848,9 @@ extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
         // inlier RMS of the returned solution, double
         double ss = 0.;
         int64_t n = 0;
-        for (int i = 0; i < c->s.npts; ++i)
-            for (int q = c->s.toff[i]; q < c->s.toff[(size_t)i + 1]; ++q) {
-                const int k = c->s.tcam[q], fo = c->s.tfo[q];
+        for (int i = 0; i < npts; ++i)
+            for (int q = toff[i]; q < toff[(size_t)i + 1]; ++q) {
+                const int k = tcam[q], fo = tfo[q];
                 const double *Pk = P.data() + 12 * (size_t)k;
                 const double *x = X.data() + 3 * (size_t)i;
                 const double e0 = Pk[0] * x[0] + Pk[1] * x[1] + Pk[2] * x[2] + Pk[3];
@@ -880,8 +869,8 @@ extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
         st->pcg_iters = c->pcg_total;
         st->sig0_final = c->sig0old;
         st->rms_px = n ? std::sqrt(ss / (double)(2 * n)) : 0.;
-        st->n_obs_kept = c->s.ntr;
-        st->n_pts_kept = c->s.npts;
+        st->n_obs_kept = c->ds.ntr;
+        st->n_pts_kept = c->ds.npts;
         st->trace_cap = keep.trace_cap;
         st->trace_sig0 = keep.trace_sig0;
         st->trace_accept = keep.trace_accept;
@@ -925,15 +914,10 @@ extern "C" int rpba_ctx_sizes(rpba_ctx *c, int64_t out[6])
 {
     if (!c || !out) return RPBA_E_INVALID;
     out[0] = c->nc;
-    out[1] = c->s.npts;
-    out[2] = c->s.ntr;
-    int64_t P = 0;
-    for (int i = 0; i < c->s.npts; ++i) {
-        const int64_t t = c->s.toff[(size_t)i + 1] - c->s.toff[i];
-        P += t * (t - 1) / 2;
-    }
-    out[3] = P;
-    out[4] = c->structure_dirty ? -1 : c->s.nblocks();
+    out[1] = c->ds.npts;
+    out[2] = c->ds.ntr;
+    out[3] = c->structure_dirty ? -1 : c->ds.npairs;
+    out[4] = c->structure_dirty ? -1 : c->ds.nblk;
     out[5] = c->nrioff;
     return RPBA_OK;
 }
@@ -947,11 +931,8 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
         (void)hipEventDestroy(t.b);
     }
     for (auto e : c->evpool) (void)hipEventDestroy(e);
-    c->d_toff.release(); c->d_tcam.release(); c->d_tpt.release(); c->d_camoff.release(); c->d_camq.release();
-    c->d_pa.release(); c->d_pb.release(); c->d_poff.release(); c->d_bk.release(); c->d_bl.release();
-    c->d_roff.release(); c->d_coff.release(); c->d_cblk.release();
-    c->d_ox.release(); c->d_oy.release(); c->d_wx.release(); c->d_wy.release(); c->d_wxy.release();
-    c->d_w0x.release(); c->d_w0y.release(); c->d_w0xy.release(); c->d_key.release(); c->d_s00.release();
+    c->ds.release();
+    c->d_wx.release(); c->d_wy.release(); c->d_wxy.release(); c->d_key.release(); c->d_s00.release();
     c->d_flag.release(); c->d_flagst.release();
     c->d_X.release(); c->d_Xn.release(); c->d_Wo.release(); c->d_Yo.release(); c->d_Vinv.release(); c->d_gp.release();
     c->d_Sd.release(); c->d_Su.release(); c->d_b.release(); c->d_Cinv.release();
@@ -961,6 +942,8 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->c_Rb.release(); c->c_Cb.release(); c->c_Rn.release(); c->c_Cn.release(); c->c_In.release();
     c->c_th.release(); c->c_thn.release(); c->c_Pst.release(); c->c_Ist.release(); c->c_mask.release();
     c->d_pcg.release();
+    c->d_Dd.release();
+    c->d_ppart.release();
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }

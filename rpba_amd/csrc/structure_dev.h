@@ -1,0 +1,67 @@
+// Device-resident index structures of one bundle block.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/rpba.h"
+#include "structure.h"
+
+namespace rpba {
+
+// point-major track arrays (one set; RobustDelete compacts into a second set
+// and swaps)
+struct Tracks {
+    int *toff = nullptr, *nrpv = nullptr, *tcam = nullptr, *tpt = nullptr, *tfo = nullptr;
+    float *ox = nullptr, *oy = nullptr, *w0x = nullptr, *w0y = nullptr, *w0xy = nullptr;
+    double *X = nullptr;
+    int alloc(size_t np, size_t nt);
+    void release();
+};
+
+struct DevStruct {
+    int nc = 0, npts = 0, ntr = 0, nblk = 0;
+    long long npairs = 0;
+    size_t sizes_cap_pts = 0, sizes_cap_tr = 0;
+    // tracks: entries of point i are [toff[i], toff[i+1]); nrpv: original point
+    // index; tfo: file-order observation index; w0*: the original weights
+    int *toff = nullptr, *nrpv = nullptr, *tcam = nullptr, *tpt = nullptr, *tfo = nullptr;
+    float *ox = nullptr, *oy = nullptr, *w0x = nullptr, *w0y = nullptr, *w0xy = nullptr;
+    // camera-major order: entries of image k are camq[camoff[k] .. camoff[k+1])
+    int *camoff = nullptr, *camq = nullptr;
+    size_t camoff_cap = 0, camq_cap = 0;
+    // pairs (a << 32 | b), cam(a) < cam(b), grouped by block in row-major
+    // block order; poff: pairs of block b; bk/bl: block (k, l); roff: blocks of
+    // row k; coff/cblk: blocks of column l (ids, k ascending)
+    unsigned long long *pab = nullptr;
+    int *poff = nullptr, *bk = nullptr, *bl = nullptr, *roff = nullptr, *coff = nullptr, *cblk = nullptr;
+    size_t pab_cap = 0, poff_cap = 0, bk_cap = 0, bl_cap = 0, roff_cap = 0, coff_cap = 0, cblk_cap = 0;
+    // scratch
+    unsigned *u32a = nullptr, *key_in = nullptr, *key_out = nullptr, *ukey = nullptr, *ucnt = nullptr, *nruns = nullptr;
+    unsigned *key2 = nullptr, *key2s = nullptr;
+    unsigned long long *val_in = nullptr;
+    int *i32a = nullptr, *i32b = nullptr, *ppoff = nullptr, *bid = nullptr, *newi = nullptr, *newoff = nullptr;
+    size_t u32a_cap = 0, key_in_cap = 0, key_out_cap = 0, ukey_cap = 0, ucnt_cap = 0, nruns_cap = 0;
+    size_t key2_cap = 0, key2s_cap = 0, val_in_cap = 0, i32a_cap = 0, i32b_cap = 0, ppoff_cap = 0, bid_cap = 0;
+    size_t newi_cap = 0, newoff_cap = 0;
+    unsigned char *tmp = nullptr;
+    size_t tmp_cap = 0;
+
+    int scratch(size_t bytes);
+    // tracks of the host structure (after PointXXvIn) plus the gathered
+    // observation values and weights
+    int upload_tracks(const Structure &s, const float *obs_xy, const std::vector<float> &w0, hipStream_t st);
+    // camera-major order, pairs and blocks; *first_empty = first image without
+    // observations (then nothing else is built) or -1
+    int build(hipStream_t st, int *first_empty);
+    // RobustDelete on the flags of the stored state; X (accepted points) is
+    // compacted in place
+    int robust_delete(const unsigned char *flag, int min_keep, double *X, hipStream_t st);
+    void swap_tracks(Tracks &b);
+    int download_tracks(std::vector<int> &toff_h, std::vector<int> &nrpv_h, std::vector<int> &tfo_h,
+                        std::vector<int> &tcam_h, hipStream_t st) const;
+    void release();
+};
+
+}  // namespace rpba

@@ -1,0 +1,435 @@
+// Device-side index structures of a bundle block (see structure_dev.h):
+// camera-major order, covisibility pairs grouped by S block, block row and
+// column indices, and RobustDelete's compaction.  Sorting is rocPRIM's stable
+// LSD radix sort, so the order inside a block is the generation order
+// (point, then track positions), the order in which the reference's
+// CalcCorrections accumulates its Schur products (rbacore.cpp:1350-1392).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <utility>
+#include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "structure_dev.h"
+
+namespace rpba {
+
+#define DCHK(x)                                    \
+    do {                                           \
+        if ((x) != hipSuccess) return RPBA_E_HIP;  \
+    } while (0)
+
+static unsigned bits_for(unsigned long long n)
+{
+    unsigned b = 1;
+    while (b < 63 && (1ull << b) < n) ++b;
+    return b;
+}
+
+template <class T>
+static int grow(T *&p, size_t &cap, size_t n)
+{
+    if (p && cap >= n) return RPBA_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)) != hipSuccess) return RPBA_E_NOMEM;
+    cap = n;
+    return RPBA_OK;
+}
+
+int DevStruct::scratch(size_t bytes)
+{
+    return grow(tmp, tmp_cap, bytes);
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void k_iota(int n, int *v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+
+// first index j with key[j] >= k*stride (k = 0..nk), sorted keys
+__global__ void k_lower_bound(int nk, unsigned stride, const unsigned *keys, int n, int *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nk) return;
+    const unsigned long long target = (unsigned long long)k * stride;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((unsigned long long)keys[mid] < target)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    out[k] = lo;
+}
+
+__global__ void k_pair_counts(int npts, const int *toff, int *cnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > npts) return;
+    if (i == npts) {
+        cnt[i] = 0;
+        return;
+    }
+    const int t = toff[i + 1] - toff[i];
+    cnt[i] = t * (t - 1) / 2;
+}
+
+__global__ void k_pairs(int npts, unsigned nc, const int *toff, const int *tcam, const int *ppoff, unsigned *key,
+                        unsigned long long *val)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const int b = toff[i], e = toff[i + 1];
+    int o = ppoff[i];
+    for (int x = b; x < e; ++x) {
+        const int cx = tcam[x];
+        for (int y = x + 1; y < e; ++y) {
+            const int cy = tcam[y];
+            const bool sw = cx > cy;
+            const unsigned lo = sw ? cy : cx, hi = sw ? cx : cy;
+            const unsigned a = sw ? y : x, bb = sw ? x : y;
+            key[o] = lo * nc + hi;
+            val[o] = ((unsigned long long)a << 32) | bb;
+            ++o;
+        }
+    }
+}
+
+__global__ void k_split_blocks(int nb, unsigned nc, const unsigned *ukey, int *bk, int *bl, unsigned *key2, int *bid)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const unsigned k = ukey[b] / nc, l = ukey[b] % nc;
+    bk[b] = (int)k;
+    bl[b] = (int)l;
+    key2[b] = l * nc + k;
+    bid[b] = b;
+}
+
+__global__ void k_set_last(int *a, int idx, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) a[idx] = v;
+}
+
+static inline unsigned nb_(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int DevStruct::build(hipStream_t st, int *first_empty)
+{
+    int rc = 0;
+    *first_empty = -1;
+    // --- camera-major: stable sort of entry ids by image ---
+    rc |= grow(camoff, camoff_cap, (size_t)nc + 1);
+    rc |= grow(camq, camq_cap, (size_t)ntr);
+    rc |= grow(u32a, u32a_cap, (size_t)ntr);
+    rc |= grow(i32a, i32a_cap, (size_t)ntr + 1);
+    if (rc) return RPBA_E_NOMEM;
+    if (ntr > 0) {
+        k_iota<<<nb_(ntr, 256), 256, 0, st>>>(ntr, i32a);
+        size_t bytes = 0;
+        const unsigned cb = bits_for((unsigned long long)nc);
+        DCHK(rocprim::radix_sort_pairs(nullptr, bytes, (const unsigned *)tcam, u32a, i32a, camq, ntr, 0u, cb, st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::radix_sort_pairs(tmp, bytes, (const unsigned *)tcam, u32a, i32a, camq, ntr, 0u, cb, st));
+    }
+    k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, 1u, u32a, ntr, camoff);
+    std::vector<int> co((size_t)nc + 1);
+    DCHK(hipMemcpyAsync(co.data(), camoff, sizeof(int) * co.size(), hipMemcpyDeviceToHost, st));
+    DCHK(hipStreamSynchronize(st));
+    for (int k = 0; k < nc; ++k)
+        if (co[(size_t)k + 1] == co[k]) {
+            *first_empty = k;
+            return RPBA_OK;
+        }
+
+    // --- covisibility pairs: count, scan, generate, stable sort by block ---
+    rc |= grow(ppoff, ppoff_cap, (size_t)npts + 1);
+    if (rc) return RPBA_E_NOMEM;
+    k_pair_counts<<<nb_(npts + 1, 256), 256, 0, st>>>(npts, toff, i32a);
+    {
+        size_t bytes = 0;
+        DCHK(rocprim::exclusive_scan(nullptr, bytes, i32a, ppoff, 0, (size_t)npts + 1, rocprim::plus<int>(), st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::exclusive_scan(tmp, bytes, i32a, ppoff, 0, (size_t)npts + 1, rocprim::plus<int>(), st));
+    }
+    int P = 0;
+    DCHK(hipMemcpyAsync(&P, ppoff + npts, sizeof(int), hipMemcpyDeviceToHost, st));
+    DCHK(hipStreamSynchronize(st));
+    npairs = P;
+    rc |= grow(key_in, key_in_cap, (size_t)P);
+    rc |= grow(key_out, key_out_cap, (size_t)P);
+    rc |= grow(val_in, val_in_cap, (size_t)P);
+    rc |= grow(pab, pab_cap, (size_t)P);
+    if (rc) return RPBA_E_NOMEM;
+    if (npts > 0) k_pairs<<<nb_(npts, 128), 128, 0, st>>>(npts, (unsigned)nc, toff, tcam, ppoff, key_in, val_in);
+    const unsigned kb = bits_for((unsigned long long)nc * (unsigned long long)nc);
+    if (P > 0) {
+        size_t bytes = 0;
+        DCHK(rocprim::radix_sort_pairs(nullptr, bytes, key_in, key_out, val_in, pab, P, 0u, kb, st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::radix_sort_pairs(tmp, bytes, key_in, key_out, val_in, pab, P, 0u, kb, st));
+    }
+
+    // --- blocks: run-length encode, offsets, (k, l), row and column indices ---
+    rc |= grow(ukey, ukey_cap, (size_t)P + 1);
+    rc |= grow(ucnt, ucnt_cap, (size_t)P + 1);
+    rc |= grow(nruns, nruns_cap, (size_t)1);
+    if (rc) return RPBA_E_NOMEM;
+    unsigned nbk = 0;
+    if (P > 0) {
+        size_t bytes = 0;
+        DCHK(rocprim::run_length_encode(nullptr, bytes, key_out, (unsigned)P, ukey, ucnt, nruns, st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::run_length_encode(tmp, bytes, key_out, (unsigned)P, ukey, ucnt, nruns, st));
+        DCHK(hipMemcpyAsync(&nbk, nruns, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        DCHK(hipStreamSynchronize(st));
+    }
+    nblk = (int)nbk;
+    rc |= grow(poff, poff_cap, (size_t)nblk + 1);
+    rc |= grow(bk, bk_cap, (size_t)nblk);
+    rc |= grow(bl, bl_cap, (size_t)nblk);
+    rc |= grow(roff, roff_cap, (size_t)nc + 1);
+    rc |= grow(coff, coff_cap, (size_t)nc + 1);
+    rc |= grow(cblk, cblk_cap, (size_t)nblk);
+    rc |= grow(key2, key2_cap, (size_t)nblk);
+    rc |= grow(key2s, key2s_cap, (size_t)nblk);
+    rc |= grow(bid, bid_cap, (size_t)nblk);
+    if (rc) return RPBA_E_NOMEM;
+    k_set_last<<<1, 1, 0, st>>>((int *)ucnt, nblk, 0);
+    {
+        size_t bytes = 0;
+        DCHK(rocprim::exclusive_scan(nullptr, bytes, (const int *)ucnt, poff, 0, (size_t)nblk + 1, rocprim::plus<int>(), st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::exclusive_scan(tmp, bytes, (const int *)ucnt, poff, 0, (size_t)nblk + 1, rocprim::plus<int>(), st));
+    }
+    if (nblk > 0) k_split_blocks<<<nb_(nblk, 256), 256, 0, st>>>(nblk, (unsigned)nc, ukey, bk, bl, key2, bid);
+    k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, (unsigned)nc, ukey, nblk, roff);
+    if (nblk > 0) {
+        size_t bytes = 0;
+        DCHK(rocprim::radix_sort_pairs(nullptr, bytes, key2, key2s, bid, cblk, nblk, 0u, kb, st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::radix_sort_pairs(tmp, bytes, key2, key2s, bid, cblk, nblk, 0u, kb, st));
+    }
+    k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, (unsigned)nc, key2s, nblk, coff);
+    DCHK(hipStreamSynchronize(st));
+    return RPBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// RobustDelete (rbacore.cpp:922-995)
+// ---------------------------------------------------------------------------
+
+__global__ void k_rd_mark(int npts, int min_keep, const int *toff, const unsigned char *flag, int *keep_pt, int *newt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > npts) return;
+    if (i == npts) {
+        keep_pt[i] = 0;
+        newt[i] = 0;
+        return;
+    }
+    const int b = toff[i], e = toff[i + 1];
+    int nf = 0;
+    for (int q = b; q < e; ++q) nf += flag[q] ? 1 : 0;
+    const bool drop = nf > 0 && (e - b) - nf < min_keep;
+    keep_pt[i] = drop ? 0 : 1;
+    newt[i] = drop ? 0 : (e - b) - nf;
+}
+
+struct RdArgs {
+    int npts;
+    const int *toff, *tcam, *tfo, *nrpv, *newi, *newoff;
+    const unsigned char *flag;
+    const float *ox, *oy, *wx, *wy, *wxy;
+    const double *X;
+    int *toff2, *tcam2, *tpt2, *tfo2, *nrpv2;
+    float *ox2, *oy2, *wx2, *wy2, *wxy2;
+    double *X2;
+};
+
+__global__ void k_rd_scatter(RdArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > a.npts) return;
+    if (i == a.npts) {
+        a.toff2[a.newi[i]] = a.newoff[i];
+        return;
+    }
+    if (a.newi[i + 1] == a.newi[i]) return;  // point erased
+    const int j = a.newi[i];
+    a.toff2[j] = a.newoff[i];
+    a.nrpv2[j] = a.nrpv[i];
+    for (int c = 0; c < 3; ++c) a.X2[3 * (size_t)j + c] = a.X[3 * (size_t)i + c];
+    int w = a.newoff[i];
+    for (int q = a.toff[i]; q < a.toff[i + 1]; ++q) {
+        if (a.flag[q]) continue;
+        a.tcam2[w] = a.tcam[q];
+        a.tpt2[w] = j;
+        a.tfo2[w] = a.tfo[q];
+        a.ox2[w] = a.ox[q];
+        a.oy2[w] = a.oy[q];
+        a.wx2[w] = a.wx[q];
+        a.wy2[w] = a.wy[q];
+        a.wxy2[w] = a.wxy[q];
+        ++w;
+    }
+}
+
+int DevStruct::robust_delete(const unsigned char *flag, int min_keep, double *X, hipStream_t st)
+{
+    int rc = 0;
+    rc |= grow(i32a, i32a_cap, (size_t)std::max(ntr, npts) + 1);
+    rc |= grow(i32b, i32b_cap, (size_t)npts + 1);
+    rc |= grow(newi, newi_cap, (size_t)npts + 1);
+    rc |= grow(newoff, newoff_cap, (size_t)npts + 1);
+    if (rc) return RPBA_E_NOMEM;
+    k_rd_mark<<<nb_(npts + 1, 256), 256, 0, st>>>(npts, min_keep, toff, flag, i32a, i32b);
+    size_t bytes = 0;
+    DCHK(rocprim::exclusive_scan(nullptr, bytes, i32a, newi, 0, (size_t)npts + 1, rocprim::plus<int>(), st));
+    if (scratch(bytes)) return RPBA_E_NOMEM;
+    DCHK(rocprim::exclusive_scan(tmp, bytes, i32a, newi, 0, (size_t)npts + 1, rocprim::plus<int>(), st));
+    DCHK(rocprim::exclusive_scan(nullptr, bytes, i32b, newoff, 0, (size_t)npts + 1, rocprim::plus<int>(), st));
+    if (scratch(bytes)) return RPBA_E_NOMEM;
+    DCHK(rocprim::exclusive_scan(tmp, bytes, i32b, newoff, 0, (size_t)npts + 1, rocprim::plus<int>(), st));
+    int sizes[2] = {0, 0};
+    DCHK(hipMemcpyAsync(&sizes[0], newi + npts, sizeof(int), hipMemcpyDeviceToHost, st));
+    DCHK(hipMemcpyAsync(&sizes[1], newoff + npts, sizeof(int), hipMemcpyDeviceToHost, st));
+    // the flagged entries of kept points are dropped; for a point without any
+    // flag every entry is kept (the flag array is zero there)
+    Tracks b;
+    rc |= b.alloc(sizes_cap_pts, sizes_cap_tr);
+    if (rc) return RPBA_E_NOMEM;
+    RdArgs a;
+    a.npts = npts; a.toff = toff; a.tcam = tcam; a.tfo = tfo; a.nrpv = nrpv; a.newi = newi; a.newoff = newoff;
+    a.flag = flag; a.ox = ox; a.oy = oy; a.wx = w0x; a.wy = w0y; a.wxy = w0xy; a.X = X;
+    a.toff2 = b.toff; a.tcam2 = b.tcam; a.tpt2 = b.tpt; a.tfo2 = b.tfo; a.nrpv2 = b.nrpv;
+    a.ox2 = b.ox; a.oy2 = b.oy; a.wx2 = b.w0x; a.wy2 = b.w0y; a.wxy2 = b.w0xy; a.X2 = b.X;
+    k_rd_scatter<<<nb_(npts + 1, 256), 256, 0, st>>>(a);
+    DCHK(hipStreamSynchronize(st));
+    const int np2 = sizes[0], nt2 = sizes[1];
+    DCHK(hipMemcpyAsync(X, b.X, sizeof(double) * 3 * (size_t)np2, hipMemcpyDeviceToDevice, st));
+    DCHK(hipStreamSynchronize(st));  // b.X is read by the copy above
+    swap_tracks(b);
+    b.release();
+    npts = np2;
+    ntr = nt2;
+    return RPBA_OK;
+}
+
+int Tracks::alloc(size_t np, size_t nt)
+{
+    size_t c = 0;
+    int rc = 0;
+    rc |= grow(toff, c, np + 1);
+    c = 0; rc |= grow(nrpv, c, np);
+    c = 0; rc |= grow(X, c, 3 * np);
+    c = 0; rc |= grow(tcam, c, nt);
+    c = 0; rc |= grow(tpt, c, nt);
+    c = 0; rc |= grow(tfo, c, nt);
+    c = 0; rc |= grow(ox, c, nt);
+    c = 0; rc |= grow(oy, c, nt);
+    c = 0; rc |= grow(w0x, c, nt);
+    c = 0; rc |= grow(w0y, c, nt);
+    c = 0; rc |= grow(w0xy, c, nt);
+    return rc;
+}
+
+void Tracks::release()
+{
+    hipFree(toff); hipFree(nrpv); hipFree(X); hipFree(tcam); hipFree(tpt); hipFree(tfo);
+    hipFree(ox); hipFree(oy); hipFree(w0x); hipFree(w0y); hipFree(w0xy);
+    toff = nrpv = tcam = tpt = tfo = nullptr;
+    ox = oy = w0x = w0y = w0xy = nullptr;
+    X = nullptr;
+}
+
+void DevStruct::swap_tracks(Tracks &b)
+{
+    std::swap(toff, b.toff); std::swap(nrpv, b.nrpv); std::swap(tcam, b.tcam); std::swap(tpt, b.tpt);
+    std::swap(tfo, b.tfo); std::swap(ox, b.ox); std::swap(oy, b.oy); std::swap(w0x, b.w0x);
+    std::swap(w0y, b.w0y); std::swap(w0xy, b.w0xy);
+}
+
+int DevStruct::upload_tracks(const Structure &s, const float *obs_xy, const std::vector<float> &w0, hipStream_t st)
+{
+    nc = s.nc;
+    npts = s.npts;
+    ntr = s.ntr;
+    sizes_cap_pts = (size_t)npts;
+    sizes_cap_tr = (size_t)ntr;
+    Tracks t;
+    if (!toff) {
+        if (t.alloc(sizes_cap_pts, sizes_cap_tr)) return RPBA_E_NOMEM;
+        toff = t.toff; nrpv = t.nrpv; tcam = t.tcam; tpt = t.tpt; tfo = t.tfo;
+        ox = t.ox; oy = t.oy; w0x = t.w0x; w0y = t.w0y; w0xy = t.w0xy;
+        (void)hipFree(t.X);
+    }
+    std::vector<float> hx((size_t)ntr), hy((size_t)ntr), a((size_t)ntr), b((size_t)ntr), c((size_t)ntr);
+    for (int q = 0; q < ntr; ++q) {
+        const int fo = s.tfo[q];
+        hx[q] = obs_xy[2 * (size_t)fo];
+        hy[q] = obs_xy[2 * (size_t)fo + 1];
+        a[q] = w0[3 * (size_t)q];
+        b[q] = w0[3 * (size_t)q + 1];
+        c[q] = w0[3 * (size_t)q + 2];
+    }
+    DCHK(hipMemcpyAsync(toff, s.toff.data(), sizeof(int) * ((size_t)npts + 1), hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(nrpv, s.nrpv.data(), sizeof(int) * (size_t)npts, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(tcam, s.tcam.data(), sizeof(int) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(tpt, s.tpt.data(), sizeof(int) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(tfo, s.tfo.data(), sizeof(int) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(ox, hx.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(oy, hy.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(w0x, a.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(w0y, b.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(w0xy, c.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    DCHK(hipStreamSynchronize(st));
+    return RPBA_OK;
+}
+
+int DevStruct::download_tracks(std::vector<int> &toff_h, std::vector<int> &nrpv_h, std::vector<int> &tfo_h,
+                               std::vector<int> &tcam_h, hipStream_t st) const
+{
+    toff_h.resize((size_t)npts + 1);
+    nrpv_h.resize((size_t)npts);
+    tfo_h.resize((size_t)ntr);
+    tcam_h.resize((size_t)ntr);
+    DCHK(hipMemcpyAsync(toff_h.data(), toff, sizeof(int) * toff_h.size(), hipMemcpyDeviceToHost, st));
+    if (npts) DCHK(hipMemcpyAsync(nrpv_h.data(), nrpv, sizeof(int) * nrpv_h.size(), hipMemcpyDeviceToHost, st));
+    if (ntr) DCHK(hipMemcpyAsync(tfo_h.data(), tfo, sizeof(int) * tfo_h.size(), hipMemcpyDeviceToHost, st));
+    if (ntr) DCHK(hipMemcpyAsync(tcam_h.data(), tcam, sizeof(int) * tcam_h.size(), hipMemcpyDeviceToHost, st));
+    DCHK(hipStreamSynchronize(st));
+    return RPBA_OK;
+}
+
+void DevStruct::release()
+{
+    Tracks t;
+    t.toff = toff; t.nrpv = nrpv; t.tcam = tcam; t.tpt = tpt; t.tfo = tfo;
+    t.ox = ox; t.oy = oy; t.w0x = w0x; t.w0y = w0y; t.w0xy = w0xy; t.X = nullptr;
+    t.release();
+    toff = nrpv = tcam = tpt = tfo = nullptr;
+    ox = oy = w0x = w0y = w0xy = nullptr;
+    auto fr = [](auto *&p, size_t &c) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        c = 0;
+    };
+    fr(camoff, camoff_cap); fr(camq, camq_cap); fr(u32a, u32a_cap); fr(i32a, i32a_cap); fr(i32b, i32b_cap);
+    fr(ppoff, ppoff_cap); fr(key_in, key_in_cap); fr(key_out, key_out_cap); fr(val_in, val_in_cap);
+    fr(pab, pab_cap); fr(ukey, ukey_cap); fr(ucnt, ucnt_cap); fr(nruns, nruns_cap); fr(poff, poff_cap);
+    fr(bk, bk_cap); fr(bl, bl_cap); fr(roff, roff_cap); fr(coff, coff_cap); fr(cblk, cblk_cap);
+    fr(key2, key2_cap); fr(key2s, key2s_cap); fr(bid, bid_cap); fr(newi, newi_cap); fr(newoff, newoff_cap);
+    fr(tmp, tmp_cap);
+}
+
+}  // namespace rpba
