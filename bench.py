@@ -53,7 +53,7 @@ def algorithmic_bytes(kernel, sz):
     if kernel == "schur_pairs":  # Y and W of every entry, pair lists, block offsets, S blocks out
         return 432 * No + 8 * P + 4 * (B + 1) + 648 * B
     if kernel == "lin_points":  # entry inputs, points, W and Y out, V^-1 and g_p out
-        return (4 + 8 + 12) * No + 432 * No + (24 + 4) * Np + (72 + 24) * Np
+        return (4 + 8 + 12) * No + 432 * No + (24 + 8) * Np + (72 + 24) * Np
     if kernel == "schur_diag":  # Y, W and g_p of every entry, diagonal terms out
         return (432 + 4 + 4) * No + 24 * Np + 720 * Nc
     if kernel == "lin_cameras":  # entry inputs and points in; S_kk and b out
@@ -193,6 +193,23 @@ def cpu_baseline(sc):
             "sample": f"1 LM iteration of pass 0 ({its} PCG iterations), {dt:.1f} s"}
 
 
+def calibrate_copy(nbytes, reps=5):
+    """Device-to-device copies of a known size: the reference kernel that scales
+    the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    device_sync()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration_copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,11 +217,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2_961")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run device-to-device copies of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the solve is not sharded); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
     run_bench(args)
 
 

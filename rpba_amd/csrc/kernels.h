@@ -5,6 +5,9 @@
 
 namespace rpba {
 
+// doubles per entry record of W and Y (27 used; 256 bytes keeps records line aligned)
+constexpr int kRec = 32;
+
 // device-resident camera state, all double, indexed by image
 struct CamPtrs {
     double *Rb, *Cb;    // base rotation / centre offset of the pass (eRM, ePC)
@@ -29,7 +32,9 @@ struct LinArgs {
     const float *ox, *oy, *wx, *wy, *wxy;
     const double *X;
     double *Wo, *Yo, *Vinv, *gp, *Sd, *bvec;
-    double *Dd;  // per camera: diagonal Schur term (81) and Y g_p (9)
+    double *Dd;  // per chunk: diagonal Schur term (81) and Y g_p (9)
+    int nch;     // chunks of the camera-major list (at most 64 entries of one image each)
+    const int *ch_start, *ch_off;  // chunk -> first entry (nch + 1); image -> first chunk (nc + 1)
 };
 
 struct SchurArgs {

@@ -6,7 +6,8 @@
 //     (wx, wy, wxy) for the current Wij and the original Wijoldv; key[q] the
 //     float robust key sig0rmv; flag[q] ISflag.
 //   points: X[3*i] accepted (XPMatrices point part), Xn trial.
-//   per entry 9x3 W = Bc^T W Bp and Y = W V^-1 as 27 contiguous doubles.
+//   per entry 9x3 W = Bc^T W Bp and Y = W V^-1 as 27 doubles in 32-double
+//     (256-byte, cache-line aligned) records.
 //   cameras: uniform 9-slot parameter blocks (rot a,b,c | C x,y,z | d0, d1, f)
 //     with a per-camera mask of the estimated slots (image 0 pose and one C
 //     component of the gauge camera are fixed, rbacore.cpp:1042-1069).
@@ -176,6 +177,32 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
 // linearisation: points (V, g_p, W, Y), one thread per point
 // ---------------------------------------------------------------------------
 
+// calibrated point columns only (the Bp part of jac_obs)
+__device__ __forceinline__ void jac_point(const CamV &c, const Ev &e, double Bp[2][3])
+{
+    const double u = e.u, v = e.v, xs3 = e.xs3;
+    const double x1 = u * xs3, x2 = v * xs3;
+    const double k2 = c.d0, k4 = c.d1, f = c.f;
+    const double xy2 = u * u + v * v, xy4 = xy2 * xy2;
+    const double xxpc = k4 * xy4 + u * (2 * k2 * u + 4 * k4 * u * xy2) + k2 * xy2 + 1.;
+    const double xypc = u * (4 * k4 * xy2 * v + 2 * k2 * v);
+    const double yxpc = (4 * k4 * u * xy2 + 2 * k2 * u) * v;
+    const double yypc = v * (2 * k2 * v + 4 * k4 * xy2 * v) + k4 * xy4 + k2 * xy2 + 1.;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const double c0 = -c.R[0 * 3 + j] * xs3 + c.R[2 * 3 + j] * x1;
+        const double c1 = -c.R[1 * 3 + j] * xs3 + c.R[2 * 3 + j] * x2;
+        const double xp = xxpc * c0 + xypc * c1;
+        const double yp = yypc * c1 + yxpc * c0;
+        Bp[0][j] = xp * f + yp * 0.;
+        Bp[1][j] = yp * f;
+    }
+}
+
+// One thread per point.  Pass 1 over the track: V = sum Bp^T W Bp and
+// g_p = sum Bp^T W r, then damping and the 3x3 inverse (rbacore.cpp:1341-1344).
+// Pass 2 recomputes each observation's Jacobian (cheaper than reading W back)
+// and writes W = Bc^T W Bp and Y = W V^-1 into the entry's 32-double records.
 __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,17 +216,9 @@ __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
         load_cam(a.cam, k, c);
         Ev ev;
         eval_obs(c, X0, X1, X2, a.ox[q], a.oy[q], a.defdiff, ev);
-        double Bc[2][9], Bp[2][3];
-        jac_obs(c, ev, a.cam.mask[k], Bc, Bp);
+        double Bp[2][3];
+        jac_point(c, ev, Bp);
         const double W0 = a.wx[q], W1 = a.wy[q], W2 = a.wxy[q];
-        double *Wo = a.Wo + (size_t)q * 27;
-#pragma unroll
-        for (int r = 0; r < 9; ++r) {
-            const double t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
-            const double t1 = Bc[0][r] * W2 + Bc[1][r] * W1;
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc) Wo[r * 3 + cc] = t0 * Bp[0][cc] + t1 * Bp[1][cc];
-        }
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const double t0 = Bp[0][r] * W0 + Bp[1][r] * W2;
@@ -219,11 +238,26 @@ __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
 #pragma unroll
     for (int j = 0; j < 3; ++j) a.gp[3 * i + j] = g[j];
     for (int q = b; q < e; ++q) {
-        const double *Wo = a.Wo + (size_t)q * 27;
-        double *Yo = a.Yo + (size_t)q * 27;
+        const int k = a.tcam[q];
+        CamV c;
+        load_cam(a.cam, k, c);
+        Ev ev;
+        eval_obs(c, X0, X1, X2, a.ox[q], a.oy[q], a.defdiff, ev);
+        double Bc[2][9], Bp[2][3];
+        jac_obs(c, ev, a.cam.mask[k], Bc, Bp);
+        const double W0 = a.wx[q], W1 = a.wy[q], W2 = a.wxy[q];
+        double *Wo = a.Wo + (size_t)q * kRec;
+        double *Yo = a.Yo + (size_t)q * kRec;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
-            const double w0 = Wo[r * 3], w1 = Wo[r * 3 + 1], w2 = Wo[r * 3 + 2];
+            const double t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
+            const double t1 = Bc[0][r] * W2 + Bc[1][r] * W1;
+            const double w0 = t0 * Bp[0][0] + t1 * Bp[1][0];
+            const double w1 = t0 * Bp[0][1] + t1 * Bp[1][1];
+            const double w2 = t0 * Bp[0][2] + t1 * Bp[1][2];
+            Wo[r * 3] = w0;
+            Wo[r * 3 + 1] = w1;
+            Wo[r * 3 + 2] = w2;
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) Yo[r * 3 + cc] = w0 * Vi[0 * 3 + cc] + w1 * Vi[1 * 3 + cc] + w2 * Vi[2 * 3 + cc];
         }
@@ -231,17 +265,19 @@ __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// diagonal Schur term per camera, atomic-free: one wavefront per image k walks
-// its entries (camera-major order) and accumulates D = sum Y W^T (81) and
-// e = sum Y g_p (9), staging each entry's Y, W and g_p through LDS.
+// diagonal Schur term per camera, atomic-free: the camera-major entry list is
+// cut into chunks of at most 64 entries of one image; one wavefront per chunk
+// accumulates D = sum Y W^T (81) and e = sum Y g_p (9), staging each entry's
+// Y, W and g_p through LDS.  k_lin_cameras adds the chunks of an image in
+// order.
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
 {
-    const int k = blockIdx.x;
+    const int ch = blockIdx.x;  // chunk: entries [ch_start[ch], ch_start[ch+1]) of one image
     const int lane = threadIdx.x;
     __shared__ double st[8 * 57];
-    const int b = a.camoff[k], e = a.camoff[k + 1];
+    const int b = a.ch_start[ch], e = a.ch_start[ch + 1];
     const int o0 = lane, o1 = lane + 64;
     // output o < 81: D(o/9, o%9); 81 <= o < 90: e(o-81)
     const int r0 = o0 / 9, c0 = o0 % 9;
@@ -255,9 +291,9 @@ __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
             const int q = a.camq[j0 + j];
             double v;
             if (idx < 27)
-                v = a.Yo[(size_t)q * 27 + idx];
+                v = a.Yo[(size_t)q * kRec + idx];
             else if (idx < 54)
-                v = a.Wo[(size_t)q * 27 + (idx - 27)];
+                v = a.Wo[(size_t)q * kRec + (idx - 27)];
             else
                 v = a.gp[3 * (size_t)a.tpt[q] + (idx - 54)];
             st[x] = v;
@@ -271,7 +307,7 @@ __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
         }
         __syncthreads();
     }
-    double *D = a.Dd + (size_t)k * 90;
+    double *D = a.Dd + (size_t)ch * 90;
     D[o0] = acc0;
     if (o1 < 90) D[o1] = acc1;
 }
@@ -327,8 +363,15 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
         const double s = wave_sum(acc[j]);
         if (lane == 0) red[w][j] = s;
     }
+    // diagonal Schur terms: the image's chunks in order
+    __shared__ double Dsh[90];
+    if (t < 90) {
+        double sd = 0.;
+        for (int ch = a.ch_off[k]; ch < a.ch_off[k + 1]; ++ch) sd += a.Dd[(size_t)ch * 90 + t];
+        Dsh[t] = sd;
+    }
     __syncthreads();
-    const double *D = a.Dd + (size_t)k * 90;
+    const double *D = Dsh;
     if (t < 81) {
         const int r = t / 9, cc = t % 9;
         const int rr = r < cc ? r : cc, c2 = r < cc ? cc : r;
@@ -363,7 +406,11 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 
 __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
 {
-    const int bid = blockIdx.x;
+    // workgroups go round-robin over the 8 XCDs; give each XCD a contiguous
+    // range of blocks so the blocks (k, l), (k, l+1), ... of one camera row
+    // share that XCD's L2 for the entries of camera k
+    const int per = (a.nblk + 7) / 8;
+    const int bid = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (bid >= a.nblk) return;
     const int lane = threadIdx.x;
     __shared__ double st[8 * 54];
@@ -377,7 +424,7 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
         for (int x = lane; x < np * 54; x += 64) {
             const int j = x / 54, idx = x % 54;
             const unsigned long long ab = a.pab[p + j];
-            st[x] = (idx < 27) ? a.Yo[(size_t)(ab >> 32) * 27 + idx] : a.Wo[(size_t)(ab & 0xffffffffull) * 27 + (idx - 27)];
+            st[x] = (idx < 27) ? a.Yo[(size_t)(ab >> 32) * kRec + idx] : a.Wo[(size_t)(ab & 0xffffffffull) * kRec + (idx - 27)];
         }
         __syncthreads();
         for (int j = 0; j < np; ++j) {
@@ -664,7 +711,7 @@ __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
     if (i >= a.npts) return;
     double t[3] = {a.gp[3 * i], a.gp[3 * i + 1], a.gp[3 * i + 2]};
     for (int q = a.toff[i]; q < a.toff[i + 1]; ++q) {
-        const double *W = a.Wo + (size_t)q * 27;
+        const double *W = a.Wo + (size_t)q * kRec;
         const double *xc = a.sol + 9 * (size_t)a.tcam[q];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -1015,9 +1062,12 @@ void launch_lin_points(const LinArgs &a, hipStream_t st)
 }
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
 {
-    if (s.nblk > 0) k_schur_pairs<<<s.nblk, 64, 0, st>>>(s);
+    if (s.nblk > 0) k_schur_pairs<<<8 * ((s.nblk + 7) / 8), 64, 0, st>>>(s);
 }
-void launch_schur_diag(const LinArgs &a, hipStream_t st) { k_schur_diag<<<a.nc, 64, 0, st>>>(a); }
+void launch_schur_diag(const LinArgs &a, hipStream_t st)
+{
+    if (a.nch > 0) k_schur_diag<<<a.nch, 64, 0, st>>>(a);
+}
 void launch_lin_cameras(const LinArgs &a, hipStream_t st) { k_lin_cameras<<<a.nc, 256, 0, st>>>(a); }
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
 {

@@ -243,11 +243,12 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_key.alloc(ntr);
     rc |= c->d_flag.alloc(ntr);
     rc |= c->d_flagst.alloc(ntr);
-    rc |= c->d_Wo.alloc(27 * ntr);
-    rc |= c->d_Yo.alloc(27 * ntr);
+    rc |= c->d_Wo.alloc((size_t)kRec * ntr);
+    rc |= c->d_Yo.alloc((size_t)kRec * ntr);
     rc |= c->d_Vinv.alloc(9 * np);
     rc |= c->d_gp.alloc(3 * np);
     rc |= c->d_Su.alloc(81 * (size_t)c->ds.nblk);
+    rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
     const int nb = std::max(cost_blocks(c->ds.ntr), (int)((np + 255) / 256 + 1));
     rc |= c->d_partial.alloc((size_t)nb);
     rc |= c->d_ipartial.alloc((size_t)nb);
@@ -359,7 +360,7 @@ static LinArgs lin_args(rpba_ctx *c)
     a.ox = c->ds.ox; a.oy = c->ds.oy; a.wx = c->d_wx.p; a.wy = c->d_wy.p; a.wxy = c->d_wxy.p;
     a.X = c->d_X.p;
     a.Wo = c->d_Wo.p; a.Yo = c->d_Yo.p; a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.Sd = c->d_Sd.p; a.bvec = c->d_b.p;
-    a.Dd = c->d_Dd.p;
+    a.Dd = c->d_Dd.p; a.nch = c->ds.nch; a.ch_start = c->ds.ch_start; a.ch_off = c->ds.ch_off;
     return a;
 }
 
@@ -721,7 +722,6 @@ static int ctx_init(rpba_ctx *c)
     rc |= c->d_scal.alloc(4);
     rc |= c->d_iscal.alloc(4);
     rc |= c->d_pcg.alloc(2);
-    rc |= c->d_Dd.alloc(90 * (size_t)nc);
     rc |= c->d_ppart.alloc(3 * (size_t)pcg_groups(nc));
     rc |= c->c_Rb.alloc(n9);
     rc |= c->c_Cb.alloc(3 * (size_t)nc);

@@ -31,6 +31,12 @@ struct DevStruct {
     // camera-major order: entries of image k are camq[camoff[k] .. camoff[k+1])
     int *camoff = nullptr, *camq = nullptr;
     size_t camoff_cap = 0, camq_cap = 0;
+    // chunks of at most kChunk entries of one image over camq: ch_start (nch + 1
+    // entry offsets), ch_off (nc + 1 first chunk of each image)
+    static constexpr int kChunk = 64;
+    int nch = 0;
+    int *ch_start = nullptr, *ch_off = nullptr;
+    size_t ch_start_cap = 0, ch_off_cap = 0;
     // pairs (a << 32 | b), cam(a) < cam(b), grouped by block in row-major
     // block order; poff: pairs of block b; bk/bl: block (k, l); roff: blocks of
     // row k; coff/cblk: blocks of column l (ids, k ascending)

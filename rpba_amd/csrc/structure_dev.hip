@@ -150,6 +150,22 @@ int DevStruct::build(hipStream_t st, int *first_empty)
             *first_empty = k;
             return RPBA_OK;
         }
+    {
+        std::vector<int> cs, cof((size_t)nc + 1, 0);
+        for (int k = 0; k < nc; ++k) {
+            cof[k] = (int)cs.size();
+            for (int j = co[k]; j < co[(size_t)k + 1]; j += kChunk) cs.push_back(j);
+        }
+        cof[nc] = (int)cs.size();
+        nch = (int)cs.size();
+        cs.push_back(ntr);
+        rc |= grow(ch_start, ch_start_cap, cs.size());
+        rc |= grow(ch_off, ch_off_cap, cof.size());
+        if (rc) return RPBA_E_NOMEM;
+        DCHK(hipMemcpyAsync(ch_start, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice, st));
+        DCHK(hipMemcpyAsync(ch_off, cof.data(), sizeof(int) * cof.size(), hipMemcpyHostToDevice, st));
+        DCHK(hipStreamSynchronize(st));
+    }
 
     // --- covisibility pairs: count, scan, generate, stable sort by block ---
     rc |= grow(ppoff, ppoff_cap, (size_t)npts + 1);
@@ -424,7 +440,7 @@ void DevStruct::release()
         p = nullptr;
         c = 0;
     };
-    fr(camoff, camoff_cap); fr(camq, camq_cap); fr(u32a, u32a_cap); fr(i32a, i32a_cap); fr(i32b, i32b_cap);
+    fr(camoff, camoff_cap); fr(camq, camq_cap); fr(ch_start, ch_start_cap); fr(ch_off, ch_off_cap); fr(u32a, u32a_cap); fr(i32a, i32a_cap); fr(i32b, i32b_cap);
     fr(ppoff, ppoff_cap); fr(key_in, key_in_cap); fr(key_out, key_out_cap); fr(val_in, val_in_cap);
     fr(pab, pab_cap); fr(ukey, ukey_cap); fr(ucnt, ucnt_cap); fr(nruns, nruns_cap); fr(poff, poff_cap);
     fr(bk, bk_cap); fr(bl, bl_cap); fr(roff, roff_cap); fr(coff, coff_cap); fr(cblk, cblk_cap);
