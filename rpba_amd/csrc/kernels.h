@@ -50,6 +50,8 @@ struct PcgArgs {
     const double *Sd, *Su, *Cinv, *b;
     const int *row_off, *col_off, *col_blk, *blk_k, *blk_l;
     double *x, *best, *r, *d, *Ad, *h, *h1, *rowdot, *part;
+    double *yN, *yT;  // once-read SpMV: per upper block S_kl d_l and S_kl^T d_k (9 each)
+    int nblk, once_read;
     PcgScalars *S;  // two slots: launch j reads S[j & 1] and writes S[(j + 1) & 1]
     int j;          // launch index inside one PCG solve
 };

@@ -592,6 +592,72 @@ __global__ __launch_bounds__(256) void k_spmv(PcgArgs a)
     }
 }
 
+// Ad = S d reading every stored block once (used when S is large):
+// k_spmv_blocks: 56 upper blocks per workgroup (two per group of 9 lanes):
+// yN[b] = S_kl d_l (row k's share) and yT[b] = S_kl^T d_k (row l's share);
+// k_spmv_gather: one workgroup per row k adds S_kk d_k, the yN of its row
+// blocks and the yT of its column blocks in a fixed order, and writes the
+// row's share of d.Ad.
+__global__ __launch_bounds__(256) void k_spmv_blocks(PcgArgs a)
+{
+    if (a.S[a.j & 1].done) return;
+    const int t = threadIdx.x;
+    const int g = t / 9, r = t % 9;
+    if (g >= 28) return;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int bb = blockIdx.x * 56 + h * 28 + g;
+        if (bb >= a.nblk) return;
+        const double *B = a.Su + 81 * (size_t)bb;
+        const double *dl = a.d + 9 * (size_t)a.blk_l[bb];
+        const double *dk = a.d + 9 * (size_t)a.blk_k[bb];
+        double yn = 0., yt = 0.;
+#pragma unroll
+        for (int cc = 0; cc < 9; ++cc) {
+            yn += B[r * 9 + cc] * dl[cc];
+            yt += B[cc * 9 + r] * dk[cc];
+        }
+        a.yN[9 * (size_t)bb + r] = yn;
+        a.yT[9 * (size_t)bb + r] = yt;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_spmv_gather(PcgArgs a)
+{
+    if (a.S[a.j & 1].done) return;
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    const int g = t / 9, r = t % 9;
+    __shared__ double red[28][9];
+    const int r0 = a.row_off[k], nu = a.row_off[k + 1] - r0;
+    const int c0 = a.col_off[k], nl = a.col_off[k + 1] - c0;
+    double acc = 0.;
+    if (g < 28) {
+        if (g == 0) {
+            const double *B = a.Sd + 81 * (size_t)k + 9 * r;
+            const double *dk = a.d + 9 * (size_t)k;
+#pragma unroll
+            for (int cc = 0; cc < 9; ++cc) acc += B[cc] * dk[cc];
+        }
+        for (int it = g; it < nu; it += 28) acc += a.yN[9 * (size_t)(r0 + it) + r];
+        for (int it = g; it < nl; it += 28) acc += a.yT[9 * (size_t)a.col_blk[c0 + it] + r];
+        red[g][r] = acc;
+    }
+    __syncthreads();
+    if (t < 9) {
+        double s = 0.;
+        for (int gg = 0; gg < 28; ++gg) s += red[gg][t];
+        a.Ad[9 * (size_t)k + t] = s;
+        red[0][t] = s * a.d[9 * (size_t)k + t];
+    }
+    __syncthreads();
+    if (t == 0) {
+        double s = 0.;
+        for (int j = 0; j < 9; ++j) s += red[0][j];
+        a.rowdot[k] = s;
+    }
+}
+
 // sum of n values in a fixed order; every thread of the block gets it
 __device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
 {
@@ -1074,7 +1140,15 @@ void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
     k_bj_inverse<<<nblk(nc, 64), 64, 0, st>>>(nc, Sd, Cinv);
 }
 void launch_pcg_init(const PcgArgs &a, hipStream_t st) { k_pcg_init<<<1, 1024, 0, st>>>(a); }
-void launch_spmv(const PcgArgs &a, hipStream_t st) { k_spmv<<<a.nc, 256, 0, st>>>(a); }
+void launch_spmv(const PcgArgs &a, hipStream_t st)
+{
+    if (a.once_read && a.nblk > 0) {
+        k_spmv_blocks<<<(a.nblk + 55) / 56, 256, 0, st>>>(a);
+        k_spmv_gather<<<a.nc, 256, 0, st>>>(a);
+    } else {
+        k_spmv<<<a.nc, 256, 0, st>>>(a);
+    }
+}
 int pcg_groups(int nc) { return (nc + 27) / 28; }
 void launch_pcg_update(const PcgArgs &a, hipStream_t st)
 {

@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -125,13 +126,14 @@ struct rpba_ctx {
     std::vector<double> Pst, Ist;    // stored cameras (host copy at pass boundaries)
     std::vector<double> Pst_init, Ist_init, R0, C0;
     bool structure_dirty = true;     // pairs/blocks must be rebuilt
+    bool spmv_once = false;          // once-read SpMV (large S)
     DevStruct ds;                    // device tracks, camera-major order, pairs, blocks
 
     // device buffers
     DBuf<float> d_wx, d_wy, d_wxy, d_key, d_s00;
     DBuf<unsigned char> d_flag, d_flagst;
     DBuf<double> d_X, d_Xn, d_Wo, d_Yo, d_Vinv, d_gp, d_Sd, d_Su, d_b, d_Cinv;
-    DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_rowdot, d_partial, d_scal, d_Dd, d_ppart;
+    DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_rowdot, d_partial, d_scal, d_Dd, d_ppart, d_yN, d_yT;
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -249,6 +251,16 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_gp.alloc(3 * np);
     rc |= c->d_Su.alloc(81 * (size_t)c->ds.nblk);
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
+    // S beyond the last-level cache: read every block once (RPBA_SPMV=1/2 forces a variant)
+    {
+        const char *ev = std::getenv("RPBA_SPMV");
+        const size_t sbytes = 648 * (size_t)c->ds.nblk;
+        c->spmv_once = ev ? (ev[0] == '1') : (sbytes > ((size_t)384 << 20));
+    }
+    if (c->spmv_once) {
+        rc |= c->d_yN.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
+        rc |= c->d_yT.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
+    }
     const int nb = std::max(cost_blocks(c->ds.ntr), (int)((np + 255) / 256 + 1));
     rc |= c->d_partial.alloc((size_t)nb);
     rc |= c->d_ipartial.alloc((size_t)nb);
@@ -373,6 +385,7 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
     a.x = c->d_x.p; a.best = c->d_best.p; a.r = c->d_r.p; a.d = c->d_d.p; a.Ad = c->d_Ad.p;
     a.h = c->d_h.p; a.h1 = c->d_h1.p; a.rowdot = c->d_rowdot.p; a.part = c->d_ppart.p; a.S = c->d_pcg.p;
+    a.yN = c->d_yN.p; a.yT = c->d_yT.p; a.nblk = c->ds.nblk; a.once_read = c->spmv_once;
     a.j = 0;
     return a;
 }
@@ -943,6 +956,8 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->c_th.release(); c->c_thn.release(); c->c_Pst.release(); c->c_Ist.release(); c->c_mask.release();
     c->d_pcg.release();
     c->d_Dd.release();
+    c->d_yN.release();
+    c->d_yT.release();
     c->d_ppart.release();
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;

@@ -132,3 +132,21 @@ def test_partition_path_is_refused():
     assert rc == capi.RPBA_E_UNSUPPORTED
     rc, _, st, _ = api.calc_plsa(sc, capi.make_flags(threads=0, force_global=1))
     assert rc == 0 and st["passes_done"] == 2
+
+
+def test_once_read_spmv_variant(scene_c1, monkeypatch):
+    """The once-read SpMV (the default when S exceeds 384 MB) sums in another
+    order.  The reference's loose PCG stop (step norm 1e5 below the first)
+    can then move by one iteration, which shifts that LM step's sig0 by up to
+    ~1e-6; decisions, labels and the converged sigma agree."""
+    monkeypatch.setenv("RPBA_SPMV", "1")
+    _say("[once-read SpMV c1]")
+    flags = capi.make_flags(robust=1)
+    rc0, out0, st0, _ = oracle.solve(scene_c1, flags, fsites=0)
+    rc1, out1, st1, _ = api.calc_plsa(scene_c1, flags)
+    assert rc1 == rc0 == 0 and st1["itt"] == st0["itt"]
+    rel = np.abs(st1["trace_sig0"] - st0["trace_sig0"]) / st0["trace_sig0"]
+    assert rel.max() < 1e-5, rel
+    assert np.array_equal(st1["trace_accept"], st0["trace_accept"])
+    assert np.array_equal(out1["obs_kept"], out0["obs_kept"])
+    assert abs(st1["sigma_out"][1] - st0["sigma_out"][1]) <= 1e-6 * st0["sigma_out"][1]
