@@ -15,8 +15,11 @@ restart is timed too.  value = K / elapsed (LM iterations per second).
 
 Also reported: time to converged (one full solve: both passes, from the
 uploaded block to the stored state on the device), the final sigma0 and RMS,
-the per-kernel HIP-event roofline of the dominant kernel, and the CPU oracle
-timed on one LM iteration of the same scene (cpu_baseline).
+the chosen form (FP32 Jacobians and W V^-1 W^T products, FP64 sums, S, PCG
+and cost): its time to reach the plain form's final sigma0 and its errors
+against the plain FP64 result, the per-kernel HIP-event roofline of the
+dominant kernel, and the CPU oracle timed on one LM iteration of the same
+scene (cpu_baseline).
 """
 import argparse
 import ctypes as C
@@ -105,13 +108,41 @@ def run_bench(args):
 
     # time to converged: one whole solve from the uploaded block
     ctx.reset()
-    device_sync()
-    t1 = time.perf_counter()
-    rc, done, fin = ctx.run(-1)
-    device_sync()
-    ttc = time.perf_counter() - t1
+    ttc, recs = timed_solve(ctx)
     rc_d, out, st = ctx.download()
     sizes = ctx.sizes()
+
+    # the chosen form: FP32 Jacobians and W V^-1 W^T records, FP64 sums, S,
+    # PCG and cost; time until its pass-1 stored sigma0 reaches the plain
+    # form's final value (x (1 + 1e-6)), and its errors against plain FP64
+    chosen = None
+    if not args.no_chosen:
+        target = st["sig0_final"] * (1.0 + 1e-6)
+        cctx = api.Context(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, precision=1))
+        cctx.run(1)
+        cctx.reset()
+        cttc, crecs = timed_solve(cctx)
+        _, cout, cst = cctx.download()
+        cctx.close()
+        reach = next((t for t, ps, sg in crecs if ps == 1 and sg <= target), None)
+        f64 = lambda a: np.asarray(a, dtype=np.float64)
+        chosen = {
+            "form": "precision=1: FP32 Jacobians and W, Y = W V^-1 records; FP64 residuals, sums, S, PCG, cost",
+            "time_to_converged_s": round(cttc, 4),
+            "time_to_reach_plain_sigma0_s": None if reach is None else round(reach, 4),
+            "speedup_vs_plain": None if reach is None else round(ttc / reach, 3),
+            "itt": cst["itt"], "pcg_iters": cst["pcg_iters"], "sigma0_final": cst["sigma_out"][1],
+            "rms_px": cst["rms_px"],
+            "err_vs_plain": {
+                "sigma0_rel": abs(cst["sigma_out"][1] - st["sigma_out"][1]) / st["sigma_out"][1],
+                "rms_rel": abs(cst["rms_px"] - st["rms_px"]) / st["rms_px"],
+                "points_max_abs": float(np.abs(f64(cout["pts"]) - f64(out["pts"])).max()),
+                "points_median_abs": float(np.median(np.abs(f64(cout["pts"]) - f64(out["pts"])))),
+                "cam_f_max_rel": float((np.abs(f64(cout["cam_f"]) - f64(out["cam_f"])) / f64(out["cam_f"])).max()),
+                "cam_t_max_abs": float(np.abs(f64(cout["cam_P"])[:, :, 3] - f64(out["cam_P"])[:, :, 3]).max()),
+                "label_diffs": int((cout["obs_kept"] != out["obs_kept"]).sum()),
+            },
+        }
 
     # roofline: the same steps again with HIP events around every launch
     ctx.reset()
@@ -163,6 +194,7 @@ def run_bench(args):
         "pcg_iters": st["pcg_iters"],
         "sigma0_final": st["sigma_out"][1],
         "rms_px": st["rms_px"],
+        "chosen": chosen,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktimes.items() if v[1] > 0},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -172,6 +204,25 @@ def run_bench(args):
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(line), flush=True)
+
+
+def timed_solve(ctx):
+    """One whole solve, one LM iteration per call: returns the wall time and,
+    per iteration, (elapsed, pass the iteration ran in, stored sig0 after)."""
+    device_sync()
+    t0 = time.perf_counter()
+    recs = []
+    while True:
+        ps, _, _ = ctx.progress()
+        rc, done, fin = ctx.run(1)
+        if rc != 0:
+            raise RuntimeError(f"rpba_ctx_run failed with status {rc}")
+        device_sync()
+        _, _, sg = ctx.progress()
+        recs.append((time.perf_counter() - t0, ps, sg))
+        if fin:
+            break
+    return time.perf_counter() - t0, recs
 
 
 def cpu_baseline(sc):
@@ -217,6 +268,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2_961")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-chosen", action="store_true", help="skip the FP32-Jacobian form")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run device-to-device copies of this many bytes (counter calibration)")
     args = ap.parse_args()

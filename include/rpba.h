@@ -82,7 +82,8 @@ typedef struct rpba_flags {
     int32_t est_k4;          /* k4 -> addflags[1] */
     int32_t est_f;           /* f  -> addflags[2] */
     int32_t k24out;          /* k24out (I/O scaling only, rpba.cpp:255-262)     */
-    int32_t precision;       /* 0 = plain FP64 (the only form built so far)     */
+    int32_t precision;       /* 0 = plain FP64; 1 = FP32 Jacobians and W V^-1 W^T
+                                products with FP64 sums, S, PCG, residuals and cost */
     int32_t preconditioner;  /* 0 = block-Jacobi (rbacore.cpp:1838-1846)        */
     int32_t force_global;    /* 1 = run the global LSA even where the reference
                                 would dispatch to the partitioned path          */
@@ -100,7 +101,7 @@ typedef struct rpba_stats {
     int32_t itt;             /* LM iterations (the reference's itt counter)    */
     int64_t pcg_iters;       /* total PCG iterations                           */
     double sig0_final;       /* sig0 of the stored state (sigma0^2 / ihw2)     */
-    double rms_px;           /* sqrt(sum |r|^2 / n) over kept observations, final state */
+    double rms_px;           /* sqrt(sum |r|^2 / 2n) over kept observations, stored state */
     int32_t n_obs_kept;
     int32_t n_pts_kept;
     double exteval_sigma;    /* ITTTT value (rpbacore.cpp:938)                 */
@@ -133,7 +134,8 @@ int rpba_ctx_create(rpba_ctx **ctx, rpba_block *io, const rpba_flags *flags,
  * the end).  *iters_done receives the iterations run, *finished 1 when both
  * passes are complete (or the solve stopped on an error, returned as status). */
 int rpba_ctx_run(rpba_ctx *ctx, int32_t max_iters, int32_t *iters_done, int32_t *finished);
-/* Restore the state right after rpba_ctx_create (device-to-device copies). */
+/* Restore the state right after rpba_ctx_create (re-uploads the initial
+ * tracks and cameras from the context's host copies). */
 int rpba_ctx_reset(rpba_ctx *ctx);
 /* OutputLSAc (rbacore.cpp:644-666): write the stored state back to the block. */
 int rpba_ctx_download(rpba_ctx *ctx, rpba_stats *stats);
@@ -141,6 +143,9 @@ int rpba_ctx_download(rpba_ctx *ctx, rpba_stats *stats);
  * launches, measured with HIP events on the context's stream. */
 int rpba_ctx_set_timing(rpba_ctx *ctx, int32_t on);
 int rpba_ctx_kernel_time(rpba_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
+/* Progress of the running solve: pass (0, 1; 2 = done), LM iterations (itt)
+ * and sig0 of the stored state (sig0old, rbacore.cpp:568) in the current pass. */
+int rpba_ctx_progress(rpba_ctx *ctx, int32_t *pass, int32_t *itt, double *sig0_stored);
 /* Problem sizes after the point filter of PointXXvIn (rbacore.cpp:1954-1955)
  * and the current covisibility structure: out[0..5] = cameras, points,
  * observations, point pairs, off-diagonal S blocks, S dimension (nrioff). */

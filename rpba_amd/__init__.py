@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "librpba_hip.so")
 # every symbol include/rpba.h declares
 EXPORTS = [
     "rpba_calc_plsa", "rpba_ctx_create", "rpba_ctx_run", "rpba_ctx_reset", "rpba_ctx_download",
-    "rpba_ctx_set_timing", "rpba_ctx_kernel_time", "rpba_ctx_sizes", "rpba_ctx_destroy",
+    "rpba_ctx_set_timing", "rpba_ctx_kernel_time", "rpba_ctx_progress", "rpba_ctx_sizes", "rpba_ctx_destroy",
     "rpba_read_bal", "rpba_write_bal", "rpba_free_block", "rpba_read_parameters", "rpba_version",
 ]
 
@@ -42,6 +42,8 @@ def lib():
     L.rpba_ctx_download.restype = C.c_int
     L.rpba_ctx_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.rpba_ctx_kernel_time.argtypes = [C.c_void_p, C.c_char_p, P(C.c_double), P(C.c_int64)]
+    L.rpba_ctx_progress.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_double)]
+    L.rpba_ctx_progress.restype = C.c_int
     L.rpba_ctx_sizes.argtypes = [C.c_void_p, P(C.c_int64)]
     L.rpba_ctx_destroy.argtypes = [C.c_void_p]
     L.rpba_ctx_destroy.restype = None

@@ -50,6 +50,12 @@ class Context:
         lib().rpba_ctx_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n))
         return ms.value, n.value
 
+    def progress(self):
+        """(pass, itt, stored sig0) of the running solve."""
+        ps, it, sg = C.c_int32(), C.c_int32(), C.c_double()
+        lib().rpba_ctx_progress(self.h, C.byref(ps), C.byref(it), C.byref(sg))
+        return ps.value, it.value, sg.value
+
     def sizes(self):
         out = (C.c_int64 * 6)()
         lib().rpba_ctx_sizes(self.h, out)

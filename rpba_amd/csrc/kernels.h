@@ -31,7 +31,9 @@ struct LinArgs {
     const int *toff, *tcam, *tpt, *camoff, *camq;
     const float *ox, *oy, *wx, *wy, *wxy;
     const double *X;
-    double *Wo, *Yo, *Vinv, *gp, *Sd, *bvec;
+    void *Wrec, *Yrec;  // per-entry W and Y records, kRec elements, double or float (rec32)
+    int rec32;
+    double *Vinv, *gp, *Sd, *bvec;
     double *Dd;  // per chunk: diagonal Schur term (81) and Y g_p (9)
     int nch;     // chunks of the camera-major list (at most 64 entries of one image each)
     const int *ch_start, *ch_off;  // chunk -> first entry (nch + 1); image -> first chunk (nc + 1)
@@ -41,8 +43,9 @@ struct SchurArgs {
     int nblk;
     const int *poff;
     const unsigned long long *pab;  // pair (a << 32 | b), cam(a) < cam(b)
-    const double *Yo, *Wo;
+    const void *Yrec, *Wrec;
     double *Su;
+    int rec32;
 };
 
 struct PcgArgs {
@@ -60,7 +63,9 @@ struct StepArgs {
     int nc, npts;
     CamPtrs cam;
     const int *toff, *tcam;
-    const double *Wo, *Vinv, *gp, *sol, *X;
+    const void *Wrec;
+    int rec32;
+    const double *Vinv, *gp, *sol, *X;
     double *Xn;
     PcgScalars *S;
 };

@@ -80,53 +80,56 @@ __device__ __forceinline__ void eval_obs(const CamV &c, double X0, double X1, do
 
 // calibrated Jacobian dr/dtheta (CalcDerivatives rbacore.cpp:1420-1616,
 // eCalcCalibratedDerivativesX/3 1620-1673) in the uniform 9-slot layout;
-// unestimated slots give zero columns
-__device__ __forceinline__ void jac_obs(const CamV &c, const Ev &e, unsigned mask, double Bc[2][9], double Bp[2][3])
+// unestimated slots give zero columns.  T = double is the plain form; T =
+// float evaluates the same expressions in FP32 from the FP64 projection.
+template <typename T>
+__device__ __forceinline__ void jac_obs(const CamV &c, const Ev &e, unsigned mask, T Bc[2][9], T Bp[2][3])
 {
-    const double u = e.u, v = e.v, xs3 = e.xs3;
-    const double x1 = u * xs3, x2 = v * xs3;
-    const double xs31 = 1. / xs3, xs32 = xs31 * xs31;
-    const double k2 = c.d0, k4 = c.d1, f = c.f;
-    const double xy2 = u * u + v * v, xy4 = xy2 * xy2;
-    const double xxpc = k4 * xy4 + u * (2 * k2 * u + 4 * k4 * u * xy2) + k2 * xy2 + 1.;
-    const double xypc = u * (4 * k4 * xy2 * v + 2 * k2 * v);
-    const double yxpc = (4 * k4 * u * xy2 + 2 * k2 * u) * v;
-    const double yypc = v * (2 * k2 * v + 4 * k4 * xy2 * v) + k4 * xy4 + k2 * xy2 + 1.;
+    const T one = (T)1, zero = (T)0;
+    const T u = (T)e.u, v = (T)e.v, xs3 = (T)e.xs3;
+    const T x1 = u * xs3, x2 = v * xs3;
+    const T xs31 = one / xs3, xs32 = xs31 * xs31;
+    const T k2 = (T)c.d0, k4 = (T)c.d1, f = (T)c.f;
+    const T xy2 = u * u + v * v, xy4 = xy2 * xy2;
+    const T xxpc = k4 * xy4 + u * (2 * k2 * u + 4 * k4 * u * xy2) + k2 * xy2 + one;
+    const T xypc = u * (4 * k4 * xy2 * v + 2 * k2 * v);
+    const T yxpc = (4 * k4 * u * xy2 + 2 * k2 * u) * v;
+    const T yypc = v * (2 * k2 * v + 4 * k4 * xy2 * v) + k4 * xy4 + k2 * xy2 + one;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const double c0 = -c.R[0 * 3 + j] * xs3 + c.R[2 * 3 + j] * x1;
-        const double c1 = -c.R[1 * 3 + j] * xs3 + c.R[2 * 3 + j] * x2;
-        const double xp = xxpc * c0 + xypc * c1;
-        const double yp = yypc * c1 + yxpc * c0;
-        Bp[0][j] = xp * f + yp * 0.;
+        const T c0 = -(T)c.R[0 * 3 + j] * xs3 + (T)c.R[2 * 3 + j] * x1;
+        const T c1 = -(T)c.R[1 * 3 + j] * xs3 + (T)c.R[2 * 3 + j] * x2;
+        const T xp = xxpc * c0 + xypc * c1;
+        const T yp = yypc * c1 + yxpc * c0;
+        Bp[0][j] = xp * f + yp * zero;
         Bp[1][j] = yp * f;
     }
-    double rc[2][3];
+    T rc[2][3];
     rc[0][0] = x2 * x1 * xs32;
-    rc[1][0] = 1. + x2 * x2 * xs32;
-    rc[0][1] = -1. - x1 * x1 * xs32;
+    rc[1][0] = one + x2 * x2 * xs32;
+    rc[0][1] = -one - x1 * x1 * xs32;
     rc[1][1] = -x2 * x1 * xs32;
     rc[0][2] = x2 * xs31;
     rc[1][2] = -x1 * xs31;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const double xp = xxpc * rc[0][j] + xypc * rc[1][j];
-        const double yp = yypc * rc[1][j] + yxpc * rc[0][j];
-        Bc[0][j] = xp * f + yp * 0.;
+        const T xp = xxpc * rc[0][j] + xypc * rc[1][j];
+        const T yp = yypc * rc[1][j] + yxpc * rc[0][j];
+        Bc[0][j] = xp * f + yp * zero;
         Bc[1][j] = yp * f;
         Bc[0][3 + j] = Bp[0][j];
         Bc[1][3 + j] = Bp[1][j];
     }
-    const double xyp2 = u * u + v * v, xyp4 = xyp2 * xyp2;
-    Bc[0][6] = -u * xyp2 * f - v * xyp2 * 0.;
+    const T xyp2 = u * u + v * v, xyp4 = xyp2 * xyp2;
+    Bc[0][6] = -u * xyp2 * f - v * xyp2 * zero;
     Bc[1][6] = -v * xyp2 * f;
-    Bc[0][7] = -u * xyp4 * f - v * xyp4 * 0.;
+    Bc[0][7] = -u * xyp4 * f - v * xyp4 * zero;
     Bc[1][7] = -v * xyp4 * f;
-    Bc[0][8] = -e.su;
-    Bc[1][8] = -e.sv;
+    Bc[0][8] = -(T)e.su;
+    Bc[1][8] = -(T)e.sv;
 #pragma unroll
     for (int j = 0; j < 9; ++j)
-        if (!(mask & (1u << j))) Bc[0][j] = Bc[1][j] = 0.;
+        if (!(mask & (1u << j))) Bc[0][j] = Bc[1][j] = zero;
 }
 
 // 3x3 inverse by cofactors (adjugate / determinant)
@@ -178,23 +181,25 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
 // ---------------------------------------------------------------------------
 
 // calibrated point columns only (the Bp part of jac_obs)
-__device__ __forceinline__ void jac_point(const CamV &c, const Ev &e, double Bp[2][3])
+template <typename T>
+__device__ __forceinline__ void jac_point(const CamV &c, const Ev &e, T Bp[2][3])
 {
-    const double u = e.u, v = e.v, xs3 = e.xs3;
-    const double x1 = u * xs3, x2 = v * xs3;
-    const double k2 = c.d0, k4 = c.d1, f = c.f;
-    const double xy2 = u * u + v * v, xy4 = xy2 * xy2;
-    const double xxpc = k4 * xy4 + u * (2 * k2 * u + 4 * k4 * u * xy2) + k2 * xy2 + 1.;
-    const double xypc = u * (4 * k4 * xy2 * v + 2 * k2 * v);
-    const double yxpc = (4 * k4 * u * xy2 + 2 * k2 * u) * v;
-    const double yypc = v * (2 * k2 * v + 4 * k4 * xy2 * v) + k4 * xy4 + k2 * xy2 + 1.;
+    const T one = (T)1, zero = (T)0;
+    const T u = (T)e.u, v = (T)e.v, xs3 = (T)e.xs3;
+    const T x1 = u * xs3, x2 = v * xs3;
+    const T k2 = (T)c.d0, k4 = (T)c.d1, f = (T)c.f;
+    const T xy2 = u * u + v * v, xy4 = xy2 * xy2;
+    const T xxpc = k4 * xy4 + u * (2 * k2 * u + 4 * k4 * u * xy2) + k2 * xy2 + one;
+    const T xypc = u * (4 * k4 * xy2 * v + 2 * k2 * v);
+    const T yxpc = (4 * k4 * u * xy2 + 2 * k2 * u) * v;
+    const T yypc = v * (2 * k2 * v + 4 * k4 * xy2 * v) + k4 * xy4 + k2 * xy2 + one;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const double c0 = -c.R[0 * 3 + j] * xs3 + c.R[2 * 3 + j] * x1;
-        const double c1 = -c.R[1 * 3 + j] * xs3 + c.R[2 * 3 + j] * x2;
-        const double xp = xxpc * c0 + xypc * c1;
-        const double yp = yypc * c1 + yxpc * c0;
-        Bp[0][j] = xp * f + yp * 0.;
+        const T c0 = -(T)c.R[0 * 3 + j] * xs3 + (T)c.R[2 * 3 + j] * x1;
+        const T c1 = -(T)c.R[1 * 3 + j] * xs3 + (T)c.R[2 * 3 + j] * x2;
+        const T xp = xxpc * c0 + xypc * c1;
+        const T yp = yypc * c1 + yxpc * c0;
+        Bp[0][j] = xp * f + yp * zero;
         Bp[1][j] = yp * f;
     }
 }
@@ -202,7 +207,9 @@ __device__ __forceinline__ void jac_point(const CamV &c, const Ev &e, double Bp[
 // One thread per point.  Pass 1 over the track: V = sum Bp^T W Bp and
 // g_p = sum Bp^T W r, then damping and the 3x3 inverse (rbacore.cpp:1341-1344).
 // Pass 2 recomputes each observation's Jacobian (cheaper than reading W back)
-// and writes W = Bc^T W Bp and Y = W V^-1 into the entry's 32-double records.
+// and writes W = Bc^T W Bp and Y = W V^-1 into the entry's 32-element records
+// (R = double: 256 bytes; R = float, the FP32 form: 128 bytes).  Sums are FP64.
+template <typename R>
 __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,16 +223,16 @@ __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
         load_cam(a.cam, k, c);
         Ev ev;
         eval_obs(c, X0, X1, X2, a.ox[q], a.oy[q], a.defdiff, ev);
-        double Bp[2][3];
-        jac_point(c, ev, Bp);
-        const double W0 = a.wx[q], W1 = a.wy[q], W2 = a.wxy[q];
+        R Bp[2][3];
+        jac_point<R>(c, ev, Bp);
+        const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            const double t0 = Bp[0][r] * W0 + Bp[1][r] * W2;
-            const double t1 = Bp[0][r] * W2 + Bp[1][r] * W1;
+            const R t0 = Bp[0][r] * W0 + Bp[1][r] * W2;
+            const R t1 = Bp[0][r] * W2 + Bp[1][r] * W1;
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) V[r * 3 + cc] += t0 * Bp[0][cc] + t1 * Bp[1][cc];
-            g[r] += t0 * ev.d1 + t1 * ev.d2;
+            for (int cc = 0; cc < 3; ++cc) V[r * 3 + cc] += (double)(t0 * Bp[0][cc] + t1 * Bp[1][cc]);
+            g[r] += (double)t0 * ev.d1 + (double)t1 * ev.d2;
         }
     }
     V[0] *= a.mult1;
@@ -237,29 +244,32 @@ __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
     for (int j = 0; j < 9; ++j) a.Vinv[9 * i + j] = Vi[j];
 #pragma unroll
     for (int j = 0; j < 3; ++j) a.gp[3 * i + j] = g[j];
+    R Vr[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) Vr[j] = (R)Vi[j];
     for (int q = b; q < e; ++q) {
         const int k = a.tcam[q];
         CamV c;
         load_cam(a.cam, k, c);
         Ev ev;
         eval_obs(c, X0, X1, X2, a.ox[q], a.oy[q], a.defdiff, ev);
-        double Bc[2][9], Bp[2][3];
-        jac_obs(c, ev, a.cam.mask[k], Bc, Bp);
-        const double W0 = a.wx[q], W1 = a.wy[q], W2 = a.wxy[q];
-        double *Wo = a.Wo + (size_t)q * kRec;
-        double *Yo = a.Yo + (size_t)q * kRec;
+        R Bc[2][9], Bp[2][3];
+        jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
+        const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
+        R *Wo = (R *)a.Wrec + (size_t)q * kRec;
+        R *Yo = (R *)a.Yrec + (size_t)q * kRec;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
-            const double t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
-            const double t1 = Bc[0][r] * W2 + Bc[1][r] * W1;
-            const double w0 = t0 * Bp[0][0] + t1 * Bp[1][0];
-            const double w1 = t0 * Bp[0][1] + t1 * Bp[1][1];
-            const double w2 = t0 * Bp[0][2] + t1 * Bp[1][2];
+            const R t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
+            const R t1 = Bc[0][r] * W2 + Bc[1][r] * W1;
+            const R w0 = t0 * Bp[0][0] + t1 * Bp[1][0];
+            const R w1 = t0 * Bp[0][1] + t1 * Bp[1][1];
+            const R w2 = t0 * Bp[0][2] + t1 * Bp[1][2];
             Wo[r * 3] = w0;
             Wo[r * 3 + 1] = w1;
             Wo[r * 3 + 2] = w2;
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) Yo[r * 3 + cc] = w0 * Vi[0 * 3 + cc] + w1 * Vi[1 * 3 + cc] + w2 * Vi[2 * 3 + cc];
+            for (int cc = 0; cc < 3; ++cc) Yo[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
         }
     }
 }
@@ -272,11 +282,14 @@ __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
 // order.
 // ---------------------------------------------------------------------------
 
+template <typename R>
 __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
 {
     const int ch = blockIdx.x;  // chunk: entries [ch_start[ch], ch_start[ch+1]) of one image
     const int lane = threadIdx.x;
-    __shared__ double st[8 * 57];
+    __shared__ R st[8 * 54];
+    __shared__ double sg[8 * 3];
+    const R *Yr = (const R *)a.Yrec, *Wr = (const R *)a.Wrec;
     const int b = a.ch_start[ch], e = a.ch_start[ch + 1];
     const int o0 = lane, o1 = lane + 64;
     // output o < 81: D(o/9, o%9); 81 <= o < 90: e(o-81)
@@ -286,24 +299,24 @@ __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
     double acc0 = 0., acc1 = 0.;
     for (int j0 = b; j0 < e; j0 += 8) {
         const int np = (e - j0 < 8) ? e - j0 : 8;
-        for (int x = lane; x < np * 57; x += 64) {
-            const int j = x / 57, idx = x % 57;
+        for (int x = lane; x < np * 54; x += 64) {
+            const int j = x / 54, idx = x % 54;
             const int q = a.camq[j0 + j];
-            double v;
-            if (idx < 27)
-                v = a.Yo[(size_t)q * kRec + idx];
-            else if (idx < 54)
-                v = a.Wo[(size_t)q * kRec + (idx - 27)];
-            else
-                v = a.gp[3 * (size_t)a.tpt[q] + (idx - 54)];
-            st[x] = v;
+            st[x] = (idx < 27) ? Yr[(size_t)q * kRec + idx] : Wr[(size_t)q * kRec + (idx - 27)];
+        }
+        if (lane < np * 3) {
+            const int j = lane / 3;
+            sg[lane] = a.gp[3 * (size_t)a.tpt[a.camq[j0 + j]] + lane % 3];
         }
         __syncthreads();
         for (int j = 0; j < np; ++j) {
-            const double *Y = st + j * 57, *W = st + j * 57 + 27, *g = st + j * 57 + 54;
-            acc0 += Y[3 * r0] * W[3 * c0] + Y[3 * r0 + 1] * W[3 * c0 + 1] + Y[3 * r0 + 2] * W[3 * c0 + 2];
-            const double *w1 = g1 ? g : W + 3 * c1;
-            acc1 += Y[3 * r1] * w1[0] + Y[3 * r1 + 1] * w1[1] + Y[3 * r1 + 2] * w1[2];
+            const R *Y = st + j * 54, *W = st + j * 54 + 27;
+            const double *g = sg + j * 3;
+            acc0 += (double)(Y[3 * r0] * W[3 * c0] + Y[3 * r0 + 1] * W[3 * c0 + 1] + Y[3 * r0 + 2] * W[3 * c0 + 2]);
+            if (g1)
+                acc1 += (double)Y[3 * r1] * g[0] + (double)Y[3 * r1 + 1] * g[1] + (double)Y[3 * r1 + 2] * g[2];
+            else
+                acc1 += (double)(Y[3 * r1] * W[3 * c1] + Y[3 * r1 + 1] * W[3 * c1 + 1] + Y[3 * r1 + 2] * W[3 * c1 + 2]);
         }
         __syncthreads();
     }
@@ -320,6 +333,7 @@ __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
 // row/column with zero right-hand side.
 // ---------------------------------------------------------------------------
 
+template <typename R>
 __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 {
     const int k = blockIdx.x;
@@ -340,10 +354,10 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
         const int p = a.tpt[q];
         Ev ev;
         eval_obs(c, a.X[3 * p], a.X[3 * p + 1], a.X[3 * p + 2], a.ox[q], a.oy[q], a.defdiff, ev);
-        double Bc[2][9], Bp[2][3];
-        jac_obs(c, ev, mask, Bc, Bp);
-        const double W0 = a.wx[q], W1 = a.wy[q], W2 = a.wxy[q];
-        double Bt[9][2];
+        R Bc[2][9], Bp[2][3];
+        jac_obs<R>(c, ev, mask, Bc, Bp);
+        const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
+        R Bt[9][2];
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
             Bt[r][0] = Bc[0][r] * W0 + Bc[1][r] * W2;
@@ -353,9 +367,9 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 #pragma unroll
         for (int r = 0; r < 9; ++r)
 #pragma unroll
-            for (int cc = r; cc < 9; ++cc) acc[n++] += Bt[r][0] * Bc[0][cc] + Bt[r][1] * Bc[1][cc];
+            for (int cc = r; cc < 9; ++cc) acc[n++] += (double)(Bt[r][0] * Bc[0][cc] + Bt[r][1] * Bc[1][cc]);
 #pragma unroll
-        for (int r = 0; r < 9; ++r) acc[45 + r] += Bt[r][0] * ev.d1 + Bt[r][1] * ev.d2;
+        for (int r = 0; r < 9; ++r) acc[45 + r] += (double)Bt[r][0] * ev.d1 + (double)Bt[r][1] * ev.d2;
     }
     const int w = t >> 6, lane = t & 63;
 #pragma unroll
@@ -404,6 +418,7 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 // batches of 8; lane o owns outputs o and o + 64.
 // ---------------------------------------------------------------------------
 
+template <typename R>
 __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
 {
     // workgroups go round-robin over the 8 XCDs; give each XCD a contiguous
@@ -413,7 +428,8 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
     const int bid = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (bid >= a.nblk) return;
     const int lane = threadIdx.x;
-    __shared__ double st[8 * 54];
+    __shared__ R st[8 * 54];
+    const R *Yr = (const R *)a.Yrec, *Wr = (const R *)a.Wrec;
     const int pb = a.poff[bid], pe = a.poff[bid + 1];
     const int o0 = lane, o1 = lane + 64;
     const int r0 = o0 / 9, c0 = o0 % 9;
@@ -424,13 +440,13 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
         for (int x = lane; x < np * 54; x += 64) {
             const int j = x / 54, idx = x % 54;
             const unsigned long long ab = a.pab[p + j];
-            st[x] = (idx < 27) ? a.Yo[(size_t)(ab >> 32) * kRec + idx] : a.Wo[(size_t)(ab & 0xffffffffull) * kRec + (idx - 27)];
+            st[x] = (idx < 27) ? Yr[(size_t)(ab >> 32) * kRec + idx] : Wr[(size_t)(ab & 0xffffffffull) * kRec + (idx - 27)];
         }
         __syncthreads();
         for (int j = 0; j < np; ++j) {
-            const double *Y = st + j * 54, *W = st + j * 54 + 27;
-            acc0 += Y[3 * r0] * W[3 * c0] + Y[3 * r0 + 1] * W[3 * c0 + 1] + Y[3 * r0 + 2] * W[3 * c0 + 2];
-            acc1 += Y[3 * r1] * W[3 * c1] + Y[3 * r1 + 1] * W[3 * c1 + 1] + Y[3 * r1 + 2] * W[3 * c1 + 2];
+            const R *Y = st + j * 54, *W = st + j * 54 + 27;
+            acc0 += (double)(Y[3 * r0] * W[3 * c0] + Y[3 * r0 + 1] * W[3 * c0 + 1] + Y[3 * r0 + 2] * W[3 * c0 + 2]);
+            acc1 += (double)(Y[3 * r1] * W[3 * c1] + Y[3 * r1 + 1] * W[3 * c1 + 1] + Y[3 * r1 + 2] * W[3 * c1 + 2]);
         }
         __syncthreads();
     }
@@ -551,26 +567,21 @@ __global__ __launch_bounds__(256) void k_spmv(PcgArgs a)
     double acc = 0.;
     if (g < 28) {
         for (int it = g; it < nitems; it += 28) {
-            const double *B;
-            const double *dv;
-            bool tr = false;
             if (it == 0) {
-                B = a.Sd + 81 * (size_t)k;
-                dv = a.d + 9 * (size_t)k;
+                const double *B = a.Sd + 81 * (size_t)k;
+                const double *dv = a.d + 9 * (size_t)k;
+#pragma unroll
+                for (int cc = 0; cc < 9; ++cc) acc += B[r * 9 + cc] * dv[cc];
             } else if (it <= nu) {
                 const int bb = a.row_off[k] + it - 1;
-                B = a.Su + 81 * (size_t)bb;
-                dv = a.d + 9 * (size_t)a.blk_l[bb];
-            } else {
-                const int bb = a.col_blk[a.col_off[k] + it - 1 - nu];
-                B = a.Su + 81 * (size_t)bb;
-                dv = a.d + 9 * (size_t)a.blk_k[bb];
-                tr = true;
-            }
-            if (!tr) {
+                const double *B = a.Su + 81 * (size_t)bb;
+                const double *dv = a.d + 9 * (size_t)a.blk_l[bb];
 #pragma unroll
                 for (int cc = 0; cc < 9; ++cc) acc += B[r * 9 + cc] * dv[cc];
             } else {
+                const int bb = a.col_blk[a.col_off[k] + it - 1 - nu];
+                const double *B = a.Su + 81 * (size_t)bb;
+                const double *dv = a.d + 9 * (size_t)a.blk_k[bb];
 #pragma unroll
                 for (int cc = 0; cc < 9; ++cc) acc += B[cc * 9 + r] * dv[cc];
             }
@@ -771,19 +782,20 @@ __global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
 
 // dp_i = V_i^-1 (g_i - sum_q W_q^T dc_k) with the double camera solution
 // (rbacore.cpp:1395-1410); Xn = X - dp (rbacore.cpp:547)
+template <typename R>
 __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.npts) return;
     double t[3] = {a.gp[3 * i], a.gp[3 * i + 1], a.gp[3 * i + 2]};
     for (int q = a.toff[i]; q < a.toff[i + 1]; ++q) {
-        const double *W = a.Wo + (size_t)q * kRec;
+        const R *W = (const R *)a.Wrec + (size_t)q * kRec;
         const double *xc = a.sol + 9 * (size_t)a.tcam[q];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             double s = 0.;
 #pragma unroll
-            for (int r = 0; r < 9; ++r) s += W[r * 3 + j] * xc[r];
+            for (int r = 0; r < 9; ++r) s += (double)W[r * 3 + j] * xc[r];
             t[j] -= s;
         }
     }
@@ -1124,17 +1136,36 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
 
 void launch_lin_points(const LinArgs &a, hipStream_t st)
 {
-    if (a.npts > 0) k_lin_points<<<nblk(a.npts, 128), 128, 0, st>>>(a);
+    if (a.npts <= 0) return;
+    if (a.rec32)
+        k_lin_points<float><<<nblk(a.npts, 128), 128, 0, st>>>(a);
+    else
+        k_lin_points<double><<<nblk(a.npts, 128), 128, 0, st>>>(a);
 }
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
 {
-    if (s.nblk > 0) k_schur_pairs<<<8 * ((s.nblk + 7) / 8), 64, 0, st>>>(s);
+    if (s.nblk <= 0) return;
+    const unsigned g = 8 * ((s.nblk + 7) / 8);
+    if (s.rec32)
+        k_schur_pairs<float><<<g, 64, 0, st>>>(s);
+    else
+        k_schur_pairs<double><<<g, 64, 0, st>>>(s);
 }
 void launch_schur_diag(const LinArgs &a, hipStream_t st)
 {
-    if (a.nch > 0) k_schur_diag<<<a.nch, 64, 0, st>>>(a);
+    if (a.nch <= 0) return;
+    if (a.rec32)
+        k_schur_diag<float><<<a.nch, 64, 0, st>>>(a);
+    else
+        k_schur_diag<double><<<a.nch, 64, 0, st>>>(a);
 }
-void launch_lin_cameras(const LinArgs &a, hipStream_t st) { k_lin_cameras<<<a.nc, 256, 0, st>>>(a); }
+void launch_lin_cameras(const LinArgs &a, hipStream_t st)
+{
+    if (a.rec32)
+        k_lin_cameras<float><<<a.nc, 256, 0, st>>>(a);
+    else
+        k_lin_cameras<double><<<a.nc, 256, 0, st>>>(a);
+}
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
 {
     k_bj_inverse<<<nblk(nc, 64), 64, 0, st>>>(nc, Sd, Cinv);
@@ -1158,7 +1189,11 @@ void launch_pcg_update(const PcgArgs &a, hipStream_t st)
 }
 void launch_backsub(const StepArgs &a, hipStream_t st)
 {
-    if (a.npts > 0) k_backsub<<<nblk(a.npts, 256), 256, 0, st>>>(a);
+    if (a.npts <= 0) return;
+    if (a.rec32)
+        k_backsub<float><<<nblk(a.npts, 256), 256, 0, st>>>(a);
+    else
+        k_backsub<double><<<nblk(a.npts, 256), 256, 0, st>>>(a);
 }
 void launch_decode(const StepArgs &a, hipStream_t st) { k_decode<<<nblk(a.nc, 64), 64, 0, st>>>(a); }
 int cost_blocks(int ntr) { return (int)nblk(ntr > 0 ? ntr : 1, 256); }

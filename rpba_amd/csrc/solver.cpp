@@ -127,12 +127,15 @@ struct rpba_ctx {
     std::vector<double> Pst_init, Ist_init, R0, C0;
     bool structure_dirty = true;     // pairs/blocks must be rebuilt
     bool spmv_once = false;          // once-read SpMV (large S)
+    int rec32 = 0;                   // FP32 Jacobians and W/Y records (precision 1)
     DevStruct ds;                    // device tracks, camera-major order, pairs, blocks
 
     // device buffers
     DBuf<float> d_wx, d_wy, d_wxy, d_key, d_s00;
     DBuf<unsigned char> d_flag, d_flagst;
-    DBuf<double> d_X, d_Xn, d_Wo, d_Yo, d_Vinv, d_gp, d_Sd, d_Su, d_b, d_Cinv;
+    DBuf<double> d_X, d_Xn, d_X0, d_Vinv, d_gp, d_Sd, d_b, d_Cinv, d_In_bak;
+    DBuf<unsigned char> d_Wrec, d_Yrec;  // element type by precision
+    DBuf<double> d_Su;
     DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_rowdot, d_partial, d_scal, d_Dd, d_ppart, d_yN, d_yT;
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
@@ -215,15 +218,29 @@ struct rpba_ctx {
 // set-up
 // ---------------------------------------------------------------------------
 
+static int restore_points(rpba_ctx *c);
+
 // tracks, observation values and original weights of the initial block, and
 // the accepted points (LSA construction, rbacore.cpp:37-123)
 static int upload_tracks(rpba_ctx *c)
 {
     int rc = c->ds.upload_tracks(c->s_init, c->io->obs_xy, c->w0_init, c->st);
     if (rc) return rc;
-    rc |= c->d_X.put(c->X_init.data(), c->X_init.size(), c->st);
-    rc |= c->d_Xn.put(c->X_init.data(), c->X_init.size(), c->st);
+    rc |= c->d_X0.put(c->X_init.data(), c->X_init.size(), c->st);
+    rc |= c->d_X.alloc(c->X_init.size());
+    rc |= c->d_Xn.alloc(c->X_init.size());
     if (rc) return RPBA_E_NOMEM;
+    return restore_points(c);
+}
+
+// accepted and trial points <- the initial points (device-to-device)
+static int restore_points(rpba_ctx *c)
+{
+    const size_t n = c->X_init.size();
+    if (n) {
+        HIPCHK(hipMemcpyAsync(c->d_X.p, c->d_X0.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->d_Xn.p, c->d_X0.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+    }
     HIPCHK(hipStreamSynchronize(c->st));
     c->structure_dirty = true;
     return RPBA_OK;
@@ -245,11 +262,12 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_key.alloc(ntr);
     rc |= c->d_flag.alloc(ntr);
     rc |= c->d_flagst.alloc(ntr);
-    rc |= c->d_Wo.alloc((size_t)kRec * ntr);
-    rc |= c->d_Yo.alloc((size_t)kRec * ntr);
+    const size_t esz = c->rec32 ? sizeof(float) : sizeof(double);
+    rc |= c->d_Wrec.alloc((size_t)kRec * esz * ntr);
+    rc |= c->d_Yrec.alloc((size_t)kRec * esz * ntr);
     rc |= c->d_Vinv.alloc(9 * np);
     rc |= c->d_gp.alloc(3 * np);
-    rc |= c->d_Su.alloc(81 * (size_t)c->ds.nblk);
+    rc |= c->d_Su.alloc(81 * (size_t)std::max(c->ds.nblk, 1));
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
     // S beyond the last-level cache: read every block once (RPBA_SPMV=1/2 forces a variant)
     {
@@ -371,7 +389,8 @@ static LinArgs lin_args(rpba_ctx *c)
     a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.tpt = c->ds.tpt; a.camoff = c->ds.camoff; a.camq = c->ds.camq;
     a.ox = c->ds.ox; a.oy = c->ds.oy; a.wx = c->d_wx.p; a.wy = c->d_wy.p; a.wxy = c->d_wxy.p;
     a.X = c->d_X.p;
-    a.Wo = c->d_Wo.p; a.Yo = c->d_Yo.p; a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.Sd = c->d_Sd.p; a.bvec = c->d_b.p;
+    a.Wrec = c->d_Wrec.p; a.Yrec = c->d_Yrec.p; a.rec32 = c->rec32;
+    a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.Sd = c->d_Sd.p; a.bvec = c->d_b.p;
     a.Dd = c->d_Dd.p; a.nch = c->ds.nch; a.ch_start = c->ds.ch_start; a.ch_off = c->ds.ch_off;
     return a;
 }
@@ -395,7 +414,8 @@ static StepArgs step_args(rpba_ctx *c, const double *sol)
     StepArgs a;
     a.nc = c->nc; a.npts = c->ds.npts; a.cam = c->cam();
     a.toff = c->ds.toff; a.tcam = c->ds.tcam;
-    a.Wo = c->d_Wo.p; a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.sol = sol; a.X = c->d_X.p; a.Xn = c->d_Xn.p;
+    a.Wrec = c->d_Wrec.p; a.rec32 = c->rec32;
+    a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.sol = sol; a.X = c->d_X.p; a.Xn = c->d_Xn.p;
     a.S = c->d_pcg.p;
     return a;
 }
@@ -411,23 +431,33 @@ static CostArgs cost_args(rpba_ctx *c)
     return a;
 }
 
-// trial cost at (trial cameras, Xn); sets the float keys (srmvflag = true)
-static int trial_cost(rpba_ctx *c, double &cost)
+// trial cost at (trial cameras, Xn) into d_scal[0]; sets the float keys
+// (srmvflag = true)
+static void launch_trial_cost(rpba_ctx *c)
 {
     CostArgs a = cost_args(c);
     c->timed("cost", [&] { launch_cost(a, c->d_scal.p, c->st); });
+}
+
+static int trial_cost(rpba_ctx *c, double &cost)
+{
+    launch_trial_cost(c);
     HIPCHK(hipMemcpyAsync(&cost, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return RPBA_OK;
 }
 
-// CalcBundleSep (rbacore.cpp:1094-1195) + theta_new = theta - x + decode
+// CalcBundleSep (rbacore.cpp:1094-1195) + theta_new = theta - x + decode.
+// The NaN / inf check of the step (rbacore.cpp:541-546) is raised on the
+// device by k_backsub (points) and k_decode (the float camera step); the
+// caller reads it with the trial cost and, when it is set, restores the trial
+// intrinsics saved here, since the reference returns before decoding.
 static int bundle_step(rpba_ctx *c)
 {
     LinArgs la = lin_args(c);
     SchurArgs sa;
     sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
-    sa.Yo = c->d_Yo.p; sa.Wo = c->d_Wo.p; sa.Su = c->d_Su.p;
+    sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.rec32 = c->rec32;
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
     c->timed("schur_pairs", [&] { launch_schur_pairs(sa, c->st); });
     c->timed("schur_diag", [&] { launch_schur_diag(la, c->st); });
@@ -436,9 +466,11 @@ static int bundle_step(rpba_ctx *c)
     PcgArgs pa = pcg_args(c);
     c->timed("pcg_init", [&] { launch_pcg_init(pa, c->st); });
     // launch j reads the scalars of slot j & 1 and writes slot (j + 1) & 1;
-    // once the stopping rule fired, later launches only pass the state on
+    // once the stopping rule fired, later launches only pass the state on.
+    // The first chunk is sized from the previous solve's iteration count.
     PcgScalars hs{};
-    int chunk = 8, launched = 0;
+    // (with kernel timing on, one iteration per poll: only working launches are timed)
+    int chunk = c->timing ? 1 : std::max(4, c->pcg_last + 2), launched = 0;
     for (;;) {
         for (int j = 0; j < chunk; ++j) {
             pa.j = launched + j;
@@ -450,25 +482,14 @@ static int bundle_step(rpba_ctx *c)
         HIPCHK(hipStreamSynchronize(c->st));
         if (hs.done) break;
         if (launched > pa.msize + 8) return RPBA_E_HIP;  // cannot happen: the kernel stops at msize
-        chunk = std::min(chunk * 2, 64);
+        if (!c->timing) chunk = std::min(std::max(chunk, 8) * 2, 64);
     }
     c->pcg_last = hs.iters;
     c->pcg_total += hs.iters;
     StepArgs sa2 = step_args(c, c->d_best.p);
     HIPCHK(hipMemsetAsync(&c->d_pcg.p->nan_flag, 0, sizeof(int), c->st));
     c->timed("backsub", [&] { launch_backsub(sa2, c->st); });
-    // NaN / inf check of the whole step before anything is decoded
-    // (rbacore.cpp:541-546): camera part checked in double as in the reference
-    int nanf = 0;
-    HIPCHK(hipMemcpyAsync(&nanf, &c->d_pcg.p->nan_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    std::vector<double> xc(9 * (size_t)c->nc);
-    HIPCHK(hipMemcpyAsync(xc.data(), c->d_best.p, sizeof(double) * xc.size(), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    for (double v : xc) {
-        const float f = (float)v;
-        if (std::isnan(f) || std::isinf(f)) nanf = 1;
-    }
-    if (nanf) return RPBA_E_NAN_RESULT;
+    HIPCHK(hipMemcpyAsync(c->d_In_bak.p, c->c_In.p, sizeof(double) * 3 * (size_t)c->nc, hipMemcpyDeviceToDevice, c->st));
     c->timed("decode", [&] { launch_decode(sa2, c->st); });
     return RPBA_OK;
 }
@@ -587,7 +608,8 @@ static int pass_end(rpba_ctx *c)
 
 static int lm_iteration(rpba_ctx *c)
 {
-    if (c->it == 0 && c->fl.robust) {
+    const bool rw = c->it == 0 && c->fl.robust;
+    if (rw) {
         c->nrunknowns = c->nrunknownsgiven;
         c->nrobs = c->nrobsgiven;
         RobustArgs ra;
@@ -596,26 +618,31 @@ static int lm_iteration(rpba_ctx *c)
         ra.key = c->d_key.p; ra.w0x = c->ds.w0x; ra.w0y = c->ds.w0y; ra.w0xy = c->ds.w0xy;
         ra.s00 = c->d_s00.p; ra.wx = c->d_wx.p; ra.wy = c->d_wy.p; ra.wxy = c->d_wxy.p;
         ra.flag = c->d_flag.p; ra.partial = c->d_partial.p; ra.ipartial = c->d_ipartial.p;
-        c->timed("reweight", [&] { launch_reweight(ra, c->d_scal.p, c->d_iscal.p, c->st); });
-        double ws = 0.;
-        int nflag = 0;
-        HIPCHK(hipMemcpyAsync(&ws, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipMemcpyAsync(&nflag, c->d_iscal.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
-        c->nrobs -= 2 * nflag;
-        c->redundancy = c->nrobs - c->nrunknowns;
-        c->wsum = (double)(2 * c->imagepointnr) / ws;
-        c->wsumg = c->wsum;
+        c->timed("reweight", [&] { launch_reweight(ra, c->d_scal.p + 1, c->d_iscal.p, c->st); });
     }
     int rc = bundle_step(c);
-    if (rc == RPBA_E_NAN_RESULT) {
+    if (rc) return rc;
+    launch_trial_cost(c);
+    // one readback per LM iteration: cost, reweighting sums, NaN flag
+    double sc[2] = {0., 0.};
+    int nflag = 0, nanf = 0;
+    HIPCHK(hipMemcpyAsync(sc, c->d_scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&nflag, c->d_iscal.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&nanf, &c->d_pcg.p->nan_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (nanf) {
+        HIPCHK(hipMemcpyAsync(c->c_In.p, c->d_In_bak.p, sizeof(double) * 3 * (size_t)c->nc, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
         c->logs("BA-Error: NANs in result vector\n");
-        return rc;
+        return RPBA_E_NAN_RESULT;
     }
-    if (rc) return rc;
-    double cost = 0.;
-    rc = trial_cost(c, cost);
-    if (rc) return rc;
+    if (rw) {
+        c->nrobs -= 2 * nflag;
+        c->redundancy = c->nrobs - c->nrunknowns;
+        c->wsum = (double)(2 * c->imagepointnr) / sc[1];
+        c->wsumg = c->wsum;
+    }
+    const double cost = sc[0];
     c->sig0 = cost * c->wsumg / (double)c->redundancy;
     if (c->sig0old < c->sig0 * c->frac) --c->iflag;
     int acc = 0;
@@ -648,6 +675,8 @@ static int ctx_init(rpba_ctx *c)
     rpba_block *io = c->io;
     const int nc = io->n_cams;
     c->nc = nc;
+    // precision 1: FP32 Jacobians and W/Y records, FP64 sums, S, PCG and cost
+    c->rec32 = c->fl.precision >= 1;
     c->addf[0] = c->fl.est_k2 != 0;
     c->addf[1] = c->fl.est_k4 != 0;
     c->addf[2] = c->fl.est_f != 0;
@@ -735,6 +764,7 @@ static int ctx_init(rpba_ctx *c)
     rc |= c->d_scal.alloc(4);
     rc |= c->d_iscal.alloc(4);
     rc |= c->d_pcg.alloc(2);
+    rc |= c->d_In_bak.alloc(3 * (size_t)nc);
     rc |= c->d_ppart.alloc(3 * (size_t)pcg_groups(nc));
     rc |= c->c_Rb.alloc(n9);
     rc |= c->c_Cb.alloc(3 * (size_t)nc);
@@ -816,7 +846,9 @@ extern "C" int rpba_ctx_reset(rpba_ctx *c)
     c->Pst = c->Pst_init;
     c->Ist = c->Ist_init;
     reset_state(c);
-    const int rc = upload_tracks(c);
+    int rc = c->ds.restore_tracks(c->st);
+    if (rc) return rc;
+    rc = restore_points(c);
     if (rc) return rc;
     return upload_initial_cameras(c);
 }
@@ -840,6 +872,9 @@ extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
     else
         P = c->Pst;
     HIPCHK(hipMemcpyAsync(In.data(), c->c_In.p, sizeof(double) * In.size(), hipMemcpyDeviceToHost, c->st));
+    std::vector<double> Is = c->Ist;  // stored intrinsics (the state the reported RMS belongs to)
+    if (c->c_Ist.p && c->c_Ist.n >= Is.size())
+        HIPCHK(hipMemcpyAsync(Is.data(), c->c_Ist.p, sizeof(double) * Is.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (io->obs_kept) std::memset(io->obs_kept, 0, (size_t)io->n_obs);
     if (io->pt_kept) std::memset(io->pt_kept, 0, (size_t)io->n_pts);
@@ -858,7 +893,8 @@ extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
         io->cam_f[k] = (float)In[3 * (size_t)k + 2];
     }
     if (st) {
-        // inlier RMS of the returned solution, double
+        // RMS over the kept observations of the stored state (stored R|t, X
+        // and intrinsics), double
         double ss = 0.;
         int64_t n = 0;
         for (int i = 0; i < npts; ++i)
@@ -870,9 +906,9 @@ extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
                 const double e1 = Pk[4] * x[0] + Pk[5] * x[1] + Pk[6] * x[2] + Pk[7];
                 const double e2 = Pk[8] * x[0] + Pk[9] * x[1] + Pk[10] * x[2] + Pk[11];
                 const double u = e0 / e2, v = e1 / e2, rho = u * u + v * v;
-                const double sc = 1. + In[3 * (size_t)k] * rho + In[3 * (size_t)k + 1] * rho * rho;
-                const double r0 = io->obs_xy[2 * (size_t)fo] - In[3 * (size_t)k + 2] * sc * u;
-                const double r1 = io->obs_xy[2 * (size_t)fo + 1] - In[3 * (size_t)k + 2] * sc * v;
+                const double sc = 1. + Is[3 * (size_t)k] * rho + Is[3 * (size_t)k + 1] * rho * rho;
+                const double r0 = io->obs_xy[2 * (size_t)fo] - Is[3 * (size_t)k + 2] * sc * u;
+                const double r1 = io->obs_xy[2 * (size_t)fo + 1] - Is[3 * (size_t)k + 2] * sc * v;
                 ss += r0 * r0 + r1 * r1;
                 n += 1;
             }
@@ -923,6 +959,15 @@ extern "C" int rpba_ctx_kernel_time(rpba_ctx *c, const char *kernel, double *tot
     return RPBA_OK;
 }
 
+extern "C" int rpba_ctx_progress(rpba_ctx *c, int32_t *pass, int32_t *itt, double *sig0_stored)
+{
+    if (!c) return RPBA_E_INVALID;
+    if (pass) *pass = c->it;
+    if (itt) *itt = c->itt;
+    if (sig0_stored) *sig0_stored = c->sig0old;
+    return c->status;
+}
+
 extern "C" int rpba_ctx_sizes(rpba_ctx *c, int64_t out[6])
 {
     if (!c || !out) return RPBA_E_INVALID;
@@ -947,7 +992,7 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->ds.release();
     c->d_wx.release(); c->d_wy.release(); c->d_wxy.release(); c->d_key.release(); c->d_s00.release();
     c->d_flag.release(); c->d_flagst.release();
-    c->d_X.release(); c->d_Xn.release(); c->d_Wo.release(); c->d_Yo.release(); c->d_Vinv.release(); c->d_gp.release();
+    c->d_X.release(); c->d_Xn.release(); c->d_X0.release(); c->d_In_bak.release(); c->d_Wrec.release(); c->d_Yrec.release(); c->d_Vinv.release(); c->d_gp.release();
     c->d_Sd.release(); c->d_Su.release(); c->d_b.release(); c->d_Cinv.release();
     c->d_x.release(); c->d_best.release(); c->d_r.release(); c->d_d.release(); c->d_Ad.release();
     c->d_h.release(); c->d_h1.release(); c->d_rowdot.release(); c->d_partial.release(); c->d_scal.release();

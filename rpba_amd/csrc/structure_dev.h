@@ -54,7 +54,12 @@ struct DevStruct {
     unsigned char *tmp = nullptr;
     size_t tmp_cap = 0;
 
+    Tracks init;           // snapshot of the initial tracks (restore_tracks)
+    int npts0 = 0, ntr0 = 0;
+
     int scratch(size_t bytes);
+    // working tracks <- snapshot (device-to-device), sizes back to the initial ones
+    int restore_tracks(hipStream_t st);
     // tracks of the host structure (after PointXXvIn) plus the gathered
     // observation values and weights
     int upload_tracks(const Structure &s, const float *obs_xy, const std::vector<float> &w0, hipStream_t st);

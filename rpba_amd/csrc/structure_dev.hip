@@ -408,7 +408,40 @@ int DevStruct::upload_tracks(const Structure &s, const float *obs_xy, const std:
     DCHK(hipMemcpyAsync(w0x, a.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
     DCHK(hipMemcpyAsync(w0y, b.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
     DCHK(hipMemcpyAsync(w0xy, c.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    // snapshot for restore_tracks
+    npts0 = npts;
+    ntr0 = ntr;
+    if (!init.toff && init.alloc(sizes_cap_pts, sizes_cap_tr)) return RPBA_E_NOMEM;
+    const size_t np = (size_t)npts, nt = (size_t)ntr;
+    DCHK(hipMemcpyAsync(init.toff, toff, sizeof(int) * (np + 1), hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(init.nrpv, nrpv, sizeof(int) * np, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(init.tcam, tcam, sizeof(int) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(init.tpt, tpt, sizeof(int) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(init.tfo, tfo, sizeof(int) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(init.ox, ox, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(init.oy, oy, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(init.w0x, w0x, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(init.w0y, w0y, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(init.w0xy, w0xy, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
     DCHK(hipStreamSynchronize(st));
+    return RPBA_OK;
+}
+
+int DevStruct::restore_tracks(hipStream_t st)
+{
+    npts = npts0;
+    ntr = ntr0;
+    const size_t np = (size_t)npts, nt = (size_t)ntr;
+    DCHK(hipMemcpyAsync(toff, init.toff, sizeof(int) * (np + 1), hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(nrpv, init.nrpv, sizeof(int) * np, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(tcam, init.tcam, sizeof(int) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(tpt, init.tpt, sizeof(int) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(tfo, init.tfo, sizeof(int) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(ox, init.ox, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(oy, init.oy, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(w0x, init.w0x, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(w0y, init.w0y, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    DCHK(hipMemcpyAsync(w0xy, init.w0xy, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
     return RPBA_OK;
 }
 
@@ -429,6 +462,7 @@ int DevStruct::download_tracks(std::vector<int> &toff_h, std::vector<int> &nrpv_
 
 void DevStruct::release()
 {
+    init.release();
     Tracks t;
     t.toff = toff; t.nrpv = nrpv; t.tcam = tcam; t.tpt = tpt; t.tfo = tfo;
     t.ox = ox; t.oy = oy; t.w0x = w0x; t.w0y = w0y; t.w0xy = w0xy; t.X = nullptr;
