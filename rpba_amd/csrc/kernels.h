@@ -51,7 +51,7 @@ struct SchurArgs {
 struct PcgArgs {
     int nc, msize;
     const double *Sd, *Su, *Cinv, *b;
-    const int *row_off, *col_off, *col_blk, *blk_k, *blk_l;
+    const int *row_off, *col_off, *col_blk, *col_pos, *blk_k, *blk_l;
     double *x, *best, *r, *d, *Ad, *h, *h1, *rowdot, *part;
     double *yN, *yT;  // once-read SpMV: per upper block S_kl d_l and S_kl^T d_k (9 each)
     int nblk, once_read;

@@ -271,6 +271,13 @@ __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) Yo[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
         }
+        // write the padding too: every cache line of the record is then fully
+        // written, and the L2 need not merge partial lines
+#pragma unroll
+        for (int j = 27; j < kRec; ++j) {
+            Wo[j] = (R)0;
+            Yo[j] = (R)0;
+        }
     }
 }
 
@@ -603,35 +610,101 @@ __global__ __launch_bounds__(256) void k_spmv(PcgArgs a)
     }
 }
 
-// Ad = S d reading every stored block once (used when S is large):
-// k_spmv_blocks: 56 upper blocks per workgroup (two per group of 9 lanes):
-// yN[b] = S_kl d_l (row k's share) and yT[b] = S_kl^T d_k (row l's share);
-// k_spmv_gather: one workgroup per row k adds S_kk d_k, the yN of its row
-// blocks and the yT of its column blocks in a fixed order, and writes the
-// row's share of d.Ad.
-__global__ __launch_bounds__(256) void k_spmv_blocks(PcgArgs a)
+// Ad = S d reading every stored block once.
+// k_spmv_tiles: one workgroup per tile of kTile consecutive upper blocks.  The
+// tile is staged through LDS with 16-byte coalesced loads; group g of 9 lanes
+// then forms S_kl^T d_k (row l's share), stored at the block's position in
+// column order so each row reads its column shares contiguously, and S_kl d_l,
+// whose run of blocks of the same row k inside the tile is summed in block
+// order into yN[first block of the run].  Launch j scans the tiles backwards
+// when j is odd, so the blocks read last by one launch (still held by the
+// memory-side cache when S is not much larger than it) are read first by the
+// next.
+// k_spmv_gather: one workgroup per row k adds S_kk d_k, the run sums of its
+// row and the yT of its column blocks in a fixed order, and writes the row's
+// share of d.Ad.
+constexpr int kTile = 28;
+
+// the next tile's S elements and block indices go to registers while the
+// current tile is computed from LDS
+#define SPMV_TILE_FETCH(ii)                                                           \
+    {                                                                                 \
+        const int tile_ = rev ? ntile - 1 - (ii) : (ii);                              \
+        const int b0_ = tile_ * kTile, nb_ = min(kTile, a.nblk - b0_);                \
+        const double2 *src_ = (const double2 *)(a.Su + 81 * (size_t)b0_);             \
+        const int n_ = nb_ * 81, n2_ = n_ / 2;                                        \
+        if (t < n2_) v0 = src_[t];                                                    \
+        if (t + 256 < n2_) v1 = src_[t + 256];                                        \
+        if (t + 512 < n2_) v2 = src_[t + 512];                                        \
+        if (t + 768 < n2_) v3 = src_[t + 768];                                        \
+        if (t + 1024 < n2_) v4 = src_[t + 1024];                                      \
+        if ((n_ & 1) && t == 0) vl = a.Su[81 * (size_t)b0_ + n_ - 1];                 \
+        if (t < nb_) {                                                                \
+            pk = a.blk_k[b0_ + t];                                                    \
+            pl = a.blk_l[b0_ + t];                                                    \
+        }                                                                             \
+    }
+
+// Persistent: workgroup w takes tiles w, w + gridDim.x, ... (grid = the
+// workgroups the chip holds at once, four per CU).
+__global__ __launch_bounds__(256) void k_spmv_tiles(PcgArgs a)
 {
     if (a.S[a.j & 1].done) return;
+    const int ntile = (a.nblk + kTile - 1) / kTile;
+    const bool rev = (a.j & 1) != 0;
     const int t = threadIdx.x;
     const int g = t / 9, r = t % 9;
-    if (g >= 28) return;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int bb = blockIdx.x * 56 + h * 28 + g;
-        if (bb >= a.nblk) return;
-        const double *B = a.Su + 81 * (size_t)bb;
-        const double *dl = a.d + 9 * (size_t)a.blk_l[bb];
-        const double *dk = a.d + 9 * (size_t)a.blk_k[bb];
-        double yn = 0., yt = 0.;
-#pragma unroll
-        for (int cc = 0; cc < 9; ++cc) {
-            yn += B[r * 9 + cc] * dl[cc];
-            yt += B[cc * 9 + r] * dk[cc];
+    // kTile * 81 doubles: a multiple of 16 bytes, so every tile starts 16-byte aligned
+    __shared__ double2 sb[kTile * 81 / 2];
+    __shared__ double yn[kTile][9];
+    __shared__ int rowk[kTile], coll[kTile];
+    double2 v0 = {0., 0.}, v1 = v0, v2 = v0, v3 = v0, v4 = v0;
+    double vl = 0.;
+    int pk = 0, pl = 0;
+    int i = blockIdx.x;
+    if (i < ntile) SPMV_TILE_FETCH(i);
+    for (; i < ntile; i += gridDim.x) {
+        const int tile = rev ? ntile - 1 - i : i;
+        const int b0 = tile * kTile, nb = min(kTile, a.nblk - b0);
+        const int n = nb * 81, n2 = n / 2;
+        if (t < n2) sb[t] = v0;
+        if (t + 256 < n2) sb[t + 256] = v1;
+        if (t + 512 < n2) sb[t + 512] = v2;
+        if (t + 768 < n2) sb[t + 768] = v3;
+        if (t + 1024 < n2) sb[t + 1024] = v4;
+        if ((n & 1) && t == 0) ((double *)sb)[n - 1] = vl;
+        if (t < nb) {
+            rowk[t] = pk;
+            coll[t] = pl;
         }
-        a.yN[9 * (size_t)bb + r] = yn;
-        a.yT[9 * (size_t)bb + r] = yt;
+        __syncthreads();
+        if (i + (int)gridDim.x < ntile) SPMV_TILE_FETCH(i + (int)gridDim.x);
+        if (g < nb) {
+            const double *B = (const double *)sb + 81 * g;
+            const double *dl = a.d + 9 * (size_t)coll[g];
+            const double *dk = a.d + 9 * (size_t)rowk[g];
+            double yv = 0., tv = 0.;
+#pragma unroll
+            for (int cc = 0; cc < 9; ++cc) {
+                yv += B[r * 9 + cc] * dl[cc];
+                tv += B[cc * 9 + r] * dk[cc];
+            }
+            yn[g][r] = yv;
+            a.yT[9 * (size_t)a.col_pos[b0 + g] + r] = tv;
+        }
+        __syncthreads();
+        if (g < nb && (g == 0 || rowk[g] != rowk[g - 1])) {
+            // the run of row rowk[g] in block order (blocks of one row are adjacent)
+            const int rk = rowk[g];
+            double sum = 0.;
+#pragma unroll
+            for (int h = 0; h < kTile; ++h) sum += (h >= g && h < nb && rowk[h] == rk) ? yn[h][r] : 0.;
+            a.yN[9 * (size_t)(b0 + g) + r] = sum;
+        }
+        __syncthreads();
     }
 }
+#undef SPMV_TILE_FETCH
 
 __global__ __launch_bounds__(256) void k_spmv_gather(PcgArgs a)
 {
@@ -640,8 +713,10 @@ __global__ __launch_bounds__(256) void k_spmv_gather(PcgArgs a)
     const int t = threadIdx.x;
     const int g = t / 9, r = t % 9;
     __shared__ double red[28][9];
-    const int r0 = a.row_off[k], nu = a.row_off[k + 1] - r0;
+    const int r0 = a.row_off[k], r1 = a.row_off[k + 1];
     const int c0 = a.col_off[k], nl = a.col_off[k + 1] - c0;
+    // runs of row k: one per tile that holds blocks of the row
+    const int t0 = r0 / kTile, nrun = (r1 > r0) ? (r1 - 1) / kTile - t0 + 1 : 0;
     double acc = 0.;
     if (g < 28) {
         if (g == 0) {
@@ -650,8 +725,22 @@ __global__ __launch_bounds__(256) void k_spmv_gather(PcgArgs a)
 #pragma unroll
             for (int cc = 0; cc < 9; ++cc) acc += B[cc] * dk[cc];
         }
-        for (int it = g; it < nu; it += 28) acc += a.yN[9 * (size_t)(r0 + it) + r];
-        for (int it = g; it < nl; it += 28) acc += a.yT[9 * (size_t)a.col_blk[c0 + it] + r];
+        for (int it = g; it < nrun; it += 28) {
+            const int s0 = max(r0, (t0 + it) * kTile);
+            acc += a.yN[9 * (size_t)s0 + r];
+        }
+        // column shares are contiguous; four loads in flight, added in order
+        const double *yc = a.yT + 9 * (size_t)c0 + r;
+        int it = g;
+        for (; it + 3 * 28 < nl; it += 4 * 28) {
+            const double v0 = yc[9 * (size_t)it], v1 = yc[9 * (size_t)(it + 28)];
+            const double v2 = yc[9 * (size_t)(it + 56)], v3 = yc[9 * (size_t)(it + 84)];
+            acc += v0;
+            acc += v1;
+            acc += v2;
+            acc += v3;
+        }
+        for (; it < nl; it += 28) acc += yc[9 * (size_t)it];
         red[g][r] = acc;
     }
     __syncthreads();
@@ -1171,10 +1260,19 @@ void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
     k_bj_inverse<<<nblk(nc, 64), 64, 0, st>>>(nc, Sd, Cinv);
 }
 void launch_pcg_init(const PcgArgs &a, hipStream_t st) { k_pcg_init<<<1, 1024, 0, st>>>(a); }
+// compute units of the current device (persistent SpMV grid)
+static int spmv_cus()
+{
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        n = 256;
+    return n;
+}
 void launch_spmv(const PcgArgs &a, hipStream_t st)
 {
     if (a.once_read && a.nblk > 0) {
-        k_spmv_blocks<<<(a.nblk + 55) / 56, 256, 0, st>>>(a);
+        const int ntile = (a.nblk + kTile - 1) / kTile;
+        k_spmv_tiles<<<std::min(ntile, 4 * spmv_cus()), 256, 0, st>>>(a);
         k_spmv_gather<<<a.nc, 256, 0, st>>>(a);
     } else {
         k_spmv<<<a.nc, 256, 0, st>>>(a);

@@ -126,7 +126,7 @@ struct rpba_ctx {
     std::vector<double> Pst, Ist;    // stored cameras (host copy at pass boundaries)
     std::vector<double> Pst_init, Ist_init, R0, C0;
     bool structure_dirty = true;     // pairs/blocks must be rebuilt
-    bool spmv_once = false;          // once-read SpMV (large S)
+    int spmv_once = 1;               // 1: once-read SpMV (tiles + gather); 0: per-row double read
     int rec32 = 0;                   // FP32 Jacobians and W/Y records (precision 1)
     DevStruct ds;                    // device tracks, camera-major order, pairs, blocks
 
@@ -269,11 +269,11 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_gp.alloc(3 * np);
     rc |= c->d_Su.alloc(81 * (size_t)std::max(c->ds.nblk, 1));
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
-    // S beyond the last-level cache: read every block once (RPBA_SPMV=1/2 forces a variant)
+    // SpMV reads every stored block once (tiles + gather); RPBA_SPMV=2 selects
+    // the per-row kernel that reads each off-diagonal block twice
     {
         const char *ev = std::getenv("RPBA_SPMV");
-        const size_t sbytes = 648 * (size_t)c->ds.nblk;
-        c->spmv_once = ev ? (ev[0] == '1') : (sbytes > ((size_t)384 << 20));
+        c->spmv_once = (ev && ev[0] == '2') ? 0 : 1;
     }
     if (c->spmv_once) {
         rc |= c->d_yN.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
@@ -401,7 +401,7 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.nc = c->nc;
     a.msize = std::max(1000, c->nrioff);  // rbacore.cpp:1848-1849
     a.Sd = c->d_Sd.p; a.Su = c->d_Su.p; a.Cinv = c->d_Cinv.p; a.b = c->d_b.p;
-    a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
+    a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.col_pos = c->ds.cpos; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
     a.x = c->d_x.p; a.best = c->d_best.p; a.r = c->d_r.p; a.d = c->d_d.p; a.Ad = c->d_Ad.p;
     a.h = c->d_h.p; a.h1 = c->d_h1.p; a.rowdot = c->d_rowdot.p; a.part = c->d_ppart.p; a.S = c->d_pcg.p;
     a.yN = c->d_yN.p; a.yT = c->d_yT.p; a.nblk = c->ds.nblk; a.once_read = c->spmv_once;

@@ -116,6 +116,12 @@ __global__ void k_split_blocks(int nb, unsigned nc, const unsigned *ukey, int *b
     bid[b] = b;
 }
 
+__global__ void k_invert(int n, const int *perm, int *inv)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) inv[perm[j]] = j;
+}
+
 __global__ void k_set_last(int *a, int idx, int v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) a[idx] = v;
@@ -216,6 +222,7 @@ int DevStruct::build(hipStream_t st, int *first_empty)
     rc |= grow(roff, roff_cap, (size_t)nc + 1);
     rc |= grow(coff, coff_cap, (size_t)nc + 1);
     rc |= grow(cblk, cblk_cap, (size_t)nblk);
+    rc |= grow(cpos, cpos_cap, (size_t)nblk);
     rc |= grow(key2, key2_cap, (size_t)nblk);
     rc |= grow(key2s, key2s_cap, (size_t)nblk);
     rc |= grow(bid, bid_cap, (size_t)nblk);
@@ -236,6 +243,7 @@ int DevStruct::build(hipStream_t st, int *first_empty)
         DCHK(rocprim::radix_sort_pairs(tmp, bytes, key2, key2s, bid, cblk, nblk, 0u, kb, st));
     }
     k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, (unsigned)nc, key2s, nblk, coff);
+    if (nblk > 0) k_invert<<<nb_(nblk, 256), 256, 0, st>>>(nblk, cblk, cpos);
     DCHK(hipStreamSynchronize(st));
     return RPBA_OK;
 }
@@ -477,7 +485,7 @@ void DevStruct::release()
     fr(camoff, camoff_cap); fr(camq, camq_cap); fr(ch_start, ch_start_cap); fr(ch_off, ch_off_cap); fr(u32a, u32a_cap); fr(i32a, i32a_cap); fr(i32b, i32b_cap);
     fr(ppoff, ppoff_cap); fr(key_in, key_in_cap); fr(key_out, key_out_cap); fr(val_in, val_in_cap);
     fr(pab, pab_cap); fr(ukey, ukey_cap); fr(ucnt, ucnt_cap); fr(nruns, nruns_cap); fr(poff, poff_cap);
-    fr(bk, bk_cap); fr(bl, bl_cap); fr(roff, roff_cap); fr(coff, coff_cap); fr(cblk, cblk_cap);
+    fr(bk, bk_cap); fr(bl, bl_cap); fr(roff, roff_cap); fr(coff, coff_cap); fr(cblk, cblk_cap); fr(cpos, cpos_cap);
     fr(key2, key2_cap); fr(key2s, key2s_cap); fr(bid, bid_cap); fr(newi, newi_cap); fr(newoff, newoff_cap);
     fr(tmp, tmp_cap);
 }
