@@ -319,11 +319,14 @@ __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
         for (int j = 0; j < np; ++j) {
             const R *Y = st + j * 54, *W = st + j * 54 + 27;
             const double *g = sg + j * 3;
-            acc0 += (double)(Y[3 * r0] * W[3 * c0] + Y[3 * r0 + 1] * W[3 * c0 + 1] + Y[3 * r0 + 2] * W[3 * c0 + 2]);
+            // products in FP64 (exact for FP32 records), as the off-diagonal blocks form them
+            acc0 += (double)Y[3 * r0] * (double)W[3 * c0] + (double)Y[3 * r0 + 1] * (double)W[3 * c0 + 1] +
+                    (double)Y[3 * r0 + 2] * (double)W[3 * c0 + 2];
             if (g1)
                 acc1 += (double)Y[3 * r1] * g[0] + (double)Y[3 * r1 + 1] * g[1] + (double)Y[3 * r1 + 2] * g[2];
             else
-                acc1 += (double)(Y[3 * r1] * W[3 * c1] + Y[3 * r1 + 1] * W[3 * c1 + 1] + Y[3 * r1 + 2] * W[3 * c1 + 2]);
+                acc1 += (double)Y[3 * r1] * (double)W[3 * c1] + (double)Y[3 * r1 + 1] * (double)W[3 * c1 + 1] +
+                        (double)Y[3 * r1 + 2] * (double)W[3 * c1 + 2];
         }
         __syncthreads();
     }
@@ -419,11 +422,19 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Schur off-diagonal blocks, atomic-free: one wavefront per upper block (k,l)
-// walks its covisibility pairs (grouped by block, point order inside) and
-// accumulates S_kl = -sum_pairs Y_a W_b^T.  Pairs are staged through LDS in
-// batches of 8; lane o owns outputs o and o + 64.
+// Schur off-diagonal blocks on the matrix cores, atomic-free: one wavefront per
+// upper block (k, l).  S_kl = -sum_pairs Y_a W_b^T is the product of the
+// 9 x 3n matrix [Y_a1 .. Y_an] and the 3n x 9 matrix [W_b1 .. W_bn]^T, run as
+// v_mfma_f64_16x16x4f64 steps over K = 3n (rows and columns 9..15 are zero
+// padding).  Operand layout of that instruction: A[i][k] in lane i + 16 k,
+// B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4.
+// K index kk = 3 p + c: pair p of the block, component c of the point.
+// FP32 records are widened to FP64 (products of two floats are exact).
 // ---------------------------------------------------------------------------
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kSchurU = 8;  // MFMA steps per load round (32 K values, 10.7 pairs)
 
 template <typename R>
 __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
@@ -435,31 +446,44 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
     const int bid = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (bid >= a.nblk) return;
     const int lane = threadIdx.x;
-    __shared__ R st[8 * 54];
+    const int i16 = lane & 15, q = lane >> 4;
     const R *Yr = (const R *)a.Yrec, *Wr = (const R *)a.Wrec;
     const int pb = a.poff[bid], pe = a.poff[bid + 1];
-    const int o0 = lane, o1 = lane + 64;
-    const int r0 = o0 / 9, c0 = o0 % 9;
-    const int r1 = (o1 < 81) ? o1 / 9 : 0, c1 = (o1 < 81) ? o1 % 9 : 0;
-    double acc0 = 0., acc1 = 0.;
-    for (int p = pb; p < pe; p += 8) {
-        const int np = (pe - p < 8) ? pe - p : 8;
-        for (int x = lane; x < np * 54; x += 64) {
-            const int j = x / 54, idx = x % 54;
-            const unsigned long long ab = a.pab[p + j];
-            st[x] = (idx < 27) ? Yr[(size_t)(ab >> 32) * kRec + idx] : Wr[(size_t)(ab & 0xffffffffull) * kRec + (idx - 27)];
-        }
+    __shared__ unsigned long long sab[64];
+    f64x4 acc = {0., 0., 0., 0.};
+    for (int c0 = pb; c0 < pe; c0 += 64) {
+        const int nc = min(64, pe - c0);
+        if (lane < nc) sab[lane] = a.pab[c0 + lane];
         __syncthreads();
-        for (int j = 0; j < np; ++j) {
-            const R *Y = st + j * 54, *W = st + j * 54 + 27;
-            acc0 += (double)(Y[3 * r0] * W[3 * c0] + Y[3 * r0 + 1] * W[3 * c0 + 1] + Y[3 * r0 + 2] * W[3 * c0 + 2]);
-            acc1 += (double)(Y[3 * r1] * W[3 * c1] + Y[3 * r1 + 1] * W[3 * c1 + 1] + Y[3 * r1 + 2] * W[3 * c1 + 2]);
+        const int kend = 3 * nc;
+        for (int k0 = 0; k0 < kend; k0 += 4 * kSchurU) {
+            double av[kSchurU], bv[kSchurU];
+#pragma unroll
+            for (int u = 0; u < kSchurU; ++u) {
+                const int kk = k0 + 4 * u + q;
+                double x = 0., y = 0.;
+                if (kk < kend && i16 < 9) {
+                    const unsigned long long ab = sab[kk / 3];
+                    const int cc = kk - 3 * (kk / 3);
+                    x = (double)Yr[(size_t)(ab >> 32) * kRec + 3 * i16 + cc];
+                    y = (double)Wr[(size_t)(ab & 0xffffffffull) * kRec + 3 * i16 + cc];
+                }
+                av[u] = x;
+                bv[u] = y;
+            }
+#pragma unroll
+            for (int u = 0; u < kSchurU; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
         }
         __syncthreads();
     }
-    double *S = a.Su + (size_t)bid * 81;
-    S[o0] = -acc0;
-    if (o1 < 81) S[o1] = -acc1;
+    if (i16 < 9) {
+        double *S = a.Su + (size_t)bid * 81;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = q + 4 * v;
+            if (i < 9) S[i * 9 + i16] = -acc[v];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
