@@ -52,9 +52,12 @@ struct PcgArgs {
     int nc, msize;
     const double *Sd, *Su, *Cinv, *b;
     const int *row_off, *col_off, *col_blk, *col_pos, *blk_k, *blk_l;
-    double *x, *best, *r, *d, *Ad, *h, *h1, *rowdot, *part;
-    double *yN, *yT;  // once-read SpMV: per upper block S_kl d_l and S_kl^T d_k (9 each)
-    int nblk, once_read;
+    double *x, *best, *r, *d, *h, *h1;
+    double *Ad;       // S_kk d_k per camera (diagonal products of the SpMV)
+    double *part;     // per camera: |alpha S d|^2, r.(h1 - h), r.h1 shares
+    double *yN, *yT;  // SpMV row-run sums (at the run's first block) and column shares (column order)
+    double *dpart;    // per SpMV workgroup: its share of d.S d
+    int nblk, ndpart; // upper blocks; SpMV workgroups (spmv_grid)
     PcgScalars *S;  // two slots: launch j reads S[j & 1] and writes S[(j + 1) & 1]
     int j;          // launch index inside one PCG solve
 };
@@ -104,6 +107,7 @@ void launch_schur_pairs(const SchurArgs &s, hipStream_t st);
 void launch_schur_diag(const LinArgs &a, hipStream_t st);
 void launch_lin_cameras(const LinArgs &a, hipStream_t st);
 int pcg_groups(int nc);
+int spmv_grid(int nblk);  // workgroups of the SpMV launch (PcgArgs::ndpart)
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st);
 void launch_pcg_init(const PcgArgs &a, hipStream_t st);
 void launch_spmv(const PcgArgs &a, hipStream_t st);

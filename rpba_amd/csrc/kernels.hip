@@ -582,71 +582,19 @@ __global__ __launch_bounds__(1024) void k_pcg_init(PcgArgs a)
     }
 }
 
-// Ad = S d, one workgroup (256) per block row k: 28 groups of 9 lanes walk the
-// row's items (diagonal, upper blocks of row k, upper blocks of column k used
-// transposed); also writes the row's share of d.Ad.
-__global__ __launch_bounds__(256) void k_spmv(PcgArgs a)
-{
-    if (a.S[a.j & 1].done) return;
-    const int k = blockIdx.x;
-    const int t = threadIdx.x;
-    const int g = t / 9, r = t % 9;
-    __shared__ double red[28][9];
-    const int nu = a.row_off[k + 1] - a.row_off[k];
-    const int nl = a.col_off[k + 1] - a.col_off[k];
-    const int nitems = 1 + nu + nl;
-    double acc = 0.;
-    if (g < 28) {
-        for (int it = g; it < nitems; it += 28) {
-            if (it == 0) {
-                const double *B = a.Sd + 81 * (size_t)k;
-                const double *dv = a.d + 9 * (size_t)k;
-#pragma unroll
-                for (int cc = 0; cc < 9; ++cc) acc += B[r * 9 + cc] * dv[cc];
-            } else if (it <= nu) {
-                const int bb = a.row_off[k] + it - 1;
-                const double *B = a.Su + 81 * (size_t)bb;
-                const double *dv = a.d + 9 * (size_t)a.blk_l[bb];
-#pragma unroll
-                for (int cc = 0; cc < 9; ++cc) acc += B[r * 9 + cc] * dv[cc];
-            } else {
-                const int bb = a.col_blk[a.col_off[k] + it - 1 - nu];
-                const double *B = a.Su + 81 * (size_t)bb;
-                const double *dv = a.d + 9 * (size_t)a.blk_k[bb];
-#pragma unroll
-                for (int cc = 0; cc < 9; ++cc) acc += B[cc * 9 + r] * dv[cc];
-            }
-        }
-        red[g][r] = acc;
-    }
-    __syncthreads();
-    if (t < 9) {
-        double s = 0.;
-        for (int gg = 0; gg < 28; ++gg) s += red[gg][t];
-        a.Ad[9 * (size_t)k + t] = s;
-        red[0][t] = s * a.d[9 * (size_t)k + t];
-    }
-    __syncthreads();
-    if (t == 0) {
-        double s = 0.;
-        for (int j = 0; j < 9; ++j) s += red[0][j];
-        a.rowdot[k] = s;
-    }
-}
-
-// Ad = S d reading every stored block once.
-// k_spmv_tiles: one workgroup per tile of kTile consecutive upper blocks.  The
-// tile is staged through LDS with 16-byte coalesced loads; group g of 9 lanes
-// then forms S_kl^T d_k (row l's share), stored at the block's position in
-// column order so each row reads its column shares contiguously, and S_kl d_l,
-// whose run of blocks of the same row k inside the tile is summed in block
-// order into yN[first block of the run].  Launch j scans the tiles backwards
-// when j is odd, so the blocks read last by one launch (still held by the
-// memory-side cache when S is not much larger than it) are read first by the
-// next.
-// k_spmv_gather: one workgroup per row k adds S_kk d_k, the run sums of its
-// row and the yT of its column blocks in a fixed order, and writes the row's
-// share of d.Ad.
+// S d reading every stored block once, in two launches per PCG iteration.
+// k_spmv_tiles: persistent workgroups over tiles of kTile consecutive upper
+// blocks.  A tile is staged through LDS with 16-byte coalesced loads; group g
+// of 9 lanes then forms S_kl^T d_k (row l's share), stored at the block's
+// position in column order so each row reads its column shares contiguously,
+// and S_kl d_l, whose run of blocks of the same row k inside the tile is
+// summed in block order into yN[first block of the run].  Launch j scans the
+// tiles backwards when j is odd, so the blocks read last by one launch (still
+// held by the memory-side cache when S is not much larger than it) are read
+// first by the next.  After its tiles a workgroup forms the diagonal products
+// yD_k = S_kk d_k of cameras k = blockIdx.x, blockIdx.x + gridDim.x, ..., and
+// writes its share of d.S d = sum_k d_k.yD_k + 2 sum_(k<l) d_k.S_kl d_l.
+// k_pcg_a (below) gathers row k of S d from yD, yN and yT.
 constexpr int kTile = 28;
 
 // the next tile's S elements and block indices go to registers while the
@@ -685,6 +633,7 @@ __global__ __launch_bounds__(256) void k_spmv_tiles(PcgArgs a)
     double2 v0 = {0., 0.}, v1 = v0, v2 = v0, v3 = v0, v4 = v0;
     double vl = 0.;
     int pk = 0, pl = 0;
+    double du = 0.;  // this lane's share of sum over upper blocks of d_k.S_kl d_l
     int i = blockIdx.x;
     if (i < ntile) SPMV_TILE_FETCH(i);
     for (; i < ntile; i += gridDim.x) {
@@ -715,6 +664,7 @@ __global__ __launch_bounds__(256) void k_spmv_tiles(PcgArgs a)
             }
             yn[g][r] = yv;
             a.yT[9 * (size_t)a.col_pos[b0 + g] + r] = tv;
+            du += dk[r] * yv;
         }
         __syncthreads();
         if (g < nb && (g == 0 || rowk[g] != rowk[g - 1])) {
@@ -727,28 +677,57 @@ __global__ __launch_bounds__(256) void k_spmv_tiles(PcgArgs a)
         }
         __syncthreads();
     }
+    // diagonal blocks
+    double dd = 0.;
+    for (int k = blockIdx.x; k < a.nc; k += gridDim.x) {
+        if (t < 9) {
+            const double *B = a.Sd + 81 * (size_t)k + 9 * t;
+            const double *dk = a.d + 9 * (size_t)k;
+            double y = 0.;
+#pragma unroll
+            for (int cc = 0; cc < 9; ++cc) y += B[cc] * dk[cc];
+            a.Ad[9 * (size_t)k + t] = y;
+            dd += dk[t] * y;
+        }
+    }
+    __shared__ double sh[4];
+    const double su = block_sum(du, sh);
+    const double sd = block_sum(dd, sh);
+    if (t == 0) a.dpart[blockIdx.x] = 2. * su + sd;
 }
 #undef SPMV_TILE_FETCH
 
-__global__ __launch_bounds__(256) void k_spmv_gather(PcgArgs a)
+// sum of n values in a fixed order; every thread of the block gets it
+__device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
 {
-    if (a.S[a.j & 1].done) return;
+    double s = 0.;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) s += v[j];
+    return block_sum(s, sh);
+}
+
+// PCG step, part 1 (one workgroup per camera k): alpha = rho / d.S d from the
+// tile partials; (S d)_k = yD_k + the run sums of row k + the column shares of
+// column k, in a fixed order; x += alpha d, r -= alpha S d, h1 = C r; the
+// camera's shares of |alpha S d|^2, r.(h1 - h) and r.h1
+__global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
+{
+    __shared__ double sh[4];
+    __shared__ double red[28][9];
+    __shared__ double rs[9], p3[3][9];
+    const PcgScalars S = a.S[a.j & 1];
+    if (S.done) return;
+    const double dAd = sum_fixed(a.dpart, a.ndpart, sh);
+    const double rho = S.rho;
+    const double alpha = (fabs(rho) < 1.e-20 || fabs(rho) > 1.e20) ? 0. : rho / dAd;
     const int k = blockIdx.x;
     const int t = threadIdx.x;
     const int g = t / 9, r = t % 9;
-    __shared__ double red[28][9];
     const int r0 = a.row_off[k], r1 = a.row_off[k + 1];
     const int c0 = a.col_off[k], nl = a.col_off[k + 1] - c0;
     // runs of row k: one per tile that holds blocks of the row
     const int t0 = r0 / kTile, nrun = (r1 > r0) ? (r1 - 1) / kTile - t0 + 1 : 0;
-    double acc = 0.;
     if (g < 28) {
-        if (g == 0) {
-            const double *B = a.Sd + 81 * (size_t)k + 9 * r;
-            const double *dk = a.d + 9 * (size_t)k;
-#pragma unroll
-            for (int cc = 0; cc < 9; ++cc) acc += B[cc] * dk[cc];
-        }
+        double acc = (g == 0) ? a.Ad[9 * (size_t)k + r] : 0.;
         for (int it = g; it < nrun; it += 28) {
             const int s0 = max(r0, (t0 + it) * kTile);
             acc += a.yN[9 * (size_t)s0 + r];
@@ -768,70 +747,32 @@ __global__ __launch_bounds__(256) void k_spmv_gather(PcgArgs a)
         red[g][r] = acc;
     }
     __syncthreads();
+    const size_t j = 9 * (size_t)k + t;
     if (t < 9) {
-        double s = 0.;
-        for (int gg = 0; gg < 28; ++gg) s += red[gg][t];
-        a.Ad[9 * (size_t)k + t] = s;
-        red[0][t] = s * a.d[9 * (size_t)k + t];
-    }
-    __syncthreads();
-    if (t == 0) {
-        double s = 0.;
-        for (int j = 0; j < 9; ++j) s += red[0][j];
-        a.rowdot[k] = s;
-    }
-}
-
-// sum of n values in a fixed order; every thread of the block gets it
-__device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
-{
-    double s = 0.;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) s += v[j];
-    return block_sum(s, sh);
-}
-
-// PCG step, part 1 (one workgroup per 28 cameras): alpha from d.Ad, x += alpha
-// d, r -= alpha Ad, h1 = C r; partial sums of |alpha Ad|^2, r.(h1 - h), r.h1
-__global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
-{
-    __shared__ double sh[4];
-    __shared__ double rs[256];
-    const PcgScalars S = a.S[a.j & 1];
-    if (S.done) return;
-    const double dAd = sum_fixed(a.rowdot, a.nc, sh);
-    const double rho = S.rho;
-    const double alpha = (fabs(rho) < 1.e-20 || fabs(rho) > 1.e20) ? 0. : rho / dAd;
-    const int t = threadIdx.x;
-    const int k = blockIdx.x * 28 + t / 9, r = t % 9;
-    const bool on = (t < 252) && (k < a.nc);
-    const size_t j = 9 * (size_t)k + r;
-    double m = 0., num = 0., rh = 0.;
-    if (on) {
+        double Ad = 0.;
+        for (int gg = 0; gg < 28; ++gg) Ad += red[gg][t];
         a.x[j] += alpha * a.d[j];
-        const double dr = alpha * a.Ad[j];
+        const double dr = alpha * Ad;
         const double rn = a.r[j] - dr;
         a.r[j] = rn;
         rs[t] = rn;
-        m = dr * dr;
+        p3[0][t] = dr * dr;
     }
     __syncthreads();
-    if (on) {
-        const double *C = a.Cinv + 81 * (size_t)k + 9 * r;
-        const double *rv = rs + (t - r);
+    if (t < 9) {
+        const double *C = a.Cinv + 81 * (size_t)k + 9 * t;
         double h1 = 0.;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) h1 += C[c] * rv[c];
+        for (int c = 0; c < 9; ++c) h1 += C[c] * rs[c];
         a.h1[j] = h1;
-        num = rs[t] * (h1 - a.h[j]);
-        rh = rs[t] * h1;
+        p3[1][t] = rs[t] * (h1 - a.h[j]);
+        p3[2][t] = rs[t] * h1;
     }
-    m = block_sum(m, sh);
-    num = block_sum(num, sh);
-    rh = block_sum(rh, sh);
-    if (t == 0) {
-        a.part[3 * blockIdx.x] = m;
-        a.part[3 * blockIdx.x + 1] = num;
-        a.part[3 * blockIdx.x + 2] = rh;
+    __syncthreads();
+    if (t < 3) {
+        double s = 0.;
+        for (int q = 0; q < 9; ++q) s += p3[t][q];
+        a.part[3 * (size_t)k + t] = s;
     }
 }
 
@@ -846,7 +787,7 @@ __global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
         if (blockIdx.x == 0 && threadIdx.x == 0) *Sn = S;
         return;
     }
-    const int G = gridDim.x;
+    const int G = a.nc;
     double m = 0., num = 0., rh = 0.;
     for (int g = threadIdx.x; g < G; g += blockDim.x) {
         m += a.part[3 * g];
@@ -1285,29 +1226,24 @@ void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
 }
 void launch_pcg_init(const PcgArgs &a, hipStream_t st) { k_pcg_init<<<1, 1024, 0, st>>>(a); }
 // compute units of the current device (persistent SpMV grid)
-static int spmv_cus()
+static int device_cus()
 {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
         n = 256;
     return n;
 }
-void launch_spmv(const PcgArgs &a, hipStream_t st)
+int spmv_grid(int nblk)
 {
-    if (a.once_read && a.nblk > 0) {
-        const int ntile = (a.nblk + kTile - 1) / kTile;
-        k_spmv_tiles<<<std::min(ntile, 4 * spmv_cus()), 256, 0, st>>>(a);
-        k_spmv_gather<<<a.nc, 256, 0, st>>>(a);
-    } else {
-        k_spmv<<<a.nc, 256, 0, st>>>(a);
-    }
+    const int ntile = (nblk + kTile - 1) / kTile;
+    return std::max(1, std::min(ntile, 4 * device_cus()));
 }
+void launch_spmv(const PcgArgs &a, hipStream_t st) { k_spmv_tiles<<<a.ndpart, 256, 0, st>>>(a); }
 int pcg_groups(int nc) { return (nc + 27) / 28; }
 void launch_pcg_update(const PcgArgs &a, hipStream_t st)
 {
-    const int G = pcg_groups(a.nc);
-    k_pcg_a<<<G, 256, 0, st>>>(a);
-    k_pcg_b<<<G, 256, 0, st>>>(a);
+    k_pcg_a<<<a.nc, 256, 0, st>>>(a);
+    k_pcg_b<<<pcg_groups(a.nc), 256, 0, st>>>(a);
 }
 void launch_backsub(const StepArgs &a, hipStream_t st)
 {

@@ -106,8 +106,19 @@ void rectify34(double *P)
 
 enum Stage { ST_SETUP = 0, ST_ITER = 1, ST_DONE = 2 };
 
+// pinned host slots for the small device-to-host readbacks (an async copy
+// into pageable memory is a blocking staged copy)
+struct Readback {
+    double sc[2];
+    int nflag, nanf;
+    PcgScalars pcg;
+    double cost, ws;
+};
+
 struct rpba_ctx {
     rpba_block *io = nullptr;
+    Readback *hb = nullptr;          // pinned
+    double *hcam = nullptr;          // pinned, 15 per camera: stored [R|t] (12) and intrinsics (3)
     rpba_flags fl{};
     rpba_log_fn log = nullptr;
     void *user = nullptr;
@@ -126,7 +137,6 @@ struct rpba_ctx {
     std::vector<double> Pst, Ist;    // stored cameras (host copy at pass boundaries)
     std::vector<double> Pst_init, Ist_init, R0, C0;
     bool structure_dirty = true;     // pairs/blocks must be rebuilt
-    int spmv_once = 1;               // 1: once-read SpMV (tiles + gather); 0: per-row double read
     int rec32 = 0;                   // FP32 Jacobians and W/Y records (precision 1)
     DevStruct ds;                    // device tracks, camera-major order, pairs, blocks
 
@@ -136,7 +146,7 @@ struct rpba_ctx {
     DBuf<double> d_X, d_Xn, d_X0, d_Vinv, d_gp, d_Sd, d_b, d_Cinv, d_In_bak;
     DBuf<unsigned char> d_Wrec, d_Yrec;  // element type by precision
     DBuf<double> d_Su;
-    DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_rowdot, d_partial, d_scal, d_Dd, d_ppart, d_yN, d_yT;
+    DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_partial, d_scal, d_Dd, d_ppart, d_yN, d_yT, d_dpart;
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -269,16 +279,9 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_gp.alloc(3 * np);
     rc |= c->d_Su.alloc(81 * (size_t)std::max(c->ds.nblk, 1));
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
-    // SpMV reads every stored block once (tiles + gather); RPBA_SPMV=2 selects
-    // the per-row kernel that reads each off-diagonal block twice
-    {
-        const char *ev = std::getenv("RPBA_SPMV");
-        c->spmv_once = (ev && ev[0] == '2') ? 0 : 1;
-    }
-    if (c->spmv_once) {
-        rc |= c->d_yN.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
-        rc |= c->d_yT.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
-    }
+    rc |= c->d_yN.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
+    rc |= c->d_yT.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
+    rc |= c->d_dpart.alloc((size_t)spmv_grid(c->ds.nblk));
     const int nb = std::max(cost_blocks(c->ds.ntr), (int)((np + 255) / 256 + 1));
     rc |= c->d_partial.alloc((size_t)nb);
     rc |= c->d_ipartial.alloc((size_t)nb);
@@ -403,8 +406,8 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.Sd = c->d_Sd.p; a.Su = c->d_Su.p; a.Cinv = c->d_Cinv.p; a.b = c->d_b.p;
     a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.col_pos = c->ds.cpos; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
     a.x = c->d_x.p; a.best = c->d_best.p; a.r = c->d_r.p; a.d = c->d_d.p; a.Ad = c->d_Ad.p;
-    a.h = c->d_h.p; a.h1 = c->d_h1.p; a.rowdot = c->d_rowdot.p; a.part = c->d_ppart.p; a.S = c->d_pcg.p;
-    a.yN = c->d_yN.p; a.yT = c->d_yT.p; a.nblk = c->ds.nblk; a.once_read = c->spmv_once;
+    a.h = c->d_h.p; a.h1 = c->d_h1.p; a.part = c->d_ppart.p; a.S = c->d_pcg.p;
+    a.yN = c->d_yN.p; a.yT = c->d_yT.p; a.dpart = c->d_dpart.p; a.nblk = c->ds.nblk; a.ndpart = spmv_grid(c->ds.nblk);
     a.j = 0;
     return a;
 }
@@ -442,8 +445,9 @@ static void launch_trial_cost(rpba_ctx *c)
 static int trial_cost(rpba_ctx *c, double &cost)
 {
     launch_trial_cost(c);
-    HIPCHK(hipMemcpyAsync(&cost, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&c->hb->cost, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    cost = c->hb->cost;
     return RPBA_OK;
 }
 
@@ -478,8 +482,9 @@ static int bundle_step(rpba_ctx *c)
             c->timed("pcg_update", [&] { launch_pcg_update(pa, c->st); });
         }
         launched += chunk;
-        HIPCHK(hipMemcpyAsync(&hs, c->d_pcg.p + (launched & 1), sizeof(hs), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(&c->hb->pcg, c->d_pcg.p + (launched & 1), sizeof(hs), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
+        hs = c->hb->pcg;
         if (hs.done) break;
         if (launched > pa.msize + 8) return RPBA_E_HIP;  // cannot happen: the kernel stops at msize
         if (!c->timing) chunk = std::min(std::max(chunk, 8) * 2, 64);
@@ -536,10 +541,9 @@ static int pass_setup(rpba_ctx *c)
         HIPCHK(hipMemsetAsync(c->d_flag.p, 0, ntr, c->st));
         HIPCHK(hipMemsetAsync(c->d_flagst.p, 0, ntr, c->st));
         launch_wsum(c->ds.ntr, c->ds.w0x, c->ds.w0y, c->d_partial.p, c->d_scal.p, c->st);
-        double ws = 0.;
-        HIPCHK(hipMemcpyAsync(&ws, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(&c->hb->ws, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
-        c->wsum = ws;
+        c->wsum = c->hb->ws;
         c->ittflag = false;
     }
     c->wsum = (double)(2 * c->imagepointnr) / c->wsum;
@@ -592,9 +596,12 @@ static int pass_end(rpba_ctx *c)
     c->stats.nunknowns[c->it] = nun;
     c->stats.passes_done = c->it + 1;
     // host copies of the stored cameras for the next EncodeCameras
-    HIPCHK(hipMemcpyAsync(c->Pst.data(), c->c_Pst.p, sizeof(double) * c->Pst.size(), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(c->Ist.data(), c->c_Ist.p, sizeof(double) * c->Ist.size(), hipMemcpyDeviceToHost, c->st));
+    double *hP = c->hcam, *hI = c->hcam + c->Pst.size();
+    HIPCHK(hipMemcpyAsync(hP, c->c_Pst.p, sizeof(double) * c->Pst.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(hI, c->c_Ist.p, sizeof(double) * c->Ist.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    std::copy(hP, hP + c->Pst.size(), c->Pst.begin());
+    std::copy(hI, hI + c->Ist.size(), c->Ist.begin());
     if (c->fl.robust && c->it == 0) {
         const int rc = c->ds.robust_delete(c->d_flagst.p, c->minimagenrdel, c->d_X.p, c->st);
         if (rc) return rc;
@@ -624,12 +631,13 @@ static int lm_iteration(rpba_ctx *c)
     if (rc) return rc;
     launch_trial_cost(c);
     // one readback per LM iteration: cost, reweighting sums, NaN flag
-    double sc[2] = {0., 0.};
-    int nflag = 0, nanf = 0;
-    HIPCHK(hipMemcpyAsync(sc, c->d_scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(&nflag, c->d_iscal.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(&nanf, &c->d_pcg.p->nan_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    Readback *hb = c->hb;
+    HIPCHK(hipMemcpyAsync(hb->sc, c->d_scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&hb->nflag, c->d_iscal.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&hb->nanf, &c->d_pcg.p->nan_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    const double sc[2] = {hb->sc[0], hb->sc[1]};
+    const int nflag = hb->nflag, nanf = hb->nanf;
     if (nanf) {
         HIPCHK(hipMemcpyAsync(c->c_In.p, c->d_In_bak.p, sizeof(double) * 3 * (size_t)c->nc, hipMemcpyDeviceToDevice, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
@@ -747,6 +755,9 @@ static int ctx_init(rpba_ctx *c)
 
     HIPCHK(hipGetDevice(&c->dev));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    if (hipHostMalloc((void **)&c->hb, sizeof(Readback), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->hcam, sizeof(double) * 15 * (size_t)nc, hipHostMallocDefault) != hipSuccess)
+        return RPBA_E_NOMEM;
     const size_t n9 = 9 * (size_t)nc;
     rc = 0;
     rc |= c->d_Sd.alloc(81 * (size_t)nc);
@@ -759,13 +770,12 @@ static int ctx_init(rpba_ctx *c)
     rc |= c->d_Ad.alloc(n9);
     rc |= c->d_h.alloc(n9);
     rc |= c->d_h1.alloc(n9);
-    rc |= c->d_rowdot.alloc((size_t)nc);
     rc |= c->d_s00.alloc((size_t)nc);
     rc |= c->d_scal.alloc(4);
     rc |= c->d_iscal.alloc(4);
     rc |= c->d_pcg.alloc(2);
     rc |= c->d_In_bak.alloc(3 * (size_t)nc);
-    rc |= c->d_ppart.alloc(3 * (size_t)pcg_groups(nc));
+    rc |= c->d_ppart.alloc(3 * (size_t)nc);
     rc |= c->c_Rb.alloc(n9);
     rc |= c->c_Cb.alloc(3 * (size_t)nc);
     if (rc) return RPBA_E_NOMEM;
@@ -984,6 +994,10 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
 {
     if (!c) return;
     if (c->st) (void)hipStreamSynchronize(c->st);
+    if (c->hb) (void)hipHostFree(c->hb);
+    if (c->hcam) (void)hipHostFree(c->hcam);
+    c->hb = nullptr;
+    c->hcam = nullptr;
     for (auto &t : c->pending) {
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
@@ -995,7 +1009,7 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_X.release(); c->d_Xn.release(); c->d_X0.release(); c->d_In_bak.release(); c->d_Wrec.release(); c->d_Yrec.release(); c->d_Vinv.release(); c->d_gp.release();
     c->d_Sd.release(); c->d_Su.release(); c->d_b.release(); c->d_Cinv.release();
     c->d_x.release(); c->d_best.release(); c->d_r.release(); c->d_d.release(); c->d_Ad.release();
-    c->d_h.release(); c->d_h1.release(); c->d_rowdot.release(); c->d_partial.release(); c->d_scal.release();
+    c->d_h.release(); c->d_h1.release(); c->d_partial.release(); c->d_scal.release();
     c->d_ipartial.release(); c->d_iscal.release();
     c->c_Rb.release(); c->c_Cb.release(); c->c_Rn.release(); c->c_Cn.release(); c->c_In.release();
     c->c_th.release(); c->c_thn.release(); c->c_Pst.release(); c->c_Ist.release(); c->c_mask.release();
@@ -1003,6 +1017,7 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_Dd.release();
     c->d_yN.release();
     c->d_yT.release();
+    c->d_dpart.release();
     c->d_ppart.release();
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;

@@ -58,7 +58,12 @@ struct DevStruct {
     size_t tmp_cap = 0;
 
     Tracks init;           // snapshot of the initial tracks (restore_tracks)
+    Tracks spare;          // RobustDelete's output buffers, swapped with the live tracks
     int npts0 = 0, ntr0 = 0;
+    // pinned host staging for the structure's small readbacks and uploads
+    int *hpin = nullptr;
+    size_t hpin_cap = 0;
+    int pinned(size_t n_ints);
 
     int scratch(size_t bytes);
     // working tracks <- snapshot (device-to-device), sizes back to the initial ones

@@ -42,6 +42,17 @@ static int grow(T *&p, size_t &cap, size_t n)
     return RPBA_OK;
 }
 
+int DevStruct::pinned(size_t n)
+{
+    if (hpin && hpin_cap >= n) return RPBA_OK;
+    if (hpin) (void)hipHostFree(hpin);
+    hpin = nullptr;
+    hpin_cap = 0;
+    if (hipHostMalloc((void **)&hpin, sizeof(int) * (n ? n : 1), hipHostMallocDefault) != hipSuccess) return RPBA_E_NOMEM;
+    hpin_cap = n;
+    return RPBA_OK;
+}
+
 int DevStruct::scratch(size_t bytes)
 {
     return grow(tmp, tmp_cap, bytes);
@@ -148,9 +159,10 @@ int DevStruct::build(hipStream_t st, int *first_empty)
         DCHK(rocprim::radix_sort_pairs(tmp, bytes, (const unsigned *)tcam, u32a, i32a, camq, ntr, 0u, cb, st));
     }
     k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, 1u, u32a, ntr, camoff);
-    std::vector<int> co((size_t)nc + 1);
-    DCHK(hipMemcpyAsync(co.data(), camoff, sizeof(int) * co.size(), hipMemcpyDeviceToHost, st));
+    if (pinned((size_t)nc + 1)) return RPBA_E_NOMEM;
+    DCHK(hipMemcpyAsync(hpin, camoff, sizeof(int) * ((size_t)nc + 1), hipMemcpyDeviceToHost, st));
     DCHK(hipStreamSynchronize(st));
+    const std::vector<int> co(hpin, hpin + nc + 1);
     for (int k = 0; k < nc; ++k)
         if (co[(size_t)k + 1] == co[k]) {
             *first_empty = k;
@@ -168,8 +180,11 @@ int DevStruct::build(hipStream_t st, int *first_empty)
         rc |= grow(ch_start, ch_start_cap, cs.size());
         rc |= grow(ch_off, ch_off_cap, cof.size());
         if (rc) return RPBA_E_NOMEM;
-        DCHK(hipMemcpyAsync(ch_start, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice, st));
-        DCHK(hipMemcpyAsync(ch_off, cof.data(), sizeof(int) * cof.size(), hipMemcpyHostToDevice, st));
+        if (pinned(cs.size() + cof.size())) return RPBA_E_NOMEM;
+        std::copy(cs.begin(), cs.end(), hpin);
+        std::copy(cof.begin(), cof.end(), hpin + cs.size());
+        DCHK(hipMemcpyAsync(ch_start, hpin, sizeof(int) * cs.size(), hipMemcpyHostToDevice, st));
+        DCHK(hipMemcpyAsync(ch_off, hpin + cs.size(), sizeof(int) * cof.size(), hipMemcpyHostToDevice, st));
         DCHK(hipStreamSynchronize(st));
     }
 
@@ -183,9 +198,9 @@ int DevStruct::build(hipStream_t st, int *first_empty)
         if (scratch(bytes)) return RPBA_E_NOMEM;
         DCHK(rocprim::exclusive_scan(tmp, bytes, i32a, ppoff, 0, (size_t)npts + 1, rocprim::plus<int>(), st));
     }
-    int P = 0;
-    DCHK(hipMemcpyAsync(&P, ppoff + npts, sizeof(int), hipMemcpyDeviceToHost, st));
+    DCHK(hipMemcpyAsync(hpin, ppoff + npts, sizeof(int), hipMemcpyDeviceToHost, st));
     DCHK(hipStreamSynchronize(st));
+    const int P = hpin[0];
     npairs = P;
     rc |= grow(key_in, key_in_cap, (size_t)P);
     rc |= grow(key_out, key_out_cap, (size_t)P);
@@ -212,8 +227,9 @@ int DevStruct::build(hipStream_t st, int *first_empty)
         DCHK(rocprim::run_length_encode(nullptr, bytes, key_out, (unsigned)P, ukey, ucnt, nruns, st));
         if (scratch(bytes)) return RPBA_E_NOMEM;
         DCHK(rocprim::run_length_encode(tmp, bytes, key_out, (unsigned)P, ukey, ucnt, nruns, st));
-        DCHK(hipMemcpyAsync(&nbk, nruns, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        DCHK(hipMemcpyAsync(hpin, nruns, sizeof(unsigned), hipMemcpyDeviceToHost, st));
         DCHK(hipStreamSynchronize(st));
+        nbk = (unsigned)hpin[0];
     }
     nblk = (int)nbk;
     rc |= grow(poff, poff_cap, (size_t)nblk + 1);
@@ -324,14 +340,15 @@ int DevStruct::robust_delete(const unsigned char *flag, int min_keep, double *X,
     DCHK(rocprim::exclusive_scan(nullptr, bytes, i32b, newoff, 0, (size_t)npts + 1, rocprim::plus<int>(), st));
     if (scratch(bytes)) return RPBA_E_NOMEM;
     DCHK(rocprim::exclusive_scan(tmp, bytes, i32b, newoff, 0, (size_t)npts + 1, rocprim::plus<int>(), st));
-    int sizes[2] = {0, 0};
-    DCHK(hipMemcpyAsync(&sizes[0], newi + npts, sizeof(int), hipMemcpyDeviceToHost, st));
-    DCHK(hipMemcpyAsync(&sizes[1], newoff + npts, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (pinned(2)) return RPBA_E_NOMEM;
+    DCHK(hipMemcpyAsync(hpin, newi + npts, sizeof(int), hipMemcpyDeviceToHost, st));
+    DCHK(hipMemcpyAsync(hpin + 1, newoff + npts, sizeof(int), hipMemcpyDeviceToHost, st));
     // the flagged entries of kept points are dropped; for a point without any
-    // flag every entry is kept (the flag array is zero there)
-    Tracks b;
-    rc |= b.alloc(sizes_cap_pts, sizes_cap_tr);
-    if (rc) return RPBA_E_NOMEM;
+    // flag every entry is kept (the flag array is zero there).  The output goes
+    // to the spare buffers (full initial capacity), which then become the live
+    // tracks; the old live buffers are the next spare.
+    Tracks &b = spare;
+    if (!b.toff && b.alloc(sizes_cap_pts, sizes_cap_tr)) return RPBA_E_NOMEM;
     RdArgs a;
     a.npts = npts; a.toff = toff; a.tcam = tcam; a.tfo = tfo; a.nrpv = nrpv; a.newi = newi; a.newoff = newoff;
     a.flag = flag; a.ox = ox; a.oy = oy; a.wx = w0x; a.wy = w0y; a.wxy = w0xy; a.X = X;
@@ -339,11 +356,9 @@ int DevStruct::robust_delete(const unsigned char *flag, int min_keep, double *X,
     a.ox2 = b.ox; a.oy2 = b.oy; a.wx2 = b.w0x; a.wy2 = b.w0y; a.wxy2 = b.w0xy; a.X2 = b.X;
     k_rd_scatter<<<nb_(npts + 1, 256), 256, 0, st>>>(a);
     DCHK(hipStreamSynchronize(st));
-    const int np2 = sizes[0], nt2 = sizes[1];
+    const int np2 = hpin[0], nt2 = hpin[1];
     DCHK(hipMemcpyAsync(X, b.X, sizeof(double) * 3 * (size_t)np2, hipMemcpyDeviceToDevice, st));
-    DCHK(hipStreamSynchronize(st));  // b.X is read by the copy above
     swap_tracks(b);
-    b.release();
     npts = np2;
     ntr = nt2;
     return RPBA_OK;
@@ -471,6 +486,10 @@ int DevStruct::download_tracks(std::vector<int> &toff_h, std::vector<int> &nrpv_
 void DevStruct::release()
 {
     init.release();
+    spare.release();
+    if (hpin) (void)hipHostFree(hpin);
+    hpin = nullptr;
+    hpin_cap = 0;
     Tracks t;
     t.toff = toff; t.nrpv = nrpv; t.tcam = tcam; t.tpt = tpt; t.tfo = tfo;
     t.ox = ox; t.oy = oy; t.w0x = w0x; t.w0y = w0y; t.w0xy = w0xy; t.X = nullptr;

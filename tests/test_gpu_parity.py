@@ -129,18 +129,3 @@ def test_partition_path_is_refused():
     assert rc == capi.RPBA_E_UNSUPPORTED
     rc, _, st, _ = api.calc_plsa(sc, capi.make_flags(threads=0, force_global=1))
     assert rc == 0 and st["passes_done"] == 2
-
-
-def test_double_read_spmv_variant(scene_c1, monkeypatch):
-    """The per-row SpMV that reads every off-diagonal block twice (RPBA_SPMV=2)
-    sums in another order than the default once-read SpMV; same bar."""
-    monkeypatch.setenv("RPBA_SPMV", "2")
-    _say("[double-read SpMV c1]")
-    flags = capi.make_flags(robust=1)
-    rc0, out0, st0, _ = oracle.solve(scene_c1, flags, fsites=0)
-    rc1, out1, st1, _ = api.calc_plsa(scene_c1, flags)
-    assert rc1 == rc0 == 0 and st1["itt"] == st0["itt"]
-    check_trajectory(st1["trace_sig0"], st0["trace_sig0"], st1["trace_pcg"], st0["trace_pcg"],
-                     st1["trace_accept"], st0["trace_accept"], st1["trace_nrobs"], st0["trace_nrobs"])
-    assert np.array_equal(out1["obs_kept"], out0["obs_kept"])
-    assert abs(st1["sigma_out"][1] - st0["sigma_out"][1]) <= 1e-6 * st0["sigma_out"][1]
