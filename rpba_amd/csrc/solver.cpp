@@ -782,7 +782,14 @@ static int ctx_init(rpba_ctx *c)
     reset_state(c);
     rc = upload_tracks(c);
     if (rc) return rc;
-    return upload_initial_cameras(c);
+    rc = upload_initial_cameras(c);
+    if (rc) return rc;
+    // the first copy into a pinned buffer pays a one-time set-up of several
+    // milliseconds: take it here rather than inside the first pass boundary
+    HIPCHK(hipMemcpyAsync(c->hcam, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->hb, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return RPBA_OK;
 }
 
 extern "C" int rpba_version(void) { return RPBA_ABI_VERSION; }

@@ -59,6 +59,12 @@ struct DevStruct {
 
     Tracks init;           // snapshot of the initial tracks (restore_tracks)
     Tracks spare;          // RobustDelete's output buffers, swapped with the live tracks
+    // RobustDelete updates the pairs and the camera-major order in place of a
+    // rebuild: emap (old entry -> new entry or -1), flags and their scans
+    int *emap = nullptr, *sflag = nullptr, *spos = nullptr, *cq2 = nullptr;
+    size_t emap_cap = 0, sflag_cap = 0, spos_cap = 0, cq2_cap = 0;
+    bool updated = false;    // the next build() has nothing to do
+    int updated_empty = -1;  // first image without entries after that update
     int npts0 = 0, ntr0 = 0;
     // pinned host staging for the structure's small readbacks and uploads
     int *hpin = nullptr;
@@ -74,6 +80,8 @@ struct DevStruct {
     // camera-major order, pairs and blocks; *first_empty = first image without
     // observations (then nothing else is built) or -1
     int build(hipStream_t st, int *first_empty);
+    int chunks_from_camoff(hipStream_t st, int *first_empty);
+    int update_after_delete(int ntr_old, hipStream_t st);
     // RobustDelete on the flags of the stored state; X (accepted points) is
     // compacted in place
     int robust_delete(const unsigned char *flag, int min_keep, double *X, hipStream_t st);

@@ -140,10 +140,50 @@ __global__ void k_set_last(int *a, int idx, int v)
 
 static inline unsigned nb_(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// chunks of at most kChunk entries of one image over the camera-major list,
+// from camoff; reports the first image without entries
+int DevStruct::chunks_from_camoff(hipStream_t st, int *first_empty)
+{
+    int rc = 0;
+    *first_empty = -1;
+    if (pinned((size_t)nc + 1)) return RPBA_E_NOMEM;
+    DCHK(hipMemcpyAsync(hpin, camoff, sizeof(int) * ((size_t)nc + 1), hipMemcpyDeviceToHost, st));
+    DCHK(hipStreamSynchronize(st));
+    const std::vector<int> co(hpin, hpin + nc + 1);
+    for (int k = 0; k < nc; ++k)
+        if (co[(size_t)k + 1] == co[k]) {
+            *first_empty = k;
+            return RPBA_OK;
+        }
+    std::vector<int> cs, cof((size_t)nc + 1, 0);
+    for (int k = 0; k < nc; ++k) {
+        cof[k] = (int)cs.size();
+        for (int j = co[k]; j < co[(size_t)k + 1]; j += kChunk) cs.push_back(j);
+    }
+    cof[nc] = (int)cs.size();
+    nch = (int)cs.size();
+    cs.push_back(ntr);
+    rc |= grow(ch_start, ch_start_cap, cs.size());
+    rc |= grow(ch_off, ch_off_cap, cof.size());
+    if (rc) return RPBA_E_NOMEM;
+    if (pinned(cs.size() + cof.size())) return RPBA_E_NOMEM;
+    std::copy(cs.begin(), cs.end(), hpin);
+    std::copy(cof.begin(), cof.end(), hpin + cs.size());
+    DCHK(hipMemcpyAsync(ch_start, hpin, sizeof(int) * cs.size(), hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(ch_off, hpin + cs.size(), sizeof(int) * cof.size(), hipMemcpyHostToDevice, st));
+    DCHK(hipStreamSynchronize(st));
+    return RPBA_OK;
+}
+
 int DevStruct::build(hipStream_t st, int *first_empty)
 {
     int rc = 0;
     *first_empty = -1;
+    if (updated) {
+        updated = false;
+        *first_empty = updated_empty;
+        return RPBA_OK;
+    }
     // --- camera-major: stable sort of entry ids by image ---
     rc |= grow(camoff, camoff_cap, (size_t)nc + 1);
     rc |= grow(camq, camq_cap, (size_t)ntr);
@@ -159,34 +199,8 @@ int DevStruct::build(hipStream_t st, int *first_empty)
         DCHK(rocprim::radix_sort_pairs(tmp, bytes, (const unsigned *)tcam, u32a, i32a, camq, ntr, 0u, cb, st));
     }
     k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, 1u, u32a, ntr, camoff);
-    if (pinned((size_t)nc + 1)) return RPBA_E_NOMEM;
-    DCHK(hipMemcpyAsync(hpin, camoff, sizeof(int) * ((size_t)nc + 1), hipMemcpyDeviceToHost, st));
-    DCHK(hipStreamSynchronize(st));
-    const std::vector<int> co(hpin, hpin + nc + 1);
-    for (int k = 0; k < nc; ++k)
-        if (co[(size_t)k + 1] == co[k]) {
-            *first_empty = k;
-            return RPBA_OK;
-        }
-    {
-        std::vector<int> cs, cof((size_t)nc + 1, 0);
-        for (int k = 0; k < nc; ++k) {
-            cof[k] = (int)cs.size();
-            for (int j = co[k]; j < co[(size_t)k + 1]; j += kChunk) cs.push_back(j);
-        }
-        cof[nc] = (int)cs.size();
-        nch = (int)cs.size();
-        cs.push_back(ntr);
-        rc |= grow(ch_start, ch_start_cap, cs.size());
-        rc |= grow(ch_off, ch_off_cap, cof.size());
-        if (rc) return RPBA_E_NOMEM;
-        if (pinned(cs.size() + cof.size())) return RPBA_E_NOMEM;
-        std::copy(cs.begin(), cs.end(), hpin);
-        std::copy(cof.begin(), cof.end(), hpin + cs.size());
-        DCHK(hipMemcpyAsync(ch_start, hpin, sizeof(int) * cs.size(), hipMemcpyHostToDevice, st));
-        DCHK(hipMemcpyAsync(ch_off, hpin + cs.size(), sizeof(int) * cof.size(), hipMemcpyHostToDevice, st));
-        DCHK(hipStreamSynchronize(st));
-    }
+    rc = chunks_from_camoff(st, first_empty);
+    if (rc || *first_empty >= 0) return rc;
 
     // --- covisibility pairs: count, scan, generate, stable sort by block ---
     rc |= grow(ppoff, ppoff_cap, (size_t)npts + 1);
@@ -294,7 +308,51 @@ struct RdArgs {
     int *toff2, *tcam2, *tpt2, *tfo2, *nrpv2;
     float *ox2, *oy2, *wx2, *wy2, *wxy2;
     double *X2;
+    int *emap;  // old entry -> new entry, or -1
 };
+
+// pair p survives when both entries do; its entries renumbered.  flag[P] = 0
+// closes the scan.
+__global__ void k_remap_pairs(int P, const unsigned long long *pab, const int *emap, unsigned long long *val, int *flag)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > P) return;
+    if (p == P) {
+        flag[p] = 0;
+        return;
+    }
+    const unsigned long long ab = pab[p];
+    const int na = emap[ab >> 32], nb = emap[ab & 0xffffffffull];
+    const bool keep = na >= 0 && nb >= 0;
+    flag[p] = keep ? 1 : 0;
+    val[p] = keep ? (((unsigned long long)na << 32) | (unsigned)nb) : 0ull;
+}
+
+__global__ void k_compact_pairs(int P, const unsigned long long *val, const int *flag, const int *pos, unsigned long long *pab)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P && flag[p]) pab[pos[p]] = val[p];
+}
+
+// off[b] = pos[off[b]] for b = 0..n (offsets into a list that was compacted)
+__global__ void k_remap_offsets(int n, int *off, const int *pos)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b <= n) off[b] = pos[off[b]];
+}
+
+__global__ void k_remap_camq(int n, const int *camq, const int *emap, int *flag)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > n) return;
+    flag[j] = (j < n && emap[camq[j]] >= 0) ? 1 : 0;
+}
+
+__global__ void k_compact_camq(int n, const int *camq, const int *emap, const int *flag, const int *pos, int *out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n && flag[j]) out[pos[j]] = emap[camq[j]];
+}
 
 __global__ void k_rd_scatter(RdArgs a)
 {
@@ -304,14 +362,21 @@ __global__ void k_rd_scatter(RdArgs a)
         a.toff2[a.newi[i]] = a.newoff[i];
         return;
     }
-    if (a.newi[i + 1] == a.newi[i]) return;  // point erased
+    if (a.newi[i + 1] == a.newi[i]) {  // point erased
+        for (int q = a.toff[i]; q < a.toff[i + 1]; ++q) a.emap[q] = -1;
+        return;
+    }
     const int j = a.newi[i];
     a.toff2[j] = a.newoff[i];
     a.nrpv2[j] = a.nrpv[i];
     for (int c = 0; c < 3; ++c) a.X2[3 * (size_t)j + c] = a.X[3 * (size_t)i + c];
     int w = a.newoff[i];
     for (int q = a.toff[i]; q < a.toff[i + 1]; ++q) {
-        if (a.flag[q]) continue;
+        if (a.flag[q]) {
+            a.emap[q] = -1;
+            continue;
+        }
+        a.emap[q] = w;
         a.tcam2[w] = a.tcam[q];
         a.tpt2[w] = j;
         a.tfo2[w] = a.tfo[q];
@@ -331,6 +396,7 @@ int DevStruct::robust_delete(const unsigned char *flag, int min_keep, double *X,
     rc |= grow(i32b, i32b_cap, (size_t)npts + 1);
     rc |= grow(newi, newi_cap, (size_t)npts + 1);
     rc |= grow(newoff, newoff_cap, (size_t)npts + 1);
+    rc |= grow(emap, emap_cap, (size_t)std::max(ntr, 1));
     if (rc) return RPBA_E_NOMEM;
     k_rd_mark<<<nb_(npts + 1, 256), 256, 0, st>>>(npts, min_keep, toff, flag, i32a, i32b);
     size_t bytes = 0;
@@ -354,13 +420,57 @@ int DevStruct::robust_delete(const unsigned char *flag, int min_keep, double *X,
     a.flag = flag; a.ox = ox; a.oy = oy; a.wx = w0x; a.wy = w0y; a.wxy = w0xy; a.X = X;
     a.toff2 = b.toff; a.tcam2 = b.tcam; a.tpt2 = b.tpt; a.tfo2 = b.tfo; a.nrpv2 = b.nrpv;
     a.ox2 = b.ox; a.oy2 = b.oy; a.wx2 = b.w0x; a.wy2 = b.w0y; a.wxy2 = b.w0xy; a.X2 = b.X;
+    a.emap = emap;
     k_rd_scatter<<<nb_(npts + 1, 256), 256, 0, st>>>(a);
     DCHK(hipStreamSynchronize(st));
     const int np2 = hpin[0], nt2 = hpin[1];
     DCHK(hipMemcpyAsync(X, b.X, sizeof(double) * 3 * (size_t)np2, hipMemcpyDeviceToDevice, st));
     swap_tracks(b);
+    const int ntr_old = ntr;
     npts = np2;
     ntr = nt2;
+    return update_after_delete(ntr_old, st);
+}
+
+// The structure follows the deletion without a rebuild.  Entries keep their
+// relative order (points in order, a point's entries in order), so the pair
+// list is still grouped by block and in point order inside a block once the
+// pairs that lost an entry are dropped, and the camera-major list is still
+// sorted once its dropped entries are removed; a rebuild would produce the
+// same lists.  A block left without pairs stays as a zero block.
+int DevStruct::update_after_delete(int ntr_old, hipStream_t st)
+{
+    int rc = 0;
+    const int P = (int)npairs;
+    const size_t n = (size_t)std::max(P, ntr_old) + 1;
+    rc |= grow(sflag, sflag_cap, n);
+    rc |= grow(spos, spos_cap, n);
+    rc |= grow(cq2, cq2_cap, (size_t)std::max(ntr, 1));
+    if (rc) return RPBA_E_NOMEM;
+    size_t bytes = 0;
+    // pairs
+    k_remap_pairs<<<nb_(P + 1, 256), 256, 0, st>>>(P, pab, emap, val_in, sflag);
+    DCHK(rocprim::exclusive_scan(nullptr, bytes, sflag, spos, 0, (size_t)P + 1, rocprim::plus<int>(), st));
+    if (scratch(bytes)) return RPBA_E_NOMEM;
+    DCHK(rocprim::exclusive_scan(tmp, bytes, sflag, spos, 0, (size_t)P + 1, rocprim::plus<int>(), st));
+    if (P > 0) k_compact_pairs<<<nb_(P, 256), 256, 0, st>>>(P, val_in, sflag, spos, pab);
+    k_remap_offsets<<<nb_(nblk + 1, 256), 256, 0, st>>>(nblk, poff, spos);
+    if (pinned(2)) return RPBA_E_NOMEM;
+    DCHK(hipMemcpyAsync(hpin, spos + P, sizeof(int), hipMemcpyDeviceToHost, st));
+    // camera-major order
+    k_remap_camq<<<nb_(ntr_old + 1, 256), 256, 0, st>>>(ntr_old, camq, emap, sflag);
+    DCHK(rocprim::exclusive_scan(nullptr, bytes, sflag, spos, 0, (size_t)ntr_old + 1, rocprim::plus<int>(), st));
+    if (scratch(bytes)) return RPBA_E_NOMEM;
+    DCHK(rocprim::exclusive_scan(tmp, bytes, sflag, spos, 0, (size_t)ntr_old + 1, rocprim::plus<int>(), st));
+    if (ntr_old > 0) k_compact_camq<<<nb_(ntr_old, 256), 256, 0, st>>>(ntr_old, camq, emap, sflag, spos, cq2);
+    k_remap_offsets<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, camoff, spos);
+    DCHK(hipStreamSynchronize(st));
+    npairs = hpin[0];
+    std::swap(camq, cq2);
+    std::swap(camq_cap, cq2_cap);
+    rc = chunks_from_camoff(st, &updated_empty);
+    if (rc) return rc;
+    updated = true;
     return RPBA_OK;
 }
 
@@ -452,6 +562,7 @@ int DevStruct::upload_tracks(const Structure &s, const float *obs_xy, const std:
 
 int DevStruct::restore_tracks(hipStream_t st)
 {
+    updated = false;  // the initial tracks need the full build
     npts = npts0;
     ntr = ntr0;
     const size_t np = (size_t)npts, nt = (size_t)ntr;
@@ -506,7 +617,7 @@ void DevStruct::release()
     fr(pab, pab_cap); fr(ukey, ukey_cap); fr(ucnt, ucnt_cap); fr(nruns, nruns_cap); fr(poff, poff_cap);
     fr(bk, bk_cap); fr(bl, bl_cap); fr(roff, roff_cap); fr(coff, coff_cap); fr(cblk, cblk_cap); fr(cpos, cpos_cap);
     fr(key2, key2_cap); fr(key2s, key2s_cap); fr(bid, bid_cap); fr(newi, newi_cap); fr(newoff, newoff_cap);
-    fr(tmp, tmp_cap);
+    fr(tmp, tmp_cap); fr(emap, emap_cap); fr(sflag, sflag_cap); fr(spos, spos_cap); fr(cq2, cq2_cap);
 }
 
 }  // namespace rpba
