@@ -95,25 +95,37 @@ __global__ void k_pair_counts(int npts, const int *toff, int *cnt)
     cnt[i] = t * (t - 1) / 2;
 }
 
-__global__ void k_pairs(int npts, unsigned nc, const int *toff, const int *tcam, const int *ppoff, unsigned *key,
+// one thread per pair: pair p belongs to point i with ppoff[i] <= p <
+// ppoff[i+1]; its local index enumerates (x, y), x < y, row by row over the
+// point's entries.  key = block (lower camera, higher camera), val = (entry of
+// the lower camera << 32) | entry of the higher camera.
+__global__ void k_pairs(int npts, int P, unsigned nc, const int *toff, const int *tcam, const int *ppoff, unsigned *key,
                         unsigned long long *val)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npts) return;
-    const int b = toff[i], e = toff[i + 1];
-    int o = ppoff[i];
-    for (int x = b; x < e; ++x) {
-        const int cx = tcam[x];
-        for (int y = x + 1; y < e; ++y) {
-            const int cy = tcam[y];
-            const bool sw = cx > cy;
-            const unsigned lo = sw ? cy : cx, hi = sw ? cx : cy;
-            const unsigned a = sw ? y : x, bb = sw ? x : y;
-            key[o] = lo * nc + hi;
-            val[o] = ((unsigned long long)a << 32) | bb;
-            ++o;
-        }
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    int lo = 0, hi = npts;  // last i with ppoff[i] <= p
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ppoff[mid] <= p)
+            lo = mid;
+        else
+            hi = mid;
     }
+    const int i = lo;
+    const int b = toff[i], t = toff[i + 1] - b;
+    int r = p - ppoff[i], x = 0;
+    while (r >= t - 1 - x) {
+        r -= t - 1 - x;
+        ++x;
+    }
+    const int ex = b + x, ey = b + x + 1 + r;
+    const int cx = tcam[ex], cy = tcam[ey];
+    const bool sw = cx > cy;
+    const unsigned lc = sw ? cy : cx, hc = sw ? cx : cy;
+    const unsigned ea = sw ? ey : ex, eb = sw ? ex : ey;
+    key[p] = lc * nc + hc;
+    val[p] = ((unsigned long long)ea << 32) | eb;
 }
 
 __global__ void k_split_blocks(int nb, unsigned nc, const unsigned *ukey, int *bk, int *bl, unsigned *key2, int *bid)
@@ -221,7 +233,7 @@ int DevStruct::build(hipStream_t st, int *first_empty)
     rc |= grow(val_in, val_in_cap, (size_t)P);
     rc |= grow(pab, pab_cap, (size_t)P);
     if (rc) return RPBA_E_NOMEM;
-    if (npts > 0) k_pairs<<<nb_(npts, 128), 128, 0, st>>>(npts, (unsigned)nc, toff, tcam, ppoff, key_in, val_in);
+    if (P > 0) k_pairs<<<nb_(P, 256), 256, 0, st>>>(npts, P, (unsigned)nc, toff, tcam, ppoff, key_in, val_in);
     const unsigned kb = bits_for((unsigned long long)nc * (unsigned long long)nc);
     if (P > 0) {
         size_t bytes = 0;
