@@ -67,6 +67,8 @@ struct DBuf {
 struct Timed {
     std::string name;
     hipEvent_t a, b;
+    int pcg_j;   // launch index inside a PCG solve, or -1
+    bool skip;   // a PCG launch after the stopping rule fired (passes the state on only)
 };
 
 double cpu_seconds()
@@ -196,13 +198,13 @@ struct rpba_ctx {
         return e;
     }
     template <class F>
-    void timed(const char *name, F f)
+    void timed(const char *name, F f, int pcg_j = -1)
     {
         if (!timing) {
             f();
             return;
         }
-        Timed t{name, ev(), ev()};
+        Timed t{name, ev(), ev(), pcg_j, false};
         (void)hipEventRecord(t.a, st);
         f();
         (void)hipEventRecord(t.b, st);
@@ -212,7 +214,7 @@ struct rpba_ctx {
     {
         for (auto &t : pending) {
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            if (!t.skip && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
                 auto &e = ktime[t.name];
                 e.first += ms;
                 e.second += 1;
@@ -473,13 +475,13 @@ static int bundle_step(rpba_ctx *c)
     // once the stopping rule fired, later launches only pass the state on.
     // The first chunk is sized from the previous solve's iteration count.
     PcgScalars hs{};
-    // (with kernel timing on, one iteration per poll: only working launches are timed)
-    int chunk = c->timing ? 1 : std::max(4, c->pcg_last + 2), launched = 0;
+    int chunk = std::max(4, c->pcg_last + 2), launched = 0;
+    const size_t first_timed = c->pending.size();
     for (;;) {
         for (int j = 0; j < chunk; ++j) {
             pa.j = launched + j;
-            c->timed("spmv", [&] { launch_spmv(pa, c->st); });
-            c->timed("pcg_update", [&] { launch_pcg_update(pa, c->st); });
+            c->timed("spmv", [&] { launch_spmv(pa, c->st); }, pa.j);
+            c->timed("pcg_update", [&] { launch_pcg_update(pa, c->st); }, pa.j);
         }
         launched += chunk;
         HIPCHK(hipMemcpyAsync(&c->hb->pcg, c->d_pcg.p + (launched & 1), sizeof(hs), hipMemcpyDeviceToHost, c->st));
@@ -487,8 +489,11 @@ static int bundle_step(rpba_ctx *c)
         hs = c->hb->pcg;
         if (hs.done) break;
         if (launched > pa.msize + 8) return RPBA_E_HIP;  // cannot happen: the kernel stops at msize
-        if (!c->timing) chunk = std::min(std::max(chunk, 8) * 2, 64);
+        chunk = std::min(std::max(chunk, 8) * 2, 64);
     }
+    // kernel timing counts the launches that ran an iteration (j < iters)
+    for (size_t q = first_timed; q < c->pending.size(); ++q)
+        if (c->pending[q].pcg_j >= hs.iters) c->pending[q].skip = true;
     c->pcg_last = hs.iters;
     c->pcg_total += hs.iters;
     StepArgs sa2 = step_args(c, c->d_best.p);
