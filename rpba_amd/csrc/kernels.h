@@ -111,6 +111,7 @@ int spmv_grid(int nblk);  // workgroups of the SpMV launch (PcgArgs::ndpart)
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st);
 void launch_pcg_init(const PcgArgs &a, hipStream_t st);
 void launch_spmv(const PcgArgs &a, hipStream_t st);
+
 void launch_pcg_update(const PcgArgs &a, hipStream_t st);
 void launch_backsub(const StepArgs &a, hipStream_t st);
 void launch_decode(const StepArgs &a, hipStream_t st);

@@ -619,11 +619,10 @@ constexpr int kTile = 28;
 
 // Persistent: workgroup w takes tiles w, w + gridDim.x, ... (grid = the
 // workgroups the chip holds at once, four per CU).
-__global__ __launch_bounds__(256) void k_spmv_tiles(PcgArgs a)
+__device__ __forceinline__ void spmv_tiles_body(const PcgArgs &a, int j)
 {
-    if (a.S[a.j & 1].done) return;
     const int ntile = (a.nblk + kTile - 1) / kTile;
-    const bool rev = (a.j & 1) != 0;
+    const bool rev = (j & 1) != 0;
     const int t = threadIdx.x;
     const int g = t / 9, r = t % 9;
     // kTile * 81 doubles: a multiple of 16 bytes, so every tile starts 16-byte aligned
@@ -697,6 +696,12 @@ __global__ __launch_bounds__(256) void k_spmv_tiles(PcgArgs a)
 }
 #undef SPMV_TILE_FETCH
 
+__global__ __launch_bounds__(256) void k_spmv_tiles(PcgArgs a)
+{
+    if (a.S[a.j & 1].done) return;
+    spmv_tiles_body(a, a.j);
+}
+
 // sum of n values in a fixed order; every thread of the block gets it
 __device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
 {
@@ -709,17 +714,14 @@ __device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
 // tile partials; (S d)_k = yD_k + the run sums of row k + the column shares of
 // column k, in a fixed order; x += alpha d, r -= alpha S d, h1 = C r; the
 // camera's shares of |alpha S d|^2, r.(h1 - h) and r.h1
-__global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
+__device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S, int k)
 {
     __shared__ double sh[4];
     __shared__ double red[28][9];
     __shared__ double rs[9], p3[3][9];
-    const PcgScalars S = a.S[a.j & 1];
-    if (S.done) return;
     const double dAd = sum_fixed(a.dpart, a.ndpart, sh);
     const double rho = S.rho;
     const double alpha = (fabs(rho) < 1.e-20 || fabs(rho) > 1.e20) ? 0. : rho / dAd;
-    const int k = blockIdx.x;
     const int t = threadIdx.x;
     const int g = t / 9, r = t % 9;
     const int r0 = a.row_off[k], r1 = a.row_off[k + 1];
@@ -774,19 +776,22 @@ __global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
         for (int q = 0; q < 9; ++q) s += p3[t][q];
         a.part[3 * (size_t)k + t] = s;
     }
+    __syncthreads();  // LDS reused by the next camera of a persistent workgroup
 }
 
-// PCG step, part 2: best iterate, stopping rule, beta, d = h1 + beta d, h = h1;
-// workgroup 0 writes the scalars of the next launch
-__global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
+__global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
+{
+    const PcgScalars S = a.S[a.j & 1];
+    if (S.done) return;
+    pcg_a_body(a, S, blockIdx.x);
+}
+
+// PCG step, part 2: best iterate, stopping rule, beta, d = h1 + beta d, h = h1
+// for the cameras of group grp (28 per group); group 0 writes the scalars of
+// the next launch
+__device__ __forceinline__ void pcg_b_body(const PcgArgs &a, const PcgScalars &S, PcgScalars *Sn, int grp)
 {
     __shared__ double sh[4];
-    const PcgScalars S = a.S[a.j & 1];
-    PcgScalars *Sn = a.S + ((a.j + 1) & 1);
-    if (S.done) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) *Sn = S;
-        return;
-    }
     const int G = a.nc;
     double m = 0., num = 0., rh = 0.;
     for (int g = threadIdx.x; g < G; g += blockDim.x) {
@@ -801,7 +806,7 @@ __global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
     const double m0 = (it == 0) ? m : S.m0;
     const bool better = m < S.mold;
     const int t = threadIdx.x;
-    const int k = blockIdx.x * 28 + t / 9;
+    const int k = grp * 28 + t / 9;
     const bool on = (t < 252) && (k < a.nc);
     const size_t j = 9 * (size_t)k + t % 9;
     if (better && on) a.best[j] = a.x[j];
@@ -811,7 +816,7 @@ __global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
     if (m * 1.e5 < m0) {
         z.done = 1;
         z.iters = it + 1;
-        if (blockIdx.x == 0 && t == 0) *Sn = z;
+        if (grp == 0 && t == 0) *Sn = z;
         return;
     }
     double beta = (fabs(num) < 1.e-10) ? 0. : num / S.rho;
@@ -827,7 +832,18 @@ __global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
         z.done = 1;
         z.iters = a.msize;
     }
-    if (blockIdx.x == 0 && t == 0) *Sn = z;
+    if (grp == 0 && t == 0) *Sn = z;
+}
+
+__global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
+{
+    const PcgScalars S = a.S[a.j & 1];
+    PcgScalars *Sn = a.S + ((a.j + 1) & 1);
+    if (S.done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *Sn = S;
+        return;
+    }
+    pcg_b_body(a, S, Sn, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1233,10 +1249,11 @@ static int device_cus()
         n = 256;
     return n;
 }
+// SpMV workgroups: four per CU (persistent over the tiles)
 int spmv_grid(int nblk)
 {
-    const int ntile = (nblk + kTile - 1) / kTile;
-    return std::max(1, std::min(ntile, 4 * device_cus()));
+    (void)nblk;
+    return 4 * device_cus();
 }
 void launch_spmv(const PcgArgs &a, hipStream_t st) { k_spmv_tiles<<<a.ndpart, 256, 0, st>>>(a); }
 int pcg_groups(int nc) { return (nc + 27) / 28; }

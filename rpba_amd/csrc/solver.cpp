@@ -453,6 +453,8 @@ static int trial_cost(rpba_ctx *c, double &cost)
     return RPBA_OK;
 }
 
+static int finish_bundle_step(rpba_ctx *c, const PcgScalars &hs);
+
 // CalcBundleSep (rbacore.cpp:1094-1195) + theta_new = theta - x + decode.
 // The NaN / inf check of the step (rbacore.cpp:541-546) is raised on the
 // device by k_backsub (points) and k_decode (the float camera step); the
@@ -494,6 +496,12 @@ static int bundle_step(rpba_ctx *c)
     // kernel timing counts the launches that ran an iteration (j < iters)
     for (size_t q = first_timed; q < c->pending.size(); ++q)
         if (c->pending[q].pcg_j >= hs.iters) c->pending[q].skip = true;
+    return finish_bundle_step(c, hs);
+}
+
+// after the PCG: back-substitution and decode of the trial state
+static int finish_bundle_step(rpba_ctx *c, const PcgScalars &hs)
+{
     c->pcg_last = hs.iters;
     c->pcg_total += hs.iters;
     StepArgs sa2 = step_args(c, c->d_best.p);
@@ -789,9 +797,9 @@ static int ctx_init(rpba_ctx *c)
     if (rc) return rc;
     rc = upload_initial_cameras(c);
     if (rc) return rc;
-    // the first copy into a pinned buffer pays a one-time set-up of several
-    // milliseconds: take it here rather than inside the first pass boundary
-    HIPCHK(hipMemcpyAsync(c->hcam, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    // the first device-to-host copy of this size pays a one-time set-up of
+    // several milliseconds: take it here rather than at the first pass boundary
+    HIPCHK(hipMemcpyAsync(c->hcam, c->d_Sd.p, sizeof(double) * 15 * (size_t)nc, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(c->hb, c->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return RPBA_OK;

@@ -129,3 +129,4 @@ def test_partition_path_is_refused():
     assert rc == capi.RPBA_E_UNSUPPORTED
     rc, _, st, _ = api.calc_plsa(sc, capi.make_flags(threads=0, force_global=1))
     assert rc == 0 and st["passes_done"] == 2
+
