@@ -204,11 +204,9 @@ __device__ __forceinline__ void jac_point(const CamV &c, const Ev &e, T Bp[2][3]
     }
 }
 
-// One thread per point.  Pass 1 over the track: V = sum Bp^T W Bp and
-// g_p = sum Bp^T W r, then damping and the 3x3 inverse (rbacore.cpp:1341-1344).
-// Pass 2 recomputes each observation's Jacobian (cheaper than reading W back)
-// and writes W = Bc^T W Bp and Y = W V^-1 into the entry's 32-element records
-// (R = double: 256 bytes; R = float, the FP32 form: 128 bytes).  Sums are FP64.
+// One thread per point, over the track: V = sum Bp^T W Bp and g_p = sum
+// Bp^T W r, then damping and the 3x3 inverse (rbacore.cpp:1341-1344).
+// Sums are FP64.
 template <typename R>
 __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
 {
@@ -244,95 +242,117 @@ __global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
     for (int j = 0; j < 9; ++j) a.Vinv[9 * i + j] = Vi[j];
 #pragma unroll
     for (int j = 0; j < 3; ++j) a.gp[3 * i + j] = g[j];
+}
+
+// One thread per track entry: recomputes the observation's Jacobian (cheaper
+// than storing it) and writes W = Bc^T W Bp and Y = W V^-1 into the entry's
+// 32-element records (R = double: 256 bytes; R = float, the FP32 form: 128
+// bytes).  Consecutive lanes write consecutive records, so a wavefront fills
+// whole cache lines.
+template <typename R>
+__global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.ntr) return;
+    const int i = a.tpt[q], k = a.tcam[q];
+    const double X0 = a.X[3 * i], X1 = a.X[3 * i + 1], X2 = a.X[3 * i + 2];
     R Vr[9];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) Vr[j] = (R)Vi[j];
-    for (int q = b; q < e; ++q) {
-        const int k = a.tcam[q];
-        CamV c;
-        load_cam(a.cam, k, c);
-        Ev ev;
-        eval_obs(c, X0, X1, X2, a.ox[q], a.oy[q], a.defdiff, ev);
-        R Bc[2][9], Bp[2][3];
-        jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
-        const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
-        R *Wo = (R *)a.Wrec + (size_t)q * kRec;
-        R *Yo = (R *)a.Yrec + (size_t)q * kRec;
+    for (int j = 0; j < 9; ++j) Vr[j] = (R)a.Vinv[9 * (size_t)i + j];
+    CamV c;
+    load_cam(a.cam, k, c);
+    Ev ev;
+    eval_obs(c, X0, X1, X2, a.ox[q], a.oy[q], a.defdiff, ev);
+    R Bc[2][9], Bp[2][3];
+    jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
+    const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
+    R *Wo = (R *)a.Wrec + (size_t)q * kRec;
+    R *Yo = (R *)a.Yrec + (size_t)q * kRec;
 #pragma unroll
-        for (int r = 0; r < 9; ++r) {
-            const R t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
-            const R t1 = Bc[0][r] * W2 + Bc[1][r] * W1;
-            const R w0 = t0 * Bp[0][0] + t1 * Bp[1][0];
-            const R w1 = t0 * Bp[0][1] + t1 * Bp[1][1];
-            const R w2 = t0 * Bp[0][2] + t1 * Bp[1][2];
-            Wo[r * 3] = w0;
-            Wo[r * 3 + 1] = w1;
-            Wo[r * 3 + 2] = w2;
+    for (int r = 0; r < 9; ++r) {
+        const R t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
+        const R t1 = Bc[0][r] * W2 + Bc[1][r] * W1;
+        const R w0 = t0 * Bp[0][0] + t1 * Bp[1][0];
+        const R w1 = t0 * Bp[0][1] + t1 * Bp[1][1];
+        const R w2 = t0 * Bp[0][2] + t1 * Bp[1][2];
+        Wo[r * 3] = w0;
+        Wo[r * 3 + 1] = w1;
+        Wo[r * 3 + 2] = w2;
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) Yo[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
-        }
-        // write the padding too: every cache line of the record is then fully
-        // written, and the L2 need not merge partial lines
+        for (int cc = 0; cc < 3; ++cc) Yo[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
+    }
+    // the padding too: every cache line of the record is fully written
 #pragma unroll
-        for (int j = 27; j < kRec; ++j) {
-            Wo[j] = (R)0;
-            Yo[j] = (R)0;
-        }
+    for (int j = 27; j < kRec; ++j) {
+        Wo[j] = (R)0;
+        Yo[j] = (R)0;
     }
 }
 
 // ---------------------------------------------------------------------------
 // diagonal Schur term per camera, atomic-free: the camera-major entry list is
 // cut into chunks of at most 64 entries of one image; one wavefront per chunk
-// accumulates D = sum Y W^T (81) and e = sum Y g_p (9), staging each entry's
-// Y, W and g_p through LDS.  k_lin_cameras adds the chunks of an image in
-// order.
+// forms D = sum Y W^T (9x9) and e = sum Y g_p (9) as one matrix product on
+// the matrix cores, [Y_1 .. Y_n] (9 x 3n) times [W_1 .. W_n | g_1 .. g_n]^T
+// (3n x 10: W^T in columns 0..8, g_p in column 9), with the lane layout of
+// k_schur_pairs.  k_lin_cameras adds the chunks of an image in order.
 // ---------------------------------------------------------------------------
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kSchurU = 8;  // MFMA steps per load round (32 K values, 10.7 entries or pairs)
 
 template <typename R>
 __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
 {
     const int ch = blockIdx.x;  // chunk: entries [ch_start[ch], ch_start[ch+1]) of one image
     const int lane = threadIdx.x;
-    __shared__ R st[8 * 54];
-    __shared__ double sg[8 * 3];
+    const int i16 = lane & 15, q = lane >> 4;
     const R *Yr = (const R *)a.Yrec, *Wr = (const R *)a.Wrec;
-    const int b = a.ch_start[ch], e = a.ch_start[ch + 1];
-    const int o0 = lane, o1 = lane + 64;
-    // output o < 81: D(o/9, o%9); 81 <= o < 90: e(o-81)
-    const int r0 = o0 / 9, c0 = o0 % 9;
-    const int r1 = (o1 < 81) ? o1 / 9 : (o1 < 90 ? o1 - 81 : 0), c1 = (o1 < 81) ? o1 % 9 : 0;
-    const bool g1 = (o1 >= 81);
-    double acc0 = 0., acc1 = 0.;
-    for (int j0 = b; j0 < e; j0 += 8) {
-        const int np = (e - j0 < 8) ? e - j0 : 8;
-        for (int x = lane; x < np * 54; x += 64) {
-            const int j = x / 54, idx = x % 54;
-            const int q = a.camq[j0 + j];
-            st[x] = (idx < 27) ? Yr[(size_t)q * kRec + idx] : Wr[(size_t)q * kRec + (idx - 27)];
-        }
-        if (lane < np * 3) {
-            const int j = lane / 3;
-            sg[lane] = a.gp[3 * (size_t)a.tpt[a.camq[j0 + j]] + lane % 3];
-        }
-        __syncthreads();
-        for (int j = 0; j < np; ++j) {
-            const R *Y = st + j * 54, *W = st + j * 54 + 27;
-            const double *g = sg + j * 3;
-            // products in FP64 (exact for FP32 records), as the off-diagonal blocks form them
-            acc0 += (double)Y[3 * r0] * (double)W[3 * c0] + (double)Y[3 * r0 + 1] * (double)W[3 * c0 + 1] +
-                    (double)Y[3 * r0 + 2] * (double)W[3 * c0 + 2];
-            if (g1)
-                acc1 += (double)Y[3 * r1] * g[0] + (double)Y[3 * r1 + 1] * g[1] + (double)Y[3 * r1 + 2] * g[2];
-            else
-                acc1 += (double)Y[3 * r1] * (double)W[3 * c1] + (double)Y[3 * r1 + 1] * (double)W[3 * c1 + 1] +
-                        (double)Y[3 * r1 + 2] * (double)W[3 * c1 + 2];
-        }
-        __syncthreads();
+    const int b = a.ch_start[ch], n = a.ch_start[ch + 1] - b;
+    __shared__ int sq[64];
+    __shared__ double sg[64 * 3];
+    if (lane < n) {
+        const int qq = a.camq[b + lane];
+        const int p = a.tpt[qq];
+        sq[lane] = qq;
+        sg[3 * lane] = a.gp[3 * (size_t)p];
+        sg[3 * lane + 1] = a.gp[3 * (size_t)p + 1];
+        sg[3 * lane + 2] = a.gp[3 * (size_t)p + 2];
     }
+    __syncthreads();
+    f64x4 acc = {0., 0., 0., 0.};
+    const int kend = 3 * n;
+    for (int k0 = 0; k0 < kend; k0 += 4 * kSchurU) {
+        double av[kSchurU], bv[kSchurU];
+#pragma unroll
+        for (int u = 0; u < kSchurU; ++u) {
+            const int kk = k0 + 4 * u + q;
+            double x = 0., y = 0.;
+            if (kk < kend) {
+                const int ent = kk / 3, cc = kk - 3 * ent;
+                const size_t rec = (size_t)sq[ent] * kRec + 3 * i16 + cc;
+                if (i16 < 9) {
+                    x = (double)Yr[rec];
+                    y = (double)Wr[rec];
+                } else if (i16 == 9) {
+                    y = sg[3 * ent + cc];
+                }
+            }
+            av[u] = x;
+            bv[u] = y;
+        }
+#pragma unroll
+        for (int u = 0; u < kSchurU; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+    }
+    // D[i][j] in lane j + 16 (i % 4), element i / 4
     double *D = a.Dd + (size_t)ch * 90;
-    D[o0] = acc0;
-    if (o1 < 90) D[o1] = acc1;
+    if (i16 < 10) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = q + 4 * v;
+            if (i < 9) D[(i16 < 9) ? i * 9 + i16 : 81 + i] = acc[v];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -431,10 +451,6 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 // K index kk = 3 p + c: pair p of the block, component c of the point.
 // FP32 records are widened to FP64 (products of two floats are exact).
 // ---------------------------------------------------------------------------
-
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-constexpr int kSchurU = 8;  // MFMA steps per load round (32 K values, 10.7 pairs)
 
 template <typename R>
 __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
@@ -1207,10 +1223,13 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
 void launch_lin_points(const LinArgs &a, hipStream_t st)
 {
     if (a.npts <= 0) return;
-    if (a.rec32)
+    if (a.rec32) {
         k_lin_points<float><<<nblk(a.npts, 128), 128, 0, st>>>(a);
-    else
+        k_lin_records<float><<<nblk(a.ntr, 128), 128, 0, st>>>(a);
+    } else {
         k_lin_points<double><<<nblk(a.npts, 128), 128, 0, st>>>(a);
+        k_lin_records<double><<<nblk(a.ntr, 128), 128, 0, st>>>(a);
+    }
 }
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
 {
