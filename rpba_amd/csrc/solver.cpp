@@ -113,13 +113,14 @@ enum Stage { ST_SETUP = 0, ST_ITER = 1, ST_DONE = 2 };
 struct Readback {
     double sc[2];
     int nflag, nanf;
-    PcgScalars pcg;
+    PcgScalars pcg[2];  // PCG scalars after chunk n, slot n & 1
     double cost, ws;
 };
 
 struct rpba_ctx {
     rpba_block *io = nullptr;
     Readback *hb = nullptr;          // pinned
+    hipEvent_t pcg_ev[2] = {nullptr, nullptr};  // end of PCG chunk n, slot n & 1
     double *hcam = nullptr;          // pinned, 15 per camera: stored [R|t] (12) and intrinsics (3)
     rpba_flags fl{};
     rpba_log_fn log = nullptr;
@@ -476,22 +477,34 @@ static int bundle_step(rpba_ctx *c)
     // launch j reads the scalars of slot j & 1 and writes slot (j + 1) & 1;
     // once the stopping rule fired, later launches only pass the state on.
     // The first chunk is sized from the previous solve's iteration count.
+    // Chunks are pipelined: chunk n + 1 is queued before the host reads the
+    // scalars after chunk n, so the GPU never waits for the host; at most one
+    // short chunk runs after the stop.
     PcgScalars hs{};
-    int chunk = std::max(4, c->pcg_last + 2), launched = 0;
+    int launched = 0, n = 0;
     const size_t first_timed = c->pending.size();
-    for (;;) {
-        for (int j = 0; j < chunk; ++j) {
+    auto queue_chunk = [&](int len) -> int {
+        for (int j = 0; j < len; ++j) {
             pa.j = launched + j;
             c->timed("spmv", [&] { launch_spmv(pa, c->st); }, pa.j);
             c->timed("pcg_update", [&] { launch_pcg_update(pa, c->st); }, pa.j);
         }
-        launched += chunk;
-        HIPCHK(hipMemcpyAsync(&c->hb->pcg, c->d_pcg.p + (launched & 1), sizeof(hs), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
-        hs = c->hb->pcg;
+        launched += len;
+        PcgScalars *slot = &c->hb->pcg[n & 1];
+        HIPCHK(hipMemcpyAsync(slot, c->d_pcg.p + (launched & 1), sizeof(PcgScalars), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipEventRecord(c->pcg_ev[n & 1], c->st));
+        ++n;
+        return RPBA_OK;
+    };
+    int rc = queue_chunk(std::max(2, c->pcg_last));
+    if (rc) return rc;
+    for (int m = 0;; ++m) {
+        rc = queue_chunk(3);
+        if (rc) return rc;
+        HIPCHK(hipEventSynchronize(c->pcg_ev[m & 1]));
+        hs = c->hb->pcg[m & 1];
         if (hs.done) break;
-        if (launched > pa.msize + 8) return RPBA_E_HIP;  // cannot happen: the kernel stops at msize
-        chunk = std::min(std::max(chunk, 8) * 2, 64);
+        if (launched > pa.msize + 16) return RPBA_E_HIP;  // cannot happen: the kernel stops at msize
     }
     // kernel timing counts the launches that ran an iteration (j < iters)
     for (size_t q = first_timed; q < c->pending.size(); ++q)
@@ -768,6 +781,8 @@ static int ctx_init(rpba_ctx *c)
 
     HIPCHK(hipGetDevice(&c->dev));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->pcg_ev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->pcg_ev[1], hipEventDisableTiming));
     if (hipHostMalloc((void **)&c->hb, sizeof(Readback), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->hcam, sizeof(double) * 15 * (size_t)nc, hipHostMallocDefault) != hipSuccess)
         return RPBA_E_NOMEM;
@@ -1014,6 +1029,8 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
 {
     if (!c) return;
     if (c->st) (void)hipStreamSynchronize(c->st);
+    for (auto &e : c->pcg_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->hb) (void)hipHostFree(c->hb);
     if (c->hcam) (void)hipHostFree(c->hcam);
     c->hb = nullptr;
