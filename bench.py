@@ -15,9 +15,9 @@ restart is timed too.  value = K / elapsed (LM iterations per second).
 
 Also reported: time to converged (one full solve: both passes, from the
 uploaded block to the stored state on the device), the final sigma0 and RMS,
-the chosen form (FP32 Jacobians and W V^-1 W^T products, FP64 sums, S, PCG
-and cost): its time to reach the plain form's final sigma0 and its errors
-against the plain FP64 result, the per-kernel HIP-event roofline of the
+the chosen form (FP32 Jacobians and W, Y = W V^-1 records, the matrix-free
+Schur product; FP64 sums, diagonal S blocks, PCG and cost): its time to reach
+the plain form's final sigma0 and its errors against the plain FP64 result, the per-kernel HIP-event roofline of the
 dominant kernel, and the CPU oracle timed on one LM iteration of the same
 scene (cpu_baseline).
 """
@@ -51,6 +51,8 @@ def algorithmic_bytes(kernel, sz):
     """Compulsory HBM bytes of one launch: every array the kernel must touch,
     once.  sz: cams, pts, obs, pairs, blocks."""
     Nc, Np, No, P, B = sz["cams"], sz["pts"], sz["obs"], sz["pairs"], sz["blocks"]
+    if kernel == "spmv" and B == 0:  # matrix-free: W and Y records once, t per entry out and in, S_kk and D_k
+        return 2 * 256 * No + 48 * No + 1296 * Nc + 144 * Nc
     if kernel == "spmv":  # S blocks once (upper + diagonal), d in, Ad out, row partials
         return 648 * (B + Nc) + 72 * Nc + 72 * Nc + 8 * Nc
     if kernel == "schur_pairs":  # Y and W of every entry, pair lists, block offsets, S blocks out
@@ -80,7 +82,7 @@ def run_bench(args):
     t0 = time.time()
     sc = scenes.generate(args.config)
     gen_s = time.time() - t0
-    flags = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1)
+    flags = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, schur=args.schur)
     t0 = time.time()
     ctx = api.Context(sc, flags)
     create_s = time.time() - t0
@@ -112,13 +114,18 @@ def run_bench(args):
     rc_d, out, st = ctx.download()
     sizes = ctx.sizes()
 
-    # the chosen form: FP32 Jacobians and W V^-1 W^T records, FP64 sums, S,
-    # PCG and cost; time until its pass-1 stored sigma0 reaches the plain
-    # form's final value (x (1 + 1e-6)), and its errors against plain FP64
+    # the chosen form: FP32 Jacobians and W, Y = W V^-1 records with the
+    # matrix-free Schur product (FP64 sums, diagonal S blocks, PCG and cost);
+    # time until its pass-1 stored sigma0 reaches the plain form's final value
+    # (x (1 + 1e-6)), and its errors against plain FP64
     chosen = None
     if not args.no_chosen:
         target = st["sig0_final"] * (1.0 + 1e-6)
-        cctx = api.Context(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, precision=1))
+        # matrix-free when one PCG iteration streams fewer bytes through the
+        # FP32 W and Y records (2 x 128 B per observation) than through the
+        # stored S blocks (648 B each)
+        mf = 2 * 128 * sizes["obs"] < 648 * sizes["blocks"]
+        cctx = api.Context(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, precision=1, schur=int(mf)))
         cctx.run(1)
         cctx.reset()
         cttc, crecs = timed_solve(cctx)
@@ -127,7 +134,10 @@ def run_bench(args):
         reach = next((t for t, ps, sg in crecs if ps == 1 and sg <= target), None)
         f64 = lambda a: np.asarray(a, dtype=np.float64)
         chosen = {
-            "form": "precision=1: FP32 Jacobians and W, Y = W V^-1 records; FP64 residuals, sums, S, PCG, cost",
+            "form": ("precision=1, schur=1: FP32 Jacobians and W, Y = W V^-1 records; matrix-free Schur product "
+                     "(off-diagonal S never formed); FP64 residuals, sums, diagonal S blocks, PCG, cost") if mf else
+                    ("precision=1, schur=0: FP32 Jacobians and W, Y = W V^-1 records; S formed block by block "
+                     "(the records stream more bytes per PCG iteration than S here); FP64 residuals, sums, S, PCG, cost"),
             "time_to_converged_s": round(cttc, 4),
             "time_to_reach_plain_sigma0_s": None if reach is None else round(reach, 4),
             "speedup_vs_plain": None if reach is None else round(ttc / reach, 3),
@@ -173,6 +183,7 @@ def run_bench(args):
 
     line = {
         "metric": "LM iters/s, plain FP64 robust LM Schur solve (time-to-converged reported beside)",
+        "form": "plain FP64, " + ("matrix-free Schur product" if args.schur else "S formed block by block"),
         "value": round(args.steps / elapsed, 3),
         "unit": "LM iters/s",
         "n_gpus": 1,
@@ -267,8 +278,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2_961")
+    ap.add_argument("--schur", type=int, default=0, choices=(0, 1),
+                    help="plain FP64 form with S formed (0, default) or the matrix-free Schur product (1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-chosen", action="store_true", help="skip the FP32-Jacobian form")
+    ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (FP32, matrix-free) form")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run device-to-device copies of this many bytes (counter calibration)")
     args = ap.parse_args()

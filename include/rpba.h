@@ -87,6 +87,10 @@ typedef struct rpba_flags {
     int32_t preconditioner;  /* 0 = block-Jacobi (rbacore.cpp:1838-1846)        */
     int32_t force_global;    /* 1 = run the global LSA even where the reference
                                 would dispatch to the partitioned path          */
+    int32_t schur;           /* 0 = S formed block by block (the reference's
+                                profile store); 1 = matrix-free: only the diagonal
+                                blocks are formed and the PCG multiplies through
+                                W and Y = W V^-1 (no covisibility pairs built)  */
 } rpba_flags;
 
 /* Optional per-solve report.  sigma values are sqrt(sig0 * ihw2) exactly as the

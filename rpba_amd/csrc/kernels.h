@@ -35,6 +35,8 @@ struct LinArgs {
     int rec32;
     double *Vinv, *gp, *Sd, *bvec;
     double *Dd;  // per chunk: diagonal Schur term (81) and Y g_p (9)
+    double *Dk;  // per camera, matrix-free form only (else null): D_k = sum of its chunks' terms (81)
+    const int *yrank;  // matrix-free form: Y records in camera-major order, entry q at yrank[q] (else null)
     int nch;     // chunks of the camera-major list (at most 64 entries of one image each)
     const int *ch_start, *ch_off;  // chunk -> first entry (nch + 1); image -> first chunk (nc + 1)
 };
@@ -56,10 +58,21 @@ struct PcgArgs {
     double *Ad;       // S_kk d_k per camera (diagonal products of the SpMV)
     double *part;     // per camera: |alpha S d|^2, r.(h1 - h), r.h1 shares
     double *yN, *yT;  // SpMV row-run sums (at the run's first block) and column shares (column order)
-    double *dpart;    // per SpMV workgroup: its share of d.S d
-    int nblk, ndpart; // upper blocks; SpMV workgroups (spmv_grid)
+    double *dpart;    // per SpMV workgroup (or per chunk, matrix-free): its share of d.S d
+    int nblk, ndpart; // upper blocks; number of dpart shares
+    const double *dsum;  // what k_pcg_a adds up for d.S d: dpart, or (matrix-free) its total at dpart[nch]
+    int ndsum;
     PcgScalars *S;  // two slots: launch j reads S[j & 1] and writes S[(j + 1) & 1]
     int j;          // launch index inside one PCG solve
+    // matrix-free product S d = (S_kk + D_k) d_k - sum_(entries q of k) Y_q t_pt(q),
+    // t_i = sum_(entries q of i) W_q^T d_cam(q), with S never formed off the diagonal
+    int implicit, npts, ntr, nch, rec32, npgrp;
+    const int *toff, *tcam, *tpt, *camq, *camrank, *ch_start, *ch_off;
+    const int *pgrp;  // point groups (first point of group c; npgrp + 1 entries)
+    const void *Wrec, *Yrec;
+    const double *Dk;  // per camera: D_k = sum_q Y_q W_q^T (81)
+    double *tc;        // per entry, camera-major: t of the entry's point (3)
+    double *cpart;     // per chunk: sum_q Y_q t_pt(q) (9)
 };
 
 struct StepArgs {

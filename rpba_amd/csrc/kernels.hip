@@ -204,6 +204,31 @@ __device__ __forceinline__ void jac_point(const CamV &c, const Ev &e, T Bp[2][3]
     }
 }
 
+// the 27 used elements of a 32-element record (aligned to its size) with
+// 16-byte loads
+__device__ __forceinline__ void load_rec27(const float *p, float *v)
+{
+    const float4 *q = (const float4 *)p;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const float4 x = q[j];
+        v[4 * j] = x.x;
+        if (4 * j + 1 < 28) v[4 * j + 1] = x.y;
+        if (4 * j + 2 < 28) v[4 * j + 2] = x.z;
+        if (4 * j + 3 < 28) v[4 * j + 3] = x.w;
+    }
+}
+__device__ __forceinline__ void load_rec27(const double *p, double *v)
+{
+    const double2 *q = (const double2 *)p;
+#pragma unroll
+    for (int j = 0; j < 14; ++j) {
+        const double2 x = q[j];
+        v[2 * j] = x.x;
+        v[2 * j + 1] = x.y;
+    }
+}
+
 // One thread per point, over the track: V = sum Bp^T W Bp and g_p = sum
 // Bp^T W r, then damping and the 3x3 inverse (rbacore.cpp:1341-1344).
 // Sums are FP64.
@@ -267,7 +292,7 @@ __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
     jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
     const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
     R *Wo = (R *)a.Wrec + (size_t)q * kRec;
-    R *Yo = (R *)a.Yrec + (size_t)q * kRec;
+    R *Yo = (R *)a.Yrec + (size_t)(a.yrank ? a.yrank[q] : q) * kRec;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const R t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
@@ -331,8 +356,9 @@ __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
             if (kk < kend) {
                 const int ent = kk / 3, cc = kk - 3 * ent;
                 const size_t rec = (size_t)sq[ent] * kRec + 3 * i16 + cc;
+                const size_t yrec = a.yrank ? (size_t)(b + ent) * kRec + 3 * i16 + cc : rec;
                 if (i16 < 9) {
-                    x = (double)Yr[rec];
+                    x = (double)Yr[yrec];
                     y = (double)Wr[rec];
                 } else if (i16 == 9) {
                     y = sg[3 * ent + cc];
@@ -439,6 +465,7 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
     __syncthreads();
     if (t < 81) a.Sd[81 * (size_t)k + t] = Ssh[t];
     if (t < 9) a.bvec[9 * (size_t)k + t] = bsh[t];
+    if (a.Dk && t < 81) a.Dk[81 * (size_t)k + t] = D[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -726,6 +753,151 @@ __device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
     return block_sum(s, sh);
 }
 
+// Matrix-free S d (no off-diagonal S stored), three launches:
+// k_mf_points: t_i = sum over its entries q of W_q^T d_cam(q), stored for every
+// entry of the point at its camera-major position.
+// k_mf_chunks: one wavefront per chunk (at most 64 entries of one image):
+// lane e forms Y_q t_pt(q) for entry q, the wavefront sums the 9 values in
+// lane order into cpart[chunk]; the chunk's share of d.S d is
+// -d_k.cpart[chunk], plus d_k.(S_kk + D_k) d_k on the image's first chunk,
+// whose lanes also store (S_kk + D_k) d_k in Ad[k].  k_pcg_a then forms
+// (S d)_k = Ad_k - sum of the image's chunk sums, in chunk order.
+// k_mf_dsum adds the chunk shares of d.S d in a fixed order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
+{
+    // points whose first entry lies in [256 c, 256 c + 256): their entries are
+    // taken 256 at a time, one per lane (u_q = W_q^T d_cam(q) to LDS), and each
+    // point's lane adds its entries in order; t_i then goes to every entry of
+    // the point at its camera-major position (tc), which k_mf_chunks streams.
+    if (a.S[a.j & 1].done) return;
+    __shared__ double su[256][3];
+    const int t = threadIdx.x, c = blockIdx.x;
+    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    if (p0 >= p1) return;
+    const int e0 = a.toff[p0], e1 = a.toff[p1];
+    const int np = p1 - p0;  // at most 128 + 1: a point has at least two entries
+    int pb = 0, pe = 0;
+    if (t < np) {
+        pb = a.toff[p0 + t];
+        pe = a.toff[p0 + t + 1];
+    }
+    double t0 = 0., t1 = 0., t2 = 0.;
+    const R *Wr = (const R *)a.Wrec;
+    for (int base = e0; base < e1; base += 256) {
+        const int q = base + t;
+        if (q < e1) {
+            R W[28];
+            load_rec27(Wr + (size_t)q * kRec, W);
+            const double *dv = a.d + 9 * (size_t)a.tcam[q];
+            double u0 = 0., u1 = 0., u2 = 0.;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const double dr = dv[r];
+                u0 += (double)W[3 * r] * dr;
+                u1 += (double)W[3 * r + 1] * dr;
+                u2 += (double)W[3 * r + 2] * dr;
+            }
+            su[t][0] = u0;
+            su[t][1] = u1;
+            su[t][2] = u2;
+        }
+        __syncthreads();
+        if (t < np) {
+            const int lo = max(pb, base), hi = min(pe, base + 256);
+            for (int x = lo; x < hi; ++x) {
+                t0 += su[x - base][0];
+                t1 += su[x - base][1];
+                t2 += su[x - base][2];
+            }
+        }
+        __syncthreads();
+    }
+    // every entry of the group copies its point's t to its camera-major slot
+    __shared__ double st[130][3];
+    if (t < np) {
+        st[t][0] = t0;
+        st[t][1] = t1;
+        st[t][2] = t2;
+    }
+    __syncthreads();
+    for (int base = e0; base < e1; base += 256) {
+        const int q = base + t;
+        if (q < e1) {
+            const int j = a.tpt[q] - p0;
+            double *o = a.tc + 3 * (size_t)a.camrank[q];
+            o[0] = st[j][0];
+            o[1] = st[j][1];
+            o[2] = st[j][2];
+        }
+    }
+}
+
+// total of the chunk shares into dpart[nch], one workgroup, fixed order
+__global__ __launch_bounds__(1024) void k_mf_dsum(PcgArgs a)
+{
+    __shared__ double sh[16];
+    if (a.S[a.j & 1].done) return;
+    const double s = sum_fixed(a.dpart, a.nch, sh);
+    if (threadIdx.x == 0) a.dpart[a.nch] = s;
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_mf_chunks(PcgArgs a)
+{
+    if (a.S[a.j & 1].done) return;
+    const int ch = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int b = a.ch_start[ch], n = a.ch_start[ch + 1] - b;
+    const R *Yr = (const R *)a.Yrec;
+    double v[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) v[r] = 0.;
+    if (lane < n) {
+        // Y records and t of the entry's point, both in camera-major order
+        const double *tq = a.tc + 3 * (size_t)(b + lane);
+        const double t0 = tq[0], t1 = tq[1], t2 = tq[2];
+        R Y[28];
+        load_rec27(Yr + (size_t)(b + lane) * kRec, Y);
+#pragma unroll
+        for (int r = 0; r < 9; ++r) v[r] = (double)Y[3 * r] * t0 + (double)Y[3 * r + 1] * t1 + (double)Y[3 * r + 2] * t2;
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) v[r] = wave_sum(v[r]);
+    const int k = a.tcam[a.camq[b]];
+    const double *dk = a.d + 9 * (size_t)k;
+    if (lane < 9) {
+        double vr = v[0];
+#pragma unroll
+        for (int r = 1; r < 9; ++r)
+            if (lane == r) vr = v[r];
+        a.cpart[9 * (size_t)ch + lane] = vr;
+    }
+    double share = 0.;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) share -= dk[r] * v[r];
+    if (ch == a.ch_off[k]) {
+        // (S_kk + D_k) d_k, row lane
+        __shared__ double ed[9];
+        if (lane < 9) {
+            const double *S = a.Sd + 81 * (size_t)k + 9 * lane;
+            const double *D = a.Dk + 81 * (size_t)k + 9 * lane;
+            double y = 0.;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) y += (S[c] + D[c]) * dk[c];
+            a.Ad[9 * (size_t)k + lane] = y;
+            ed[lane] = y;
+        }
+        __syncthreads();
+        double e = 0.;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) e += dk[r] * ed[r];
+        share += e;
+    }
+    if (lane == 0) a.dpart[ch] = share;
+}
+
+
 // PCG step, part 1 (one workgroup per camera k): alpha = rho / d.S d from the
 // tile partials; (S d)_k = yD_k + the run sums of row k + the column shares of
 // column k, in a fixed order; x += alpha d, r -= alpha S d, h1 = C r; the
@@ -735,16 +907,25 @@ __device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S
     __shared__ double sh[4];
     __shared__ double red[28][9];
     __shared__ double rs[9], p3[3][9];
-    const double dAd = sum_fixed(a.dpart, a.ndpart, sh);
+    const double dAd = sum_fixed(a.dsum, a.ndsum, sh);
     const double rho = S.rho;
     const double alpha = (fabs(rho) < 1.e-20 || fabs(rho) > 1.e20) ? 0. : rho / dAd;
     const int t = threadIdx.x;
     const int g = t / 9, r = t % 9;
+    if (a.implicit) {
+        // (S d)_k = (S_kk + D_k) d_k - the chunk sums of image k, in chunk order
+        if (g < 28) {
+            double acc = 0.;
+            if (g == 0)
+                for (int ch = a.ch_off[k]; ch < a.ch_off[k + 1]; ++ch) acc += a.cpart[9 * (size_t)ch + r];
+            red[g][r] = (g == 0) ? a.Ad[9 * (size_t)k + r] - acc : 0.;
+        }
+    } else if (g < 28) {
     const int r0 = a.row_off[k], r1 = a.row_off[k + 1];
     const int c0 = a.col_off[k], nl = a.col_off[k + 1] - c0;
     // runs of row k: one per tile that holds blocks of the row
     const int t0 = r0 / kTile, nrun = (r1 > r0) ? (r1 - 1) / kTile - t0 + 1 : 0;
-    if (g < 28) {
+    {
         double acc = (g == 0) ? a.Ad[9 * (size_t)k + r] : 0.;
         for (int it = g; it < nrun; it += 28) {
             const int s0 = max(r0, (t0 + it) * kTile);
@@ -763,6 +944,7 @@ __device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S
         }
         for (; it < nl; it += 28) acc += yc[9 * (size_t)it];
         red[g][r] = acc;
+    }
     }
     __syncthreads();
     const size_t j = 9 * (size_t)k + t;
@@ -1274,7 +1456,22 @@ int spmv_grid(int nblk)
     (void)nblk;
     return 4 * device_cus();
 }
-void launch_spmv(const PcgArgs &a, hipStream_t st) { k_spmv_tiles<<<a.ndpart, 256, 0, st>>>(a); }
+void launch_spmv(const PcgArgs &a, hipStream_t st)
+{
+    if (!a.implicit) {
+        k_spmv_tiles<<<a.ndpart, 256, 0, st>>>(a);
+        return;
+    }
+    if (a.npgrp <= 0 || a.nch <= 0) return;
+    if (a.rec32) {
+        k_mf_points<float><<<a.npgrp, 256, 0, st>>>(a);
+        k_mf_chunks<float><<<a.nch, 64, 0, st>>>(a);
+    } else {
+        k_mf_points<double><<<a.npgrp, 256, 0, st>>>(a);
+        k_mf_chunks<double><<<a.nch, 64, 0, st>>>(a);
+    }
+    k_mf_dsum<<<1, 1024, 0, st>>>(a);
+}
 int pcg_groups(int nc) { return (nc + 27) / 28; }
 void launch_pcg_update(const PcgArgs &a, hipStream_t st)
 {

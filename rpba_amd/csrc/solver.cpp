@@ -141,6 +141,7 @@ struct rpba_ctx {
     std::vector<double> Pst_init, Ist_init, R0, C0;
     bool structure_dirty = true;     // pairs/blocks must be rebuilt
     int rec32 = 0;                   // FP32 Jacobians and W/Y records (precision 1)
+    bool implicit = false;           // matrix-free S product (schur 1)
     DevStruct ds;                    // device tracks, camera-major order, pairs, blocks
 
     // device buffers
@@ -150,6 +151,7 @@ struct rpba_ctx {
     DBuf<unsigned char> d_Wrec, d_Yrec;  // element type by precision
     DBuf<double> d_Su;
     DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_partial, d_scal, d_Dd, d_ppart, d_yN, d_yT, d_dpart;
+    DBuf<double> d_t, d_cpart, d_Dk;  // matrix-free form
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -284,7 +286,12 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
     rc |= c->d_yN.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
     rc |= c->d_yT.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
-    rc |= c->d_dpart.alloc((size_t)spmv_grid(c->ds.nblk));
+    rc |= c->d_dpart.alloc((size_t)std::max(spmv_grid(c->ds.nblk), c->ds.nch + 1));
+    if (c->implicit) {
+        rc |= c->d_t.alloc(3 * std::max<size_t>(ntr, 1));
+        rc |= c->d_cpart.alloc(9 * (size_t)std::max(c->ds.nch, 1));
+        rc |= c->d_Dk.alloc(81 * (size_t)c->nc);
+    }
     const int nb = std::max(cost_blocks(c->ds.ntr), (int)((np + 255) / 256 + 1));
     rc |= c->d_partial.alloc((size_t)nb);
     rc |= c->d_ipartial.alloc((size_t)nb);
@@ -398,6 +405,8 @@ static LinArgs lin_args(rpba_ctx *c)
     a.Wrec = c->d_Wrec.p; a.Yrec = c->d_Yrec.p; a.rec32 = c->rec32;
     a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.Sd = c->d_Sd.p; a.bvec = c->d_b.p;
     a.Dd = c->d_Dd.p; a.nch = c->ds.nch; a.ch_start = c->ds.ch_start; a.ch_off = c->ds.ch_off;
+    a.Dk = c->implicit ? c->d_Dk.p : nullptr;
+    a.yrank = c->implicit ? c->ds.camrank : nullptr;
     return a;
 }
 
@@ -410,8 +419,16 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.col_pos = c->ds.cpos; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
     a.x = c->d_x.p; a.best = c->d_best.p; a.r = c->d_r.p; a.d = c->d_d.p; a.Ad = c->d_Ad.p;
     a.h = c->d_h.p; a.h1 = c->d_h1.p; a.part = c->d_ppart.p; a.S = c->d_pcg.p;
-    a.yN = c->d_yN.p; a.yT = c->d_yT.p; a.dpart = c->d_dpart.p; a.nblk = c->ds.nblk; a.ndpart = spmv_grid(c->ds.nblk);
+    a.yN = c->d_yN.p; a.yT = c->d_yT.p; a.dpart = c->d_dpart.p; a.nblk = c->ds.nblk;
+    a.ndpart = c->implicit ? c->ds.nch : spmv_grid(c->ds.nblk);
+    a.dsum = c->implicit ? c->d_dpart.p + c->ds.nch : c->d_dpart.p;
+    a.ndsum = c->implicit ? 1 : a.ndpart;
     a.j = 0;
+    a.implicit = c->implicit ? 1 : 0; a.npts = c->ds.npts; a.ntr = c->ds.ntr; a.nch = c->ds.nch; a.rec32 = c->rec32;
+    a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.tpt = c->ds.tpt; a.camq = c->ds.camq; a.camrank = c->ds.camrank;
+    a.pgrp = c->ds.pgrp; a.npgrp = c->ds.npgrp;
+    a.ch_start = c->ds.ch_start; a.ch_off = c->ds.ch_off;
+    a.Wrec = c->d_Wrec.p; a.Yrec = c->d_Yrec.p; a.Dk = c->d_Dk.p; a.tc = c->d_t.p; a.cpart = c->d_cpart.p;
     return a;
 }
 
@@ -468,7 +485,7 @@ static int bundle_step(rpba_ctx *c)
     sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
     sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.rec32 = c->rec32;
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
-    c->timed("schur_pairs", [&] { launch_schur_pairs(sa, c->st); });
+    if (!c->implicit) c->timed("schur_pairs", [&] { launch_schur_pairs(sa, c->st); });
     c->timed("schur_diag", [&] { launch_schur_diag(la, c->st); });
     c->timed("lin_cameras", [&] { launch_lin_cameras(la, c->st); });
     c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
@@ -711,6 +728,8 @@ static int ctx_init(rpba_ctx *c)
     c->nc = nc;
     // precision 1: FP32 Jacobians and W/Y records, FP64 sums, S, PCG and cost
     c->rec32 = c->fl.precision >= 1;
+    c->implicit = c->fl.schur == 1;
+    c->ds.with_pairs = !c->implicit;
     c->addf[0] = c->fl.est_k2 != 0;
     c->addf[1] = c->fl.est_k4 != 0;
     c->addf[2] = c->fl.est_f != 0;
@@ -835,6 +854,7 @@ static bool partition_dispatch(const rpba_flags *f, int nc)
 extern "C" int rpba_ctx_create(rpba_ctx **out, rpba_block *io, const rpba_flags *flags, rpba_log_fn log, void *user)
 {
     if (!out || !io || !flags || io->n_cams < 1 || io->n_pts < 0 || io->n_obs < 0) return RPBA_E_INVALID;
+    if (flags->precision < 0 || flags->precision > 1 || flags->schur < 0 || flags->schur > 1) return RPBA_E_INVALID;
     if (!io->obs_cam || !io->obs_pt || !io->obs_xy || !io->cam_P || !io->cam_f || !io->cam_d || !io->pts)
         return RPBA_E_INVALID;
     int ndev = 0;
@@ -1055,6 +1075,9 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_yN.release();
     c->d_yT.release();
     c->d_dpart.release();
+    c->d_t.release();
+    c->d_cpart.release();
+    c->d_Dk.release();
     c->d_ppart.release();
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;

@@ -31,6 +31,13 @@ struct DevStruct {
     // camera-major order: entries of image k are camq[camoff[k] .. camoff[k+1])
     int *camoff = nullptr, *camq = nullptr;
     size_t camoff_cap = 0, camq_cap = 0;
+    int *camrank = nullptr;  // entry -> its position in camera-major order (camrank[camq[j]] = j)
+    size_t camrank_cap = 0;
+    // point groups of the matrix-free product: group c holds the points whose
+    // first entry lies in [256 c, 256 c + 256): points pgrp[c] .. pgrp[c+1]-1
+    int *pgrp = nullptr;
+    size_t pgrp_cap = 0;
+    int npgrp = 0;
     // chunks of at most kChunk entries of one image over camq: ch_start (nch + 1
     // entry offsets), ch_off (nc + 1 first chunk of each image)
     static constexpr int kChunk = 64;
@@ -63,6 +70,7 @@ struct DevStruct {
     // rebuild: emap (old entry -> new entry or -1), flags and their scans
     int *emap = nullptr, *sflag = nullptr, *spos = nullptr, *cq2 = nullptr;
     size_t emap_cap = 0, sflag_cap = 0, spos_cap = 0, cq2_cap = 0;
+    bool with_pairs = true;  // build the covisibility pairs and S blocks (not needed by the matrix-free form)
     bool updated = false;    // the next build() has nothing to do
     int updated_empty = -1;  // first image without entries after that update
     int npts0 = 0, ntr0 = 0;
@@ -82,6 +90,7 @@ struct DevStruct {
     int build(hipStream_t st, int *first_empty);
     int chunks_from_camoff(hipStream_t st, int *first_empty);
     int update_after_delete(int ntr_old, hipStream_t st);
+    int point_groups(hipStream_t st);
     // RobustDelete on the flags of the stored state; X (accepted points) is
     // compacted in place
     int robust_delete(const unsigned char *flag, int min_keep, double *X, hipStream_t st);

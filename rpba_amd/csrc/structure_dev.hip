@@ -152,6 +152,14 @@ __global__ void k_set_last(int *a, int idx, int v)
 
 static inline unsigned nb_(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+int DevStruct::point_groups(hipStream_t st)
+{
+    npgrp = (ntr + 255) / 256;
+    if (grow(pgrp, pgrp_cap, (size_t)npgrp + 1)) return RPBA_E_NOMEM;
+    k_lower_bound<<<nb_(npgrp + 1, 256), 256, 0, st>>>(npgrp, 256u, (const unsigned *)toff, npts, pgrp);
+    return RPBA_OK;
+}
+
 // chunks of at most kChunk entries of one image over the camera-major list,
 // from camoff; reports the first image without entries
 int DevStruct::chunks_from_camoff(hipStream_t st, int *first_empty)
@@ -211,8 +219,16 @@ int DevStruct::build(hipStream_t st, int *first_empty)
         DCHK(rocprim::radix_sort_pairs(tmp, bytes, (const unsigned *)tcam, u32a, i32a, camq, ntr, 0u, cb, st));
     }
     k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, 1u, u32a, ntr, camoff);
+    if (grow(camrank, camrank_cap, (size_t)std::max(ntr, 1))) return RPBA_E_NOMEM;
+    if (ntr > 0) k_invert<<<nb_(ntr, 256), 256, 0, st>>>(ntr, camq, camrank);
+    if (point_groups(st)) return RPBA_E_NOMEM;
     rc = chunks_from_camoff(st, first_empty);
     if (rc || *first_empty >= 0) return rc;
+    if (!with_pairs) {
+        npairs = 0;
+        nblk = 0;
+        return RPBA_OK;
+    }
 
     // --- covisibility pairs: count, scan, generate, stable sort by block ---
     rc |= grow(ppoff, ppoff_cap, (size_t)npts + 1);
@@ -460,15 +476,18 @@ int DevStruct::update_after_delete(int ntr_old, hipStream_t st)
     rc |= grow(cq2, cq2_cap, (size_t)std::max(ntr, 1));
     if (rc) return RPBA_E_NOMEM;
     size_t bytes = 0;
-    // pairs
-    k_remap_pairs<<<nb_(P + 1, 256), 256, 0, st>>>(P, pab, emap, val_in, sflag);
-    DCHK(rocprim::exclusive_scan(nullptr, bytes, sflag, spos, 0, (size_t)P + 1, rocprim::plus<int>(), st));
-    if (scratch(bytes)) return RPBA_E_NOMEM;
-    DCHK(rocprim::exclusive_scan(tmp, bytes, sflag, spos, 0, (size_t)P + 1, rocprim::plus<int>(), st));
-    if (P > 0) k_compact_pairs<<<nb_(P, 256), 256, 0, st>>>(P, val_in, sflag, spos, pab);
-    k_remap_offsets<<<nb_(nblk + 1, 256), 256, 0, st>>>(nblk, poff, spos);
     if (pinned(2)) return RPBA_E_NOMEM;
-    DCHK(hipMemcpyAsync(hpin, spos + P, sizeof(int), hipMemcpyDeviceToHost, st));
+    hpin[0] = 0;
+    if (with_pairs) {
+        // pairs
+        k_remap_pairs<<<nb_(P + 1, 256), 256, 0, st>>>(P, pab, emap, val_in, sflag);
+        DCHK(rocprim::exclusive_scan(nullptr, bytes, sflag, spos, 0, (size_t)P + 1, rocprim::plus<int>(), st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::exclusive_scan(tmp, bytes, sflag, spos, 0, (size_t)P + 1, rocprim::plus<int>(), st));
+        if (P > 0) k_compact_pairs<<<nb_(P, 256), 256, 0, st>>>(P, val_in, sflag, spos, pab);
+        k_remap_offsets<<<nb_(nblk + 1, 256), 256, 0, st>>>(nblk, poff, spos);
+        DCHK(hipMemcpyAsync(hpin, spos + P, sizeof(int), hipMemcpyDeviceToHost, st));
+    }
     // camera-major order
     k_remap_camq<<<nb_(ntr_old + 1, 256), 256, 0, st>>>(ntr_old, camq, emap, sflag);
     DCHK(rocprim::exclusive_scan(nullptr, bytes, sflag, spos, 0, (size_t)ntr_old + 1, rocprim::plus<int>(), st));
@@ -480,6 +499,8 @@ int DevStruct::update_after_delete(int ntr_old, hipStream_t st)
     npairs = hpin[0];
     std::swap(camq, cq2);
     std::swap(camq_cap, cq2_cap);
+    if (ntr > 0) k_invert<<<nb_(ntr, 256), 256, 0, st>>>(ntr, camq, camrank);
+    if (point_groups(st)) return RPBA_E_NOMEM;
     rc = chunks_from_camoff(st, &updated_empty);
     if (rc) return rc;
     updated = true;
@@ -624,7 +645,7 @@ void DevStruct::release()
         p = nullptr;
         c = 0;
     };
-    fr(camoff, camoff_cap); fr(camq, camq_cap); fr(ch_start, ch_start_cap); fr(ch_off, ch_off_cap); fr(u32a, u32a_cap); fr(i32a, i32a_cap); fr(i32b, i32b_cap);
+    fr(camoff, camoff_cap); fr(camq, camq_cap); fr(camrank, camrank_cap); fr(pgrp, pgrp_cap); fr(ch_start, ch_start_cap); fr(ch_off, ch_off_cap); fr(u32a, u32a_cap); fr(i32a, i32a_cap); fr(i32b, i32b_cap);
     fr(ppoff, ppoff_cap); fr(key_in, key_in_cap); fr(key_out, key_out_cap); fr(val_in, val_in_cap);
     fr(pab, pab_cap); fr(ukey, ukey_cap); fr(ucnt, ucnt_cap); fr(nruns, nruns_cap); fr(poff, poff_cap);
     fr(bk, bk_cap); fr(bl, bl_cap); fr(roff, roff_cap); fr(coff, coff_cap); fr(cblk, cblk_cap); fr(cpos, cpos_cap);

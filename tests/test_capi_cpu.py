@@ -86,3 +86,11 @@ def test_parameters_reader(tmp_path):
 def test_invalid_arguments_are_rejected_without_gpu():
     L = rpba_amd.lib()
     assert L.rpba_calc_plsa(None, None, capi.LOG_FN(0), None, None) == capi.RPBA_E_INVALID
+
+
+def test_unknown_form_selectors_are_rejected_without_gpu(tiny_scene):
+    """precision and schur outside {0, 1} fail before any device call."""
+    from rpba_amd import api
+    for kw in ({"precision": 2}, {"schur": 2}, {"schur": -1}):
+        rc, _, _, _ = api.calc_plsa(tiny_scene, capi.make_flags(**kw))
+        assert rc == capi.RPBA_E_INVALID, kw

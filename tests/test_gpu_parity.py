@@ -130,3 +130,15 @@ def test_partition_path_is_refused():
     rc, _, st, _ = api.calc_plsa(sc, capi.make_flags(threads=0, force_global=1))
     assert rc == 0 and st["passes_done"] == 2
 
+
+
+def test_matrix_free_schur_matches_oracle(scene_c1):
+    """schur = 1 never forms the off-diagonal blocks; the oracle forms S block
+    by block as the reference does.  The product is the same sum in another
+    order, so the same bar applies."""
+    _cmp_solves(scene_c1, capi.make_flags(robust=1, schur=1), "c1 matrix-free")
+
+
+def test_matrix_free_schur_variants_match_oracle():
+    sc = scenes.generate(None, seed=31, n_cams=12, n_pts=600, track="uniform2_8", outlier_frac=0.03)
+    _cmp_solves(sc, capi.make_flags(robust=0, k2=1, k4=0, f=1, schur=1), "matrix-free r0 k2 f")
