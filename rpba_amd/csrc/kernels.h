@@ -76,9 +76,10 @@ struct PcgArgs {
 };
 
 struct StepArgs {
-    int nc, npts;
+    int nc, npts, npgrp;
     CamPtrs cam;
     const int *toff, *tcam;
+    const int *pgrp;  // point groups (first point of group c)
     const void *Wrec;
     int rec32;
     const double *Vinv, *gp, *sol, *X;

@@ -229,6 +229,32 @@ __device__ __forceinline__ void load_rec27(const double *p, double *v)
     }
 }
 
+// a whole 32-element record (27 values, then zero padding) with 16-byte stores
+__device__ __forceinline__ void store_rec32(float *p, const float *v)
+{
+    float4 *q = (float4 *)p;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float4 x;
+        x.x = (4 * j < 27) ? v[4 * j] : 0.f;
+        x.y = (4 * j + 1 < 27) ? v[4 * j + 1] : 0.f;
+        x.z = (4 * j + 2 < 27) ? v[4 * j + 2] : 0.f;
+        x.w = (4 * j + 3 < 27) ? v[4 * j + 3] : 0.f;
+        q[j] = x;
+    }
+}
+__device__ __forceinline__ void store_rec32(double *p, const double *v)
+{
+    double2 *q = (double2 *)p;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        double2 x;
+        x.x = (2 * j < 27) ? v[2 * j] : 0.;
+        x.y = (2 * j + 1 < 27) ? v[2 * j + 1] : 0.;
+        q[j] = x;
+    }
+}
+
 // One thread per point, over the track: V = sum Bp^T W Bp and g_p = sum
 // Bp^T W r, then damping and the 3x3 inverse (rbacore.cpp:1341-1344).
 // Sums are FP64.
@@ -291,8 +317,7 @@ __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
     R Bc[2][9], Bp[2][3];
     jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
     const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
-    R *Wo = (R *)a.Wrec + (size_t)q * kRec;
-    R *Yo = (R *)a.Yrec + (size_t)(a.yrank ? a.yrank[q] : q) * kRec;
+    R Wv[27], Yv[27];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const R t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
@@ -300,18 +325,15 @@ __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
         const R w0 = t0 * Bp[0][0] + t1 * Bp[1][0];
         const R w1 = t0 * Bp[0][1] + t1 * Bp[1][1];
         const R w2 = t0 * Bp[0][2] + t1 * Bp[1][2];
-        Wo[r * 3] = w0;
-        Wo[r * 3 + 1] = w1;
-        Wo[r * 3 + 2] = w2;
+        Wv[r * 3] = w0;
+        Wv[r * 3 + 1] = w1;
+        Wv[r * 3 + 2] = w2;
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) Yo[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
+        for (int cc = 0; cc < 3; ++cc) Yv[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
     }
-    // the padding too: every cache line of the record is fully written
-#pragma unroll
-    for (int j = 27; j < kRec; ++j) {
-        Wo[j] = (R)0;
-        Yo[j] = (R)0;
-    }
+    // whole records, padding included: every cache line is fully written
+    store_rec32((R *)a.Wrec + (size_t)q * kRec, Wv);
+    store_rec32((R *)a.Yrec + (size_t)(a.yrank ? a.yrank[q] : q) * kRec, Yv);
 }
 
 // ---------------------------------------------------------------------------
@@ -1053,27 +1075,64 @@ __global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
 template <typename R>
 __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.npts) return;
-    double t[3] = {a.gp[3 * i], a.gp[3 * i + 1], a.gp[3 * i + 2]};
-    for (int q = a.toff[i]; q < a.toff[i + 1]; ++q) {
-        const R *W = (const R *)a.Wrec + (size_t)q * kRec;
-        const double *xc = a.sol + 9 * (size_t)a.tcam[q];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            double s = 0.;
-#pragma unroll
-            for (int r = 0; r < 9; ++r) s += (double)W[r * 3 + j] * xc[r];
-            t[j] -= s;
-        }
+    // a workgroup per group of points whose first entry lies in a 256-entry
+    // window; lane per entry u_q = W_q^T dc_cam(q) to LDS, then each point's
+    // lane subtracts its entries from g_p in entry order
+    __shared__ double su[256][3];
+    const int t = threadIdx.x, c = blockIdx.x;
+    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    if (p0 >= p1) return;
+    const int e0 = a.toff[p0], e1 = a.toff[p1];
+    const int np = p1 - p0;
+    const int i = p0 + t;
+    int pb = 0, pe = 0;
+    double t0 = 0., t1 = 0., t2 = 0.;
+    if (t < np) {
+        pb = a.toff[i];
+        pe = a.toff[i + 1];
+        t0 = a.gp[3 * (size_t)i];
+        t1 = a.gp[3 * (size_t)i + 1];
+        t2 = a.gp[3 * (size_t)i + 2];
     }
+    const R *Wr = (const R *)a.Wrec;
+    for (int base = e0; base < e1; base += 256) {
+        const int q = base + t;
+        if (q < e1) {
+            R W[28];
+            load_rec27(Wr + (size_t)q * kRec, W);
+            const double *xc = a.sol + 9 * (size_t)a.tcam[q];
+            double u0 = 0., u1 = 0., u2 = 0.;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const double xr = xc[r];
+                u0 += (double)W[r * 3] * xr;
+                u1 += (double)W[r * 3 + 1] * xr;
+                u2 += (double)W[r * 3 + 2] * xr;
+            }
+            su[t][0] = u0;
+            su[t][1] = u1;
+            su[t][2] = u2;
+        }
+        __syncthreads();
+        if (t < np) {
+            const int lo = max(pb, base), hi = min(pe, base + 256);
+            for (int x = lo; x < hi; ++x) {
+                t0 -= su[x - base][0];
+                t1 -= su[x - base][1];
+                t2 -= su[x - base][2];
+            }
+        }
+        __syncthreads();
+    }
+    if (t >= np) return;
     const double *Vi = a.Vinv + 9 * (size_t)i;
     bool bad = false;
+    const double tt[3] = {t0, t1, t2};
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        const double dp = Vi[r * 3] * t[0] + Vi[r * 3 + 1] * t[1] + Vi[r * 3 + 2] * t[2];
+        const double dp = Vi[r * 3] * tt[0] + Vi[r * 3 + 1] * tt[1] + Vi[r * 3 + 2] * tt[2];
         if (isnan(dp) || isinf(dp)) bad = true;
-        a.Xn[3 * i + r] = a.X[3 * i + r] - dp;
+        a.Xn[3 * (size_t)i + r] = a.X[3 * (size_t)i + r] - dp;
     }
     if (bad) a.S->nan_flag = 1;
 }
@@ -1480,11 +1539,11 @@ void launch_pcg_update(const PcgArgs &a, hipStream_t st)
 }
 void launch_backsub(const StepArgs &a, hipStream_t st)
 {
-    if (a.npts <= 0) return;
+    if (a.npts <= 0 || a.npgrp <= 0) return;
     if (a.rec32)
-        k_backsub<float><<<nblk(a.npts, 256), 256, 0, st>>>(a);
+        k_backsub<float><<<a.npgrp, 256, 0, st>>>(a);
     else
-        k_backsub<double><<<nblk(a.npts, 256), 256, 0, st>>>(a);
+        k_backsub<double><<<a.npgrp, 256, 0, st>>>(a);
 }
 void launch_decode(const StepArgs &a, hipStream_t st) { k_decode<<<nblk(a.nc, 64), 64, 0, st>>>(a); }
 int cost_blocks(int ntr) { return (int)nblk(ntr > 0 ? ntr : 1, 256); }

@@ -436,7 +436,7 @@ static StepArgs step_args(rpba_ctx *c, const double *sol)
 {
     StepArgs a;
     a.nc = c->nc; a.npts = c->ds.npts; a.cam = c->cam();
-    a.toff = c->ds.toff; a.tcam = c->ds.tcam;
+    a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.pgrp = c->ds.pgrp; a.npgrp = c->ds.npgrp;
     a.Wrec = c->d_Wrec.p; a.rec32 = c->rec32;
     a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.sol = sol; a.X = c->d_X.p; a.Xn = c->d_Xn.p;
     a.S = c->d_pcg.p;
