@@ -25,7 +25,8 @@ struct PcgScalars {
 };
 
 struct LinArgs {
-    int nc, npts, ntr;
+    int nc, npts, ntr, npgrp;
+    const int *pgrp;  // point groups (first point of group c)
     double defdiff, mult1;
     CamPtrs cam;
     const int *toff, *tcam, *tpt, *camoff, *camq;

@@ -255,44 +255,68 @@ __device__ __forceinline__ void store_rec32(double *p, const double *v)
     }
 }
 
-// One thread per point, over the track: V = sum Bp^T W Bp and g_p = sum
-// Bp^T W r, then damping and the 3x3 inverse (rbacore.cpp:1341-1344).
-// Sums are FP64.
+// V = sum Bp^T W Bp and g_p = sum Bp^T W r per point, then damping and the
+// 3x3 inverse (rbacore.cpp:1341-1344).  A workgroup per group of points whose
+// first entry lies in a 256-entry window: lane per entry computes its terms
+// (to LDS), then each point's lane adds its entries in order.  Sums are FP64.
 template <typename R>
-__global__ __launch_bounds__(128) void k_lin_points(LinArgs a)
+__global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.npts) return;
-    const double X0 = a.X[3 * i], X1 = a.X[3 * i + 1], X2 = a.X[3 * i + 2];
-    double V[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
-    const int b = a.toff[i], e = a.toff[i + 1];
-    for (int q = b; q < e; ++q) {
-        const int k = a.tcam[q];
-        CamV c;
-        load_cam(a.cam, k, c);
-        Ev ev;
-        eval_obs(c, X0, X1, X2, a.ox[q], a.oy[q], a.defdiff, ev);
-        R Bp[2][3];
-        jac_point<R>(c, ev, Bp);
-        const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const R t0 = Bp[0][r] * W0 + Bp[1][r] * W2;
-            const R t1 = Bp[0][r] * W2 + Bp[1][r] * W1;
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc) V[r * 3 + cc] += (double)(t0 * Bp[0][cc] + t1 * Bp[1][cc]);
-            g[r] += (double)t0 * ev.d1 + (double)t1 * ev.d2;
-        }
+    __shared__ double sv[256][12];
+    const int t = threadIdx.x, c = blockIdx.x;
+    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    if (p0 >= p1) return;
+    const int e0 = a.toff[p0], e1 = a.toff[p1];
+    const int np = p1 - p0;
+    const int i = p0 + t;
+    int pb = 0, pe = 0;
+    if (t < np) {
+        pb = a.toff[i];
+        pe = a.toff[i + 1];
     }
+    double V[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+    for (int base = e0; base < e1; base += 256) {
+        const int q = base + t;
+        if (q < e1) {
+            const int k = a.tcam[q], ip = a.tpt[q];
+            CamV cv;
+            load_cam(a.cam, k, cv);
+            Ev ev;
+            eval_obs(cv, a.X[3 * (size_t)ip], a.X[3 * (size_t)ip + 1], a.X[3 * (size_t)ip + 2], a.ox[q], a.oy[q], a.defdiff, ev);
+            R Bp[2][3];
+            jac_point<R>(cv, ev, Bp);
+            const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const R t0 = Bp[0][r] * W0 + Bp[1][r] * W2;
+                const R t1 = Bp[0][r] * W2 + Bp[1][r] * W1;
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) sv[t][r * 3 + cc] = (double)(t0 * Bp[0][cc] + t1 * Bp[1][cc]);
+                sv[t][9 + r] = (double)t0 * ev.d1 + (double)t1 * ev.d2;
+            }
+        }
+        __syncthreads();
+        if (t < np) {
+            const int lo = max(pb, base), hi = min(pe, base + 256);
+            for (int x = lo; x < hi; ++x) {
+#pragma unroll
+                for (int j = 0; j < 9; ++j) V[j] += sv[x - base][j];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) g[j] += sv[x - base][9 + j];
+            }
+        }
+        __syncthreads();
+    }
+    if (t >= np) return;
     V[0] *= a.mult1;
     V[4] *= a.mult1;
     V[8] *= a.mult1;
     double Vi[9];
     inv3(V, Vi);
 #pragma unroll
-    for (int j = 0; j < 9; ++j) a.Vinv[9 * i + j] = Vi[j];
+    for (int j = 0; j < 9; ++j) a.Vinv[9 * (size_t)i + j] = Vi[j];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) a.gp[3 * i + j] = g[j];
+    for (int j = 0; j < 3; ++j) a.gp[3 * (size_t)i + j] = g[j];
 }
 
 // One thread per track entry: recomputes the observation's Jacobian (cheaper
@@ -1463,12 +1487,12 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
 
 void launch_lin_points(const LinArgs &a, hipStream_t st)
 {
-    if (a.npts <= 0) return;
+    if (a.npts <= 0 || a.npgrp <= 0) return;
     if (a.rec32) {
-        k_lin_points<float><<<nblk(a.npts, 128), 128, 0, st>>>(a);
+        k_lin_points<float><<<a.npgrp, 256, 0, st>>>(a);
         k_lin_records<float><<<nblk(a.ntr, 128), 128, 0, st>>>(a);
     } else {
-        k_lin_points<double><<<nblk(a.npts, 128), 128, 0, st>>>(a);
+        k_lin_points<double><<<a.npgrp, 256, 0, st>>>(a);
         k_lin_records<double><<<nblk(a.ntr, 128), 128, 0, st>>>(a);
     }
 }

@@ -396,7 +396,7 @@ static int upload_initial_cameras(rpba_ctx *c)
 static LinArgs lin_args(rpba_ctx *c)
 {
     LinArgs a;
-    a.nc = c->nc; a.npts = c->ds.npts; a.ntr = c->ds.ntr;
+    a.nc = c->nc; a.npts = c->ds.npts; a.ntr = c->ds.ntr; a.npgrp = c->ds.npgrp; a.pgrp = c->ds.pgrp;
     a.defdiff = c->defdiff; a.mult1 = 1. + c->mult;
     a.cam = c->cam();
     a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.tpt = c->ds.tpt; a.camoff = c->ds.camoff; a.camq = c->ds.camq;
