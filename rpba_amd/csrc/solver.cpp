@@ -121,6 +121,8 @@ struct rpba_ctx {
     rpba_block *io = nullptr;
     Readback *hb = nullptr;          // pinned
     hipEvent_t pcg_ev[2] = {nullptr, nullptr};  // end of PCG chunk n, slot n & 1
+    hipStream_t st2 = nullptr;                   // off-diagonal Schur blocks, beside st
+    hipEvent_t ev_rec = nullptr, ev_pairs = nullptr;
     double *hcam = nullptr;          // pinned, 15 per camera: stored [R|t] (12) and intrinsics (3)
     rpba_flags fl{};
     rpba_log_fn log = nullptr;
@@ -485,12 +487,24 @@ static int bundle_step(rpba_ctx *c)
     sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
     sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.rec32 = c->rec32;
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
-    if (!c->implicit) c->timed("schur_pairs", [&] { launch_schur_pairs(sa, c->st); });
+    // the off-diagonal blocks depend only on the records: without kernel
+    // timing they run on a second stream, beside the diagonal terms, the
+    // camera blocks, their inverses and the PCG set-up
+    const bool side = !c->implicit && !c->timing;
+    if (side) {
+        HIPCHK(hipEventRecord(c->ev_rec, c->st));
+        HIPCHK(hipStreamWaitEvent(c->st2, c->ev_rec, 0));
+        launch_schur_pairs(sa, c->st2);
+        HIPCHK(hipEventRecord(c->ev_pairs, c->st2));
+    } else if (!c->implicit) {
+        c->timed("schur_pairs", [&] { launch_schur_pairs(sa, c->st); });
+    }
     c->timed("schur_diag", [&] { launch_schur_diag(la, c->st); });
     c->timed("lin_cameras", [&] { launch_lin_cameras(la, c->st); });
     c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
     PcgArgs pa = pcg_args(c);
     c->timed("pcg_init", [&] { launch_pcg_init(pa, c->st); });
+    if (side) HIPCHK(hipStreamWaitEvent(c->st, c->ev_pairs, 0));
     // launch j reads the scalars of slot j & 1 and writes slot (j + 1) & 1;
     // once the stopping rule fired, later launches only pass the state on.
     // The first chunk is sized from the previous solve's iteration count.
@@ -800,6 +814,9 @@ static int ctx_init(rpba_ctx *c)
 
     HIPCHK(hipGetDevice(&c->dev));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_rec, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_pairs, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&c->pcg_ev[0], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&c->pcg_ev[1], hipEventDisableTiming));
     if (hipHostMalloc((void **)&c->hb, sizeof(Readback), hipHostMallocDefault) != hipSuccess ||
@@ -1049,8 +1066,11 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
 {
     if (!c) return;
     if (c->st) (void)hipStreamSynchronize(c->st);
+    if (c->st2) (void)hipStreamSynchronize(c->st2);
     for (auto &e : c->pcg_ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->ev_rec) (void)hipEventDestroy(c->ev_rec);
+    if (c->ev_pairs) (void)hipEventDestroy(c->ev_pairs);
     if (c->hb) (void)hipHostFree(c->hb);
     if (c->hcam) (void)hipHostFree(c->hcam);
     c->hb = nullptr;
@@ -1080,6 +1100,7 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_Dk.release();
     c->d_ppart.release();
     if (c->st) (void)hipStreamDestroy(c->st);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
     delete c;
 }
 
