@@ -1,7 +1,10 @@
 // `rpba` host driver: the reference's main (rpba.cpp:45-176) on the device
 // solve.  Reads ./parameters, the BAL file InputDir/File, runs the solve and
-// prints the same lines.  An optional `Output = 1` key writes InputDir/out-File
-// in the input format (the reference's OutputBlock call, rpba.cpp:144-150).
+// prints the same lines.  Optional keys: `Output = 1` writes InputDir/out-File
+// in the input format (the reference's OutputBlock call, rpba.cpp:144-150);
+// `Cloud = 1` writes InputDir/cloud.txt with the kept points and prints the
+// number of dropped points (rpba.cpp:154-173).  Both calls are commented out
+// in the reference.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -17,7 +20,7 @@ static void to_stdout(const char *line, void *user)
     os->append(line);
 }
 
-static int read_output_key(const char *path)
+static int read_int_key(const char *path, const char *key)
 {
     FILE *fh = std::fopen(path, "r");
     if (!fh) return 0;
@@ -26,7 +29,7 @@ static int read_output_key(const char *path)
     while (std::fgets(line, sizeof line, fh)) {
         char k[256];
         int v = 0;
-        if (std::sscanf(line, " %255[^= \t] = %d", k, &v) == 2 && std::strcmp(k, "Output") == 0) out = v;
+        if (std::sscanf(line, " %255[^= \t] = %d", k, &v) == 2 && std::strcmp(k, key) == 0) out = v;
     }
     std::fclose(fh);
     return out;
@@ -76,9 +79,24 @@ int main(int argc, char **argv)
     std::printf("\n\nBundle adjustment runtime: %g s\n\n", secs);
     if (rc != RPBA_OK && rc != RPBA_E_NO_OBS_IN_IMAGE && rc != RPBA_E_LOW_REDUNDANCY && rc != RPBA_E_NAN_RESULT)
         std::fprintf(stderr, "rpba: solve failed with status %d\n", rc);
-    if (!fl.exteval && read_output_key("parameters")) {
+    if (!fl.exteval && read_int_key("parameters", "Output")) {
         const std::string out = std::string(dir) + "/out-" + file;
         rpba_write_bal(out.c_str(), &io, fl.k24out);
+    }
+    if (read_int_key("parameters", "Cloud")) {
+        const std::string cloud = std::string(dir) + "/cloud.txt";
+        std::printf("Outputfile: %s\n", cloud.c_str());
+        FILE *fc = std::fopen(cloud.c_str(), "w");
+        int counter = 0;
+        for (int i = 0; i < io.n_pts; ++i) {
+            if (io.pt_kept && !io.pt_kept[i]) {
+                ++counter;
+                continue;
+            }
+            if (fc) std::fprintf(fc, "%.8g %.8g %.8g\n", io.pts[3 * i], io.pts[3 * i + 1], io.pts[3 * i + 2]);
+        }
+        if (fc) std::fclose(fc);
+        std::printf("Counter: %d\n", counter);
     }
     rpba_free_block(&io);
     return rc == RPBA_OK ? 0 : 2;
