@@ -91,6 +91,10 @@ typedef struct rpba_flags {
                                 profile store); 1 = matrix-free: only the diagonal
                                 blocks are formed and the PCG multiplies through
                                 W and Y = W V^-1 (no covisibility pairs built)  */
+    int32_t solver;          /* 0 = block-Jacobi PCG (SparsePCG, rbacore.cpp:1807-1914);
+                                1 = dense Cholesky of the reduced camera system on
+                                the FP64 matrix cores (exact solve; a form of its
+                                own, for small or dense S)                       */
 } rpba_flags;
 
 /* Optional per-solve report.  sigma values are sqrt(sig0 * ihw2) exactly as the

@@ -45,6 +45,8 @@ def _load():
     lib.orc_step.restype = C.c_int
     lib.oracle_pcg_dense.argtypes = [C.c_int, P(C.c_int), P(C.c_double), P(C.c_double), P(C.c_double)]
     lib.oracle_pcg_dense.restype = C.c_int
+    lib.oracle_chol_solve.argtypes = [C.c_int, P(C.c_double), P(C.c_double), P(C.c_double)]
+    lib.oracle_chol_solve.restype = C.c_int
     lib.oracle_cayley.argtypes = [C.c_double, C.c_double, C.c_double, P(C.c_double)]
     lib.oracle_rectify.argtypes = [P(C.c_double), C.c_int]
     lib.oracle_aa_to_R.argtypes = [C.c_double, C.c_double, C.c_double, P(C.c_float)]
@@ -160,6 +162,16 @@ def pcg_dense(block_sizes, S, b):
     x = np.zeros_like(b)
     its = lib().oracle_pcg_dense(bs.size, _p(bs, C.c_int), _p(S, C.c_double), _p(b, C.c_double), _p(x, C.c_double))
     return x, its
+
+
+def chol_solve(S, b):
+    """Dense Cholesky solve of the dense form (solver = 1); None if S is not
+    positive definite."""
+    A = np.array(S, dtype=np.float64, order="C", copy=True)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    rc = lib().oracle_chol_solve(b.size, _p(A, C.c_double), _p(b, C.c_double), _p(x, C.c_double))
+    return None if rc else (x, np.tril(A))
 
 
 def cayley(a, b, c):

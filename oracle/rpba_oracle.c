@@ -236,6 +236,7 @@ float oracle_upper_median(const float *v, int n)
 
 typedef struct orc {
     int fsites, robust, nraddpi, addf[3];
+    int solver;              /* 0 = block-Jacobi PCG (the reference); 1 = dense Cholesky */
     int nimg, XXnr;
     float ihw2;
     double defdiff;
@@ -312,6 +313,7 @@ orc *orc_create(const rpba_block *io, const rpba_flags *f, int fsites)
     orc *o = (orc *)xcalloc(1, sizeof(orc));
     const int nc = io->n_cams, np = io->n_pts, no = io->n_obs;
     o->fsites = fsites;
+    o->solver = f->solver;
     o->robust = f->robust != 0;
     o->addf[0] = f->est_k2 != 0; o->addf[1] = f->est_k4 != 0; o->addf[2] = f->est_f != 0;
     o->nraddpi = o->addf[0] + o->addf[1] + o->addf[2];
@@ -939,6 +941,63 @@ int oracle_block_pcg(int nb, int begini, const int *off, double *const *Sb, cons
     return (k < msize) ? k + 1 : k;
 }
 
+/* Dense Cholesky solve of a symmetric positive definite system (row-major
+ * A[n*n], overwritten by L in its lower triangle): A = L L^T by the
+ * Cholesky-Crout recurrence in column order, then L y = b, L^T x = y.  This is
+ * the exact solve of the reduced camera system used by the dense form
+ * (solver = 1); the reference solves the same system with SparsePCG
+ * (rbacore.cpp:1807-1914).  Returns 0, or -1 at the first pivot <= 0. */
+int oracle_chol_solve(int n, double *A, const double *b, double *x)
+{
+    for (int j = 0; j < n; ++j) {
+        double d = A[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+        if (!(d > 0.)) return -1;
+        const double ljj = sqrt(d);
+        A[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double s = A[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+            A[(size_t)i * n + j] = s / ljj;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k) s -= A[(size_t)i * n + k] * x[k];
+        x[i] = s / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = x[i];
+        for (int k = i + 1; k < n; ++k) s -= A[(size_t)k * n + i] * x[k];
+        x[i] = s / A[(size_t)i * n + i];
+    }
+    return 0;
+}
+
+/* The dense form's solve of S x = b over the touched blocks (solver = 1):
+ * S assembled densely from the upper blocks, both triangles, then
+ * oracle_chol_solve.  A non-positive pivot gives a NaN step, which the
+ * caller's NaN check turns into the reference's error path. */
+static void dense_solve(orc *o, double *sol)
+{
+    const int nc = o->nimg, n = o->nrioff;
+    double *A = (double *)xcalloc((size_t)n * n + 1, sizeof(double));
+    for (int k = 0; k < nc; ++k)
+        for (int l = k; l < nc; ++l) {
+            const size_t id = (size_t)k * nc + l;
+            if (!o->Sf[id]) continue;
+            for (int r = 0; r < bsz(o, k); ++r)
+                for (int c = 0; c < bsz(o, l); ++c) {
+                    const double v = o->Sb[id][r * MAXB + c];
+                    A[(size_t)(o->camparv[k] + r) * n + o->camparv[l] + c] = v;
+                    A[(size_t)(o->camparv[l] + c) * n + o->camparv[k] + r] = v;
+                }
+        }
+    if (oracle_chol_solve(n, A, o->ex, sol))
+        for (int j = 0; j < n; ++j) sol[j] = NAN;
+    free(A);
+}
+
 /* CalcBundleSep (rbacore.cpp:1094-1195) */
 static int bundle_sep(orc *o, double mult)
 {
@@ -959,7 +1018,12 @@ static int bundle_sep(orc *o, double mult)
         for (int j = 0; j < m; ++j) D[j * MAXB + j] += 1.e-2;
     }
     double *sol = (double *)xcalloc((size_t)o->nrioff + 1, sizeof(double));
-    o->pcg_last = oracle_block_pcg(nc, o->begini, o->camparv, o->Sb, o->Sf, o->ex, sol);
+    if (o->solver == 1) {
+        dense_solve(o, sol);
+        o->pcg_last = 0;
+    } else {
+        o->pcg_last = oracle_block_pcg(nc, o->begini, o->camparv, o->Sb, o->Sf, o->ex, sol);
+    }
     o->pcg_total += o->pcg_last;
     memcpy(o->ex, sol, sizeof(double) * (size_t)o->nrioff);
     free(sol);

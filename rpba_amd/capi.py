@@ -41,7 +41,7 @@ class RpbaFlags(C.Structure):
         ("robust", C.c_int32), ("exteval", C.c_int32), ("threads", C.c_int32),
         ("est_k2", C.c_int32), ("est_k4", C.c_int32), ("est_f", C.c_int32),
         ("k24out", C.c_int32), ("precision", C.c_int32), ("preconditioner", C.c_int32),
-        ("force_global", C.c_int32), ("schur", C.c_int32),
+        ("force_global", C.c_int32), ("schur", C.c_int32), ("solver", C.c_int32),
     ]
 
 
@@ -102,8 +102,9 @@ class Block:
 
 
 def make_flags(robust=1, exteval=0, threads=1, k2=1, k4=1, f=1, k24out=1, precision=0,
-               preconditioner=0, force_global=0, schur=0):
-    return RpbaFlags(robust, exteval, threads, k2, k4, f, k24out, precision, preconditioner, force_global, schur)
+               preconditioner=0, force_global=0, schur=0, solver=0):
+    return RpbaFlags(robust, exteval, threads, k2, k4, f, k24out, precision, preconditioner, force_global, schur,
+                     solver)
 
 
 class Trace:

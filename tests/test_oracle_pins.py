@@ -94,8 +94,8 @@ def test_upper_median(n):
 # Jacobians by finite differences; Schur by dense normal equations
 # --------------------------------------------------------------------------
 
-def _probe(scene, k2=1, k4=1, f=1):
-    h = oracle.Handle(scene, capi.make_flags(robust=1, k2=k2, k4=k4, f=f))
+def _probe(scene, k2=1, k4=1, f=1, solver=0):
+    h = oracle.Handle(scene, capi.make_flags(robust=1, k2=k2, k4=k4, f=f, solver=solver))
     h.setup_pass0()
     return h
 
@@ -253,6 +253,56 @@ def test_pcg_against_cholesky_and_replay(seed):
     # the step-norm rule stops loosely: residual reduced well below b
     assert np.linalg.norm(S @ x - b) < 0.1 * np.linalg.norm(b)
     assert np.linalg.norm(x - xs) < 0.2 * np.linalg.norm(xs)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_dense_cholesky_against_numpy(seed):
+    """The dense form's solve: factor equal to numpy's Cholesky, solution to
+    numpy's solve; an indefinite matrix is refused."""
+    rng = np.random.default_rng(300 + seed)
+    n = int(rng.integers(5, 80))
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    S = (Q * np.logspace(-2, 4, n)) @ Q.T
+    S = 0.5 * (S + S.T)
+    b = rng.normal(size=n)
+    x, L = oracle.chol_solve(S, b)
+    Ln = np.linalg.cholesky(S)
+    assert np.abs(L - Ln).max() < 1e-10 * np.abs(Ln).max()
+    xs = np.linalg.solve(S, b)
+    assert np.abs(x - xs).max() < 1e-8 * np.abs(xs).max()
+    S2 = S.copy()
+    S2[n // 2, n // 2] = -1.0
+    assert oracle.chol_solve(S2, b) is None
+
+
+def test_dense_form_step_is_exact(tiny_scene):
+    """solver = 1 replaces the PCG by the exact solve of the same S and b:
+    the camera step equals numpy's solve (float-cast, H3) and the point step
+    follows by back-substitution."""
+    h = _probe(tiny_scene, solver=1)
+    d = h.dims()
+    nc = d["nrioff"]
+    So, bo = h.linearize(1e-3)
+    x, its = h.step(1e-3)
+    assert its == 0
+    xs = np.linalg.solve(So, bo)
+    # float cast of the camera step (rbacore.cpp:1185): within one float ulp
+    assert np.array_equal(x[:nc], x[:nc].astype(np.float32).astype(np.float64))
+    assert np.abs(x[:nc] - xs).max() < 1e-6 * np.abs(xs).max()
+
+
+def test_dense_form_solve_converges(scene_c1):
+    """Whole robust solve with the exact camera step: a final sigma0 within
+    1e-3 of the PCG solve's and the same robust labels up to the borderline
+    observations of a different trajectory (at most 0.1%)."""
+    f0 = capi.make_flags(robust=1)
+    f1 = capi.make_flags(robust=1, solver=1)
+    rc0, out0, st0, _ = oracle.solve(scene_c1, f0)
+    rc1, out1, st1, _ = oracle.solve(scene_c1, f1)
+    assert rc0 == 0 and rc1 == 0
+    assert st1["pcg_iters"] == 0
+    assert abs(st1["sigma_out"][1] - st0["sigma_out"][1]) < 1e-3 * st0["sigma_out"][1]
+    assert (out0["obs_kept"] != out1["obs_kept"]).sum() <= 1e-3 * out0["obs_kept"].size
 
 
 # --------------------------------------------------------------------------
