@@ -151,6 +151,10 @@ int rpba_ctx_download(rpba_ctx *ctx, rpba_stats *stats);
  * launches, measured with HIP events on the context's stream. */
 int rpba_ctx_set_timing(rpba_ctx *ctx, int32_t on);
 int rpba_ctx_kernel_time(rpba_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
+/* Dense form (solver = 1): the trailing-update variant the autotuner chose at
+ * rpba_ctx_create and the timings it compared, one line.  No counterpart in
+ * the reference (its solve has no tuned kernels). */
+int rpba_ctx_tuning(rpba_ctx *ctx, char *buf, int32_t cap);
 /* Progress of the running solve: pass (0, 1; 2 = done), LM iterations (itt)
  * and sig0 of the stored state (sig0old, rbacore.cpp:568) in the current pass. */
 int rpba_ctx_progress(rpba_ctx *ctx, int32_t *pass, int32_t *itt, double *sig0_stored);
@@ -159,6 +163,18 @@ int rpba_ctx_progress(rpba_ctx *ctx, int32_t *pass, int32_t *itt, double *sig0_s
  * observations, point pairs, off-diagonal S blocks, S dimension (nrioff). */
 int rpba_ctx_sizes(rpba_ctx *ctx, int64_t out[6]);
 void rpba_ctx_destroy(rpba_ctx *ctx);
+
+/* Batched dense solve on the FP64 matrix cores: nsys independent symmetric
+ * positive definite systems S_m x_m = b_m of dimension 9 * ncam[m], each S_m
+ * full and row-major, concatenated in S; b and x concatenated likewise.  The
+ * kernels are those of the dense form (solver = 1).  variant < 0: the
+ * autotuner picks the trailing-update variant; else variant 0 (LDS-staged) or
+ * 1 (direct) with a persistent grid of `cap` workgroups.  Every variant gives
+ * the same bits.  No counterpart in the reference (its systems are solved by
+ * SparsePCG, rbacore.cpp:1807-1914); exposed for callers with their own
+ * batches of camera systems. */
+int rpba_dense_solve_batch(int32_t nsys, const int32_t *ncam, const double *S, const double *b, double *x,
+                           int32_t variant, int32_t cap);
 
 /* BAL text I/O (rpba.cpp:180-338), host only.
  * rpba_read_bal allocates the arrays of *io with malloc; free with

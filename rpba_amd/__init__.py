@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "librpba_hip.so")
 EXPORTS = [
     "rpba_calc_plsa", "rpba_ctx_create", "rpba_ctx_run", "rpba_ctx_reset", "rpba_ctx_download",
     "rpba_ctx_set_timing", "rpba_ctx_kernel_time", "rpba_ctx_progress", "rpba_ctx_sizes", "rpba_ctx_destroy",
+    "rpba_ctx_tuning", "rpba_dense_solve_batch",
     "rpba_read_bal", "rpba_write_bal", "rpba_free_block", "rpba_read_parameters", "rpba_version",
 ]
 
@@ -45,6 +46,11 @@ def lib():
     L.rpba_ctx_progress.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_double)]
     L.rpba_ctx_progress.restype = C.c_int
     L.rpba_ctx_sizes.argtypes = [C.c_void_p, P(C.c_int64)]
+    L.rpba_ctx_tuning.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+    L.rpba_ctx_tuning.restype = C.c_int
+    L.rpba_dense_solve_batch.argtypes = [C.c_int32, P(C.c_int32), P(C.c_double), P(C.c_double), P(C.c_double),
+                                         C.c_int32, C.c_int32]
+    L.rpba_dense_solve_batch.restype = C.c_int
     L.rpba_ctx_destroy.argtypes = [C.c_void_p]
     L.rpba_ctx_destroy.restype = None
     L.rpba_read_bal.argtypes = [C.c_char_p, C.c_int32, P(capi.RpbaBlock)]

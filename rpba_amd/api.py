@@ -21,6 +21,26 @@ def calc_plsa(scene, flags=None, trace_cap=512):
     return rc, blk.result(), tr.as_dict(), logc.lines
 
 
+def dense_solve_batch(systems, rhs, variant=-1, cap=0):
+    """Solve the SPD systems (list of (9k x 9k) arrays) with the batched dense
+    Cholesky of the dense form; returns the list of solutions."""
+    import numpy as np
+    ncam = np.array([s.shape[0] // 9 for s in systems], dtype=np.int32)
+    S = np.concatenate([np.ascontiguousarray(s, dtype=np.float64).ravel() for s in systems])
+    b = np.concatenate([np.asarray(r, dtype=np.float64) for r in rhs])
+    x = np.zeros_like(b)
+    P = C.POINTER
+    rc = lib().rpba_dense_solve_batch(len(systems), ncam.ctypes.data_as(P(C.c_int32)), S.ctypes.data_as(P(C.c_double)),
+                                      b.ctypes.data_as(P(C.c_double)), x.ctypes.data_as(P(C.c_double)), variant, cap)
+    if rc != 0:
+        raise RuntimeError(f"rpba_dense_solve_batch failed with status {rc}")
+    out, o = [], 0
+    for s in systems:
+        out.append(x[o:o + s.shape[0]])
+        o += s.shape[0]
+    return out
+
+
 class Context:
     """Resumable solve (rpba_ctx_*): create uploads the block, run advances
     the LM iterations, download writes the stored state back."""
@@ -60,6 +80,12 @@ class Context:
         out = (C.c_int64 * 6)()
         lib().rpba_ctx_sizes(self.h, out)
         return dict(zip(["cams", "pts", "obs", "pairs", "blocks", "sdim"], list(out)))
+
+    def tuning(self):
+        """The dense form's autotuner choice and timings (one line)."""
+        buf = C.create_string_buffer(512)
+        lib().rpba_ctx_tuning(self.h, buf, 512)
+        return buf.value.decode()
 
     def download(self, trace_cap=512):
         tr = capi.Trace(trace_cap)

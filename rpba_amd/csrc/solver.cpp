@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/rpba.h"
+#include "dense.h"
 #include "kernels.h"
 #include "structure.h"
 #include "structure_dev.h"
@@ -145,6 +146,7 @@ struct rpba_ctx {
     int rec32 = 0;                   // FP32 Jacobians and W/Y records (precision 1)
     bool implicit = false;           // matrix-free S product (schur 1)
     DevStruct ds;                    // device tracks, camera-major order, pairs, blocks
+    DenseSolver dense;               // dense Cholesky form (solver 1)
 
     // device buffers
     DBuf<float> d_wx, d_wy, d_wxy, d_key, d_s00;
@@ -501,6 +503,17 @@ static int bundle_step(rpba_ctx *c)
     }
     c->timed("schur_diag", [&] { launch_schur_diag(la, c->st); });
     c->timed("lin_cameras", [&] { launch_lin_cameras(la, c->st); });
+    if (c->fl.solver == 1) {
+        // exact solve of the same S and b on the matrix cores
+        if (side) HIPCHK(hipStreamWaitEvent(c->st, c->ev_pairs, 0));
+        c->timed("dense_assemble", [&] {
+            c->dense.assemble(c->d_Sd.p, c->d_Su.p, c->ds.bk, c->ds.bl, c->ds.nblk, c->d_b.p, c->st);
+        });
+        c->dense.factor_solve(c->d_best.p, c->st, [&](const char *nm, auto f) { c->timed(nm, f); });
+        PcgScalars hs{};
+        hs.done = 1;
+        return finish_bundle_step(c, hs);
+    }
     c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
     PcgArgs pa = pcg_args(c);
     c->timed("pcg_init", [&] { launch_pcg_init(pa, c->st); });
@@ -843,6 +856,13 @@ static int ctx_init(rpba_ctx *c)
     rc |= c->c_Rb.alloc(n9);
     rc |= c->c_Cb.alloc(3 * (size_t)nc);
     if (rc) return RPBA_E_NOMEM;
+    if (c->fl.solver == 1) {
+        const int cam0 = 0;
+        rc = c->dense.plan(1, &cam0, &nc, nc);
+        if (rc) return rc;
+        rc = c->dense.autotune(c->st);
+        if (rc) return rc;
+    }
     reset_state(c);
     rc = upload_tracks(c);
     if (rc) return rc;
@@ -872,6 +892,8 @@ extern "C" int rpba_ctx_create(rpba_ctx **out, rpba_block *io, const rpba_flags 
 {
     if (!out || !io || !flags || io->n_cams < 1 || io->n_pts < 0 || io->n_obs < 0) return RPBA_E_INVALID;
     if (flags->precision < 0 || flags->precision > 1 || flags->schur < 0 || flags->schur > 1) return RPBA_E_INVALID;
+    // the dense form factors the formed S: it needs the off-diagonal blocks
+    if (flags->solver < 0 || flags->solver > 1 || (flags->solver == 1 && flags->schur == 1)) return RPBA_E_INVALID;
     if (!io->obs_cam || !io->obs_pt || !io->obs_xy || !io->cam_P || !io->cam_f || !io->cam_d || !io->pts)
         return RPBA_E_INVALID;
     int ndev = 0;
@@ -1024,6 +1046,14 @@ extern "C" int rpba_ctx_set_timing(rpba_ctx *c, int32_t on)
     if (!c) return RPBA_E_INVALID;
     c->timing = on != 0;
     c->ktime.clear();
+    return RPBA_OK;
+}
+
+extern "C" int rpba_ctx_tuning(rpba_ctx *c, char *buf, int32_t cap)
+{
+    if (!c || !buf || cap < 1) return RPBA_E_INVALID;
+    const std::string t = c->fl.solver == 1 ? c->dense.tuning() : std::string("block-Jacobi PCG (no tuned kernels)");
+    std::snprintf(buf, (size_t)cap, "%s", t.c_str());
     return RPBA_OK;
 }
 
