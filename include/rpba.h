@@ -176,6 +176,38 @@ void rpba_ctx_destroy(rpba_ctx *ctx);
 int rpba_dense_solve_batch(int32_t nsys, const int32_t *ncam, const double *S, const double *b, double *x,
                            int32_t variant, int32_t cap);
 
+/* Batched independent blocks: nblocks bundle blocks solved together on one
+ * GPU, each by the global robust LM of LSA::Adjust (rbacore.cpp:333-616) with
+ * its own LM state, as the partitioned mode solves its METIS blocks one LSA
+ * per OpenMP thread (rpbacore.cpp:219-235).  Every LM iteration of all blocks
+ * runs as one launch sequence over the union of the blocks (replayed from a
+ * hipGraph), and the blocks' reduced camera systems are factored together by
+ * the batched dense Cholesky (solver = 1 is implied; schur must be 0).  Passes
+ * run in lockstep: a block that ends a pass early waits, unchanged.  Each
+ * block needs at least 3 images, and all blocks the same image sizes.
+ * Block results are written back into the blocks by rpba_batch_download;
+ * stats and status are arrays of nblocks entries (either may be NULL). */
+typedef struct rpba_batch rpba_batch;
+typedef void (*rpba_batch_log_fn)(int32_t block, const char *line, void *user);
+int rpba_batch_create(rpba_batch **batch, int32_t nblocks, rpba_block *blocks, const rpba_flags *flags,
+                      rpba_batch_log_fn log, void *user);
+/* at most max_iters lockstep LM iterations (< 0: to the end) */
+int rpba_batch_run(rpba_batch *batch, int32_t max_iters, int32_t *iters_done, int32_t *finished);
+int rpba_batch_reset(rpba_batch *batch);
+int rpba_batch_download(rpba_batch *batch, rpba_stats *stats, int32_t *status);
+/* 1 (default): replay the LM iteration from a hipGraph; 0: eager launches */
+int rpba_batch_set_graph(rpba_batch *batch, int32_t on);
+int rpba_batch_set_timing(rpba_batch *batch, int32_t on);
+int rpba_batch_kernel_time(rpba_batch *batch, const char *kernel, double *total_ms, int64_t *launches);
+int rpba_batch_tuning(rpba_batch *batch, char *buf, int32_t cap);
+/* out[0..5] = cameras, points, observations, pairs, off-diagonal S blocks,
+ * doubles of dense storage, over the union */
+int rpba_batch_sizes(rpba_batch *batch, int64_t out[6]);
+void rpba_batch_destroy(rpba_batch *batch);
+/* create + run + download + destroy */
+int rpba_calc_plsa_batch(int32_t nblocks, rpba_block *blocks, const rpba_flags *flags, rpba_batch_log_fn log,
+                         void *user, rpba_stats *stats, int32_t *status);
+
 /* BAL text I/O (rpba.cpp:180-338), host only.
  * rpba_read_bal allocates the arrays of *io with malloc; free with
  * rpba_free_block.  iwidth = iheight = 1 as InputBlock sets them. */

@@ -17,6 +17,9 @@ EXPORTS = [
     "rpba_calc_plsa", "rpba_ctx_create", "rpba_ctx_run", "rpba_ctx_reset", "rpba_ctx_download",
     "rpba_ctx_set_timing", "rpba_ctx_kernel_time", "rpba_ctx_progress", "rpba_ctx_sizes", "rpba_ctx_destroy",
     "rpba_ctx_tuning", "rpba_dense_solve_batch",
+    "rpba_batch_create", "rpba_batch_run", "rpba_batch_reset", "rpba_batch_download", "rpba_batch_set_graph",
+    "rpba_batch_set_timing", "rpba_batch_kernel_time", "rpba_batch_tuning", "rpba_batch_sizes", "rpba_batch_destroy",
+    "rpba_calc_plsa_batch",
     "rpba_read_bal", "rpba_write_bal", "rpba_free_block", "rpba_read_parameters", "rpba_version",
 ]
 
@@ -51,6 +54,25 @@ def lib():
     L.rpba_dense_solve_batch.argtypes = [C.c_int32, P(C.c_int32), P(C.c_double), P(C.c_double), P(C.c_double),
                                          C.c_int32, C.c_int32]
     L.rpba_dense_solve_batch.restype = C.c_int
+    L.rpba_batch_create.argtypes = [P(C.c_void_p), C.c_int32, P(capi.RpbaBlock), P(capi.RpbaFlags),
+                                    capi.BATCH_LOG_FN, C.c_void_p]
+    L.rpba_batch_create.restype = C.c_int
+    L.rpba_batch_run.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32)]
+    L.rpba_batch_run.restype = C.c_int
+    L.rpba_batch_reset.argtypes = [C.c_void_p]
+    L.rpba_batch_reset.restype = C.c_int
+    L.rpba_batch_download.argtypes = [C.c_void_p, P(capi.RpbaStats), P(C.c_int32)]
+    L.rpba_batch_download.restype = C.c_int
+    L.rpba_batch_set_graph.argtypes = [C.c_void_p, C.c_int32]
+    L.rpba_batch_set_timing.argtypes = [C.c_void_p, C.c_int32]
+    L.rpba_batch_kernel_time.argtypes = [C.c_void_p, C.c_char_p, P(C.c_double), P(C.c_int64)]
+    L.rpba_batch_tuning.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+    L.rpba_batch_sizes.argtypes = [C.c_void_p, P(C.c_int64)]
+    L.rpba_batch_destroy.argtypes = [C.c_void_p]
+    L.rpba_batch_destroy.restype = None
+    L.rpba_calc_plsa_batch.argtypes = [C.c_int32, P(capi.RpbaBlock), P(capi.RpbaFlags), capi.BATCH_LOG_FN,
+                                       C.c_void_p, P(capi.RpbaStats), P(C.c_int32)]
+    L.rpba_calc_plsa_batch.restype = C.c_int
     L.rpba_ctx_destroy.argtypes = [C.c_void_p]
     L.rpba_ctx_destroy.restype = None
     L.rpba_read_bal.argtypes = [C.c_char_p, C.c_int32, P(capi.RpbaBlock)]

@@ -102,3 +102,83 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Batched independent blocks (rpba_batch_*): all blocks' LM iterations
+    run together on the GPU; download writes every block back."""
+
+    def __init__(self, scenes_, flags=None):
+        self.flags = flags or capi.make_flags()
+        self.blks = [capi.Block(s) for s in scenes_]
+        self.arr = (capi.RpbaBlock * len(self.blks))(*[b.c for b in self.blks])
+        self.lines = [[] for _ in self.blks]
+        self.fn = capi.BATCH_LOG_FN(self._cb)
+        self.h = C.c_void_p()
+        rc = lib().rpba_batch_create(C.byref(self.h), len(self.blks), self.arr, C.byref(self.flags), self.fn, None)
+        if rc != 0:
+            raise RuntimeError(f"rpba_batch_create failed with status {rc}")
+
+    def _cb(self, block, line, user):
+        self.lines[block].append(line.decode())
+
+    def run(self, max_iters=-1):
+        done, fin = C.c_int32(), C.c_int32()
+        rc = lib().rpba_batch_run(self.h, max_iters, C.byref(done), C.byref(fin))
+        return rc, done.value, bool(fin.value)
+
+    def reset(self):
+        return lib().rpba_batch_reset(self.h)
+
+    def set_graph(self, on):
+        return lib().rpba_batch_set_graph(self.h, 1 if on else 0)
+
+    def set_timing(self, on):
+        return lib().rpba_batch_set_timing(self.h, 1 if on else 0)
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(), C.c_int64()
+        lib().rpba_batch_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def tuning(self):
+        buf = C.create_string_buffer(1024)
+        lib().rpba_batch_tuning(self.h, buf, 1024)
+        return buf.value.decode()
+
+    def sizes(self):
+        out = (C.c_int64 * 6)()
+        lib().rpba_batch_sizes(self.h, out)
+        return dict(zip(["cams", "pts", "obs", "pairs", "blocks", "dense_doubles"], list(out)))
+
+    def download(self, trace_cap=512):
+        n = len(self.blks)
+        trs = [capi.Trace(trace_cap) for _ in range(n)]
+        stats = (capi.RpbaStats * n)(*[t.stats for t in trs])
+        status = (C.c_int32 * n)()
+        rc = lib().rpba_batch_download(self.h, stats, status)
+        out = []
+        for i in range(n):
+            trs[i].stats = stats[i]
+            out.append((int(status[i]), self.blks[i].result(), trs[i].as_dict(), list(self.lines[i])))
+        return rc, out
+
+    def close(self):
+        if self.h:
+            lib().rpba_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def calc_plsa_batch(scenes_, flags=None, trace_cap=512):
+    """Returns a list of (status, outputs, stats, log lines), one per block."""
+    b = Batch(scenes_, flags)
+    rc, n, fin = b.run(-1)
+    rd, out = b.download(trace_cap)
+    b.close()
+    return out

@@ -59,6 +59,7 @@ class RpbaStats(C.Structure):
 
 
 LOG_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_void_p)
+BATCH_LOG_FN = C.CFUNCTYPE(None, C.c_int32, C.c_char_p, C.c_void_p)
 
 
 def _ptr(a, ctype):

@@ -86,6 +86,10 @@ struct StepArgs {
     const double *Vinv, *gp, *sol, *X;
     double *Xn;
     PcgScalars *S;
+    // batched blocks (else null): per camera bit 0 = its block takes this
+    // step (decode skipped otherwise), bit 1 = first camera of its block
+    // (pose fixed, never decoded; image 0 when null)
+    const unsigned char *camflag = nullptr;
 };
 
 struct CostArgs {
@@ -97,6 +101,7 @@ struct CostArgs {
     const double *X;
     float *key;
     double *partial;
+    double *terms = nullptr;  // per-entry FP64 cost terms (batched blocks), or null
 };
 
 struct RobustArgs {
@@ -107,6 +112,8 @@ struct RobustArgs {
     unsigned char *flag;
     double *partial;
     int *ipartial;
+    // batched blocks (else null): iflag of the camera's block
+    const int *iflag_blk = nullptr, *camblk = nullptr;
 };
 
 struct ExtArgs {
@@ -136,5 +143,27 @@ void launch_reweight(const RobustArgs &a, double *wsum_out, int *nflag_out, hipS
 void launch_wsum(int n, const float *wx, const float *wy, double *partial, double *out, hipStream_t st);
 void launch_store_cams(const CamPtrs &cp, int nc, hipStream_t st);
 void launch_exteval(const ExtArgs &a, double *sg_out, int *t_out, hipStream_t st);
+
+// batched independent blocks (one union of block-diagonal structure):
+// cameras, points and track entries of block b are the contiguous ranges
+// coff[b].., poff[b].., eoff[b]..
+struct BatchArgs {
+    int nb, nc, npts, ntr;
+    const int *coff, *poff, *eoff;  // nb + 1 each
+    const int *camblk, *ptblk;      // camera -> block, point -> block
+    const int *tpt;                 // entry -> point
+};
+// per block, in entry order: out[3b] = sum terms, out[3b+1] = sum (wa + wb),
+// out[3b+2] = count of flags (each input may be null)
+void launch_block_sums(const BatchArgs &b, const double *terms, const float *wa, const float *wb,
+                       const unsigned char *flag, double *out, hipStream_t st);
+// per block: 1 if a camera step (as float) or a trial point is not finite
+void launch_block_nan(const BatchArgs &b, const double *sol, const double *Xn, int *nanb, hipStream_t st);
+// StoreResultsIteration for the blocks with accept[b] != 0: cameras, points, flags
+void launch_batch_store(const BatchArgs &b, const unsigned char *accept, const CamPtrs &cp, const double *Xn,
+                        double *X, const unsigned char *flag, unsigned char *flagst, hipStream_t st);
+// trial intrinsics back from the copy taken before the step, for blocks with nanb[b]
+void launch_batch_restore_in(const BatchArgs &b, const int *nanb, const double *In_bak, double *In, hipStream_t st);
+void launch_copy(double *dst, const double *src, size_t n, hipStream_t st);
 
 }  // namespace rpba

@@ -1212,6 +1212,8 @@ __global__ __launch_bounds__(64) void k_decode(StepArgs a)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.nc) return;
+    if (a.camflag && !(a.camflag[k] & 1)) return;
+    const bool first = a.camflag ? (a.camflag[k] & 2) != 0 : k == 0;
     double th[9];
     bool bad = false;
 #pragma unroll
@@ -1227,7 +1229,7 @@ __global__ __launch_bounds__(64) void k_decode(StepArgs a)
     }
     if (bad) a.S->nan_flag = 1;
     const unsigned mask = a.cam.mask[k];
-    if (k > 0) {
+    if (!first) {
         double Ra[9], R[9];
         cayley(th[0], th[1], th[2], Ra);
         const double *Rb = a.cam.Rb + 9 * (size_t)k;
@@ -1276,6 +1278,7 @@ __global__ __launch_bounds__(256) void k_cost(CostArgs a)
             if (a.write_key) a.key[q] = key;
             cst = ev.d1 * ev.d1 * (double)W0 + 2. * ev.d1 * ev.d2 * (double)W2 + ev.d2 * ev.d2 * (double)W1;
         }
+        if (a.terms) a.terms[q] = cst;
     }
     const double s = block_sum(cst, sh);
     if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
@@ -1364,7 +1367,8 @@ __global__ __launch_bounds__(256) void k_median(RobustArgs a)
         const unsigned raw = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
         const float med = __uint_as_float(raw);
         float s = 1.f / (med * 2.1981f);
-        if (a.iflag > 1) s *= 0.001f;
+        const int ifl = a.iflag_blk ? a.iflag_blk[a.camblk[k]] : a.iflag;
+        if (ifl > 1) s *= 0.001f;
         a.s00[k] = s;
     }
 }
@@ -1596,6 +1600,110 @@ void launch_exteval(const ExtArgs &a, double *sg_out, int *t_out, hipStream_t st
     const int nb = (int)nblk(a.npts > 0 ? a.npts : 1, 256);
     k_exteval<<<nb, 256, 0, st>>>(a);
     k_final_sum<<<1, 1024, 0, st>>>(a.partial, nb, a.ipartial, sg_out, t_out);
+}
+
+// ---------------------------------------------------------------------------
+// batched independent blocks
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_block_sums(BatchArgs b, const double *terms, const float *wa, const float *wb,
+                                                    const unsigned char *flag, double *out)
+{
+    __shared__ double sh[4];
+    const int blk = blockIdx.x;
+    const int e0 = b.eoff[blk], e1 = b.eoff[blk + 1];
+    double s0 = 0., s1 = 0., s2 = 0.;
+    for (int q = e0 + (int)threadIdx.x; q < e1; q += 256) {
+        if (terms) s0 += terms[q];
+        if (wa) s1 += (double)wa[q] + (double)wb[q];
+        if (flag) s2 += (double)flag[q];
+    }
+    s0 = block_sum(s0, sh);
+    s1 = block_sum(s1, sh);
+    s2 = block_sum(s2, sh);
+    if (threadIdx.x == 0) {
+        out[3 * blk] = s0;
+        out[3 * blk + 1] = s1;
+        out[3 * blk + 2] = s2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_block_nan(BatchArgs b, const double *sol, const double *Xn, int *nanb)
+{
+    __shared__ int any;
+    const int blk = blockIdx.x;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    int bad = 0;
+    for (int j = 9 * b.coff[blk] + (int)threadIdx.x; j < 9 * b.coff[blk + 1]; j += 256) {
+        const float xf = (float)sol[j];
+        if (isnan(xf) || isinf(xf)) bad = 1;
+    }
+    for (int j = 3 * b.poff[blk] + (int)threadIdx.x; j < 3 * b.poff[blk + 1]; j += 256)
+        if (!isfinite(Xn[j])) bad = 1;
+    if (bad) any = 1;  // every writer stores the same value
+    __syncthreads();
+    if (threadIdx.x == 0) nanb[blk] = any;
+}
+
+__global__ __launch_bounds__(256) void k_batch_store(BatchArgs b, const unsigned char *accept, CamPtrs cp,
+                                                     const double *Xn, double *X, const unsigned char *flag,
+                                                     unsigned char *flagst)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < b.nc && accept[b.camblk[j]]) {
+        const double *R = cp.Rn + 9 * (size_t)j, *C = cp.Cn + 3 * (size_t)j;
+        double *P = cp.Pst + 12 * (size_t)j;
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) P[r * 4 + c] = R[r * 3 + c];
+            P[r * 4 + 3] = R[r * 3 + 0] * C[0] + R[r * 3 + 1] * C[1] + R[r * 3 + 2] * C[2];
+        }
+        for (int s = 0; s < 3; ++s) cp.Ist[3 * (size_t)j + s] = cp.In[3 * (size_t)j + s];
+        for (int s = 0; s < 9; ++s) cp.th[9 * (size_t)j + s] = cp.thn[9 * (size_t)j + s];
+    }
+    if (j < b.npts && accept[b.ptblk[j]])
+        for (int s = 0; s < 3; ++s) X[3 * (size_t)j + s] = Xn[3 * (size_t)j + s];
+    for (int q = j; q < b.ntr; q += gridDim.x * blockDim.x)
+        if (accept[b.ptblk[b.tpt[q]]]) flagst[q] = flag[q];
+}
+
+__global__ __launch_bounds__(64) void k_batch_restore_in(BatchArgs b, const int *nanb, const double *In_bak, double *In)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= b.nc || !nanb[b.camblk[k]]) return;
+    for (int s = 0; s < 3; ++s) In[3 * (size_t)k + s] = In_bak[3 * (size_t)k + s];
+}
+
+__global__ __launch_bounds__(256) void k_copy_d(double *dst, const double *src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+void launch_block_sums(const BatchArgs &b, const double *terms, const float *wa, const float *wb,
+                       const unsigned char *flag, double *out, hipStream_t st)
+{
+    k_block_sums<<<b.nb, 256, 0, st>>>(b, terms, wa, wb, flag, out);
+}
+void launch_block_nan(const BatchArgs &b, const double *sol, const double *Xn, int *nanb, hipStream_t st)
+{
+    k_block_nan<<<b.nb, 256, 0, st>>>(b, sol, Xn, nanb);
+}
+void launch_batch_store(const BatchArgs &b, const unsigned char *accept, const CamPtrs &cp, const double *Xn,
+                        double *X, const unsigned char *flag, unsigned char *flagst, hipStream_t st)
+{
+    const long long n = std::max(1, std::max(b.nc, b.npts));
+    const int g = (int)std::min<long long>(nblk(n, 256), 4096);
+    k_batch_store<<<g, 256, 0, st>>>(b, accept, cp, Xn, X, flag, flagst);
+}
+void launch_batch_restore_in(const BatchArgs &b, const int *nanb, const double *In_bak, double *In, hipStream_t st)
+{
+    k_batch_restore_in<<<nblk(std::max(b.nc, 1), 64), 64, 0, st>>>(b, nanb, In_bak, In);
+}
+void launch_copy(double *dst, const double *src, size_t n, hipStream_t st)
+{
+    if (!n) return;
+    const int g = (int)std::min<size_t>((n + 255) / 256, 2048);
+    k_copy_d<<<g, 256, 0, st>>>(dst, src, n);
 }
 
 }  // namespace rpba

@@ -26,6 +26,26 @@ CONFIGS = {
 }
 
 
+# Batched blocks: 64 independent blocks, cameras per block log-uniform in
+# [20, 400], points 150 per camera clipped to [2000, 60000], track length
+# 2 + geometric with mean 6.  Block b has seed 4000 + b.
+BATCH_CONFIG = dict(seed=4, n_blocks=64, cam_lo=20, cam_hi=400, pts_per_cam=150, pts_lo=2000, pts_hi=60000,
+                    track_mean=6.0, outlier_frac=0.02)
+
+
+def generate_batch(seed=4, n_blocks=64, cam_lo=20, cam_hi=400, pts_per_cam=150, pts_lo=2000, pts_hi=60000,
+                   track_mean=6.0, outlier_frac=0.02):
+    """Independent blocks of uneven size (the batched-blocks workload)."""
+    rng = np.random.default_rng(seed)
+    ncams = np.round(np.exp(rng.uniform(np.log(cam_lo), np.log(cam_hi), n_blocks))).astype(int)
+    blocks = []
+    for b, nc in enumerate(ncams.tolist()):
+        npt = int(np.clip(pts_per_cam * nc, pts_lo, pts_hi))
+        blocks.append(generate(None, seed=1000 * seed + b, n_cams=nc, n_pts=npt, track="geometric",
+                               track_mean=track_mean, outlier_frac=outlier_frac))
+    return blocks
+
+
 def rodrigues(aa):
     """Angle-axis (n,3) -> rotation matrices (n,3,3), double, with the
     formula InputBlock uses (rpba.cpp:343-366)."""
@@ -73,15 +93,28 @@ def _tracks(rng, n_pts, n_cams, lengths):
     np.cumsum(lengths, out=off[1:])
     for t in np.unique(lengths):
         idx = np.nonzero(lengths == t)[0]
+        if 2 * t > n_cams:
+            # long tracks in few cameras: rejection would rarely succeed, so
+            # take the first t of a random permutation per row
+            rows = np.argsort(rng.random((idx.size, n_cams)), axis=1)[:, :t].astype(np.int64)
+            rows.sort(axis=1)
+            pos = off[idx][:, None] + np.arange(t)[None, :]
+            cams[pos.ravel()] = rows.ravel()
+            continue
         rows = rng.integers(0, n_cams, size=(idx.size, t))
         rows.sort(axis=1)
-        while True:
+        for _ in range(2000):
             bad = np.nonzero((np.diff(rows, axis=1) == 0).any(axis=1))[0] if t > 1 else np.zeros(0, int)
             if bad.size == 0:
                 break
             redo = rng.integers(0, n_cams, size=(bad.size, t))
             redo.sort(axis=1)
             rows[bad] = redo
+        else:
+            # rows that drew a repeated camera every time: permutation draw
+            bad = np.nonzero((np.diff(rows, axis=1) == 0).any(axis=1))[0]
+            if bad.size:
+                rows[bad] = np.sort(np.argsort(rng.random((bad.size, n_cams)), axis=1)[:, :t], axis=1)
         pos = off[idx][:, None] + np.arange(t)[None, :]
         cams[pos.ravel()] = rows.ravel()
     return cams, off

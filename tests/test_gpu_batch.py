@@ -1,0 +1,80 @@
+"""Batched independent blocks (rpba_batch_*): every block's result against
+the oracle's solve of that block alone with the same exact camera solve
+(solver = 1), under the trajectory rule of the parity tests; hipGraph replay
+against eager launches (bitwise); reset and re-run (bitwise)."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from rpba_amd import api, capi, scenes
+from tolerance import check_trajectory
+
+pytestmark = pytest.mark.gpu
+
+
+def _blocks(seed, sizes, outliers=0.03):
+    return [scenes.generate(None, seed=seed + i, n_cams=nc, n_pts=npt, track="uniform2_8", outlier_frac=outliers)
+            for i, (nc, npt) in enumerate(sizes)]
+
+
+def _compare_block(sc, flags, got, tag):
+    rc1, out1, st1, log1 = got
+    rc0, out0, st0, log0 = oracle.solve(sc, flags, fsites=0)
+    print(f"[{tag}] n_cams={sc['n_cams']} itt oracle {st0['itt']} batch {st1['itt']}", flush=True)
+    assert rc1 == rc0
+    assert st1["passes_done"] == st0["passes_done"]
+    assert st1["itt"] == st0["itt"], (st0["trace_sig0"], st1["trace_sig0"])
+    check_trajectory(st1["trace_sig0"], st0["trace_sig0"], st1["trace_pcg"], st0["trace_pcg"],
+                     st1["trace_accept"], st0["trace_accept"], st1["trace_nrobs"], st0["trace_nrobs"])
+    for p in range(st0["passes_done"]):
+        for key in ("sigma_in", "sigma_out"):
+            assert abs(st1[key][p] - st0[key][p]) <= 1e-6 * abs(st0[key][p])
+        for key in ("redundancy", "nobs", "nunknowns"):
+            assert st1[key][p] == st0[key][p]
+    assert np.array_equal(out1["obs_kept"], out0["obs_kept"])
+    assert np.array_equal(out1["pt_kept"], out0["pt_kept"])
+    for key in ("cam_P", "cam_f", "cam_d", "pts"):
+        a, b = out1[key].astype(np.float64), out0[key].astype(np.float64)
+        assert np.abs(a - b).max() <= 2e-6 * max(np.abs(b).max(), 1e-30), key
+    assert len(log1) == len(log0)
+    for l0, l1 in zip(log0, log1):
+        t0, t1 = l0.split(), l1.split()
+        assert t0[0] == t1[0] and t0[-4:] == t1[-4:]
+
+
+@pytest.mark.parametrize("robust", [1, 0])
+def test_batch_blocks_match_oracle(robust):
+    sizes = [(8, 300), (20, 900), (5, 200), (33, 1500), (12, 500), (3, 120)]
+    blocks = _blocks(500 + 10 * robust, sizes)
+    flags = capi.make_flags(robust=robust, solver=1)
+    print(f"[batch robust={robust}] {len(blocks)} blocks", flush=True)
+    res = api.calc_plsa_batch(blocks, flags)
+    for i, (sc, got) in enumerate(zip(blocks, res)):
+        _compare_block(sc, flags, got, f"block {i}")
+
+
+def test_batch_graph_replay_equals_eager_and_reset():
+    blocks = _blocks(700, [(10, 400), (25, 1200), (6, 250), (40, 2000)])
+    flags = capi.make_flags(robust=1, solver=1)
+    b = api.Batch(blocks, flags)
+    print("[batch graph] tuning:", b.tuning(), flush=True)
+    rc, n, fin = b.run(-1)
+    assert rc == 0 and fin
+    rd, g1 = b.download()
+    assert "graph launches 0" not in b.tuning()
+    b.reset()
+    b.set_graph(False)
+    rc, n2, fin = b.run(-1)
+    assert rc == 0 and fin and n2 == n
+    rd, e1 = b.download()
+    b.reset()
+    b.set_graph(True)
+    rc, n3, fin = b.run(-1)
+    rd, g2 = b.download()
+    b.close()
+    for (s1, o1, t1, _), (s2, o2, t2, _), (s3, o3, t3, _) in zip(g1, e1, g2):
+        assert s1 == s2 == s3 == 0
+        for key in ("cam_P", "cam_f", "cam_d", "pts", "obs_kept", "pt_kept"):
+            assert np.array_equal(o1[key], o2[key]) and np.array_equal(o1[key], o3[key]), key
+        assert np.array_equal(t1["trace_sig0"], t2["trace_sig0"])
+        assert np.array_equal(t1["trace_sig0"], t3["trace_sig0"])
