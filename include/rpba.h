@@ -84,7 +84,13 @@ typedef struct rpba_flags {
     int32_t k24out;          /* k24out (I/O scaling only, rpba.cpp:255-262)     */
     int32_t precision;       /* 0 = plain FP64; 1 = FP32 Jacobians and W V^-1 W^T
                                 products with FP64 sums, S, PCG, residuals and cost */
-    int32_t preconditioner;  /* 0 = block-Jacobi (rbacore.cpp:1838-1846)        */
+    int32_t preconditioner;  /* 0 = block-Jacobi C = S_kk^-1 (rbacore.cpp:1838-1846);
+                                1 = block-Jacobi with the first-order Neumann
+                                correction C - w C N C (N: off-diagonal part of S),
+                                w = pc_omega;
+                                2 = the same with w predicted per LM iteration by
+                                the learned graph network (rpba_lp_*), block-Jacobi
+                                when it declines or the PCG breaks down      */
     int32_t force_global;    /* 1 = run the global LSA even where the reference
                                 would dispatch to the partitioned path          */
     int32_t schur;           /* 0 = S formed block by block (the reference's
@@ -95,6 +101,7 @@ typedef struct rpba_flags {
                                 1 = dense Cholesky of the reduced camera system on
                                 the FP64 matrix cores (exact solve; a form of its
                                 own, for small or dense S)                       */
+    float pc_omega;          /* preconditioner 1: the Neumann weight w in [0, 1) */
 } rpba_flags;
 
 /* Optional per-solve report.  sigma values are sqrt(sig0 * ihw2) exactly as the
@@ -207,6 +214,52 @@ void rpba_batch_destroy(rpba_batch *batch);
 /* create + run + download + destroy */
 int rpba_calc_plsa_batch(int32_t nblocks, rpba_block *blocks, const rpba_flags *flags, rpba_batch_log_fn log,
                          void *user, rpba_stats *stats, int32_t *status);
+
+/* Learned correction of the block-Jacobi preconditioner (preconditioner = 2).
+ * A graph network over the camera covisibility graph of S (node per camera,
+ * edge per non-zero block S_kl with weight |S_kl|_F / sqrt(|S_kk|_F |S_ll|_F),
+ * RPBA_LP_FEATURES node features from the 9x9 block S_kk and the node's
+ * edges; 3 message-passing layers of width 64, BF16 weights and activations,
+ * FP32 arithmetic) predicts the weight w of M^-1 = C - w C N C.  Trained by
+ * the library's own kernels (Adam on (w - w*)^2, w* the weight with the
+ * fewest PCG iterations on a logged S).  No counterpart in the reference (its
+ * SparsePCG uses block-Jacobi, rbacore.cpp:1838-1846); the PCG stopping rule
+ * is unchanged. */
+#define RPBA_LP_FEATURES 48
+typedef struct rpba_lp rpba_lp;
+/* graphs in CSR over global node indices; the adjacency of every graph must
+ * be symmetric with equal weights both ways (as the camera graph of S is) */
+typedef struct rpba_lp_graphs {
+    int32_t n_graphs, n_nodes;
+    int64_t n_edges;
+    const int32_t *goff;     /* [n_graphs + 1] first node of each graph */
+    const float *feat;       /* [n_nodes * RPBA_LP_FEATURES] */
+    const int32_t *adj_off;  /* [n_nodes + 1] */
+    const int32_t *adj_nbr;  /* [n_edges] */
+    const float *adj_w;      /* [n_edges] */
+    const float *label;      /* [n_graphs] target weights, or NULL */
+} rpba_lp_graphs;
+/* params NULL: random initialisation from seed */
+int rpba_lp_create(rpba_lp **model, const float *params, uint64_t seed);
+void rpba_lp_destroy(rpba_lp *model);
+int rpba_lp_n_params(void);
+int rpba_lp_get_params(rpba_lp *model, float *params);
+int rpba_lp_predict(rpba_lp *model, const rpba_lp_graphs *graphs, float *omega);
+/* loss = mean over graphs of (w - label)^2 and its gradient (rpba_lp_n_params floats) */
+int rpba_lp_gradient(rpba_lp *model, const rpba_lp_graphs *graphs, float *omega, float *loss, float *grad);
+/* full-batch Adam steps; loss_trace[steps] (may be NULL) */
+int rpba_lp_train(rpba_lp *model, const rpba_lp_graphs *graphs, int32_t steps, float lr, float *loss_trace);
+/* attach a model to a context (preconditioner = 2; NULL: block-Jacobi) */
+int rpba_ctx_set_lp(rpba_ctx *ctx, rpba_lp *model);
+/* the weight w the last PCG of the context ran with */
+int rpba_ctx_last_omega(rpba_ctx *ctx, float *omega);
+/* training sample from the context's last LM iteration (S formed, PCG
+ * solver): the camera graph of its S (feat [n_cams * RPBA_LP_FEATURES],
+ * adj_off [n_cams + 1], adj_nbr / adj_w [*n_edges capacity in, count out;
+ * written when the capacity suffices]) and the PCG iterations of that S and b
+ * for each weight in omegas (0 = block-Jacobi) */
+int rpba_ctx_lp_sample(rpba_ctx *ctx, int32_t n_omega, const float *omegas, int32_t *iters, float *feat,
+                       int32_t *adj_off, int32_t *adj_nbr, float *adj_w, int64_t *n_edges);
 
 /* BAL text I/O (rpba.cpp:180-338), host only.
  * rpba_read_bal allocates the arrays of *io with malloc; free with

@@ -45,6 +45,9 @@ def _load():
     lib.orc_step.restype = C.c_int
     lib.oracle_pcg_dense.argtypes = [C.c_int, P(C.c_int), P(C.c_double), P(C.c_double), P(C.c_double)]
     lib.oracle_pcg_dense.restype = C.c_int
+    lib.oracle_pcg_dense_w.argtypes = [C.c_int, P(C.c_int), P(C.c_double), P(C.c_double), P(C.c_double),
+                                       C.c_double]
+    lib.oracle_pcg_dense_w.restype = C.c_int
     lib.oracle_chol_solve.argtypes = [C.c_int, P(C.c_double), P(C.c_double), P(C.c_double)]
     lib.oracle_chol_solve.restype = C.c_int
     lib.oracle_cayley.argtypes = [C.c_double, C.c_double, C.c_double, P(C.c_double)]
@@ -155,12 +158,13 @@ class Handle:
         return x, its
 
 
-def pcg_dense(block_sizes, S, b):
+def pcg_dense(block_sizes, S, b, omega=0.0):
     bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
     S = np.ascontiguousarray(S, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.zeros_like(b)
-    its = lib().oracle_pcg_dense(bs.size, _p(bs, C.c_int), _p(S, C.c_double), _p(b, C.c_double), _p(x, C.c_double))
+    its = lib().oracle_pcg_dense_w(bs.size, _p(bs, C.c_int), _p(S, C.c_double), _p(b, C.c_double),
+                                   _p(x, C.c_double), omega)
     return x, its
 
 

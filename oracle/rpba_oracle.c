@@ -237,6 +237,7 @@ float oracle_upper_median(const float *v, int n)
 typedef struct orc {
     int fsites, robust, nraddpi, addf[3];
     int solver;              /* 0 = block-Jacobi PCG (the reference); 1 = dense Cholesky */
+    double omega;            /* preconditioner 1: Neumann weight (0 = block-Jacobi) */
     int nimg, XXnr;
     float ihw2;
     double defdiff;
@@ -314,6 +315,7 @@ orc *orc_create(const rpba_block *io, const rpba_flags *f, int fsites)
     const int nc = io->n_cams, np = io->n_pts, no = io->n_obs;
     o->fsites = fsites;
     o->solver = f->solver;
+    o->omega = f->preconditioner == 1 ? (double)f->pc_omega : 0.;
     o->robust = f->robust != 0;
     o->addf[0] = f->est_k2 != 0; o->addf[1] = f->est_k4 != 0; o->addf[2] = f->est_f != 0;
     o->nraddpi = o->addf[0] + o->addf[1] + o->addf[2];
@@ -857,11 +859,68 @@ static double dotb(int nb, int begini, const int *off, const double *a, const do
     return c;
 }
 
+/* y = N d over the off-diagonal touched blocks (both triangles) */
+static void spmv_off(int nb, int begini, const int *off, double *const *Sb, const unsigned char *Sf,
+                     const double *d, double *y)
+{
+    for (int i = begini; i < nb; ++i)
+        for (int j = off[i]; j < off[i + 1]; ++j) y[j] = 0.;
+    for (int l = begini; l < nb; ++l) {
+        const int nl_ = off[l + 1] - off[l];
+        for (int k = l - 1; k >= begini; --k) {
+            const size_t id = (size_t)k * nb + l;
+            if (!Sf[id]) continue;
+            const double *B = Sb[id];
+            const int nk = off[k + 1] - off[k];
+            for (int r = 0; r < nk; ++r) {
+                double s = 0.;
+                for (int c = 0; c < nl_; ++c) s += B[r * MAXB + c] * d[off[l] + c];
+                y[off[k] + r] += s;
+            }
+            for (int c = 0; c < nl_; ++c) {
+                double s = 0.;
+                for (int r = 0; r < nk; ++r) s += B[r * MAXB + c] * d[off[k] + r];
+                y[off[l] + c] += s;
+            }
+        }
+    }
+}
+
+/* preconditioner: h = C r (block-Jacobi), or with omega > 0 the first-order
+ * Neumann correction h = C r - omega C N C r (N = off-diagonal part of S) */
+static void precond(int nb, int begini, const int *off, double *const *Sb, const unsigned char *Sf,
+                    const double *Cinv, double omega, const double *r, double *h, double *t, double *u)
+{
+    for (int i = begini; i < nb; ++i) {
+        const int m = off[i + 1] - off[i];
+        const double *Ci = Cinv + (size_t)i * MAXB * MAXB;
+        for (int rr = 0; rr < m; ++rr) {
+            double s = 0.;
+            for (int cc = 0; cc < m; ++cc) s += Ci[rr * m + cc] * r[off[i] + cc];
+            h[off[i] + rr] = s;
+        }
+    }
+    if (omega == 0.) return;
+    for (int j = off[begini]; j < off[nb]; ++j) t[j] = h[j];
+    spmv_off(nb, begini, off, Sb, Sf, t, u);
+    for (int i = begini; i < nb; ++i) {
+        const int m = off[i + 1] - off[i];
+        const double *Ci = Cinv + (size_t)i * MAXB * MAXB;
+        for (int rr = 0; rr < m; ++rr) {
+            double s = 0.;
+            for (int cc = 0; cc < m; ++cc) s += Ci[rr * m + cc] * u[off[i] + cc];
+            h[off[i] + rr] = t[off[i] + rr] - omega * s;
+        }
+    }
+}
+
 /* Block-Jacobi PCG with the reference's step-norm stopping rule and
  * best-iterate return (SparsePCG rbacore.cpp:1807-1914).  b in, x out.
- * Returns the iteration count. */
-int oracle_block_pcg(int nb, int begini, const int *off, double *const *Sb, const unsigned char *Sf,
-                     const double *b, double *xout)
+ * omega > 0 adds the first-order Neumann correction to the preconditioner
+ * (the corrected form; the reference has omega = 0).  Returns the iteration
+ * count. */
+int oracle_block_pcg_w(int nb, int begini, const int *off, double *const *Sb, const unsigned char *Sf,
+                       const double *b, double *xout, double omega)
 {
     const int n = off[nb];
     double *x = (double *)xcalloc((size_t)n + 1, sizeof(double));
@@ -873,6 +932,8 @@ int oracle_block_pcg(int nb, int begini, const int *off, double *const *Sb, cons
     double *h = (double *)xcalloc((size_t)n + 1, sizeof(double));
     double *h1 = (double *)xcalloc((size_t)n + 1, sizeof(double));
     double *hd = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *tw = (double *)xcalloc((size_t)n + 1, sizeof(double));
+    double *uw = (double *)xcalloc((size_t)n + 1, sizeof(double));
     double *Cinv = (double *)xcalloc((size_t)nb * MAXB * MAXB, sizeof(double));
     /* every vector starts as a copy of b; x = 0 (rbacore.cpp:1826-1836) */
     for (int j = 0; j < n; ++j) r[j] = r1[j] = d[j] = Ad[j] = h[j] = h1[j] = xo[j] = b[j];
@@ -884,14 +945,9 @@ int oracle_block_pcg(int nb, int begini, const int *off, double *const *Sb, cons
             for (int cc = 0; cc < m; ++cc) a[rr * m + cc] = Sb[(size_t)i * nb + i][rr * MAXB + cc];
         if (m > 0) oracle_invlu(m, a, Cinv + (size_t)i * MAXB * MAXB);
         msize += m;
-        const double *Ci = Cinv + (size_t)i * MAXB * MAXB;
-        for (int rr = 0; rr < m; ++rr) {
-            double s = 0.;
-            for (int cc = 0; cc < m; ++cc) s += Ci[rr * m + cc] * r[off[i] + cc];
-            h[off[i] + rr] = s;
-        }
-        for (int rr = 0; rr < m; ++rr) d[off[i] + rr] = h[off[i] + rr];
     }
+    precond(nb, begini, off, Sb, Sf, Cinv, omega, r, h, tw, uw);
+    for (int j = off[begini]; j < off[nb]; ++j) d[j] = h[j];
     if (msize < 1000) msize = 1000;
     double mold = 1.e50, m0 = 1.e50;
     int k;
@@ -908,14 +964,8 @@ int oracle_block_pcg(int nb, int begini, const int *off, double *const *Sb, cons
                 r1[j] = r[j] - dr;
                 m += dr * dr;
             }
-            const int mm = off[i + 1] - off[i];
-            const double *Ci = Cinv + (size_t)i * MAXB * MAXB;
-            for (int rr = 0; rr < mm; ++rr) {
-                double s = 0.;
-                for (int cc = 0; cc < mm; ++cc) s += Ci[rr * mm + cc] * r1[off[i] + cc];
-                h1[off[i] + rr] = s;
-            }
         }
+        precond(nb, begini, off, Sb, Sf, Cinv, omega, r1, h1, tw, uw);
         if (k == 0) m0 = m;
         if (m < mold) {
             memcpy(xo, x, sizeof(double) * (size_t)n);
@@ -938,7 +988,14 @@ int oracle_block_pcg(int nb, int begini, const int *off, double *const *Sb, cons
     /* entries of blocks below begini keep the copy of b (exivold = exiv) */
     memcpy(xout, xo, sizeof(double) * (size_t)n);
     free(x); free(xo); free(r); free(r1); free(d); free(Ad); free(h); free(h1); free(hd); free(Cinv);
+    free(tw); free(uw);
     return (k < msize) ? k + 1 : k;
+}
+
+int oracle_block_pcg(int nb, int begini, const int *off, double *const *Sb, const unsigned char *Sf,
+                     const double *b, double *xout)
+{
+    return oracle_block_pcg_w(nb, begini, off, Sb, Sf, b, xout, 0.);
 }
 
 /* Dense Cholesky solve of a symmetric positive definite system (row-major
@@ -1022,7 +1079,7 @@ static int bundle_sep(orc *o, double mult)
         dense_solve(o, sol);
         o->pcg_last = 0;
     } else {
-        o->pcg_last = oracle_block_pcg(nc, o->begini, o->camparv, o->Sb, o->Sf, o->ex, sol);
+        o->pcg_last = oracle_block_pcg_w(nc, o->begini, o->camparv, o->Sb, o->Sf, o->ex, sol, o->omega);
     }
     o->pcg_total += o->pcg_last;
     memcpy(o->ex, sol, sizeof(double) * (size_t)o->nrioff);
@@ -1484,7 +1541,13 @@ int orc_step(orc *o, double mult, double *x)
 }
 
 /* PCG on a dense symmetric matrix cut into blocks of the given sizes */
+int oracle_pcg_dense_w(int nb, const int *bs, const double *Sd, const double *b, double *x, double omega);
 int oracle_pcg_dense(int nb, const int *bs, const double *Sd, const double *b, double *x)
+{
+    return oracle_pcg_dense_w(nb, bs, Sd, b, x, 0.);
+}
+
+int oracle_pcg_dense_w(int nb, const int *bs, const double *Sd, const double *b, double *x, double omega)
 {
     int *off = (int *)xcalloc((size_t)nb + 1, sizeof(int));
     for (int i = 0; i < nb; ++i) off[i + 1] = off[i] + bs[i];
@@ -1504,7 +1567,7 @@ int oracle_pcg_dense(int nb, const int *bs, const double *Sd, const double *b, d
             Sb[(size_t)k * nb + l] = B;
             Sf[(size_t)k * nb + l] = 1;
         }
-    const int it = oracle_block_pcg(nb, 0, off, Sb, Sf, b, x);
+    const int it = oracle_block_pcg_w(nb, 0, off, Sb, Sf, b, x, omega);
     for (size_t i = 0; i < (size_t)nb * nb; ++i) free(Sb[i]);
     free(Sb); free(Sf); free(off);
     return it;

@@ -20,6 +20,8 @@ EXPORTS = [
     "rpba_batch_create", "rpba_batch_run", "rpba_batch_reset", "rpba_batch_download", "rpba_batch_set_graph",
     "rpba_batch_set_timing", "rpba_batch_kernel_time", "rpba_batch_tuning", "rpba_batch_sizes", "rpba_batch_destroy",
     "rpba_calc_plsa_batch",
+    "rpba_lp_create", "rpba_lp_destroy", "rpba_lp_n_params", "rpba_lp_get_params", "rpba_lp_predict",
+    "rpba_lp_gradient", "rpba_lp_train", "rpba_ctx_set_lp", "rpba_ctx_last_omega", "rpba_ctx_lp_sample",
     "rpba_read_bal", "rpba_write_bal", "rpba_free_block", "rpba_read_parameters", "rpba_version",
 ]
 
@@ -73,6 +75,20 @@ def lib():
     L.rpba_calc_plsa_batch.argtypes = [C.c_int32, P(capi.RpbaBlock), P(capi.RpbaFlags), capi.BATCH_LOG_FN,
                                        C.c_void_p, P(capi.RpbaStats), P(C.c_int32)]
     L.rpba_calc_plsa_batch.restype = C.c_int
+    G = P(capi.RpbaLpGraphs)
+    L.rpba_lp_create.argtypes = [P(C.c_void_p), P(C.c_float), C.c_uint64]
+    L.rpba_lp_create.restype = C.c_int
+    L.rpba_lp_destroy.argtypes = [C.c_void_p]
+    L.rpba_lp_destroy.restype = None
+    L.rpba_lp_n_params.restype = C.c_int
+    L.rpba_lp_get_params.argtypes = [C.c_void_p, P(C.c_float)]
+    L.rpba_lp_predict.argtypes = [C.c_void_p, G, P(C.c_float)]
+    L.rpba_lp_gradient.argtypes = [C.c_void_p, G, P(C.c_float), P(C.c_float), P(C.c_float)]
+    L.rpba_lp_train.argtypes = [C.c_void_p, G, C.c_int32, C.c_float, P(C.c_float)]
+    L.rpba_ctx_set_lp.argtypes = [C.c_void_p, C.c_void_p]
+    L.rpba_ctx_last_omega.argtypes = [C.c_void_p, P(C.c_float)]
+    L.rpba_ctx_lp_sample.argtypes = [C.c_void_p, C.c_int32, P(C.c_float), P(C.c_int32), P(C.c_float), P(C.c_int32),
+                                     P(C.c_int32), P(C.c_float), P(C.c_int64)]
     L.rpba_ctx_destroy.argtypes = [C.c_void_p]
     L.rpba_ctx_destroy.restype = None
     L.rpba_read_bal.argtypes = [C.c_char_p, C.c_int32, P(capi.RpbaBlock)]

@@ -81,6 +81,39 @@ class Context:
         lib().rpba_ctx_sizes(self.h, out)
         return dict(zip(["cams", "pts", "obs", "pairs", "blocks", "sdim"], list(out)))
 
+    def set_lp(self, model):
+        """Attach a graph network (LpModel) for preconditioner = 2."""
+        self._lp = model
+        return lib().rpba_ctx_set_lp(self.h, model.h if model is not None else None)
+
+    def last_omega(self):
+        w = C.c_float()
+        lib().rpba_ctx_last_omega(self.h, C.byref(w))
+        return w.value
+
+    def lp_sample(self, omegas):
+        """Camera graph of the last LM iteration's S and the PCG iterations of
+        that S and b for each weight in omegas."""
+        import numpy as np
+        P = C.POINTER
+        nc = self.blk.n_cams
+        om = np.ascontiguousarray(omegas, dtype=np.float32)
+        its = np.zeros(om.size, dtype=np.int32)
+        feat = np.zeros((nc, capi.LP_FEATURES), dtype=np.float32)
+        off = np.zeros(nc + 1, dtype=np.int32)
+        ne = C.c_int64(0)
+        rc = lib().rpba_ctx_lp_sample(self.h, 0, None, None, None, None, None, None, C.byref(ne))
+        if rc != 0:
+            raise RuntimeError(f"rpba_ctx_lp_sample failed with status {rc}")
+        nbr = np.zeros(max(ne.value, 1), dtype=np.int32)
+        w = np.zeros(max(ne.value, 1), dtype=np.float32)
+        rc = lib().rpba_ctx_lp_sample(self.h, om.size, om.ctypes.data_as(P(C.c_float)), its.ctypes.data_as(P(C.c_int32)),
+                                      feat.ctypes.data_as(P(C.c_float)), off.ctypes.data_as(P(C.c_int32)),
+                                      nbr.ctypes.data_as(P(C.c_int32)), w.ctypes.data_as(P(C.c_float)), C.byref(ne))
+        if rc != 0:
+            raise RuntimeError(f"rpba_ctx_lp_sample failed with status {rc}")
+        return {"feat": feat, "adj_off": off, "adj_nbr": nbr[:ne.value], "adj_w": w[:ne.value], "iters": its}
+
     def tuning(self):
         """The dense form's autotuner choice and timings (one line)."""
         buf = C.create_string_buffer(512)
@@ -182,3 +215,107 @@ def calc_plsa_batch(scenes_, flags=None, trace_cap=512):
     rd, out = b.download(trace_cap)
     b.close()
     return out
+
+
+def lp_graphs(graphs, labels=None):
+    """Pack a list of graphs (dicts feat, adj_off, adj_nbr, adj_w) into one
+    rpba_lp_graphs; returns (struct, arrays kept alive)."""
+    import numpy as np
+    P = C.POINTER
+    n = [g["feat"].shape[0] for g in graphs]
+    goff = np.concatenate([[0], np.cumsum(n)]).astype(np.int32)
+    feat = np.ascontiguousarray(np.concatenate([g["feat"] for g in graphs]), dtype=np.float32)
+    offs, nbrs, ws = [np.zeros(1, np.int64)], [], []
+    e0 = 0
+    for g, n0 in zip(graphs, goff[:-1]):
+        offs.append(np.asarray(g["adj_off"][1:], np.int64) + e0)
+        nbrs.append(np.asarray(g["adj_nbr"], np.int64) + n0)
+        ws.append(np.asarray(g["adj_w"], np.float32))
+        e0 += len(g["adj_nbr"])
+    adj_off = np.concatenate(offs).astype(np.int32)
+    adj_nbr = np.ascontiguousarray(np.concatenate(nbrs) if nbrs else np.zeros(0), dtype=np.int32)
+    adj_w = np.ascontiguousarray(np.concatenate(ws) if ws else np.zeros(0), dtype=np.float32)
+    lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.float32)
+    keep = [goff, feat, adj_off, adj_nbr, adj_w, lab]
+    st = capi.RpbaLpGraphs(len(graphs), int(goff[-1]), int(adj_nbr.size),
+                           goff.ctypes.data_as(P(C.c_int32)), feat.ctypes.data_as(P(C.c_float)),
+                           adj_off.ctypes.data_as(P(C.c_int32)),
+                           adj_nbr.ctypes.data_as(P(C.c_int32)) if adj_nbr.size else None,
+                           adj_w.ctypes.data_as(P(C.c_float)) if adj_w.size else None,
+                           lab.ctypes.data_as(P(C.c_float)) if lab is not None else None)
+    return st, keep
+
+
+class LpModel:
+    """The learned preconditioner's graph network on the GPU (rpba_lp_*)."""
+
+    def __init__(self, params=None, seed=0):
+        import numpy as np
+        self.h = C.c_void_p()
+        p = None
+        if params is not None:
+            self._p = np.ascontiguousarray(params, dtype=np.float32)
+            p = self._p.ctypes.data_as(C.POINTER(C.c_float))
+        rc = lib().rpba_lp_create(C.byref(self.h), p, seed)
+        if rc != 0:
+            raise RuntimeError(f"rpba_lp_create failed with status {rc}")
+
+    @staticmethod
+    def n_params():
+        return lib().rpba_lp_n_params()
+
+    @classmethod
+    def load(cls, path):
+        import numpy as np
+        return cls(np.fromfile(path, dtype="<f4"))
+
+    def params(self):
+        import numpy as np
+        out = np.zeros(self.n_params(), dtype=np.float32)
+        lib().rpba_lp_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def save(self, path):
+        self.params().astype("<f4").tofile(path)
+
+    def predict(self, graphs):
+        import numpy as np
+        st, keep = lp_graphs(graphs)
+        out = np.zeros(len(graphs), dtype=np.float32)
+        rc = lib().rpba_lp_predict(self.h, C.byref(st), out.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            raise RuntimeError(f"rpba_lp_predict failed with status {rc}")
+        return out
+
+    def gradient(self, graphs, labels):
+        import numpy as np
+        st, keep = lp_graphs(graphs, labels)
+        om = np.zeros(len(graphs), dtype=np.float32)
+        g = np.zeros(self.n_params(), dtype=np.float32)
+        loss = C.c_float()
+        P = C.POINTER
+        rc = lib().rpba_lp_gradient(self.h, C.byref(st), om.ctypes.data_as(P(C.c_float)), C.byref(loss),
+                                    g.ctypes.data_as(P(C.c_float)))
+        if rc != 0:
+            raise RuntimeError(f"rpba_lp_gradient failed with status {rc}")
+        return om, loss.value, g
+
+    def train(self, graphs, labels, steps, lr):
+        import numpy as np
+        st, keep = lp_graphs(graphs, labels)
+        tr = np.zeros(max(steps, 1), dtype=np.float32)
+        rc = lib().rpba_lp_train(self.h, C.byref(st), steps, lr, tr.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            raise RuntimeError(f"rpba_lp_train failed with status {rc}")
+        return tr[:steps]
+
+    def close(self):
+        if self.h:
+            lib().rpba_lp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -41,7 +41,7 @@ class RpbaFlags(C.Structure):
         ("robust", C.c_int32), ("exteval", C.c_int32), ("threads", C.c_int32),
         ("est_k2", C.c_int32), ("est_k4", C.c_int32), ("est_f", C.c_int32),
         ("k24out", C.c_int32), ("precision", C.c_int32), ("preconditioner", C.c_int32),
-        ("force_global", C.c_int32), ("schur", C.c_int32), ("solver", C.c_int32),
+        ("force_global", C.c_int32), ("schur", C.c_int32), ("solver", C.c_int32), ("pc_omega", C.c_float),
     ]
 
 
@@ -59,6 +59,15 @@ class RpbaStats(C.Structure):
 
 
 LOG_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_void_p)
+LP_FEATURES = 48
+
+
+class RpbaLpGraphs(C.Structure):
+    _fields_ = [
+        ("n_graphs", C.c_int32), ("n_nodes", C.c_int32), ("n_edges", C.c_int64),
+        ("goff", _i32p), ("feat", _f32p), ("adj_off", _i32p), ("adj_nbr", _i32p), ("adj_w", _f32p),
+        ("label", _f32p),
+    ]
 BATCH_LOG_FN = C.CFUNCTYPE(None, C.c_int32, C.c_char_p, C.c_void_p)
 
 
@@ -103,9 +112,9 @@ class Block:
 
 
 def make_flags(robust=1, exteval=0, threads=1, k2=1, k4=1, f=1, k24out=1, precision=0,
-               preconditioner=0, force_global=0, schur=0, solver=0):
+               preconditioner=0, force_global=0, schur=0, solver=0, pc_omega=0.0):
     return RpbaFlags(robust, exteval, threads, k2, k4, f, k24out, precision, preconditioner, force_global, schur,
-                     solver)
+                     solver, pc_omega)
 
 
 class Trace:

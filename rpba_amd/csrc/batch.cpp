@@ -89,7 +89,8 @@ struct rpba_batch {
     DBuf<float> d_wx, d_wy, d_wxy, d_key, d_s00;
     DBuf<unsigned char> d_flag, d_flagst, d_camflag, d_accept, d_Wrec, d_Yrec;
     DBuf<double> d_X, d_Xn, d_X0, d_Vinv, d_gp, d_Sd, d_b, d_sol, d_In_bak, d_Su, d_Dd, d_terms, d_bsum, d_partial;
-    DBuf<int> d_coff, d_poff, d_eoff, d_camblk, d_ptblk, d_iflag, d_nanb, d_ipartial, d_iscal;
+    DBuf<double> d_bpart;
+    DBuf<int> d_coff, d_poff, d_eoff, d_camblk, d_ptblk, d_iflag, d_nanb, d_ipartial, d_iscal, d_npart;
     DBuf<double> d_scal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -328,6 +329,8 @@ static int batch_init(rpba_batch *c)
     rc |= c->d_iflag.alloc((size_t)nb);
     rc |= c->d_nanb.alloc((size_t)nb);
     rc |= c->d_bsum.alloc(3 * (size_t)nb);
+    rc |= c->d_bpart.alloc(3 * (size_t)block_slices() * nb);
+    rc |= c->d_npart.alloc((size_t)block_slices() * nb);
     rc |= c->d_coff.alloc((size_t)nb + 1);
     rc |= c->d_poff.alloc((size_t)nb + 1);
     rc |= c->d_eoff.alloc((size_t)nb + 1);
@@ -588,8 +591,8 @@ static void iteration_launches(rpba_batch *c)
     BatchArgs ba = c->bargs();
     c->timed("block_sums", [&] {
         launch_block_sums(ba, c->d_terms.p, rw ? c->d_wx.p : nullptr, rw ? c->d_wy.p : nullptr,
-                          rw ? c->d_flag.p : nullptr, c->d_bsum.p, c->st);
-        launch_block_nan(ba, c->d_sol.p, c->d_Xn.p, c->d_nanb.p, c->st);
+                          rw ? c->d_flag.p : nullptr, c->d_bsum.p, c->d_bpart.p, c->st);
+        launch_block_nan(ba, c->d_sol.p, c->d_Xn.p, c->d_nanb.p, c->d_npart.p, c->st);
     });
 }
 
@@ -613,7 +616,7 @@ static int block_costs(rpba_batch *c, bool with_weights)
     launch_cost(ca, c->d_scal.p, c->st);
     BatchArgs ba = c->bargs();
     launch_block_sums(ba, c->d_terms.p, with_weights ? c->ds.w0x : nullptr, with_weights ? c->ds.w0y : nullptr,
-                      nullptr, c->d_bsum.p, c->st);
+                      nullptr, c->d_bsum.p, c->d_bpart.p, c->st);
     HIPCHK(hipMemcpyAsync(c->h_bsum, c->d_bsum.p, sizeof(double) * 3 * (size_t)c->nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return RPBA_OK;
@@ -841,6 +844,7 @@ extern "C" void rpba_batch_destroy(rpba_batch *c)
     c->d_Dd.release(); c->d_terms.release(); c->d_bsum.release(); c->d_partial.release();
     c->d_coff.release(); c->d_poff.release(); c->d_eoff.release(); c->d_camblk.release(); c->d_ptblk.release();
     c->d_iflag.release(); c->d_nanb.release(); c->d_ipartial.release(); c->d_iscal.release(); c->d_scal.release();
+    c->d_bpart.release(); c->d_npart.release();
     c->c_Rb.release(); c->c_Cb.release(); c->c_Rn.release(); c->c_Cn.release(); c->c_In.release();
     c->c_th.release(); c->c_thn.release(); c->c_Pst.release(); c->c_Ist.release(); c->c_mask.release();
     c->d_pcg.release();

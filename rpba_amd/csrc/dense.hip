@@ -71,59 +71,75 @@ __global__ __launch_bounds__(128) void k_dense_off(DenseArgs a, const int *camsy
     a.A[a.base[m] + (kl + i) * ld + ll + j] = Su[81LL * e + t];
 }
 
-// diagonal tile (k, k): unblocked right-looking Cholesky in LDS
+// diagonal tile (k, k): unblocked right-looking Cholesky in LDS (column-major,
+// odd stride).  Step j: pivot, column scale, then the trailing update with
+// thread t on column j+1.. (t & 63) and every fourth row from t >> 6, four
+// rows loaded before any is written.  Element order of operations:
+// a_rc -= l_rj l_cj for j = 0, 1, ..., then l_rj = a_rj / l_jj, as in the
+// textbook column algorithm.
 __global__ __launch_bounds__(256) void k_potrf(DenseArgs a, const DenseItem *it, int k)
 {
     const DenseItem w = it[blockIdx.x];
     const int ld = a.ld[w.m];
     double *A = a.A + a.base[w.m] + (long long)k * T * ld + (long long)k * T;
     __shared__ double L[T * LP];
-    const int t = threadIdx.x;
-    for (int e = t; e < T * T; e += 256) {
-        const int c = e / T, r = e - c * T;
-        L[c * LP + r] = A[(long long)c * ld + r];
-    }
+    const int t = threadIdx.x, c = t & 63, rg = t >> 6;
+    for (int cc = rg; cc < T; cc += 4) L[cc * LP + c] = A[(long long)cc * ld + c];
     __syncthreads();
     for (int j = 0; j < T; ++j) {
-        const double d = sqrt(L[j * LP + j]);  // a pivot <= 0 gives NaN, caught by the step's NaN check
+        // a pivot <= 0 gives NaN, caught by the step's NaN check
+        const double d = sqrt(L[j * LP + j]);
         __syncthreads();
-        if (t == 0) L[j * LP + j] = d;
-        for (int r = j + 1 + t; r < T; r += 256) L[j * LP + r] = L[j * LP + r] / d;
+        if (t == j) L[j * LP + j] = d;
+        if (t > j && t < T) L[j * LP + t] = L[j * LP + t] / d;
         __syncthreads();
-        const int mm = T - j - 1;
-        for (int e = t; e < mm * mm; e += 256) {
-            const int c = j + 1 + e / mm, r = j + 1 + e % mm;
-            if (r >= c) L[c * LP + r] -= L[j * LP + r] * L[j * LP + c];
+        if (c > j) {
+            const double ljc = L[j * LP + c];
+            double *Lc = L + c * LP;
+            const double *Lj = L + j * LP;
+            int r = c + rg;
+            for (; r + 12 < T; r += 16) {
+                const double a0 = Lj[r], a1 = Lj[r + 4], a2 = Lj[r + 8], a3 = Lj[r + 12];
+                const double b0 = Lc[r], b1 = Lc[r + 4], b2 = Lc[r + 8], b3 = Lc[r + 12];
+                Lc[r] = b0 - a0 * ljc;
+                Lc[r + 4] = b1 - a1 * ljc;
+                Lc[r + 8] = b2 - a2 * ljc;
+                Lc[r + 12] = b3 - a3 * ljc;
+            }
+            for (; r < T; r += 4) Lc[r] = Lc[r] - Lj[r] * ljc;
         }
         __syncthreads();
     }
-    for (int e = t; e < T * T; e += 256) {
-        const int c = e / T, r = e - c * T;
-        if (r >= c) A[(long long)c * ld + r] = L[c * LP + r];
-    }
+    for (int cc = rg; cc < T; cc += 4)
+        if (c >= cc) A[(long long)cc * ld + c] = L[cc * LP + c];
 }
 
 // tile (i, k) below the diagonal: X L_kk^T = A_ik by forward substitution,
-// one lane per row
+// lane r holding row r of X in registers, L_kk in LDS (broadcast reads)
 __global__ __launch_bounds__(64) void k_trsm(DenseArgs a, const DenseItem *it, int k)
 {
     const DenseItem w = it[blockIdx.x];
     const int ld = a.ld[w.m];
     const double *Lkk = a.A + a.base[w.m] + (long long)k * T * ld + (long long)k * T;
     double *X = a.A + a.base[w.m] + (long long)k * T * ld + (long long)w.i * T;
-    __shared__ double L[T * LP], Xs[T * LP];
+    __shared__ double L[T * LP];
     const int r = threadIdx.x;
+    double x[T];
+#pragma unroll
     for (int c = 0; c < T; ++c) {
         L[c * LP + r] = Lkk[(long long)c * ld + r];
-        Xs[c * LP + r] = X[(long long)c * ld + r];
+        x[c] = X[(long long)c * ld + r];
     }
     __syncthreads();
+#pragma unroll
     for (int j = 0; j < T; ++j) {
-        double s = Xs[j * LP + r];
-        for (int m = 0; m < j; ++m) s -= Xs[m * LP + r] * L[m * LP + j];
-        Xs[j * LP + r] = s / L[j * LP + j];
+        double s = x[j];
+#pragma unroll
+        for (int m = 0; m < j; ++m) s -= x[m] * L[m * LP + j];
+        x[j] = s / L[j * LP + j];
     }
-    for (int c = 0; c < T; ++c) X[(long long)c * ld + r] = Xs[c * LP + r];
+#pragma unroll
+    for (int c = 0; c < T; ++c) X[(long long)c * ld + r] = x[c];
 }
 
 // trailing update C_ij -= L_ik L_jk^T (tile row i may be the rhs row) as

@@ -74,6 +74,9 @@ struct PcgArgs {
     const double *Dk;  // per camera: D_k = sum_q Y_q W_q^T (81)
     double *tc;        // per entry, camera-major: t of the entry's point (3)
     double *cpart;     // per chunk: sum_q Y_q t_pt(q) (9)
+    // corrected preconditioner (S formed only): h = C r - w C N C r with the
+    // weight w in device memory (null: block-Jacobi)
+    float *omega = nullptr;
 };
 
 struct StepArgs {
@@ -135,6 +138,11 @@ void launch_pcg_init(const PcgArgs &a, hipStream_t st);
 void launch_spmv(const PcgArgs &a, hipStream_t st);
 
 void launch_pcg_update(const PcgArgs &a, hipStream_t st);
+// corrected preconditioner: after launch_pcg_init, h = C b - w C N C b, d = h,
+// rho = r.h (block-Jacobi again when rho is not positive); per iteration
+// launch_pcg_update_w replaces launch_pcg_update (a second product N (C r))
+void launch_pcg_init_w(const PcgArgs &a, hipStream_t st);
+void launch_pcg_update_w(const PcgArgs &a, hipStream_t st);
 void launch_backsub(const StepArgs &a, hipStream_t st);
 void launch_decode(const StepArgs &a, hipStream_t st);
 int cost_blocks(int ntr);
@@ -154,11 +162,14 @@ struct BatchArgs {
     const int *tpt;                 // entry -> point
 };
 // per block, in entry order: out[3b] = sum terms, out[3b+1] = sum (wa + wb),
-// out[3b+2] = count of flags (each input may be null)
+// out[3b+2] = count of flags (each input may be null); part holds
+// 3 * block_slices() doubles per block
 void launch_block_sums(const BatchArgs &b, const double *terms, const float *wa, const float *wb,
-                       const unsigned char *flag, double *out, hipStream_t st);
-// per block: 1 if a camera step (as float) or a trial point is not finite
-void launch_block_nan(const BatchArgs &b, const double *sol, const double *Xn, int *nanb, hipStream_t st);
+                       const unsigned char *flag, double *out, double *part, hipStream_t st);
+// per block: 1 if a camera step (as float) or a trial point is not finite;
+// part holds block_slices() ints per block
+void launch_block_nan(const BatchArgs &b, const double *sol, const double *Xn, int *nanb, int *part, hipStream_t st);
+int block_slices();
 // StoreResultsIteration for the blocks with accept[b] != 0: cameras, points, flags
 void launch_batch_store(const BatchArgs &b, const unsigned char *accept, const CamPtrs &cp, const double *Xn,
                         double *X, const unsigned char *flag, unsigned char *flagst, hipStream_t st);

@@ -1030,6 +1030,118 @@ __global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
     pcg_a_body(a, S, blockIdx.x);
 }
 
+// u = N t for the corrected preconditioner (N: the off-diagonal blocks of S,
+// both triangles), one workgroup per camera k: lanes in 28 groups of 9 take
+// the row blocks (k, l) (S_kl t_l) and then the column blocks (l, k)
+// (S_lk^T t_l) of camera k; the groups' sums are added in group order into u_k
+__global__ __launch_bounds__(256) void k_offdiag_mv(PcgArgs a, const double *t_in, double *u)
+{
+    const PcgScalars S = a.S[a.j & 1];
+    if (S.done) return;
+    __shared__ double red[28][9];
+    const int k = blockIdx.x, t = threadIdx.x;
+    const int g = t / 9, r = t % 9;
+    if (g < 28) {
+        const int r0 = a.row_off[k], nr = a.row_off[k + 1] - r0;
+        const int c0 = a.col_off[k], nl = a.col_off[k + 1] - c0;
+        double acc = 0.;
+        for (int e = g; e < nr + nl; e += 28) {
+            if (e < nr) {
+                const int b = r0 + e;
+                const double *B = a.Su + 81 * (size_t)b + 9 * r;
+                const double *tl = t_in + 9 * (size_t)a.blk_l[b];
+                double s = 0.;
+#pragma unroll
+                for (int c = 0; c < 9; ++c) s += B[c] * tl[c];
+                acc += s;
+            } else {
+                const int b = a.col_blk[c0 + e - nr];
+                const double *B = a.Su + 81 * (size_t)b + r;
+                const double *tl = t_in + 9 * (size_t)a.blk_k[b];
+                double s = 0.;
+#pragma unroll
+                for (int c = 0; c < 9; ++c) s += B[9 * c] * tl[c];
+                acc += s;
+            }
+        }
+        red[g][r] = acc;
+    }
+    __syncthreads();
+    if (t < 9) {
+        double s = 0.;
+        for (int gg = 0; gg < 28; ++gg) s += red[gg][t];
+        u[9 * (size_t)k + t] = s;
+    }
+}
+
+// corrected preconditioner, h1 = t - w C_k u_k with t = C r already in tv (h
+// at the set-up, h1 in an iteration) and u = N t from k_offdiag_mv (in Ad,
+// free once k_pcg_a has used S d).  Set-up (init = 1): h = d = h1 and the
+// share r.h; an iteration: h1 and the shares r.(h1 - h), r.h1 (the
+// |alpha S d|^2 share of k_pcg_a stays).
+__global__ __launch_bounds__(64) void k_pcg_c(PcgArgs a, int init)
+{
+    const PcgScalars S = a.S[a.j & 1];
+    if (!init && S.done) return;
+    __shared__ double us[9], p2[2][9];
+    const int k = blockIdx.x, t = threadIdx.x;
+    const float om = *a.omega;
+    if (t < 9) us[t] = a.Ad[9 * (size_t)k + t];
+    __syncthreads();
+    if (t < 9) {
+        const size_t j = 9 * (size_t)k + t;
+        const double *C = a.Cinv + 81 * (size_t)k + 9 * t;
+        double cu = 0.;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) cu += C[c] * us[c];
+        double *tv = init ? a.h : a.h1;
+        const double h1 = tv[j] - (double)om * cu;
+        const double rr = a.r[j];
+        if (init) {
+            a.h[j] = h1;
+            a.d[j] = h1;
+            p2[1][t] = rr * h1;
+        } else {
+            p2[0][t] = rr * (h1 - a.h[j]);
+            p2[1][t] = rr * h1;
+            a.h1[j] = h1;
+        }
+    }
+    __syncthreads();
+    if (t < 2 && !(init && t == 0)) {
+        double sum = 0.;
+        for (int q = 0; q < 9; ++q) sum += p2[t][q];
+        a.part[3 * (size_t)k + 1 + t] = sum;
+    }
+}
+
+// set-up of the corrected preconditioner, single workgroup: rho = r.h from
+// the camera shares; when it is not positive (the correction is not positive
+// definite along b) the weight is set to 0 and h = d = C b again
+__global__ __launch_bounds__(1024) void k_pcg_rho(PcgArgs a)
+{
+    __shared__ double sh[16];
+    double rh = 0.;
+    for (int k = threadIdx.x; k < a.nc; k += blockDim.x) rh += a.part[3 * (size_t)k + 2];
+    rh = block_sum(rh, sh);
+    if (!(rh > 0.) || !isfinite(rh)) {
+        __syncthreads();
+        if (threadIdx.x == 0) *a.omega = 0.f;
+        for (int k = threadIdx.x; k < a.nc; k += blockDim.x) apply_cinv(a.Cinv, a.r, a.h, k);
+        __syncthreads();
+        double s = 0.;
+        for (int j = threadIdx.x; j < 9 * a.nc; j += blockDim.x) {
+            a.d[j] = a.h[j];
+            s += a.r[j] * a.h[j];
+        }
+        rh = block_sum(s, sh);
+    }
+    if (threadIdx.x == 0) {
+        a.S[0].rho = rh;
+        a.S[1].rho = rh;
+    }
+}
+
 // PCG step, part 2: best iterate, stopping rule, beta, d = h1 + beta d, h = h1
 // for the cameras of group grp (28 per group); group 0 writes the scalars of
 // the next launch
@@ -1560,6 +1672,21 @@ void launch_spmv(const PcgArgs &a, hipStream_t st)
     k_mf_dsum<<<1, 1024, 0, st>>>(a);
 }
 int pcg_groups(int nc) { return (nc + 27) / 28; }
+void launch_pcg_init_w(const PcgArgs &a, hipStream_t st)
+{
+    PcgArgs b = a;
+    b.j = 0;
+    k_offdiag_mv<<<a.nc, 256, 0, st>>>(b, a.h, a.Ad);  // N (C b)
+    k_pcg_c<<<a.nc, 64, 0, st>>>(b, 1);
+    k_pcg_rho<<<1, 1024, 0, st>>>(b);
+}
+void launch_pcg_update_w(const PcgArgs &a, hipStream_t st)
+{
+    k_pcg_a<<<a.nc, 256, 0, st>>>(a);
+    k_offdiag_mv<<<a.nc, 256, 0, st>>>(a, a.h1, a.Ad);  // N (C r)
+    k_pcg_c<<<a.nc, 64, 0, st>>>(a, 0);
+    k_pcg_b<<<pcg_groups(a.nc), 256, 0, st>>>(a);
+}
 void launch_pcg_update(const PcgArgs &a, hipStream_t st)
 {
     k_pcg_a<<<a.nc, 256, 0, st>>>(a);
@@ -1606,14 +1733,19 @@ void launch_exteval(const ExtArgs &a, double *sg_out, int *t_out, hipStream_t st
 // batched independent blocks
 // ---------------------------------------------------------------------------
 
+// per (block, slice): sums over one of kSlices equal parts of the block's
+// entry range, then per block the slices in order
+constexpr int kSlices = 32;
+
 __global__ __launch_bounds__(256) void k_block_sums(BatchArgs b, const double *terms, const float *wa, const float *wb,
-                                                    const unsigned char *flag, double *out)
+                                                    const unsigned char *flag, double *part)
 {
     __shared__ double sh[4];
-    const int blk = blockIdx.x;
-    const int e0 = b.eoff[blk], e1 = b.eoff[blk + 1];
+    const int blk = blockIdx.y, sl = blockIdx.x;
+    const long long e0 = b.eoff[blk], n = b.eoff[blk + 1] - e0;
+    const int q0 = (int)(e0 + n * sl / kSlices), q1 = (int)(e0 + n * (sl + 1) / kSlices);
     double s0 = 0., s1 = 0., s2 = 0.;
-    for (int q = e0 + (int)threadIdx.x; q < e1; q += 256) {
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += 256) {
         if (terms) s0 += terms[q];
         if (wa) s1 += (double)wa[q] + (double)wb[q];
         if (flag) s2 += (double)flag[q];
@@ -1622,28 +1754,56 @@ __global__ __launch_bounds__(256) void k_block_sums(BatchArgs b, const double *t
     s1 = block_sum(s1, sh);
     s2 = block_sum(s2, sh);
     if (threadIdx.x == 0) {
-        out[3 * blk] = s0;
-        out[3 * blk + 1] = s1;
-        out[3 * blk + 2] = s2;
+        double *o = part + 3 * ((size_t)blk * kSlices + sl);
+        o[0] = s0;
+        o[1] = s1;
+        o[2] = s2;
     }
 }
 
-__global__ __launch_bounds__(256) void k_block_nan(BatchArgs b, const double *sol, const double *Xn, int *nanb)
+__global__ __launch_bounds__(64) void k_block_sums_fin(int nb, const double *part, double *out)
+{
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= nb) return;
+    double s0 = 0., s1 = 0., s2 = 0.;
+    for (int sl = 0; sl < kSlices; ++sl) {
+        const double *o = part + 3 * ((size_t)blk * kSlices + sl);
+        s0 += o[0];
+        s1 += o[1];
+        s2 += o[2];
+    }
+    out[3 * blk] = s0;
+    out[3 * blk + 1] = s1;
+    out[3 * blk + 2] = s2;
+}
+
+__global__ __launch_bounds__(256) void k_block_nan(BatchArgs b, const double *sol, const double *Xn, int *part)
 {
     __shared__ int any;
-    const int blk = blockIdx.x;
+    const int blk = blockIdx.y, sl = blockIdx.x;
     if (threadIdx.x == 0) any = 0;
     __syncthreads();
     int bad = 0;
-    for (int j = 9 * b.coff[blk] + (int)threadIdx.x; j < 9 * b.coff[blk + 1]; j += 256) {
+    const long long c0 = 9LL * b.coff[blk], nc = 9LL * b.coff[blk + 1] - c0;
+    for (long long j = c0 + nc * sl / kSlices + threadIdx.x; j < c0 + nc * (sl + 1) / kSlices; j += 256) {
         const float xf = (float)sol[j];
         if (isnan(xf) || isinf(xf)) bad = 1;
     }
-    for (int j = 3 * b.poff[blk] + (int)threadIdx.x; j < 3 * b.poff[blk + 1]; j += 256)
+    const long long p0 = 3LL * b.poff[blk], np = 3LL * b.poff[blk + 1] - p0;
+    for (long long j = p0 + np * sl / kSlices + threadIdx.x; j < p0 + np * (sl + 1) / kSlices; j += 256)
         if (!isfinite(Xn[j])) bad = 1;
     if (bad) any = 1;  // every writer stores the same value
     __syncthreads();
-    if (threadIdx.x == 0) nanb[blk] = any;
+    if (threadIdx.x == 0) part[(size_t)blk * kSlices + sl] = any;
+}
+
+__global__ __launch_bounds__(64) void k_block_nan_fin(int nb, const int *part, int *nanb)
+{
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= nb) return;
+    int any = 0;
+    for (int sl = 0; sl < kSlices; ++sl) any |= part[(size_t)blk * kSlices + sl];
+    nanb[blk] = any;
 }
 
 __global__ __launch_bounds__(256) void k_batch_store(BatchArgs b, const unsigned char *accept, CamPtrs cp,
@@ -1680,14 +1840,17 @@ __global__ __launch_bounds__(256) void k_copy_d(double *dst, const double *src, 
 }
 
 void launch_block_sums(const BatchArgs &b, const double *terms, const float *wa, const float *wb,
-                       const unsigned char *flag, double *out, hipStream_t st)
+                       const unsigned char *flag, double *out, double *part, hipStream_t st)
 {
-    k_block_sums<<<b.nb, 256, 0, st>>>(b, terms, wa, wb, flag, out);
+    k_block_sums<<<dim3(kSlices, b.nb), 256, 0, st>>>(b, terms, wa, wb, flag, part);
+    k_block_sums_fin<<<nblk(b.nb, 64), 64, 0, st>>>(b.nb, part, out);
 }
-void launch_block_nan(const BatchArgs &b, const double *sol, const double *Xn, int *nanb, hipStream_t st)
+void launch_block_nan(const BatchArgs &b, const double *sol, const double *Xn, int *nanb, int *part, hipStream_t st)
 {
-    k_block_nan<<<b.nb, 256, 0, st>>>(b, sol, Xn, nanb);
+    k_block_nan<<<dim3(kSlices, b.nb), 256, 0, st>>>(b, sol, Xn, part);
+    k_block_nan_fin<<<nblk(b.nb, 64), 64, 0, st>>>(b.nb, part, nanb);
 }
+int block_slices() { return kSlices; }
 void launch_batch_store(const BatchArgs &b, const unsigned char *accept, const CamPtrs &cp, const double *Xn,
                         double *X, const unsigned char *flag, unsigned char *flagst, hipStream_t st)
 {

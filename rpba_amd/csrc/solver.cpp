@@ -21,6 +21,7 @@
 #include "dense.h"
 #include "host_util.h"
 #include "kernels.h"
+#include "lp.h"
 #include "structure.h"
 #include "structure_dev.h"
 
@@ -80,6 +81,12 @@ struct rpba_ctx {
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
     DBuf<PcgScalars> d_pcg;
+    DBuf<float> d_omega;             // weight of the corrected preconditioner (preconditioner 1, 2)
+    float omega_last = 0.f;          // the weight the last PCG ran with (host copy)
+    rpba_lp *lp = nullptr;           // graph network of preconditioner 2 (caller-owned)
+    // the camera graph of S (features, symmetric CSR) for the network
+    DBuf<float> lp_nrm, lp_eb, lp_feat, lp_adj_w;
+    DBuf<int> lp_adj_off, lp_adj_nbr, lp_goff;
 
     // LM state
     int stage = ST_SETUP, it = 0, iflag = 0, iter = 0, itt = 0;
@@ -353,6 +360,7 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.pgrp = c->ds.pgrp; a.npgrp = c->ds.npgrp;
     a.ch_start = c->ds.ch_start; a.ch_off = c->ds.ch_off;
     a.Wrec = c->d_Wrec.p; a.Yrec = c->d_Yrec.p; a.Dk = c->d_Dk.p; a.tc = c->d_t.p; a.cpart = c->d_cpart.p;
+    a.omega = c->fl.preconditioner >= 1 ? c->d_omega.p : nullptr;
     return a;
 }
 
@@ -396,6 +404,8 @@ static int trial_cost(rpba_ctx *c, double &cost)
 }
 
 static int finish_bundle_step(rpba_ctx *c, const PcgScalars &hs);
+static int pcg_solve(rpba_ctx *c, PcgArgs &pa, bool wait_pairs, PcgScalars &hs);
+static int lp_predict(rpba_ctx *c);
 
 // CalcBundleSep (rbacore.cpp:1094-1195) + theta_new = theta - x + decode.
 // The NaN / inf check of the step (rbacore.cpp:541-546) is raised on the
@@ -436,22 +446,53 @@ static int bundle_step(rpba_ctx *c)
     }
     c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
     PcgArgs pa = pcg_args(c);
-    c->timed("pcg_init", [&] { launch_pcg_init(pa, c->st); });
-    if (side) HIPCHK(hipStreamWaitEvent(c->st, c->ev_pairs, 0));
+    if (pa.omega) {
+        // the corrected preconditioner multiplies by the off-diagonal blocks,
+        // and its weight is the fixed one or the graph network's prediction
+        // from this S
+        if (side) HIPCHK(hipStreamWaitEvent(c->st, c->ev_pairs, 0));
+        if (c->fl.preconditioner == 2) {
+            const int rcl = lp_predict(c);
+            if (rcl) return rcl;
+        } else {
+            HIPCHK(hipMemcpyAsync(c->d_omega.p, &c->fl.pc_omega, sizeof(float), hipMemcpyHostToDevice, c->st));
+        }
+    }
+    PcgScalars hs{};
+    const int rc = pcg_solve(c, pa, side && !pa.omega, hs);
+    if (rc) return rc;
+    return finish_bundle_step(c, hs);
+}
+
+// SparsePCG (rbacore.cpp:1807-1914) on the device: set-up, then three launches
+// per iteration (five with the corrected preconditioner).  wait_pairs: the
+// off-diagonal blocks are still being formed on the second stream.
+static int pcg_solve(rpba_ctx *c, PcgArgs &pa, bool wait_pairs, PcgScalars &hs)
+{
+    c->timed("pcg_init", [&] {
+        launch_pcg_init(pa, c->st);
+        if (pa.omega) launch_pcg_init_w(pa, c->st);
+    });
+    if (wait_pairs) HIPCHK(hipStreamWaitEvent(c->st, c->ev_pairs, 0));
     // launch j reads the scalars of slot j & 1 and writes slot (j + 1) & 1;
     // once the stopping rule fired, later launches only pass the state on.
     // The first chunk is sized from the previous solve's iteration count.
     // Chunks are pipelined: chunk n + 1 is queued before the host reads the
     // scalars after chunk n, so the GPU never waits for the host; at most one
     // short chunk runs after the stop.
-    PcgScalars hs{};
+    hs = PcgScalars{};
     int launched = 0, n = 0;
     const size_t first_timed = c->pending.size();
     auto queue_chunk = [&](int len) -> int {
         for (int j = 0; j < len; ++j) {
             pa.j = launched + j;
             c->timed("spmv", [&] { launch_spmv(pa, c->st); }, pa.j);
-            c->timed("pcg_update", [&] { launch_pcg_update(pa, c->st); }, pa.j);
+            c->timed("pcg_update", [&] {
+                if (pa.omega)
+                    launch_pcg_update_w(pa, c->st);
+                else
+                    launch_pcg_update(pa, c->st);
+            }, pa.j);
         }
         launched += len;
         PcgScalars *slot = &c->hb->pcg[n & 1];
@@ -473,7 +514,57 @@ static int bundle_step(rpba_ctx *c)
     // kernel timing counts the launches that ran an iteration (j < iters)
     for (size_t q = first_timed; q < c->pending.size(); ++q)
         if (c->pending[q].pcg_j >= hs.iters) c->pending[q].skip = true;
-    return finish_bundle_step(c, hs);
+    if (pa.omega) HIPCHK(hipMemcpy(&c->omega_last, pa.omega, sizeof(float), hipMemcpyDeviceToHost));
+    return RPBA_OK;
+}
+
+// the camera graph of the current S into the context's buffers
+static int lp_graph(rpba_ctx *c, LpGraphs &G)
+{
+    const int nc = c->nc, nb = c->ds.nblk;
+    int rc = 0;
+    rc |= c->lp_nrm.alloc((size_t)nc);
+    rc |= c->lp_eb.alloc((size_t)std::max(nb, 1));
+    rc |= c->lp_feat.alloc((size_t)nc * kLpF);
+    rc |= c->lp_adj_off.alloc((size_t)nc + 1);
+    rc |= c->lp_adj_nbr.alloc(2 * (size_t)std::max(nb, 1));
+    rc |= c->lp_adj_w.alloc(2 * (size_t)std::max(nb, 1));
+    if (!c->lp_goff.p) {
+        const int go[2] = {0, nc};
+        rc |= c->lp_goff.put(go, 2, c->st);
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    if (rc) return RPBA_E_NOMEM;
+    lp_graph_from_S(nc, nb, c->d_Sd.p, c->d_Su.p, c->ds.bk, c->ds.bl, c->ds.roff, c->ds.coff, c->ds.cblk, c->lp_nrm.p,
+                    c->lp_eb.p, c->lp_feat.p, c->lp_adj_off.p, c->lp_adj_nbr.p, c->lp_adj_w.p, c->st);
+    G.ng = 1;
+    G.nn = nc;
+    G.ne = 2LL * nb;
+    G.goff = c->lp_goff.p;
+    G.feat = c->lp_feat.p;
+    G.adj_off = c->lp_adj_off.p;
+    G.adj_nbr = c->lp_adj_nbr.p;
+    G.adj_w = c->lp_adj_w.p;
+    G.label = nullptr;
+    return RPBA_OK;
+}
+
+// preconditioner 2: the network's weight for this S into d_omega (0, i.e.
+// block-Jacobi, when no network is attached)
+static int lp_predict(rpba_ctx *c)
+{
+    if (!c->lp) {
+        const float z = 0.f;
+        HIPCHK(hipMemcpyAsync(c->d_omega.p, &z, sizeof(float), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        return RPBA_OK;
+    }
+    LpGraphs G;
+    int rc = 0;
+    c->timed("lp_graph", [&] { rc = lp_graph(c, G); });
+    if (rc) return rc;
+    c->timed("lp_forward", [&] { rc = lp_model_forward(c->lp, G, c->d_omega.p, c->st); });
+    return rc;
 }
 
 // after the PCG: back-substitution and decode of the trial state
@@ -771,6 +862,7 @@ static int ctx_init(rpba_ctx *c)
     rc |= c->d_scal.alloc(4);
     rc |= c->d_iscal.alloc(4);
     rc |= c->d_pcg.alloc(2);
+    rc |= c->d_omega.alloc(1);
     rc |= c->d_In_bak.alloc(3 * (size_t)nc);
     rc |= c->d_ppart.alloc(3 * (size_t)nc);
     rc |= c->c_Rb.alloc(n9);
@@ -814,6 +906,10 @@ extern "C" int rpba_ctx_create(rpba_ctx **out, rpba_block *io, const rpba_flags 
     if (flags->precision < 0 || flags->precision > 1 || flags->schur < 0 || flags->schur > 1) return RPBA_E_INVALID;
     // the dense form factors the formed S: it needs the off-diagonal blocks
     if (flags->solver < 0 || flags->solver > 1 || (flags->solver == 1 && flags->schur == 1)) return RPBA_E_INVALID;
+    // the corrected preconditioner multiplies by the formed off-diagonal blocks
+    if (flags->preconditioner < 0 || flags->preconditioner > 2) return RPBA_E_INVALID;
+    if (flags->preconditioner >= 1 && (flags->schur != 0 || flags->solver != 0)) return RPBA_E_INVALID;
+    if (flags->preconditioner == 1 && !(flags->pc_omega >= 0.f && flags->pc_omega < 1.f)) return RPBA_E_INVALID;
     if (!io->obs_cam || !io->obs_pt || !io->obs_xy || !io->cam_P || !io->cam_f || !io->cam_d || !io->pts)
         return RPBA_E_INVALID;
     int ndev = 0;
@@ -977,6 +1073,55 @@ extern "C" int rpba_ctx_tuning(rpba_ctx *c, char *buf, int32_t cap)
     return RPBA_OK;
 }
 
+extern "C" int rpba_ctx_set_lp(rpba_ctx *c, rpba_lp *model)
+{
+    if (!c) return RPBA_E_INVALID;
+    c->lp = model;
+    return RPBA_OK;
+}
+
+extern "C" int rpba_ctx_last_omega(rpba_ctx *c, float *omega)
+{
+    if (!c || !omega) return RPBA_E_INVALID;
+    *omega = c->omega_last;
+    return RPBA_OK;
+}
+
+extern "C" int rpba_ctx_lp_sample(rpba_ctx *c, int32_t n_omega, const float *omegas, int32_t *iters, float *feat,
+                                  int32_t *adj_off, int32_t *adj_nbr, float *adj_w, int64_t *n_edges)
+{
+    if (!c || n_omega < 0 || (n_omega && (!omegas || !iters)) || !n_edges) return RPBA_E_INVALID;
+    if (c->implicit || c->fl.solver != 0 || c->ds.nblk < 0 || !c->d_Su.p || c->tr_sig0.empty()) return RPBA_E_INVALID;
+    LpGraphs G;
+    int rc = lp_graph(c, G);
+    if (rc) return rc;
+    const int64_t ne = G.ne, cap = *n_edges;
+    *n_edges = ne;
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (feat) HIPCHK(hipMemcpy(feat, c->lp_feat.p, sizeof(float) * (size_t)c->nc * kLpF, hipMemcpyDeviceToHost));
+    if (adj_off) HIPCHK(hipMemcpy(adj_off, c->lp_adj_off.p, sizeof(int) * ((size_t)c->nc + 1), hipMemcpyDeviceToHost));
+    if (ne <= cap && ne > 0) {
+        if (adj_nbr) HIPCHK(hipMemcpy(adj_nbr, c->lp_adj_nbr.p, sizeof(int) * (size_t)ne, hipMemcpyDeviceToHost));
+        if (adj_w) HIPCHK(hipMemcpy(adj_w, c->lp_adj_w.p, sizeof(float) * (size_t)ne, hipMemcpyDeviceToHost));
+    }
+    // PCG iterations of this S and b (the last LM iteration's) per weight
+    const int keep_pc = c->fl.preconditioner, keep_last = c->pcg_last;
+    const float keep_w = c->omega_last;
+    for (int i = 0; i < n_omega && rc == 0; ++i) {
+        c->fl.preconditioner = omegas[i] > 0.f ? 1 : 0;
+        PcgArgs pa = pcg_args(c);
+        if (pa.omega && hipMemcpyAsync(c->d_omega.p, &omegas[i], sizeof(float), hipMemcpyHostToDevice, c->st) != hipSuccess)
+            rc = RPBA_E_HIP;
+        PcgScalars hs{};
+        if (!rc) rc = pcg_solve(c, pa, false, hs);
+        iters[i] = hs.iters;
+    }
+    c->fl.preconditioner = keep_pc;
+    c->pcg_last = keep_last;
+    c->omega_last = keep_w;
+    return rc;
+}
+
 extern "C" int rpba_ctx_kernel_time(rpba_ctx *c, const char *kernel, double *total_ms, int64_t *launches)
 {
     if (!c || !kernel) return RPBA_E_INVALID;
@@ -1041,6 +1186,9 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->c_Rb.release(); c->c_Cb.release(); c->c_Rn.release(); c->c_Cn.release(); c->c_In.release();
     c->c_th.release(); c->c_thn.release(); c->c_Pst.release(); c->c_Ist.release(); c->c_mask.release();
     c->d_pcg.release();
+    c->d_omega.release();
+    c->lp_nrm.release(); c->lp_eb.release(); c->lp_feat.release(); c->lp_adj_w.release();
+    c->lp_adj_off.release(); c->lp_adj_nbr.release(); c->lp_goff.release();
     c->d_Dd.release();
     c->d_yN.release();
     c->d_yT.release();

@@ -197,15 +197,22 @@ def test_schur_matches_dense_elimination(tiny_scene, mult):
 # PCG: against dense Cholesky and an independent replay of the rule
 # --------------------------------------------------------------------------
 
-def _replay_pcg(bs, S, b):
+def _replay_pcg(bs, S, b, omega=0.0):
     """Block-Jacobi PCG, Polak-Ribiere beta clipped at 0, stop when the step
     norm |alpha A d|^2 falls 1e5 below the first one, return the iterate with
-    the smallest step norm; at most max(1000, n) iterations."""
+    the smallest step norm; at most max(1000, n) iterations.  omega > 0: the
+    preconditioner C - omega C N C with N the off-diagonal blocks of S."""
     off = np.concatenate([[0], np.cumsum(bs)])
     C = np.zeros_like(S)
     for i in range(len(bs)):
         sl = slice(off[i], off[i + 1])
         C[sl, sl] = np.linalg.inv(S[sl, sl])
+    if omega:
+        N = S.copy()
+        for i in range(len(bs)):
+            sl = slice(off[i], off[i + 1])
+            N[sl, sl] = 0.0
+        C = C - omega * C @ N @ C
     x = np.zeros_like(b)
     r = b.copy()
     h = C @ r
@@ -253,6 +260,25 @@ def test_pcg_against_cholesky_and_replay(seed):
     # the step-norm rule stops loosely: residual reduced well below b
     assert np.linalg.norm(S @ x - b) < 0.1 * np.linalg.norm(b)
     assert np.linalg.norm(x - xs) < 0.2 * np.linalg.norm(xs)
+
+
+@pytest.mark.parametrize("seed,omega", [(0, 0.5), (1, 0.75), (2, 0.3)])
+def test_neumann_pcg_against_replay(seed, omega):
+    """The corrected preconditioner (preconditioner = 1): same iteration count
+    and iterate as the numpy replay with M^-1 = C - omega C N C, and fewer
+    iterations than block-Jacobi on a block-coupled SPD system."""
+    rng = np.random.default_rng(200 + seed)
+    bs = np.array([9] * int(rng.integers(6, 14)))
+    n = bs.sum()
+    A = rng.normal(size=(n, 3 * n))
+    S = A @ A.T / n + 0.05 * np.eye(n)
+    b = rng.normal(size=n)
+    x, its = oracle.pcg_dense(bs, S, b, omega)
+    xr, itr = _replay_pcg(bs, S, b, omega)
+    assert its == itr
+    assert np.allclose(x, xr, rtol=1e-6, atol=1e-8 * np.abs(xr).max())
+    x0, it0 = oracle.pcg_dense(bs, S, b)
+    assert np.linalg.norm(S @ x - b) < 0.1 * np.linalg.norm(b)
 
 
 @pytest.mark.parametrize("seed", range(4))
