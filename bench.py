@@ -114,30 +114,33 @@ def run_bench(args):
     rc_d, out, st = ctx.download()
     sizes = ctx.sizes()
 
-    # the chosen form: FP32 Jacobians and W, Y = W V^-1 records with the
-    # matrix-free Schur product (FP64 sums, diagonal S blocks, PCG and cost);
-    # time until its pass-1 stored sigma0 reaches the plain form's final value
-    # (x (1 + 1e-6)), and its errors against plain FP64
-    chosen = None
-    if not args.no_chosen:
-        target = st["sig0_final"] * (1.0 + 1e-6)
-        # matrix-free when one PCG iteration streams fewer bytes through the
-        # FP32 W and Y records (2 x 128 B per observation) than through the
-        # stored S blocks (648 B each)
-        mf = 2 * 128 * sizes["obs"] < 648 * sizes["blocks"]
-        cctx = api.Context(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, precision=1, schur=int(mf)))
+    # the other forms: their time until the pass-1 stored sigma0 reaches the
+    # plain form's final value (x (1 + 1e-6)) and their errors against plain
+    # FP64.  The chosen form: FP32 Jacobians and W, Y = W V^-1 records, the
+    # learned correction of the block-Jacobi preconditioner (the graph
+    # network's Neumann weight per LM iteration), and the matrix-free Schur
+    # product where its records stream fewer bytes than S (2 x 128 B per
+    # observation against 648 B per block; the learned correction needs S
+    # formed, so it is dropped there).
+    target = st["sig0_final"] * (1.0 + 1e-6)
+    f64 = lambda a: np.asarray(a, dtype=np.float64)
+    lp_model = None
+    from rpba_amd import lp_train
+    if os.path.exists(lp_train.WEIGHTS):
+        lp_model = api.LpModel.load(lp_train.WEIGHTS)
+
+    def eval_form(fl, desc):
+        cctx = api.Context(sc, fl)
+        if fl.preconditioner == 2:
+            cctx.set_lp(lp_model)
         cctx.run(1)
         cctx.reset()
         cttc, crecs = timed_solve(cctx)
         _, cout, cst = cctx.download()
         cctx.close()
         reach = next((t for t, ps, sg in crecs if ps == 1 and sg <= target), None)
-        f64 = lambda a: np.asarray(a, dtype=np.float64)
-        chosen = {
-            "form": ("precision=1, schur=1: FP32 Jacobians and W, Y = W V^-1 records; matrix-free Schur product "
-                     "(off-diagonal S never formed); FP64 residuals, sums, diagonal S blocks, PCG, cost") if mf else
-                    ("precision=1, schur=0: FP32 Jacobians and W, Y = W V^-1 records; S formed block by block "
-                     "(the records stream more bytes per PCG iteration than S here); FP64 residuals, sums, S, PCG, cost"),
+        return {
+            "form": desc,
             "time_to_converged_s": round(cttc, 4),
             "time_to_reach_plain_sigma0_s": None if reach is None else round(reach, 4),
             "speedup_vs_plain": None if reach is None else round(ttc / reach, 3),
@@ -153,6 +156,20 @@ def run_bench(args):
                 "label_diffs": int((cout["obs_kept"] != out["obs_kept"]).sum()),
             },
         }
+
+    chosen = learned = None
+    if not args.no_chosen:
+        mf = 2 * 128 * sizes["obs"] < 648 * sizes["blocks"]
+        pc = 0 if (mf or lp_model is None) else 2
+        desc = ("precision=1: FP32 Jacobians and W, Y = W V^-1 records; FP64 residuals, sums, S, PCG, cost; " +
+                ("schur=1: matrix-free Schur product (off-diagonal S never formed)" if mf else
+                 "S formed block by block (the records stream more bytes per PCG iteration than S here)") +
+                ("; preconditioner=2: learned Neumann weight per LM iteration" if pc == 2 else ""))
+        chosen = eval_form(capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, precision=1, schur=int(mf),
+                                           preconditioner=pc), desc)
+        if lp_model is not None and not args.schur:
+            learned = eval_form(capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, preconditioner=2),
+                                "plain FP64 with the learned correction of block-Jacobi (preconditioner=2)")
 
     # roofline: the same steps again with HIP events around every launch
     ctx.reset()
@@ -206,6 +223,7 @@ def run_bench(args):
         "sigma0_final": st["sigma_out"][1],
         "rms_px": st["rms_px"],
         "chosen": chosen,
+        "learned_fp64": learned,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktimes.items() if v[1] > 0},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
