@@ -310,8 +310,12 @@ def run_batch_bench(args):
             ctx.close()
         return tot, its, sigs
 
-    seq_pcg, its_pcg, sig_pcg = seq(0)
-    seq_dense, its_dense, sig_dense = seq(1)
+    if args.no_sequential:
+        seq_pcg = seq_dense = float("nan")
+        sig_pcg = sig_dense = sig
+    else:
+        seq_pcg, its_pcg, sig_pcg = seq(0)
+        seq_dense, its_dense, sig_dense = seq(1)
     sig = np.asarray(sig)
     line = {
         "metric": "time to converged, 64 independent blocks (batched form vs one block after another)",
@@ -412,6 +416,8 @@ def main():
                     help="plain FP64 form with S formed (0, default) or the matrix-free Schur product (1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (FP32, matrix-free) form")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="c4_batch: skip the one-block-after-another comparison (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run device-to-device copies of this many bytes (counter calibration)")
     args = ap.parse_args()

@@ -175,16 +175,34 @@ __global__ __launch_bounds__(256) void k_update(DenseArgs a, const DenseItem *it
         f64x4 acc[4];
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) acc[cb] = f64x4{0., 0., 0., 0.};
+        if (V == 0) {
 #pragma unroll 4
-        for (int k0 = 0; k0 < T; k0 += 4) {
-            const int kk = k0 + q;
-            const double bv = (V == 0) ? Ls[kk * LP + rb * 16 + i16] : Li[(long long)kk * ld + rb * 16 + i16];
-            double av[4];
+            for (int k0 = 0; k0 < T; k0 += 4) {
+                const int kk = k0 + q;
+                const double bv = Ls[kk * LP + rb * 16 + i16];
+                double av[4];
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb)
-                av[cb] = (V == 0) ? Ls[T * LP + kk * LP + cb * 16 + i16] : Lj[(long long)kk * ld + cb * 16 + i16];
+                for (int cb = 0; cb < 4; ++cb) av[cb] = Ls[T * LP + kk * LP + cb * 16 + i16];
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[cb], bv, acc[cb], 0, 0, 0);
+                for (int cb = 0; cb < 4; ++cb) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[cb], bv, acc[cb], 0, 0, 0);
+            }
+        } else {
+            // every operand of the item is loaded before the first product, so
+            // the wavefront waits for memory once rather than once per K step
+            // (80 doubles per lane); the products then run back to back
+            double bvv[T / 4], avv[4][T / 4];
+#pragma unroll
+            for (int s = 0; s < T / 4; ++s) {
+                const int kk = 4 * s + q;
+                bvv[s] = Li[(long long)kk * ld + rb * 16 + i16];
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) avv[cb][s] = Lj[(long long)kk * ld + cb * 16 + i16];
+            }
+#pragma unroll
+            for (int s = 0; s < T / 4; ++s)
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb)
+                    acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(avv[cb][s], bvv[s], acc[cb], 0, 0, 0);
         }
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
