@@ -78,3 +78,19 @@ def test_batch_graph_replay_equals_eager_and_reset():
             assert np.array_equal(o1[key], o2[key]) and np.array_equal(o1[key], o3[key]), key
         assert np.array_equal(t1["trace_sig0"], t2["trace_sig0"])
         assert np.array_equal(t1["trace_sig0"], t3["trace_sig0"])
+
+
+def test_batch_block_error_is_isolated():
+    """A block whose redundancy is negative stops with the reference's error
+    (rbacore.cpp:468-476) and keeps its initial state; the other blocks solve
+    as if alone."""
+    good = _blocks(900, [(9, 350), (14, 600)])
+    bad = scenes.generate(None, seed=950, n_cams=5, n_pts=8, track="uniform2_8", outlier_frac=0.0)
+    blocks = [good[0], bad, good[1]]
+    flags = capi.make_flags(robust=1, solver=1)
+    res = api.calc_plsa_batch(blocks, flags)
+    assert res[1][0] == capi.RPBA_E_LOW_REDUNDANCY
+    assert any("Low redundancy" in line for line in res[1][3])
+    for i in (0, 2):
+        _compare_block(blocks[i], flags, res[i], f"block {i}")
+    _compare_block(bad, flags, res[1], "failing block")
