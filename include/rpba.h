@@ -154,6 +154,9 @@ int rpba_ctx_run(rpba_ctx *ctx, int32_t max_iters, int32_t *iters_done, int32_t 
 int rpba_ctx_reset(rpba_ctx *ctx);
 /* OutputLSAc (rbacore.cpp:644-666): write the stored state back to the block. */
 int rpba_ctx_download(rpba_ctx *ctx, rpba_stats *stats);
+/* 1: PCG iterations are replayed two at a time from a hipGraph; 0 (default):
+ * launched one by one, which measured as fast.  Same results either way. */
+int rpba_ctx_set_graph(rpba_ctx *ctx, int32_t on);
 /* Kernel timing of the last rpba_ctx_run with timing on: name -> total ms and
  * launches, measured with HIP events on the context's stream. */
 int rpba_ctx_set_timing(rpba_ctx *ctx, int32_t on);

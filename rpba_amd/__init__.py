@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "librpba_hip.so")
 EXPORTS = [
     "rpba_calc_plsa", "rpba_ctx_create", "rpba_ctx_run", "rpba_ctx_reset", "rpba_ctx_download",
     "rpba_ctx_set_timing", "rpba_ctx_kernel_time", "rpba_ctx_progress", "rpba_ctx_sizes", "rpba_ctx_destroy",
-    "rpba_ctx_tuning", "rpba_dense_solve_batch",
+    "rpba_ctx_tuning", "rpba_dense_solve_batch", "rpba_ctx_set_graph",
     "rpba_batch_create", "rpba_batch_run", "rpba_batch_reset", "rpba_batch_download", "rpba_batch_set_graph",
     "rpba_batch_set_timing", "rpba_batch_kernel_time", "rpba_batch_tuning", "rpba_batch_sizes", "rpba_batch_destroy",
     "rpba_calc_plsa_batch",
@@ -52,6 +52,7 @@ def lib():
     L.rpba_ctx_progress.restype = C.c_int
     L.rpba_ctx_sizes.argtypes = [C.c_void_p, P(C.c_int64)]
     L.rpba_ctx_tuning.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+    L.rpba_ctx_set_graph.argtypes = [C.c_void_p, C.c_int32]
     L.rpba_ctx_tuning.restype = C.c_int
     L.rpba_dense_solve_batch.argtypes = [C.c_int32, P(C.c_int32), P(C.c_double), P(C.c_double), P(C.c_double),
                                          C.c_int32, C.c_int32]

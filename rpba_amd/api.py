@@ -65,6 +65,10 @@ class Context:
     def set_timing(self, on):
         return lib().rpba_ctx_set_timing(self.h, 1 if on else 0)
 
+    def set_graph(self, on):
+        """PCG iterations replayed from a hipGraph, or launched one by one (the default)."""
+        return lib().rpba_ctx_set_graph(self.h, 1 if on else 0)
+
     def kernel_time(self, name):
         ms, n = C.c_double(), C.c_int64()
         lib().rpba_ctx_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n))

@@ -82,6 +82,14 @@ struct rpba_ctx {
     DBuf<unsigned> c_mask;
     DBuf<PcgScalars> d_pcg;
     DBuf<float> d_omega;             // weight of the corrected preconditioner (preconditioner 1, 2)
+    // two PCG iterations (launch indices of parities 0 and 1) captured in a
+    // hipGraph; the kernels see only the parity of the launch index, so one
+    // graph serves every chunk that starts at an even index.  Recaptured when
+    // the argument block changes (a new structure, another form).
+    hipGraph_t pcg_graph = nullptr;
+    hipGraphExec_t pcg_gexec = nullptr;
+    PcgArgs pcg_graph_args{};
+    bool pcg_graph_on = false;       // measured no faster than launches (rpba_ctx_set_graph)
     float omega_last = 0.f;          // the weight the last PCG ran with (host copy)
     rpba_lp *lp = nullptr;           // graph network of preconditioner 2 (caller-owned)
     // the camera graph of S (features, symmetric CSR) for the network
@@ -344,6 +352,7 @@ static LinArgs lin_args(rpba_ctx *c)
 static PcgArgs pcg_args(rpba_ctx *c)
 {
     PcgArgs a;
+    std::memset(&a, 0, sizeof a);  // padding too: the PCG graph compares argument blocks bytewise
     a.nc = c->nc;
     a.msize = std::max(1000, c->nrioff);  // rbacore.cpp:1848-1849
     a.Sd = c->d_Sd.p; a.Su = c->d_Su.p; a.Cinv = c->d_Cinv.p; a.b = c->d_b.p;
@@ -483,7 +492,41 @@ static int pcg_solve(rpba_ctx *c, PcgArgs &pa, bool wait_pairs, PcgScalars &hs)
     hs = PcgScalars{};
     int launched = 0, n = 0;
     const size_t first_timed = c->pending.size();
+    const bool use_graph = c->pcg_graph_on && !c->timing;
+    if (use_graph) {
+        PcgArgs key = pa;
+        key.j = 0;
+        if (!c->pcg_gexec || std::memcmp(&key, &c->pcg_graph_args, sizeof key) != 0) {
+            if (c->pcg_gexec) (void)hipGraphExecDestroy(c->pcg_gexec);
+            if (c->pcg_graph) (void)hipGraphDestroy(c->pcg_graph);
+            c->pcg_gexec = nullptr;
+            c->pcg_graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+            for (int j = 0; j < 2; ++j) {
+                key.j = j;
+                launch_spmv(key, c->st);
+                if (key.omega)
+                    launch_pcg_update_w(key, c->st);
+                else
+                    launch_pcg_update(key, c->st);
+            }
+            HIPCHK(hipStreamEndCapture(c->st, &c->pcg_graph));
+            HIPCHK(hipGraphInstantiate(&c->pcg_gexec, c->pcg_graph, nullptr, nullptr, 0));
+            key.j = 0;
+            c->pcg_graph_args = key;
+        }
+    }
     auto queue_chunk = [&](int len) -> int {
+        if (use_graph) {
+            len += len & 1;  // whole graph replays: chunks of even length from an even index
+            for (int j = 0; j < len; j += 2) HIPCHK(hipGraphLaunch(c->pcg_gexec, c->st));
+            launched += len;
+            PcgScalars *slot = &c->hb->pcg[n & 1];
+            HIPCHK(hipMemcpyAsync(slot, c->d_pcg.p + (launched & 1), sizeof(PcgScalars), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipEventRecord(c->pcg_ev[n & 1], c->st));
+            ++n;
+            return RPBA_OK;
+        }
         for (int j = 0; j < len; ++j) {
             pa.j = launched + j;
             c->timed("spmv", [&] { launch_spmv(pa, c->st); }, pa.j);
@@ -504,7 +547,7 @@ static int pcg_solve(rpba_ctx *c, PcgArgs &pa, bool wait_pairs, PcgScalars &hs)
     int rc = queue_chunk(std::max(2, c->pcg_last));
     if (rc) return rc;
     for (int m = 0;; ++m) {
-        rc = queue_chunk(3);
+        rc = queue_chunk(use_graph ? 2 : 3);
         if (rc) return rc;
         HIPCHK(hipEventSynchronize(c->pcg_ev[m & 1]));
         hs = c->hb->pcg[m & 1];
@@ -1073,6 +1116,13 @@ extern "C" int rpba_ctx_tuning(rpba_ctx *c, char *buf, int32_t cap)
     return RPBA_OK;
 }
 
+extern "C" int rpba_ctx_set_graph(rpba_ctx *c, int32_t on)
+{
+    if (!c) return RPBA_E_INVALID;
+    c->pcg_graph_on = on != 0;
+    return RPBA_OK;
+}
+
 extern "C" int rpba_ctx_set_lp(rpba_ctx *c, rpba_lp *model)
 {
     if (!c) return RPBA_E_INVALID;
@@ -1187,6 +1237,8 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->c_th.release(); c->c_thn.release(); c->c_Pst.release(); c->c_Ist.release(); c->c_mask.release();
     c->d_pcg.release();
     c->d_omega.release();
+    if (c->pcg_gexec) (void)hipGraphExecDestroy(c->pcg_gexec);
+    if (c->pcg_graph) (void)hipGraphDestroy(c->pcg_graph);
     c->lp_nrm.release(); c->lp_eb.release(); c->lp_feat.release(); c->lp_adj_w.release();
     c->lp_adj_off.release(); c->lp_adj_nbr.release(); c->lp_goff.release();
     c->d_Dd.release();

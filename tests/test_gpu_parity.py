@@ -142,3 +142,19 @@ def test_matrix_free_schur_matches_oracle(scene_c1):
 def test_matrix_free_schur_variants_match_oracle():
     sc = scenes.generate(None, seed=31, n_cams=12, n_pts=600, track="uniform2_8", outlier_frac=0.03)
     _cmp_solves(sc, capi.make_flags(robust=0, k2=1, k4=0, f=1, schur=1), "matrix-free r0 k2 f")
+
+
+def test_pcg_graph_replay_same_bits(scene_c1):
+    """The PCG replayed from a hipGraph gives the same bits as launches."""
+    flags = capi.make_flags(robust=1)
+    outs = []
+    for on in (False, True, True):
+        ctx = api.Context(scene_c1, flags)
+        ctx.set_graph(on)
+        rc, n, fin = ctx.run(-1)
+        assert rc == 0 and fin
+        outs.append(ctx.download())
+        ctx.close()
+    for rc, out, st in outs[1:]:
+        assert np.array_equal(out["pts"], outs[0][1]["pts"]) and st["sig0_final"] == outs[0][2]["sig0_final"]
+        assert np.array_equal(st["trace_sig0"], outs[0][2]["trace_sig0"])
