@@ -4,13 +4,19 @@
 // in the input format (the reference's OutputBlock call, rpba.cpp:144-150);
 // `Cloud = 1` writes InputDir/cloud.txt with the kept points and prints the
 // number of dropped points (rpba.cpp:154-173).  Both calls are commented out
-// in the reference.
+// in the reference.  The forms of this library have keys of their own, which
+// the reference ignores: Precision, Schur, Solver, Preconditioner (0, 1, 2),
+// PcOmega (the Neumann weight of Preconditioner = 1) and LpWeights (the graph
+// network of Preconditioner = 2; default: lp_weights.f32 beside the library,
+// one directory above this binary).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
+
+#include <unistd.h>
 
 #include "../../include/rpba.h"
 
@@ -33,6 +39,64 @@ static int read_int_key(const char *path, const char *key)
     }
     std::fclose(fh);
     return out;
+}
+
+static bool read_str_key(const char *path, const char *key, std::string &out)
+{
+    FILE *fh = std::fopen(path, "r");
+    if (!fh) return false;
+    char line[4096];
+    bool found = false;
+    while (std::fgets(line, sizeof line, fh)) {
+        char k[256], v[3072];
+        if (std::sscanf(line, " %255[^= \t] = %3071s", k, v) == 2 && std::strcmp(k, key) == 0) {
+            out = v;
+            found = true;
+        }
+    }
+    std::fclose(fh);
+    return found;
+}
+
+// the graph network's parameters from a little-endian float32 file
+static rpba_lp *load_lp(const std::string &path)
+{
+    FILE *fh = std::fopen(path.c_str(), "rb");
+    if (!fh) return nullptr;
+    const int n = rpba_lp_n_params();
+    float *p = (float *)std::malloc(sizeof(float) * (size_t)n);
+    const size_t got = p ? std::fread(p, sizeof(float), (size_t)n, fh) : 0;
+    std::fclose(fh);
+    rpba_lp *m = nullptr;
+    if (got == (size_t)n && rpba_lp_create(&m, p, 0) != RPBA_OK) m = nullptr;
+    std::free(p);
+    return m;
+}
+
+static std::string exe_dir()
+{
+    char buf[4096];
+    const ssize_t n = readlink("/proc/self/exe", buf, sizeof buf - 1);
+    if (n <= 0) return ".";
+    buf[n] = 0;
+    std::string s(buf);
+    const size_t sl = s.rfind('/');
+    return sl == std::string::npos ? std::string(".") : s.substr(0, sl);
+}
+
+// the global solve through a context with the graph network attached
+// (Preconditioner = 2); the same call sequence as rpba_calc_plsa
+static int solve_with_lp(rpba_block *io, const rpba_flags *fl, rpba_lp *m, std::string *os)
+{
+    rpba_ctx *ctx = nullptr;
+    int rc = rpba_ctx_create(&ctx, io, fl, to_stdout, os);
+    if (rc) return rc;
+    rpba_ctx_set_lp(ctx, m);
+    int32_t done = 0, fin = 0;
+    rc = rpba_ctx_run(ctx, -1, &done, &fin);
+    const int rd = rpba_ctx_download(ctx, nullptr);
+    rpba_ctx_destroy(ctx);
+    return rc ? rc : rd;
 }
 
 int main(int argc, char **argv)
@@ -60,6 +124,19 @@ int main(int argc, char **argv)
     std::printf("Focal length: %d\n\n", fl.est_f);
     std::printf("K24out: %d\n\n", fl.k24out);
     fl.threads = mt;
+    fl.precision = read_int_key("parameters", "Precision");
+    fl.schur = read_int_key("parameters", "Schur");
+    fl.solver = read_int_key("parameters", "Solver");
+    fl.preconditioner = read_int_key("parameters", "Preconditioner");
+    std::string sw;
+    if (read_str_key("parameters", "PcOmega", sw)) fl.pc_omega = std::strtof(sw.c_str(), nullptr);
+    rpba_lp *lp = nullptr;
+    if (fl.preconditioner == 2 && !fl.exteval) {
+        std::string wpath = exe_dir() + "/../lp_weights.f32";
+        read_str_key("parameters", "LpWeights", wpath);
+        lp = load_lp(wpath);
+        if (!lp) std::fprintf(stderr, "rpba: cannot load the graph network from %s; block-Jacobi\n", wpath.c_str());
+    }
 
     rpba_block io;
     if (rpba_read_bal(inputfile.c_str(), fl.k24out, &io) != RPBA_OK) {
@@ -73,7 +150,7 @@ int main(int argc, char **argv)
 
     std::string os;
     const auto t0 = std::chrono::steady_clock::now();
-    const int rc = rpba_calc_plsa(&io, &fl, to_stdout, &os, nullptr);
+    const int rc = lp ? solve_with_lp(&io, &fl, lp, &os) : rpba_calc_plsa(&io, &fl, to_stdout, &os, nullptr);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (io.n_cams < 50 || mt == 1) std::fputs(os.c_str(), stdout);
     std::printf("\n\nBundle adjustment runtime: %g s\n\n", secs);
@@ -99,5 +176,6 @@ int main(int argc, char **argv)
         std::printf("Counter: %d\n", counter);
     }
     rpba_free_block(&io);
+    if (lp) rpba_lp_destroy(lp);
     return rc == RPBA_OK ? 0 : 2;
 }

@@ -21,7 +21,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "lp_weights.f32")
-OMEGAS = [0.0, 0.2, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9]
+OMEGAS = [0.0, 0.2, 0.4, 0.5, 0.6, 0.7, 0.8, 0.85, 0.9, 0.95]
 TRAIN_SEED0 = 1000   # training scenes: seeds 1000 ..
 HOLDOUT_SEED0 = 2000  # held-out scenes: seeds 2000 ..
 
@@ -107,7 +107,7 @@ def evaluate(model, seeds, max_iters=40):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=WEIGHTS)
-    ap.add_argument("--scenes", type=int, default=40)
+    ap.add_argument("--scenes", type=int, default=60)
     ap.add_argument("--holdout", type=int, default=8)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--lr", type=float, default=2e-3)

@@ -83,3 +83,44 @@ def test_driver_exteval(tmp_path):
     itt = [l for l in out.splitlines() if l.startswith("ITTTT: ")]
     assert len(itt) == 1 and float(itt[0].split()[1]) > 0
     assert not (tmp_path / "out-scene.txt").exists()
+
+
+@pytest.mark.parametrize("extra", ["Solver = 1\n", "Preconditioner = 1\nPcOmega = 0.7\n", "Preconditioner = 2\n",
+                                   "Precision = 1\nSchur = 1\n"])
+def test_driver_form_keys(tmp_path, extra):
+    """The forms' keys (ignored by the reference) select the same form as the
+    library flags: the driver's printed Sigma0 lines equal the library's."""
+    sc = scenes.generate(None, seed=46, n_cams=10, n_pts=400, track="uniform2_8", outlier_frac=0.02)
+    scenes.write_bal(str(tmp_path / "scene.txt"), sc)
+    _params(tmp_path, extra=extra + "Output = 1\n")
+    rc, out = _run(tmp_path)
+    assert rc == 0, out
+    sig = [l for l in out.splitlines() if l.startswith("Sigma0:")]
+    kv = dict(l.split("=") for l in extra.strip().splitlines())
+    kv = {k.strip(): v.strip() for k, v in kv.items()}
+    fl = capi.make_flags(robust=1, threads=1, precision=int(kv.get("Precision", 0)), schur=int(kv.get("Schur", 0)),
+                         solver=int(kv.get("Solver", 0)), preconditioner=int(kv.get("Preconditioner", 0)),
+                         pc_omega=float(kv.get("PcOmega", 0)))
+    L = rpba_amd.lib()
+    blk = capi.RpbaBlock()
+    assert L.rpba_read_bal(str(tmp_path / "scene.txt").encode(), 1, C.byref(blk)) == 0
+    try:
+        sc2 = {"n_cams": blk.n_cams, "n_pts": blk.n_pts, "n_obs": blk.n_obs,
+               "obs_cam": np.ctypeslib.as_array(blk.obs_cam, shape=(blk.n_obs,)).copy(),
+               "obs_pt": np.ctypeslib.as_array(blk.obs_pt, shape=(blk.n_obs,)).copy(),
+               "obs_xy": np.ctypeslib.as_array(blk.obs_xy, shape=(2 * blk.n_obs,)).reshape(-1, 2).copy(),
+               "cam_P": np.ctypeslib.as_array(blk.cam_P, shape=(12 * blk.n_cams,)).reshape(-1, 3, 4).copy(),
+               "cam_f": np.ctypeslib.as_array(blk.cam_f, shape=(blk.n_cams,)).copy(),
+               "cam_d": np.ctypeslib.as_array(blk.cam_d, shape=(2 * blk.n_cams,)).reshape(-1, 2).copy(),
+               "pts": np.ctypeslib.as_array(blk.pts, shape=(3 * blk.n_pts,)).reshape(-1, 3).copy()}
+    finally:
+        L.rpba_free_block(C.byref(blk))
+    ctx = api.Context(sc2, fl)
+    if fl.preconditioner == 2:
+        from rpba_amd import lp_train
+        ctx.set_lp(api.LpModel.load(lp_train.WEIGHTS))
+    rc2, n, fin = ctx.run(-1)
+    assert rc2 == 0
+    lines = ctx.logc.lines
+    ctx.close()
+    assert [l.strip() for l in lines if l.startswith("Sigma0:")] == [l.strip() for l in sig]
