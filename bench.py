@@ -376,6 +376,8 @@ def run_batch_bench(args):
         "tuning": tuning,
         "setup_s": {"generate": round(gen_s, 2), "create_incl_autotune": round(create_s, 2)},
     }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_blocks(blocks)
     print(json.dumps(line), flush=True)
 
 
@@ -415,6 +417,28 @@ def cpu_baseline(sc):
     h.close()
     return {"value": round(1.0 / dt, 5), "unit": "LM iters/s", "cores": 1, "kind": "port",
             "sample": f"1 LM iteration of pass 0 ({its} PCG iterations), {dt:.1f} s"}
+
+
+def cpu_baseline_blocks(blocks, budget_s=15.0):
+    """The CPU oracle's per-block solve (the reference's LSA with block-Jacobi
+    PCG, single-threaded) on a bounded sample: whole solves of the blocks in
+    order until about budget_s of CPU time; block LM iterations per second."""
+    from oracle import oracle
+    from rpba_amd import capi
+
+    t0 = time.perf_counter()
+    its, n, cams = 0, 0, 0
+    for sc in blocks:
+        rc, out, st, log = oracle.solve(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1))
+        its += st["itt"]
+        n += 1
+        cams += sc["n_cams"]
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(its / dt, 4), "unit": "block LM iters/s", "cores": 1, "kind": "port",
+            "sample": f"whole solves of the first {n} of {len(blocks)} blocks ({cams} cameras, {its} LM iterations), "
+                      f"{dt:.1f} s"}
 
 
 def calibrate_copy(nbytes, reps=5):

@@ -94,3 +94,19 @@ def test_batch_block_error_is_isolated():
     for i in (0, 2):
         _compare_block(blocks[i], flags, res[i], f"block {i}")
     _compare_block(bad, flags, res[1], "failing block")
+
+
+def test_batch_fp32_records_match_single_block_solves():
+    """precision = 1 in the batch: each block equals the single-block GPU
+    solve of the same form (FP32 records, dense solve) under the trajectory
+    rule (the two differ only in the order of the cost sums)."""
+    blocks = _blocks(1200, [(10, 400), (22, 900), (6, 250)])
+    flags = capi.make_flags(robust=1, solver=1, precision=1)
+    res = api.calc_plsa_batch(blocks, flags)
+    for i, (sc, (rc1, out1, st1, log1)) in enumerate(zip(blocks, res)):
+        rc0, out0, st0, log0 = api.calc_plsa(sc, flags)
+        print(f"[batch fp32 block {i}] itt {st0['itt']} {st1['itt']}", flush=True)
+        assert rc1 == rc0 == 0 and st1["itt"] == st0["itt"]
+        check_trajectory(st1["trace_sig0"], st0["trace_sig0"], st1["trace_pcg"], st0["trace_pcg"],
+                         st1["trace_accept"], st0["trace_accept"], st1["trace_nrobs"], st0["trace_nrobs"])
+        assert np.array_equal(out1["obs_kept"], out0["obs_kept"])
