@@ -450,7 +450,7 @@ static int build_structure(rpba_batch *c)
     rc |= c->d_Yrec.alloc((size_t)kRec * esz * ntr);
     rc |= c->d_Vinv.alloc(9 * (size_t)np);
     rc |= c->d_gp.alloc(3 * (size_t)np);
-    rc |= c->d_Su.alloc(81 * (size_t)std::max(c->ds.nblk, 1));
+    rc |= c->d_Su.alloc(su_doubles(c->ds.nblk));
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
     const int nbk = std::max(cost_blocks(c->ds.ntr), (int)((np + 255) / 256 + 1));
     rc |= c->d_partial.alloc((size_t)nbk);

@@ -2,6 +2,7 @@
 // cores (the exact-solve form, solver = 1).  The plan, storage and the
 // reason for each kernel's shape are in dense.h.
 #include "dense.h"
+#include "kernels.h"
 
 #include <algorithm>
 #include <chrono>
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(128) void k_dense_off(DenseArgs a, const int *camsy
     const int ld = a.ld[m];
     const long long kl = 9LL * (k - a.cam0[m]), ll = 9LL * (l - a.cam0[m]);
     const int i = t / 9, j = t - 9 * i;
-    a.A[a.base[m] + (kl + i) * ld + ll + j] = Su[81LL * e + t];
+    a.A[a.base[m] + (kl + i) * ld + ll + j] = Su[su_index(e, i, j)];
 }
 
 // diagonal tile (k, k): unblocked right-looking Cholesky in LDS (column-major,
@@ -528,22 +529,26 @@ extern "C" int rpba_dense_solve_batch(int32_t nsys, const int32_t *ncam, const d
     std::vector<double> Sd(81 * (size_t)total), Su;
     std::vector<int> bk, bl;
     size_t soff = 0;
+    long long nbt = 0;
+    for (int m = 0; m < nsys; ++m) nbt += (long long)nc[m] * (nc[m] - 1) / 2;
+    Su.assign(su_doubles(nbt), 0.);
     for (int m = 0; m < nsys; ++m) {
         const size_t n = 9 * (size_t)nc[m];
         const double *Sm = S + soff;
         for (int k = 0; k < nc[m]; ++k)
             for (int l = k; l < nc[m]; ++l) {
-                double *dst;
                 if (k == l) {
-                    dst = Sd.data() + 81 * (size_t)(cam0[m] + k);
+                    double *dst = Sd.data() + 81 * (size_t)(cam0[m] + k);
+                    for (int i = 0; i < 9; ++i)
+                        for (int j = 0; j < 9; ++j) dst[9 * i + j] = Sm[(9 * (size_t)k + i) * n + 9 * (size_t)l + j];
                 } else {
-                    Su.resize(Su.size() + 81);
-                    dst = Su.data() + Su.size() - 81;
+                    const size_t e = bk.size();
                     bk.push_back(cam0[m] + k);
                     bl.push_back(cam0[m] + l);
+                    for (int i = 0; i < 9; ++i)
+                        for (int j = 0; j < 9; ++j)
+                            Su[su_index(e, i, j)] = Sm[(9 * (size_t)k + i) * n + 9 * (size_t)l + j];
                 }
-                for (int i = 0; i < 9; ++i)
-                    for (int j = 0; j < 9; ++j) dst[9 * i + j] = Sm[(9 * (size_t)k + i) * n + 9 * (size_t)l + j];
             }
         soff += n * n;
     }

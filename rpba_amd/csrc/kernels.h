@@ -8,6 +8,23 @@ namespace rpba {
 // doubles per entry record of W and Y (27 used; 256 bytes keeps records line aligned)
 constexpr int kRec = 32;
 
+// Upper off-diagonal blocks of S (9 x 9, block list blk_k/blk_l) are stored in
+// slabs of kSlab consecutive blocks.  Inside a slab, element (i, j) of block g
+// sits at j * 9 kSlab + 9 g + i: for a fixed column j the rows of all kSlab
+// blocks are contiguous, so lane 9 g + i of a wavefront reading column j of a
+// slab reads 63 consecutive doubles (504 bytes).
+constexpr int kSlab = 7;
+__host__ __device__ inline size_t su_index(size_t b, int i, int j)
+{
+    return (b / kSlab) * (81 * kSlab) + (size_t)j * (9 * kSlab) + (b % kSlab) * 9 + i;
+}
+// doubles of the slab store for nblk blocks (the last slab padded)
+inline size_t su_doubles(long long nblk)
+{
+    const long long ns = (nblk + kSlab - 1) / kSlab;
+    return 81 * (size_t)kSlab * (size_t)(ns > 0 ? ns : 1);
+}
+
 // device-resident camera state, all double, indexed by image
 struct CamPtrs {
     double *Rb, *Cb;    // base rotation / centre offset of the pass (eRM, ePC)

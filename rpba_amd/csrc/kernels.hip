@@ -15,6 +15,7 @@
 //     off-diagonal blocks Su[b] of the covisibility list.
 // Every reduction runs in a fixed order: results are bitwise reproducible.
 #include <hip/hip_runtime.h>
+#include <cstdlib>
 
 #include "kernels.h"
 
@@ -566,11 +567,10 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
         __syncthreads();
     }
     if (i16 < 9) {
-        double *S = a.Su + (size_t)bid * 81;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = q + 4 * v;
-            if (i < 9) S[i * 9 + i16] = -acc[v];
+            if (i < 9) a.Su[su_index(bid, i, i16)] = -acc[v];
         }
     }
 }
@@ -671,99 +671,123 @@ __global__ __launch_bounds__(1024) void k_pcg_init(PcgArgs a)
     }
 }
 
-// S d reading every stored block once, in two launches per PCG iteration.
-// k_spmv_tiles: persistent workgroups over tiles of kTile consecutive upper
-// blocks.  A tile is staged through LDS with 16-byte coalesced loads; group g
-// of 9 lanes then forms S_kl^T d_k (row l's share), stored at the block's
-// position in column order so each row reads its column shares contiguously,
-// and S_kl d_l, whose run of blocks of the same row k inside the tile is
-// summed in block order into yN[first block of the run].  Launch j scans the
-// tiles backwards when j is odd, so the blocks read last by one launch (still
-// held by the memory-side cache when S is not much larger than it) are read
-// first by the next.  After its tiles a workgroup forms the diagonal products
-// yD_k = S_kk d_k of cameras k = blockIdx.x, blockIdx.x + gridDim.x, ..., and
-// writes its share of d.S d = sum_k d_k.yD_k + 2 sum_(k<l) d_k.S_kl d_l.
+// S d reading every stored block once.  k_spmv_slabs: persistent, and every
+// wavefront runs on its own (no workgroup barrier inside the slab loop).
+// Wavefront w of the grid takes slabs w, w + W, ... (W = all wavefronts of the
+// launch); launch j scans the slabs backwards when j is odd, so the blocks read
+// last by one launch (still held by the memory-side cache when S is not much
+// larger than it) are read first by the next.  Lane 9 g + r (g < 7) holds row
+// r of block g of the slab: nine loads, one per column, each covering 63
+// consecutive doubles across the wavefront; the next slab's loads are in
+// flight while the current one is computed.  The lane forms (S_kl d_l)_r, and
+// the run of blocks of one row k inside the slab is summed in block order
+// into yN[first block of the run].  For S_kl^T d_k the products
+// S_kl[r][j] d_k[r] go through the wavefront's LDS, where lane 9 g + j adds
+// column j over r in order; the result is stored in block order (yT), so a
+// wavefront's seven shares are one contiguous 504-byte store (stores to the
+// blocks' positions in column order scatter 72-byte pieces over the whole
+// array, which at tens of GB costs far more than gathering them back through
+// the column list in k_pcg_a).  After its
+// slabs a workgroup forms the diagonal products yD_k = S_kk d_k of cameras
+// k = blockIdx.x, blockIdx.x + gridDim.x, ..., and writes its share of
+// d.S d = sum_k d_k.yD_k + 2 sum_(k<l) d_k.S_kl d_l.
 // k_pcg_a (below) gathers row k of S d from yD, yN and yT.
-constexpr int kTile = 28;
+constexpr int kSpmvWaves = 4;  // wavefronts per workgroup (256 threads)
 
-// the next tile's S elements and block indices go to registers while the
-// current tile is computed from LDS
-#define SPMV_TILE_FETCH(ii)                                                           \
+// orders LDS accesses of one wavefront (lanes exchange values through LDS)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Issue order per slab s, chosen for the in-order vector memory counter:
+// the d entries of slab s (indices fetched two slabs earlier, so complete),
+// then the columns of slab s + W and the indices of slab s + 2W.  Waiting
+// for the d entries then leaves the next slab's columns in flight while slab
+// s is computed.  Every load is issued unconditionally (addresses clamped
+// into the arrays: lane 63 reads lane 62's column entries, slabs past the
+// end read the last slab, blocks past the end the last block) so that the
+// compiler can count them; values of such lanes are never used.
+#define SPMV_SLAB_DATA(ss)                                                            \
     {                                                                                 \
-        const int tile_ = rev ? ntile - 1 - (ii) : (ii);                              \
-        const int b0_ = tile_ * kTile, nb_ = min(kTile, a.nblk - b0_);                \
-        const double2 *src_ = (const double2 *)(a.Su + 81 * (size_t)b0_);             \
-        const int n_ = nb_ * 81, n2_ = n_ / 2;                                        \
-        if (t < n2_) v0 = src_[t];                                                    \
-        if (t + 256 < n2_) v1 = src_[t + 256];                                        \
-        if (t + 512 < n2_) v2 = src_[t + 512];                                        \
-        if (t + 768 < n2_) v3 = src_[t + 768];                                        \
-        if (t + 1024 < n2_) v4 = src_[t + 1024];                                      \
-        if ((n_ & 1) && t == 0) vl = a.Su[81 * (size_t)b0_ + n_ - 1];                 \
-        if (t < nb_) {                                                                \
-            pk = a.blk_k[b0_ + t];                                                    \
-            pl = a.blk_l[b0_ + t];                                                    \
-        }                                                                             \
+        const int ssc_ = min((ss), nslab - 1);                                        \
+        const int sl_ = rev ? nslab - 1 - ssc_ : ssc_;                                \
+        const double *src_ = a.Su + (size_t)sl_ * (81 * kSlab) + lc;                  \
+        _Pragma("unroll") for (int cc = 0; cc < 9; ++cc) v[cc] = src_[cc * 9 * kSlab]; \
+    }
+#define SPMV_SLAB_IDX(ss, K, L)                                                       \
+    {                                                                                 \
+        const int ssc_ = min((ss), nslab - 1);                                        \
+        const int sl_ = rev ? nslab - 1 - ssc_ : ssc_;                                \
+        const int b_ = min(sl_ * kSlab + gc, a.nblk - 1);                             \
+        K = a.blk_k[b_];                                                              \
+        L = a.blk_l[b_];                                                              \
     }
 
-// Persistent: workgroup w takes tiles w, w + gridDim.x, ... (grid = the
-// workgroups the chip holds at once, four per CU).
-__device__ __forceinline__ void spmv_tiles_body(const PcgArgs &a, int j)
+__device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j)
 {
-    const int ntile = (a.nblk + kTile - 1) / kTile;
+    const int nslab = (a.nblk + kSlab - 1) / kSlab;
     const bool rev = (j & 1) != 0;
-    const int t = threadIdx.x;
-    const int g = t / 9, r = t % 9;
-    // kTile * 81 doubles: a multiple of 16 bytes, so every tile starts 16-byte aligned
-    __shared__ double2 sb[kTile * 81 / 2];
-    __shared__ double yn[kTile][9];
-    __shared__ int rowk[kTile], coll[kTile];
-    double2 v0 = {0., 0.}, v1 = v0, v2 = v0, v3 = v0, v4 = v0;
-    double vl = 0.;
-    int pk = 0, pl = 0;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int g = lane / 9, r = lane - 9 * g;  // g = 7: lane 63, idle
+    const int lc = min(lane, 9 * kSlab - 1), gc = min(g, kSlab - 1);
+    const int W = gridDim.x * kSpmvWaves;
+    __shared__ double P[kSpmvWaves][kSlab][81];  // S_kl[rr][jj] d_k[rr] at [g][9 jj + rr]
+    __shared__ double yn[kSpmvWaves][kSlab][9];
+    __shared__ double dls[kSpmvWaves][kSlab + 1][9];  // row kSlab: lane 63
+    __shared__ int rowk[kSpmvWaves][kSlab];
+    double v[9];
+    int k0, l0;  // block g of slab s
+    int k1, l1;  // block g of slab s + W
     double du = 0.;  // this lane's share of sum over upper blocks of d_k.S_kl d_l
-    int i = blockIdx.x;
-    if (i < ntile) SPMV_TILE_FETCH(i);
-    for (; i < ntile; i += gridDim.x) {
-        const int tile = rev ? ntile - 1 - i : i;
-        const int b0 = tile * kTile, nb = min(kTile, a.nblk - b0);
-        const int n = nb * 81, n2 = n / 2;
-        if (t < n2) sb[t] = v0;
-        if (t + 256 < n2) sb[t + 256] = v1;
-        if (t + 512 < n2) sb[t + 512] = v2;
-        if (t + 768 < n2) sb[t + 768] = v3;
-        if (t + 1024 < n2) sb[t + 1024] = v4;
-        if ((n & 1) && t == 0) ((double *)sb)[n - 1] = vl;
-        if (t < nb) {
-            rowk[t] = pk;
-            coll[t] = pl;
-        }
-        __syncthreads();
-        if (i + (int)gridDim.x < ntile) SPMV_TILE_FETCH(i + (int)gridDim.x);
+    int s = blockIdx.x * kSpmvWaves + w;
+    if (s < nslab) {
+        SPMV_SLAB_IDX(s, k0, l0);
+        SPMV_SLAB_DATA(s);
+        SPMV_SLAB_IDX(s + W, k1, l1);
+    }
+    for (; s < nslab; s += W) {
+        const int slab = rev ? nslab - 1 - s : s;
+        const int b0 = slab * kSlab, nb = min(kSlab, a.nblk - b0);
+        const double dk = a.d[9 * (size_t)k0 + r];
+        const double dl = a.d[9 * (size_t)l0 + r];
+        double c[9];
+#pragma unroll
+        for (int cc = 0; cc < 9; ++cc) c[cc] = v[cc];
+        const int k = k0;
+        k0 = k1; l0 = l1;
+        SPMV_SLAB_DATA(s + W);
+        SPMV_SLAB_IDX(s + 2 * W, k1, l1);
+        dls[w][g][r] = dl;  // every lane: keeps the load ahead of the prefetch
         if (g < nb) {
-            const double *B = (const double *)sb + 81 * g;
-            const double *dl = a.d + 9 * (size_t)coll[g];
-            const double *dk = a.d + 9 * (size_t)rowk[g];
+            if (r == 0) rowk[w][g] = k;
+#pragma unroll
+            for (int jj = 0; jj < 9; ++jj) P[w][g][9 * jj + r] = c[jj] * dk;
+        }
+        wave_lds_sync();
+        if (g < nb) {
             double yv = 0., tv = 0.;
 #pragma unroll
             for (int cc = 0; cc < 9; ++cc) {
-                yv += B[r * 9 + cc] * dl[cc];
-                tv += B[cc * 9 + r] * dk[cc];
+                yv += c[cc] * dls[w][g][cc];
+                tv += P[w][g][9 * r + cc];
             }
-            yn[g][r] = yv;
-            a.yT[9 * (size_t)a.col_pos[b0 + g] + r] = tv;
-            du += dk[r] * yv;
+            yn[w][g][r] = yv;
+            a.yT[9 * (size_t)(b0 + g) + r] = tv;
+            du += dk * yv;
         }
-        __syncthreads();
-        if (g < nb && (g == 0 || rowk[g] != rowk[g - 1])) {
+        wave_lds_sync();
+        if (g < nb && (g == 0 || rowk[w][g] != rowk[w][g - 1])) {
             // the run of row rowk[g] in block order (blocks of one row are adjacent)
-            const int rk = rowk[g];
+            const int rk = rowk[w][g];
             double sum = 0.;
 #pragma unroll
-            for (int h = 0; h < kTile; ++h) sum += (h >= g && h < nb && rowk[h] == rk) ? yn[h][r] : 0.;
+            for (int h = 0; h < kSlab; ++h) sum += (h >= g && h < nb && rowk[w][h] == rk) ? yn[w][h][r] : 0.;
             a.yN[9 * (size_t)(b0 + g) + r] = sum;
         }
-        __syncthreads();
+        wave_lds_sync();  // the next slab overwrites this wavefront's LDS
     }
     // diagonal blocks
     double dd = 0.;
@@ -783,12 +807,13 @@ __device__ __forceinline__ void spmv_tiles_body(const PcgArgs &a, int j)
     const double sd = block_sum(dd, sh);
     if (t == 0) a.dpart[blockIdx.x] = 2. * su + sd;
 }
-#undef SPMV_TILE_FETCH
+#undef SPMV_SLAB_DATA
+#undef SPMV_SLAB_IDX
 
-__global__ __launch_bounds__(256) void k_spmv_tiles(PcgArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spmv_slabs(PcgArgs a)
 {
     if (a.S[a.j & 1].done) return;
-    spmv_tiles_body(a, a.j);
+    spmv_slabs_body(a, a.j);
 }
 
 // sum of n values in a fixed order; every thread of the block gets it
@@ -969,26 +994,29 @@ __device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S
     } else if (g < 28) {
     const int r0 = a.row_off[k], r1 = a.row_off[k + 1];
     const int c0 = a.col_off[k], nl = a.col_off[k + 1] - c0;
-    // runs of row k: one per tile that holds blocks of the row
-    const int t0 = r0 / kTile, nrun = (r1 > r0) ? (r1 - 1) / kTile - t0 + 1 : 0;
+    // runs of row k: one per slab that holds blocks of the row
+    const int t0 = r0 / kSlab, nrun = (r1 > r0) ? (r1 - 1) / kSlab - t0 + 1 : 0;
     {
         double acc = (g == 0) ? a.Ad[9 * (size_t)k + r] : 0.;
         for (int it = g; it < nrun; it += 28) {
-            const int s0 = max(r0, (t0 + it) * kTile);
+            const int s0 = max(r0, (t0 + it) * kSlab);
             acc += a.yN[9 * (size_t)s0 + r];
         }
-        // column shares are contiguous; four loads in flight, added in order
-        const double *yc = a.yT + 9 * (size_t)c0 + r;
+        // column shares S_lk^T d_l of the blocks (l, k), l < k, stored in block
+        // order: gathered through the column list; four loads in flight,
+        // added in order
+        const int *cb = a.col_blk + c0;
         int it = g;
         for (; it + 3 * 28 < nl; it += 4 * 28) {
-            const double v0 = yc[9 * (size_t)it], v1 = yc[9 * (size_t)(it + 28)];
-            const double v2 = yc[9 * (size_t)(it + 56)], v3 = yc[9 * (size_t)(it + 84)];
+            const int b0 = cb[it], b1 = cb[it + 28], b2 = cb[it + 56], b3 = cb[it + 84];
+            const double v0 = a.yT[9 * (size_t)b0 + r], v1 = a.yT[9 * (size_t)b1 + r];
+            const double v2 = a.yT[9 * (size_t)b2 + r], v3 = a.yT[9 * (size_t)b3 + r];
             acc += v0;
             acc += v1;
             acc += v2;
             acc += v3;
         }
-        for (; it < nl; it += 28) acc += yc[9 * (size_t)it];
+        for (; it < nl; it += 28) acc += a.yT[9 * (size_t)cb[it] + r];
         red[g][r] = acc;
     }
     }
@@ -1048,19 +1076,17 @@ __global__ __launch_bounds__(256) void k_offdiag_mv(PcgArgs a, const double *t_i
         for (int e = g; e < nr + nl; e += 28) {
             if (e < nr) {
                 const int b = r0 + e;
-                const double *B = a.Su + 81 * (size_t)b + 9 * r;
                 const double *tl = t_in + 9 * (size_t)a.blk_l[b];
                 double s = 0.;
 #pragma unroll
-                for (int c = 0; c < 9; ++c) s += B[c] * tl[c];
+                for (int c = 0; c < 9; ++c) s += a.Su[su_index(b, r, c)] * tl[c];
                 acc += s;
             } else {
                 const int b = a.col_blk[c0 + e - nr];
-                const double *B = a.Su + 81 * (size_t)b + r;
                 const double *tl = t_in + 9 * (size_t)a.blk_k[b];
                 double s = 0.;
 #pragma unroll
-                for (int c = 0; c < 9; ++c) s += B[9 * c] * tl[c];
+                for (int c = 0; c < 9; ++c) s += a.Su[su_index(b, c, r)] * tl[c];
                 acc += s;
             }
         }
@@ -1650,15 +1676,27 @@ static int device_cus()
     return n;
 }
 // SpMV workgroups: four per CU (persistent over the tiles)
+// workgroups of the SpMV: as many as the chip holds at once (its occupancy
+// per CU times the CUs), unless RPBA_SPMV_WPC (1..16) sets the count per CU
+// for a launch-shape sweep.  The grid changes the order of the d.S d partial
+// sums, not the products.
 int spmv_grid(int nblk)
 {
     (void)nblk;
-    return 4 * device_cus();
+    static int per_cu = 0;
+    if (!per_cu) {
+        const char *e = getenv("RPBA_SPMV_WPC");
+        const int v = e ? atoi(e) : 0;
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_spmv_slabs, 256, 0) != hipSuccess || occ < 1) occ = 4;
+        per_cu = (v >= 1 && v <= 16) ? v : occ;
+    }
+    return per_cu * device_cus();
 }
 void launch_spmv(const PcgArgs &a, hipStream_t st)
 {
     if (!a.implicit) {
-        k_spmv_tiles<<<a.ndpart, 256, 0, st>>>(a);
+        k_spmv_slabs<<<a.ndpart, 256, 0, st>>>(a);
         return;
     }
     if (a.npgrp <= 0 || a.nch <= 0) return;

@@ -4,6 +4,7 @@
 // in a fixed order (per-node loops in edge order, per-chunk partials summed in
 // chunk order), so training is bitwise reproducible.
 #include "lp.h"
+#include "kernels.h"
 
 #include <algorithm>
 #include <cmath>
@@ -68,7 +69,11 @@ __global__ __launch_bounds__(256) void k_lp_edges(int nblk, const double *Su, co
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nblk) return;
     double s = 0.;
-    for (int j = 0; j < 81; ++j) s += Su[81 * (size_t)b + j] * Su[81 * (size_t)b + j];
+    for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) {
+            const double v = Su[su_index(b, i, j)];
+            s += v * v;
+        }
     const double den = sqrt((double)nrm[bk[b]] * (double)nrm[bl[b]]);
     eb[b] = den > 0. ? (float)(sqrt(s) / den) : 0.f;
 }

@@ -221,7 +221,7 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_Yrec.alloc((size_t)kRec * esz * ntr);
     rc |= c->d_Vinv.alloc(9 * np);
     rc |= c->d_gp.alloc(3 * np);
-    rc |= c->d_Su.alloc(81 * (size_t)std::max(c->ds.nblk, 1));
+    rc |= c->d_Su.alloc(su_doubles(c->ds.nblk));
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
     rc |= c->d_yN.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
     rc |= c->d_yT.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
