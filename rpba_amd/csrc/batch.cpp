@@ -450,7 +450,7 @@ static int build_structure(rpba_batch *c)
     rc |= c->d_Yrec.alloc((size_t)kRec * esz * ntr);
     rc |= c->d_Vinv.alloc(9 * (size_t)np);
     rc |= c->d_gp.alloc(3 * (size_t)np);
-    rc |= c->d_Su.alloc(su_doubles(c->ds.nblk));
+    rc |= c->d_Su.alloc(81 * (size_t)std::max(c->ds.nblk, 1));
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
     const int nbk = std::max(cost_blocks(c->ds.ntr), (int)((np + 255) / 256 + 1));
     rc |= c->d_partial.alloc((size_t)nbk);
@@ -573,13 +573,13 @@ static void iteration_launches(rpba_batch *c)
     LinArgs la = lin_args(c);
     SchurArgs sa;
     sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
-    sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.rec32 = c->rec32;
+    sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.sm = c->ds.sumap(); sa.rec32 = c->rec32;
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
     c->timed("schur_pairs", [&] { launch_schur_pairs(sa, c->st); });
     c->timed("schur_diag", [&] { launch_schur_diag(la, c->st); });
     c->timed("lin_cameras", [&] { launch_lin_cameras(la, c->st); });
     c->timed("dense_assemble", [&] {
-        c->dense.assemble(c->d_Sd.p, c->d_Su.p, c->ds.bk, c->ds.bl, c->ds.nblk, c->d_b.p, c->st);
+        c->dense.assemble(c->d_Sd.p, c->d_Su.p, c->ds.sumap(), c->ds.bk, c->ds.bl, c->ds.nblk, c->d_b.p, c->st);
     });
     c->dense.factor_solve(c->d_sol.p, c->st, [&](const char *nm, auto f) { c->timed(nm, f); });
     StepArgs sa2 = step_args(c, c->d_sol.p);

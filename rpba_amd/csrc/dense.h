@@ -20,6 +20,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include "kernels.h"
 
 #include <cstdint>
 #include <string>
@@ -54,7 +55,7 @@ public:
     // zero the systems, then scatter S_kk (81 per camera, row-major), the
     // upper blocks S_kl (k < l, row-major 9 x 9, block list blk_k/blk_l) and
     // b (9 per camera) into the lower triangles and right-hand-side rows.
-    void assemble(const double *Sd, const double *Su, const int *blk_k, const int *blk_l, int nblk,
+    void assemble(const double *Sd, const double *Su, const SuMap &sm, const int *blk_k, const int *blk_l, int nblk,
                   const double *b, hipStream_t st);
     // factor and solve; writes sol (9 per camera of the planned systems).
     // timer(name, launch) wraps each launch (for per-kernel HIP-event timing).

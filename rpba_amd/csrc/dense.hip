@@ -59,8 +59,8 @@ __global__ __launch_bounds__(128) void k_dense_diag(DenseArgs a, const int *cams
 
 // upper block S_kl (k < l) of camera rows k, columns l: lower element
 // (row 9 l + j, column 9 k + i) = S_kl[i][j]
-__global__ __launch_bounds__(128) void k_dense_off(DenseArgs a, const int *camsys, const double *Su, const int *bk,
-                                                   const int *bl)
+__global__ __launch_bounds__(128) void k_dense_off(DenseArgs a, const int *camsys, const double *Su, SuMap sm,
+                                                   const int *bk, const int *bl)
 {
     const int e = blockIdx.x, t = threadIdx.x;
     if (t >= 81) return;
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(128) void k_dense_off(DenseArgs a, const int *camsy
     const int ld = a.ld[m];
     const long long kl = 9LL * (k - a.cam0[m]), ll = 9LL * (l - a.cam0[m]);
     const int i = t / 9, j = t - 9 * i;
-    a.A[a.base[m] + (kl + i) * ld + ll + j] = Su[su_index(e, i, j)];
+    a.A[a.base[m] + (kl + i) * ld + ll + j] = Su[su_at(sm, e, i, j)];
 }
 
 // diagonal tile (k, k): unblocked right-looking Cholesky in LDS (column-major,
@@ -390,15 +390,15 @@ double DenseSolver::flops() const
     return f;
 }
 
-void DenseSolver::assemble(const double *Sd, const double *Su, const int *blk_k, const int *blk_l, int nblk,
-                           const double *b, hipStream_t st)
+void DenseSolver::assemble(const double *Sd, const double *Su, const SuMap &sm, const int *blk_k, const int *blk_l,
+                           int nblk, const double *b, hipStream_t st)
 {
     DenseArgs a = args(nullptr);
     const long long g = std::min<long long>((total_ + 255) / 256, 4096);
     k_dense_fill<<<(int)std::max<long long>(g, 1), 256, 0, st>>>(A_, total_);
     if (nsys_) k_dense_pad<<<nsys_, 64, 0, st>>>(a);
     if (ncam_total_) k_dense_diag<<<ncam_total_, 128, 0, st>>>(a, camsys_, Sd, b);
-    if (nblk) k_dense_off<<<nblk, 128, 0, st>>>(a, camsys_, Su, blk_k, blk_l);
+    if (nblk) k_dense_off<<<nblk, 128, 0, st>>>(a, camsys_, Su, sm, blk_k, blk_l);
 }
 
 void DenseSolver::launch_potrf(const DenseArgs &a, int k, hipStream_t st)
@@ -529,9 +529,10 @@ extern "C" int rpba_dense_solve_batch(int32_t nsys, const int32_t *ncam, const d
     std::vector<double> Sd(81 * (size_t)total), Su;
     std::vector<int> bk, bl;
     size_t soff = 0;
+    // one block per slab (slot b, slab b): element (i, j) of block b at 81 b + 9 j + i
     long long nbt = 0;
     for (int m = 0; m < nsys; ++m) nbt += (long long)nc[m] * (nc[m] - 1) / 2;
-    Su.assign(su_doubles(nbt), 0.);
+    Su.assign(81 * (size_t)std::max(nbt, 1LL), 0.);
     for (int m = 0; m < nsys; ++m) {
         const size_t n = 9 * (size_t)nc[m];
         const double *Sm = S + soff;
@@ -547,7 +548,7 @@ extern "C" int rpba_dense_solve_batch(int32_t nsys, const int32_t *ncam, const d
                     bl.push_back(cam0[m] + l);
                     for (int i = 0; i < 9; ++i)
                         for (int j = 0; j < 9; ++j)
-                            Su[su_index(e, i, j)] = Sm[(9 * (size_t)k + i) * n + 9 * (size_t)l + j];
+                            Su[81 * e + 9 * j + i] = Sm[(9 * (size_t)k + i) * n + 9 * (size_t)l + j];
                 }
             }
         soff += n * n;
@@ -566,6 +567,8 @@ extern "C" int rpba_dense_solve_batch(int32_t nsys, const int32_t *ncam, const d
     rc |= dmalloc(&dx, n9);
     rc |= dmalloc(&dk, nb);
     rc |= dmalloc(&dl, nb);
+    int *dio = nullptr;  // 0, 1, ..., nb: slots, slabs and slab starts of one block per slab
+    rc |= dmalloc(&dio, nb + 1);
     bool ok = rc == 0;
     if (ok) {
         ok &= hipMemcpy(dSd, Sd.data(), sizeof(double) * Sd.size(), hipMemcpyHostToDevice) == hipSuccess;
@@ -573,6 +576,9 @@ extern "C" int rpba_dense_solve_batch(int32_t nsys, const int32_t *ncam, const d
             ok &= hipMemcpy(dSu, Su.data(), sizeof(double) * Su.size(), hipMemcpyHostToDevice) == hipSuccess;
             ok &= hipMemcpy(dk, bk.data(), sizeof(int) * nb, hipMemcpyHostToDevice) == hipSuccess;
             ok &= hipMemcpy(dl, bl.data(), sizeof(int) * nb, hipMemcpyHostToDevice) == hipSuccess;
+            std::vector<int> io(nb + 1);
+            for (size_t e = 0; e <= nb; ++e) io[e] = (int)e;
+            ok &= hipMemcpy(dio, io.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice) == hipSuccess;
         }
         ok &= hipMemcpy(db, b, sizeof(double) * n9, hipMemcpyHostToDevice) == hipSuccess;
     }
@@ -583,12 +589,13 @@ extern "C" int rpba_dense_solve_batch(int32_t nsys, const int32_t *ncam, const d
             ds.set_variant(variant, cap);
     }
     if (ok) {
-        ds.assemble(dSd, dSu, dk, dl, (int)nb, db, st);
+        const SuMap sm{dio, dio, dio};
+        ds.assemble(dSd, dSu, sm, dk, dl, (int)nb, db, st);
         ds.factor_solve(dx, st, [](const char *, auto f) { f(); });
         ok = hipStreamSynchronize(st) == hipSuccess && hipGetLastError() == hipSuccess;
         if (ok) ok = hipMemcpy(x, dx, sizeof(double) * n9, hipMemcpyDeviceToHost) == hipSuccess;
     }
-    dfree(dSd); dfree(dSu); dfree(db); dfree(dx); dfree(dk); dfree(dl);
+    dfree(dSd); dfree(dSu); dfree(db); dfree(dx); dfree(dk); dfree(dl); dfree(dio);
     (void)hipStreamDestroy(st);
     if (!ok) return rc ? rc : RPBA_E_HIP;
     return RPBA_OK;

@@ -8,21 +8,24 @@ namespace rpba {
 // doubles per entry record of W and Y (27 used; 256 bytes keeps records line aligned)
 constexpr int kRec = 32;
 
-// Upper off-diagonal blocks of S (9 x 9, block list blk_k/blk_l) are stored in
-// slabs of kSlab consecutive blocks.  Inside a slab, element (i, j) of block g
-// sits at j * 9 kSlab + 9 g + i: for a fixed column j the rows of all kSlab
-// blocks are contiguous, so lane 9 g + i of a wavefront reading column j of a
-// slab reads 63 consecutive doubles (504 bytes).
+// Upper off-diagonal blocks of S (9 x 9, block list blk_k/blk_l, row-major
+// order) are stored in strip slabs.  Strip s holds rows kSlab s .. kSlab s +
+// kSlab - 1; its slab for column l holds the m blocks (k, l) of those rows
+// that exist, k ascending, and starts at block slot base.  Inside the slab,
+// element (i, j) of the slab's block number g' sits at 81 base + 9 m j +
+// 9 g' + i: for a fixed column j the rows of all m blocks are contiguous, so
+// lanes 9 g + i of a wavefront read them as one run of 9 m doubles.
 constexpr int kSlab = 7;
-__host__ __device__ inline size_t su_index(size_t b, int i, int j)
+struct SuMap {
+    const int *slot;       // block -> slot (position in strip, column, row order)
+    const int *slot_slab;  // slot -> slab
+    const int *slab_bs;    // slab -> first slot (nslab + 1)
+};
+__device__ __forceinline__ size_t su_at(const SuMap &m, int b, int i, int j)
 {
-    return (b / kSlab) * (81 * kSlab) + (size_t)j * (9 * kSlab) + (b % kSlab) * 9 + i;
-}
-// doubles of the slab store for nblk blocks (the last slab padded)
-inline size_t su_doubles(long long nblk)
-{
-    const long long ns = (nblk + kSlab - 1) / kSlab;
-    return 81 * (size_t)kSlab * (size_t)(ns > 0 ? ns : 1);
+    const int q = m.slot[b], s = m.slot_slab[q];
+    const int base = m.slab_bs[s], n = m.slab_bs[s + 1] - base;
+    return 81 * (size_t)base + (size_t)(9 * n) * j + 9 * (size_t)(q - base) + i;
 }
 
 // device-resident camera state, all double, indexed by image
@@ -65,6 +68,7 @@ struct SchurArgs {
     const unsigned long long *pab;  // pair (a << 32 | b), cam(a) < cam(b)
     const void *Yrec, *Wrec;
     double *Su;
+    SuMap sm;
     int rec32;
 };
 
@@ -75,7 +79,12 @@ struct PcgArgs {
     double *x, *best, *r, *d, *h, *h1;
     double *Ad;       // S_kk d_k per camera (diagonal products of the SpMV)
     double *part;     // per camera: |alpha S d|^2, r.(h1 - h), r.h1 shares
-    double *yN, *yT;  // SpMV row-run sums (at the run's first block) and column shares (column order)
+    SuMap sm;
+    // strip slabs of S (see SuMap) and the SpMV's partial products: rowpart
+    // (per chunk, 7 rows x 9) and colpart (per slab, 9)
+    int nslab, nchunk, nstrip, slab_chunk;
+    const int *slab_bs, *slab_l, *slab_mask, *strip_s0, *chunk_off, *chunk_strip, *cs_off, *cs_list;
+    double *rowpart, *colpart;
     double *dpart;    // per SpMV workgroup (or per chunk, matrix-free): its share of d.S d
     int nblk, ndpart; // upper blocks; number of dpart shares
     const double *dsum;  // what k_pcg_a adds up for d.S d: dpart, or (matrix-free) its total at dpart[nch]

@@ -570,7 +570,7 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = q + 4 * v;
-            if (i < 9) a.Su[su_index(bid, i, i16)] = -acc[v];
+            if (i < 9) a.Su[su_at(a.sm, bid, i, i16)] = -acc[v];
         }
     }
 }
@@ -671,27 +671,26 @@ __global__ __launch_bounds__(1024) void k_pcg_init(PcgArgs a)
     }
 }
 
-// S d reading every stored block once.  k_spmv_slabs: persistent, and every
-// wavefront runs on its own (no workgroup barrier inside the slab loop).
-// Wavefront w of the grid takes slabs w, w + W, ... (W = all wavefronts of the
-// launch); launch j scans the slabs backwards when j is odd, so the blocks read
-// last by one launch (still held by the memory-side cache when S is not much
-// larger than it) are read first by the next.  Lane 9 g + r (g < 7) holds row
-// r of block g of the slab: nine loads, one per column, each covering 63
-// consecutive doubles across the wavefront; the next slab's loads are in
-// flight while the current one is computed.  The lane forms (S_kl d_l)_r, and
-// the run of blocks of one row k inside the slab is summed in block order
-// into yN[first block of the run].  For S_kl^T d_k the products
+// S d reading every stored block once, from the strip slabs (SuMap).
+// k_spmv_slabs: persistent, and every wavefront runs on its own (no workgroup
+// barrier inside the chunk loop).  Wavefront w of the grid takes chunks w,
+// w + W, ... (W = all wavefronts of the launch; launch j takes them in reverse
+// when j is odd, so the blocks read last by one launch, still held by the
+// memory-side cache when S is not much larger than it, are read first by the
+// next).  A chunk is up to slab_chunk consecutive slabs of one strip (rows
+// k0 .. k0 + 6).  Lane 9 g + r holds row r of the block (k0 + g, l) of each
+// slab: nine loads, one per column, each a run of 9 m consecutive doubles over
+// the slab's m blocks (lanes of absent rows load nothing and count as 0).
+// (S_kl d_l)_r is added over the chunk's slabs in the lane's register and
+// stored once per chunk (rowpart).  For S_kl^T d_k the products
 // S_kl[r][j] d_k[r] go through the wavefront's LDS, where lane 9 g + j adds
-// column j over r in order; the result is stored in block order (yT), so a
-// wavefront's seven shares are one contiguous 504-byte store (stores to the
-// blocks' positions in column order scatter 72-byte pieces over the whole
-// array, which at tens of GB costs far more than gathering them back through
-// the column list in k_pcg_a).  After its
-// slabs a workgroup forms the diagonal products yD_k = S_kk d_k of cameras
+// column j over r in order; the slab's blocks' shares are then added over g
+// in order into one column share per slab (colpart).  After its chunks a
+// workgroup forms the diagonal products yD_k = S_kk d_k of cameras
 // k = blockIdx.x, blockIdx.x + gridDim.x, ..., and writes its share of
 // d.S d = sum_k d_k.yD_k + 2 sum_(k<l) d_k.S_kl d_l.
-// k_pcg_a (below) gathers row k of S d from yD, yN and yT.
+// k_pcg_a (below) gathers row k of S d from yD, the row partials of its
+// strip's chunks and the column shares of the slabs of column k.
 constexpr int kSpmvWaves = 4;  // wavefronts per workgroup (256 threads)
 
 // orders LDS accesses of one wavefront (lanes exchange values through LDS)
@@ -702,92 +701,56 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Issue order per slab s, chosen for the in-order vector memory counter:
-// the d entries of slab s (indices fetched two slabs earlier, so complete),
-// then the columns of slab s + W and the indices of slab s + 2W.  Waiting
-// for the d entries then leaves the next slab's columns in flight while slab
-// s is computed.  Every load is issued unconditionally (addresses clamped
-// into the arrays: lane 63 reads lane 62's column entries, slabs past the
-// end read the last slab, blocks past the end the last block) so that the
-// compiler can count them; values of such lanes are never used.
-#define SPMV_SLAB_DATA(ss)                                                            \
-    {                                                                                 \
-        const int ssc_ = min((ss), nslab - 1);                                        \
-        const int sl_ = rev ? nslab - 1 - ssc_ : ssc_;                                \
-        const double *src_ = a.Su + (size_t)sl_ * (81 * kSlab) + lc;                  \
-        _Pragma("unroll") for (int cc = 0; cc < 9; ++cc) v[cc] = src_[cc * 9 * kSlab]; \
-    }
-#define SPMV_SLAB_IDX(ss, K, L)                                                       \
-    {                                                                                 \
-        const int ssc_ = min((ss), nslab - 1);                                        \
-        const int sl_ = rev ? nslab - 1 - ssc_ : ssc_;                                \
-        const int b_ = min(sl_ * kSlab + gc, a.nblk - 1);                             \
-        K = a.blk_k[b_];                                                              \
-        L = a.blk_l[b_];                                                              \
-    }
-
 __device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j)
 {
-    const int nslab = (a.nblk + kSlab - 1) / kSlab;
     const bool rev = (j & 1) != 0;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    const int g = lane / 9, r = lane - 9 * g;  // g = 7: lane 63, idle
-    const int lc = min(lane, 9 * kSlab - 1), gc = min(g, kSlab - 1);
+    const int g = lane / 9, r = lane - 9 * g;  // g = 7: lane 63, never present
+    const int gc = min(g, kSlab - 1);
     const int W = gridDim.x * kSpmvWaves;
-    __shared__ double P[kSpmvWaves][kSlab][81];  // S_kl[rr][jj] d_k[rr] at [g][9 jj + rr]
-    __shared__ double yn[kSpmvWaves][kSlab][9];
-    __shared__ double dls[kSpmvWaves][kSlab + 1][9];  // row kSlab: lane 63
-    __shared__ int rowk[kSpmvWaves][kSlab];
-    double v[9];
-    int k0, l0;  // block g of slab s
-    int k1, l1;  // block g of slab s + W
+    __shared__ double P[kSpmvWaves][kSlab + 1][81];  // S_kl[rr][jj] d_k[rr] at [g][9 jj + rr]
+    __shared__ double T[kSpmvWaves][kSlab + 1][9];   // block g's column share
     double du = 0.;  // this lane's share of sum over upper blocks of d_k.S_kl d_l
-    int s = blockIdx.x * kSpmvWaves + w;
-    if (s < nslab) {
-        SPMV_SLAB_IDX(s, k0, l0);
-        SPMV_SLAB_DATA(s);
-        SPMV_SLAB_IDX(s + W, k1, l1);
-    }
-    for (; s < nslab; s += W) {
-        const int slab = rev ? nslab - 1 - s : s;
-        const int b0 = slab * kSlab, nb = min(kSlab, a.nblk - b0);
-        const double dk = a.d[9 * (size_t)k0 + r];
-        const double dl = a.d[9 * (size_t)l0 + r];
-        double c[9];
+    for (int ci = blockIdx.x * kSpmvWaves + w; ci < a.nchunk; ci += W) {
+        const int c = rev ? a.nchunk - 1 - ci : ci;
+        const int st = a.chunk_strip[c];
+        const int f = a.strip_s0[st] + a.slab_chunk * (c - a.chunk_off[st]);
+        const int e = min(f + a.slab_chunk, a.strip_s0[st + 1]);
+        const int k = min(kSlab * st + gc, a.nc - 1);
+        const double dk = a.d[9 * (size_t)k + r];
+        double yrow = 0.;
+        for (int q = f; q < e; ++q) {
+            const int base = a.slab_bs[q], m = a.slab_bs[q + 1] - base;
+            const int mask = a.slab_mask[q], l = a.slab_l[q];
+            const bool present = g < kSlab && ((mask >> g) & 1);
+            const int rank = __popc(mask & ((1 << g) - 1));
+            const double *src = a.Su + 81 * (size_t)base + 9 * rank + r;
+            double cv[9];
 #pragma unroll
-        for (int cc = 0; cc < 9; ++cc) c[cc] = v[cc];
-        const int k = k0;
-        k0 = k1; l0 = l1;
-        SPMV_SLAB_DATA(s + W);
-        SPMV_SLAB_IDX(s + 2 * W, k1, l1);
-        dls[w][g][r] = dl;  // every lane: keeps the load ahead of the prefetch
-        if (g < nb) {
-            if (r == 0) rowk[w][g] = k;
+            for (int cc = 0; cc < 9; ++cc) cv[cc] = present ? src[(size_t)(9 * m) * cc] : 0.;
+            const double *dl = a.d + 9 * (size_t)l;
+            double yv = 0.;
 #pragma unroll
-            for (int jj = 0; jj < 9; ++jj) P[w][g][9 * jj + r] = c[jj] * dk;
-        }
-        wave_lds_sync();
-        if (g < nb) {
-            double yv = 0., tv = 0.;
-#pragma unroll
-            for (int cc = 0; cc < 9; ++cc) {
-                yv += c[cc] * dls[w][g][cc];
-                tv += P[w][g][9 * r + cc];
-            }
-            yn[w][g][r] = yv;
-            a.yT[9 * (size_t)(b0 + g) + r] = tv;
+            for (int cc = 0; cc < 9; ++cc) yv += cv[cc] * dl[cc];
+            yrow += yv;
             du += dk * yv;
-        }
-        wave_lds_sync();
-        if (g < nb && (g == 0 || rowk[w][g] != rowk[w][g - 1])) {
-            // the run of row rowk[g] in block order (blocks of one row are adjacent)
-            const int rk = rowk[w][g];
-            double sum = 0.;
 #pragma unroll
-            for (int h = 0; h < kSlab; ++h) sum += (h >= g && h < nb && rowk[w][h] == rk) ? yn[w][h][r] : 0.;
-            a.yN[9 * (size_t)(b0 + g) + r] = sum;
+            for (int jj = 0; jj < 9; ++jj) P[w][g][9 * jj + r] = cv[jj] * dk;
+            wave_lds_sync();
+            double tv = 0.;
+#pragma unroll
+            for (int cc = 0; cc < 9; ++cc) tv += P[w][g][9 * r + cc];
+            T[w][g][r] = tv;
+            wave_lds_sync();
+            if (lane < 9) {
+                double cs = 0.;
+#pragma unroll
+                for (int h = 0; h < kSlab; ++h) cs += T[w][h][lane];
+                a.colpart[9 * (size_t)q + lane] = cs;
+            }
+            wave_lds_sync();  // the next slab overwrites this wavefront's LDS
         }
-        wave_lds_sync();  // the next slab overwrites this wavefront's LDS
+        if (lane < 9 * kSlab) a.rowpart[63 * (size_t)c + lane] = yrow;
     }
     // diagonal blocks
     double dd = 0.;
@@ -807,8 +770,6 @@ __device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j)
     const double sd = block_sum(dd, sh);
     if (t == 0) a.dpart[blockIdx.x] = 2. * su + sd;
 }
-#undef SPMV_SLAB_DATA
-#undef SPMV_SLAB_IDX
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spmv_slabs(PcgArgs a)
 {
@@ -992,31 +953,27 @@ __device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S
             red[g][r] = (g == 0) ? a.Ad[9 * (size_t)k + r] - acc : 0.;
         }
     } else if (g < 28) {
-    const int r0 = a.row_off[k], r1 = a.row_off[k + 1];
-    const int c0 = a.col_off[k], nl = a.col_off[k + 1] - c0;
-    // runs of row k: one per slab that holds blocks of the row
-    const int t0 = r0 / kSlab, nrun = (r1 > r0) ? (r1 - 1) / kSlab - t0 + 1 : 0;
+    // row k: the row partials of its strip's chunks, in chunk order
+    const int st = k / kSlab, gk = k - kSlab * st;
+    const int c0 = a.chunk_off[st], nck = a.chunk_off[st + 1] - c0;
     {
         double acc = (g == 0) ? a.Ad[9 * (size_t)k + r] : 0.;
-        for (int it = g; it < nrun; it += 28) {
-            const int s0 = max(r0, (t0 + it) * kSlab);
-            acc += a.yN[9 * (size_t)s0 + r];
-        }
-        // column shares S_lk^T d_l of the blocks (l, k), l < k, stored in block
-        // order: gathered through the column list; four loads in flight,
+        for (int it = g; it < nck; it += 28) acc += a.rowpart[63 * (size_t)(c0 + it) + 9 * gk + r];
+        // column k: the shares of its slabs, strip order; four loads in flight,
         // added in order
-        const int *cb = a.col_blk + c0;
+        const int q0 = a.cs_off[k], nq = a.cs_off[k + 1] - q0;
+        const int *cl = a.cs_list + q0;
         int it = g;
-        for (; it + 3 * 28 < nl; it += 4 * 28) {
-            const int b0 = cb[it], b1 = cb[it + 28], b2 = cb[it + 56], b3 = cb[it + 84];
-            const double v0 = a.yT[9 * (size_t)b0 + r], v1 = a.yT[9 * (size_t)b1 + r];
-            const double v2 = a.yT[9 * (size_t)b2 + r], v3 = a.yT[9 * (size_t)b3 + r];
+        for (; it + 3 * 28 < nq; it += 4 * 28) {
+            const int b0 = cl[it], b1 = cl[it + 28], b2 = cl[it + 56], b3 = cl[it + 84];
+            const double v0 = a.colpart[9 * (size_t)b0 + r], v1 = a.colpart[9 * (size_t)b1 + r];
+            const double v2 = a.colpart[9 * (size_t)b2 + r], v3 = a.colpart[9 * (size_t)b3 + r];
             acc += v0;
             acc += v1;
             acc += v2;
             acc += v3;
         }
-        for (; it < nl; it += 28) acc += a.yT[9 * (size_t)cb[it] + r];
+        for (; it < nq; it += 28) acc += a.colpart[9 * (size_t)cl[it] + r];
         red[g][r] = acc;
     }
     }
@@ -1079,14 +1036,14 @@ __global__ __launch_bounds__(256) void k_offdiag_mv(PcgArgs a, const double *t_i
                 const double *tl = t_in + 9 * (size_t)a.blk_l[b];
                 double s = 0.;
 #pragma unroll
-                for (int c = 0; c < 9; ++c) s += a.Su[su_index(b, r, c)] * tl[c];
+                for (int c = 0; c < 9; ++c) s += a.Su[su_at(a.sm, b, r, c)] * tl[c];
                 acc += s;
             } else {
                 const int b = a.col_blk[c0 + e - nr];
                 const double *tl = t_in + 9 * (size_t)a.blk_k[b];
                 double s = 0.;
 #pragma unroll
-                for (int c = 0; c < 9; ++c) s += a.Su[su_index(b, c, r)] * tl[c];
+                for (int c = 0; c < 9; ++c) s += a.Su[su_at(a.sm, b, c, r)] * tl[c];
                 acc += s;
             }
         }

@@ -18,6 +18,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include "kernels.h"
 
 #include <cstdint>
 #include <vector>
@@ -47,7 +48,7 @@ struct LpGraphs {
 // features and CSR of the camera graph of S, from the block structure of the
 // solve (upper blocks bk/bl, row_off, col_off/col_blk); adj_off[k] =
 // row_off[k] + col_off[k]
-void lp_graph_from_S(int nc, int nblk, const double *Sd, const double *Su, const int *bk, const int *bl,
+void lp_graph_from_S(int nc, int nblk, const double *Sd, const double *Su, const SuMap &sm, const int *bk, const int *bl,
                      const int *row_off, const int *col_off, const int *col_blk, float *nrm, float *eb, float *feat,
                      int *adj_off, int *adj_nbr, float *adj_w, hipStream_t st);
 

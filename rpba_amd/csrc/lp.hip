@@ -63,7 +63,7 @@ __global__ __launch_bounds__(64) void k_lp_norms(int nc, const double *Sd, float
     nrm[k] = (float)sqrt(s);
 }
 
-__global__ __launch_bounds__(256) void k_lp_edges(int nblk, const double *Su, const int *bk, const int *bl,
+__global__ __launch_bounds__(256) void k_lp_edges(int nblk, const double *Su, SuMap sm, const int *bk, const int *bl,
                                                   const float *nrm, float *eb)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_lp_edges(int nblk, const double *Su, co
     double s = 0.;
     for (int i = 0; i < 9; ++i)
         for (int j = 0; j < 9; ++j) {
-            const double v = Su[su_index(b, i, j)];
+            const double v = Su[su_at(sm, b, i, j)];
             s += v * v;
         }
     const double den = sqrt((double)nrm[bk[b]] * (double)nrm[bl[b]]);
@@ -359,13 +359,13 @@ void dfree(Tp *&p)
 
 }  // namespace
 
-void lp_graph_from_S(int nc, int nblk, const double *Sd, const double *Su, const int *bk, const int *bl,
+void lp_graph_from_S(int nc, int nblk, const double *Sd, const double *Su, const SuMap &sm, const int *bk, const int *bl,
                      const int *row_off, const int *col_off, const int *col_blk, float *nrm, float *eb, float *feat,
                      int *adj_off, int *adj_nbr, float *adj_w, hipStream_t st)
 {
     if (nc <= 0) return;
     k_lp_norms<<<(nc + 63) / 64, 64, 0, st>>>(nc, Sd, nrm);
-    if (nblk > 0) k_lp_edges<<<(nblk + 255) / 256, 256, 0, st>>>(nblk, Su, bk, bl, nrm, eb);
+    if (nblk > 0) k_lp_edges<<<(nblk + 255) / 256, 256, 0, st>>>(nblk, Su, sm, bk, bl, nrm, eb);
     k_lp_csr<<<nc, 64, 0, st>>>(nc, Sd, bk, bl, row_off, col_off, col_blk, eb, feat, adj_off, adj_nbr, adj_w);
 }
 

@@ -75,7 +75,7 @@ struct rpba_ctx {
     DBuf<double> d_X, d_Xn, d_X0, d_Vinv, d_gp, d_Sd, d_b, d_Cinv, d_In_bak;
     DBuf<unsigned char> d_Wrec, d_Yrec;  // element type by precision
     DBuf<double> d_Su;
-    DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_partial, d_scal, d_Dd, d_ppart, d_yN, d_yT, d_dpart;
+    DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_partial, d_scal, d_Dd, d_ppart, d_rowpart, d_colpart, d_dpart;
     DBuf<double> d_t, d_cpart, d_Dk;  // matrix-free form
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
@@ -206,6 +206,7 @@ static int restore_points(rpba_ctx *c)
 static int build_structure(rpba_ctx *c)
 {
     int empty = -1;
+    c->ds.spmv_waves = 4 * spmv_grid(0);
     int rc = c->ds.build(c->st, &empty);
     if (rc) return rc;
     if (empty >= 0) return -1000 - empty;
@@ -221,10 +222,10 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_Yrec.alloc((size_t)kRec * esz * ntr);
     rc |= c->d_Vinv.alloc(9 * np);
     rc |= c->d_gp.alloc(3 * np);
-    rc |= c->d_Su.alloc(su_doubles(c->ds.nblk));
+    rc |= c->d_Su.alloc(81 * (size_t)std::max(c->ds.nblk, 1));
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
-    rc |= c->d_yN.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
-    rc |= c->d_yT.alloc(9 * (size_t)std::max(c->ds.nblk, 1));
+    rc |= c->d_rowpart.alloc(63 * (size_t)std::max(c->ds.nchunk, 1));
+    rc |= c->d_colpart.alloc(9 * (size_t)std::max(c->ds.nslab, 1));
     rc |= c->d_dpart.alloc((size_t)std::max(spmv_grid(c->ds.nblk), c->ds.nch + 1));
     if (c->implicit) {
         rc |= c->d_t.alloc(3 * std::max<size_t>(ntr, 1));
@@ -359,7 +360,12 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.col_pos = c->ds.cpos; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
     a.x = c->d_x.p; a.best = c->d_best.p; a.r = c->d_r.p; a.d = c->d_d.p; a.Ad = c->d_Ad.p;
     a.h = c->d_h.p; a.h1 = c->d_h1.p; a.part = c->d_ppart.p; a.S = c->d_pcg.p;
-    a.yN = c->d_yN.p; a.yT = c->d_yT.p; a.dpart = c->d_dpart.p; a.nblk = c->ds.nblk;
+    a.sm = c->ds.sumap();
+    a.nslab = c->ds.nslab; a.nchunk = c->ds.nchunk; a.nstrip = c->ds.nstrip; a.slab_chunk = c->ds.slab_chunk;
+    a.slab_bs = c->ds.slab_bs; a.slab_l = c->ds.slab_l; a.slab_mask = c->ds.slab_mask; a.strip_s0 = c->ds.strip_s0;
+    a.chunk_off = c->ds.chunk_off; a.chunk_strip = c->ds.chunk_strip; a.cs_off = c->ds.cs_off; a.cs_list = c->ds.cs_list;
+    a.rowpart = c->d_rowpart.p; a.colpart = c->d_colpart.p;
+    a.dpart = c->d_dpart.p; a.nblk = c->ds.nblk;
     a.ndpart = c->implicit ? c->ds.nch : spmv_grid(c->ds.nblk);
     a.dsum = c->implicit ? c->d_dpart.p + c->ds.nch : c->d_dpart.p;
     a.ndsum = c->implicit ? 1 : a.ndpart;
@@ -426,7 +432,7 @@ static int bundle_step(rpba_ctx *c)
     LinArgs la = lin_args(c);
     SchurArgs sa;
     sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
-    sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.rec32 = c->rec32;
+    sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.sm = c->ds.sumap(); sa.rec32 = c->rec32;
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
     // the off-diagonal blocks depend only on the records: without kernel
     // timing they run on a second stream, beside the diagonal terms, the
@@ -446,7 +452,7 @@ static int bundle_step(rpba_ctx *c)
         // exact solve of the same S and b on the matrix cores
         if (side) HIPCHK(hipStreamWaitEvent(c->st, c->ev_pairs, 0));
         c->timed("dense_assemble", [&] {
-            c->dense.assemble(c->d_Sd.p, c->d_Su.p, c->ds.bk, c->ds.bl, c->ds.nblk, c->d_b.p, c->st);
+            c->dense.assemble(c->d_Sd.p, c->d_Su.p, c->ds.sumap(), c->ds.bk, c->ds.bl, c->ds.nblk, c->d_b.p, c->st);
         });
         c->dense.factor_solve(c->d_best.p, c->st, [&](const char *nm, auto f) { c->timed(nm, f); });
         PcgScalars hs{};
@@ -578,7 +584,7 @@ static int lp_graph(rpba_ctx *c, LpGraphs &G)
         HIPCHK(hipStreamSynchronize(c->st));
     }
     if (rc) return RPBA_E_NOMEM;
-    lp_graph_from_S(nc, nb, c->d_Sd.p, c->d_Su.p, c->ds.bk, c->ds.bl, c->ds.roff, c->ds.coff, c->ds.cblk, c->lp_nrm.p,
+    lp_graph_from_S(nc, nb, c->d_Sd.p, c->d_Su.p, c->ds.sumap(), c->ds.bk, c->ds.bl, c->ds.roff, c->ds.coff, c->ds.cblk, c->lp_nrm.p,
                     c->lp_eb.p, c->lp_feat.p, c->lp_adj_off.p, c->lp_adj_nbr.p, c->lp_adj_w.p, c->st);
     G.ng = 1;
     G.nn = nc;
@@ -1242,8 +1248,8 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->lp_nrm.release(); c->lp_eb.release(); c->lp_feat.release(); c->lp_adj_w.release();
     c->lp_adj_off.release(); c->lp_adj_nbr.release(); c->lp_goff.release();
     c->d_Dd.release();
-    c->d_yN.release();
-    c->d_yT.release();
+    c->d_rowpart.release();
+    c->d_colpart.release();
     c->d_dpart.release();
     c->d_t.release();
     c->d_cpart.release();

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/rpba.h"
+#include "kernels.h"
 #include "structure.h"
 
 namespace rpba {
@@ -53,6 +54,27 @@ struct DevStruct {
     int *cpos = nullptr;
     size_t pab_cap = 0, poff_cap = 0, bk_cap = 0, bl_cap = 0, roff_cap = 0, coff_cap = 0, cblk_cap = 0;
     size_t cpos_cap = 0;
+    // strip slabs (the storage order of the upper blocks of S): strip s holds
+    // rows kSlab s .. kSlab s + kSlab - 1; its slab for column l holds the
+    // blocks (k, l) of those rows, k ascending.  sslot: block -> slot (its
+    // position in strip, column, row order), sperm the inverse; slot_slab:
+    // slot -> slab; slab_bs (nslab + 1): first slot of each slab; slab_l,
+    // slab_mask: the slab's column and the rows k - kSlab s present (bits);
+    // strip_s0 (nstrip + 1): first slab of each strip.  Chunks of at most
+    // slab_chunk consecutive slabs of one strip: chunk_off (nstrip + 1): first
+    // chunk of each strip; chunk_strip: the strip of each chunk.  cs_off
+    // (nc + 1) / cs_list: the slabs of column l in strip order.
+    int nslab = 0, nstrip = 0, nchunk = 0, slab_chunk = 8;
+    int spmv_waves = 0;  // wavefronts of the SpMV launch (sets the chunk length)
+    int *sslot = nullptr, *sperm = nullptr, *slot_slab = nullptr, *slab_bs = nullptr, *slab_l = nullptr;
+    int *slab_mask = nullptr, *strip_s0 = nullptr, *chunk_off = nullptr, *chunk_strip = nullptr;
+    int *cs_off = nullptr, *cs_list = nullptr;
+    unsigned *slab_key = nullptr, *slab_cnt = nullptr;
+    size_t sslot_cap = 0, sperm_cap = 0, slot_slab_cap = 0, slab_bs_cap = 0, slab_l_cap = 0, slab_mask_cap = 0;
+    size_t strip_s0_cap = 0, chunk_off_cap = 0, chunk_strip_cap = 0, cs_off_cap = 0, cs_list_cap = 0;
+    size_t slab_key_cap = 0, slab_cnt_cap = 0;
+    int strip_slabs(hipStream_t st);
+    SuMap sumap() const { return SuMap{sslot, slot_slab, slab_bs}; }
     // scratch
     unsigned *u32a = nullptr, *key_in = nullptr, *key_out = nullptr, *ukey = nullptr, *ucnt = nullptr, *nruns = nullptr;
     unsigned *key2 = nullptr, *key2s = nullptr;

@@ -15,6 +15,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "structure_dev.h"
+#include "kernels.h"
 
 namespace rpba {
 
@@ -302,7 +303,132 @@ int DevStruct::build(hipStream_t st, int *first_empty)
     }
     k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, (unsigned)nc, key2s, nblk, coff);
     if (nblk > 0) k_invert<<<nb_(nblk, 256), 256, 0, st>>>(nblk, cblk, cpos);
+    rc = strip_slabs(st);
+    if (rc) return rc;
     DCHK(hipStreamSynchronize(st));
+    return RPBA_OK;
+}
+
+__global__ void k_strip_keys(int nb, unsigned nc, const int *bk, const int *bl, unsigned *key, int *bid)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    key[b] = (unsigned)(bk[b] / kSlab) * nc + (unsigned)bl[b];
+    bid[b] = b;
+}
+
+// per slab: column, rows present, and the slab of each of its slots
+__global__ void k_slab_info(int ns, unsigned nc, const unsigned *skey, const int *bs, const int *sperm, const int *bk,
+                            int *sl, int *smask, int *slot_slab)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    sl[s] = (int)(skey[s] % nc);
+    int m = 0;
+    for (int q = bs[s]; q < bs[s + 1]; ++q) {
+        m |= 1 << (bk[sperm[q]] % kSlab);
+        slot_slab[q] = s;
+    }
+    smask[s] = m;
+}
+
+__global__ void k_chunk_counts(int nstrip, int T, const int *strip_s0, int *cnt)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nstrip) return;
+    cnt[s] = (s == nstrip) ? 0 : (strip_s0[s + 1] - strip_s0[s] + T - 1) / T;
+}
+
+__global__ void k_chunk_fill(int nstrip, const int *chunk_off, int *chunk_strip)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nstrip) return;
+    for (int c = chunk_off[s]; c < chunk_off[s + 1]; ++c) chunk_strip[c] = s;
+}
+
+// Strip slabs of the upper blocks (after the row-major block list exists).
+int DevStruct::strip_slabs(hipStream_t st)
+{
+    int rc = 0;
+    nstrip = (nc + kSlab - 1) / kSlab;
+    const size_t nb1 = (size_t)std::max(nblk, 1);
+    rc |= grow(sslot, sslot_cap, nb1);
+    rc |= grow(sperm, sperm_cap, nb1);
+    rc |= grow(slot_slab, slot_slab_cap, nb1);
+    rc |= grow(slab_key, slab_key_cap, nb1);
+    rc |= grow(slab_cnt, slab_cnt_cap, nb1 + 1);
+    rc |= grow(strip_s0, strip_s0_cap, (size_t)nstrip + 1);
+    rc |= grow(chunk_off, chunk_off_cap, (size_t)nstrip + 1);
+    rc |= grow(cs_off, cs_off_cap, (size_t)nc + 1);
+    rc |= grow(i32a, i32a_cap, (size_t)nstrip + 1);
+    if (rc) return RPBA_E_NOMEM;
+    nslab = 0;
+    if (nblk > 0) {
+        // stable sort of the row-major block ids by (strip, column): rows stay ascending inside a slab
+        k_strip_keys<<<nb_(nblk, 256), 256, 0, st>>>(nblk, (unsigned)nc, bk, bl, key2, bid);
+        const unsigned kb = bits_for((unsigned long long)nstrip * (unsigned long long)nc);
+        size_t bytes = 0;
+        DCHK(rocprim::radix_sort_pairs(nullptr, bytes, key2, key2s, bid, sperm, nblk, 0u, kb, st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::radix_sort_pairs(tmp, bytes, key2, key2s, bid, sperm, nblk, 0u, kb, st));
+        bytes = 0;
+        DCHK(rocprim::run_length_encode(nullptr, bytes, key2s, (unsigned)nblk, slab_key, slab_cnt, nruns, st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::run_length_encode(tmp, bytes, key2s, (unsigned)nblk, slab_key, slab_cnt, nruns, st));
+        DCHK(hipMemcpyAsync(hpin, nruns, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        DCHK(hipStreamSynchronize(st));
+        nslab = (int)((unsigned)hpin[0]);
+    }
+    const size_t ns1 = (size_t)std::max(nslab, 1);
+    rc |= grow(slab_bs, slab_bs_cap, ns1 + 1);
+    rc |= grow(slab_l, slab_l_cap, ns1);
+    rc |= grow(slab_mask, slab_mask_cap, ns1);
+    rc |= grow(cs_list, cs_list_cap, ns1);
+    if (rc) return RPBA_E_NOMEM;
+    k_set_last<<<1, 1, 0, st>>>((int *)slab_cnt, nslab, 0);
+    {
+        size_t bytes = 0;
+        DCHK(rocprim::exclusive_scan(nullptr, bytes, (const int *)slab_cnt, slab_bs, 0, (size_t)nslab + 1,
+                                     rocprim::plus<int>(), st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::exclusive_scan(tmp, bytes, (const int *)slab_cnt, slab_bs, 0, (size_t)nslab + 1,
+                                     rocprim::plus<int>(), st));
+    }
+    if (nslab > 0) {
+        k_slab_info<<<nb_(nslab, 256), 256, 0, st>>>(nslab, (unsigned)nc, slab_key, slab_bs, sperm, bk, slab_l,
+                                                     slab_mask, slot_slab);
+        k_invert<<<nb_(nblk, 256), 256, 0, st>>>(nblk, sperm, sslot);
+    }
+    k_lower_bound<<<nb_(nstrip + 1, 256), 256, 0, st>>>(nstrip, (unsigned)nc, slab_key, nslab, strip_s0);
+    // chunks: about three per SpMV wavefront, 4 to 64 slabs each
+    slab_chunk = 8;
+    if (spmv_waves > 0) slab_chunk = std::min(64, std::max(4, nslab / (3 * spmv_waves)));
+    const int T = slab_chunk;
+    k_chunk_counts<<<nb_(nstrip + 1, 256), 256, 0, st>>>(nstrip, T, strip_s0, i32a);
+    {
+        size_t bytes = 0;
+        DCHK(rocprim::exclusive_scan(nullptr, bytes, i32a, chunk_off, 0, (size_t)nstrip + 1, rocprim::plus<int>(), st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::exclusive_scan(tmp, bytes, i32a, chunk_off, 0, (size_t)nstrip + 1, rocprim::plus<int>(), st));
+    }
+    DCHK(hipMemcpyAsync(hpin, chunk_off + nstrip, sizeof(int), hipMemcpyDeviceToHost, st));
+    DCHK(hipStreamSynchronize(st));
+    nchunk = hpin[0];
+    if (grow(chunk_strip, chunk_strip_cap, (size_t)std::max(nchunk, 1))) return RPBA_E_NOMEM;
+    k_chunk_fill<<<nb_(nstrip, 256), 256, 0, st>>>(nstrip, chunk_off, chunk_strip);
+    // slabs of each column, strip order (stable sort of slab ids by column)
+    if (nslab > 0) {
+        if (grow(key_in, key_in_cap, ns1) || grow(key_out, key_out_cap, ns1) || grow(i32b, i32b_cap, ns1))
+            return RPBA_E_NOMEM;
+        k_iota<<<nb_(nslab, 256), 256, 0, st>>>(nslab, i32b);
+        DCHK(hipMemcpyAsync(key_in, slab_l, sizeof(int) * (size_t)nslab, hipMemcpyDeviceToDevice, st));
+        const unsigned cb = bits_for((unsigned long long)nc);
+        size_t bytes = 0;
+        DCHK(rocprim::radix_sort_pairs(nullptr, bytes, key_in, key_out, i32b, cs_list, nslab, 0u, cb, st));
+        if (scratch(bytes)) return RPBA_E_NOMEM;
+        DCHK(rocprim::radix_sort_pairs(tmp, bytes, key_in, key_out, i32b, cs_list, nslab, 0u, cb, st));
+    }
+    k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, 1u, key_out, nslab, cs_off);
     return RPBA_OK;
 }
 
@@ -647,6 +773,10 @@ void DevStruct::release()
     };
     fr(camoff, camoff_cap); fr(camq, camq_cap); fr(camrank, camrank_cap); fr(pgrp, pgrp_cap); fr(ch_start, ch_start_cap); fr(ch_off, ch_off_cap); fr(u32a, u32a_cap); fr(i32a, i32a_cap); fr(i32b, i32b_cap);
     fr(ppoff, ppoff_cap); fr(key_in, key_in_cap); fr(key_out, key_out_cap); fr(val_in, val_in_cap);
+    fr(sslot, sslot_cap); fr(sperm, sperm_cap); fr(slot_slab, slot_slab_cap); fr(slab_bs, slab_bs_cap);
+    fr(slab_l, slab_l_cap); fr(slab_mask, slab_mask_cap); fr(strip_s0, strip_s0_cap); fr(chunk_off, chunk_off_cap);
+    fr(chunk_strip, chunk_strip_cap); fr(cs_off, cs_off_cap); fr(cs_list, cs_list_cap);
+    fr(slab_key, slab_key_cap); fr(slab_cnt, slab_cnt_cap);
     fr(pab, pab_cap); fr(ukey, ukey_cap); fr(ucnt, ucnt_cap); fr(nruns, nruns_cap); fr(poff, poff_cap);
     fr(bk, bk_cap); fr(bl, bl_cap); fr(roff, roff_cap); fr(coff, coff_cap); fr(cblk, cblk_cap); fr(cpos, cpos_cap);
     fr(key2, key2_cap); fr(key2s, key2s_cap); fr(bid, bid_cap); fr(newi, newi_cap); fr(newoff, newoff_cap);
