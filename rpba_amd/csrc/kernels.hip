@@ -701,56 +701,119 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j)
+// The read-only arrays of the slab walk (and d) are separate restrict kernel
+// arguments: memory the kernel provably never writes, so their wave-uniform
+// loads go through the scalar unit and never wait on the vector loads of S.
+
+// A wavefront's walk over its slabs: chunk ci of its sequence (ci = w, w + W,
+// ...), chunk id c of strip st, slab q, up to e.  ok = 0 past the last slab;
+// q is then slab 0, so that the loads of that position stay in bounds.
+#define SPMV_CHUNK_START(CI, C, Q, E, ST, OK)                                         \
+    {                                                                                 \
+        const int ci_ = (CI);                                                         \
+        OK = ci_ < a.nchunk;                                                          \
+        C = 0; Q = 0; E = 0; ST = 0;                                                  \
+        if (OK) {                                                                     \
+            C = rev ? a.nchunk - 1 - ci_ : ci_;                                       \
+            ST = chunk_strip[C];                                                      \
+            Q = strip_s0[ST] + a.slab_chunk * (C - chunk_off[ST]);                    \
+            E = min(Q + a.slab_chunk, strip_s0[ST + 1]);                              \
+        }                                                                             \
+    }
+// position after (CI, C, Q, E, ST, OK) into (NCI, NC, NQ, NE, NST, NOK)
+#define SPMV_ADVANCE(CI, C, Q, E, ST, OK, NCI, NC, NQ, NE, NST, NOK)                  \
+    {                                                                                 \
+        if (OK && Q + 1 < E) {                                                        \
+            NCI = CI; NC = C; NQ = Q + 1; NE = E; NST = ST; NOK = 1;                  \
+        } else if (OK) {                                                              \
+            NCI = CI + W;                                                             \
+            SPMV_CHUNK_START(NCI, NC, NQ, NE, NST, NOK);                              \
+        } else {                                                                      \
+            NCI = CI; NC = C; NQ = Q; NE = E; NST = ST; NOK = 0;                      \
+        }                                                                             \
+    }
+// the nine column entries of the lane's row in slab Q and d_k[r] of that row;
+// every lane loads (absent rows read the slab's first block and are zeroed
+// when used), so the compiler counts the loads exactly
+#define SPMV_LOAD(Q, ST, V, DK)                                                       \
+    {                                                                                 \
+        const int base_ = slab_bs[Q], m_ = slab_bs[(Q) + 1] - base_;                   \
+        const int mask_ = slab_mask[Q];                                               \
+        const bool pr_ = g < kSlab && ((mask_ >> g) & 1);                             \
+        const int rank_ = pr_ ? __popc(mask_ & ((1 << g) - 1)) : 0;                   \
+        const double *src_ = a.Su + 81 * (size_t)base_ + 9 * rank_ + r;               \
+        _Pragma("unroll") for (int cc = 0; cc < 9; ++cc) V[cc] = src_[(size_t)(9 * m_) * cc]; \
+        const int k_ = min(kSlab * (ST) + min(g, kSlab - 1), a.nc - 1);               \
+        DK = d[9 * (size_t)k_ + r];                                                   \
+    }
+// one slab: products, column share, and the row partial at the chunk's end
+#define SPMV_SLAB(C, Q, E, V, DK)                                                     \
+    {                                                                                 \
+        const int mask_ = slab_mask[Q], l_ = slab_l[Q];                               \
+        const bool pr_ = g < kSlab && ((mask_ >> g) & 1);                             \
+        double cv_[9];                                                                \
+        _Pragma("unroll") for (int cc = 0; cc < 9; ++cc) cv_[cc] = pr_ ? V[cc] : 0.;  \
+        const double *dl_ = d + 9 * (size_t)l_;                                       \
+        double yv_ = 0.;                                                              \
+        _Pragma("unroll") for (int cc = 0; cc < 9; ++cc) yv_ += cv_[cc] * dl_[cc];    \
+        yrow += yv_;                                                                  \
+        du += DK * yv_;                                                               \
+        _Pragma("unroll") for (int jj = 0; jj < 9; ++jj) P[w][g][9 * jj + r] = cv_[jj] * DK; \
+        wave_lds_sync();                                                              \
+        double tv_ = 0.;                                                              \
+        _Pragma("unroll") for (int cc = 0; cc < 9; ++cc) tv_ += P[w][g][9 * r + cc];  \
+        T[w][g][r] = tv_;                                                             \
+        wave_lds_sync();                                                              \
+        if (lane < 9) {                                                               \
+            double cs_ = 0.;                                                          \
+            _Pragma("unroll") for (int h = 0; h < kSlab; ++h) cs_ += T[w][h][lane];   \
+            a.colpart[9 * (size_t)(Q) + lane] = cs_;                                  \
+        }                                                                             \
+        wave_lds_sync();                                                              \
+        if ((Q) + 1 == (E)) {                                                         \
+            if (lane < 9 * kSlab) a.rowpart[63 * (size_t)(C) + lane] = yrow;          \
+            yrow = 0.;                                                                \
+        }                                                                             \
+    }
+
+__device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j, const int *__restrict__ chunk_strip,
+                                                const int *__restrict__ strip_s0, const int *__restrict__ chunk_off,
+                                                const int *__restrict__ slab_bs, const int *__restrict__ slab_mask,
+                                                const int *__restrict__ slab_l, const double *__restrict__ d)
 {
     const bool rev = (j & 1) != 0;
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int t = threadIdx.x, lane = t & 63;
+    // wavefront index, uniform (held in a scalar register, so the walk over
+    // chunks and slabs is scalar)
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int g = lane / 9, r = lane - 9 * g;  // g = 7: lane 63, never present
-    const int gc = min(g, kSlab - 1);
     const int W = gridDim.x * kSpmvWaves;
     __shared__ double P[kSpmvWaves][kSlab + 1][81];  // S_kl[rr][jj] d_k[rr] at [g][9 jj + rr]
     __shared__ double T[kSpmvWaves][kSlab + 1][9];   // block g's column share
-    double du = 0.;  // this lane's share of sum over upper blocks of d_k.S_kl d_l
-    for (int ci = blockIdx.x * kSpmvWaves + w; ci < a.nchunk; ci += W) {
-        const int c = rev ? a.nchunk - 1 - ci : ci;
-        const int st = a.chunk_strip[c];
-        const int f = a.strip_s0[st] + a.slab_chunk * (c - a.chunk_off[st]);
-        const int e = min(f + a.slab_chunk, a.strip_s0[st + 1]);
-        const int k = min(kSlab * st + gc, a.nc - 1);
-        const double dk = a.d[9 * (size_t)k + r];
-        double yrow = 0.;
-        for (int q = f; q < e; ++q) {
-            const int base = a.slab_bs[q], m = a.slab_bs[q + 1] - base;
-            const int mask = a.slab_mask[q], l = a.slab_l[q];
-            const bool present = g < kSlab && ((mask >> g) & 1);
-            const int rank = __popc(mask & ((1 << g) - 1));
-            const double *src = a.Su + 81 * (size_t)base + 9 * rank + r;
-            double cv[9];
-#pragma unroll
-            for (int cc = 0; cc < 9; ++cc) cv[cc] = present ? src[(size_t)(9 * m) * cc] : 0.;
-            const double *dl = a.d + 9 * (size_t)l;
-            double yv = 0.;
-#pragma unroll
-            for (int cc = 0; cc < 9; ++cc) yv += cv[cc] * dl[cc];
-            yrow += yv;
-            du += dk * yv;
-#pragma unroll
-            for (int jj = 0; jj < 9; ++jj) P[w][g][9 * jj + r] = cv[jj] * dk;
-            wave_lds_sync();
-            double tv = 0.;
-#pragma unroll
-            for (int cc = 0; cc < 9; ++cc) tv += P[w][g][9 * r + cc];
-            T[w][g][r] = tv;
-            wave_lds_sync();
-            if (lane < 9) {
-                double cs = 0.;
-#pragma unroll
-                for (int h = 0; h < kSlab; ++h) cs += T[w][h][lane];
-                a.colpart[9 * (size_t)q + lane] = cs;
-            }
-            wave_lds_sync();  // the next slab overwrites this wavefront's LDS
+    double du = 0.;    // this lane's share of sum over upper blocks of d_k.S_kl d_l
+    double yrow = 0.;  // row k0 + g of the current chunk: sum over its slabs of (S_kl d_l)_r
+    // two slabs in flight: register sets A and B alternate
+    int ci0, c0, q0, e0, st0, ok0, ci1, c1, q1, e1, st1, ok1;
+    SPMV_CHUNK_START(blockIdx.x * kSpmvWaves + w, c0, q0, e0, st0, ok0);
+    ci0 = blockIdx.x * kSpmvWaves + w;
+    SPMV_ADVANCE(ci0, c0, q0, e0, st0, ok0, ci1, c1, q1, e1, st1, ok1);
+    double vA[9], vB[9], dkA, dkB;
+    if (ok0) {
+        SPMV_LOAD(q0, st0, vA, dkA);
+        SPMV_LOAD(q1, st1, vB, dkB);
+        while (true) {
+            int ci2, c2, q2, e2, st2, ok2, ci3, c3, q3, e3, st3, ok3;
+            SPMV_ADVANCE(ci1, c1, q1, e1, st1, ok1, ci2, c2, q2, e2, st2, ok2);
+            SPMV_SLAB(c0, q0, e0, vA, dkA);
+            SPMV_LOAD(q2, st2, vA, dkA);
+            if (!ok1) break;
+            SPMV_ADVANCE(ci2, c2, q2, e2, st2, ok2, ci3, c3, q3, e3, st3, ok3);
+            SPMV_SLAB(c1, q1, e1, vB, dkB);
+            SPMV_LOAD(q3, st3, vB, dkB);
+            if (!ok2) break;
+            ci0 = ci2; c0 = c2; q0 = q2; e0 = e2; st0 = st2; ok0 = ok2;
+            ci1 = ci3; c1 = c3; q1 = q3; e1 = e3; st1 = st3; ok1 = ok3;
         }
-        if (lane < 9 * kSlab) a.rowpart[63 * (size_t)c + lane] = yrow;
     }
     // diagonal blocks
     double dd = 0.;
@@ -771,10 +834,18 @@ __device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j)
     if (t == 0) a.dpart[blockIdx.x] = 2. * su + sd;
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spmv_slabs(PcgArgs a)
+#undef SPMV_CHUNK_START
+#undef SPMV_ADVANCE
+#undef SPMV_LOAD
+#undef SPMV_SLAB
+
+__global__ __launch_bounds__(256) void k_spmv_slabs(PcgArgs a, const int *__restrict__ chunk_strip,
+                                                    const int *__restrict__ strip_s0, const int *__restrict__ chunk_off,
+                                                    const int *__restrict__ slab_bs, const int *__restrict__ slab_mask,
+                                                    const int *__restrict__ slab_l, const double *__restrict__ d)
 {
     if (a.S[a.j & 1].done) return;
-    spmv_slabs_body(a, a.j);
+    spmv_slabs_body(a, a.j, chunk_strip, strip_s0, chunk_off, slab_bs, slab_mask, slab_l, d);
 }
 
 // sum of n values in a fixed order; every thread of the block gets it
@@ -1653,7 +1724,8 @@ int spmv_grid(int nblk)
 void launch_spmv(const PcgArgs &a, hipStream_t st)
 {
     if (!a.implicit) {
-        k_spmv_slabs<<<a.ndpart, 256, 0, st>>>(a);
+        k_spmv_slabs<<<a.ndpart, 256, 0, st>>>(a, a.chunk_strip, a.strip_s0, a.chunk_off, a.slab_bs, a.slab_mask,
+                                               a.slab_l, a.d);
         return;
     }
     if (a.npgrp <= 0 || a.nch <= 0) return;
