@@ -21,12 +21,13 @@ struct SuMap {
     const int *slot_slab;  // slot -> slab
     const int *slab_bs;    // slab -> first slot (nslab + 1)
 };
-__device__ __forceinline__ size_t su_at(const SuMap &m, int b, int i, int j)
+__device__ __forceinline__ size_t su_at_slot(const SuMap &m, int q, int i, int j)
 {
-    const int q = m.slot[b], s = m.slot_slab[q];
+    const int s = m.slot_slab[q];
     const int base = m.slab_bs[s], n = m.slab_bs[s + 1] - base;
     return 81 * (size_t)base + (size_t)(9 * n) * j + 9 * (size_t)(q - base) + i;
 }
+__device__ __forceinline__ size_t su_at(const SuMap &m, int b, int i, int j) { return su_at_slot(m, m.slot[b], i, j); }
 
 // device-resident camera state, all double, indexed by image
 struct CamPtrs {
@@ -69,6 +70,7 @@ struct SchurArgs {
     const void *Yrec, *Wrec;
     double *Su;
     SuMap sm;
+    const int *sperm;  // slot -> block
     int rec32;
 };
 

@@ -529,12 +529,16 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 template <typename R>
 __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
 {
-    // workgroups go round-robin over the 8 XCDs; give each XCD a contiguous
-    // range of blocks so the blocks (k, l), (k, l+1), ... of one camera row
-    // share that XCD's L2 for the entries of camera k
+    // Blocks are taken in slot order (strip, column, row; see SuMap): the up to
+    // seven blocks (k0 .. k0 + 6, l) of a slab, which share its cache lines,
+    // are computed back to back, and along a strip the Y records of its seven
+    // cameras stay in L2 while the W records of each column camera l are read
+    // for all seven rows at once.  Workgroups go round-robin over the 8 XCDs;
+    // each XCD gets a contiguous range of slots.
     const int per = (a.nblk + 7) / 8;
-    const int bid = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (bid >= a.nblk) return;
+    const int slot = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (slot >= a.nblk) return;
+    const int bid = a.sperm[slot];
     const int lane = threadIdx.x;
     const int i16 = lane & 15, q = lane >> 4;
     const R *Yr = (const R *)a.Yrec, *Wr = (const R *)a.Wrec;
@@ -570,7 +574,7 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = q + 4 * v;
-            if (i < 9) a.Su[su_at(a.sm, bid, i, i16)] = -acc[v];
+            if (i < 9) a.Su[su_at_slot(a.sm, slot, i, i16)] = -acc[v];
         }
     }
 }
