@@ -60,16 +60,19 @@ __global__ __launch_bounds__(128) void k_dense_diag(DenseArgs a, const int *cams
 // upper block S_kl (k < l) of camera rows k, columns l: lower element
 // (row 9 l + j, column 9 k + i) = S_kl[i][j]
 __global__ __launch_bounds__(128) void k_dense_off(DenseArgs a, const int *camsys, const double *Su, SuMap sm,
-                                                   const int *bk, const int *bl)
+                                                   const int *bk, const int *bl, int nblk)
 {
-    const int e = blockIdx.x, t = threadIdx.x;
+    // workgroups stride over the blocks (a dispatch counts its work-items in 32 bits)
+    const int t = threadIdx.x;
     if (t >= 81) return;
-    const int k = bk[e], l = bl[e];
-    const int m = camsys[k];
-    const int ld = a.ld[m];
-    const long long kl = 9LL * (k - a.cam0[m]), ll = 9LL * (l - a.cam0[m]);
     const int i = t / 9, j = t - 9 * i;
-    a.A[a.base[m] + (kl + i) * ld + ll + j] = Su[su_at(sm, e, i, j)];
+    for (int e = blockIdx.x; e < nblk; e += gridDim.x) {
+        const int k = bk[e], l = bl[e];
+        const int m = camsys[k];
+        const int ld = a.ld[m];
+        const long long kl = 9LL * (k - a.cam0[m]), ll = 9LL * (l - a.cam0[m]);
+        a.A[a.base[m] + (kl + i) * ld + ll + j] = Su[su_at(sm, e, i, j)];
+    }
 }
 
 // diagonal tile (k, k): unblocked right-looking Cholesky in LDS (column-major,
@@ -398,7 +401,7 @@ void DenseSolver::assemble(const double *Sd, const double *Su, const SuMap &sm, 
     k_dense_fill<<<(int)std::max<long long>(g, 1), 256, 0, st>>>(A_, total_);
     if (nsys_) k_dense_pad<<<nsys_, 64, 0, st>>>(a);
     if (ncam_total_) k_dense_diag<<<ncam_total_, 128, 0, st>>>(a, camsys_, Sd, b);
-    if (nblk) k_dense_off<<<nblk, 128, 0, st>>>(a, camsys_, Su, sm, blk_k, blk_l);
+    if (nblk) k_dense_off<<<std::min(nblk, 1 << 20), 128, 0, st>>>(a, camsys_, Su, sm, blk_k, blk_l, nblk);
 }
 
 void DenseSolver::launch_potrf(const DenseArgs &a, int k, hipStream_t st)

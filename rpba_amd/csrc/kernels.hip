@@ -527,17 +527,8 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 // ---------------------------------------------------------------------------
 
 template <typename R>
-__global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
+__device__ __forceinline__ void schur_block(const SchurArgs &a, int slot)
 {
-    // Blocks are taken in slot order (strip, column, row; see SuMap): the up to
-    // seven blocks (k0 .. k0 + 6, l) of a slab, which share its cache lines,
-    // are computed back to back, and along a strip the Y records of its seven
-    // cameras stay in L2 while the W records of each column camera l are read
-    // for all seven rows at once.  Workgroups go round-robin over the 8 XCDs;
-    // each XCD gets a contiguous range of slots.
-    const int per = (a.nblk + 7) / 8;
-    const int slot = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (slot >= a.nblk) return;
     const int bid = a.sperm[slot];
     const int lane = threadIdx.x;
     const int i16 = lane & 15, q = lane >> 4;
@@ -578,6 +569,28 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
         }
     }
 }
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
+{
+    // Blocks are taken in slot order (strip, column, row; see SuMap): the up to
+    // seven blocks (k0 .. k0 + 6, l) of a slab, which share its cache lines,
+    // are computed back to back, and along a strip the Y records of its seven
+    // cameras stay in L2 while the W records of each column camera l are read
+    // for all seven rows at once.  Workgroups go round-robin over the 8 XCDs;
+    // each XCD gets a contiguous range of slots, which its workgroups stride
+    // through (the grid is bounded: a dispatch counts its work-items in 32
+    // bits, and one 64-lane workgroup per block overflows that from 67M
+    // blocks on).
+    const int per = (a.nblk + 7) / 8;
+    const int gx = (int)(gridDim.x / 8);
+    for (int it = (int)(blockIdx.x / 8); it < per; it += gx) {
+        const int slot = (int)(blockIdx.x % 8) * per + it;
+        if (slot >= a.nblk) return;
+        schur_block<R>(a, slot);
+    }
+}
+
 
 // ---------------------------------------------------------------------------
 // PCG (SparsePCG rbacore.cpp:1807-1914) with device-held scalars
@@ -1673,7 +1686,8 @@ void launch_lin_points(const LinArgs &a, hipStream_t st)
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
 {
     if (s.nblk <= 0) return;
-    const unsigned g = 8 * ((s.nblk + 7) / 8);
+    // at most 8 x 65536 workgroups (33.5M work-items), striding over the blocks
+    const unsigned g = 8 * (unsigned)std::min((s.nblk + 7) / 8, 65536);
     if (s.rec32)
         k_schur_pairs<float><<<g, 64, 0, st>>>(s);
     else
