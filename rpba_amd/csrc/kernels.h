@@ -71,6 +71,9 @@ struct SchurArgs {
     double *Su;
     SuMap sm;
     const int *sperm;  // slot -> block
+    int nslab;
+    const int *slab_bs;  // strip slabs (see SuMap): first slot of each slab (nslab + 1)
+    long long npairs;
     int rec32;
 };
 

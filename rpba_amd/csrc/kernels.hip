@@ -592,6 +592,103 @@ __global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
 }
 
 
+// orders LDS accesses of one wavefront (lanes exchange values through LDS)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Off-diagonal blocks, one wavefront per slab (the form used by the solver):
+// lane 9 g + i accumulates row i of the slab's block g,
+// S_kl[i][j] = -sum over the block's pairs p (in order) and c = 0..2 of
+// Y_a[i][c] W_b[j][c], as FP64 fused multiply-adds in that order (the K order
+// of the matrix-core form above).  Y_a's row i is the lane's own 24 bytes of
+// the record; W_b goes through the wavefront's LDS (each lane of the block
+// stores three of its 27 values).  The next pair's index and records are in
+// flight while the current pair is added; every load is issued (indices
+// clamped into the arrays) so the compiler counts them exactly.  Stores:
+// for each column j the slab's rows are one run of 9 m doubles.  Slabs go in
+// contiguous ranges per XCD, four wavefronts per workgroup, grid-stride.
+template <typename R>
+__global__ __launch_bounds__(256) void k_schur_slabs(SchurArgs a)
+{
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int g = lane / 9, i = lane - 9 * g;  // g = 7: lane 63, idle
+    __shared__ double Wsh[4][kSlab + 1][27];
+    const R *Yr = (const R *)a.Yrec, *Wr = (const R *)a.Wrec;
+    const int per = (a.nslab + 7) / 8;
+    const int x = (int)(blockIdx.x % 8);
+    const int stride = (int)(gridDim.x / 8) * 4;
+    for (int it = (int)(blockIdx.x / 8) * 4 + w; it < per; it += stride) {
+        const int s = x * per + it;
+        if (s >= a.nslab) break;
+        const int base = a.slab_bs[s], m = a.slab_bs[s + 1] - base;
+        int p0 = 0, np = 0;
+        if (g < m) {
+            const int b = a.sperm[base + g];
+            p0 = a.poff[b];
+            np = a.poff[b + 1] - p0;
+        }
+        // pair p's index (clamped into the block's list, or pair 0 for idle lanes)
+        auto pidx = [&](int p) -> long long {
+            return np > 0 ? (long long)p0 + min(p, np - 1) : 0;
+        };
+        double acc[9];
+#pragma unroll
+        for (int j = 0; j < 9; ++j) acc[j] = 0.;
+        unsigned long long ab = a.pab[pidx(0)];
+        unsigned long long abn = a.pab[pidx(1)];
+        R y[3], wv[3], yn[3], wn[3];
+        {
+            const R *yr = Yr + (size_t)(ab >> 32) * kRec + 3 * i;
+            const R *wr = Wr + (size_t)(ab & 0xffffffffull) * kRec + 3 * i;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                y[c] = yr[c];
+                wv[c] = wr[c];
+            }
+        }
+        for (int p = 0;; ++p) {
+            const bool act = g < m && p < np;
+            if (!__any(act)) break;
+            // in flight: pair p + 2's index, pair p + 1's records
+            const unsigned long long ab2 = a.pab[pidx(p + 2)];
+            {
+                const R *yr = Yr + (size_t)(abn >> 32) * kRec + 3 * i;
+                const R *wr = Wr + (size_t)(abn & 0xffffffffull) * kRec + 3 * i;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    yn[c] = yr[c];
+                    wn[c] = wr[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Wsh[w][g][3 * i + c] = (double)wv[c];
+            wave_lds_sync();
+            if (act) {
+#pragma unroll
+                for (int j = 0; j < 9; ++j)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) acc[j] = fma((double)y[c], Wsh[w][g][3 * j + c], acc[j]);
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                y[c] = yn[c];
+                wv[c] = wn[c];
+            }
+            abn = ab2;
+        }
+        if (g < m) {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) a.Su[81 * (size_t)base + (size_t)(9 * m) * j + 9 * g + i] = -acc[j];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // PCG (SparsePCG rbacore.cpp:1807-1914) with device-held scalars
 // ---------------------------------------------------------------------------
@@ -710,13 +807,6 @@ __global__ __launch_bounds__(1024) void k_pcg_init(PcgArgs a)
 // strip's chunks and the column shares of the slabs of column k.
 constexpr int kSpmvWaves = 4;  // wavefronts per workgroup (256 threads)
 
-// orders LDS accesses of one wavefront (lanes exchange values through LDS)
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // The read-only arrays of the slab walk (and d) are separate restrict kernel
 // arguments: memory the kernel provably never writes, so their wave-uniform
@@ -1686,6 +1776,16 @@ void launch_lin_points(const LinArgs &a, hipStream_t st)
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
 {
     if (s.nblk <= 0) return;
+    if (s.nslab > 0 && s.slab_bs) {
+        // four wavefronts (slabs) per workgroup, at most 8 x 16384 workgroups
+        const int per = (s.nslab + 7) / 8;
+        const unsigned g = 8 * (unsigned)std::min((per + 3) / 4, 16384);
+        if (s.rec32)
+            k_schur_slabs<float><<<g, 256, 0, st>>>(s);
+        else
+            k_schur_slabs<double><<<g, 256, 0, st>>>(s);
+        return;
+    }
     // at most 8 x 65536 workgroups (33.5M work-items), striding over the blocks
     const unsigned g = 8 * (unsigned)std::min((s.nblk + 7) / 8, 65536);
     if (s.rec32)

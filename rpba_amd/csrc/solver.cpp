@@ -432,7 +432,7 @@ static int bundle_step(rpba_ctx *c)
     LinArgs la = lin_args(c);
     SchurArgs sa;
     sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
-    sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.sm = c->ds.sumap(); sa.sperm = c->ds.sperm; sa.rec32 = c->rec32;
+    sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.sm = c->ds.sumap(); sa.sperm = c->ds.sperm; sa.nslab = c->ds.nslab; sa.slab_bs = c->ds.slab_bs; sa.npairs = c->ds.npairs; sa.rec32 = c->rec32;
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
     // the off-diagonal blocks depend only on the records: without kernel
     // timing they run on a second stream, beside the diagonal terms, the
