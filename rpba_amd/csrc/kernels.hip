@@ -366,8 +366,13 @@ __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
 // cut into chunks of at most 64 entries of one image; one wavefront per chunk
 // forms D = sum Y W^T (9x9) and e = sum Y g_p (9) as one matrix product on
 // the matrix cores, [Y_1 .. Y_n] (9 x 3n) times [W_1 .. W_n | g_1 .. g_n]^T
-// (3n x 10: W^T in columns 0..8, g_p in column 9), with the lane layout of
-// k_schur_pairs.  k_lin_cameras adds the chunks of an image in order.
+// (3n x 10: W^T in columns 0..8, g_p in column 9), as v_mfma_f64_16x16x4f64
+// steps over K = 3n (rows and columns 10..15 are zero padding).  Operand
+// layout of that instruction: A[i][k] in lane i + 16 k, B[k][j] in lane
+// j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4; K index kk = 3 e + c:
+// entry e of the chunk, component c of the point.  FP32 records are widened
+// to FP64 (products of two floats are exact).  k_lin_cameras adds the chunks
+// of an image in order.
 // ---------------------------------------------------------------------------
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -516,82 +521,6 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Schur off-diagonal blocks on the matrix cores, atomic-free: one wavefront per
-// upper block (k, l).  S_kl = -sum_pairs Y_a W_b^T is the product of the
-// 9 x 3n matrix [Y_a1 .. Y_an] and the 3n x 9 matrix [W_b1 .. W_bn]^T, run as
-// v_mfma_f64_16x16x4f64 steps over K = 3n (rows and columns 9..15 are zero
-// padding).  Operand layout of that instruction: A[i][k] in lane i + 16 k,
-// B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4.
-// K index kk = 3 p + c: pair p of the block, component c of the point.
-// FP32 records are widened to FP64 (products of two floats are exact).
-// ---------------------------------------------------------------------------
-
-template <typename R>
-__device__ __forceinline__ void schur_block(const SchurArgs &a, int slot)
-{
-    const int bid = a.sperm[slot];
-    const int lane = threadIdx.x;
-    const int i16 = lane & 15, q = lane >> 4;
-    const R *Yr = (const R *)a.Yrec, *Wr = (const R *)a.Wrec;
-    const int pb = a.poff[bid], pe = a.poff[bid + 1];
-    __shared__ unsigned long long sab[64];
-    f64x4 acc = {0., 0., 0., 0.};
-    for (int c0 = pb; c0 < pe; c0 += 64) {
-        const int nc = min(64, pe - c0);
-        if (lane < nc) sab[lane] = a.pab[c0 + lane];
-        __syncthreads();
-        const int kend = 3 * nc;
-        for (int k0 = 0; k0 < kend; k0 += 4 * kSchurU) {
-            double av[kSchurU], bv[kSchurU];
-#pragma unroll
-            for (int u = 0; u < kSchurU; ++u) {
-                const int kk = k0 + 4 * u + q;
-                double x = 0., y = 0.;
-                if (kk < kend && i16 < 9) {
-                    const unsigned long long ab = sab[kk / 3];
-                    const int cc = kk - 3 * (kk / 3);
-                    x = (double)Yr[(size_t)(ab >> 32) * kRec + 3 * i16 + cc];
-                    y = (double)Wr[(size_t)(ab & 0xffffffffull) * kRec + 3 * i16 + cc];
-                }
-                av[u] = x;
-                bv[u] = y;
-            }
-#pragma unroll
-            for (int u = 0; u < kSchurU; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    if (i16 < 9) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = q + 4 * v;
-            if (i < 9) a.Su[su_at_slot(a.sm, slot, i, i16)] = -acc[v];
-        }
-    }
-}
-
-template <typename R>
-__global__ __launch_bounds__(64) void k_schur_pairs(SchurArgs a)
-{
-    // Blocks are taken in slot order (strip, column, row; see SuMap): the up to
-    // seven blocks (k0 .. k0 + 6, l) of a slab, which share its cache lines,
-    // are computed back to back, and along a strip the Y records of its seven
-    // cameras stay in L2 while the W records of each column camera l are read
-    // for all seven rows at once.  Workgroups go round-robin over the 8 XCDs;
-    // each XCD gets a contiguous range of slots, which its workgroups stride
-    // through (the grid is bounded: a dispatch counts its work-items in 32
-    // bits, and one 64-lane workgroup per block overflows that from 67M
-    // blocks on).
-    const int per = (a.nblk + 7) / 8;
-    const int gx = (int)(gridDim.x / 8);
-    for (int it = (int)(blockIdx.x / 8); it < per; it += gx) {
-        const int slot = (int)(blockIdx.x % 8) * per + it;
-        if (slot >= a.nblk) return;
-        schur_block<R>(a, slot);
-    }
-}
-
-
 // orders LDS accesses of one wavefront (lanes exchange values through LDS)
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -600,11 +529,12 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Off-diagonal blocks, one wavefront per slab (the form used by the solver):
-// lane 9 g + i accumulates row i of the slab's block g,
-// S_kl[i][j] = -sum over the block's pairs p (in order) and c = 0..2 of
-// Y_a[i][c] W_b[j][c], as FP64 fused multiply-adds in that order (the K order
-// of the matrix-core form above).  Y_a's row i is the lane's own 24 bytes of
+// Off-diagonal blocks, atomic-free, one wavefront per strip slab: lane 9 g + i
+// accumulates row i of the slab's block g, S_kl[i][j] = -sum over the block's
+// pairs p (in order) and c = 0..2 of Y_a[i][c] W_b[j][c], as FP64 fused
+// multiply-adds in that order (the K order of a matrix-core product of
+// [Y_a1 .. Y_an] and [W_b1 .. W_bn]^T, which measured slower here: a block has
+// 2 to 30 pairs, far too few to fill a 16 x 16 x 4 tile step).  Y_a's row i is the lane's own 24 bytes of
 // the record; W_b goes through the wavefront's LDS (each lane of the block
 // stores three of its 27 values).  The next pair's index and records are in
 // flight while the current pair is added; every load is issued (indices
@@ -905,22 +835,26 @@ __device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j, const i
     ci0 = blockIdx.x * kSpmvWaves + w;
     SPMV_ADVANCE(ci0, c0, q0, e0, st0, ok0, ci1, c1, q1, e1, st1, ok1);
     double vA[9], vB[9], dkA, dkB;
+    // One iteration: slab p0 from set A, slab p1 (if any) from set B, and
+    // both sets reloaded unconditionally, so that every path back to the loop
+    // head has issued A's loads before B's and the compiler waits only for
+    // the set in use.
     if (ok0) {
         SPMV_LOAD(q0, st0, vA, dkA);
         SPMV_LOAD(q1, st1, vB, dkB);
-        while (true) {
+        do {
             int ci2, c2, q2, e2, st2, ok2, ci3, c3, q3, e3, st3, ok3;
             SPMV_ADVANCE(ci1, c1, q1, e1, st1, ok1, ci2, c2, q2, e2, st2, ok2);
+            SPMV_ADVANCE(ci2, c2, q2, e2, st2, ok2, ci3, c3, q3, e3, st3, ok3);
             SPMV_SLAB(c0, q0, e0, vA, dkA);
             SPMV_LOAD(q2, st2, vA, dkA);
-            if (!ok1) break;
-            SPMV_ADVANCE(ci2, c2, q2, e2, st2, ok2, ci3, c3, q3, e3, st3, ok3);
-            SPMV_SLAB(c1, q1, e1, vB, dkB);
+            if (ok1) SPMV_SLAB(c1, q1, e1, vB, dkB);
             SPMV_LOAD(q3, st3, vB, dkB);
-            if (!ok2) break;
-            ci0 = ci2; c0 = c2; q0 = q2; e0 = e2; st0 = st2; ok0 = ok2;
-            ci1 = ci3; c1 = c3; q1 = q3; e1 = e3; st1 = st3; ok1 = ok3;
-        }
+            ci0 = ci2; c0 = c2; q0 = q2; e0 = e2; st0 = st2;
+            ci1 = ci3; c1 = c3; q1 = q3; e1 = e3; st1 = st3;
+            ok0 = ok1 ? ok2 : 0;
+            ok1 = ok3;
+        } while (ok0);
     }
     // diagonal blocks
     double dd = 0.;
@@ -1775,23 +1709,14 @@ void launch_lin_points(const LinArgs &a, hipStream_t st)
 }
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
 {
-    if (s.nblk <= 0) return;
-    if (s.nslab > 0 && s.slab_bs) {
-        // four wavefronts (slabs) per workgroup, at most 8 x 16384 workgroups
-        const int per = (s.nslab + 7) / 8;
-        const unsigned g = 8 * (unsigned)std::min((per + 3) / 4, 16384);
-        if (s.rec32)
-            k_schur_slabs<float><<<g, 256, 0, st>>>(s);
-        else
-            k_schur_slabs<double><<<g, 256, 0, st>>>(s);
-        return;
-    }
-    // at most 8 x 65536 workgroups (33.5M work-items), striding over the blocks
-    const unsigned g = 8 * (unsigned)std::min((s.nblk + 7) / 8, 65536);
+    if (s.nblk <= 0 || s.nslab <= 0) return;
+    // four wavefronts (slabs) per workgroup, at most 8 x 16384 workgroups
+    const int per = (s.nslab + 7) / 8;
+    const unsigned g = 8 * (unsigned)std::min((per + 3) / 4, 16384);
     if (s.rec32)
-        k_schur_pairs<float><<<g, 64, 0, st>>>(s);
+        k_schur_slabs<float><<<g, 256, 0, st>>>(s);
     else
-        k_schur_pairs<double><<<g, 64, 0, st>>>(s);
+        k_schur_slabs<double><<<g, 256, 0, st>>>(s);
 }
 void launch_schur_diag(const LinArgs &a, hipStream_t st)
 {
