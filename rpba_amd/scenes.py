@@ -86,6 +86,23 @@ def _look_at_origin(centers):
     return np.stack([x, y, z], axis=1)  # rows = camera axes in world coordinates
 
 
+def _window_tracks(rng, n_pts, n_cams, lengths, window):
+    """Distinct cameras per point from a window of `window` consecutive
+    cameras (on the circle) around a random first camera, sorted: banded
+    covisibility, so the reduced camera system is sparse."""
+    cams = np.empty(int(lengths.sum()), dtype=np.int32)
+    off = np.zeros(n_pts + 1, dtype=np.int64)
+    np.cumsum(lengths, out=off[1:])
+    start = rng.integers(0, n_cams, n_pts)
+    for t in np.unique(lengths):
+        idx = np.nonzero(lengths == t)[0]
+        pick = np.argsort(rng.random((idx.size, window)), axis=1)[:, :t]
+        rows = np.sort((start[idx][:, None] + pick) % n_cams, axis=1)
+        pos = off[idx][:, None] + np.arange(t)[None, :]
+        cams[pos.ravel()] = rows.ravel()
+    return cams, off
+
+
 def _tracks(rng, n_pts, n_cams, lengths):
     """Distinct cameras per point, uniform without replacement, sorted."""
     cams = np.empty(int(lengths.sum()), dtype=np.int32)
@@ -123,7 +140,7 @@ def _tracks(rng, n_pts, n_cams, lengths):
 def generate(name=None, *, seed=1, n_cams=50, n_pts=5000, track="geometric", track_mean=9.0,
              outlier_frac=0.02, noise_px=0.5, outlier_px=50.0, f0=500.0, f_spread=0.05,
              k1_std=1e-3, k2_std=1e-5, rot_noise=0.01, t_noise=0.05, x_noise=0.05,
-             radius=10.0, ball=3.0):
+             radius=10.0, ball=3.0, cam_window=None):
     """Return a scene dict (numpy arrays) in the reference's data model plus the
     ground truth.  ``name`` selects one of CONFIGS."""
     if name is not None:
@@ -150,7 +167,11 @@ def generate(name=None, *, seed=1, n_cams=50, n_pts=5000, track="geometric", tra
         p = 1.0 / (track_mean - 1.0)
         lengths = 2 + rng.geometric(p, n_pts) - 1
     lengths = np.minimum(lengths, n_cams).astype(np.int64)
-    obs_cam, off = _tracks(rng, n_pts, n_cams, lengths)
+    if cam_window:
+        lengths = np.minimum(lengths, cam_window)
+        obs_cam, off = _window_tracks(rng, n_pts, n_cams, lengths, int(cam_window))
+    else:
+        obs_cam, off = _tracks(rng, n_pts, n_cams, lengths)
     obs_pt = np.repeat(np.arange(n_pts, dtype=np.int32), lengths)
     n_obs = obs_cam.size
 

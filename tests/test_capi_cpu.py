@@ -94,3 +94,17 @@ def test_unknown_form_selectors_are_rejected_without_gpu(tiny_scene):
     for kw in ({"precision": 2}, {"schur": 2}, {"schur": -1}):
         rc, _, _, _ = api.calc_plsa(tiny_scene, capi.make_flags(**kw))
         assert rc == capi.RPBA_E_INVALID, kw
+
+
+def test_window_tracks_are_banded_and_distinct():
+    """The camera-window generator (sparse, banded covisibility) gives every
+    point distinct cameras inside one window of consecutive cameras."""
+    sc = scenes.generate(None, seed=51, n_cams=60, n_pts=3000, track="uniform2_8", outlier_frac=0.02, cam_window=10)
+    cam, pt = sc["obs_cam"], sc["obs_pt"]
+    off = np.r_[0, np.cumsum(np.bincount(pt, minlength=sc["n_pts"]))]
+    for i in range(sc["n_pts"]):
+        c = np.sort(cam[off[i]:off[i + 1]])
+        assert len(c) >= 2 and len(np.unique(c)) == len(c)
+        # on the circle of 60 cameras, all of them fit in 10 consecutive ones
+        gaps = np.diff(np.r_[c, c[0] + 60])
+        assert 60 - gaps.max() <= 9

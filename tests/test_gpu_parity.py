@@ -132,6 +132,22 @@ def test_partition_path_is_refused():
 
 
 
+def _banded_scene():
+    # each point seen by cameras within a window of 10 consecutive ones: S is
+    # banded (540 of 1,770 blocks), and the strip slabs S is stored in hold
+    # every count of rows from 1 to 7, with blocks missing inside a strip
+    return scenes.generate(None, seed=51, n_cams=60, n_pts=3000, track="uniform2_8", outlier_frac=0.02,
+                           cam_window=10)
+
+
+def test_banded_covisibility_matches_oracle():
+    _cmp_solves(_banded_scene(), capi.make_flags(robust=1), "banded")
+
+
+def test_banded_covisibility_matrix_free_matches_oracle():
+    _cmp_solves(_banded_scene(), capi.make_flags(robust=1, schur=1), "banded-mf")
+
+
 def test_matrix_free_schur_matches_oracle(scene_c1):
     """schur = 1 never forms the off-diagonal blocks; the oracle forms S block
     by block as the reference does.  The product is the same sum in another
