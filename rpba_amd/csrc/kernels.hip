@@ -231,30 +231,63 @@ __device__ __forceinline__ void load_rec27(const double *p, double *v)
 }
 
 // a whole 32-element record (27 values, then zero padding) with 16-byte stores
-__device__ __forceinline__ void store_rec32(float *p, const float *v)
+// orders LDS accesses of one wavefront (lanes exchange values through LDS)
+__device__ __forceinline__ void wave_lds_sync()
 {
-    float4 *q = (float4 *)p;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 16-byte piece j of a 27-value record (zero padding after value 26)
+__device__ __forceinline__ uint4 rec_piece(const double *v, int j)
+{
+    double2 d;
+    d.x = (2 * j < 27) ? v[2 * j] : 0.;
+    d.y = (2 * j + 1 < 27) ? v[2 * j + 1] : 0.;
+    return __builtin_bit_cast(uint4, d);
+}
+__device__ __forceinline__ uint4 rec_piece(const float *v, int j)
+{
+    float4 f;
+    f.x = (4 * j < 27) ? v[4 * j] : 0.f;
+    f.y = (4 * j + 1 < 27) ? v[4 * j + 1] : 0.f;
+    f.z = (4 * j + 2 < 27) ? v[4 * j + 2] : 0.f;
+    f.w = (4 * j + 3 < 27) ? v[4 * j + 3] : 0.f;
+    return __builtin_bit_cast(uint4, f);
+}
+
+// The records of a wavefront's entries q0 .. q0 + nq - 1 (nq <= 64), written
+// as whole 128-byte lines: each lane's record goes to the wavefront's LDS eight
+// 16-byte pieces at a time, then lane l stores piece l % 8 of record
+// 8 p + l / 8, so one store instruction writes eight full lines (a lane
+// storing its own record would touch 64 lines per instruction).  Record of
+// entry q at rank[q] (or q).  Every lane of the wavefront calls this; lanes
+// past the entries (lane >= nq) hold no record.
+template <typename R>
+__device__ __forceinline__ void store_records_wave(R *base, const int *rank, int q0, int nq, const R *v, uint4 (*stg)[9],
+                                                   int lane)
+{
+    constexpr int NP = kRec * (int)sizeof(R) / 16;  // pieces per record: 16 (FP64) or 8 (FP32)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        float4 x;
-        x.x = (4 * j < 27) ? v[4 * j] : 0.f;
-        x.y = (4 * j + 1 < 27) ? v[4 * j + 1] : 0.f;
-        x.z = (4 * j + 2 < 27) ? v[4 * j + 2] : 0.f;
-        x.w = (4 * j + 3 < 27) ? v[4 * j + 3] : 0.f;
-        q[j] = x;
+    for (int h = 0; h < NP / 8; ++h) {
+        if (lane < nq) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) stg[lane][j] = rec_piece(v, 8 * h + j);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const int rr = 8 * p + (lane >> 3), pp = lane & 7;
+            if (rr < nq) {
+                const size_t rec = rank ? (size_t)rank[q0 + rr] : (size_t)(q0 + rr);
+                ((uint4 *)(base + rec * kRec))[8 * h + pp] = stg[rr][pp];
+            }
+        }
+        wave_lds_sync();
     }
 }
-__device__ __forceinline__ void store_rec32(double *p, const double *v)
-{
-    double2 *q = (double2 *)p;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        double2 x;
-        x.x = (2 * j < 27) ? v[2 * j] : 0.;
-        x.y = (2 * j + 1 < 27) ? v[2 * j + 1] : 0.;
-        q[j] = x;
-    }
-}
+
 
 // V = sum Bp^T W Bp and g_p = sum Bp^T W r per point, then damping and the
 // 3x3 inverse (rbacore.cpp:1341-1344).  A workgroup per group of points whose
@@ -328,9 +361,13 @@ __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
 template <typename R>
 __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
 {
+    __shared__ uint4 stg[2][64][9];  // per wavefront: 64 records' staged pieces
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.ntr) return;
-    const int i = a.tpt[q], k = a.tcam[q];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q0 = q - lane, nq = min(64, a.ntr - q0);
+    if (nq <= 0) return;  // the whole wavefront is past the entries
+    const int qc = min(q, a.ntr - 1);  // lanes past the entries compute entry ntr - 1 and store nothing
+    const int i = a.tpt[qc], k = a.tcam[qc];
     const double X0 = a.X[3 * i], X1 = a.X[3 * i + 1], X2 = a.X[3 * i + 2];
     R Vr[9];
 #pragma unroll
@@ -338,10 +375,10 @@ __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
     CamV c;
     load_cam(a.cam, k, c);
     Ev ev;
-    eval_obs(c, X0, X1, X2, a.ox[q], a.oy[q], a.defdiff, ev);
+    eval_obs(c, X0, X1, X2, a.ox[qc], a.oy[qc], a.defdiff, ev);
     R Bc[2][9], Bp[2][3];
     jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
-    const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
+    const R W0 = (R)a.wx[qc], W1 = (R)a.wy[qc], W2 = (R)a.wxy[qc];
     R Wv[27], Yv[27];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
@@ -357,8 +394,8 @@ __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
         for (int cc = 0; cc < 3; ++cc) Yv[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
     }
     // whole records, padding included: every cache line is fully written
-    store_rec32((R *)a.Wrec + (size_t)q * kRec, Wv);
-    store_rec32((R *)a.Yrec + (size_t)(a.yrank ? a.yrank[q] : q) * kRec, Yv);
+    store_records_wave<R>((R *)a.Wrec, nullptr, q0, nq, Wv, stg[w], lane);
+    store_records_wave<R>((R *)a.Yrec, a.yrank, q0, nq, Yv, stg[w], lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -521,13 +558,6 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// orders LDS accesses of one wavefront (lanes exchange values through LDS)
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Off-diagonal blocks, atomic-free, one wavefront per strip slab: lane 9 g + i
 // accumulates row i of the slab's block g, S_kl[i][j] = -sum over the block's
