@@ -787,8 +787,18 @@ constexpr int kSpmvWaves = 4;  // wavefronts per workgroup (256 threads)
             E = min(Q + a.slab_chunk, strip_s0[ST + 1]);                              \
         }                                                                             \
     }
-// position after (CI, C, Q, E, ST, OK) into (NCI, NC, NQ, NE, NST, NOK)
-#define SPMV_ADVANCE(CI, C, Q, E, ST, OK, NCI, NC, NQ, NE, NST, NOK)                  \
+// slab Q's first slot B, block count M, rows present K and column L, loaded
+// (scalar) when the position is found, a slab ahead of their use
+#define SPMV_META(Q, B, M, K, L)                                                      \
+    {                                                                                 \
+        B = slab_bs[Q];                                                               \
+        M = slab_bs[(Q) + 1] - B;                                                     \
+        K = slab_mask[Q];                                                             \
+        L = slab_l[Q];                                                                \
+    }
+// position after (CI, C, Q, E, ST, OK) into (NCI, NC, NQ, NE, NST, NOK), and
+// the new slab's metadata
+#define SPMV_ADVANCE(CI, C, Q, E, ST, OK, NCI, NC, NQ, NE, NST, NOK, NB, NM, NK, NL)  \
     {                                                                                 \
         if (OK && Q + 1 < E) {                                                        \
             NCI = CI; NC = C; NQ = Q + 1; NE = E; NST = ST; NOK = 1;                  \
@@ -798,14 +808,14 @@ constexpr int kSpmvWaves = 4;  // wavefronts per workgroup (256 threads)
         } else {                                                                      \
             NCI = CI; NC = C; NQ = Q; NE = E; NST = ST; NOK = 0;                      \
         }                                                                             \
+        SPMV_META(NQ, NB, NM, NK, NL);                                                \
     }
 // the nine column entries of the lane's row in slab Q and d_k[r] of that row;
 // every lane loads (absent rows read the slab's first block and are zeroed
 // when used), so the compiler counts the loads exactly
-#define SPMV_LOAD(Q, ST, V, DK)                                                       \
+#define SPMV_LOAD(B, M, K, ST, V, DK)                                                 \
     {                                                                                 \
-        const int base_ = slab_bs[Q], m_ = slab_bs[(Q) + 1] - base_;                   \
-        const int mask_ = slab_mask[Q];                                               \
+        const int base_ = (B), m_ = (M), mask_ = (K);                                 \
         const bool pr_ = g < kSlab && ((mask_ >> g) & 1);                             \
         const int rank_ = pr_ ? __popc(mask_ & ((1 << g) - 1)) : 0;                   \
         const double *src_ = a.Su + 81 * (size_t)base_ + 9 * rank_ + r;               \
@@ -814,9 +824,9 @@ constexpr int kSpmvWaves = 4;  // wavefronts per workgroup (256 threads)
         DK = d[9 * (size_t)k_ + r];                                                   \
     }
 // one slab: products, column share, and the row partial at the chunk's end
-#define SPMV_SLAB(C, Q, E, V, DK)                                                     \
+#define SPMV_SLAB(C, Q, E, K, L, V, DK)                                               \
     {                                                                                 \
-        const int mask_ = slab_mask[Q], l_ = slab_l[Q];                               \
+        const int mask_ = (K), l_ = (L);                                              \
         const bool pr_ = g < kSlab && ((mask_ >> g) & 1);                             \
         double cv_[9];                                                                \
         _Pragma("unroll") for (int cc = 0; cc < 9; ++cc) cv_[cc] = pr_ ? V[cc] : 0.;  \
@@ -861,27 +871,32 @@ __device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j, const i
     double yrow = 0.;  // row k0 + g of the current chunk: sum over its slabs of (S_kl d_l)_r
     // two slabs in flight: register sets A and B alternate
     int ci0, c0, q0, e0, st0, ok0, ci1, c1, q1, e1, st1, ok1;
+    int b0, m0, k0, l0, b1, m1, k1, l1;  // the two slabs' metadata
     SPMV_CHUNK_START(blockIdx.x * kSpmvWaves + w, c0, q0, e0, st0, ok0);
     ci0 = blockIdx.x * kSpmvWaves + w;
-    SPMV_ADVANCE(ci0, c0, q0, e0, st0, ok0, ci1, c1, q1, e1, st1, ok1);
+    SPMV_META(q0, b0, m0, k0, l0);
+    SPMV_ADVANCE(ci0, c0, q0, e0, st0, ok0, ci1, c1, q1, e1, st1, ok1, b1, m1, k1, l1);
     double vA[9], vB[9], dkA, dkB;
     // One iteration: slab p0 from set A, slab p1 (if any) from set B, and
     // both sets reloaded unconditionally, so that every path back to the loop
     // head has issued A's loads before B's and the compiler waits only for
     // the set in use.
     if (ok0) {
-        SPMV_LOAD(q0, st0, vA, dkA);
-        SPMV_LOAD(q1, st1, vB, dkB);
+        SPMV_LOAD(b0, m0, k0, st0, vA, dkA);
+        SPMV_LOAD(b1, m1, k1, st1, vB, dkB);
         do {
             int ci2, c2, q2, e2, st2, ok2, ci3, c3, q3, e3, st3, ok3;
-            SPMV_ADVANCE(ci1, c1, q1, e1, st1, ok1, ci2, c2, q2, e2, st2, ok2);
-            SPMV_ADVANCE(ci2, c2, q2, e2, st2, ok2, ci3, c3, q3, e3, st3, ok3);
-            SPMV_SLAB(c0, q0, e0, vA, dkA);
-            SPMV_LOAD(q2, st2, vA, dkA);
-            if (ok1) SPMV_SLAB(c1, q1, e1, vB, dkB);
-            SPMV_LOAD(q3, st3, vB, dkB);
+            int b2, m2, k2, l2, b3, m3, k3, l3;
+            SPMV_ADVANCE(ci1, c1, q1, e1, st1, ok1, ci2, c2, q2, e2, st2, ok2, b2, m2, k2, l2);
+            SPMV_ADVANCE(ci2, c2, q2, e2, st2, ok2, ci3, c3, q3, e3, st3, ok3, b3, m3, k3, l3);
+            SPMV_SLAB(c0, q0, e0, k0, l0, vA, dkA);
+            SPMV_LOAD(b2, m2, k2, st2, vA, dkA);
+            if (ok1) SPMV_SLAB(c1, q1, e1, k1, l1, vB, dkB);
+            SPMV_LOAD(b3, m3, k3, st3, vB, dkB);
             ci0 = ci2; c0 = c2; q0 = q2; e0 = e2; st0 = st2;
             ci1 = ci3; c1 = c3; q1 = q3; e1 = e3; st1 = st3;
+            b0 = b2; m0 = m2; k0 = k2; l0 = l2;
+            b1 = b3; m1 = m3; k1 = k3; l1 = l3;
             ok0 = ok1 ? ok2 : 0;
             ok1 = ok3;
         } while (ok0);
@@ -906,11 +921,12 @@ __device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j, const i
 }
 
 #undef SPMV_CHUNK_START
+#undef SPMV_META
 #undef SPMV_ADVANCE
 #undef SPMV_LOAD
 #undef SPMV_SLAB
 
-__global__ __launch_bounds__(256) void k_spmv_slabs(PcgArgs a, const int *__restrict__ chunk_strip,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spmv_slabs(PcgArgs a, const int *__restrict__ chunk_strip,
                                                     const int *__restrict__ strip_s0, const int *__restrict__ chunk_off,
                                                     const int *__restrict__ slab_bs, const int *__restrict__ slab_mask,
                                                     const int *__restrict__ slab_l, const double *__restrict__ d)
