@@ -1577,12 +1577,16 @@ __device__ __forceinline__ unsigned ord_bits(float f)
 
 // exact upper median (sorted[n/2]) of key*redfac per image by 4-pass radix
 // select; s00[k] = 1/(median * 2.1981f), x0.001 while iflag > 1
+constexpr int kMedianLds = 8192;  // entries of an image whose keys k_median keeps in LDS
+
 __global__ __launch_bounds__(256) void k_median(RobustArgs a)
 {
     const int k = blockIdx.x;
     const int b = a.camoff[k], e = a.camoff[k + 1], n = e - b;
     __shared__ unsigned hist[256];
     __shared__ unsigned s_prefix, s_rank;
+    __shared__ unsigned ukey[kMedianLds];  // the image's ordered key bits (when it has at most kMedianLds entries)
+    const bool staged = n <= kMedianLds;
     if (threadIdx.x == 0) {
         s_prefix = 0;
         s_rank = (unsigned)(n / 2);
@@ -1593,10 +1597,16 @@ __global__ __launch_bounds__(256) void k_median(RobustArgs a)
         __syncthreads();
         const unsigned prefix = s_prefix;
         for (int j = b + threadIdx.x; j < e; j += blockDim.x) {
-            const int q = a.camq[j];
-            const int p = a.tpt[q];
-            const float kv = a.key[q] * redfac_of(a.toff[p + 1] - a.toff[p]);
-            const unsigned u = ord_bits(kv);
+            unsigned u;
+            if (staged && shift != 24) {
+                u = ukey[j - b];
+            } else {
+                const int q = a.camq[j];
+                const int p = a.tpt[q];
+                const float kv = a.key[q] * redfac_of(a.toff[p + 1] - a.toff[p]);
+                u = ord_bits(kv);
+                if (staged) ukey[j - b] = u;
+            }
             if ((u & maskbits) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1u);
         }
         __syncthreads();
