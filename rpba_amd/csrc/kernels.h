@@ -80,7 +80,7 @@ struct SchurArgs {
 struct PcgArgs {
     int nc, msize;
     const double *Sd, *Su, *Cinv, *b;
-    const int *row_off, *col_off, *col_blk, *col_pos, *blk_k, *blk_l;
+    const int *row_off, *col_off, *col_blk, *blk_k, *blk_l;
     double *x, *best, *r, *d, *h, *h1;
     double *Ad;       // S_kk d_k per camera (diagonal products of the SpMV)
     double *part;     // per camera: |alpha S d|^2, r.(h1 - h), r.h1 shares

@@ -357,7 +357,7 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.nc = c->nc;
     a.msize = std::max(1000, c->nrioff);  // rbacore.cpp:1848-1849
     a.Sd = c->d_Sd.p; a.Su = c->d_Su.p; a.Cinv = c->d_Cinv.p; a.b = c->d_b.p;
-    a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.col_pos = c->ds.cpos; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
+    a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
     a.x = c->d_x.p; a.best = c->d_best.p; a.r = c->d_r.p; a.d = c->d_d.p; a.Ad = c->d_Ad.p;
     a.h = c->d_h.p; a.h1 = c->d_h1.p; a.part = c->d_ppart.p; a.S = c->d_pcg.p;
     a.sm = c->ds.sumap();

@@ -47,13 +47,10 @@ struct DevStruct {
     size_t ch_start_cap = 0, ch_off_cap = 0;
     // pairs (a << 32 | b), cam(a) < cam(b), grouped by block in row-major
     // block order; poff: pairs of block b; bk/bl: block (k, l); roff: blocks of
-    // row k; coff/cblk: blocks of column l (ids, k ascending); cpos: position
-    // of each block in that column order (cpos[cblk[j]] = j)
+    // row k; coff/cblk: blocks of column l (ids, k ascending)
     unsigned long long *pab = nullptr;
     int *poff = nullptr, *bk = nullptr, *bl = nullptr, *roff = nullptr, *coff = nullptr, *cblk = nullptr;
-    int *cpos = nullptr;
     size_t pab_cap = 0, poff_cap = 0, bk_cap = 0, bl_cap = 0, roff_cap = 0, coff_cap = 0, cblk_cap = 0;
-    size_t cpos_cap = 0;
     // strip slabs (the storage order of the upper blocks of S): strip s holds
     // rows kSlab s .. kSlab s + kSlab - 1; its slab for column l holds the
     // blocks (k, l) of those rows, k ascending.  sslot: block -> slot (its

@@ -281,7 +281,6 @@ int DevStruct::build(hipStream_t st, int *first_empty)
     rc |= grow(roff, roff_cap, (size_t)nc + 1);
     rc |= grow(coff, coff_cap, (size_t)nc + 1);
     rc |= grow(cblk, cblk_cap, (size_t)nblk);
-    rc |= grow(cpos, cpos_cap, (size_t)nblk);
     rc |= grow(key2, key2_cap, (size_t)nblk);
     rc |= grow(key2s, key2s_cap, (size_t)nblk);
     rc |= grow(bid, bid_cap, (size_t)nblk);
@@ -302,7 +301,6 @@ int DevStruct::build(hipStream_t st, int *first_empty)
         DCHK(rocprim::radix_sort_pairs(tmp, bytes, key2, key2s, bid, cblk, nblk, 0u, kb, st));
     }
     k_lower_bound<<<nb_(nc + 1, 256), 256, 0, st>>>(nc, (unsigned)nc, key2s, nblk, coff);
-    if (nblk > 0) k_invert<<<nb_(nblk, 256), 256, 0, st>>>(nblk, cblk, cpos);
     rc = strip_slabs(st);
     if (rc) return rc;
     DCHK(hipStreamSynchronize(st));
@@ -778,7 +776,7 @@ void DevStruct::release()
     fr(chunk_strip, chunk_strip_cap); fr(cs_off, cs_off_cap); fr(cs_list, cs_list_cap);
     fr(slab_key, slab_key_cap); fr(slab_cnt, slab_cnt_cap);
     fr(pab, pab_cap); fr(ukey, ukey_cap); fr(ucnt, ucnt_cap); fr(nruns, nruns_cap); fr(poff, poff_cap);
-    fr(bk, bk_cap); fr(bl, bl_cap); fr(roff, roff_cap); fr(coff, coff_cap); fr(cblk, cblk_cap); fr(cpos, cpos_cap);
+    fr(bk, bk_cap); fr(bl, bl_cap); fr(roff, roff_cap); fr(coff, coff_cap); fr(cblk, cblk_cap);
     fr(key2, key2_cap); fr(key2s, key2s_cap); fr(bid, bid_cap); fr(newi, newi_cap); fr(newoff, newoff_cap);
     fr(tmp, tmp_cap); fr(emap, emap_cap); fr(sflag, sflag_cap); fr(spos, spos_cap); fr(cq2, cq2_cap);
 }
