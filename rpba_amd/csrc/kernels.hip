@@ -1097,11 +1097,11 @@ __device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S
     __shared__ double sh[4];
     __shared__ double red[28][9];
     __shared__ double rs[9], p3[3][9];
-    const double dAd = sum_fixed(a.dsum, a.ndsum, sh);
-    const double rho = S.rho;
-    const double alpha = (fabs(rho) < 1.e-20 || fabs(rho) > 1.e20) ? 0. : rho / dAd;
     const int t = threadIdx.x;
     const int g = t / 9, r = t % 9;
+    // the camera's own entries, loaded with the gathers below
+    const size_t j = 9 * (size_t)k + (t < 9 ? t : 0);
+    const double xj = a.x[j], dj = a.d[j], rj = a.r[j];
     if (a.implicit) {
         // (S d)_k = (S_kk + D_k) d_k - the chunk sums of image k, in chunk order
         if (g < 28) {
@@ -1135,14 +1135,18 @@ __device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S
         red[g][r] = acc;
     }
     }
+    // d.S d after the gathers, so that its loads overlap theirs (its first
+    // barrier also publishes red)
+    const double dAd = sum_fixed(a.dsum, a.ndsum, sh);
+    const double rho = S.rho;
+    const double alpha = (fabs(rho) < 1.e-20 || fabs(rho) > 1.e20) ? 0. : rho / dAd;
     __syncthreads();
-    const size_t j = 9 * (size_t)k + t;
     if (t < 9) {
         double Ad = 0.;
         for (int gg = 0; gg < 28; ++gg) Ad += red[gg][t];
-        a.x[j] += alpha * a.d[j];
+        a.x[j] = xj + alpha * dj;
         const double dr = alpha * Ad;
-        const double rn = a.r[j] - dr;
+        const double rn = rj - dr;
         a.r[j] = rn;
         rs[t] = rn;
         p3[0][t] = dr * dr;
