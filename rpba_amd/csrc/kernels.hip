@@ -1292,25 +1292,43 @@ __global__ __launch_bounds__(1024) void k_pcg_rho(PcgArgs a)
 // the next launch
 __device__ __forceinline__ void pcg_b_body(const PcgArgs &a, const PcgScalars &S, PcgScalars *Sn, int grp)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[3][4];
     const int G = a.nc;
+    const int t = threadIdx.x;
+    const int k = grp * 28 + t / 9;
+    const bool on = (t < 252) && (k < a.nc);
+    const size_t j = on ? 9 * (size_t)k + t % 9 : 0;
+    // this thread's entries, loaded with the shares below
+    const double xj = a.x[j], h1 = a.h1[j], dj = a.d[j];
     double m = 0., num = 0., rh = 0.;
     for (int g = threadIdx.x; g < G; g += blockDim.x) {
         m += a.part[3 * g];
         num += a.part[3 * g + 1];
         rh += a.part[3 * g + 2];
     }
-    m = block_sum(m, sh);
-    num = block_sum(num, sh);
-    rh = block_sum(rh, sh);
+    // the three block sums in one pass (each in the order of block_sum)
+    {
+        m = wave_sum(m);
+        num = wave_sum(num);
+        rh = wave_sum(rh);
+        const int w = t >> 6, nw = blockDim.x >> 6;
+        if ((t & 63) == 0) {
+            sh[0][w] = m;
+            sh[1][w] = num;
+            sh[2][w] = rh;
+        }
+        __syncthreads();
+        m = num = rh = 0.;
+        for (int i = 0; i < nw; ++i) {
+            m += sh[0][i];
+            num += sh[1][i];
+            rh += sh[2][i];
+        }
+    }
     const int it = S.k;
     const double m0 = (it == 0) ? m : S.m0;
     const bool better = m < S.mold;
-    const int t = threadIdx.x;
-    const int k = grp * 28 + t / 9;
-    const bool on = (t < 252) && (k < a.nc);
-    const size_t j = 9 * (size_t)k + t % 9;
-    if (better && on) a.best[j] = a.x[j];
+    if (better && on) a.best[j] = xj;
     PcgScalars z = S;
     z.m0 = m0;
     if (better) z.mold = m;
@@ -1323,8 +1341,7 @@ __device__ __forceinline__ void pcg_b_body(const PcgArgs &a, const PcgScalars &S
     double beta = (fabs(num) < 1.e-10) ? 0. : num / S.rho;
     if (beta < 0.) beta = 0.;
     if (on) {
-        const double h1 = a.h1[j];
-        a.d[j] = h1 + beta * a.d[j];
+        a.d[j] = h1 + beta * dj;
         a.h[j] = h1;
     }
     z.rho = rh;
