@@ -230,7 +230,6 @@ __device__ __forceinline__ void load_rec27(const double *p, double *v)
     }
 }
 
-// a whole 32-element record (27 values, then zero padding) with 16-byte stores
 // orders LDS accesses of one wavefront (lanes exchange values through LDS)
 __device__ __forceinline__ void wave_lds_sync()
 {
