@@ -205,31 +205,6 @@ __device__ __forceinline__ void jac_point(const CamV &c, const Ev &e, T Bp[2][3]
     }
 }
 
-// the 27 used elements of a 32-element record (aligned to its size) with
-// 16-byte loads
-__device__ __forceinline__ void load_rec27(const float *p, float *v)
-{
-    const float4 *q = (const float4 *)p;
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-        const float4 x = q[j];
-        v[4 * j] = x.x;
-        if (4 * j + 1 < 28) v[4 * j + 1] = x.y;
-        if (4 * j + 2 < 28) v[4 * j + 2] = x.z;
-        if (4 * j + 3 < 28) v[4 * j + 3] = x.w;
-    }
-}
-__device__ __forceinline__ void load_rec27(const double *p, double *v)
-{
-    const double2 *q = (const double2 *)p;
-#pragma unroll
-    for (int j = 0; j < 14; ++j) {
-        const double2 x = q[j];
-        v[2 * j] = x.x;
-        v[2 * j + 1] = x.y;
-    }
-}
-
 // orders LDS accesses of one wavefront (lanes exchange values through LDS)
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -254,6 +229,52 @@ __device__ __forceinline__ uint4 rec_piece(const float *v, int j)
     f.z = (4 * j + 2 < 27) ? v[4 * j + 2] : 0.f;
     f.w = (4 * j + 3 < 27) ? v[4 * j + 3] : 0.f;
     return __builtin_bit_cast(uint4, f);
+}
+
+// The 27 values of the records of a wavefront's entries q0 .. q0 + nq - 1
+// (nq <= 64, record of entry q at recs + q kRec) into lane q - q0's v, read as
+// whole lines: PP 16-byte pieces of every record go through the wavefront's
+// LDS per pass, lane l loading piece l % PP of record (64 / PP) p + l / PP, so
+// one load instruction covers 64 / PP records' consecutive pieces (a lane
+// loading its own record touches 64 lines per instruction).  Every lane of
+// the wavefront calls this; lanes past the entries get nothing.
+template <typename R, int PP>
+__device__ __forceinline__ void load_records_wave(const R *recs, int q0, int nq, R *v, uint4 (*stg)[PP + 1], int lane)
+{
+    constexpr int NV = 16 / (int)sizeof(R);  // values per piece
+    constexpr int NP = (27 + NV - 1) / NV;   // pieces holding the 27 values
+    constexpr int RPI = 64 / PP;             // records per load instruction
+#pragma unroll
+    for (int h = 0; h < (NP + PP - 1) / PP; ++h) {
+#pragma unroll
+        for (int p = 0; p < PP; ++p) {
+            const int rr = RPI * p + lane / PP, pp = lane % PP;
+            if (rr < nq && PP * h + pp < NP)
+                stg[rr][pp] = ((const uint4 *)(recs + (size_t)(q0 + rr) * kRec))[PP * h + pp];
+        }
+        wave_lds_sync();
+        if (lane < nq) {
+#pragma unroll
+            for (int j = 0; j < PP; ++j) {
+                const int pc = PP * h + j;
+                if (pc < NP) {
+                    const uint4 u = stg[lane][j];
+                    if constexpr (sizeof(R) == 8) {
+                        const double2 d = __builtin_bit_cast(double2, u);
+                        if (2 * pc < 27) v[2 * pc] = d.x;
+                        if (2 * pc + 1 < 27) v[2 * pc + 1] = d.y;
+                    } else {
+                        const float4 f = __builtin_bit_cast(float4, u);
+                        if (4 * pc < 27) v[4 * pc] = f.x;
+                        if (4 * pc + 1 < 27) v[4 * pc + 1] = f.y;
+                        if (4 * pc + 2 < 27) v[4 * pc + 2] = f.z;
+                        if (4 * pc + 3 < 27) v[4 * pc + 3] = f.w;
+                    }
+                }
+            }
+        }
+        wave_lds_sync();
+    }
 }
 
 // The records of a wavefront's entries q0 .. q0 + nq - 1 (nq <= 64), written
@@ -973,11 +994,14 @@ __global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
     }
     double t0 = 0., t1 = 0., t2 = 0.;
     const R *Wr = (const R *)a.Wrec;
+    __shared__ uint4 stg[4][64][5];  // per wavefront: four 16-byte pieces of 64 records
+    const int lane = t & 63, wv = t >> 6;
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
+        R W[28];
+        const int q0 = base + 64 * wv;
+        load_records_wave<R, 4>(Wr, q0, min(64, max(0, e1 - q0)), W, stg[wv], lane);
         if (q < e1) {
-            R W[28];
-            load_rec27(Wr + (size_t)q * kRec, W);
             const double *dv = a.d + 9 * (size_t)a.tcam[q];
             double u0 = 0., u1 = 0., u2 = 0.;
 #pragma unroll
@@ -1042,12 +1066,14 @@ __global__ __launch_bounds__(64) void k_mf_chunks(PcgArgs a)
     double v[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) v[r] = 0.;
+    // the chunk's Y records (camera-major, consecutive) read as whole lines
+    __shared__ uint4 stg[64][9];
+    R Y[28];
+    load_records_wave<R, 8>(Yr, b, n, Y, stg, lane);
     if (lane < n) {
-        // Y records and t of the entry's point, both in camera-major order
+        // t of the entry's point, in camera-major order
         const double *tq = a.tc + 3 * (size_t)(b + lane);
         const double t0 = tq[0], t1 = tq[1], t2 = tq[2];
-        R Y[28];
-        load_rec27(Yr + (size_t)(b + lane) * kRec, Y);
 #pragma unroll
         for (int r = 0; r < 9; ++r) v[r] = (double)Y[3 * r] * t0 + (double)Y[3 * r + 1] * t1 + (double)Y[3 * r + 2] * t2;
     }
@@ -1392,11 +1418,14 @@ __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
         t2 = a.gp[3 * (size_t)i + 2];
     }
     const R *Wr = (const R *)a.Wrec;
+    __shared__ uint4 stg[4][64][5];  // per wavefront: four 16-byte pieces of 64 records
+    const int lane = t & 63, wv = t >> 6;
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
+        R W[28];
+        const int q0 = base + 64 * wv;
+        load_records_wave<R, 4>(Wr, q0, min(64, max(0, e1 - q0)), W, stg[wv], lane);
         if (q < e1) {
-            R W[28];
-            load_rec27(Wr + (size_t)q * kRec, W);
             const double *xc = a.sol + 9 * (size_t)a.tcam[q];
             double u0 = 0., u1 = 0., u2 = 0.;
 #pragma unroll
