@@ -1126,7 +1126,14 @@ __device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S
     const int g = t / 9, r = t % 9;
     // the camera's own entries, loaded with the gathers below
     const size_t j = 9 * (size_t)k + (t < 9 ? t : 0);
-    const double xj = a.x[j], dj = a.d[j], rj = a.r[j];
+    const double xj = a.x[j], dj = a.d[j], rj = a.r[j], hj = a.h[j];
+    // the lane's row of the block-Jacobi inverse, in flight during the gathers
+    double Crow[9];
+    {
+        const double *C = a.Cinv + 81 * (size_t)k + 9 * (t < 9 ? t : 0);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Crow[c] = C[c];
+    }
     if (a.implicit) {
         // (S d)_k = (S_kk + D_k) d_k - the chunk sums of image k, in chunk order
         if (g < 28) {
@@ -1178,12 +1185,11 @@ __device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S
     }
     __syncthreads();
     if (t < 9) {
-        const double *C = a.Cinv + 81 * (size_t)k + 9 * t;
         double h1 = 0.;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) h1 += C[c] * rs[c];
+        for (int c = 0; c < 9; ++c) h1 += Crow[c] * rs[c];
         a.h1[j] = h1;
-        p3[1][t] = rs[t] * (h1 - a.h[j]);
+        p3[1][t] = rs[t] * (h1 - hj);
         p3[2][t] = rs[t] * h1;
     }
     __syncthreads();
