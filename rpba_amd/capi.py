@@ -89,6 +89,9 @@ class Block:
         self.cam_f = np.array(scene["cam_f"], dtype=np.float32).copy()
         self.cam_d = np.array(scene["cam_d"], dtype=np.float32).reshape(-1, 2).copy()
         self.pts = np.array(scene["pts"], dtype=np.float32).reshape(-1, 3).copy()
+        iw, ih = scene.get("img_w"), scene.get("img_h")
+        self.img_w = None if iw is None else np.ascontiguousarray(iw, dtype=np.int32)
+        self.img_h = None if ih is None else np.ascontiguousarray(ih, dtype=np.int32)
         self.n_cams = self.cam_f.shape[0]
         self.n_pts = self.pts.shape[0]
         self.n_obs = self.obs_cam.shape[0]
@@ -100,7 +103,7 @@ class Block:
             _ptr(self.obs_xy, C.c_float), _ptr(self.obs_w, C.c_float),
             _ptr(self.cam_P, C.c_float), _ptr(self.cam_f, C.c_float),
             _ptr(self.cam_d, C.c_float), _ptr(self.pts, C.c_float),
-            None, None,
+            _ptr(self.img_w, C.c_int32), _ptr(self.img_h, C.c_int32),
             _ptr(self.obs_kept, C.c_uint8), _ptr(self.pt_kept, C.c_uint8),
         )
 

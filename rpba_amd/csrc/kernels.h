@@ -150,12 +150,15 @@ struct RobustArgs {
     const int *iflag_blk = nullptr, *camblk = nullptr;
 };
 
+// Exteval over tracks in point-major CSR (toff[npts + 1], entries q):
+// xy[2q] = xg, w[3q] = (wx, wy, wxy) or null for unit weights.
 struct ExtArgs {
-    int npts;
+    int npts, ncams, nraddp;
     const int *toff, *cam;
-    const float *ox, *oy, *wx, *wy, *wxy, *P, *cf, *cd, *pts;
-    double *partial;
-    int *ipartial;
+    const float *xy, *w, *P, *cf, *cd, *pts;
+    const int *img_w, *img_h;  // null = 1 (as InputBlock sets them)
+    double *partial;           // per workgroup: sum of the point terms
+    int *tpartial, *cpartial;  // per workgroup: sum of t and count of points with t > 1
 };
 
 void launch_lin_points(const LinArgs &a, hipStream_t st);
@@ -181,7 +184,11 @@ void launch_cost(const CostArgs &a, double *out, hipStream_t st);
 void launch_reweight(const RobustArgs &a, double *wsum_out, int *nflag_out, hipStream_t st);
 void launch_wsum(int n, const float *wx, const float *wy, double *partial, double *out, hipStream_t st);
 void launch_store_cams(const CamPtrs &cp, int nc, hipStream_t st);
-void launch_exteval(const ExtArgs &a, double *sg_out, int *t_out, hipStream_t st);
+// workgroups of the Exteval launch (partial slots needed)
+int exteval_groups(int npts);
+// out[0] = sigma (ITTTT value), out[1] = nobs, out[2] = dof (EvalLSA,
+// rpbacore.cpp:923-938), computed on the device
+void launch_exteval(const ExtArgs &a, double *out, hipStream_t st);
 
 // batched independent blocks (one union of block-diagonal structure):
 // cameras, points and track entries of block b are the contiguous ranges

@@ -1764,7 +1764,8 @@ __global__ __launch_bounds__(256) void k_exteval(ExtArgs a)
         t = e - b;
         if (t > 1) {
             double ws = 0.;
-            for (int q = b; q < e; ++q) ws += (double)a.wx[q] + (double)a.wy[q];
+            for (int q = b; q < e; ++q)
+                ws += a.w ? (double)a.w[3 * (size_t)q] + (double)a.w[3 * (size_t)q + 1] : 2.;
             ws = ws / (double)(2 * t);
             const double X0 = a.pts[3 * i], X1 = a.pts[3 * i + 1], X2 = a.pts[3 * i + 2];
             double s = 0.;
@@ -1786,8 +1787,10 @@ __global__ __launch_bounds__(256) void k_exteval(ExtArgs a)
                 p2 *= z;
                 p1 *= (double)a.cf[c];
                 p2 *= (double)a.cf[c];
-                const double dx = (double)a.ox[q] - p1, dy = (double)a.oy[q] - p2;
-                s += dx * dx * a.wx[q] + dy * dy * a.wy[q] + 2 * dx * dy * a.wxy[q];
+                const double dx = (double)a.xy[2 * (size_t)q] - p1, dy = (double)a.xy[2 * (size_t)q + 1] - p2;
+                const float wx = a.w ? a.w[3 * (size_t)q] : 1.f, wy = a.w ? a.w[3 * (size_t)q + 1] : 1.f;
+                const float wxy = a.w ? a.w[3 * (size_t)q + 2] : 0.f;
+                s += dx * dx * wx + dy * dy * wy + 2 * dx * dy * wxy;
             }
             contrib = s / ws;
         } else
@@ -1795,9 +1798,44 @@ __global__ __launch_bounds__(256) void k_exteval(ExtArgs a)
     }
     const double sg = block_sum(contrib, sh);
     const double tt = block_sum((double)t, sh);
+    const double cc = block_sum(t > 1 ? 1. : 0., sh);
     if (threadIdx.x == 0) {
         a.partial[blockIdx.x] = sg;
-        a.ipartial[blockIdx.x] = (int)tt;
+        a.tpartial[blockIdx.x] = (int)tt;
+        a.cpartial[blockIdx.x] = (int)cc;
+    }
+}
+
+// one workgroup: the partials in a fixed order, then sigma as EvalLSA forms it
+// (rpbacore.cpp:923-938): dof = sum(2t - 3) - 6 (n - 2) - 5 - n * nraddp over
+// points with t > 1, ihw2 = mean over images of ((w + h) / 2)^2 accumulated in
+// float (rpbacore.cpp:87-90), sigma = sqrt(sum / dof * ihw2)
+__global__ __launch_bounds__(1024) void k_exteval_finish(ExtArgs a, int nb, double *out)
+{
+    __shared__ double sh[16];
+    double s = 0., ts = 0., cs = 0.;
+    for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+        s += a.partial[j];
+        ts += (double)a.tpartial[j];
+        cs += (double)a.cpartial[j];
+    }
+    s = block_sum(s, sh);
+    ts = block_sum(ts, sh);
+    cs = block_sum(cs, sh);
+    if (threadIdx.x == 0) {
+        const long long tsum = (long long)ts, cnt = (long long)cs;
+        const long long red = 2 * tsum - 3 * cnt, nobs = 2 * tsum;
+        const long long dof = red - (long long)(a.ncams - 2) * 6 - 5 - (long long)a.ncams * a.nraddp;
+        float ihw2 = 0.f;
+        for (int k = 0; k < a.ncams; ++k) {
+            const int w = a.img_w ? a.img_w[k] : 1, h = a.img_h ? a.img_h[k] : 1;
+            ihw2 += (float)((h + w) * (h + w)) * 0.25f;
+        }
+        ihw2 /= (float)a.ncams;
+        const double sg = s * (1. / (double)dof);
+        out[0] = sqrt(sg * (double)ihw2);
+        out[1] = (double)nobs;
+        out[2] = (double)dof;
     }
 }
 
@@ -1943,11 +1981,12 @@ void launch_wsum(int n, const float *wx, const float *wy, double *partial, doubl
     k_final_sum<<<1, 1024, 0, st>>>(partial, nb, nullptr, out, nullptr);
 }
 void launch_store_cams(const CamPtrs &cp, int nc, hipStream_t st) { k_store_cams<<<nblk(nc, 64), 64, 0, st>>>(cp, nc); }
-void launch_exteval(const ExtArgs &a, double *sg_out, int *t_out, hipStream_t st)
+int exteval_groups(int npts) { return (int)nblk(npts > 0 ? npts : 1, 256); }
+void launch_exteval(const ExtArgs &a, double *out, hipStream_t st)
 {
-    const int nb = (int)nblk(a.npts > 0 ? a.npts : 1, 256);
+    const int nb = exteval_groups(a.npts);
     k_exteval<<<nb, 256, 0, st>>>(a);
-    k_final_sum<<<1, 1024, 0, st>>>(a.partial, nb, a.ipartial, sg_out, t_out);
+    k_exteval_finish<<<1, 1024, 0, st>>>(a, nb, out);
 }
 
 // ---------------------------------------------------------------------------

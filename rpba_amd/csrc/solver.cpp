@@ -1260,7 +1260,9 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     delete c;
 }
 
-// Exteval on the device (EvalLSA rpbacore.cpp:882-941)
+// Exteval on the device (EvalLSA rpbacore.cpp:882-941): the observations are
+// grouped into point-major tracks on the host (pointXXvio, file order within a
+// track), uploaded, and evaluated by launch_exteval, which also forms sigma.
 static int exteval(rpba_block *io, const rpba_flags *f, rpba_log_fn log, void *user, rpba_stats *stats)
 {
     const int nc = io->n_cams, np = io->n_pts, no = io->n_obs;
@@ -1274,77 +1276,63 @@ static int exteval(rpba_block *io, const rpba_flags *f, rpba_log_fn log, void *u
     for (int i = 0; i < np; ++i) cnt[(size_t)i + 1] += cnt[i];
     const size_t nt = (size_t)cnt[np];
     std::vector<int> cam(nt);
-    std::vector<float> ox(nt), oy(nt), wx(nt), wy(nt), wxy(nt);
+    std::vector<float> xy(2 * nt), w(io->obs_w ? 3 * nt : 0);
     std::vector<int> pos(cnt.begin(), cnt.end() - 1);
     for (int i = 0; i < no; ++i) {
         if (io->obs_cam[i] < 0) continue;
         const size_t q = (size_t)pos[io->obs_pt[i]]++;
         cam[q] = io->obs_cam[i];
-        ox[q] = io->obs_xy[2 * (size_t)i];
-        oy[q] = io->obs_xy[2 * (size_t)i + 1];
-        wx[q] = io->obs_w ? io->obs_w[3 * (size_t)i] : 1.f;
-        wy[q] = io->obs_w ? io->obs_w[3 * (size_t)i + 1] : 1.f;
-        wxy[q] = io->obs_w ? io->obs_w[3 * (size_t)i + 2] : 0.f;
-    }
-    int64_t red = 0, nobs = 0;
-    for (int i = 0; i < np; ++i) {
-        const int t = cnt[(size_t)i + 1] - cnt[i];
-        if (t > 1) {
-            red += 2 * t - 3;
-            nobs += 2 * t;
-        }
+        xy[2 * q] = io->obs_xy[2 * (size_t)i];
+        xy[2 * q + 1] = io->obs_xy[2 * (size_t)i + 1];
+        if (io->obs_w)
+            for (int j = 0; j < 3; ++j) w[3 * q + j] = io->obs_w[3 * (size_t)i + j];
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return RPBA_E_HIP;
     hipStream_t st;
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    DBuf<int> d_off, d_cam, d_ip;
-    DBuf<float> d_ox, d_oy, d_wx, d_wy, d_wxy, d_P, d_f, d_d, d_pts;
+    DBuf<int> d_off, d_cam, d_tp, d_cp, d_iw, d_ih;
+    DBuf<float> d_xy, d_w, d_P, d_f, d_d, d_pts;
     DBuf<double> d_part, d_out;
     int rc = 0;
     rc |= d_off.put(cnt.data(), cnt.size(), st);
     rc |= d_cam.put(cam.data(), nt, st);
-    rc |= d_ox.put(ox.data(), nt, st);
-    rc |= d_oy.put(oy.data(), nt, st);
-    rc |= d_wx.put(wx.data(), nt, st);
-    rc |= d_wy.put(wy.data(), nt, st);
-    rc |= d_wxy.put(wxy.data(), nt, st);
+    rc |= d_xy.put(xy.data(), 2 * nt, st);
+    if (io->obs_w) rc |= d_w.put(w.data(), 3 * nt, st);
+    if (io->img_w) rc |= d_iw.put(io->img_w, (size_t)nc, st);
+    if (io->img_h) rc |= d_ih.put(io->img_h, (size_t)nc, st);
     rc |= d_P.put(io->cam_P, 12 * (size_t)nc, st);
     rc |= d_f.put(io->cam_f, (size_t)nc, st);
     rc |= d_d.put(io->cam_d, 2 * (size_t)nc, st);
     rc |= d_pts.put(io->pts, 3 * (size_t)np, st);
-    const int nb = (np + 255) / 256 + 1;
+    const int nb = exteval_groups(np);
     rc |= d_part.alloc((size_t)nb);
-    rc |= d_ip.alloc((size_t)nb);
-    rc |= d_out.alloc(2);
-    double sg = 0.;
-    int tsum = 0;
+    rc |= d_tp.alloc((size_t)nb);
+    rc |= d_cp.alloc((size_t)nb);
+    rc |= d_out.alloc(3);
+    double res[3] = {0., 0., 0.};
     if (!rc) {
         ExtArgs a;
-        a.npts = np; a.toff = d_off.p; a.cam = d_cam.p; a.ox = d_ox.p; a.oy = d_oy.p;
-        a.wx = d_wx.p; a.wy = d_wy.p; a.wxy = d_wxy.p; a.P = d_P.p; a.cf = d_f.p; a.cd = d_d.p; a.pts = d_pts.p;
-        a.partial = d_part.p; a.ipartial = d_ip.p;
-        launch_exteval(a, d_out.p, d_ip.p + (nb - 1), st);
-        if (hipMemcpyAsync(&sg, d_out.p, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) rc = RPBA_E_HIP;
+        a.npts = np; a.ncams = nc;
+        a.nraddp = (f->est_k2 != 0) + (f->est_k4 != 0) + (f->est_f != 0);
+        a.toff = d_off.p; a.cam = d_cam.p; a.xy = d_xy.p; a.w = io->obs_w ? d_w.p : nullptr;
+        a.P = d_P.p; a.cf = d_f.p; a.cd = d_d.p; a.pts = d_pts.p;
+        a.img_w = io->img_w ? d_iw.p : nullptr; a.img_h = io->img_h ? d_ih.p : nullptr;
+        a.partial = d_part.p; a.tpartial = d_tp.p; a.cpartial = d_cp.p;
+        launch_exteval(a, d_out.p, st);
+        if (hipGetLastError() != hipSuccess) rc = RPBA_E_HIP;
+        if (hipMemcpyAsync(res, d_out.p, sizeof res, hipMemcpyDeviceToHost, st) != hipSuccess) rc = RPBA_E_HIP;
         if (hipStreamSynchronize(st) != hipSuccess) rc = RPBA_E_HIP;
     }
-    (void)tsum;
-    d_off.release(); d_cam.release(); d_ip.release(); d_ox.release(); d_oy.release(); d_wx.release(); d_wy.release();
-    d_wxy.release(); d_P.release(); d_f.release(); d_d.release(); d_pts.release(); d_part.release(); d_out.release();
+    d_off.release(); d_cam.release(); d_tp.release(); d_cp.release(); d_iw.release(); d_ih.release();
+    d_xy.release(); d_w.release(); d_P.release(); d_f.release(); d_d.release(); d_pts.release();
+    d_part.release(); d_out.release();
     (void)hipStreamDestroy(st);
     if (rc) return rc;
-    const int nraddp = (f->est_k2 != 0) + (f->est_k4 != 0) + (f->est_f != 0);
-    const int64_t dof = red - (int64_t)(nc - 2) * 6 - 5 - (int64_t)nc * nraddp;
-    float ihw2 = 0.f;
-    for (int i = 0; i < nc; ++i) {
-        const int w = io->img_w ? io->img_w[i] : 1, h = io->img_h ? io->img_h[i] : 1;
-        ihw2 += (float)((h + w) * (h + w)) * 0.25f;
-    }
-    ihw2 /= (float)nc;
-    sg *= 1. / (double)dof;
-    const double s = std::sqrt(sg * (double)ihw2);
+    const double s = res[0];
+    const long long nobs = (long long)res[1], dof = (long long)res[2];
     char buf[160];
-    std::snprintf(buf, sizeof buf, "ITTTT: %g %lld %lld\n", s, (long long)nobs, (long long)dof);
+    std::snprintf(buf, sizeof buf, "ITTTT: %g %lld %lld\n", s, nobs, dof);
     if (log) log(buf, user);
     if (stats) {
         stats->exteval_sigma = s;
