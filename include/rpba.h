@@ -44,6 +44,7 @@ enum rpba_status {
     RPBA_E_NAN_RESULT = -3,      /* "BA-Error: NANs in result vector"    rbacore.cpp:541-546 */
     RPBA_E_INVALID = -10,        /* malformed arguments (bad index, duplicate camera in a track) */
     RPBA_E_UNSUPPORTED = -11,    /* partitioned path (rpbacore.cpp:100-510) requested; not built */
+    RPBA_E_SCRATCH = -12,        /* input larger than a no-allocation entry point's fixed scratch */
     RPBA_E_HIP = -20,            /* HIP runtime error */
     RPBA_E_NOMEM = -21           /* host or device allocation failed */
 };
@@ -263,6 +264,27 @@ int rpba_ctx_last_omega(rpba_ctx *ctx, float *omega);
  * for each weight in omegas (0 = block-Jacobi) */
 int rpba_ctx_lp_sample(rpba_ctx *ctx, int32_t n_omega, const float *omegas, int32_t *iters, float *feat,
                        int32_t *adj_off, int32_t *adj_nbr, float *adj_w, int64_t *n_edges);
+
+/* Exteval (EvalLSA, rpbacore.cpp:882-941) from device-resident tracks, with
+ * no device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same kernels as
+ * rpba_calc_plsa with exteval = 1 run on `stream` (a hipStream_t; NULL = the
+ * null stream) and the call returns without synchronising.
+ *   toff [n_pts + 1]: the tracks (pointXXvio) as CSR offsets over entries q;
+ *   tcam [nt], txy [2 nt] (xg), tw [3 nt] (wx, wy, wxy) or NULL = unit weights;
+ *   cam_P [12 n_cams], cam_f [n_cams], cam_d [2 n_cams], pts [3 n_pts];
+ *   img_w / img_h [n_cams] or NULL (= 1);
+ *   out [3] (device): sigma (the ITTTT value), nobs, dof.
+ * All pointers are device pointers.  Partial sums go to a fixed device array
+ * of the library (one per device, held until the process ends), which limits
+ * n_pts to rpba_exteval_device_max_points() (8,388,608); larger inputs return
+ * RPBA_E_SCRATCH.  Calls that overlap on one device race on that array. */
+int rpba_exteval_device(int32_t n_cams, int32_t n_pts, const int32_t *toff, const int32_t *tcam,
+                        const float *txy, const float *tw, const float *cam_P, const float *cam_f,
+                        const float *cam_d, const float *pts, const int32_t *img_w, const int32_t *img_h,
+                        int32_t est_k2, int32_t est_k4, int32_t est_f, double *out, void *stream);
+int rpba_exteval_device_max_points(void);
 
 /* BAL text I/O (rpba.cpp:180-338), host only.
  * rpba_read_bal allocates the arrays of *io with malloc; free with
