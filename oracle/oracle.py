@@ -58,7 +58,27 @@ def _load():
     lib.oracle_invlu.argtypes = [C.c_int, P(C.c_double), P(C.c_double)]
     lib.oracle_upper_median.argtypes = [P(C.c_float), C.c_int]
     lib.oracle_upper_median.restype = C.c_float
+    lib.oracle_set_mutation.argtypes = [C.c_int]
+    lib.oracle_set_order.argtypes = [C.c_int]
+    lib.orc_set_script.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+    lib.orc_set_rec.argtypes = [C.c_void_p, P(OrcRec)]
+    lib.orc_cams.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+    lib.orc_delete_with_flags.argtypes = [C.c_void_p, P(C.c_uint8), P(C.c_int), P(C.c_int), P(C.c_int)]
     return lib
+
+
+class OrcRec(C.Structure):
+    """Per-LM-iteration records of the oracle's state (orc_rec)."""
+    _d = C.POINTER(C.c_double)
+    _i = C.POINTER(C.c_int)
+    _fields_ = [("cap", C.c_int), ("len", C.c_int), ("init_key", C.c_double),
+                ("lin_cam", _d), ("lin_pts", _d), ("trial_cam", _d), ("trial_pts", _d), ("trial_key", _d),
+                ("stored_cam", _d), ("stored_pts", _d), ("key_used", _d),
+                ("flag_fp", _d), ("flagnew_fp", _d), ("iflag", _i), ("pass", _i)]
+
+    DBL = ("lin_cam", "lin_pts", "trial_cam", "trial_pts", "trial_key", "stored_cam", "stored_pts", "key_used",
+           "flag_fp", "flagnew_fp")
+    INT = ("iflag", "pass")
 
 
 _lib = None
@@ -93,6 +113,33 @@ def exteval(scene, flags=None, fsites=0):
     return s, nobs.value, dof.value
 
 
+class mutation:
+    """Context manager planting one of the oracle's test mutations (see the
+    head of rpba_oracle.c); the pins must fail under each."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().oracle_set_mutation(self.m)
+
+    def __exit__(self, *a):
+        lib().oracle_set_mutation(0)
+
+
+class order:
+    """Context manager for a summation-order variant of the oracle."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().oracle_set_order(self.m)
+
+    def __exit__(self, *a):
+        lib().oracle_set_order(0)
+
+
 class Handle:
     """One oracle solver state, for probing single steps."""
 
@@ -111,6 +158,59 @@ class Handle:
 
     def setup_pass0(self):
         lib().orc_setup_pass0(self.h)
+
+    def adjust(self, script=None, rec_cap=0, trace_cap=512):
+        """LSA::Adjust on this state.  script: sig0 values that replace the
+        computed ones in order (the initial cost of pass 0, then every trial);
+        rec_cap > 0 records the state of each LM iteration.  Returns
+        (status, stats, log, records)."""
+        self._script = None if script is None else np.ascontiguousarray(script, dtype=np.float64)
+        if self._script is not None:
+            lib().orc_set_script(self.h, self._script.size, _p(self._script, C.c_double))
+        rec = None
+        if rec_cap:
+            self._recarr = {k: np.zeros(rec_cap, np.float64) for k in OrcRec.DBL}
+            self._recarr.update({k: np.zeros(rec_cap, np.int32) for k in OrcRec.INT})
+            rec = OrcRec(cap=rec_cap, len=0)
+            for k in OrcRec.DBL:
+                setattr(rec, k, _p(self._recarr[k], C.c_double))
+            for k in OrcRec.INT:
+                setattr(rec, k, _p(self._recarr[k], C.c_int))
+            lib().orc_set_rec(self.h, C.byref(rec))
+        tr = capi.Trace(trace_cap)
+        logc = capi.LogCollector()
+        rc = lib().orc_adjust(self.h, logc.fn, None, C.byref(tr.stats))
+        lib().orc_set_rec(self.h, None)
+        lib().orc_set_script(self.h, 0, None)
+        recs = None
+        if rec is not None:
+            recs = {k: v[:rec.len].copy() for k, v in self._recarr.items()}
+            recs["init_key"] = rec.init_key
+        return rc, tr.as_dict(), logc.lines, recs
+
+    def output(self):
+        """OutputLSAc into a copy of the block's arrays."""
+        lib().orc_output(self.h, C.byref(self.blk.c))
+        return self.blk.result()
+
+    def delete_with_flags(self, flags_q):
+        """RobustDelete with the given labels per track entry (point-major
+        order).  Returns (original indices of the kept points, file-order
+        indices of the kept observations)."""
+        d = self.dims()
+        f = np.ascontiguousarray(flags_q, dtype=np.uint8)
+        assert f.size == d["ntr"]
+        kp = np.zeros(d["npts"], np.int32)
+        ko = np.zeros(d["ntr"], np.int32)
+        cnt = np.zeros(2, np.int32)
+        lib().orc_delete_with_flags(self.h, _p(f, C.c_uint8), _p(kp, C.c_int), _p(ko, C.c_int), _p(cnt, C.c_int))
+        return kp[:cnt[0]].copy(), ko[:cnt[1]].copy()
+
+    def intrinsics(self, which):
+        """(f, d0, d1) per image: which = 0 stored, 1 the last trial."""
+        out = np.zeros((self.dims()["nimg"], 3), dtype=np.float64)
+        lib().orc_cams(self.h, which, _p(out, C.c_double))
+        return out
 
     def dims(self):
         d = (C.c_int * 8)()

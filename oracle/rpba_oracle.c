@@ -42,6 +42,24 @@
 
 #define MAXB 9
 
+/* Test knobs, set only by tests/: a planted mistake (to show that the pins of
+ * the LM control and of RobustDelete catch it) and a summation-order variant
+ * (to measure how far rounding alone moves a trajectory).  Both are 0 in
+ * every other use.
+ *   mutation 1: after a rejected step the trial cameras are restored from the
+ *               stored ones (H4 broken)
+ *   mutation 2: the first iteration of a pass is stored only if better (H5)
+ *   mutation 3: ISflagnew is copied from ISflag on every iteration (H6)
+ *   mutation 4: RobustDelete drops a point with minimagenrdel left (<=)
+ *   mutation 5: the itt cap breaks one iteration early (itt >= 200)
+ *   mutation 6: the output intrinsics come from the stored state (H19)
+ *   order 1:    S d sums its row products in the reverse column order
+ *   order 2:    the Schur pair products are fused multiply-adds */
+static int g_mut = 0;
+static int g_order = 0;
+void oracle_set_mutation(int m) { g_mut = m; }
+void oracle_set_order(int m) { g_order = m; }
+
 /* ------------------------------------------------------------------------- */
 /* small math                                                                */
 /* ------------------------------------------------------------------------- */
@@ -234,6 +252,19 @@ float oracle_upper_median(const float *v, int n)
 /* solver state                                                              */
 /* ------------------------------------------------------------------------- */
 
+/* per-LM-iteration records of the solver state (tests only): fingerprints
+ * (position-weighted sums) of the camera and point states and robust keys */
+typedef struct orc_rec {
+    int cap, len;
+    double init_key;           /* keys after the initial cost evaluation of pass 0 */
+    double *lin_cam, *lin_pts; /* the state the linearisation (CalcBundleSep) used */
+    double *trial_cam, *trial_pts, *trial_key; /* the trial after its decode and cost */
+    double *stored_cam, *stored_pts;           /* the stored state after the decision */
+    double *key_used;          /* keys Reweight2D labelled from (0: it did not run) */
+    double *flag_fp, *flagnew_fp; /* fingerprints of ISflag / ISflagnew after the decision */
+    int *iflag, *pass;
+} orc_rec;
+
 typedef struct orc {
     int fsites, robust, nraddpi, addf[3];
     int solver;              /* 0 = block-Jacobi PCG (the reference); 1 = dense Cholesky */
@@ -273,6 +304,11 @@ typedef struct orc {
     int64_t pcg_total;
     int pcg_last;
     int lasterr;
+    /* tests only: scripted sig0 values replacing the computed ones, records */
+    const double *script;
+    int script_len, script_pos;
+    orc_rec *rec;
+    double *Rs, *Cs;         /* stored cameras in the (R, C) form of eRMnew / ePCnew */
 } orc;
 
 #define FS(o, v) ((o)->fsites ? (double)(float)(v) : (double)(v))
@@ -297,7 +333,7 @@ void orc_destroy(orc *o)
     free(o->nrpv); free(o->toff); free(o->timg); free(o->tfo); free(o->wold);
     free(o->nrobvo); free(o->nro); free(o->Wij); free(o->sig0rmv); free(o->ISflag); free(o->ISflagnew);
     free(o->PM); free(o->cf); free(o->cd0); free(o->cd1); free(o->nf); free(o->nd0); free(o->nd1);
-    free(o->Rb); free(o->Cb); free(o->Rn); free(o->Cn); free(o->camparv);
+    free(o->Rb); free(o->Cb); free(o->Rn); free(o->Cn); free(o->camparv); free(o->Rs); free(o->Cs);
     free(o->XP); free(o->XPn); free(o->XX); free(o->XXn); free(o->x); free(o->ex);
     if (o->Sb) {
         for (size_t i = 0; i < (size_t)o->nimg * o->nimg; ++i) free(o->Sb[i]);
@@ -402,6 +438,8 @@ orc *orc_create(const rpba_block *io, const rpba_flags *f, int fsites)
     o->Cb = (double *)xcalloc((size_t)3 * nc, sizeof(double));
     o->Rn = (double *)xcalloc((size_t)9 * nc, sizeof(double));
     o->Cn = (double *)xcalloc((size_t)3 * nc, sizeof(double));
+    o->Rs = (double *)xcalloc((size_t)9 * nc, sizeof(double));
+    o->Cs = (double *)xcalloc((size_t)3 * nc, sizeof(double));
     oracle_rectify(o->PM, 4);
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c) o->Rn[r * 3 + c] = o->PM[r * 4 + c];
@@ -765,8 +803,14 @@ static void corrections(orc *o, int mode, double mult1, const double *XM, int sr
                 double *D = sblock(o, ij, ij);
                 double *gc = o->ex + o->camparv[ij];
                 for (int r = 0; r < ni; ++r) {
-                    for (int c = 0; c < ni; ++c)
-                        D[r * MAXB + c] -= N[r * 3 + 0] * Wa[c * 3 + 0] + N[r * 3 + 1] * Wa[c * 3 + 1] + N[r * 3 + 2] * Wa[c * 3 + 2];
+                    for (int c = 0; c < ni; ++c) {
+                        if (g_order & 2)
+                            D[r * MAXB + c] = fma(-N[r * 3 + 2], Wa[c * 3 + 2],
+                                                  fma(-N[r * 3 + 1], Wa[c * 3 + 1],
+                                                      fma(-N[r * 3 + 0], Wa[c * 3 + 0], D[r * MAXB + c])));
+                        else
+                            D[r * MAXB + c] -= N[r * 3 + 0] * Wa[c * 3 + 0] + N[r * 3 + 1] * Wa[c * 3 + 1] + N[r * 3 + 2] * Wa[c * 3 + 2];
+                    }
                     gc[r] -= N[r * 3 + 0] * g[0] + N[r * 3 + 1] * g[1] + N[r * 3 + 2] * g[2];
                 }
                 for (int iis = is + 1; iis < t; ++iis) {
@@ -777,15 +821,27 @@ static void corrections(orc *o, int mode, double mult1, const double *XM, int sr
                     if (ij < ii) {
                         double *B = sblock(o, ij, ii);
                         for (int r = 0; r < ni; ++r)
-                            for (int c = 0; c < nb; ++c)
-                                B[r * MAXB + c] -= N[r * 3 + 0] * Wb[c * 3 + 0] + N[r * 3 + 1] * Wb[c * 3 + 1] + N[r * 3 + 2] * Wb[c * 3 + 2];
+                            for (int c = 0; c < nb; ++c) {
+                                if (g_order & 2)
+                                    B[r * MAXB + c] = fma(-N[r * 3 + 2], Wb[c * 3 + 2],
+                                                          fma(-N[r * 3 + 1], Wb[c * 3 + 1],
+                                                              fma(-N[r * 3 + 0], Wb[c * 3 + 0], B[r * MAXB + c])));
+                                else
+                                    B[r * MAXB + c] -= N[r * 3 + 0] * Wb[c * 3 + 0] + N[r * 3 + 1] * Wb[c * 3 + 1] + N[r * 3 + 2] * Wb[c * 3 + 2];
+                            }
                     } else {
                         /* ij > ii: block (ii, ij) -= eNlij[iis] * eNlijN^T; ij == ii
                          * cannot occur (duplicate cameras in a track are rejected) */
                         double *B = sblock(o, ii, ij);
                         for (int r = 0; r < nb; ++r)
-                            for (int c = 0; c < ni; ++c)
-                                B[r * MAXB + c] -= Wb[r * 3 + 0] * N[c * 3 + 0] + Wb[r * 3 + 1] * N[c * 3 + 1] + Wb[r * 3 + 2] * N[c * 3 + 2];
+                            for (int c = 0; c < ni; ++c) {
+                                if (g_order & 2)
+                                    B[r * MAXB + c] = fma(-Wb[r * 3 + 2], N[c * 3 + 2],
+                                                          fma(-Wb[r * 3 + 1], N[c * 3 + 1],
+                                                              fma(-Wb[r * 3 + 0], N[c * 3 + 0], B[r * MAXB + c])));
+                                else
+                                    B[r * MAXB + c] -= Wb[r * 3 + 0] * N[c * 3 + 0] + Wb[r * 3 + 1] * N[c * 3 + 1] + Wb[r * 3 + 2] * N[c * 3 + 2];
+                            }
                     }
                 }
             }
@@ -834,7 +890,10 @@ static void spmv(int nb, int begini, const int *off, double *const *Sb, const un
             const int nk = off[k + 1] - off[k];
             for (int r = 0; r < nk; ++r) {
                 double s = 0.;
-                for (int c = 0; c < nl_; ++c) s += B[r * MAXB + c] * d[off[l] + c];
+                if (g_order & 1)
+                    for (int c = nl_ - 1; c >= 0; --c) s += B[r * MAXB + c] * d[off[l] + c];
+                else
+                    for (int c = 0; c < nl_; ++c) s += B[r * MAXB + c] * d[off[l] + c];
                 y[off[k] + r] += s;
             }
             if (k != l)
@@ -1134,6 +1193,8 @@ static void reweight(orc *o, int iflag)
 static void store(orc *o)
 {
     memcpy(o->XP, o->XPn, sizeof(double) * o->xpsize);
+    memcpy(o->Rs, o->Rn, sizeof(double) * 9 * (size_t)o->nimg);
+    memcpy(o->Cs, o->Cn, sizeof(double) * 3 * (size_t)o->nimg);
     for (int k = 0; k < o->nimg; ++k) {
         double *P = o->PM + 12 * k;
         const double *R = o->Rn + 9 * k, *C = o->Cn + 3 * k;
@@ -1160,7 +1221,8 @@ static void robust_delete(orc *o)
         int nf = 0;
         const int t = o->toff[i + 1] - o->toff[i];
         for (int q = o->toff[i]; q < o->toff[i + 1]; ++q) nf += o->ISflagnew[o->nro[q]] ? 1 : 0;
-        if (nf > 0 && t - nf < o->minimagenrdel) continue; /* point erased */
+        const int left = t - nf;
+        if (nf > 0 && (g_mut == 4 ? left <= o->minimagenrdel : left < o->minimagenrdel)) continue; /* point erased */
         const int b = nt2;
         for (int q = o->toff[i]; q < o->toff[i + 1]; ++q) {
             if (nf > 0 && o->ISflagnew[o->nro[q]]) continue;
@@ -1188,6 +1250,73 @@ static void robust_delete(orc *o)
 /* ------------------------------------------------------------------------- */
 
 static double fsq(const orc *o, double v) { return o->fsites ? (double)sqrtf((float)v) : sqrt(v); }
+
+/* position-weighted sum: equal states give equal values, different ones
+ * (practically always) different values */
+static double fp_add(double acc, const double *v, size_t n, size_t base)
+{
+    for (size_t i = 0; i < n; ++i) acc += v[i] * (1. + 1.e-3 * (double)((base + i) % 977));
+    return acc;
+}
+static double fp_cams(const orc *o, const double *R, const double *Cc, const double *f, const double *d0,
+                      const double *d1)
+{
+    const size_t n = (size_t)o->nimg;
+    double a = fp_add(0., R, 9 * n, 0);
+    a = fp_add(a, Cc, 3 * n, 9 * n);
+    a = fp_add(a, f, n, 12 * n);
+    a = fp_add(a, d0, n, 13 * n);
+    return fp_add(a, d1, n, 14 * n);
+}
+static double fp_keys(const orc *o)
+{
+    double a = 0.;
+    for (int q = 0; q < o->ntr; ++q) a += (double)o->sig0rmv[q] * (1. + 1.e-3 * (double)(q % 977));
+    return a;
+}
+static double fp_flags(const unsigned char *f, int n)
+{
+    double c = 0.;
+    for (int q = 0; q < n; ++q) c += f[q] ? (double)(q + 1) : 0.;
+    return c;
+}
+/* the next scripted sig0, if a script is set */
+static void script_sig0(orc *o)
+{
+    if (o->script && o->script_pos < o->script_len) o->sig0 = o->script[o->script_pos++];
+}
+
+void orc_set_script(orc *o, int n, const double *seq)
+{
+    o->script = seq;
+    o->script_len = n;
+    o->script_pos = 0;
+}
+void orc_set_rec(orc *o, orc_rec *rec) { o->rec = rec; }
+
+/* RobustDelete alone on the current tracks, with the labels ISflagnew given
+ * per track entry (point-major order q).  Writes the original indices of the
+ * kept points (kept_pts[0..np')) and the file-order indices of the kept
+ * observations (kept_obs[0..nt')), and np', nt' into counts. */
+void orc_delete_with_flags(orc *o, const unsigned char *flags_q, int *kept_pts, int *kept_obs, int *counts)
+{
+    for (int q = 0; q < o->ntr; ++q) o->ISflagnew[o->nro[q]] = flags_q[q] ? 1 : 0;
+    robust_delete(o);
+    for (int i = 0; i < o->npts; ++i) kept_pts[i] = o->nrpv[i];
+    for (int q = 0; q < o->ntr; ++q) kept_obs[q] = o->tfo[q];
+    counts[0] = o->npts;
+    counts[1] = o->ntr;
+}
+/* intrinsics (f, d0, d1) per image: which = 0 stored (cameras), 1 last trial
+ * (camerasnew) */
+void orc_cams(const orc *o, int which, double *out)
+{
+    for (int k = 0; k < o->nimg; ++k) {
+        out[3 * k] = which ? o->nf[k] : o->cf[k];
+        out[3 * k + 1] = which ? o->nd0[k] : o->cd0[k];
+        out[3 * k + 2] = which ? o->nd1[k] : o->cd1[k];
+    }
+}
 
 int orc_adjust(orc *o, rpba_log_fn log, void *user, rpba_stats *st)
 {
@@ -1229,7 +1358,11 @@ int orc_adjust(orc *o, rpba_log_fn log, void *user, rpba_stats *st)
             decode_cameras(o, o->XP);
             corrections(o, 0, 1. + mult, o->XP, 1);
             o->sig0 = FS(o, FS(o, o->sig0 * o->wsumg) / (double)(float)o->redundancy);
+            script_sig0(o);
             o->sig0in = o->sig0old = o->sig0;
+            memcpy(o->Rs, o->Rn, sizeof(double) * 9 * (size_t)o->nimg);
+            memcpy(o->Cs, o->Cn, sizeof(double) * 3 * (size_t)o->nimg);
+            if (o->rec) o->rec->init_key = fp_keys(o);
         } else {
             o->sig0in = o->sig0old;
         }
@@ -1237,6 +1370,9 @@ int orc_adjust(orc *o, rpba_log_fn log, void *user, rpba_stats *st)
         int iter = 1;
         const int nrunknownsgiven = o->nrunknowns, nrobsgiven = o->nrobs;
         while (iflag > 0) {
+            orc_rec *rc_ = (o->rec && o->rec->len < o->rec->cap) ? o->rec : NULL;
+            const int ri = rc_ ? rc_->len : 0;
+            if (rc_) rc_->key_used[ri] = (it == 0 && robust) ? fp_keys(o) : 0.;
             if (it == 0 && robust) {
                 o->nrunknowns = nrunknownsgiven;
                 o->nrobs = nrobsgiven;
@@ -1244,6 +1380,10 @@ int orc_adjust(orc *o, rpba_log_fn log, void *user, rpba_stats *st)
                 o->redundancy = o->nrobs - o->nrunknowns;
                 o->wsum = o->fsites ? (double)((float)(2 * o->imagepointnr) / (float)o->wsum) : (double)(2 * o->imagepointnr) / o->wsum;
                 o->wsumg = o->wsum;
+            }
+            if (rc_) {
+                rc_->lin_cam[ri] = fp_cams(o, o->Rn, o->Cn, o->nf, o->nd0, o->nd1);
+                rc_->lin_pts[ri] = fp_add(0., o->XP + o->nrioff, 3 * (size_t)o->npts, 0);
             }
             bundle_sep(o, mult);
             for (int i = 0; i < o->nrunknowns; ++i) {
@@ -1259,12 +1399,34 @@ int orc_adjust(orc *o, rpba_log_fn log, void *user, rpba_stats *st)
             decode_cameras(o, o->XPn);
             corrections(o, 0, 1. + mult, o->XPn, 1);
             o->sig0 = FS(o, FS(o, o->sig0 * o->wsumg) / (double)(float)o->redundancy);
+            script_sig0(o);
+            if (rc_) {
+                rc_->trial_cam[ri] = fp_cams(o, o->Rn, o->Cn, o->nf, o->nd0, o->nd1);
+                rc_->trial_pts[ri] = fp_add(0., o->XPn + o->nrioff, 3 * (size_t)o->npts, 0);
+                rc_->trial_key[ri] = fp_keys(o);
+            }
             int acc = 0;
             if (o->sig0old < FS(o, o->sig0 * frac)) --iflag;
-            if (o->sig0 <= o->sig0old || iter == 1) {
+            if (o->sig0 <= o->sig0old || (iter == 1 && g_mut != 2)) {
                 store(o);
                 o->sig0old = o->sig0;
                 acc = 1;
+            } else if (g_mut == 1) {
+                memcpy(o->Rn, o->Rs, sizeof(double) * 9 * (size_t)o->nimg);
+                memcpy(o->Cn, o->Cs, sizeof(double) * 3 * (size_t)o->nimg);
+                memcpy(o->nf, o->cf, sizeof(double) * (size_t)o->nimg);
+                memcpy(o->nd0, o->cd0, sizeof(double) * (size_t)o->nimg);
+                memcpy(o->nd1, o->cd1, sizeof(double) * (size_t)o->nimg);
+            }
+            if (!acc && g_mut == 3) memcpy(o->ISflagnew, o->ISflag, (size_t)o->ntr);
+            if (rc_) {
+                rc_->stored_cam[ri] = fp_cams(o, o->Rs, o->Cs, o->cf, o->cd0, o->cd1);
+                rc_->stored_pts[ri] = fp_add(0., o->XX, 3 * (size_t)o->npts, 0);
+                rc_->flag_fp[ri] = fp_flags(o->ISflag, o->ntr);
+                rc_->flagnew_fp[ri] = fp_flags(o->ISflagnew, o->ntr);
+                rc_->iflag[ri] = iflag;
+                rc_->pass[ri] = it;
+                rc_->len++;
             }
             if (st && st->trace_sig0 && st->trace_len < st->trace_cap) {
                 st->trace_sig0[st->trace_len] = o->sig0;
@@ -1273,7 +1435,7 @@ int orc_adjust(orc *o, rpba_log_fn log, void *user, rpba_stats *st)
                 if (st->trace_nrobs) st->trace_nrobs[st->trace_len] = o->nrobs;
                 st->trace_len++;
             }
-            if (itt > 200) break;
+            if (g_mut == 5 ? itt >= 200 : itt > 200) break;
             ++itt;
             ++iter;
         }
@@ -1321,9 +1483,9 @@ void orc_output(orc *o, rpba_block *io)
     }
     for (int k = 0; k < o->nimg; ++k) {
         for (int j = 0; j < 12; ++j) io->cam_P[12 * k + j] = (float)o->PM[12 * k + j];
-        io->cam_f[k] = (float)o->nf[k];
-        io->cam_d[2 * k] = (float)o->nd0[k];
-        io->cam_d[2 * k + 1] = (float)o->nd1[k];
+        io->cam_f[k] = (float)(g_mut == 6 ? o->cf[k] : o->nf[k]);
+        io->cam_d[2 * k] = (float)(g_mut == 6 ? o->cd0[k] : o->nd0[k]);
+        io->cam_d[2 * k + 1] = (float)(g_mut == 6 ? o->cd1[k] : o->nd1[k]);
     }
 }
 

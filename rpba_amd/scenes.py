@@ -214,6 +214,28 @@ def generate(name=None, *, seed=1, n_cams=50, n_pts=5000, track="geometric", tra
     }
 
 
+def rms_px(scene, out):
+    """Per-coordinate RMS reprojection error in pixels, sqrt(sum |r|^2 / 2n),
+    over the kept observations of a solved block (its output R|t, points and
+    intrinsics), with the reference's camera model (rbacore.cpp:1238-1289;
+    the residual is xg - p with p = f s (P X)_xy / (P X)_z)."""
+    kept = np.asarray(out["obs_kept"]).astype(bool)
+    cam = scene["obs_cam"][kept]
+    P = np.asarray(out["cam_P"], dtype=np.float64).reshape(-1, 3, 4)[cam]
+    X = np.asarray(out["pts"], dtype=np.float64).reshape(-1, 3)[scene["obs_pt"][kept]]
+    e = np.einsum("nij,nj->ni", P[:, :, :3], X) + P[:, :, 3]
+    z = e[:, 2]
+    iz = np.where(np.abs(z) < 1e-3, np.where(z < 0, -1e3, 1e3), 1.0 / np.where(z == 0, 1.0, z))
+    u, v = e[:, 0] * iz, e[:, 1] * iz
+    rho = u * u + v * v
+    d = np.asarray(out["cam_d"], dtype=np.float64).reshape(-1, 2)[cam]
+    f = np.asarray(out["cam_f"], dtype=np.float64)[cam]
+    sfac = 1.0 + d[:, 0] * rho + d[:, 1] * rho * rho
+    xy = np.asarray(scene["obs_xy"], dtype=np.float64).reshape(-1, 2)[kept]
+    r = xy - np.stack([f * sfac * u, f * sfac * v], axis=1)
+    return float(np.sqrt((r * r).sum() / max(2 * r.shape[0], 1)))
+
+
 def write_bal(path, scene):
     """BAL text in the layout InputBlock reads (rpba.cpp:195-279): header,
     observations `cam pt x y` (file x = -xg), cameras `a b c t0 t1 t2 f k1 k2`

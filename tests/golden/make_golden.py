@@ -29,6 +29,12 @@ CASES = {
     "c2_robust": dict(scene=("c2_961", {}), flags=dict(robust=1)),
 }
 
+# Large scenes: the fixture also keeps a strided sample of the solved cameras
+# and points, so the GPU forms' parameter errors can be stated against it.
+LARGE = {
+    "c3_robust": dict(scene=("c3_4585", {}), flags=dict(robust=1), cam_stride=23, pt_stride=997),
+}
+
 
 def make_scene(spec):
     name, kw = spec
@@ -40,7 +46,7 @@ def digest(a):
 
 
 def run(case):
-    c = CASES[case]
+    c = CASES[case] if case in CASES else LARGE[case]
     sc = make_scene(c["scene"])
     flags = capi.make_flags(**c["flags"])
     t = time.time()
@@ -65,6 +71,15 @@ def run(case):
         "cam_t_first": [float(x) for x in out["cam_P"][:4, :, 3].ravel()],
         "log": log, "oracle_seconds": round(dt, 2),
     }
+    if "cam_stride" in c:
+        cs, ps = c["cam_stride"], c["pt_stride"]
+        fix["cam_stride"], fix["pt_stride"] = cs, ps
+        fix["cam_P_sample"] = out["cam_P"][::cs].astype(np.float64).ravel().tolist()
+        fix["cam_f_sample"] = out["cam_f"][::cs].astype(np.float64).tolist()
+        fix["cam_d_sample"] = out["cam_d"][::cs].astype(np.float64).ravel().tolist()
+        fix["pts_sample"] = out["pts"][::ps].astype(np.float64).ravel().tolist()
+        fix["sig0_final"] = float(st["sig0_final"])
+        fix["rms_px"] = scenes.rms_px(sc, out)
     with open(os.path.join(HERE, f"{case}.json"), "w") as fh:
         json.dump(fix, fh, indent=1)
     print(case, "itt", st["itt"], "seconds", round(dt, 1), flush=True)
