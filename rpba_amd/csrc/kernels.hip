@@ -2070,8 +2070,10 @@ __global__ __launch_bounds__(256) void k_batch_store(BatchArgs b, const unsigned
                                                      const double *Xn, double *X, const unsigned char *flag,
                                                      unsigned char *flagst)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < b.nc && accept[b.camblk[j]]) {
+    // grid-stride over cameras, points and entries: the launch grid is capped
+    const int j0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int j = j0; j < b.nc; j += stride) {
+        if (!accept[b.camblk[j]]) continue;
         const double *R = cp.Rn + 9 * (size_t)j, *C = cp.Cn + 3 * (size_t)j;
         double *P = cp.Pst + 12 * (size_t)j;
         for (int r = 0; r < 3; ++r) {
@@ -2081,9 +2083,10 @@ __global__ __launch_bounds__(256) void k_batch_store(BatchArgs b, const unsigned
         for (int s = 0; s < 3; ++s) cp.Ist[3 * (size_t)j + s] = cp.In[3 * (size_t)j + s];
         for (int s = 0; s < 9; ++s) cp.th[9 * (size_t)j + s] = cp.thn[9 * (size_t)j + s];
     }
-    if (j < b.npts && accept[b.ptblk[j]])
-        for (int s = 0; s < 3; ++s) X[3 * (size_t)j + s] = Xn[3 * (size_t)j + s];
-    for (int q = j; q < b.ntr; q += gridDim.x * blockDim.x)
+    for (int j = j0; j < b.npts; j += stride)
+        if (accept[b.ptblk[j]])
+            for (int s = 0; s < 3; ++s) X[3 * (size_t)j + s] = Xn[3 * (size_t)j + s];
+    for (int q = j0; q < b.ntr; q += stride)
         if (accept[b.ptblk[b.tpt[q]]]) flagst[q] = flag[q];
 }
 

@@ -33,17 +33,19 @@ BATCH_CONFIG = dict(seed=4, n_blocks=64, cam_lo=20, cam_hi=400, pts_per_cam=150,
                     track_mean=6.0, outlier_frac=0.02)
 
 
-def generate_batch(seed=4, n_blocks=64, cam_lo=20, cam_hi=400, pts_per_cam=150, pts_lo=2000, pts_hi=60000,
-                   track_mean=6.0, outlier_frac=0.02):
-    """Independent blocks of uneven size (the batched-blocks workload)."""
+def batch_block(b, seed=4, n_blocks=64, cam_lo=20, cam_hi=400, pts_per_cam=150, pts_lo=2000, pts_hi=60000,
+                track_mean=6.0, outlier_frac=0.02):
+    """Block b of the batched-blocks workload alone."""
     rng = np.random.default_rng(seed)
-    ncams = np.round(np.exp(rng.uniform(np.log(cam_lo), np.log(cam_hi), n_blocks))).astype(int)
-    blocks = []
-    for b, nc in enumerate(ncams.tolist()):
-        npt = int(np.clip(pts_per_cam * nc, pts_lo, pts_hi))
-        blocks.append(generate(None, seed=1000 * seed + b, n_cams=nc, n_pts=npt, track="geometric",
-                               track_mean=track_mean, outlier_frac=outlier_frac))
-    return blocks
+    nc = int(np.round(np.exp(rng.uniform(np.log(cam_lo), np.log(cam_hi), n_blocks)))[b])
+    npt = int(np.clip(pts_per_cam * nc, pts_lo, pts_hi))
+    return generate(None, seed=1000 * seed + b, n_cams=nc, n_pts=npt, track="geometric",
+                    track_mean=track_mean, outlier_frac=outlier_frac)
+
+
+def generate_batch(seed=4, n_blocks=64, **kw):
+    """Independent blocks of uneven size (the batched-blocks workload)."""
+    return [batch_block(b, seed=seed, n_blocks=n_blocks, **kw) for b in range(n_blocks)]
 
 
 def rodrigues(aa):
@@ -140,9 +142,16 @@ def _tracks(rng, n_pts, n_cams, lengths):
 def generate(name=None, *, seed=1, n_cams=50, n_pts=5000, track="geometric", track_mean=9.0,
              outlier_frac=0.02, noise_px=0.5, outlier_px=50.0, f0=500.0, f_spread=0.05,
              k1_std=1e-3, k2_std=1e-5, rot_noise=0.01, t_noise=0.05, x_noise=0.05,
-             radius=10.0, ball=3.0, cam_window=None):
+             radius=10.0, ball=3.0, cam_window=None, weights=None, img_size=None):
     """Return a scene dict (numpy arrays) in the reference's data model plus the
-    ground truth.  ``name`` selects one of CONFIGS."""
+    ground truth.  ``name`` selects one of CONFIGS.
+
+    weights="random": per-observation 2x2 information matrices (wx, wy, wxy)
+    with wx, wy in [0.5, 2] and the correlation wxy / sqrt(wx wy) in
+    [-0.6, 0.6]; the pixel noise is drawn with covariance noise_px^2 W^-1.
+    img_size=(w, h): image sizes per camera (w + k mod 7, h), which set ihw2
+    and the residual clamp 5000 / sqrt(ihw2) (rpbacore.cpp:87-90,
+    rbacore.cpp:48)."""
     if name is not None:
         cfg = dict(CONFIGS[name])
         seed = cfg.pop("seed")
@@ -182,7 +191,22 @@ def generate(name=None, *, seed=1, n_cams=50, n_pts=5000, track="geometric", tra
     rho = u * u + v * v
     s = 1.0 + k1[obs_cam] * rho + k2[obs_cam] * rho * rho
     xg = np.stack([f_true[obs_cam] * s * u, f_true[obs_cam] * s * v], axis=1)
-    xg += rng.normal(0.0, noise_px, xg.shape)
+    if weights == "random":
+        wa = rng.uniform(0.5, 2.0, n_obs)
+        wb = rng.uniform(0.5, 2.0, n_obs)
+        wc = rng.uniform(-0.6, 0.6, n_obs) * np.sqrt(wa * wb)
+        obs_w = np.stack([wa, wb, wc], axis=1).astype(np.float32)
+        # noise with covariance W^-1: x = L^-T z for W = L L^T
+        l11 = np.sqrt(wa)
+        l21 = wc / l11
+        l22 = np.sqrt(wb - l21 * l21)
+        z = rng.normal(0.0, noise_px, (n_obs, 2))
+        e2 = z[:, 1] / l22
+        e1 = (z[:, 0] - l21 * e2) / l11
+        xg += np.stack([e1, e2], axis=1)
+    else:
+        obs_w = None
+        xg += rng.normal(0.0, noise_px, xg.shape)
     n_out = int(round(outlier_frac * n_obs))
     out_idx = rng.choice(n_obs, n_out, replace=False)
     xg[out_idx] += rng.uniform(-outlier_px, outlier_px, (n_out, 2))
@@ -199,10 +223,14 @@ def generate(name=None, *, seed=1, n_cams=50, n_pts=5000, track="geometric", tra
     cam_P = np.concatenate([R0f, t0[:, :, None]], axis=2)
 
     obs_xy = xg.astype(np.float32)  # reference xg (already -file xy)
-    return {
+    extra = {}
+    if img_size is not None:
+        extra["img_w"] = (int(img_size[0]) + np.arange(n_cams) % 7).astype(np.int32)
+        extra["img_h"] = np.full(n_cams, int(img_size[1]), np.int32)
+    return {**extra,
         "n_cams": n_cams, "n_pts": n_pts, "n_obs": n_obs,
         "obs_cam": obs_cam.astype(np.int32), "obs_pt": obs_pt,
-        "obs_xy": obs_xy, "obs_w": None,
+        "obs_xy": obs_xy, "obs_w": obs_w,
         "cam_aa": aa0, "cam_t": t0,
         "cam_P": cam_P.astype(np.float32),
         "cam_f": f_true.astype(np.float32),

@@ -64,6 +64,31 @@ def test_exteval_matches_oracle(scene_c1):
     assert np.array_equal(out["pts"], scene_c1["pts"])
 
 
+def _weighted_c1():
+    """The 50-camera scene with per-observation 2x2 weights (cross term
+    included) and image sizes: ihw2 = 563^2 scales sigma and clamps residual
+    components at 8.9 px, which the planted 50 px outliers reach."""
+    return scenes.generate(None, seed=1, n_cams=50, n_pts=5000, track="uniform2_8", outlier_frac=0.02,
+                           weights="random", img_size=(640, 480))
+
+
+@pytest.mark.parametrize("robust", [1, 0])
+def test_weighted_scene_with_image_sizes_matches_oracle(robust):
+    sc = _weighted_c1()
+    st0, st1, _ = _cmp_solves(sc, capi.make_flags(robust=robust), f"weighted c1 robust={robust}")
+    assert st0["sigma_out"][1] > 100.0  # sigma in the ihw2 scale
+
+
+def test_weighted_exteval_matches_oracle():
+    sc = _weighted_c1()
+    _say("[exteval weighted c1]")
+    fl = capi.make_flags(exteval=1)
+    s0, n0, d0 = oracle.exteval(sc, fl, fsites=0)
+    rc, out, st, log = api.calc_plsa(sc, fl)
+    assert rc == 0 and st["exteval_nobs"] == n0 and st["exteval_dof"] == d0
+    assert abs(st["exteval_sigma"] - s0) <= 1e-12 * s0
+
+
 def test_robust_solve_c1_matches_oracle(scene_c1):
     st0, st1, out1 = _cmp_solves(scene_c1, capi.make_flags(robust=1), "c1 robust")
     assert abs(st1["sigma_out"][1] - 0.5) < 0.02

@@ -193,6 +193,79 @@ def test_schur_matches_dense_elimination(tiny_scene, mult):
     assert np.allclose(x[nc:], xp, rtol=1e-4, atol=1e-6 * np.abs(xp).max())
 
 
+def _weighted_tiny(img_size=None):
+    return scenes.generate(None, seed=13, n_cams=6, n_pts=60, track="uniform2_8", outlier_frac=0.0,
+                           weights="random", img_size=img_size)
+
+
+def _weight_blocks(scene):
+    """Block-diagonal observation weight matrix [[wx, wxy], [wxy, wy]] in the
+    file order of the observations (PrepareWeights2D, rbacore.cpp:704-783)."""
+    w = scene["obs_w"].astype(np.float64)
+    n = w.shape[0]
+    Wm = np.zeros((2 * n, 2 * n))
+    for q in range(n):
+        Wm[2 * q:2 * q + 2, 2 * q:2 * q + 2] = [[w[q, 0], w[q, 2]], [w[q, 2], w[q, 1]]]
+    return Wm
+
+
+def test_weighted_normal_equations_match_dense_elimination():
+    """Non-unit weights with a cross term: U/V/W/g and the Schur step equal
+    the dense elimination of J^T W J (the wxy term enters Bc^T W, Bp^T W and
+    both off-diagonal entries, rbacore.cpp:1540-1609)."""
+    sc = _weighted_tiny()
+    h = _probe(sc)
+    d = h.dims()
+    r, _ = h.residuals(h.theta())
+    J = _dense_jacobian(h, sc)
+    Wm = _weight_blocks(sc)
+    nc = d["nrioff"]
+    mult = 1e-3
+    N = J.T @ Wm @ J
+    g = J.T @ Wm @ r
+    Ucc, Ncp, Npp = N[:nc, :nc], N[:nc, nc:], N[nc:, nc:].copy()
+    Npp[np.diag_indices_from(Npp)] *= 1.0 + mult
+    Vinv = np.linalg.inv(Npp)
+    S = Ucc - Ncp @ Vinv @ Ncp.T
+    S[np.diag_indices_from(S)] *= 1.0 + mult
+    S[np.diag_indices_from(S)] += 1e-2
+    b = g[:nc] - Ncp @ Vinv @ g[nc:]
+    So, bo = h.linearize(mult)
+    assert np.abs(So - S).max() < 1e-9 * np.abs(S).max()
+    assert np.abs(bo - b).max() < 1e-9 * np.abs(b).max()
+    # the same scene with the cross term dropped gives another S: the term matters
+    sc0 = dict(sc, obs_w=sc["obs_w"] * np.array([1, 1, 0], np.float32))
+    S0, _ = _probe(sc0).linearize(mult)
+    assert np.abs(S0 - So).max() > 1e-4 * np.abs(So).max()
+
+
+def test_weighted_keys_and_residual_clamp():
+    """The robust key of an observation is r^T W r under its original
+    weights (sig0rmv, rbacore.cpp:1312-1327), and residual components are
+    clamped at 5000 / sqrt(ihw2) with ihw2 the mean of ((w + h) / 2)^2 over the
+    images (rbacore.cpp:48, 1273-1289; rpbacore.cpp:87-90)."""
+    sc = _weighted_tiny(img_size=(640, 480))
+    xy = sc["obs_xy"].copy()
+    xy[:5, 0] += 60.0  # far beyond the clamp
+    xy[5:10, 1] -= 60.0
+    sc = dict(sc, obs_xy=xy)
+    h = _probe(sc)
+    r, key = h.residuals(h.theta())
+    r = r.reshape(-1, 2)
+    w = sc["obs_w"].astype(np.float64)
+    quad = r[:, 0] ** 2 * w[:, 0] + 2 * r[:, 0] * r[:, 1] * w[:, 2] + r[:, 1] ** 2 * w[:, 1]
+    # keys are float: the float rounding of the residuals and products only
+    assert np.allclose(key, quad, rtol=2e-6, atol=0)
+    ihw2 = np.float32(0)
+    for wi, hi in zip(sc["img_w"], sc["img_h"]):
+        ihw2 += np.float32((int(hi) + int(wi)) ** 2) * np.float32(0.25)
+    ihw2 /= np.float32(sc["n_cams"])
+    clamp = 5000.0 / float(np.sqrt(ihw2))
+    assert 8.0 < clamp < 9.5
+    assert np.all(np.abs(r) <= clamp)
+    assert np.all(r[:5, 0] == clamp) and np.all(r[5:10, 1] == -clamp)
+
+
 # --------------------------------------------------------------------------
 # PCG: against dense Cholesky and an independent replay of the rule
 # --------------------------------------------------------------------------
