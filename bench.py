@@ -1,10 +1,11 @@
 #!/usr/bin/env python3
 """Benchmark: plain-FP64 robust LM bundle adjustment on one MI355X.
 
-Workload: the 961-camera synthetic scene (187,103 points, ~1.69M
-observations, track length 2 + geometric with mean 9, 0.5 px noise, 3%
+Workload (default): the 13,682-camera synthetic scene (4.46M points, 29.0M
+observations, track length 2 + geometric with mean 6.5, 0.5 px noise, 3%
 outliers, k2/k4/f estimated, Robust = 1, Threads = 1 semantics), generated
-with seed 2 and resident in HBM before timing starts.
+with seed 5 and resident in HBM before timing starts.  --config c2_961 and
+c3_4585 run the smaller scenes; --config c4_batch the batched blocks.
 
 A step is one LM iteration of the reference's Adjust loop (rbacore.cpp:515-581:
 reweighting, linearisation + Schur + PCG + back-substitution, trial decode and
@@ -14,13 +15,15 @@ the timed region it restarts from the initial state (rpba_ctx_reset) and the
 restart is timed too.  value = K / elapsed (LM iterations per second).
 
 Also reported: time to converged (one full solve: both passes, from the
-uploaded block to the stored state on the device), the final sigma0 and RMS,
-the other forms (FP32 Jacobians and W, Y = W V^-1 records, the matrix-free
-Schur product, the learned correction of the block-Jacobi preconditioner, and
-combinations): each one's time to reach the plain form's final sigma0 and its
-errors against the plain FP64 result, the fastest of them that reaches it as
-"chosen"; the per-kernel HIP-event roofline of the dominant kernel, and the
-CPU oracle timed on one LM iteration of the same scene (cpu_baseline).
+uploaded block to the stored state on the device) and the whole drop-in call
+from host buffers (rpba_calc_plsa: PointXXvIn, uploads, solve, download); the
+final sigma0 and RMS; the other forms (FP32 Jacobians and W, Y = W V^-1
+records, the matrix-free Schur product, the learned correction of the
+block-Jacobi preconditioner, and combinations), each under a PCG-iteration and
+wall-time budget, with its time to reach the plain form's final sigma0 and its
+errors against plain FP64, the fastest that reaches it as "chosen"; the
+per-kernel HIP-event roofline of the dominant kernel; and the CPU oracle timed
+on a bounded sample of one LM iteration of the same scene (cpu_baseline).
 """
 import argparse
 import ctypes as C
@@ -78,6 +81,17 @@ def algorithmic_bytes(kernel, sz):
     return None
 
 
+# forms with an open defect on a scene: not evaluated there (the reason is
+# reported in the line)
+FORM_SKIP = {
+    ("c5_13682", "fp32_records"): "FP32 W / Y records: the first pass-1 PCG does not meet the step-norm rule on "
+                                  "this scene (S is not consistently rounded; see DESIGN)",
+    ("c3_4585", "fp32_learned"): "FP32 W / Y records with the learned preconditioner: the first pass-1 PCG does "
+                                 "not meet the step-norm rule on this scene (see DESIGN)",
+}
+FORM_WALL_S = 60.0  # a form evaluation stops after this much wall time
+
+
 def run_bench(args):
     import numpy as np
 
@@ -88,11 +102,12 @@ def run_bench(args):
     t0 = time.time()
     sc = scenes.generate(args.config)
     gen_s = time.time() - t0
-    progress(f"generated {args.config} in {gen_s:.1f} s")
+    progress(f"generated {args.config} ({sc['n_cams']} cameras, {sc['n_obs']} observations) in {gen_s:.1f} s")
     flags = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, schur=args.schur)
     t0 = time.time()
     ctx = api.Context(sc, flags)
     create_s = time.time() - t0
+    progress(f"context created in {create_s:.1f} s")
 
     def run_steps(n):
         left = n
@@ -115,37 +130,56 @@ def run_bench(args):
     run_steps(args.steps)
     device_sync()
     elapsed = time.perf_counter() - t0
+    progress(f"timed {args.steps} LM iterations: {args.steps / elapsed:.2f} LM iters/s")
 
     # time to converged: one whole solve from the uploaded block
     ctx.reset()
-    ttc, recs = timed_solve(ctx)
+    ttc, recs = timed_solve(ctx, "plain")
     rc_d, out, st = ctx.download()
     sizes = ctx.sizes()
-    progress(f"timed {args.steps} LM iterations; whole solve {ttc:.3f} s")
+    progress(f"whole plain solve {ttc:.3f} s, sigma0 {st['sigma_out'][1]:.6f}")
+    pcg_per_it = [int(x) for x in st["trace_pcg"]]
+
+    # the drop-in call from host buffers: PointXXvIn, uploads, the solve and
+    # the download (rpba_calc_plsa), as the reference's runtime window
+    # (rpba.cpp:127-140) times CalcPLSA
+    t0 = time.perf_counter()
+    rc_c, out_c, st_c, _ = api.calc_plsa(sc, flags)
+    call_s = time.perf_counter() - t0
+    progress(f"rpba_calc_plsa from host buffers {call_s:.3f} s (status {rc_c})")
 
     # the other forms: their time until the pass-1 stored sigma0 reaches the
     # plain form's final value (x (1 + 1e-6)) and their errors against plain
     # FP64 (FP32 records, the matrix-free Schur product, the learned
     # correction of block-Jacobi, and their combinations)
     target = st["sig0_final"] * (1.0 + 1e-6)
-    f64 = lambda a: np.asarray(a, dtype=np.float64)
+    f64 = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
     lp_model = None
     from rpba_amd import lp_train
     if os.path.exists(lp_train.WEIGHTS):
         lp_model = api.LpModel.load(lp_train.WEIGHTS)
+    # every PCG of a form may run 8x the plain solve's longest one (at least
+    # 200 iterations) before the evaluation stops as "budget reached"
+    pcg_budget = max(200, 8 * max(pcg_per_it or [25]))
 
-    def eval_form(fl, desc):
+    def eval_form(name, fl, desc):
         cctx = api.Context(sc, fl)
         if fl.preconditioner == 2:
             cctx.set_lp(lp_model)
-        cctx.run(1)
+        cctx.set_pcg_budget(pcg_budget)
+        rc, _, _ = cctx.run(1)  # warm-up of the form's kernels
         cctx.reset()
-        cttc, crecs = timed_solve(cctx)
+        res = {"form": desc}
+        try:
+            cttc, crecs = timed_solve(cctx, name, wall_limit=FORM_WALL_S)
+        except FormStopped as e:
+            cctx.close()
+            progress(f"form {name}: stopped ({e})")
+            return dict(res, stopped=str(e), time_to_reach_plain_sigma0_s=None)
         _, cout, cst = cctx.download()
         cctx.close()
         reach = next((t for t, ps, sg in crecs if ps == 1 and sg <= target), None)
-        return {
-            "form": desc,
+        return dict(res, **{
             "time_to_converged_s": round(cttc, 4),
             "time_to_reach_plain_sigma0_s": None if reach is None else round(reach, 4),
             "speedup_vs_plain": None if reach is None else round(ttc / reach, 3),
@@ -160,37 +194,43 @@ def run_bench(args):
                 "cam_t_max_abs": float(np.abs(f64(cout["cam_P"])[:, :, 3] - f64(out["cam_P"])[:, :, 3]).max()),
                 "label_diffs": int((cout["obs_kept"] != out["obs_kept"]).sum()),
             },
-        }
+        })
 
     # candidate forms, each a whole solve; the chosen one is the fastest to
     # reach the plain form's final sigma0 (x (1 + 1e-6))
     chosen = learned = None
     forms = {}
+    skipped = {}
     if not args.no_chosen:
         mf_ok = not args.schur
+        base = dict(robust=1, k2=1, k4=1, f=1, threads=1)
         cands = [
-            ("fp32_records", capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, precision=1),
+            ("fp32_records", capi.make_flags(**base, precision=1),
              "precision=1: FP32 Jacobians and W, Y = W V^-1 records; FP64 residuals, sums, S, PCG, cost"),
-            ("fp32_matrix_free", capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, precision=1, schur=1),
+            ("fp32_matrix_free", capi.make_flags(**base, precision=1, schur=1),
              "precision=1, schur=1: FP32 records, matrix-free Schur product (off-diagonal S never formed)"),
-            ("fp64_matrix_free", capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, schur=1),
+            ("fp64_matrix_free", capi.make_flags(**base, schur=1),
              "schur=1: FP64 matrix-free Schur product (off-diagonal S never formed)"),
         ]
         if lp_model is not None:
             cands += [
-                ("fp64_learned", capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, preconditioner=2),
+                ("fp64_learned", capi.make_flags(**base, preconditioner=2),
                  "preconditioner=2: FP64 with the learned correction of block-Jacobi"),
-                ("fp32_learned", capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, precision=1,
-                                                 preconditioner=2),
+                ("fp32_learned", capi.make_flags(**base, precision=1, preconditioner=2),
                  "precision=1, preconditioner=2: FP32 records with the learned correction of block-Jacobi"),
             ]
         for name, fl, desc in cands:
             if fl.schur and not mf_ok:
                 continue
-            forms[name] = eval_form(fl, desc)
-            progress(f"form {name} solved in {forms[name]['time_to_converged_s']} s")
+            if (args.config, name) in FORM_SKIP:
+                skipped[name] = FORM_SKIP[(args.config, name)]
+                progress(f"form {name}: skipped ({skipped[name]})")
+                continue
+            forms[name] = eval_form(name, fl, desc)
+            progress(f"form {name} done: {forms[name].get('time_to_converged_s')} s, reaches plain sigma0 at "
+                     f"{forms[name].get('time_to_reach_plain_sigma0_s')} s")
         reached = [(v["time_to_reach_plain_sigma0_s"], k) for k, v in forms.items()
-                   if v["time_to_reach_plain_sigma0_s"] is not None]
+                   if v.get("time_to_reach_plain_sigma0_s") is not None]
         if reached:
             t_best, k_best = min(reached)
             chosen = dict(forms[k_best], name=k_best)
@@ -224,6 +264,14 @@ def run_bench(args):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
+    def form_summary(v):
+        keep = ("time_to_converged_s", "time_to_reach_plain_sigma0_s", "speedup_vs_plain", "itt", "pcg_iters",
+                "sigma0_final", "stopped")
+        d = {kk: v[kk] for kk in keep if kk in v}
+        if "err_vs_plain" in v:
+            d["sigma0_rel_vs_plain"] = v["err_vs_plain"]["sigma0_rel"]
+        return d
+
     line = {
         "metric": "LM iters/s, plain FP64 robust LM Schur solve (time-to-converged reported beside)",
         "form": "plain FP64, " + ("matrix-free Schur product" if args.schur else "S formed block by block"),
@@ -244,16 +292,16 @@ def run_bench(args):
                    "pairs": sz0["pairs"], "s_blocks": sz0["blocks"], "s_dim": sz0["sdim"],
                    "parallelism": "single GPU"},
         "time_to_converged_s": round(ttc, 4),
+        "rpba_calc_plsa_s": round(call_s, 4),
         "itt": st["itt"],
         "pcg_iters": st["pcg_iters"],
+        "pcg_per_lm_iteration": pcg_per_it,
         "sigma0_final": st["sigma_out"][1],
         "rms_px": st["rms_px"],
         "chosen": chosen,
         "learned_fp64": learned,
-        "forms": {k: {kk: v[kk] for kk in ("time_to_converged_s", "time_to_reach_plain_sigma0_s", "speedup_vs_plain",
-                                            "itt", "pcg_iters", "sigma0_final")} | {"sigma0_rel_vs_plain":
-                                                                                    v["err_vs_plain"]["sigma0_rel"]}
-                  for k, v in forms.items()},
+        "forms": {k: form_summary(v) for k, v in forms.items()},
+        "forms_skipped": skipped,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktimes.items() if v[1] > 0},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -261,7 +309,8 @@ def run_bench(args):
         "setup_s": {"generate": round(gen_s, 2), "create": round(create_s, 2)},
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(sc)
+        k_avg = st["pcg_iters"] / max(st["itt"], 1)
+        line["cpu_baseline"] = cpu_baseline(sc, sizes, k_avg, args.config)
     print(json.dumps(line), flush=True)
 
 
@@ -381,42 +430,87 @@ def run_batch_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def timed_solve(ctx):
-    """One whole solve, one LM iteration per call: returns the wall time and,
-    per iteration, (elapsed, pass the iteration ran in, stored sig0 after)."""
+class FormStopped(Exception):
+    pass
+
+
+def timed_solve(ctx, name="", wall_limit=None):
+    """One whole solve, one LM iteration per call (a progress line each):
+    returns the wall time and, per iteration, (elapsed, pass the iteration
+    ran in, stored sig0 after).  Raises FormStopped when the run reports the
+    PCG budget or the wall limit passes."""
     device_sync()
     t0 = time.perf_counter()
     recs = []
     while True:
         ps, _, _ = ctx.progress()
         rc, done, fin = ctx.run(1)
+        if rc == -13:  # RPBA_E_BUDGET
+            raise FormStopped("a PCG reached the evaluation's iteration budget")
         if rc != 0:
             raise RuntimeError(f"rpba_ctx_run failed with status {rc}")
         device_sync()
-        _, _, sg = ctx.progress()
-        recs.append((time.perf_counter() - t0, ps, sg))
+        _, itt, sg = ctx.progress()
+        el = time.perf_counter() - t0
+        recs.append((el, ps, sg))
+        progress(f"{name} LM iteration {len(recs)} (pass {ps}): stored sig0 {sg:.6g}, {el:.3f} s")
         if fin:
             break
+        if wall_limit is not None and el > wall_limit:
+            raise FormStopped(f"wall limit of {wall_limit:.0f} s")
     return time.perf_counter() - t0, recs
 
 
-def cpu_baseline(sc):
-    """The CPU oracle (single-threaded C restatement of the reference's LSA)
-    on a bounded sample: one LM iteration of pass 0 of the same scene
-    (linearisation + Schur + PCG + back-substitution, then the trial cost)."""
+def host_info():
+    """nproc, the CPU model and the RAM of the host the CPU baseline ran on."""
+    model, mem_gb = None, None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as fh:
+            for ln in fh:
+                if ln.startswith("MemTotal"):
+                    mem_gb = round(int(ln.split()[1]) / 1024 ** 2, 1)
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "ram_gb": mem_gb}
+
+
+def cpu_baseline(sc, sizes, k_avg, config, target_s=12.0):
+    """The CPU oracle (single-threaded C restatement of the reference's global
+    LSA, the rbacore path) on a bounded sample of one pass-0 LM iteration of
+    the same scene: the per-point work of CalcCorrections (linearisation with
+    the Schur pair loop, back-substitution, cost) over every s-th point, and
+    two S d products over the blocks that sample formed.  Scaled to the whole
+    iteration: s x the per-point time, plus the per-block S d time x the
+    blocks of the full S x the plain solve's PCG iterations per LM iteration."""
     from oracle import oracle
     from rpba_amd import capi
 
-    h = oracle.Handle(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1))
-    h.setup_pass0()
+    # stride from the pair count: about target_s of per-point work at ~0.7 us
+    # per pair
+    stride = max(1, int(round(sizes["pairs"] * 0.7e-6 / target_s)))
     t0 = time.perf_counter()
-    x, its = h.step(1e-3)
-    th = h.theta() - x
-    h.residuals(th)
-    dt = time.perf_counter() - t0
+    h = oracle.Handle(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1))
+    r = h.sample_iteration(stride, 2)
     h.close()
-    return {"value": round(1.0 / dt, 5), "unit": "LM iters/s", "cores": 1, "kind": "port",
-            "sample": f"1 LM iteration of pass 0 ({its} PCG iterations), {dt:.1f} s"}
+    wall = time.perf_counter() - t0
+    per_point = stride * (r["lin_schur_s"] + r["backsub_s"] + r["cost_s"])
+    spmv = r["spmv_s_per_block"] * (sizes["blocks"] + sizes["cams"]) * k_avg
+    t_iter = per_point + spmv
+    return {"value": round(1.0 / t_iter, 6), "unit": "LM iters/s", "cores": 1, "kind": "port",
+            "sample": f"{config}, one pass-0 LM iteration: every {stride}th point's linearisation + Schur pairs "
+                      f"+ back-substitution + cost ({int(r['points'])} points, {int(r['pairs'])} pairs, "
+                      f"{r['lin_schur_s'] + r['backsub_s'] + r['cost_s']:.1f} s) scaled by {stride}, plus 2 S d "
+                      f"products over the {int(r['blocks'])} blocks it formed ({r['spmv_s_per_block'] * 1e9:.0f} ns "
+                      f"per block) scaled to {sizes['blocks'] + sizes['cams']} blocks x {k_avg:.1f} PCG "
+                      f"iterations; {wall:.1f} s wall in all",
+            "seconds_per_lm_iteration": round(t_iter, 2),
+            "host": host_info()}
 
 
 def cpu_baseline_blocks(blocks, budget_s=15.0):
@@ -438,7 +532,7 @@ def cpu_baseline_blocks(blocks, budget_s=15.0):
     dt = time.perf_counter() - t0
     return {"value": round(its / dt, 4), "unit": "block LM iters/s", "cores": 1, "kind": "port",
             "sample": f"whole solves of the first {n} of {len(blocks)} blocks ({cams} cameras, {its} LM iterations), "
-                      f"{dt:.1f} s"}
+                      f"{dt:.1f} s", "host": host_info()}
 
 
 def calibrate_copy(nbytes, reps=5):
@@ -461,11 +555,11 @@ def calibrate_copy(nbytes, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=10,
                     help="untimed LM iterations first (the default covers one whole solve: every kernel, the "
                          "pass boundary and the restart, so first-launch code loading stays out of the timing)")
-    ap.add_argument("--config", default="c2_961")
+    ap.add_argument("--config", default="c5_13682")
     ap.add_argument("--schur", type=int, default=0, choices=(0, 1),
                     help="plain FP64 form with S formed (0, default) or the matrix-free Schur product (1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

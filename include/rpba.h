@@ -45,6 +45,7 @@ enum rpba_status {
     RPBA_E_INVALID = -10,        /* malformed arguments (bad index, duplicate camera in a track) */
     RPBA_E_UNSUPPORTED = -11,    /* partitioned path (rpbacore.cpp:100-510) requested; not built */
     RPBA_E_SCRATCH = -12,        /* input larger than a no-allocation entry point's fixed scratch */
+    RPBA_E_BUDGET = -13,         /* a PCG solve reached the bound of rpba_ctx_set_pcg_budget */
     RPBA_E_HIP = -20,            /* HIP runtime error */
     RPBA_E_NOMEM = -21           /* host or device allocation failed */
 };
@@ -155,6 +156,12 @@ int rpba_ctx_run(rpba_ctx *ctx, int32_t max_iters, int32_t *iters_done, int32_t 
 int rpba_ctx_reset(rpba_ctx *ctx);
 /* OutputLSAc (rbacore.cpp:644-666): write the stored state back to the block. */
 int rpba_ctx_download(rpba_ctx *ctx, rpba_stats *stats);
+/* Bound every PCG solve of the context to max_iters iterations (0, the
+ * default: the reference's cap max(1000, n), rbacore.cpp:1848-1849); a PCG
+ * that reaches the bound stops the run with RPBA_E_BUDGET, the stored state
+ * kept.  No counterpart in the reference: for callers that evaluate forms
+ * under a time budget. */
+int rpba_ctx_set_pcg_budget(rpba_ctx *ctx, int32_t max_iters);
 /* 1: PCG iterations are replayed two at a time from a hipGraph; 0 (default):
  * launched one by one, which measured as fast.  Same results either way. */
 int rpba_ctx_set_graph(rpba_ctx *ctx, int32_t on);

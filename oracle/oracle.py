@@ -63,6 +63,8 @@ def _load():
     lib.orc_set_script.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
     lib.orc_set_rec.argtypes = [C.c_void_p, P(OrcRec)]
     lib.orc_cams.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+    lib.orc_sample_iteration.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double)]
+    lib.orc_sample_iteration.restype = C.c_int
     lib.orc_delete_with_flags.argtypes = [C.c_void_p, P(C.c_uint8), P(C.c_int), P(C.c_int), P(C.c_int)]
     return lib
 
@@ -205,6 +207,16 @@ class Handle:
         cnt = np.zeros(2, np.int32)
         lib().orc_delete_with_flags(self.h, _p(f, C.c_uint8), _p(kp, C.c_int), _p(ko, C.c_int), _p(cnt, C.c_int))
         return kp[:cnt[0]].copy(), ko[:cnt[1]].copy()
+
+    def sample_iteration(self, stride, reps=2):
+        """Timed sample of one pass-0 LM iteration (orc_sample_iteration):
+        the per-point work over every stride-th point and reps S d products
+        over the blocks it formed."""
+        out = np.zeros(8, dtype=np.float64)
+        rc = lib().orc_sample_iteration(self.h, stride, reps, _p(out, C.c_double))
+        assert rc == 0, rc
+        keys = ["lin_schur_s", "backsub_s", "cost_s", "spmv_s_per_block", "blocks", "points", "obs", "pairs"]
+        return dict(zip(keys, out.tolist()))
 
     def intrinsics(self, which):
         """(f, d0, d1) per image: which = 0 stored, 1 the last trial."""

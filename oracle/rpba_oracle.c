@@ -37,6 +37,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "../include/rpba.h"
 
@@ -309,6 +310,7 @@ typedef struct orc {
     int script_len, script_pos;
     orc_rec *rec;
     double *Rs, *Cs;         /* stored cameras in the (R, C) form of eRMnew / ePCnew */
+    int pt_stride;           /* timing sample (orc_sample_iteration): every pt_stride-th point; 0 = all */
 } orc;
 
 #define FS(o, v) ((o)->fsites ? (double)(float)(v) : (double)(v))
@@ -723,7 +725,8 @@ static void corrections(orc *o, int mode, double mult1, const double *XM, int sr
     const int mt = o->maxtrack > 0 ? o->maxtrack : 1;
     double *Wl = (double *)xcalloc((size_t)mt * MAXB * 3, sizeof(double)); /* eNlij */
     int *nl = (int *)xcalloc((size_t)mt, sizeof(int));
-    for (int i = 0; i < o->npts; ++i) {
+    const int pstride = o->pt_stride > 1 ? o->pt_stride : 1;
+    for (int i = 0; i < o->npts; i += pstride) {
         double V[9] = {0}, g[3] = {0};
         double *X = o->XXn + 3 * i;
         for (int j = 0; j < 3; ++j) X[j] = XM[o->nrioff + 3 * i + j];
@@ -1609,6 +1612,73 @@ int orc_setup_pass0(orc *o)
     decode_cameras(o, o->XP);
     corrections(o, 0, 1.001, o->XP, 1);
     return 0;
+}
+
+/* CPU-baseline sample of one LM iteration of pass 0 (timing only; its
+ * numbers are not a solve): the per-point work of CalcCorrections -- the
+ * linearisation with the Schur pair loop, the back-substitution and the trial
+ * cost (rbacore.cpp:1201-1410) -- over every stride-th point, then reps S d
+ * products (eAAdv, rbacore.cpp:2048-2068) over the blocks that sample formed.
+ * out: [0] s linearise + Schur, [1] s back-substitution, [2] s cost, [3] s per
+ * stored block per S d product, [4] stored blocks (diagonal included), [5]
+ * points sampled, [6] observations sampled, [7] point pairs sampled. */
+int orc_sample_iteration(orc *o, int stride, int reps, double *out)
+{
+    struct timespec t0, t1;
+#define ORC_NOW(t) clock_gettime(CLOCK_MONOTONIC, &(t))
+#define ORC_SEC(a, b) ((double)((b).tv_sec - (a).tv_sec) + 1e-9 * (double)((b).tv_nsec - (a).tv_nsec))
+    o->imagepointnr = o->ntr;
+    o->nrobs = 2 * o->ntr;
+    int rc = count_images(o, NULL, NULL);
+    if (rc) return rc;
+    prepare_weights(o);
+    for (int i = 0; i < o->npts; ++i)
+        for (int j = 0; j < 3; ++j) o->XP[o->nrioff + 3 * i + j] = o->XX[3 * i + j];
+    encode_cameras(o);
+    o->nrunknowns = o->nrioff3 + o->nraddpi * o->nimg;
+    decode_cameras(o, o->XP);
+    o->pt_stride = stride;
+    const int nc = o->nimg;
+    o->begini = (o->nraddpi > 0) ? 0 : 1;
+    for (int i = o->begini; i < nc; ++i) sblock(o, i, i);
+    ORC_NOW(t0);
+    corrections(o, 1, 1.001, o->XP, 0);
+    ORC_NOW(t1);
+    out[0] = ORC_SEC(t0, t1);
+    ORC_NOW(t0);
+    corrections(o, 2, 1.001, o->XP, 0);
+    ORC_NOW(t1);
+    out[1] = ORC_SEC(t0, t1);
+    ORC_NOW(t0);
+    corrections(o, 0, 1.001, o->XP, 1);
+    ORC_NOW(t1);
+    out[2] = ORC_SEC(t0, t1);
+    long long nblk = 0;
+    for (size_t id = 0; id < (size_t)nc * nc; ++id) nblk += o->Sf[id] ? 1 : 0;
+    double *d = (double *)xcalloc((size_t)o->nrioff + 1, sizeof(double));
+    double *y = (double *)xcalloc((size_t)o->nrioff + 1, sizeof(double));
+    for (int j = 0; j < o->nrioff; ++j) d[j] = 1.e-3 * (double)(j % 17);
+    ORC_NOW(t0);
+    for (int r = 0; r < reps; ++r) spmv(nc, o->begini, o->camparv, o->Sb, o->Sf, d, y);
+    ORC_NOW(t1);
+    out[3] = ORC_SEC(t0, t1) / ((double)(reps > 0 ? reps : 1) * (double)(nblk > 0 ? nblk : 1));
+    out[4] = (double)nblk;
+    long long np = 0, no = 0, npair = 0;
+    for (int i = 0; i < o->npts; i += (stride > 1 ? stride : 1)) {
+        const long long t = o->toff[i + 1] - o->toff[i];
+        ++np;
+        no += t;
+        npair += t * (t - 1) / 2;
+    }
+    out[5] = (double)np;
+    out[6] = (double)no;
+    out[7] = (double)npair;
+    free(d);
+    free(y);
+    o->pt_stride = 0;
+    return 0;
+#undef ORC_NOW
+#undef ORC_SEC
 }
 
 /* dims: nimg, npts, ntr, nrioff, nrioff3, normcamera, esvmaxm, nraddpi */

@@ -23,7 +23,7 @@ EXPORTS = [
     "rpba_lp_create", "rpba_lp_destroy", "rpba_lp_n_params", "rpba_lp_get_params", "rpba_lp_predict",
     "rpba_lp_gradient", "rpba_lp_train", "rpba_ctx_set_lp", "rpba_ctx_last_omega", "rpba_ctx_lp_sample",
     "rpba_read_bal", "rpba_write_bal", "rpba_free_block", "rpba_read_parameters", "rpba_version",
-    "rpba_exteval_device", "rpba_exteval_device_max_points",
+    "rpba_exteval_device", "rpba_exteval_device_max_points", "rpba_ctx_set_pcg_budget",
 ]
 
 _lib = None
@@ -54,6 +54,8 @@ def lib():
     L.rpba_ctx_sizes.argtypes = [C.c_void_p, P(C.c_int64)]
     L.rpba_ctx_tuning.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
     L.rpba_ctx_set_graph.argtypes = [C.c_void_p, C.c_int32]
+    L.rpba_ctx_set_pcg_budget.argtypes = [C.c_void_p, C.c_int32]
+    L.rpba_ctx_set_pcg_budget.restype = C.c_int
     L.rpba_ctx_tuning.restype = C.c_int
     L.rpba_dense_solve_batch.argtypes = [C.c_int32, P(C.c_int32), P(C.c_double), P(C.c_double), P(C.c_double),
                                          C.c_int32, C.c_int32]

@@ -65,6 +65,11 @@ class Context:
     def set_timing(self, on):
         return lib().rpba_ctx_set_timing(self.h, 1 if on else 0)
 
+    def set_pcg_budget(self, n):
+        """Every PCG solve stops at n iterations and the run then reports
+        RPBA_E_BUDGET (0: the reference's cap)."""
+        return lib().rpba_ctx_set_pcg_budget(self.h, n)
+
     def set_graph(self, on):
         """PCG iterations replayed from a hipGraph, or launched one by one (the default)."""
         return lib().rpba_ctx_set_graph(self.h, 1 if on else 0)

@@ -18,6 +18,7 @@ RPBA_E_NAN_RESULT = -3
 RPBA_E_INVALID = -10
 RPBA_E_UNSUPPORTED = -11
 RPBA_E_SCRATCH = -12
+RPBA_E_BUDGET = -13
 RPBA_E_HIP = -20
 RPBA_E_NOMEM = -21
 

@@ -105,6 +105,7 @@ struct rpba_ctx {
     int status = RPBA_OK;
     int64_t pcg_total = 0;
     int pcg_last = 0;
+    int pcg_budget = 0;  // 0: the reference's cap; else every PCG stops there and the run reports it
     rpba_stats stats{};
     std::vector<double> tr_sig0;
     std::vector<int> tr_acc, tr_pcg, tr_nrobs;
@@ -356,6 +357,7 @@ static PcgArgs pcg_args(rpba_ctx *c)
     std::memset(&a, 0, sizeof a);  // padding too: the PCG graph compares argument blocks bytewise
     a.nc = c->nc;
     a.msize = std::max(1000, c->nrioff);  // rbacore.cpp:1848-1849
+    if (c->pcg_budget > 0) a.msize = std::min(a.msize, c->pcg_budget);
     a.Sd = c->d_Sd.p; a.Su = c->d_Su.p; a.Cinv = c->d_Cinv.p; a.b = c->d_b.p;
     a.row_off = c->ds.roff; a.col_off = c->ds.coff; a.col_blk = c->ds.cblk; a.blk_k = c->ds.bk; a.blk_l = c->ds.bl;
     a.x = c->d_x.p; a.best = c->d_best.p; a.r = c->d_r.p; a.d = c->d_d.p; a.Ad = c->d_Ad.p;
@@ -476,6 +478,7 @@ static int bundle_step(rpba_ctx *c)
     PcgScalars hs{};
     const int rc = pcg_solve(c, pa, side && !pa.omega, hs);
     if (rc) return rc;
+    if (c->pcg_budget > 0 && hs.iters >= pa.msize && pa.msize < std::max(1000, c->nrioff)) return RPBA_E_BUDGET;
     return finish_bundle_step(c, hs);
 }
 
@@ -1119,6 +1122,13 @@ extern "C" int rpba_ctx_tuning(rpba_ctx *c, char *buf, int32_t cap)
     if (!c || !buf || cap < 1) return RPBA_E_INVALID;
     const std::string t = c->fl.solver == 1 ? c->dense.tuning() : std::string("block-Jacobi PCG (no tuned kernels)");
     std::snprintf(buf, (size_t)cap, "%s", t.c_str());
+    return RPBA_OK;
+}
+
+extern "C" int rpba_ctx_set_pcg_budget(rpba_ctx *c, int32_t max_iters)
+{
+    if (!c || max_iters < 0) return RPBA_E_INVALID;
+    c->pcg_budget = max_iters;
     return RPBA_OK;
 }
 
