@@ -147,6 +147,28 @@ def run_bench(args):
     rc_c, out_c, st_c, _ = api.calc_plsa(sc, flags)
     call_s = time.perf_counter() - t0
     progress(f"rpba_calc_plsa from host buffers {call_s:.3f} s (status {rc_c})")
+    # the same call's phases through the resumable form
+    ph = {}
+    t0 = time.perf_counter()
+    pctx = api.Context(sc, flags)
+    device_sync()
+    ph["create"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pctx.run(1)
+    device_sync()
+    ph["first_lm_iteration_incl_structure_and_allocation"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pctx.run(-1)
+    device_sync()
+    ph["rest_of_solve"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pctx.download()
+    ph["download"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pctx.close()
+    ph["destroy"] = time.perf_counter() - t0
+    ph = {k: round(v, 4) for k, v in ph.items()}
+    progress(f"phases of the call: {ph}")
 
     # the other forms: their time until the pass-1 stored sigma0 reaches the
     # plain form's final value (x (1 + 1e-6)) and their errors against plain
@@ -293,6 +315,7 @@ def run_bench(args):
                    "parallelism": "single GPU"},
         "time_to_converged_s": round(ttc, 4),
         "rpba_calc_plsa_s": round(call_s, 4),
+        "rpba_calc_plsa_phases_s": ph,
         "itt": st["itt"],
         "pcg_iters": st["pcg_iters"],
         "pcg_per_lm_iteration": pcg_per_it,

@@ -184,6 +184,13 @@ void launch_cost(const CostArgs &a, double *out, hipStream_t st);
 void launch_reweight(const RobustArgs &a, double *wsum_out, int *nflag_out, hipStream_t st);
 void launch_wsum(int n, const float *wx, const float *wy, double *partial, double *out, hipStream_t st);
 void launch_store_cams(const CamPtrs &cp, int nc, hipStream_t st);
+// OutputLSAc on the device: obs_kept (file order) and pt_kept set to 1 for the
+// kept entries and points (both cleared by the caller), *ss_out = sum of the
+// squared residuals of the stored state; partial holds out_groups(ntr) doubles
+int out_groups(int ntr);
+void launch_output(int ntr, int npts, const int *tfo, const int *tcam, const int *tpt, const int *nrpv, const float *ox,
+                   const float *oy, const double *Pst, const double *Ist, const double *X, unsigned char *obs_kept,
+                   unsigned char *pt_kept, double *partial, double *ss_out, hipStream_t st);
 // workgroups of the Exteval launch (partial slots needed)
 int exteval_groups(int npts);
 // out[0] = sigma (ITTTT value), out[1] = nobs, out[2] = dof (EvalLSA,

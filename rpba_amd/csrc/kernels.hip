@@ -1839,6 +1839,40 @@ __global__ __launch_bounds__(1024) void k_exteval_finish(ExtArgs a, int nb, doub
     }
 }
 
+// OutputLSAc + PointXXvOut on the device (rbacore.cpp:644-666, 1978-2010):
+// the kept observations' flags in file order, and per workgroup the sum of
+// squared residuals of the stored state (stored R|t, stored intrinsics,
+// accepted points) for the RMS.  A fixed grid, so the sum is reproducible.
+__global__ __launch_bounds__(256) void k_out_entries(int ntr, const int *tfo, const int *tcam, const int *tpt,
+                                                     const float *ox, const float *oy, const double *Pst,
+                                                     const double *Ist, const double *X, unsigned char *obs_kept,
+                                                     double *partial)
+{
+    __shared__ double sh[4];
+    double s = 0.;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ntr; q += gridDim.x * blockDim.x) {
+        obs_kept[tfo[q]] = 1;
+        const int k = tcam[q], i = tpt[q];
+        const double *Pk = Pst + 12 * (size_t)k;
+        const double *x = X + 3 * (size_t)i;
+        const double e0 = Pk[0] * x[0] + Pk[1] * x[1] + Pk[2] * x[2] + Pk[3];
+        const double e1 = Pk[4] * x[0] + Pk[5] * x[1] + Pk[6] * x[2] + Pk[7];
+        const double e2 = Pk[8] * x[0] + Pk[9] * x[1] + Pk[10] * x[2] + Pk[11];
+        const double u = e0 / e2, v = e1 / e2, rho = u * u + v * v;
+        const double sc = 1. + Ist[3 * (size_t)k] * rho + Ist[3 * (size_t)k + 1] * rho * rho;
+        const double r0 = (double)ox[q] - Ist[3 * (size_t)k + 2] * sc * u;
+        const double r1 = (double)oy[q] - Ist[3 * (size_t)k + 2] * sc * v;
+        s += r0 * r0 + r1 * r1;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_out_points(int npts, const int *nrpv, unsigned char *pt_kept)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npts; i += gridDim.x * blockDim.x) pt_kept[nrpv[i]] = 1;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1981,6 +2015,16 @@ void launch_wsum(int n, const float *wx, const float *wy, double *partial, doubl
     k_final_sum<<<1, 1024, 0, st>>>(partial, nb, nullptr, out, nullptr);
 }
 void launch_store_cams(const CamPtrs &cp, int nc, hipStream_t st) { k_store_cams<<<nblk(nc, 64), 64, 0, st>>>(cp, nc); }
+int out_groups(int ntr) { return (int)std::min<long long>(nblk(ntr > 0 ? ntr : 1, 256), 2048); }
+void launch_output(int ntr, int npts, const int *tfo, const int *tcam, const int *tpt, const int *nrpv, const float *ox,
+                   const float *oy, const double *Pst, const double *Ist, const double *X, unsigned char *obs_kept,
+                   unsigned char *pt_kept, double *partial, double *ss_out, hipStream_t st)
+{
+    const int g = out_groups(ntr);
+    k_out_entries<<<g, 256, 0, st>>>(ntr, tfo, tcam, tpt, ox, oy, Pst, Ist, X, obs_kept, partial);
+    k_final_sum<<<1, 1024, 0, st>>>(partial, g, nullptr, ss_out, nullptr);
+    if (npts > 0) k_out_points<<<(unsigned)std::min<long long>(nblk(npts, 256), 2048), 256, 0, st>>>(npts, nrpv, pt_kept);
+}
 int exteval_groups(int npts) { return (int)nblk(npts > 0 ? npts : 1, 256); }
 void launch_exteval(const ExtArgs &a, double *out, hipStream_t st)
 {

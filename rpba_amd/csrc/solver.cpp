@@ -58,9 +58,7 @@ struct rpba_ctx {
     int nrioff = 0, nrioff3 = 0;
     float ihw2 = 1.f;
     double defdiff = 5000.;
-    Structure s_init;                // tracks after PointXXvIn (host, for resets)
-    std::vector<float> w0_init;      // Wijoldv at the start, 3 per track entry
-    std::vector<double> X_init;      // points at the start
+    int npts_init = 0;               // points kept by PointXXvIn (d_X0 holds them)
     std::vector<double> Pst, Ist;    // stored cameras (host copy at pass boundaries)
     std::vector<double> Pst_init, Ist_init, R0, C0;
     bool structure_dirty = true;     // pairs/blocks must be rebuilt
@@ -177,13 +175,33 @@ static int restore_points(rpba_ctx *c);
 
 // tracks, observation values and original weights of the initial block, and
 // the accepted points (LSA construction, rbacore.cpp:37-123)
+// RPBA_SETUP_TRACE=1: the set-up and download steps with their wall times on
+// stderr (the stream synchronised at each mark)
+static void setup_mark(rpba_ctx *c, const char *what)
+{
+    static const bool on = [] {
+        const char *e = std::getenv("RPBA_SETUP_TRACE");
+        return e && e[0] == '1';
+    }();
+    if (!on) return;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    static auto t0 = std::chrono::steady_clock::now();
+    const auto t = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "rpba setup: %-28s %9.1f ms\n", what, std::chrono::duration<double, std::milli>(t - t0).count());
+    t0 = t;
+}
+
+// PointXXvIn on the device from the block's file-order arrays (tracks, the
+// observations and original weights gathered per entry, the kept points)
 static int upload_tracks(rpba_ctx *c)
 {
-    int rc = c->ds.upload_tracks(c->s_init, c->io->obs_xy, c->w0_init, c->st);
+    if (c->d_X0.alloc(3 * (size_t)std::max(c->io->n_pts, 1))) return RPBA_E_NOMEM;
+    int rc = c->ds.build_tracks_device(c->io, c->d_X0.p, c->st);
     if (rc) return rc;
-    rc |= c->d_X0.put(c->X_init.data(), c->X_init.size(), c->st);
-    rc |= c->d_X.alloc(c->X_init.size());
-    rc |= c->d_Xn.alloc(c->X_init.size());
+    setup_mark(c, "tracks on the device");
+    c->npts_init = c->ds.npts;
+    rc |= c->d_X.alloc(3 * (size_t)c->npts_init);
+    rc |= c->d_Xn.alloc(3 * (size_t)c->npts_init);
     if (rc) return RPBA_E_NOMEM;
     return restore_points(c);
 }
@@ -191,7 +209,7 @@ static int upload_tracks(rpba_ctx *c)
 // accepted and trial points <- the initial points (device-to-device)
 static int restore_points(rpba_ctx *c)
 {
-    const size_t n = c->X_init.size();
+    const size_t n = 3 * (size_t)c->npts_init;
     if (n) {
         HIPCHK(hipMemcpyAsync(c->d_X.p, c->d_X0.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
         HIPCHK(hipMemcpyAsync(c->d_Xn.p, c->d_X0.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
@@ -325,6 +343,10 @@ static int upload_initial_cameras(rpba_ctx *c)
     rc |= c->c_Rn.put(Rn.data(), Rn.size(), c->st);
     rc |= c->c_Cn.put(Cn.data(), Cn.size(), c->st);
     rc |= c->c_In.put(In.data(), In.size(), c->st);
+    // the stored state (read by rpba_ctx_download) exists on the device from
+    // here on, also for a solve that stops before its first pass set-up
+    rc |= c->c_Pst.put(c->Pst.data(), c->Pst.size(), c->st);
+    rc |= c->c_Ist.put(c->Ist.data(), c->Ist.size(), c->st);
     if (rc) return RPBA_E_NOMEM;
     HIPCHK(hipStreamSynchronize(c->st));
     return RPBA_OK;
@@ -832,19 +854,7 @@ static int ctx_init(rpba_ctx *c)
     c->ihw2 = ihw2 / (float)nc;
     c->defdiff = 5000. / (double)sqrtf(c->ihw2);  // rbacore.cpp:48
     c->minimagenrdel = nc > 2 ? 3 : 2;
-    Structure &s = c->s_init;
-    int rc = build_tracks(io, s);
-    if (rc) return rc;
-    c->X_init.assign(3 * (size_t)s.npts, 0.);
-    for (int i = 0; i < s.npts; ++i)
-        for (int j = 0; j < 3; ++j) c->X_init[3 * (size_t)i + j] = (double)io->pts[3 * (size_t)s.nrpv[i] + j];
-    c->w0_init.resize(3 * (size_t)s.ntr);
-    for (int q = 0; q < s.ntr; ++q) {
-        const int fo = s.tfo[q];
-        c->w0_init[3 * (size_t)q] = io->obs_w ? io->obs_w[3 * (size_t)fo] : 1.f;
-        c->w0_init[3 * (size_t)q + 1] = io->obs_w ? io->obs_w[3 * (size_t)fo + 1] : 1.f;
-        c->w0_init[3 * (size_t)q + 2] = io->obs_w ? io->obs_w[3 * (size_t)fo + 2] : 0.f;
-    }
+    int rc = 0;
     c->Pst.resize(12 * (size_t)nc);
     c->Ist.resize(3 * (size_t)nc);
     for (int k = 0; k < nc; ++k) {
@@ -888,6 +898,7 @@ static int ctx_init(rpba_ctx *c)
     c->Pst_init = c->Pst;
     c->Ist_init = c->Ist;
 
+    setup_mark(c, "host set-up");
     HIPCHK(hipGetDevice(&c->dev));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
@@ -927,11 +938,14 @@ static int ctx_init(rpba_ctx *c)
         rc = c->dense.autotune(c->st);
         if (rc) return rc;
     }
+    setup_mark(c, "streams and small buffers");
     reset_state(c);
     rc = upload_tracks(c);
     if (rc) return rc;
+    setup_mark(c, "points restored");
     rc = upload_initial_cameras(c);
     if (rc) return rc;
+    setup_mark(c, "cameras");
     // the first device-to-host copy of this size pays a one-time set-up of
     // several milliseconds: take it here rather than at the first pass boundary
     HIPCHK(hipMemcpyAsync(c->hcam, c->d_Sd.p, sizeof(double) * 15 * (size_t)nc, hipMemcpyDeviceToHost, c->st));
@@ -1030,34 +1044,58 @@ extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
     if (!c) return RPBA_E_INVALID;
     rpba_block *io = c->io;
     const int nc = c->nc;
-    std::vector<int> toff, nrpv, tfo, tcam;
-    int rc = c->ds.download_tracks(toff, nrpv, tfo, tcam, c->st);
-    if (rc) return rc;
-    const int npts = c->ds.npts;
+    const int npts = c->ds.npts, ntr = c->ds.ntr;
+    // kept flags in file order and the stored state's squared residuals on
+    // the device; the flags, the accepted points and their original indices
+    // come back (the tracks themselves stay on the device)
+    unsigned char *d_ok = nullptr, *d_pk = nullptr;
+    double *d_part = nullptr;
+    const int g = out_groups(ntr);
+    // every buffer the output kernels read must be there at its size
+    if (!c->c_Pst.p || c->c_Pst.n < 12 * (size_t)nc || !c->c_Ist.p || c->c_Ist.n < 3 * (size_t)nc || !c->c_In.p ||
+        c->c_In.n < 3 * (size_t)nc || (npts && (!c->d_X.p || c->d_X.n < 3 * (size_t)npts || !c->ds.nrpv)) ||
+        (ntr && (!c->ds.tfo || !c->ds.tcam || !c->ds.tpt || !c->ds.ox || !c->ds.oy)))
+        return RPBA_E_INVALID;
+    if (hipMalloc((void **)&d_ok, std::max<size_t>((size_t)io->n_obs, 1)) != hipSuccess ||
+        hipMalloc((void **)&d_pk, std::max<size_t>((size_t)io->n_pts, 1)) != hipSuccess ||
+        hipMalloc((void **)&d_part, sizeof(double) * ((size_t)g + 1)) != hipSuccess) {
+        if (d_ok) (void)hipFree(d_ok);
+        if (d_pk) (void)hipFree(d_pk);
+        if (d_part) (void)hipFree(d_part);
+        return RPBA_E_NOMEM;
+    }
     std::vector<double> X(3 * (size_t)npts), P(12 * (size_t)nc), In(3 * (size_t)nc);
-    // the device holds the stored state once pass 0 has been set up; an error
-    // before that leaves the initial host values (image 0 rectified) current
-    if (npts) HIPCHK(hipMemcpyAsync(X.data(), c->d_X.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, c->st));
-    const bool dev_cams = c->c_Pst.p && c->c_Pst.n >= P.size();
-    if (dev_cams)
-        HIPCHK(hipMemcpyAsync(P.data(), c->c_Pst.p, sizeof(double) * P.size(), hipMemcpyDeviceToHost, c->st));
-    else
-        P = c->Pst;
-    HIPCHK(hipMemcpyAsync(In.data(), c->c_In.p, sizeof(double) * In.size(), hipMemcpyDeviceToHost, c->st));
-    std::vector<double> Is = c->Ist;  // stored intrinsics (the state the reported RMS belongs to)
-    if (c->c_Ist.p && c->c_Ist.n >= Is.size())
-        HIPCHK(hipMemcpyAsync(Is.data(), c->c_Ist.p, sizeof(double) * Is.size(), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    if (io->obs_kept) std::memset(io->obs_kept, 0, (size_t)io->n_obs);
-    if (io->pt_kept) std::memset(io->pt_kept, 0, (size_t)io->n_pts);
+    std::vector<int> nrpv((size_t)npts);
+    double ss = 0.;
+    int rc = RPBA_OK;
+    if (hipMemsetAsync(d_ok, 0, (size_t)io->n_obs, c->st) != hipSuccess ||
+        hipMemsetAsync(d_pk, 0, (size_t)io->n_pts, c->st) != hipSuccess)
+        rc = RPBA_E_HIP;
+    if (!rc) {
+        launch_output(ntr, npts, c->ds.tfo, c->ds.tcam, c->ds.tpt, c->ds.nrpv, c->ds.ox, c->ds.oy, c->c_Pst.p, c->c_Ist.p,
+                      c->d_X.p, d_ok, d_pk, d_part, d_part + g, c->st);
+        if (hipGetLastError() != hipSuccess) rc = RPBA_E_HIP;
+    }
+    // the device holds the stored state from the set-up on (restore_points,
+    // upload_initial_cameras); R|t from the stored state, intrinsics from the
+    // last trial (OutputLSAc)
+    if (!rc && ((npts && hipMemcpyAsync(X.data(), c->d_X.p, sizeof(double) * X.size(), hipMemcpyDeviceToHost, c->st) != hipSuccess) ||
+                (npts && hipMemcpyAsync(nrpv.data(), c->ds.nrpv, sizeof(int) * (size_t)npts, hipMemcpyDeviceToHost, c->st) != hipSuccess) ||
+                hipMemcpyAsync(P.data(), c->c_Pst.p, sizeof(double) * P.size(), hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+                hipMemcpyAsync(In.data(), c->c_In.p, sizeof(double) * In.size(), hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+                hipMemcpyAsync(&ss, d_part + g, sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+                (io->obs_kept && io->n_obs && hipMemcpyAsync(io->obs_kept, d_ok, (size_t)io->n_obs, hipMemcpyDeviceToHost, c->st) != hipSuccess) ||
+                (io->pt_kept && io->n_pts && hipMemcpyAsync(io->pt_kept, d_pk, (size_t)io->n_pts, hipMemcpyDeviceToHost, c->st) != hipSuccess)))
+        rc = RPBA_E_HIP;
+    if (hipStreamSynchronize(c->st) != hipSuccess && !rc) rc = RPBA_E_HIP;
+    (void)hipFree(d_ok);
+    (void)hipFree(d_pk);
+    (void)hipFree(d_part);
+    if (rc) return rc;
     for (int i = 0; i < npts; ++i) {
         const int p = nrpv[i];
         for (int j = 0; j < 3; ++j) io->pts[3 * (size_t)p + j] = (float)X[3 * (size_t)i + j];
-        if (io->pt_kept) io->pt_kept[p] = 1;
-        for (int q = toff[i]; q < toff[(size_t)i + 1]; ++q)
-            if (io->obs_kept) io->obs_kept[tfo[q]] = 1;
     }
-    // OutputLSAc: R|t from the stored state, intrinsics from the last trial
     for (int k = 0; k < nc; ++k) {
         for (int j = 0; j < 12; ++j) io->cam_P[12 * (size_t)k + j] = (float)P[12 * (size_t)k + j];
         io->cam_d[2 * (size_t)k] = (float)In[3 * (size_t)k];
@@ -1065,25 +1103,7 @@ extern "C" int rpba_ctx_download(rpba_ctx *c, rpba_stats *st)
         io->cam_f[k] = (float)In[3 * (size_t)k + 2];
     }
     if (st) {
-        // RMS over the kept observations of the stored state (stored R|t, X
-        // and intrinsics), double
-        double ss = 0.;
-        int64_t n = 0;
-        for (int i = 0; i < npts; ++i)
-            for (int q = toff[i]; q < toff[(size_t)i + 1]; ++q) {
-                const int k = tcam[q], fo = tfo[q];
-                const double *Pk = P.data() + 12 * (size_t)k;
-                const double *x = X.data() + 3 * (size_t)i;
-                const double e0 = Pk[0] * x[0] + Pk[1] * x[1] + Pk[2] * x[2] + Pk[3];
-                const double e1 = Pk[4] * x[0] + Pk[5] * x[1] + Pk[6] * x[2] + Pk[7];
-                const double e2 = Pk[8] * x[0] + Pk[9] * x[1] + Pk[10] * x[2] + Pk[11];
-                const double u = e0 / e2, v = e1 / e2, rho = u * u + v * v;
-                const double sc = 1. + Is[3 * (size_t)k] * rho + Is[3 * (size_t)k + 1] * rho * rho;
-                const double r0 = io->obs_xy[2 * (size_t)fo] - Is[3 * (size_t)k + 2] * sc * u;
-                const double r1 = io->obs_xy[2 * (size_t)fo + 1] - Is[3 * (size_t)k + 2] * sc * v;
-                ss += r0 * r0 + r1 * r1;
-                n += 1;
-            }
+        const int64_t n = ntr;
         rpba_stats keep = *st;
         *st = c->stats;
         st->itt = c->itt;

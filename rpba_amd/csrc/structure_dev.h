@@ -104,6 +104,12 @@ struct DevStruct {
     // tracks of the host structure (after PointXXvIn) plus the gathered
     // observation values and weights
     int upload_tracks(const Structure &s, const float *obs_xy, const std::vector<float> &w0, hipStream_t st);
+    // PointXXvIn (rbacore.cpp:1918-1974) on the device from the block's
+    // file-order arrays: tracks in file order within a point, points with
+    // |Z| <= 1e-8 or fewer than two observations left out; the kept points'
+    // coordinates go to X0 (3 * n_pts doubles of room).  Returns
+    // RPBA_E_INVALID for an index out of range or a camera twice in one track.
+    int build_tracks_device(const rpba_block *io, double *X0, hipStream_t st);
     // camera-major order, pairs and blocks; *first_empty = first image without
     // observations (then nothing else is built) or -1
     int build(hipStream_t st, int *first_empty);

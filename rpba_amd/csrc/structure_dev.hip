@@ -665,6 +665,204 @@ void DevStruct::swap_tracks(Tracks &b)
     std::swap(w0y, b.w0y); std::swap(w0xy, b.w0xy);
 }
 
+// ---------------------------------------------------------------------------
+// PointXXvIn on the device (rbacore.cpp:1918-1974)
+// ---------------------------------------------------------------------------
+
+// per observation of the block: the point as a sort key (n_pts for an absent
+// observation, image < 0, so it sorts last), its file index as the value, the
+// point's count, and the index checks
+__global__ __launch_bounds__(256) void k_px_keys(int no, int nc, int np, const int *cam, const int *pt,
+                                                 unsigned *key, int *val, int *cnt, int *bad)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < no; i += gridDim.x * blockDim.x) {
+        const int c = cam[i], p = pt[i];
+        unsigned k = (unsigned)np;
+        if (c >= 0) {
+            if (c >= nc || p < 0 || p >= np)
+                atomicOr(bad, 1);
+            else {
+                k = (unsigned)p;
+                atomicAdd(cnt + p, 1);
+            }
+        }
+        key[i] = k;
+        val[i] = i;
+    }
+}
+
+// per point: a camera twice in its track is an error (checked for every
+// point, as the host reading does); kept = |Z| > 1e-8 and at least two
+// observations (rbacore.cpp:1954-1955); kept entry count for the scan
+__global__ __launch_bounds__(256) void k_px_points(int np, const int *off, const int *sval, const int *cam,
+                                                   const float *pts, int *keepp, int *keept, int *bad)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+        const int b = off[i], e = off[i + 1], t = e - b;
+        for (int x = b; x < e; ++x) {
+            const int cx = cam[sval[x]];
+            for (int y = x + 1; y < e; ++y)
+                if (cam[sval[y]] == cx) atomicOr(bad, 2);
+        }
+        const bool keep = fabs((double)pts[3 * (size_t)i + 2]) > 1.e-8 && t >= 2;
+        keepp[i] = keep ? 1 : 0;
+        keept[i] = keep ? t : 0;
+    }
+}
+
+// kept points: original index, offsets, coordinates; their entries: image,
+// kept point, file index, observation and original weights
+__global__ __launch_bounds__(256) void k_px_scatter(int np, const int *off, const int *sval, const int *keepp,
+                                                    const int *pnew, const int *tnew, const int *cam, const float *xy,
+                                                    const float *w, const float *pts, int *toff, int *nrpv, int *tcam,
+                                                    int *tpt, int *tfo, float *ox, float *oy, float *w0x, float *w0y,
+                                                    float *w0xy, double *X0, int npk_total, int ntr_total)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+        if (!keepp[i]) continue;
+        const int k = pnew[i], q0 = tnew[i];
+        nrpv[k] = i;
+        toff[k] = q0;
+        if (k == npk_total - 1) toff[k + 1] = ntr_total;
+        for (int j = 0; j < 3; ++j) X0[3 * (size_t)k + j] = (double)pts[3 * (size_t)i + j];
+        const int b = off[i], e = off[i + 1];
+        for (int x = b; x < e; ++x) {
+            const int fo = sval[x], q = q0 + (x - b);
+            tcam[q] = cam[fo];
+            tpt[q] = k;
+            tfo[q] = fo;
+            ox[q] = xy[2 * (size_t)fo];
+            oy[q] = xy[2 * (size_t)fo + 1];
+            w0x[q] = w ? w[3 * (size_t)fo] : 1.f;
+            w0y[q] = w ? w[3 * (size_t)fo + 1] : 1.f;
+            w0xy[q] = w ? w[3 * (size_t)fo + 2] : 0.f;
+        }
+    }
+}
+
+static inline unsigned px_grid(long long n) { return (unsigned)std::min<long long>((n + 255) / 256 + 1, 8192); }
+
+int DevStruct::build_tracks_device(const rpba_block *io, double *X0, hipStream_t st)
+{
+    const int nc = io->n_cams, np = io->n_pts, no = io->n_obs;
+    this->nc = nc;
+    // the block's file-order arrays, uploaded as they are
+    int *d_cam = nullptr, *d_pt = nullptr, *sval = nullptr, *val = nullptr, *cnt = nullptr, *off = nullptr;
+    int *keepp = nullptr, *keept = nullptr, *pnew = nullptr, *tnew = nullptr, *bad = nullptr;
+    unsigned *key = nullptr, *skey = nullptr;
+    float *d_xy = nullptr, *d_w = nullptr, *d_pts = nullptr;
+    void *tmpb = nullptr;
+    int rc = RPBA_OK;
+    auto fail = [&](int r) { rc = r; };
+    const size_t n1 = (size_t)std::max(no, 1), p1 = (size_t)np + 1;
+    if (hipMalloc((void **)&d_cam, sizeof(int) * n1) != hipSuccess || hipMalloc((void **)&d_pt, sizeof(int) * n1) != hipSuccess ||
+        hipMalloc((void **)&d_xy, sizeof(float) * 2 * n1) != hipSuccess ||
+        (io->obs_w && hipMalloc((void **)&d_w, sizeof(float) * 3 * n1) != hipSuccess) ||
+        hipMalloc((void **)&d_pts, sizeof(float) * 3 * p1) != hipSuccess || hipMalloc((void **)&key, sizeof(unsigned) * n1) != hipSuccess ||
+        hipMalloc((void **)&skey, sizeof(unsigned) * n1) != hipSuccess || hipMalloc((void **)&val, sizeof(int) * n1) != hipSuccess ||
+        hipMalloc((void **)&sval, sizeof(int) * n1) != hipSuccess || hipMalloc((void **)&cnt, sizeof(int) * (p1 + 1)) != hipSuccess ||
+        hipMalloc((void **)&off, sizeof(int) * (p1 + 1)) != hipSuccess || hipMalloc((void **)&keepp, sizeof(int) * (p1 + 1)) != hipSuccess ||
+        hipMalloc((void **)&keept, sizeof(int) * (p1 + 1)) != hipSuccess || hipMalloc((void **)&pnew, sizeof(int) * (p1 + 1)) != hipSuccess ||
+        hipMalloc((void **)&tnew, sizeof(int) * (p1 + 1)) != hipSuccess || hipMalloc((void **)&bad, sizeof(int)) != hipSuccess)
+        fail(RPBA_E_NOMEM);
+    if (!rc && pinned(4)) fail(RPBA_E_NOMEM);
+    if (!rc) {
+        if (no) {
+            if (hipMemcpyAsync(d_cam, io->obs_cam, sizeof(int) * (size_t)no, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(d_pt, io->obs_pt, sizeof(int) * (size_t)no, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(d_xy, io->obs_xy, sizeof(float) * 2 * (size_t)no, hipMemcpyHostToDevice, st) != hipSuccess ||
+                (io->obs_w && hipMemcpyAsync(d_w, io->obs_w, sizeof(float) * 3 * (size_t)no, hipMemcpyHostToDevice, st) != hipSuccess))
+                fail(RPBA_E_HIP);
+        }
+        if (!rc && np && hipMemcpyAsync(d_pts, io->pts, sizeof(float) * 3 * (size_t)np, hipMemcpyHostToDevice, st) != hipSuccess)
+            fail(RPBA_E_HIP);
+    }
+    if (!rc && (hipMemsetAsync(cnt, 0, sizeof(int) * (p1 + 1), st) != hipSuccess || hipMemsetAsync(bad, 0, sizeof(int), st) != hipSuccess))
+        fail(RPBA_E_HIP);
+    if (!rc && no) k_px_keys<<<px_grid(no), 256, 0, st>>>(no, nc, np, d_cam, d_pt, key, val, cnt, bad);
+    // stable sort of the file indices by point (file order within a point)
+    size_t bytes = 0;
+    const unsigned kb = bits_for((unsigned long long)np + 1);
+    if (!rc && no) {
+        if (rocprim::radix_sort_pairs(nullptr, bytes, key, skey, val, sval, (size_t)no, 0u, kb, st) != hipSuccess) fail(RPBA_E_HIP);
+        if (!rc && hipMalloc(&tmpb, std::max<size_t>(bytes, 1)) != hipSuccess) fail(RPBA_E_NOMEM);
+        if (!rc && rocprim::radix_sort_pairs(tmpb, bytes, key, skey, val, sval, (size_t)no, 0u, kb, st) != hipSuccess)
+            fail(RPBA_E_HIP);
+        if (tmpb) (void)hipFree(tmpb);
+        tmpb = nullptr;
+    }
+    // per-point offsets into the sorted list, then the kept points
+    auto scan = [&](const int *in, int *out, size_t n) -> int {
+        size_t b2 = 0;
+        if (rocprim::exclusive_scan(nullptr, b2, in, out, 0, n, rocprim::plus<int>(), st) != hipSuccess) return RPBA_E_HIP;
+        void *t2 = nullptr;
+        if (hipMalloc(&t2, std::max<size_t>(b2, 1)) != hipSuccess) return RPBA_E_NOMEM;
+        const int r2 = rocprim::exclusive_scan(t2, b2, in, out, 0, n, rocprim::plus<int>(), st) != hipSuccess ? (int)RPBA_E_HIP : (int)RPBA_OK;
+        (void)hipFree(t2);
+        return r2;
+    };
+    if (!rc) rc = scan(cnt, off, p1);
+    if (!rc && np) k_px_points<<<px_grid(np), 256, 0, st>>>(np, off, sval, d_cam, d_pts, keepp, keept, bad);
+    if (!rc && hipMemsetAsync(keepp + np, 0, sizeof(int), st) != hipSuccess) fail(RPBA_E_HIP);
+    if (!rc && hipMemsetAsync(keept + np, 0, sizeof(int), st) != hipSuccess) fail(RPBA_E_HIP);
+    if (!rc) rc = scan(keepp, pnew, p1);
+    if (!rc) rc = scan(keept, tnew, p1);
+    if (!rc) {
+        if (hipMemcpyAsync(hpin, pnew + np, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hpin + 1, tnew + np, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(hpin + 2, bad, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            fail(RPBA_E_HIP);
+    }
+    if (!rc && hpin[2]) fail(RPBA_E_INVALID);
+    if (!rc) {
+        npts = hpin[0];
+        ntr = hpin[1];
+        sizes_cap_pts = (size_t)npts;
+        sizes_cap_tr = (size_t)ntr;
+        Tracks t;
+        if (!toff) {
+            if (t.alloc(sizes_cap_pts, sizes_cap_tr)) fail(RPBA_E_NOMEM);
+            if (!rc) {
+                toff = t.toff; nrpv = t.nrpv; tcam = t.tcam; tpt = t.tpt; tfo = t.tfo;
+                ox = t.ox; oy = t.oy; w0x = t.w0x; w0y = t.w0y; w0xy = t.w0xy;
+                (void)hipFree(t.X);
+            }
+        }
+    }
+    if (!rc && hipMemsetAsync(toff, 0, sizeof(int), st) != hipSuccess) fail(RPBA_E_HIP);
+    if (!rc && np)
+        k_px_scatter<<<px_grid(np), 256, 0, st>>>(np, off, sval, keepp, pnew, tnew, d_cam, d_xy, d_w, d_pts, toff, nrpv, tcam,
+                                                  tpt, tfo, ox, oy, w0x, w0y, w0xy, X0, npts, ntr);
+    if (!rc && hipGetLastError() != hipSuccess) fail(RPBA_E_HIP);
+    if (!rc) {
+        // snapshot for restore_tracks
+        npts0 = npts;
+        ntr0 = ntr;
+        if (!init.toff && init.alloc(sizes_cap_pts, sizes_cap_tr)) fail(RPBA_E_NOMEM);
+    }
+    if (!rc) {
+        const size_t pn = (size_t)npts, nt = (size_t)ntr;
+        if (hipMemcpyAsync(init.toff, toff, sizeof(int) * (pn + 1), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(init.nrpv, nrpv, sizeof(int) * pn, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(init.tcam, tcam, sizeof(int) * nt, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(init.tpt, tpt, sizeof(int) * nt, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(init.tfo, tfo, sizeof(int) * nt, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(init.ox, ox, sizeof(float) * nt, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(init.oy, oy, sizeof(float) * nt, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(init.w0x, w0x, sizeof(float) * nt, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(init.w0y, w0y, sizeof(float) * nt, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(init.w0xy, w0xy, sizeof(float) * nt, hipMemcpyDeviceToDevice, st) != hipSuccess)
+            fail(RPBA_E_HIP);
+    }
+    if (hipStreamSynchronize(st) != hipSuccess && !rc) fail(RPBA_E_HIP);
+    for (void *pp : {(void *)d_cam, (void *)d_pt, (void *)d_xy, (void *)d_w, (void *)d_pts, (void *)key, (void *)skey,
+                     (void *)val, (void *)sval, (void *)cnt, (void *)off, (void *)keepp, (void *)keept, (void *)pnew,
+                     (void *)tnew, (void *)bad})
+        if (pp) (void)hipFree(pp);
+    return rc;
+}
+
 int DevStruct::upload_tracks(const Structure &s, const float *obs_xy, const std::vector<float> &w0, hipStream_t st)
 {
     nc = s.nc;
