@@ -63,6 +63,8 @@ def _load():
     lib.orc_set_script.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
     lib.orc_set_rec.argtypes = [C.c_void_p, P(OrcRec)]
     lib.orc_cams.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+    lib.oracle_set_diag.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double), C.c_int]
+    lib.oracle_diag_trace_len.restype = C.c_int
     lib.orc_sample_iteration.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double)]
     lib.orc_sample_iteration.restype = C.c_int
     lib.orc_delete_with_flags.argtypes = [C.c_void_p, P(C.c_uint8), P(C.c_int), P(C.c_int), P(C.c_int)]

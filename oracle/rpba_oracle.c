@@ -61,6 +61,34 @@ static int g_order = 0;
 void oracle_set_mutation(int m) { g_mut = m; }
 void oracle_set_order(int m) { g_order = m; }
 
+/* Diagnostics of the FP32 forms (tests/golden/fp32_diagnostic.py only):
+ *   prec 1: the GPU's FP32 records emulated -- the Jacobians Bc, Bp rounded
+ *           to float, Bc^T W and W = Bc^T W Bp formed and rounded in float,
+ *           Y = W V^-1 from a float V^-1 rounded in float; S sums the
+ *           products of those floats in double;
+ *   prec 2: consistently rounded factors -- with the weight W = L L^T, the
+ *           whitened Jacobians L^T Bc and L^T Bp rounded to float once, and
+ *           U, V, W, g, Y and S all formed from them in double.
+ * pcg_cap > 0 lowers the PCG cap; the step-norm ratios m / m0 of PCG solve
+ * number pcg_trace_call (0-based, counted from the setting) go to
+ * pcg_trace[0..cap); stop_after > 0 ends Adjust after that many LM
+ * iterations. */
+static int g_prec = 0, g_pcg_cap = 0, g_stop_after = 0;
+static double *g_pcg_trace = NULL;
+static int g_pcg_trace_cap = 0, g_pcg_trace_call = -1, g_pcg_calls = 0, g_pcg_trace_len = 0;
+void oracle_set_diag(int prec, int pcg_cap, int stop_after, int trace_call, double *trace, int trace_cap)
+{
+    g_prec = prec;
+    g_pcg_cap = pcg_cap;
+    g_stop_after = stop_after;
+    g_pcg_trace_call = trace_call;
+    g_pcg_trace = trace;
+    g_pcg_trace_cap = trace_cap;
+    g_pcg_calls = 0;
+    g_pcg_trace_len = 0;
+}
+int oracle_diag_trace_len(void) { return g_pcg_trace_len; }
+
 /* ------------------------------------------------------------------------- */
 /* small math                                                                */
 /* ------------------------------------------------------------------------- */
@@ -741,10 +769,32 @@ static void corrections(orc *o, int mode, double mult1, const double *XM, int sr
                 const int n = derivs(o, k, &e, Bc, Bp);
                 nl[ks] = n;
                 double Bct[MAXB][2], Bpt[3][2];
+                float Wunit[3] = {1.f, 1.f, 0.f};
+                if (g_prec == 1) {
+                    for (int j = 0; j < n; ++j) { Bc[j] = (float)Bc[j]; Bc[MAXB + j] = (float)Bc[MAXB + j]; }
+                    for (int j = 0; j < 3; ++j) { Bp[j] = (float)Bp[j]; Bp[3 + j] = (float)Bp[3 + j]; }
+                } else if (g_prec == 2) {
+                    /* W = L L^T; whitened columns L^T B rounded once; residual L^T r */
+                    const double l11 = sqrt((double)W[0]), l21 = (double)W[2] / l11;
+                    const double l22 = sqrt((double)W[1] - l21 * l21);
+                    for (int j = 0; j < n; ++j) {
+                        const double a0 = l11 * Bc[j] + l21 * Bc[MAXB + j], a1 = l22 * Bc[MAXB + j];
+                        Bc[j] = (float)a0; Bc[MAXB + j] = (float)a1;
+                    }
+                    for (int j = 0; j < 3; ++j) {
+                        const double a0 = l11 * Bp[j] + l21 * Bp[3 + j], a1 = l22 * Bp[3 + j];
+                        Bp[j] = (float)a0; Bp[3 + j] = (float)a1;
+                    }
+                    const double r0 = l11 * e.diff1 + l21 * e.diff2, r1 = l22 * e.diff2;
+                    e.diff1 = r0;
+                    e.diff2 = r1;
+                    W = Wunit;
+                }
                 for (int j = 0; j < n; ++j) {
                     const double xp = Bc[j], yp = Bc[MAXB + j];
                     Bct[j][0] = xp * W[0] + yp * W[2];
                     Bct[j][1] = xp * W[2] + yp * W[1];
+                    if (g_prec == 1) { Bct[j][0] = (float)Bct[j][0]; Bct[j][1] = (float)Bct[j][1]; }
                 }
                 if (mode == 1 && n > 0) {
                     double *U = sblock(o, k, k);
@@ -756,7 +806,10 @@ static void corrections(orc *o, int mode, double mult1, const double *XM, int sr
                 }
                 double *Wq = Wl + (size_t)ks * MAXB * 3;
                 for (int r = 0; r < n; ++r)
-                    for (int c = 0; c < 3; ++c) Wq[r * 3 + c] = Bct[r][0] * Bp[c] + Bct[r][1] * Bp[3 + c];
+                    for (int c = 0; c < 3; ++c) {
+                        Wq[r * 3 + c] = Bct[r][0] * Bp[c] + Bct[r][1] * Bp[3 + c];
+                        if (g_prec == 1) Wq[r * 3 + c] = (float)Wq[r * 3 + c];
+                    }
                 for (int j = 0; j < 3; ++j) {
                     const double xp = Bp[j], yp = Bp[3 + j];
                     Bpt[j][0] = xp * W[0] + yp * W[2];
@@ -801,8 +854,14 @@ static void corrections(orc *o, int mode, double mult1, const double *XM, int sr
                 const double *Wa = Wl + (size_t)is * MAXB * 3;
                 double N[MAXB * 3]; /* eNlijN = eNlij * V^-1 */
                 for (int r = 0; r < ni; ++r)
-                    for (int c = 0; c < 3; ++c)
-                        N[r * 3 + c] = Wa[r * 3 + 0] * Vi[0 * 3 + c] + Wa[r * 3 + 1] * Vi[1 * 3 + c] + Wa[r * 3 + 2] * Vi[2 * 3 + c];
+                    for (int c = 0; c < 3; ++c) {
+                        if (g_prec == 1) {
+                            const float v0 = (float)Vi[0 * 3 + c], v1 = (float)Vi[1 * 3 + c], v2 = (float)Vi[2 * 3 + c];
+                            const float w0 = (float)Wa[r * 3 + 0], w1 = (float)Wa[r * 3 + 1], w2 = (float)Wa[r * 3 + 2];
+                            N[r * 3 + c] = (float)((float)((float)(w0 * v0) + (float)(w1 * v1)) + (float)(w2 * v2));
+                        } else
+                            N[r * 3 + c] = Wa[r * 3 + 0] * Vi[0 * 3 + c] + Wa[r * 3 + 1] * Vi[1 * 3 + c] + Wa[r * 3 + 2] * Vi[2 * 3 + c];
+                    }
                 double *D = sblock(o, ij, ij);
                 double *gc = o->ex + o->camparv[ij];
                 for (int r = 0; r < ni; ++r) {
@@ -1011,6 +1070,9 @@ int oracle_block_pcg_w(int nb, int begini, const int *off, double *const *Sb, co
     precond(nb, begini, off, Sb, Sf, Cinv, omega, r, h, tw, uw);
     for (int j = off[begini]; j < off[nb]; ++j) d[j] = h[j];
     if (msize < 1000) msize = 1000;
+    if (g_pcg_cap > 0 && msize > g_pcg_cap) msize = g_pcg_cap;
+    const int trace_this = (g_pcg_trace && g_pcg_calls == g_pcg_trace_call);
+    ++g_pcg_calls;
     double mold = 1.e50, m0 = 1.e50;
     int k;
     for (k = 0; k < msize; ++k) {
@@ -1029,6 +1091,10 @@ int oracle_block_pcg_w(int nb, int begini, const int *off, double *const *Sb, co
         }
         precond(nb, begini, off, Sb, Sf, Cinv, omega, r1, h1, tw, uw);
         if (k == 0) m0 = m;
+        if (trace_this && k < g_pcg_trace_cap) {
+            g_pcg_trace[k] = m / m0;
+            g_pcg_trace_len = k + 1;
+        }
         if (m < mold) {
             memcpy(xo, x, sizeof(double) * (size_t)n);
             mold = m;
@@ -1440,6 +1506,7 @@ int orc_adjust(orc *o, rpba_log_fn log, void *user, rpba_stats *st)
             }
             if (g_mut == 5 ? itt >= 200 : itt > 200) break;
             ++itt;
+            if (g_stop_after > 0 && itt >= g_stop_after) break;
             ++iter;
         }
         if (rc) break;
@@ -1460,6 +1527,7 @@ int orc_adjust(orc *o, rpba_log_fn log, void *user, rpba_stats *st)
         }
         if (robust && it == 0) robust_delete(o);
         ittflag = 1;
+        if (g_stop_after > 0 && itt >= g_stop_after) break;
     }
     if (st) {
         st->itt = itt;
