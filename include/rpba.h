@@ -43,7 +43,7 @@ enum rpba_status {
     RPBA_E_LOW_REDUNDANCY = -2,  /* "BA-Error: Low redundancy"           rbacore.cpp:468-476 */
     RPBA_E_NAN_RESULT = -3,      /* "BA-Error: NANs in result vector"    rbacore.cpp:541-546 */
     RPBA_E_INVALID = -10,        /* malformed arguments (bad index, duplicate camera in a track) */
-    RPBA_E_UNSUPPORTED = -11,    /* partitioned path (rpbacore.cpp:100-510) requested; not built */
+    RPBA_E_UNSUPPORTED = -11,    /* a form selector the partitioned solve does not take (it runs plain FP64, block-Jacobi PCG) */
     RPBA_E_SCRATCH = -12,        /* input larger than a no-allocation entry point's fixed scratch */
     RPBA_E_BUDGET = -13,         /* a PCG solve reached the bound of rpba_ctx_set_pcg_budget */
     RPBA_E_HIP = -20,            /* HIP runtime error */
@@ -135,11 +135,32 @@ typedef struct rpba_stats {
 
 typedef void (*rpba_log_fn)(const char *line, void *user);
 
-/* Whole solve (Exteval, or the global robust LM), device resident.
- * Mirrors CalcPLSA: rpbacore.cpp:28-99 (setup, dispatch, Exteval) and
- * 511-527 (single LSA: ctor, Adjust(robust, os, false, -1), OutputLSAc). */
+/* Whole solve, device resident.  Mirrors CalcPLSA: rpbacore.cpp:28-99
+ * (setup, dispatch, Exteval), 511-527 (single LSA: ctor, Adjust(robust, os,
+ * false, -1), OutputLSAc) and, where the reference dispatches to it
+ * (nrthreads = min(Threads, iround(N / 70)) > 1 and N > 5, Threads < 1 = every
+ * processor of the host, rpbacore.cpp:79-92), the partitioned solve
+ * (rpbacore.cpp:100-510): per outer iteration one GCP-mode LSA step of every
+ * image part and the per-point re-triangulation CalcLSAXMulti, with the image
+ * partition from rpba_partition_images.  Its stats: itt = outer iterations,
+ * trace_* per outer iteration (sig0g, 1 when saved as the best, the PCG
+ * iterations of its parts, nrobs), sig0_final = the best sig0g, rms_px of the
+ * output state; it takes plain FP64 forms only (RPBA_E_UNSUPPORTED
+ * otherwise).  force_global runs the global LSA instead. */
 int rpba_calc_plsa(rpba_block *io, const rpba_flags *flags,
                    rpba_log_fn log, void *user, rpba_stats *stats);
+
+/* The image partition of the partitioned solve; replaces PartitionpointXXv
+ * (rpbacore.cpp:945-1037), which calls METIS_PartGraphRecursive on the
+ * covisibility graph (images joined when they share a point of two or more
+ * observations; vertex weight (int) pow(sum of the track lengths of the
+ * image's points, 0.3333)).  Here: deterministic recursive bisection of the
+ * same graph with the same weights (graph growing from a pseudo-peripheral
+ * image, boundary moves that cut fewer edges within a 3% balance), not
+ * METIS's.  part[n_cams] receives part ids 0 .. n_parts - 1, every part
+ * non-empty; 1 <= n_parts <= n_cams. */
+int rpba_partition_images(int32_t n_cams, int32_t n_obs, const int32_t *obs_cam, const int32_t *obs_pt,
+                          int32_t n_pts, int32_t n_parts, int32_t *part);
 
 /* Resumable form of the same global solve. */
 typedef struct rpba_ctx rpba_ctx;

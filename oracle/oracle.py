@@ -68,6 +68,14 @@ def _load():
     lib.orc_sample_iteration.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double)]
     lib.orc_sample_iteration.restype = C.c_int
     lib.orc_delete_with_flags.argtypes = [C.c_void_p, P(C.c_uint8), P(C.c_int), P(C.c_int), P(C.c_int)]
+    lib.oracle_partition_solve.argtypes = [P(capi.RpbaBlock), P(capi.RpbaFlags), C.c_int, P(C.c_int32), capi.LOG_FN,
+                                           C.c_void_p, P(capi.RpbaStats)]
+    lib.oracle_partition_solve.restype = C.c_int
+    lib.oracle_lsax_point.argtypes = [P(C.c_double), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
+                                      P(C.c_int32), C.c_int, P(C.c_int32), C.c_int, P(C.c_float), P(C.c_int32),
+                                      C.c_int, P(C.c_double), C.c_int, P(C.c_int32), P(C.c_uint8), P(C.c_float),
+                                      C.c_int, C.c_int, C.c_int, P(C.c_double)]
+    lib.oracle_lsax_point.restype = C.c_int
     return lib
 
 
@@ -107,6 +115,51 @@ def solve(scene, flags=None, fsites=0, trace_cap=512):
     logc = capi.LogCollector()
     rc = lib().oracle_calc_plsa(C.byref(blk.c), C.byref(flags), fsites, logc.fn, None, C.byref(tr.stats))
     return rc, blk.result(), tr.as_dict(), logc.lines
+
+
+def partition_solve(scene, flags, part, trace_cap=512):
+    """The partitioned solve (CalcPLSA's parallel branch) on the image
+    partition part (one part id per camera).  Returns (status, outputs,
+    stats, log): stats["itt"] = outer iterations, trace_* per outer iteration
+    (sig0g, saved as best, PCG iterations of its blocks, nrobs)."""
+    part = np.ascontiguousarray(part, dtype=np.int32)
+    blk = capi.Block(scene)
+    tr = capi.Trace(trace_cap)
+    logc = capi.LogCollector()
+    rc = lib().oracle_partition_solve(C.byref(blk.c), C.byref(flags), int(part.max()) + 1,
+                                      part.ctypes.data_as(C.POINTER(C.c_int32)), logc.fn, None, C.byref(tr.stats))
+    return rc, blk.result(), tr.as_dict(), logc.lines
+
+
+def lsax_point(PM, cf, cd, obs_xy, obs_w, obs_cam, fo, X, part, spart, pit, robustmode=0, weightonly=0, wxyflag=1,
+               sig0wv=None):
+    """CalcLSAXMulti for one point (the oracle's lsax_point).  Returns
+    (convflag, X, kept file indices, w3 per slot (6 each), sig0 share,
+    redundancy share)."""
+    P = C.POINTER
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    PM, cf, cd, obs_xy = f64(PM), f32(cf), f32(cd), f32(obs_xy)
+    obs_w = None if obs_w is None else f32(obs_w)
+    obs_cam, fo, part, spart = i32(obs_cam), i32(fo).copy(), i32(part), i32(spart)
+    pit = np.ascontiguousarray(pit, dtype=np.uint8)
+    sig0wv = f32(np.zeros(len(cf)) if sig0wv is None else sig0wv)
+    X = f64(X).copy()
+    w3 = np.zeros(6 * max(len(spart), 1), dtype=np.float32)
+    out = np.zeros(3, dtype=np.float64)
+    n_out = np.zeros(1, dtype=np.int32)
+    conv = lib().oracle_lsax_point(PM.ctypes.data_as(P(C.c_double)), cf.ctypes.data_as(P(C.c_float)),
+                                   cd.ctypes.data_as(P(C.c_float)), obs_xy.ctypes.data_as(P(C.c_float)),
+                                   None if obs_w is None else obs_w.ctypes.data_as(P(C.c_float)),
+                                   obs_cam.ctypes.data_as(P(C.c_int32)), len(cf), part.ctypes.data_as(P(C.c_int32)),
+                                   int(part.max()) + 1, sig0wv.ctypes.data_as(P(C.c_float)),
+                                   fo.ctypes.data_as(P(C.c_int32)), len(fo), X.ctypes.data_as(P(C.c_double)),
+                                   len(spart), spart.ctypes.data_as(P(C.c_int32)), pit.ctypes.data_as(P(C.c_uint8)),
+                                   w3.ctypes.data_as(P(C.c_float)), robustmode, weightonly, wxyflag,
+                                   out.ctypes.data_as(P(C.c_double)))
+    n = int(round(out[2]))
+    return conv, X, fo[:n].copy(), w3.reshape(-1, 6), float(out[0]), int(round(out[1]))
 
 
 def exteval(scene, flags=None, fsites=0):

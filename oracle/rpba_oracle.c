@@ -339,6 +339,16 @@ typedef struct orc {
     orc_rec *rec;
     double *Rs, *Cs;         /* stored cameras in the (R, C) form of eRMnew / ePCnew */
     int pt_stride;           /* timing sample (orc_sample_iteration): every pt_stride-th point; 0 = all */
+    /* GCP mode of a partition block (aoflag, rbacore.cpp:233-329, 1677-1729):
+     * no gauge camera, every image with the full 6 + a block; gcpi[i] is the
+     * kept point's index into the GCP centres gcpc (3 each) or -1; the n-th
+     * kept GCP point takes entry n of the weight lists W3 (6 lists of nW3) */
+    int aoflag, nrcam_unk;
+    int *gcpi;
+    const float *W3;
+    int nW3;
+    const double *gcpc;
+    double sig0ao;
 } orc;
 
 #define FS(o, v) ((o)->fsites ? (double)(float)(v) : (double)(v))
@@ -370,6 +380,7 @@ void orc_destroy(orc *o)
         free(o->Sb);
     }
     free(o->Sf);
+    free(o->gcpi);
     free(o);
 }
 
@@ -563,6 +574,28 @@ static void encode_cameras(orc *o)
 {
     const int nc = o->nimg;
     for (int k = 0; k < nc + 1; ++k) o->camparv[k] = 0;
+    if (o->aoflag) {
+        /* GCP branch (rbacore.cpp:1005-1009, 1025-1082): every image, image 0
+         * included, is re-orthonormalised and gets a block of nrpari slots
+         * (pose increments zero, then its own intrinsics) */
+        for (int k = 0; k < nc; ++k) {
+            const int cp = o->camparv[k];
+            double *P = o->PM + 12 * k;
+            oracle_rectify(P, 4);
+            double *R = o->Rb + 9 * k, *C = o->Cb + 3 * k;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) R[r * 3 + c] = P[r * 4 + c];
+            for (int c = 0; c < 3; ++c) C[c] = R[0 * 3 + c] * P[3] + R[1 * 3 + c] * P[7] + R[2 * 3 + c] * P[11];
+            for (int i = 0; i < 6; ++i) o->XP[cp + i] = 0.;
+            int a = 5 + cp;
+            if (o->addf[0]) o->XP[++a] = o->cd0[k];
+            if (o->addf[1]) o->XP[++a] = o->cd1[k];
+            if (o->addf[2]) o->XP[++a] = o->cf[k];
+            o->camparv[k + 1] = cp + o->nrpari;
+        }
+        o->nrioff3 = o->nrioff + 3 * o->npts;
+        return;
+    }
     o->camparv[1] = o->nraddpi;
     int addp = -1;
     if (o->addf[0]) o->XP[++addp] = o->cd0[0];
@@ -608,7 +641,8 @@ static void encode_cameras(orc *o)
  * pose; every other image gets Cay(theta_rot) * base, rectified. */
 static void decode_cameras(orc *o, const double *XM)
 {
-    for (int k = 1; k < o->nimg; ++k) {
+    /* GCP mode: image 0 is decoded like the others (rbacore.cpp:1740) */
+    for (int k = o->aoflag ? 0 : 1; k < o->nimg; ++k) {
         const int cp = o->camparv[k];
         double Ra[9], *Rn = o->Rn + 9 * k;
         const double *Rb = o->Rb + 9 * k;
@@ -633,6 +667,7 @@ static void decode_cameras(orc *o, const double *XM)
         if (o->addf[1]) o->nd1[k] = XM[++a + cp];
         if (o->addf[2]) o->nf[k] = XM[++a + cp];
     }
+    if (o->aoflag) return;
     int a = -1;
     if (o->addf[0]) o->nd0[0] = XM[++a];
     if (o->addf[1]) o->nd1[0] = XM[++a];
@@ -714,7 +749,7 @@ static int derivs(const orc *o, int k, const obs_eval_t *e, double *Bc, double *
     id1[0] = -u * xyp4 * f - v * xyp4 * 0.; id1[1] = -v * xyp4 * f;
     iff[0] = -e->su; iff[1] = -e->sv;
     int n = 0;
-    if (k > 0) {
+    if (k > 0 || o->aoflag) { /* k >= begini (rbacore.cpp:1441) */
         double rc[2][3];
         rc[0][0] = x2 * x1 * xs32;       rc[1][0] = 1. + x2 * x2 * xs32;
         rc[0][1] = -1. - x1 * x1 * xs32; rc[1][1] = -x2 * x1 * xs32;
@@ -748,7 +783,8 @@ static double *sblock(orc *o, int k, int l)
  * 1 = linearise + Schur, 2 = back-substitution into x. */
 static void corrections(orc *o, int mode, double mult1, const double *XM, int srmv)
 {
-    double sig0 = 0.;
+    double sig0 = 0., sao = 0.;
+    int gcpnum = 0;
     int kkk = o->nrioff - 1;
     const int mt = o->maxtrack > 0 ? o->maxtrack : 1;
     double *Wl = (double *)xcalloc((size_t)mt * MAXB * 3, sizeof(double)); /* eNlij */
@@ -837,6 +873,27 @@ static void corrections(orc *o, int mode, double mult1, const double *XM, int sr
                 else
                     sig0 += e.diff1 * e.diff1 * (double)W[0] + 2. * e.diff1 * e.diff2 * (double)W[2] +
                             e.diff2 * e.diff2 * (double)W[1];
+            }
+        }
+        if (o->aoflag && o->gcpi[i] > -1) {
+            /* CalcDerivatives3DGCP (rbacore.cpp:1677-1729): the prior on the
+             * point's 3D position, B = -I, weight W3 of this GCP number */
+            const int gn = gcpnum++;
+            const double *cg = o->gcpc + 3 * (size_t)o->gcpi[i];
+            const double d0 = cg[0] - X[0], d1 = cg[1] - X[1], d2 = cg[2] - X[2];
+            double w[6];
+            for (int v = 0; v < 6; ++v) w[v] = (double)o->W3[(size_t)v * o->nW3 + gn];
+            sao += d0 * d0 * w[0] + d1 * d1 * w[1] + d2 * d2 * w[2] + 2 * (d0 * d1 * w[3] + d0 * d2 * w[4] + d1 * d2 * w[5]);
+            if (mode > 0) {
+                const double Ws[3][3] = {{w[0], w[3], w[4]}, {w[3], w[1], w[5]}, {w[4], w[5], w[2]}};
+                double Bt[3][3];
+                for (int j = 0; j < 3; ++j)
+                    for (int c = 0; c < 3; ++c) Bt[j][c] = -Ws[j][c];
+                const double dd[3] = {d0, d1, d2};
+                for (int j = 0; j < 3; ++j) {
+                    for (int c = 0; c < 3; ++c) V[j * 3 + c] += -Bt[j][c];
+                    g[j] += Bt[j][0] * dd[0] + Bt[j][1] * dd[1] + Bt[j][2] * dd[2];
+                }
             }
         }
         if (mode == 0) continue;
@@ -935,6 +992,7 @@ static void corrections(orc *o, int mode, double mult1, const double *XM, int sr
     free(Wl);
     free(nl);
     o->sig0 = sig0;
+    o->sig0ao = sao;
 }
 
 /* y = S d over the touched blocks, both triangles (eAAdv rbacore.cpp:2048-2068) */
@@ -1072,7 +1130,7 @@ int oracle_block_pcg_w(int nb, int begini, const int *off, double *const *Sb, co
     if (msize < 1000) msize = 1000;
     if (g_pcg_cap > 0 && msize > g_pcg_cap) msize = g_pcg_cap;
     const int trace_this = (g_pcg_trace && g_pcg_calls == g_pcg_trace_call);
-    ++g_pcg_calls;
+    if (g_pcg_trace) ++g_pcg_calls; /* only while a diagnostic traces (the partitioned solve runs blocks in threads) */
     double mold = 1.e50, m0 = 1.e50;
     int k;
     for (k = 0; k < msize; ++k) {
@@ -1188,7 +1246,7 @@ static int bundle_sep(orc *o, double mult)
 {
     const double mult1 = 1. + mult;
     const int nc = o->nimg;
-    o->begini = (o->nraddpi > 0) ? 0 : 1;
+    o->begini = (o->aoflag || o->nraddpi > 0) ? 0 : 1; /* rbacore.cpp:1104-1106 */
     for (size_t id = 0; id < (size_t)nc * nc; ++id) {
         o->Sf[id] = 0;
         if (o->Sb[id]) memset(o->Sb[id], 0, sizeof(double) * MAXB * MAXB);
@@ -1707,7 +1765,7 @@ int orc_sample_iteration(orc *o, int stride, int reps, double *out)
     decode_cameras(o, o->XP);
     o->pt_stride = stride;
     const int nc = o->nimg;
-    o->begini = (o->nraddpi > 0) ? 0 : 1;
+    o->begini = (o->aoflag || o->nraddpi > 0) ? 0 : 1; /* rbacore.cpp:1104-1106 */
     for (int i = o->begini; i < nc; ++i) sblock(o, i, i);
     ORC_NOW(t0);
     corrections(o, 1, 1.001, o->XP, 0);
@@ -1803,7 +1861,7 @@ void orc_linearize(orc *o, double mult, double *Sd, double *b)
 {
     const int nc = o->nimg, n = o->nrioff;
     const double mult1 = 1. + mult;
-    o->begini = (o->nraddpi > 0) ? 0 : 1;
+    o->begini = (o->aoflag || o->nraddpi > 0) ? 0 : 1; /* rbacore.cpp:1104-1106 */
     for (size_t id = 0; id < (size_t)nc * nc; ++id) {
         o->Sf[id] = 0;
         if (o->Sb[id]) memset(o->Sb[id], 0, sizeof(double) * MAXB * MAXB);
@@ -1872,3 +1930,6 @@ int oracle_pcg_dense_w(int nb, const int *bs, const double *Sd, const double *b,
     free(Sb); free(Sf); free(off);
     return it;
 }
+
+/* the partitioned solve (rpbacore.cpp:100-510, 536-877) */
+#include "rpba_oracle_part.c"

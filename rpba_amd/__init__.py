@@ -23,7 +23,7 @@ EXPORTS = [
     "rpba_lp_create", "rpba_lp_destroy", "rpba_lp_n_params", "rpba_lp_get_params", "rpba_lp_predict",
     "rpba_lp_gradient", "rpba_lp_train", "rpba_ctx_set_lp", "rpba_ctx_last_omega", "rpba_ctx_lp_sample",
     "rpba_read_bal", "rpba_write_bal", "rpba_free_block", "rpba_read_parameters", "rpba_version",
-    "rpba_exteval_device", "rpba_exteval_device_max_points", "rpba_ctx_set_pcg_budget",
+    "rpba_exteval_device", "rpba_exteval_device_max_points", "rpba_ctx_set_pcg_budget", "rpba_partition_images",
 ]
 
 _lib = None
@@ -108,5 +108,8 @@ def lib():
     L.rpba_exteval_device.argtypes = [C.c_int32, C.c_int32] + [vp] * 10 + [C.c_int32] * 3 + [vp, vp]
     L.rpba_exteval_device.restype = C.c_int
     L.rpba_exteval_device_max_points.restype = C.c_int
+    L.rpba_partition_images.argtypes = [C.c_int32, C.c_int32, P(C.c_int32), P(C.c_int32), C.c_int32, C.c_int32,
+                                        P(C.c_int32)]
+    L.rpba_partition_images.restype = C.c_int
     _lib = L
     return L

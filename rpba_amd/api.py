@@ -21,6 +21,21 @@ def calc_plsa(scene, flags=None, trace_cap=512):
     return rc, blk.result(), tr.as_dict(), logc.lines
 
 
+def partition_images(scene, nparts):
+    """The image partition of the partitioned solve (rpba_partition_images):
+    one part id per camera.  Host code only; needs no GPU."""
+    import numpy as np
+    cam = np.ascontiguousarray(scene["obs_cam"], dtype=np.int32)
+    pt = np.ascontiguousarray(scene["obs_pt"], dtype=np.int32)
+    part = np.zeros(scene["n_cams"], dtype=np.int32)
+    P = C.POINTER(C.c_int32)
+    rc = lib().rpba_partition_images(scene["n_cams"], len(cam), cam.ctypes.data_as(P), pt.ctypes.data_as(P),
+                                     scene["n_pts"], nparts, part.ctypes.data_as(P))
+    if rc != 0:
+        raise RuntimeError(f"rpba_partition_images failed with status {rc}")
+    return part
+
+
 def dense_solve_batch(systems, rhs, variant=-1, cap=0):
     """Solve the SPD systems (list of (9k x 9k) arrays) with the batched dense
     Cholesky of the dense form; returns the list of solutions."""

@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # operation order of the reference formulas; packed FP32 instructions off.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
-SOURCES = ["kernels.hip", "noalloc.hip", "dense.hip", "lp.hip", "structure_dev.hip", "solver.cpp", "batch.cpp", "structure.cpp", "bal_io.cpp"]
+SOURCES = ["kernels.hip", "noalloc.hip", "dense.hip", "lp.hip", "structure_dev.hip", "partition_kernels.hip", "solver.cpp", "partition.cpp", "batch.cpp", "structure.cpp", "bal_io.cpp"]
 
 
 def _run(cmd):

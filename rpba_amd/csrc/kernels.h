@@ -61,6 +61,12 @@ struct LinArgs {
     const int *yrank;  // matrix-free form: Y records in camera-major order, entry q at yrank[q] (else null)
     int nch;     // chunks of the camera-major list (at most 64 entries of one image each)
     const int *ch_start, *ch_off;  // chunk -> first entry (nch + 1); image -> first chunk (nc + 1)
+    // partition blocks (the GCP mode of LSA): per point the index of its
+    // position prior or -1, the priors' centres (3 doubles each) and
+    // weights (6 floats each: XX, YY, ZZ, XY, XZ, YZ); null elsewhere
+    const int *gcp_of = nullptr;
+    const double *gcp_c = nullptr;
+    const float *gcp_w = nullptr;
 };
 
 struct SchurArgs {

@@ -362,6 +362,26 @@ __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
         __syncthreads();
     }
     if (t >= np) return;
+    if (a.gcp_of) {
+        // the point's position prior (CalcDerivatives3DGCP rbacore.cpp:1677-1729):
+        // B = -I, so V += W3 and g += -W3 (centre - X), added after its
+        // observations and before the damping
+        const int gk = a.gcp_of[i];
+        if (gk >= 0) {
+            const double *cg = a.gcp_c + 3 * (size_t)gk;
+            const float *w = a.gcp_w + 6 * (size_t)gk;
+            const double dd[3] = {cg[0] - a.X[3 * (size_t)i], cg[1] - a.X[3 * (size_t)i + 1], cg[2] - a.X[3 * (size_t)i + 2]};
+            const double Ws[9] = {(double)w[0], (double)w[3], (double)w[4], (double)w[3], (double)w[1],
+                                  (double)w[5], (double)w[4], (double)w[5], (double)w[2]};
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const double b0 = -Ws[r * 3], b1 = -Ws[r * 3 + 1], b2 = -Ws[r * 3 + 2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) V[r * 3 + c] += -(-Ws[r * 3 + c]);
+                g[r] += b0 * dd[0] + b1 * dd[1] + b2 * dd[2];
+            }
+        }
+    }
     V[0] *= a.mult1;
     V[4] *= a.mult1;
     V[8] *= a.mult1;

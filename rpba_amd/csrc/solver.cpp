@@ -5,6 +5,7 @@
 // in kernels.hip; the host keeps the scalars of the state machine and reads
 // back a few of them per LM iteration.
 #include <hip/hip_runtime.h>
+#include <sched.h>
 
 #include <algorithm>
 #include <chrono>
@@ -22,6 +23,7 @@
 #include "host_util.h"
 #include "kernels.h"
 #include "lp.h"
+#include "partition.h"
 #include "structure.h"
 #include "structure_dev.h"
 
@@ -956,14 +958,30 @@ static int ctx_init(rpba_ctx *c)
 
 extern "C" int rpba_version(void) { return RPBA_ABI_VERSION; }
 
-static bool partition_dispatch(const rpba_flags *f, int nc)
+// processors available to this process (omp_get_num_procs, system.h:49-76)
+static int host_procs()
+{
+    cpu_set_t cs;
+    CPU_ZERO(&cs);
+    if (sched_getaffinity(0, sizeof cs, &cs) == 0) {
+        const int n = CPU_COUNT(&cs);
+        if (n > 0) return n;
+    }
+    const int n = (int)std::thread::hardware_concurrency();
+    return n > 0 ? n : 1;
+}
+
+// the number of parts of CalcPLSA's partitioned branch, or 0 for the global
+// solve: nrthreads = min(Threads, iround(N / 70)), Threads < 1 meaning every
+// processor (rpba.cpp:85-86), partitioned when nrthreads > 1 and N > 5
+// (rpbacore.cpp:79-92)
+static int partition_dispatch(const rpba_flags *f, int nc)
 {
     int mt = f->threads;
-    if (mt < 1) mt = (int)std::thread::hardware_concurrency();
-    if (mt < 1) mt = 1;
+    if (mt < 1) mt = host_procs();
     const int minnrthreads = (int)std::lround((double)((float)nc / 70.f));  // rpbacore.cpp:79-85
     const int nrthreads = std::min(mt, minnrthreads);
-    return nrthreads > 1 && nc > 5;  // rpbacore.cpp:92
+    return (nrthreads > 1 && nc > 5) ? nrthreads : 0;  // rpbacore.cpp:92
 }
 
 extern "C" int rpba_ctx_create(rpba_ctx **out, rpba_block *io, const rpba_flags *flags, rpba_log_fn log, void *user)
@@ -1376,9 +1394,16 @@ extern "C" int rpba_calc_plsa(rpba_block *io, const rpba_flags *flags, rpba_log_
 {
     if (!io || !flags) return RPBA_E_INVALID;
     if (flags->exteval) return exteval(io, flags, log, user, stats);  // rpbacore.cpp:94-97
-    if (partition_dispatch(flags, io->n_cams) && !flags->force_global) {
-        if (log) log("rpba: the partitioned solve (rpbacore.cpp:100-510) is not available; set Threads = 1\n", user);
-        return RPBA_E_UNSUPPORTED;
+    const int nparts = partition_dispatch(flags, io->n_cams);
+    if (nparts && !flags->force_global) {
+        // the partitioned solve (rpbacore.cpp:100-510): plain FP64 arithmetic
+        // and the block-Jacobi PCG of the reference
+        if (flags->precision != 0 || flags->preconditioner != 0 || flags->schur != 0 || flags->solver != 0)
+            return RPBA_E_UNSUPPORTED;
+        if (!io->obs_cam || !io->obs_pt || !io->obs_xy || !io->cam_P || !io->cam_f || !io->cam_d || !io->pts ||
+            io->n_cams < 1 || io->n_pts < 0 || io->n_obs < 0)
+            return RPBA_E_INVALID;
+        return rpba::partitioned_solve(io, flags, nparts, log, user, stats);
     }
     rpba_ctx *c = nullptr;
     int rc = rpba_ctx_create(&c, io, flags, log, user);

@@ -148,14 +148,6 @@ def test_error_no_observations_in_image():
     assert np.array_equal(out1["pts"], out0["pts"])
 
 
-def test_partition_path_is_refused():
-    sc = scenes.generate(None, seed=9, n_cams=150, n_pts=300, track="uniform2_8", outlier_frac=0.0)
-    rc, _, _, log = api.calc_plsa(sc, capi.make_flags(threads=0))
-    assert rc == capi.RPBA_E_UNSUPPORTED
-    rc, _, st, _ = api.calc_plsa(sc, capi.make_flags(threads=0, force_global=1))
-    assert rc == 0 and st["passes_done"] == 2
-
-
 
 def _banded_scene():
     # each point seen by cameras within a window of 10 consecutive ones: S is
