@@ -68,6 +68,12 @@ def _load():
     lib.orc_sample_iteration.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double)]
     lib.orc_sample_iteration.restype = C.c_int
     lib.orc_delete_with_flags.argtypes = [C.c_void_p, P(C.c_uint8), P(C.c_int), P(C.c_int), P(C.c_int)]
+    lib.orc_create_block_io.argtypes = [P(capi.RpbaBlock), P(capi.RpbaFlags), P(C.c_int32), P(C.c_float), C.c_int,
+                                        P(C.c_double), C.c_int, C.c_int]
+    lib.orc_create_block_io.restype = C.c_void_p
+    lib.orc_setup_block.argtypes = [C.c_void_p]
+    lib.orc_setup_block.restype = C.c_int
+    lib.orc_kept_points.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_int32)]
     lib.oracle_partition_solve.argtypes = [P(capi.RpbaBlock), P(capi.RpbaFlags), C.c_int, P(C.c_int32), capi.LOG_FN,
                                            C.c_void_p, P(capi.RpbaStats)]
     lib.oracle_partition_solve.restype = C.c_int
@@ -323,6 +329,38 @@ class Handle:
         x = np.zeros(self.dims()["nrioff3"], dtype=np.float64)
         its = lib().orc_step(self.h, mult, _p(x, C.c_double))
         return x, its
+
+
+class BlockHandle(Handle):
+    """A part's LSA in GCP mode (the partitioned solve's block step) on a
+    plain block, set up to the trial cameras of its linearisation; the probes
+    of Handle apply.  gcpin: per point its GCP number or -1; W3: (nW3, 6) the
+    part's weight list (XX, YY, ZZ, XY, XZ, YZ); gcpc: (n, 3) the centres."""
+
+    def __init__(self, scene, flags, gcpin, W3, gcpc, nrcam_unk=None, rect0=0):
+        self.flags = flags
+        self.blk = capi.Block(scene)
+        self._gcpin = np.ascontiguousarray(gcpin, dtype=np.int32)
+        W3 = np.asarray(W3, dtype=np.float32).reshape(-1, 6)
+        self._nW3 = W3.shape[0]
+        self._W3 = np.ascontiguousarray(W3.T)  # six lists of nW3
+        self._gcpc = np.ascontiguousarray(gcpc, dtype=np.float64).reshape(-1)
+        if self._gcpc.size == 0:
+            self._gcpc = np.zeros(3)
+        if self._W3.size == 0:
+            self._W3 = np.zeros(6, dtype=np.float32)
+        n = scene["n_cams"] if nrcam_unk is None else nrcam_unk
+        self.h = lib().orc_create_block_io(C.byref(self.blk.c), C.byref(self.flags), _p(self._gcpin, C.c_int32),
+                                           _p(self._W3, C.c_float), self._nW3, _p(self._gcpc, C.c_double), n, rect0)
+
+    def setup(self):
+        return lib().orc_setup_block(self.h)
+
+    def kept(self):
+        n = self.dims()["npts"]
+        a, g = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        lib().orc_kept_points(self.h, _p(a, C.c_int32), _p(g, C.c_int32))
+        return a, g
 
 
 def pcg_dense(block_sizes, S, b, omega=0.0):

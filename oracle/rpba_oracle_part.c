@@ -902,3 +902,70 @@ int oracle_lsax_point(const double *PMd, const float *cf, const float *cd, const
     out[2] = (double)nk;
     return conv;
 }
+
+/* test hooks: a part's LSA in GCP mode built from a plain block (its images
+ * 0 .. n_cams - 1, tracks in the block's file order, gcpin per point), then
+ * the pass set-up of the step up to the trial cameras of the linearisation,
+ * so that the probes (orc_dims, orc_camparv, orc_get_theta, orc_residuals,
+ * orc_jacobians, orc_linearize, orc_step) work on it */
+orc *orc_create_block_io(const rpba_block *io, const rpba_flags *f, const int *gcpin, const float *W3, int nW3,
+                         const double *gcpc, int nrcam_unk, int rect0)
+{
+    const int nc = io->n_cams, np = io->n_pts, no = io->n_obs;
+    int *toff = (int *)xcalloc((size_t)np + 1, sizeof(int));
+    for (int q = 0; q < no; ++q) toff[io->obs_pt[q] + 1]++;
+    for (int i = 0; i < np; ++i) toff[i + 1] += toff[i];
+    int *pos = (int *)xcalloc((size_t)np + 1, sizeof(int));
+    memcpy(pos, toff, sizeof(int) * (size_t)np);
+    int *tcam = (int *)xcalloc((size_t)no + 1, sizeof(int)), *tfo = (int *)xcalloc((size_t)no + 1, sizeof(int));
+    for (int q = 0; q < no; ++q) {
+        const int at = pos[io->obs_pt[q]]++;
+        tcam[at] = io->obs_cam[q];
+        tfo[at] = q;
+    }
+    double *PM = (double *)xcalloc((size_t)12 * nc, sizeof(double)), *X = (double *)xcalloc((size_t)3 * np + 3, sizeof(double));
+    for (int i = 0; i < 12 * nc; ++i) PM[i] = (double)io->cam_P[i];
+    for (int i = 0; i < 3 * np; ++i) X[i] = (double)io->pts[i];
+    float ihw2 = 0.f;
+    for (int i = 0; i < nc; ++i) {
+        const int w = io->img_w ? io->img_w[i] : 1, h = io->img_h ? io->img_h[i] : 1;
+        ihw2 += (float)((h + w) * (h + w)) * 0.25f;
+    }
+    ihw2 /= (float)nc;
+    const int addf[3] = {f->est_k2 != 0, f->est_k4 != 0, f->est_f != 0};
+    orc *o = orc_create_block(nc, PM, io->cam_f, io->cam_d, np, X, toff, tcam, tfo, io->obs_xy, io->obs_w, gcpin, addf,
+                              ihw2, nrcam_unk, rect0);
+    o->W3 = W3;
+    o->nW3 = nW3;
+    o->gcpc = gcpc;
+    free(toff); free(pos); free(tcam); free(tfo); free(PM); free(X);
+    return o;
+}
+
+int orc_setup_block(orc *o)
+{
+    o->imagepointnr = o->ntr;
+    int nrgcp = 0;
+    for (int i = 0; i < o->npts; ++i)
+        if (o->gcpi[i] > -1) ++nrgcp;
+    o->nrobs = 2 * o->ntr + 3 * nrgcp;
+    const int rc = count_images(o, NULL, NULL);
+    if (rc) return rc;
+    prepare_weights(o);
+    for (int i = 0; i < o->npts; ++i)
+        for (int j = 0; j < 3; ++j) o->XP[o->nrioff + 3 * i + j] = o->XX[3 * i + j];
+    encode_cameras(o);
+    o->nrunknowns = o->nrioff3 + o->nraddpi * o->nrcam_unk;
+    memcpy(o->XXn, o->XX, sizeof(double) * 3 * (size_t)o->npts);
+    decode_cameras(o, o->XP);
+    return 0;
+}
+
+/* the kept points' original indices (nrpv) and GCP numbers */
+void orc_kept_points(const orc *o, int *nrpv, int *gcpi)
+{
+    for (int i = 0; i < o->npts; ++i) {
+        nrpv[i] = o->nrpv[i];
+        gcpi[i] = o->gcpi ? o->gcpi[i] : -1;
+    }
+}
