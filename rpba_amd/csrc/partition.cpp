@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -231,6 +232,27 @@ int partition_images(int nc, int nobs, const int *obs_cam, const int *obs_pt, in
 
 namespace {
 
+// RPBA_PART_TRACE=1: the phases of every outer iteration with their wall
+// times on stderr (the stream synchronised at each mark)
+struct PhaseClock {
+    bool on = false;
+    std::chrono::steady_clock::time_point t;
+    PhaseClock()
+    {
+        const char *e = std::getenv("RPBA_PART_TRACE");
+        on = e && e[0] == '1';
+        t = std::chrono::steady_clock::now();
+    }
+    void mark(hipStream_t st, const char *what, int iter)
+    {
+        if (!on) return;
+        if (st) (void)hipStreamSynchronize(st);
+        const auto n = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "rpba part: iter %2d %-26s %9.1f ms\n", iter, what, std::chrono::duration<double, std::milli>(n - t).count());
+        t = n;
+    }
+};
+
 struct PartSolver {
     rpba_block *io = nullptr;
     rpba_log_fn log = nullptr;
@@ -254,16 +276,19 @@ struct PartSolver {
     DBuf<double> d_X, d_PMd, d_nv, d_sg;
     DBuf<float> d_cf, d_cd, d_oxy, d_ow, d_sig0wv, d_w3, d_wk, d_keys;
     DBuf<unsigned char> d_pit, d_dl;
-    std::vector<char> logbuf;
     int64_t pcg_total = 0;
+    PhaseClock clk;
+    int cur_iter = 0;
 
     void logs(const char *s)
     {
         if (log) log(s, user);
     }
+    std::vector<int> ecam, eslot;  // per entry: image, slot (entry_index)
     int point_pass(int robustmode, int weightonly, int wxyflag, std::vector<double> &sg, std::vector<int> &rd,
                    std::vector<float> *keys);
     int block_step(int iter, const int addf[3], int &pcg_it);
+    void entry_index();
 };
 
 // CalcLSAXMulti over every point with a track (rpbacore.cpp:311-392)
@@ -312,59 +337,63 @@ int PartSolver::point_pass(int robustmode, int weightonly, int wxyflag, std::vec
 
 // every valid part's LSA step (GCP mode, one iteration, rbacore.cpp:333-616 with
 // itbreakflag; OutputLSAP 620-640) as one launch sequence over their union
+// per current entry e (trk positions base[i] .. base[i] + len[i]): its
+// image and its slot, the point's slot of the entry's part
+void PartSolver::entry_index()
+{
+    ecam.resize(trk.size());
+    eslot.resize(trk.size());
+    for (int i = 0; i < pxs; ++i) {
+        const long long b = base[i], s0 = soff[i];
+        for (int q = 0; q < len[i]; ++q) {
+            const int c = io->obs_cam[trk[(size_t)(b + q)]];
+            ecam[(size_t)(b + q)] = c;
+            const int p = part[c];
+            long long s = s0;
+            while (spart[(size_t)s] != p) ++s;  // the slots hold every part of the initial track
+            eslot[(size_t)(b + q)] = (int)s;
+        }
+    }
+}
+
 int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
 {
     pcg_it = 0;
     const int nraddpi = addf[0] + addf[1] + addf[2], nrpari = 6 + nraddpi;
-    // each part's points (global order) and their entries in the part
-    std::vector<long long> lo((size_t)P + 1, 0);
-    for (int i = 0; i < pxs; ++i) {
-        if (!len[i]) continue;
-        for (long long s = soff[i]; s < soff[(size_t)i + 1]; ++s) {
-            int c = 0;
-            for (int q = 0; q < len[i]; ++q) c += part[io->obs_cam[trk[(size_t)(base[i] + q)]]] == spart[(size_t)s];
-            if (c) lo[(size_t)spart[(size_t)s] + 1]++;
-        }
-    }
-    for (int p = 0; p < P; ++p) lo[(size_t)p + 1] += lo[p];
-    std::vector<int> lpt((size_t)lo[P]), lcnt((size_t)lo[P]);
-    std::vector<long long> lslot((size_t)lo[P]);
-    {
-        std::vector<long long> w(lo.begin(), lo.end() - 1);
-        for (int i = 0; i < pxs; ++i) {
-            if (!len[i]) continue;
-            for (long long s = soff[i]; s < soff[(size_t)i + 1]; ++s) {
-                int c = 0;
-                for (int q = 0; q < len[i]; ++q) c += part[io->obs_cam[trk[(size_t)(base[i] + q)]]] == spart[(size_t)s];
-                if (!c) continue;
-                const long long at = w[(size_t)spart[(size_t)s]]++;
-                lpt[(size_t)at] = i;
-                lcnt[(size_t)at] = c;
-                lslot[(size_t)at] = s;
-            }
-        }
-    }
+    const size_t NS = spart.size();
+    entry_index();
+    // entries of each point in each of its parts
+    std::vector<int> scnt(NS, 0);
+    for (int i = 0; i < pxs; ++i)
+        for (int q = 0; q < len[i]; ++q) scnt[(size_t)eslot[(size_t)(base[i] + q)]]++;
     // PointXXvIn with GCP information (rbacore.cpp:1949-1962): a point stays
-    // with two entries, or one if its GCP number is > 0; validity of each part
-    // (an image without entries, or a negative redundancy, ends its Adjust
-    // before the step: rbacore.cpp:401-430, 466-476)
+    // in a part with two entries there, or one if its GCP number is > 0;
+    // validity of each part (an image without entries, or a negative
+    // redundancy, ends its Adjust before the step: rbacore.cpp:401-430, 466-476)
     std::vector<char> valid((size_t)P, 1), lowred((size_t)P, 0);
     std::vector<int> npk((size_t)P, 0), ntk((size_t)P, 0), ngk((size_t)P, 0);
-    std::vector<int> imgcnt((size_t)N, 0);
-    for (int p = 0; p < P; ++p) {
-        for (long long a = lo[p]; a < lo[(size_t)p + 1]; ++a) {
-            const int i = lpt[(size_t)a], c = lcnt[(size_t)a];
-            const bool keep = std::fabs(XXd[3 * (size_t)i + 2]) > 1.e-8 && (c >= 2 || (gcpinfo[i] > 0 && c > 0));
-            bkept[(size_t)lslot[(size_t)a]] = keep ? 1 : 0;
+    for (int i = 0; i < pxs; ++i) {
+        if (!len[i]) continue;
+        const bool zok = std::fabs(XXd[3 * (size_t)i + 2]) > 1.e-8;
+        for (long long sl = soff[i]; sl < soff[(size_t)i + 1]; ++sl) {
+            const int c = scnt[(size_t)sl];
+            if (!c) continue;
+            const bool keep = zok && (c >= 2 || (gcpinfo[i] > 0 && c > 0));
+            bkept[(size_t)sl] = keep ? 1 : 0;
             if (!keep) continue;
+            const int p = spart[(size_t)sl];
             npk[p]++;
             ntk[p] += c;
             if (gcpinfo[i] > -1) ngk[p]++;
-            for (int q = 0; q < len[i]; ++q) {
-                const int j = io->obs_cam[trk[(size_t)(base[i] + q)]];
-                if (part[j] == p) imgcnt[j]++;
-            }
         }
+    }
+    std::vector<int> imgcnt((size_t)N, 0);
+    for (int i = 0; i < pxs; ++i)
+        for (int q = 0; q < len[i]; ++q) {
+            const size_t e = (size_t)(base[i] + q);
+            if (bkept[(size_t)eslot[e]]) imgcnt[ecam[e]]++;
+        }
+    for (int p = 0; p < P; ++p) {
         for (int k = pcnt[p]; k < pcnt[(size_t)p + 1]; ++k)
             if (imgcnt[pimg[k]] == 0) valid[p] = 0;
         if (valid[p]) {
@@ -384,12 +413,16 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
         if (valid[p] || lowred[p])
             for (int k = pcnt[p]; k < pcnt[(size_t)p + 1]; ++k) rectify34(PMd.data() + 12 * (size_t)pimg[k]);
     }
-    // the union of the valid parts
+    // the union of the valid parts: cameras part by part, points part-major
+    // in global order, entries in track order
     std::vector<int> ucam, boff(1, 0), ublk_of;  // union camera -> image; block -> first union camera
-    std::vector<int> bpart;                      // union block -> part
+    std::vector<int> bpart, pblk((size_t)P, -1);  // union block -> part; part -> union block
     std::vector<int> uidx((size_t)N, -1);
+    std::vector<long long> uoff((size_t)P + 1, 0);
     for (int p = 0; p < P; ++p) {
+        uoff[(size_t)p + 1] = uoff[p] + (valid[p] ? npk[p] : 0);
         if (!valid[p]) continue;
+        pblk[p] = (int)bpart.size();
         for (int k = pcnt[p]; k < pcnt[(size_t)p + 1]; ++k) {
             uidx[pimg[k]] = (int)ucam.size();
             ucam.push_back(pimg[k]);
@@ -399,50 +432,76 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
         boff.push_back((int)ucam.size());
     }
     const int nb = (int)bpart.size(), nc = (int)ucam.size();
+    clk.mark(st, "block lists and validity", cur_iter);
     if (nb == 0) return RPBA_OK;
+    const int nup = (int)uoff[P];
+    std::vector<int> up(NS, -1);
     Structure us;
     us.nc = nc;
-    us.toff.push_back(0);
-    std::vector<float> w0;
-    std::vector<double> Xu;
-    std::vector<int> gcp_of, ptblk;
-    std::vector<double> gcc;
-    std::vector<float> gcw;
-    for (int bb = 0; bb < nb; ++bb) {
-        const int p = bpart[bb];
-        int gn = 0;
-        for (long long a = lo[p]; a < lo[(size_t)p + 1]; ++a) {
-            if (!bkept[(size_t)lslot[(size_t)a]]) continue;
-            const int i = lpt[(size_t)a];
-            const int up = (int)us.nrpv.size();
-            us.nrpv.push_back(i);
-            for (int q = 0; q < len[i]; ++q) {
-                const int fo = trk[(size_t)(base[i] + q)], j = io->obs_cam[fo];
-                if (part[j] != p) continue;
-                us.tcam.push_back(uidx[j]);
-                us.tpt.push_back(up);
-                us.tfo.push_back(fo);
-                w0.push_back(io->obs_w ? io->obs_w[3 * (size_t)fo] : 1.f);
-                w0.push_back(io->obs_w ? io->obs_w[3 * (size_t)fo + 1] : 1.f);
-                w0.push_back(io->obs_w ? io->obs_w[3 * (size_t)fo + 2] : 0.f);
-            }
-            us.toff.push_back((int)us.tcam.size());
-            for (int j = 0; j < 3; ++j) Xu.push_back(XXd[3 * (size_t)i + j]);
-            ptblk.push_back(bb);
-            if (gcpinfo[i] > -1) {
-                // the gn-th kept GCP point of the part takes entry gn of its list
-                gcp_of.push_back((int)(gcc.size() / 3));
-                for (int j = 0; j < 3; ++j) gcc.push_back(gcpc[3 * (size_t)gcpinfo[i] + j]);
-                const std::vector<float> &L = w3list[(size_t)p];
-                for (int v = 0; v < 6; ++v) gcw.push_back(6 * (size_t)gn + v < L.size() ? L[6 * (size_t)gn + v] : 0.f);
-                ++gn;
-            } else {
-                gcp_of.push_back(-1);
+    us.npts = nup;
+    us.nrpv.assign((size_t)nup, 0);
+    us.toff.assign((size_t)nup + 1, 0);
+    std::vector<double> Xu(3 * (size_t)std::max(nup, 1), 0.);
+    std::vector<int> ptblk((size_t)std::max(nup, 1), 0), gcp_of((size_t)std::max(nup, 1), -1);
+    {
+        std::vector<long long> w(uoff.begin(), uoff.end() - 1);
+        for (int i = 0; i < pxs; ++i) {
+            if (!len[i]) continue;
+            for (long long sl = soff[i]; sl < soff[(size_t)i + 1]; ++sl) {
+                const int p = spart[(size_t)sl];
+                if (!scnt[(size_t)sl] || !bkept[(size_t)sl] || !valid[p]) continue;
+                const int u = (int)w[p]++;
+                up[(size_t)sl] = u;
+                us.nrpv[u] = i;
+                us.toff[(size_t)u + 1] = scnt[(size_t)sl];
+                for (int j = 0; j < 3; ++j) Xu[3 * (size_t)u + j] = XXd[3 * (size_t)i + j];
+                ptblk[u] = pblk[p];
             }
         }
     }
-    us.npts = (int)us.nrpv.size();
-    us.ntr = (int)us.tcam.size();
+    for (int u = 0; u < nup; ++u) us.toff[(size_t)u + 1] += us.toff[u];
+    us.ntr = us.toff[nup];
+    clk.mark(st, "union points", cur_iter);
+    us.tcam.assign((size_t)us.ntr, 0);
+    us.tpt.assign((size_t)us.ntr, 0);
+    us.tfo.assign((size_t)us.ntr, 0);
+    std::vector<float> w0(3 * (size_t)us.ntr);
+    {
+        std::vector<int> pos(us.toff.begin(), us.toff.end() - 1);
+        for (int i = 0; i < pxs; ++i)
+            for (int q = 0; q < len[i]; ++q) {
+                const size_t e = (size_t)(base[i] + q);
+                const int u = up[(size_t)eslot[e]];
+                if (u < 0) continue;
+                const int at = pos[u]++, fo = trk[e];
+                us.tcam[at] = uidx[ecam[e]];
+                us.tpt[at] = u;
+                us.tfo[at] = fo;
+                w0[3 * (size_t)at] = io->obs_w ? io->obs_w[3 * (size_t)fo] : 1.f;
+                w0[3 * (size_t)at + 1] = io->obs_w ? io->obs_w[3 * (size_t)fo + 1] : 1.f;
+                w0[3 * (size_t)at + 2] = io->obs_w ? io->obs_w[3 * (size_t)fo + 2] : 0.f;
+            }
+    }
+    clk.mark(st, "union entries", cur_iter);
+    // the priors: the gn-th kept GCP point of a part takes entry gn of its list
+    std::vector<double> gcc;
+    std::vector<float> gcw;
+    {
+        int gn = 0, cur = -1;
+        for (int u = 0; u < nup; ++u) {
+            if (ptblk[u] != cur) {
+                cur = ptblk[u];
+                gn = 0;
+            }
+            const int i = us.nrpv[u];
+            if (gcpinfo[i] < 0) continue;
+            const std::vector<float> &L = w3list[(size_t)bpart[cur]];
+            gcp_of[u] = (int)(gcc.size() / 3);
+            for (int j = 0; j < 3; ++j) gcc.push_back(gcpc[3 * (size_t)gcpinfo[i] + j]);
+            for (int v = 0; v < 6; ++v) gcw.push_back(6 * (size_t)gn + v < L.size() ? L[6 * (size_t)gn + v] : 0.f);
+            ++gn;
+        }
+    }
     // camera state of the pass (EncodeCameras in GCP mode: every camera has its
     // six pose slots and the estimated intrinsics; no gauge camera)
     std::vector<double> Rb(9 * (size_t)nc), Cb(3 * (size_t)nc), th(9 * (size_t)nc, 0.), In(3 * (size_t)nc),
@@ -465,6 +524,7 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
         In[3 * (size_t)u + 2] = (double)cf[ucam[u]];
         for (int j = 0; j < 3; ++j) th[9 * (size_t)u + 6 + j] = In[3 * (size_t)u + j];
     }
+    clk.mark(st, "priors and cameras", cur_iter);
     // device structure of the union (pairs within parts only)
     std::unique_ptr<DevStruct> ds(new DevStruct());
     ds->with_pairs = true;
@@ -476,6 +536,7 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     if (rc) return rc;
     if (empty >= 0) return RPBA_E_INVALID;  // ruled out above
     if (ds->nslab <= 0 || ds->nchunk <= 0) return RPBA_E_INVALID;  // a part's S without off-diagonal blocks
+    clk.mark(st, "union structure (device)", cur_iter);
     const size_t ntr = (size_t)ds->ntr, np = (size_t)ds->npts;
     DBuf<double> Wrec, Yrec, Vinv, gp, Sd, Su, bv, Dd, rowpart, colpart, dpart, Cinv, x, best, r, dv, Ad, h, h1, X, Xn, Sdf,
         ev, c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist, gc_c;
@@ -561,6 +622,7 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     launch_lin_cameras(la, st);
     launch_bj_inverse(nc, Sd.p, Cinv.p, st);
     HIPCHK(hipGetLastError());
+    clk.mark(st, "linearise and Schur", cur_iter);
     // the parts' PCGs (SparsePCG with begini = 0, rbacore.cpp:1807-1914)
     PcgArgs pa;
     std::memset(&pa, 0, sizeof pa);
@@ -598,6 +660,7 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     }
     (void)hipHostFree(hd);
     if (rc) return rc;
+    clk.mark(st, "segmented PCG", cur_iter);
     // back-substitution, the per-part NaN check of the step, decode, store
     StepArgs s1 = s0;
     s1.sol = best.p;
@@ -613,6 +676,7 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     HIPCHK(hipMemcpyAsync(hs.data(), segS.p, sizeof(SegScal) * (size_t)nb, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     ds->release();
+    clk.mark(st, "step, decode, store", cur_iter);
     // OutputLSAP: the stored matrices and camerasnew (float) of every part
     // that stored; a part whose step was NaN keeps its encoded matrices and
     // its intrinsics (rbacore.cpp:541-546 returns before the decode)
@@ -773,6 +837,8 @@ int partitioned_solve(rpba_block *io, const rpba_flags *f, int P, rpba_log_fn lo
     const int nrppt = pxs / P;
     while (iflag > 0 && iter < 51) {
         int robustit = 0, pcg_it = 0;
+        S.cur_iter = iter;
+        S.clk.mark(nullptr, "start", iter);
         if (iter > 1) {
             int addf[3] = {S.addf0[0], S.addf0[1], S.addf0[2]};
             if (iter < 3) addf[0] = addf[1] = addf[2] = 0;
@@ -785,35 +851,31 @@ int partitioned_solve(rpba_block *io, const rpba_flags *f, int P, rpba_log_fn lo
             if (S.robust) sig0frac = 1.02f;
         }
         if (iter > 1) {
-            // pointXXv from the parts' tracks, part by part (rpbacore.cpp:276-289)
+            // pointXXv from the parts' tracks, part by part (rpbacore.cpp:276-289);
+            // the slots of the entries are those block_step indexed
             std::vector<int> tmp;
             for (int i = 0; i < pxs; ++i) {
                 if (!S.len[i]) continue;
                 tmp.clear();
                 const long long b = S.base[i];
-                for (long long s = S.soff[i]; s < S.soff[(size_t)i + 1]; ++s) {
-                    const int p = S.spart[(size_t)s];
-                    bool has = false;
-                    for (int q = 0; q < S.len[i]; ++q) has |= S.part[io->obs_cam[S.trk[(size_t)(b + q)]]] == p;
-                    if (!has || !S.bkept[(size_t)s]) continue;
-                    for (int q = 0; q < S.len[i]; ++q) {
-                        const int fo = S.trk[(size_t)(b + q)];
-                        if (S.part[io->obs_cam[fo]] == p) tmp.push_back(fo);
-                    }
+                for (long long sl = S.soff[i]; sl < S.soff[(size_t)i + 1]; ++sl) {
+                    if (!S.bkept[(size_t)sl]) continue;
+                    for (int q = 0; q < S.len[i]; ++q)
+                        if (S.eslot[(size_t)(b + q)] == sl) tmp.push_back(S.trk[(size_t)(b + q)]);
                 }
                 std::copy(tmp.begin(), tmp.end(), S.trk.begin() + b);
                 S.len[i] = tmp.size() < 2 ? 0 : (int)tmp.size();
             }
         }
+        S.clk.mark(nullptr, "tracks regrouped", iter);
         // pitflags of the points with entries (rpbacore.cpp:301-309)
+        S.entry_index();
         for (int i = 0; i < pxs; ++i) {
             if (!S.len[i]) continue;
-            for (long long s = S.soff[i]; s < S.soff[(size_t)i + 1]; ++s) {
-                bool has = false;
-                for (int q = 0; q < S.len[i]; ++q) has |= S.part[io->obs_cam[S.trk[(size_t)(S.base[i] + q)]]] == S.spart[(size_t)s];
-                S.pit[(size_t)s] = has ? 1 : 0;
-            }
+            for (long long sl = S.soff[i]; sl < S.soff[(size_t)i + 1]; ++sl) S.pit[(size_t)sl] = 0;
+            for (int q = 0; q < S.len[i]; ++q) S.pit[(size_t)S.eslot[(size_t)(S.base[i] + q)]] = 1;
         }
+        S.clk.mark(nullptr, "pitflags", iter);
         if (robustit) {
             // the robust statistics pass and the per-image scales (rpbacore.cpp:311-375)
             std::vector<int> len0 = S.len;
@@ -837,9 +899,11 @@ int partitioned_solve(rpba_block *io, const rpba_flags *f, int P, rpba_log_fn lo
                 }
             }
         }
+        S.clk.mark(nullptr, "robust statistics pass", iter);
         // the main point pass (rpbacore.cpp:378-392)
         rc = S.point_pass(robustit ? 2 : 0, iter == 1 ? 1 : 0, 1, sgv, rdv, nullptr);
         if (rc) return rc;
+        S.clk.mark(nullptr, "main point pass", iter);
         sig0g = 0.;
         long redundancy = 0;
         for (int t = 0; t < P; ++t) {
@@ -900,6 +964,7 @@ int partitioned_solve(rpba_block *io, const rpba_flags *f, int P, rpba_log_fn lo
                 }
             }
         }
+        S.clk.mark(nullptr, "sums, best state, GCP lists", iter);
         // iflag (rpbacore.cpp:479-493)
         if (iter > 1) {
             if (sig0g <= sig0opt) {
