@@ -137,6 +137,7 @@ def run_bench(args):
     ttc, recs = timed_solve(ctx, "plain")
     rc_d, out, st = ctx.download()
     sizes = ctx.sizes()
+    plain_mf = sizes["blocks"] == 0  # the product the plain form took (schur = 2: by bytes)
     progress(f"whole plain solve {ttc:.3f} s, sigma0 {st['sigma_out'][1]:.6f}")
     pcg_per_it = [int(x) for x in st["trace_pcg"]]
 
@@ -224,16 +225,20 @@ def run_bench(args):
     forms = {}
     skipped = {}
     if not args.no_chosen:
-        mf_ok = not args.schur
         base = dict(robust=1, k2=1, k4=1, f=1, threads=1)
         cands = [
             ("fp32_records", capi.make_flags(**base, precision=1),
              "precision=1: FP32 Jacobians and W, Y = W V^-1 records; FP64 residuals, sums, S, PCG, cost"),
             ("fp32_matrix_free", capi.make_flags(**base, precision=1, schur=1),
              "precision=1, schur=1: FP32 records, matrix-free Schur product (off-diagonal S never formed)"),
-            ("fp64_matrix_free", capi.make_flags(**base, schur=1),
-             "schur=1: FP64 matrix-free Schur product (off-diagonal S never formed)"),
         ]
+        # the FP64 product the plain form did not take
+        if plain_mf:
+            cands.append(("fp64_s_formed", capi.make_flags(**base, schur=0),
+                          "schur=0: FP64 with S formed block by block"))
+        else:
+            cands.append(("fp64_matrix_free", capi.make_flags(**base, schur=1),
+                          "schur=1: FP64 matrix-free Schur product (off-diagonal S never formed)"))
         if lp_model is not None:
             cands += [
                 ("fp64_learned", capi.make_flags(**base, preconditioner=2),
@@ -242,8 +247,6 @@ def run_bench(args):
                  "precision=1, preconditioner=2: FP32 records with the learned correction of block-Jacobi"),
             ]
         for name, fl, desc in cands:
-            if fl.schur and not mf_ok:
-                continue
             if (args.config, name) in FORM_SKIP:
                 skipped[name] = FORM_SKIP[(args.config, name)]
                 progress(f"form {name}: skipped ({skipped[name]})")
@@ -281,7 +284,7 @@ def run_bench(args):
     alg = algorithmic_bytes(dom, sz0)
     achieved = alg / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.config}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}{'_mf' if sz0['blocks'] == 0 else ''}_{args.config}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -296,7 +299,8 @@ def run_bench(args):
 
     line = {
         "metric": "LM iters/s, plain FP64 robust LM Schur solve (time-to-converged reported beside)",
-        "form": "plain FP64, " + ("matrix-free Schur product" if args.schur else "S formed block by block"),
+        "form": "plain FP64, " + ("matrix-free Schur product" if plain_mf else "S formed block by block") +
+                (" (the product streaming fewer bytes per PCG iteration)" if args.schur == 2 else ""),
         "value": round(args.steps / elapsed, 3),
         "unit": "LM iters/s",
         "n_gpus": 1,
@@ -333,8 +337,66 @@ def run_bench(args):
     }
     if not args.no_cpu_baseline:
         k_avg = st["pcg_iters"] / max(st["itt"], 1)
-        line["cpu_baseline"] = cpu_baseline(sc, sizes, k_avg, args.config)
+        # the reference forms S: the sample is sized and scaled by the formed
+        # structure's pairs and blocks
+        fsz = sizes
+        if plain_mf:
+            fctx = api.Context(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, schur=0))
+            fctx.run(1)
+            fsz = fctx.sizes()
+            fctx.close()
+        line["cpu_baseline"] = cpu_baseline(sc, fsz, k_avg, args.config)
+    if args.partition_config != "none":
+        line["partitioned"] = partitioned_leg(args.partition_config, args.no_cpu_baseline)
     print(json.dumps(line), flush=True)
+
+
+def partitioned_leg(config, no_cpu):
+    """The reference's default dispatch (Threads = 0: every processor of the
+    host, rpba.cpp:85-86) on a scene of more than 104 cameras: the partitioned
+    solve of rpbacore.cpp:100-510 with min(processors, iround(N / 70)) parts.
+    The whole rpba_calc_plsa call from host buffers (partition, per outer
+    iteration the parts' GCP-mode steps on the GPU and the point pass), beside
+    the CPU oracle's restatement of the same solve on the same partition with
+    OpenMP over the parts and point ranges as the reference runs them."""
+    import numpy as np
+
+    from rpba_amd import api, capi, scenes
+
+    sc = scenes.generate(config)
+    procs = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    P = min(procs, int(round(sc["n_cams"] / 70)))
+    flags = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=0)
+    api.calc_plsa(sc, flags)  # warm-up (code loading)
+    device_sync()
+    t0 = time.perf_counter()
+    rc, out, st, log = api.calc_plsa(sc, flags)
+    device_sync()
+    gpu_s = time.perf_counter() - t0
+    its = [l for l in log if l.startswith("IT:")]
+    sig = float(its[-1].split()[3]) if its else None
+    progress(f"partitioned solve on {config}: {P} parts, {gpu_s:.3f} s, {st['itt']} outer iterations, status {rc}")
+    res = {"workload": f"{config}: {sc['n_cams']} cameras, {sc['n_obs']} observations, robust, k2 k4 f, Threads = 0",
+           "parts": P, "status": rc, "gpu_call_s": round(gpu_s, 4), "outer_iterations": st["itt"],
+           "pcg_iters": st["pcg_iters"], "sigma_last_IT_line": sig, "rms_px": st["rms_px"]}
+    if not no_cpu:
+        from oracle import oracle
+        part = api.partition_images(sc, P)
+        t0 = time.perf_counter()
+        rc0, out0, st0, log0 = oracle.partition_solve(sc, flags, part)
+        cpu_s = time.perf_counter() - t0
+        omp = int(os.environ.get("OMP_NUM_THREADS", procs))
+        res["cpu_baseline"] = {"seconds": round(cpu_s, 3), "cores": min(P, omp, procs), "kind": "port",
+                               "sample": f"the whole partitioned solve of {config} ({st0['itt']} outer iterations) "
+                                         "by the oracle's C restatement, parts and point ranges in OpenMP threads",
+                               "host": host_info()}
+        res["gpu_vs_cpu"] = round(cpu_s / gpu_s, 3)
+        n = min(st["itt"], st0["itt"])
+        s0, s1 = np.asarray(st0["trace_sig0"][:n]), np.asarray(st["trace_sig0"][:n])
+        res["sig0g_rel_vs_oracle_max"] = float(np.max(np.abs(s1 - s0) / s0)) if n else None
+        res["labels_equal_oracle"] = bool(np.array_equal(out["obs_kept"], out0["obs_kept"]))
+        progress(f"partitioned solve on {config}: oracle {cpu_s:.2f} s ({res['cpu_baseline']['cores']} threads)")
+    return res
 
 
 def run_batch_bench(args):
@@ -583,9 +645,12 @@ def main():
                     help="untimed LM iterations first (the default covers one whole solve: every kernel, the "
                          "pass boundary and the restart, so first-launch code loading stays out of the timing)")
     ap.add_argument("--config", default="c5_13682")
-    ap.add_argument("--schur", type=int, default=0, choices=(0, 1),
-                    help="plain FP64 form with S formed (0, default) or the matrix-free Schur product (1)")
+    ap.add_argument("--schur", type=int, default=2, choices=(0, 1, 2),
+                    help="plain FP64 form with S formed (0), the matrix-free Schur product (1) or whichever "
+                         "streams fewer bytes per PCG iteration (2, default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--partition-config", default="c2_961",
+                    help="scene of the partitioned-solve leg (Threads = 0 dispatch); none to skip")
     ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (FP32, matrix-free) form")
     ap.add_argument("--no-sequential", action="store_true",
                     help="c4_batch: skip the one-block-after-another comparison (profiling runs)")

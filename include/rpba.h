@@ -98,7 +98,12 @@ typedef struct rpba_flags {
     int32_t schur;           /* 0 = S formed block by block (the reference's
                                 profile store); 1 = matrix-free: only the diagonal
                                 blocks are formed and the PCG multiplies through
-                                W and Y = W V^-1 (no covisibility pairs built)  */
+                                W and Y = W V^-1 (no covisibility pairs built);
+                                2 = whichever streams fewer bytes per PCG
+                                iteration: 648 (B + N_c) with B bounded by
+                                min(N_c (N_c - 1) / 2, point pairs), against
+                                (2 record bytes + 48) N_o (formed with solver 1
+                                or a corrected preconditioner)               */
     int32_t solver;          /* 0 = block-Jacobi PCG (SparsePCG, rbacore.cpp:1807-1914);
                                 1 = dense Cholesky of the reduced camera system on
                                 the FP64 matrix cores (exact solve; a form of its

@@ -945,6 +945,26 @@ static int ctx_init(rpba_ctx *c)
     rc = upload_tracks(c);
     if (rc) return rc;
     setup_mark(c, "points restored");
+    if (c->fl.schur == 2) {
+        // the product of the PCG by bytes per iteration: S formed streams
+        // 648 (B + N_c) bytes (B <= min(N_c (N_c - 1) / 2, point pairs)), the
+        // matrix-free product the W and Y records and the point shares,
+        // (2 record bytes + 48) N_o; the sums are the same in another order
+        std::vector<int> toff((size_t)c->ds.npts + 1);
+        HIPCHK(hipMemcpyAsync(toff.data(), c->ds.toff, sizeof(int) * toff.size(), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        double pairs = 0.;
+        for (int i = 0; i < c->ds.npts; ++i) {
+            const double t = (double)(toff[(size_t)i + 1] - toff[i]);
+            pairs += t * (t - 1.) * 0.5;
+        }
+        const double bup = std::min(0.5 * (double)nc * (double)(nc - 1), pairs);
+        const double formed = 648. * (bup + (double)nc);
+        const double rec = (double)kRec * (c->rec32 ? 4. : 8.);
+        const double free_ = (2. * rec + 48.) * (double)c->ds.ntr;
+        c->implicit = c->fl.solver == 0 && c->fl.preconditioner == 0 && free_ < formed;
+        c->ds.with_pairs = !c->implicit;
+    }
     rc = upload_initial_cameras(c);
     if (rc) return rc;
     setup_mark(c, "cameras");
@@ -987,12 +1007,12 @@ static int partition_dispatch(const rpba_flags *f, int nc)
 extern "C" int rpba_ctx_create(rpba_ctx **out, rpba_block *io, const rpba_flags *flags, rpba_log_fn log, void *user)
 {
     if (!out || !io || !flags || io->n_cams < 1 || io->n_pts < 0 || io->n_obs < 0) return RPBA_E_INVALID;
-    if (flags->precision < 0 || flags->precision > 1 || flags->schur < 0 || flags->schur > 1) return RPBA_E_INVALID;
+    if (flags->precision < 0 || flags->precision > 1 || flags->schur < 0 || flags->schur > 2) return RPBA_E_INVALID;
     // the dense form factors the formed S: it needs the off-diagonal blocks
     if (flags->solver < 0 || flags->solver > 1 || (flags->solver == 1 && flags->schur == 1)) return RPBA_E_INVALID;
     // the corrected preconditioner multiplies by the formed off-diagonal blocks
     if (flags->preconditioner < 0 || flags->preconditioner > 2) return RPBA_E_INVALID;
-    if (flags->preconditioner >= 1 && (flags->schur != 0 || flags->solver != 0)) return RPBA_E_INVALID;
+    if (flags->preconditioner >= 1 && (flags->schur == 1 || flags->solver != 0)) return RPBA_E_INVALID;
     if (flags->preconditioner == 1 && !(flags->pc_omega >= 0.f && flags->pc_omega < 1.f)) return RPBA_E_INVALID;
     if (!io->obs_cam || !io->obs_pt || !io->obs_xy || !io->cam_P || !io->cam_f || !io->cam_d || !io->pts)
         return RPBA_E_INVALID;
@@ -1398,7 +1418,7 @@ extern "C" int rpba_calc_plsa(rpba_block *io, const rpba_flags *flags, rpba_log_
     if (nparts && !flags->force_global) {
         // the partitioned solve (rpbacore.cpp:100-510): plain FP64 arithmetic
         // and the block-Jacobi PCG of the reference
-        if (flags->precision != 0 || flags->preconditioner != 0 || flags->schur != 0 || flags->solver != 0)
+        if (flags->precision != 0 || flags->preconditioner != 0 || flags->schur == 1 || flags->solver != 0)
             return RPBA_E_UNSUPPORTED;
         if (!io->obs_cam || !io->obs_pt || !io->obs_xy || !io->cam_P || !io->cam_f || !io->cam_d || !io->pts ||
             io->n_cams < 1 || io->n_pts < 0 || io->n_obs < 0)

@@ -89,9 +89,10 @@ def test_invalid_arguments_are_rejected_without_gpu():
 
 
 def test_unknown_form_selectors_are_rejected_without_gpu(tiny_scene):
-    """precision and schur outside {0, 1} fail before any device call."""
+    """precision outside {0, 1} and schur outside {0, 1, 2} fail before any
+    device call."""
     from rpba_amd import api
-    for kw in ({"precision": 2}, {"schur": 2}, {"schur": -1}):
+    for kw in ({"precision": 2}, {"schur": 3}, {"schur": -1}):
         rc, _, _, _ = api.calc_plsa(tiny_scene, capi.make_flags(**kw))
         assert rc == capi.RPBA_E_INVALID, kw
 

@@ -191,3 +191,23 @@ def test_pcg_graph_replay_same_bits(scene_c1):
     for rc, out, st in outs[1:]:
         assert np.array_equal(out["pts"], outs[0][1]["pts"]) and st["sig0_final"] == outs[0][2]["sig0_final"]
         assert np.array_equal(st["trace_sig0"], outs[0][2]["trace_sig0"])
+
+
+@pytest.mark.parametrize("n_cams,n_pts", [(50, 5000), (300, 5000)])
+def test_auto_product_is_the_explicit_one(n_cams, n_pts):
+    """schur = 2 takes the product streaming fewer bytes per PCG iteration
+    (S formed on the 50-camera scene, matrix-free on 300 cameras with few
+    points per camera pair); its solve is bitwise the explicit form's."""
+    sc = scenes.generate(None, seed=23, n_cams=n_cams, n_pts=n_pts, track="uniform2_8", outlier_frac=0.02)
+    _say(f"[auto product] {n_cams} cameras")
+    ctx = api.Context(sc, capi.make_flags(robust=1, schur=2))
+    ctx.run(-1)
+    rc, out, st = ctx.download()
+    mf = ctx.sizes()["blocks"] == 0
+    ctx.close()
+    assert rc == 0
+    assert mf == (n_cams == 300)
+    rc1, out1, st1, _ = api.calc_plsa(sc, capi.make_flags(robust=1, schur=1 if mf else 0))
+    assert rc1 == 0
+    assert np.array_equal(np.asarray(st["trace_sig0"]), np.asarray(st1["trace_sig0"]))
+    assert np.array_equal(out["obs_kept"], out1["obs_kept"]) and np.array_equal(out["pts"], out1["pts"])
