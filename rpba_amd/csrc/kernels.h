@@ -16,6 +16,9 @@ constexpr int kRec = 32;
 // 9 g' + i: for a fixed column j the rows of all m blocks are contiguous, so
 // lanes 9 g + i of a wavefront read them as one run of 9 m doubles.
 constexpr int kSlab = 7;
+// workgroups of the matrix-free product's d.S d sum (k_mf_dsum): its partials
+// follow the nch chunk shares in dpart
+constexpr int kMfDsumGroups = 256;
 struct SuMap {
     const int *slot;       // block -> slot (position in strip, column, row order)
     const int *slot_slab;  // slot -> slab
@@ -109,7 +112,8 @@ struct PcgArgs {
     const int *pgrp;  // point groups (first point of group c; npgrp + 1 entries)
     const void *Wrec, *Yrec;
     const double *Dk;  // per camera: D_k = sum_q Y_q W_q^T (81)
-    double *tc;        // per entry, camera-major: t of the entry's point (3)
+    double *tc;        // per point: t (3)
+    const int *cq_pt;  // camera-major entry -> its point
     double *cpart;     // per chunk: sum_q Y_q t_pt(q) (9)
     // corrected preconditioner (S formed only): h = C r - w C N C r with the
     // weight w in device memory (null: block-Jacobi)
@@ -172,6 +176,9 @@ void launch_schur_pairs(const SchurArgs &s, hipStream_t st);
 void launch_schur_diag(const LinArgs &a, hipStream_t st);
 void launch_lin_cameras(const LinArgs &a, hipStream_t st);
 int pcg_groups(int nc);
+// cq_pt[j] = tpt[camq[j]]: the point of every camera-major entry (the
+// matrix-free product's camera side gathers t through it)
+void launch_cq_pt(int ntr, const int *camq, const int *tpt, int *cq_pt, hipStream_t st);
 int spmv_grid(int nblk);  // workgroups of the SpMV launch (PcgArgs::ndpart)
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st);
 void launch_pcg_init(const PcgArgs &a, hipStream_t st);

@@ -984,22 +984,23 @@ __device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
 }
 
 // Matrix-free S d (no off-diagonal S stored), three launches:
-// k_mf_points: t_i = sum over its entries q of W_q^T d_cam(q), stored for every
-// entry of the point at its camera-major position.
+// k_mf_points: t_i = sum over its entries q of W_q^T d_cam(q), stored once per
+// point.
 // k_mf_chunks: one wavefront per chunk (at most 64 entries of one image):
-// lane e forms Y_q t_pt(q) for entry q, the wavefront sums the 9 values in
-// lane order into cpart[chunk]; the chunk's share of d.S d is
+// lane e gathers t_pt(q) of entry q (through cq_pt) and forms Y_q t_pt(q), the
+// wavefront sums the 9 values in lane order into cpart[chunk]; the chunk's
+// share of d.S d is
 // -d_k.cpart[chunk], plus d_k.(S_kk + D_k) d_k on the image's first chunk,
 // whose lanes also store (S_kk + D_k) d_k in Ad[k].  k_pcg_a then forms
 // (S d)_k = Ad_k - sum of the image's chunk sums, in chunk order.
-// k_mf_dsum adds the chunk shares of d.S d in a fixed order.
+// k_mf_dsum adds the chunk shares of d.S d in a fixed order, in
+// kMfDsumGroups contiguous ranges.
 template <typename R>
 __global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
 {
     // points whose first entry lies in [256 c, 256 c + 256): their entries are
     // taken 256 at a time, one per lane (u_q = W_q^T d_cam(q) to LDS), and each
-    // point's lane adds its entries in order; t_i then goes to every entry of
-    // the point at its camera-major position (tc), which k_mf_chunks streams.
+    // point's lane adds its entries in order and stores t_i (tc).
     if (a.S[a.j & 1].done) return;
     __shared__ double su[256][3];
     const int t = threadIdx.x, c = blockIdx.x;
@@ -1046,33 +1047,37 @@ __global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
         }
         __syncthreads();
     }
-    // every entry of the group copies its point's t to its camera-major slot
-    __shared__ double st[130][3];
+    // t of the group's points, once per point (k_mf_chunks gathers it for
+    // its camera-major entries: 24 bytes per point, which the memory-side
+    // cache holds, instead of a copy per entry)
     if (t < np) {
-        st[t][0] = t0;
-        st[t][1] = t1;
-        st[t][2] = t2;
-    }
-    __syncthreads();
-    for (int base = e0; base < e1; base += 256) {
-        const int q = base + t;
-        if (q < e1) {
-            const int j = a.tpt[q] - p0;
-            double *o = a.tc + 3 * (size_t)a.camrank[q];
-            o[0] = st[j][0];
-            o[1] = st[j][1];
-            o[2] = st[j][2];
-        }
+        double *o = a.tc + 3 * (size_t)(p0 + t);
+        o[0] = t0;
+        o[1] = t1;
+        o[2] = t2;
     }
 }
 
 // total of the chunk shares into dpart[nch], one workgroup, fixed order
-__global__ __launch_bounds__(1024) void k_mf_dsum(PcgArgs a)
+// workgroup g adds the chunk shares of its contiguous range in a fixed
+// order; k_pcg_a adds the kMfDsumGroups partials in order
+__global__ __launch_bounds__(256) void k_mf_dsum(PcgArgs a)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[4];
     if (a.S[a.j & 1].done) return;
-    const double s = sum_fixed(a.dpart, a.nch, sh);
-    if (threadIdx.x == 0) a.dpart[a.nch] = s;
+    const int per = (a.nch + kMfDsumGroups - 1) / kMfDsumGroups;
+    const int c0 = blockIdx.x * per, c1 = min(a.nch, c0 + per);
+    double s = 0.;
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) s += a.dpart[c];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) a.dpart[a.nch + blockIdx.x] = s;
+}
+
+// camera-major entry j -> the point of its entry
+__global__ void k_cq_pt(int ntr, const int *camq, const int *tpt, int *cq_pt)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ntr) cq_pt[j] = tpt[camq[j]];
 }
 
 template <typename R>
@@ -1091,8 +1096,8 @@ __global__ __launch_bounds__(64) void k_mf_chunks(PcgArgs a)
     R Y[28];
     load_records_wave<R, 8>(Yr, b, n, Y, stg, lane);
     if (lane < n) {
-        // t of the entry's point, in camera-major order
-        const double *tq = a.tc + 3 * (size_t)(b + lane);
+        // t of the entry's point
+        const double *tq = a.tc + 3 * (size_t)a.cq_pt[b + lane];
         const double t0 = tq[0], t1 = tq[1], t2 = tq[2];
 #pragma unroll
         for (int r = 0; r < 9; ++r) v[r] = (double)Y[3 * r] * t0 + (double)Y[3 * r + 1] * t1 + (double)Y[3 * r + 2] * t2;
@@ -1982,9 +1987,13 @@ void launch_spmv(const PcgArgs &a, hipStream_t st)
         k_mf_points<double><<<a.npgrp, 256, 0, st>>>(a);
         k_mf_chunks<double><<<a.nch, 64, 0, st>>>(a);
     }
-    k_mf_dsum<<<1, 1024, 0, st>>>(a);
+    k_mf_dsum<<<kMfDsumGroups, 256, 0, st>>>(a);
 }
 int pcg_groups(int nc) { return (nc + 27) / 28; }
+void launch_cq_pt(int ntr, const int *camq, const int *tpt, int *cq_pt, hipStream_t st)
+{
+    if (ntr > 0) k_cq_pt<<<nblk(ntr, 256), 256, 0, st>>>(ntr, camq, tpt, cq_pt);
+}
 void launch_pcg_init_w(const PcgArgs &a, hipStream_t st)
 {
     PcgArgs b = a;

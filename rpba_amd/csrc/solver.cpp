@@ -77,6 +77,7 @@ struct rpba_ctx {
     DBuf<double> d_Su;
     DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_partial, d_scal, d_Dd, d_ppart, d_rowpart, d_colpart, d_dpart;
     DBuf<double> d_t, d_cpart, d_Dk;  // matrix-free form
+    DBuf<int> d_cqpt;                  // matrix-free form: camera-major entry -> point
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -247,9 +248,10 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
     rc |= c->d_rowpart.alloc(63 * (size_t)std::max(c->ds.nchunk, 1));
     rc |= c->d_colpart.alloc(9 * (size_t)std::max(c->ds.nslab, 1));
-    rc |= c->d_dpart.alloc((size_t)std::max(spmv_grid(c->ds.nblk), c->ds.nch + 1));
+    rc |= c->d_dpart.alloc((size_t)std::max(spmv_grid(c->ds.nblk), c->ds.nch + kMfDsumGroups));
     if (c->implicit) {
-        rc |= c->d_t.alloc(3 * std::max<size_t>(ntr, 1));
+        rc |= c->d_t.alloc(3 * std::max<size_t>(np, 1));
+        rc |= c->d_cqpt.alloc(std::max<size_t>(ntr, 1));
         rc |= c->d_cpart.alloc(9 * (size_t)std::max(c->ds.nch, 1));
         rc |= c->d_Dk.alloc(81 * (size_t)c->nc);
     }
@@ -394,13 +396,14 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.dpart = c->d_dpart.p; a.nblk = c->ds.nblk;
     a.ndpart = c->implicit ? c->ds.nch : spmv_grid(c->ds.nblk);
     a.dsum = c->implicit ? c->d_dpart.p + c->ds.nch : c->d_dpart.p;
-    a.ndsum = c->implicit ? 1 : a.ndpart;
+    a.ndsum = c->implicit ? kMfDsumGroups : a.ndpart;
     a.j = 0;
     a.implicit = c->implicit ? 1 : 0; a.npts = c->ds.npts; a.ntr = c->ds.ntr; a.nch = c->ds.nch; a.rec32 = c->rec32;
     a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.tpt = c->ds.tpt; a.camq = c->ds.camq; a.camrank = c->ds.camrank;
     a.pgrp = c->ds.pgrp; a.npgrp = c->ds.npgrp;
     a.ch_start = c->ds.ch_start; a.ch_off = c->ds.ch_off;
     a.Wrec = c->d_Wrec.p; a.Yrec = c->d_Yrec.p; a.Dk = c->d_Dk.p; a.tc = c->d_t.p; a.cpart = c->d_cpart.p;
+    a.cq_pt = c->implicit ? c->d_cqpt.p : nullptr;
     a.omega = c->fl.preconditioner >= 1 ? c->d_omega.p : nullptr;
     return a;
 }
@@ -486,6 +489,7 @@ static int bundle_step(rpba_ctx *c)
         return finish_bundle_step(c, hs);
     }
     c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
+    if (c->implicit) launch_cq_pt(c->ds.ntr, c->ds.camq, c->ds.tpt, c->d_cqpt.p, c->st);
     PcgArgs pa = pcg_args(c);
     if (pa.omega) {
         // the corrected preconditioner multiplies by the off-diagonal blocks,
@@ -1320,6 +1324,7 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_colpart.release();
     c->d_dpart.release();
     c->d_t.release();
+    c->d_cqpt.release();
     c->d_cpart.release();
     c->d_Dk.release();
     c->d_ppart.release();
