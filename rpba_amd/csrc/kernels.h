@@ -16,6 +16,11 @@ constexpr int kRec = 32;
 // 9 g' + i: for a fixed column j the rows of all m blocks are contiguous, so
 // lanes 9 g + i of a wavefront read them as one run of 9 m doubles.
 constexpr int kSlab = 7;
+// entries per point group: group c holds the points whose first entry lies in
+// [kPtGroup c, kPtGroup (c + 1)).  The point kernels take a group's entries 256
+// at a time; below 256 so that a group whose last point runs past the window
+// still takes one pass unless that point has more than 17 entries.
+constexpr int kPtGroup = 240;
 // workgroups of the matrix-free product's d.S d sum (k_mf_dsum): its partials
 // follow the nch chunk shares in dpart
 constexpr int kMfDsumGroups = 256;
@@ -114,6 +119,7 @@ struct PcgArgs {
     const double *Dk;  // per camera: D_k = sum_q Y_q W_q^T (81)
     double *tc;        // per point: t (3)
     const int *cq_pt;  // camera-major entry -> its point
+    const int *ch_cam; // chunk -> its image
     double *cpart;     // per chunk: sum_q Y_q t_pt(q) (9)
     // corrected preconditioner (S formed only): h = C r - w C N C r with the
     // weight w in device memory (null: block-Jacobi)
@@ -177,8 +183,10 @@ void launch_schur_diag(const LinArgs &a, hipStream_t st);
 void launch_lin_cameras(const LinArgs &a, hipStream_t st);
 int pcg_groups(int nc);
 // cq_pt[j] = tpt[camq[j]]: the point of every camera-major entry (the
-// matrix-free product's camera side gathers t through it)
-void launch_cq_pt(int ntr, const int *camq, const int *tpt, int *cq_pt, hipStream_t st);
+// matrix-free product's camera side gathers t through it); ch_cam[c]: the
+// image of chunk c
+void launch_mf_index(int ntr, int nch, const int *camq, const int *tpt, const int *tcam, const int *ch_start,
+                     int *cq_pt, int *ch_cam, hipStream_t st);
 int spmv_grid(int nblk);  // workgroups of the SpMV launch (PcgArgs::ndpart)
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st);
 void launch_pcg_init(const PcgArgs &a, hipStream_t st);

@@ -233,31 +233,57 @@ __device__ __forceinline__ uint4 rec_piece(const float *v, int j)
 
 // The 27 values of the records of a wavefront's entries q0 .. q0 + nq - 1
 // (nq <= 64, record of entry q at recs + q kRec) into lane q - q0's v, read as
-// whole lines: PP 16-byte pieces of every record go through the wavefront's
-// LDS per pass, lane l loading piece l % PP of record (64 / PP) p + l / PP, so
-// one load instruction covers 64 / PP records' consecutive pieces (a lane
-// loading its own record touches 64 lines per instruction).  Every lane of
-// the wavefront calls this; lanes past the entries get nothing.
+// whole lines: lane l loads piece l % PP of record (64 / PP) p + l / PP, so
+// one load instruction covers 64 / PP records' consecutive 16-byte pieces (a
+// lane loading its own record touches 64 lines per instruction), and the
+// pieces reach their record's lane through the wavefront's LDS, PP pieces of
+// every record per pass.  Two halves: rec_issue issues every load of the
+// wavefront's records into raw (one round trip to memory; the caller may
+// issue loads of its own behind them), rec_transpose hands the pieces on.
+// Every lane of the wavefront calls both; lanes past the entries get nothing.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 template <typename R, int PP>
-__device__ __forceinline__ void load_records_wave(const R *recs, int q0, int nq, R *v, uint4 (*stg)[PP + 1], int lane)
+struct RecWave {
+    static constexpr int NV = 16 / (int)sizeof(R);  // values per piece
+    static constexpr int NP = (27 + NV - 1) / NV;   // pieces holding the 27 values
+    static constexpr int RPI = 64 / PP;             // records per load instruction
+    static constexpr int NH = (NP + PP - 1) / PP;   // passes through LDS
+    u32x4 raw[NH * PP];
+};
+
+template <typename R, int PP>
+__device__ __forceinline__ void rec_issue(const R *recs, int q0, int qlast, RecWave<R, PP> &w, int lane)
 {
-    constexpr int NV = 16 / (int)sizeof(R);  // values per piece
-    constexpr int NP = (27 + NV - 1) / NV;   // pieces holding the 27 values
-    constexpr int RPI = 64 / PP;             // records per load instruction
+    // qlast: a valid record index (q0 .. q0 + nq - 1 when nq > 0); loads past
+    // the entries or the 27 values are clamped onto valid pieces and dropped
+    // later, so that no load sits under a branch (a branch makes the compiler
+    // wait for the load at its end)
+    using W = RecWave<R, PP>;
 #pragma unroll
-    for (int h = 0; h < (NP + PP - 1) / PP; ++h) {
+    for (int h = 0; h < W::NH; ++h) {
 #pragma unroll
         for (int p = 0; p < PP; ++p) {
-            const int rr = RPI * p + lane / PP, pp = lane % PP;
-            if (rr < nq && PP * h + pp < NP)
-                stg[rr][pp] = ((const uint4 *)(recs + (size_t)(q0 + rr) * kRec))[PP * h + pp];
+            const int rr = W::RPI * p + lane / PP, pp = lane % PP;
+            const int q = min(q0 + rr, qlast), pc = min(PP * h + pp, W::NP - 1);
+            w.raw[h * PP + p] = ((const u32x4 *)(recs + (size_t)q * kRec))[pc];
         }
+    }
+}
+
+template <typename R, int PP>
+__device__ __forceinline__ void rec_transpose(const RecWave<R, PP> &w, int nq, R *v, uint4 (*stg)[PP + 1], int lane)
+{
+    using W = RecWave<R, PP>;
+#pragma unroll
+    for (int h = 0; h < W::NH; ++h) {
+#pragma unroll
+        for (int p = 0; p < PP; ++p) *(u32x4 *)&stg[W::RPI * p + lane / PP][lane % PP] = w.raw[h * PP + p];
         wave_lds_sync();
         if (lane < nq) {
 #pragma unroll
             for (int j = 0; j < PP; ++j) {
                 const int pc = PP * h + j;
-                if (pc < NP) {
+                if (pc < W::NP) {
                     const uint4 u = stg[lane][j];
                     if constexpr (sizeof(R) == 8) {
                         const double2 d = __builtin_bit_cast(double2, u);
@@ -310,8 +336,8 @@ __device__ __forceinline__ void store_records_wave(R *base, const int *rank, int
 
 
 // V = sum Bp^T W Bp and g_p = sum Bp^T W r per point, then damping and the
-// 3x3 inverse (rbacore.cpp:1341-1344).  A workgroup per group of points whose
-// first entry lies in a 256-entry window: lane per entry computes its terms
+// 3x3 inverse (rbacore.cpp:1341-1344).  A workgroup per point group (first
+// entry in a kPtGroup-entry window): lane per entry computes its terms
 // (to LDS), then each point's lane adds its entries in order.  Sums are FP64.
 template <typename R>
 __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
@@ -998,16 +1024,18 @@ __device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
 template <typename R>
 __global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
 {
-    // points whose first entry lies in [256 c, 256 c + 256): their entries are
-    // taken 256 at a time, one per lane (u_q = W_q^T d_cam(q) to LDS), and each
-    // point's lane adds its entries in order and stores t_i (tc).
+    // the points of a group (first entry in a kPtGroup-entry window): their
+    // entries are taken 256 at a time, one per lane (u_q = W_q^T d_cam(q) to
+    // LDS), and each point's lane adds its entries in order and stores t_i
+    // (tc).  A lane's image index, its wavefront's records and then the
+    // image's d are issued before the first wait.
     if (a.S[a.j & 1].done) return;
     __shared__ double su[256][3];
     const int t = threadIdx.x, c = blockIdx.x;
     const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
     if (p0 >= p1) return;
     const int e0 = a.toff[p0], e1 = a.toff[p1];
-    const int np = p1 - p0;  // at most 128 + 1: a point has at least two entries
+    const int np = p1 - p0;  // at most kPtGroup / 2 + 1: a point has at least two entries
     int pb = 0, pe = 0;
     if (t < np) {
         pb = a.toff[p0 + t];
@@ -1019,11 +1047,17 @@ __global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
     const int lane = t & 63, wv = t >> 6;
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
+        const int cam = a.tcam[min(q, e1 - 1)];
+        const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
+        RecWave<R, 4> rw;
+        rec_issue<R, 4>(Wr, q0, e1 - 1, rw, lane);
+        double dv[9];
+        const double *dp = a.d + 9 * (size_t)cam;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) dv[r] = dp[r];
         R W[28];
-        const int q0 = base + 64 * wv;
-        load_records_wave<R, 4>(Wr, q0, min(64, max(0, e1 - q0)), W, stg[wv], lane);
+        rec_transpose<R, 4>(rw, nq, W, stg[wv], lane);
         if (q < e1) {
-            const double *dv = a.d + 9 * (size_t)a.tcam[q];
             double u0 = 0., u1 = 0., u2 = 0.;
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
@@ -1073,11 +1107,13 @@ __global__ __launch_bounds__(256) void k_mf_dsum(PcgArgs a)
     if (threadIdx.x == 0) a.dpart[a.nch + blockIdx.x] = s;
 }
 
-// camera-major entry j -> the point of its entry
-__global__ void k_cq_pt(int ntr, const int *camq, const int *tpt, int *cq_pt)
+// camera-major entry j -> the point of its entry; chunk -> its image
+__global__ void k_mf_index(int ntr, int nch, const int *camq, const int *tpt, const int *tcam, const int *ch_start,
+                           int *cq_pt, int *ch_cam)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < ntr) cq_pt[j] = tpt[camq[j]];
+    if (j < nch) ch_cam[j] = tcam[camq[ch_start[j]]];
 }
 
 template <typename R>
@@ -1087,24 +1123,27 @@ __global__ __launch_bounds__(64) void k_mf_chunks(PcgArgs a)
     const int ch = blockIdx.x;
     const int lane = threadIdx.x;
     const int b = a.ch_start[ch], n = a.ch_start[ch + 1] - b;
+    const int k = a.ch_cam[ch];
     const R *Yr = (const R *)a.Yrec;
+    // the entry's point, then the chunk's Y records (camera-major,
+    // consecutive, read as whole lines) and the point's t behind them
+    const int pq = a.cq_pt[b + min(lane, n - 1)];  // a chunk has at least one entry
+    __shared__ uint4 stg[64][9];
+    RecWave<R, 8> rw;
+    rec_issue<R, 8>(Yr, b, b + n - 1, rw, lane);
+    const double *tq = a.tc + 3 * (size_t)pq;
+    const double t0 = tq[0], t1 = tq[1], t2 = tq[2];
+    R Y[28];
+    rec_transpose<R, 8>(rw, n, Y, stg, lane);
     double v[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) v[r] = 0.;
-    // the chunk's Y records (camera-major, consecutive) read as whole lines
-    __shared__ uint4 stg[64][9];
-    R Y[28];
-    load_records_wave<R, 8>(Yr, b, n, Y, stg, lane);
     if (lane < n) {
-        // t of the entry's point
-        const double *tq = a.tc + 3 * (size_t)a.cq_pt[b + lane];
-        const double t0 = tq[0], t1 = tq[1], t2 = tq[2];
 #pragma unroll
         for (int r = 0; r < 9; ++r) v[r] = (double)Y[3 * r] * t0 + (double)Y[3 * r + 1] * t1 + (double)Y[3 * r + 2] * t2;
     }
 #pragma unroll
     for (int r = 0; r < 9; ++r) v[r] = wave_sum(v[r]);
-    const int k = a.tcam[a.camq[b]];
     const double *dk = a.d + 9 * (size_t)k;
     if (lane < 9) {
         double vr = v[0];
@@ -1429,9 +1468,9 @@ __global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
 template <typename R>
 __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
 {
-    // a workgroup per group of points whose first entry lies in a 256-entry
-    // window; lane per entry u_q = W_q^T dc_cam(q) to LDS, then each point's
-    // lane subtracts its entries from g_p in entry order
+    // a workgroup per point group (first entry in a kPtGroup-entry window);
+    // lane per entry u_q = W_q^T dc_cam(q) to LDS, then each point's lane
+    // subtracts its entries from g_p in entry order
     __shared__ double su[256][3];
     const int t = threadIdx.x, c = blockIdx.x;
     const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
@@ -1453,11 +1492,17 @@ __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
     const int lane = t & 63, wv = t >> 6;
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
+        const int cam = a.tcam[min(q, e1 - 1)];
+        const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
+        RecWave<R, 4> rw;
+        rec_issue<R, 4>(Wr, q0, e1 - 1, rw, lane);
+        double xc[9];
+        const double *xp = a.sol + 9 * (size_t)cam;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) xc[r] = xp[r];
         R W[28];
-        const int q0 = base + 64 * wv;
-        load_records_wave<R, 4>(Wr, q0, min(64, max(0, e1 - q0)), W, stg[wv], lane);
+        rec_transpose<R, 4>(rw, nq, W, stg[wv], lane);
         if (q < e1) {
-            const double *xc = a.sol + 9 * (size_t)a.tcam[q];
             double u0 = 0., u1 = 0., u2 = 0.;
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
@@ -1990,9 +2035,11 @@ void launch_spmv(const PcgArgs &a, hipStream_t st)
     k_mf_dsum<<<kMfDsumGroups, 256, 0, st>>>(a);
 }
 int pcg_groups(int nc) { return (nc + 27) / 28; }
-void launch_cq_pt(int ntr, const int *camq, const int *tpt, int *cq_pt, hipStream_t st)
+void launch_mf_index(int ntr, int nch, const int *camq, const int *tpt, const int *tcam, const int *ch_start,
+                     int *cq_pt, int *ch_cam, hipStream_t st)
 {
-    if (ntr > 0) k_cq_pt<<<nblk(ntr, 256), 256, 0, st>>>(ntr, camq, tpt, cq_pt);
+    const int n = std::max(ntr, nch);
+    if (n > 0) k_mf_index<<<nblk(n, 256), 256, 0, st>>>(ntr, nch, camq, tpt, tcam, ch_start, cq_pt, ch_cam);
 }
 void launch_pcg_init_w(const PcgArgs &a, hipStream_t st)
 {

@@ -77,7 +77,7 @@ struct rpba_ctx {
     DBuf<double> d_Su;
     DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_partial, d_scal, d_Dd, d_ppart, d_rowpart, d_colpart, d_dpart;
     DBuf<double> d_t, d_cpart, d_Dk;  // matrix-free form
-    DBuf<int> d_cqpt;                  // matrix-free form: camera-major entry -> point
+    DBuf<int> d_cqpt, d_chcam;         // matrix-free form: camera-major entry -> point, chunk -> image
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -252,6 +252,7 @@ static int build_structure(rpba_ctx *c)
     if (c->implicit) {
         rc |= c->d_t.alloc(3 * std::max<size_t>(np, 1));
         rc |= c->d_cqpt.alloc(std::max<size_t>(ntr, 1));
+        rc |= c->d_chcam.alloc((size_t)std::max(c->ds.nch, 1));
         rc |= c->d_cpart.alloc(9 * (size_t)std::max(c->ds.nch, 1));
         rc |= c->d_Dk.alloc(81 * (size_t)c->nc);
     }
@@ -404,6 +405,7 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.ch_start = c->ds.ch_start; a.ch_off = c->ds.ch_off;
     a.Wrec = c->d_Wrec.p; a.Yrec = c->d_Yrec.p; a.Dk = c->d_Dk.p; a.tc = c->d_t.p; a.cpart = c->d_cpart.p;
     a.cq_pt = c->implicit ? c->d_cqpt.p : nullptr;
+    a.ch_cam = c->implicit ? c->d_chcam.p : nullptr;
     a.omega = c->fl.preconditioner >= 1 ? c->d_omega.p : nullptr;
     return a;
 }
@@ -489,7 +491,9 @@ static int bundle_step(rpba_ctx *c)
         return finish_bundle_step(c, hs);
     }
     c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
-    if (c->implicit) launch_cq_pt(c->ds.ntr, c->ds.camq, c->ds.tpt, c->d_cqpt.p, c->st);
+    if (c->implicit)
+        launch_mf_index(c->ds.ntr, c->ds.nch, c->ds.camq, c->ds.tpt, c->ds.tcam, c->ds.ch_start, c->d_cqpt.p,
+                        c->d_chcam.p, c->st);
     PcgArgs pa = pcg_args(c);
     if (pa.omega) {
         // the corrected preconditioner multiplies by the off-diagonal blocks,
@@ -1325,6 +1329,7 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_dpart.release();
     c->d_t.release();
     c->d_cqpt.release();
+    c->d_chcam.release();
     c->d_cpart.release();
     c->d_Dk.release();
     c->d_ppart.release();

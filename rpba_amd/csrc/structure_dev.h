@@ -34,8 +34,8 @@ struct DevStruct {
     size_t camoff_cap = 0, camq_cap = 0;
     int *camrank = nullptr;  // entry -> its position in camera-major order (camrank[camq[j]] = j)
     size_t camrank_cap = 0;
-    // point groups of the matrix-free product: group c holds the points whose
-    // first entry lies in [256 c, 256 c + 256): points pgrp[c] .. pgrp[c+1]-1
+    // point groups of the point kernels: group c holds the points whose first
+    // entry lies in [kPtGroup c, kPtGroup (c + 1)): points pgrp[c] .. pgrp[c+1]-1
     int *pgrp = nullptr;
     size_t pgrp_cap = 0;
     int npgrp = 0;

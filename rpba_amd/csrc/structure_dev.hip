@@ -155,9 +155,9 @@ static inline unsigned nb_(long long n, int bs) { return (unsigned)((n + bs - 1)
 
 int DevStruct::point_groups(hipStream_t st)
 {
-    npgrp = (ntr + 255) / 256;
+    npgrp = (ntr + kPtGroup - 1) / kPtGroup;
     if (grow(pgrp, pgrp_cap, (size_t)npgrp + 1)) return RPBA_E_NOMEM;
-    k_lower_bound<<<nb_(npgrp + 1, 256), 256, 0, st>>>(npgrp, 256u, (const unsigned *)toff, npts, pgrp);
+    k_lower_bound<<<nb_(npgrp + 1, 256), 256, 0, st>>>(npgrp, (unsigned)kPtGroup, (const unsigned *)toff, npts, pgrp);
     return RPBA_OK;
 }
 
