@@ -75,6 +75,11 @@ struct LinArgs {
     const int *gcp_of = nullptr;
     const double *gcp_c = nullptr;
     const float *gcp_w = nullptr;
+    // camera-major copies of the per-entry inputs (entry j of camq: its
+    // point, observation and weights), streamed by the camera kernels; null:
+    // they gather through camq
+    const int *cm_pt = nullptr;
+    const float *cm_ox = nullptr, *cm_oy = nullptr, *cm_wx = nullptr, *cm_wy = nullptr, *cm_wxy = nullptr;
 };
 
 struct SchurArgs {
@@ -182,11 +187,16 @@ void launch_schur_pairs(const SchurArgs &s, hipStream_t st);
 void launch_schur_diag(const LinArgs &a, hipStream_t st);
 void launch_lin_cameras(const LinArgs &a, hipStream_t st);
 int pcg_groups(int nc);
-// cq_pt[j] = tpt[camq[j]]: the point of every camera-major entry (the
-// matrix-free product's camera side gathers t through it); ch_cam[c]: the
-// image of chunk c
-void launch_mf_index(int ntr, int nch, const int *camq, const int *tpt, const int *tcam, const int *ch_start,
-                     int *cq_pt, int *ch_cam, hipStream_t st);
+// The camera-major entry table: for entry j of camq its point (pt), its
+// observation (ox, oy) and current weights (wx, wy, wxy), copied from the
+// point-major arrays once per change of structure or weights, so that the
+// camera kernels stream them; ch_cam[c]: the image of chunk c
+struct CmTable {
+    int *pt;
+    float *ox, *oy, *wx, *wy, *wxy;
+    int *ch_cam;
+};
+void launch_cm_table(const LinArgs &a, const CmTable &t, hipStream_t st);
 int spmv_grid(int nblk);  // workgroups of the SpMV launch (PcgArgs::ndpart)
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st);
 void launch_pcg_init(const PcgArgs &a, hipStream_t st);

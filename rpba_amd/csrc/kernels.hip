@@ -242,6 +242,7 @@ __device__ __forceinline__ uint4 rec_piece(const float *v, int j)
 // issue loads of its own behind them), rec_transpose hands the pieces on.
 // Every lane of the wavefront calls both; lanes past the entries get nothing.
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kMfChunkWaves = 4;  // chunks (wavefronts) per workgroup of k_mf_chunks
 template <typename R, int PP>
 struct RecWave {
     static constexpr int NV = 16 / (int)sizeof(R);  // values per piece
@@ -493,7 +494,7 @@ __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
     __shared__ double sg[64 * 3];
     if (lane < n) {
         const int qq = a.camq[b + lane];
-        const int p = a.tpt[qq];
+        const int p = a.cm_pt ? a.cm_pt[b + lane] : a.tpt[qq];
         sq[lane] = qq;
         sg[3 * lane] = a.gp[3 * (size_t)p];
         sg[3 * lane + 1] = a.gp[3 * (size_t)p + 1];
@@ -560,14 +561,31 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
     double acc[54];
 #pragma unroll
     for (int j = 0; j < 54; ++j) acc[j] = 0.;
+    const bool cm = a.cm_pt != nullptr;
     for (int j = b + t; j < e; j += 256) {
-        const int q = a.camq[j];
-        const int p = a.tpt[q];
+        int p;
+        float ox, oy, w0, w1, w2;
+        if (cm) {
+            p = a.cm_pt[j];
+            ox = a.cm_ox[j];
+            oy = a.cm_oy[j];
+            w0 = a.cm_wx[j];
+            w1 = a.cm_wy[j];
+            w2 = a.cm_wxy[j];
+        } else {
+            const int q = a.camq[j];
+            p = a.tpt[q];
+            ox = a.ox[q];
+            oy = a.oy[q];
+            w0 = a.wx[q];
+            w1 = a.wy[q];
+            w2 = a.wxy[q];
+        }
         Ev ev;
-        eval_obs(c, a.X[3 * p], a.X[3 * p + 1], a.X[3 * p + 2], a.ox[q], a.oy[q], a.defdiff, ev);
+        eval_obs(c, a.X[3 * p], a.X[3 * p + 1], a.X[3 * p + 2], ox, oy, a.defdiff, ev);
         R Bc[2][9], Bp[2][3];
         jac_obs<R>(c, ev, mask, Bc, Bp);
-        const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
+        const R W0 = (R)w0, W1 = (R)w1, W2 = (R)w2;
         R Bt[9][2];
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
@@ -1018,7 +1036,7 @@ __device__ __forceinline__ double sum_fixed(const double *v, int n, double *sh)
 // share of d.S d is
 // -d_k.cpart[chunk], plus d_k.(S_kk + D_k) d_k on the image's first chunk,
 // whose lanes also store (S_kk + D_k) d_k in Ad[k].  k_pcg_a then forms
-// (S d)_k = Ad_k - sum of the image's chunk sums, in chunk order.
+// (S d)_k = Ad_k - the image's chunk sums (28 interleaved runs, each in chunk order).
 // k_mf_dsum adds the chunk shares of d.S d in a fixed order, in
 // kMfDsumGroups contiguous ranges.
 template <typename R>
@@ -1043,20 +1061,20 @@ __global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
     }
     double t0 = 0., t1 = 0., t2 = 0.;
     const R *Wr = (const R *)a.Wrec;
-    __shared__ uint4 stg[4][64][5];  // per wavefront: four 16-byte pieces of 64 records
+    __shared__ uint4 stg[4][64][9];  // per wavefront: eight 16-byte pieces (one line) of 64 records
     const int lane = t & 63, wv = t >> 6;
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
         const int cam = a.tcam[min(q, e1 - 1)];
         const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
-        RecWave<R, 4> rw;
-        rec_issue<R, 4>(Wr, q0, e1 - 1, rw, lane);
+        RecWave<R, 8> rw;
+        rec_issue<R, 8>(Wr, q0, e1 - 1, rw, lane);
         double dv[9];
         const double *dp = a.d + 9 * (size_t)cam;
 #pragma unroll
         for (int r = 0; r < 9; ++r) dv[r] = dp[r];
         R W[28];
-        rec_transpose<R, 4>(rw, nq, W, stg[wv], lane);
+        rec_transpose<R, 8>(rw, nq, W, stg[wv], lane);
         if (q < e1) {
             double u0 = 0., u1 = 0., u2 = 0.;
 #pragma unroll
@@ -1107,34 +1125,45 @@ __global__ __launch_bounds__(256) void k_mf_dsum(PcgArgs a)
     if (threadIdx.x == 0) a.dpart[a.nch + blockIdx.x] = s;
 }
 
-// camera-major entry j -> the point of its entry; chunk -> its image
-__global__ void k_mf_index(int ntr, int nch, const int *camq, const int *tpt, const int *tcam, const int *ch_start,
-                           int *cq_pt, int *ch_cam)
+// camera-major entry j: its point, observation and weights; chunk -> its image
+__global__ void k_cm_table(LinArgs a, CmTable t)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < ntr) cq_pt[j] = tpt[camq[j]];
-    if (j < nch) ch_cam[j] = tcam[camq[ch_start[j]]];
+    if (j < a.ntr) {
+        const int q = a.camq[j];
+        t.pt[j] = a.tpt[q];
+        t.ox[j] = a.ox[q];
+        t.oy[j] = a.oy[q];
+        t.wx[j] = a.wx[q];
+        t.wy[j] = a.wy[q];
+        t.wxy[j] = a.wxy[q];
+    }
+    if (j < a.nch) t.ch_cam[j] = a.tcam[a.camq[a.ch_start[j]]];
 }
 
 template <typename R>
-__global__ __launch_bounds__(64) void k_mf_chunks(PcgArgs a)
+__global__ __launch_bounds__(64 * kMfChunkWaves) void k_mf_chunks(PcgArgs a)
 {
+    // a wavefront per chunk, kMfChunkWaves chunks per workgroup (fewer
+    // workgroups to start, also when the PCG has stopped and they all return)
     if (a.S[a.j & 1].done) return;
-    const int ch = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int wv = threadIdx.x >> 6;
+    const int ch = blockIdx.x * kMfChunkWaves + wv;
+    if (ch >= a.nch) return;  // the whole wavefront; no workgroup barrier follows
+    const int lane = threadIdx.x & 63;
     const int b = a.ch_start[ch], n = a.ch_start[ch + 1] - b;
     const int k = a.ch_cam[ch];
     const R *Yr = (const R *)a.Yrec;
     // the entry's point, then the chunk's Y records (camera-major,
     // consecutive, read as whole lines) and the point's t behind them
     const int pq = a.cq_pt[b + min(lane, n - 1)];  // a chunk has at least one entry
-    __shared__ uint4 stg[64][9];
+    __shared__ uint4 stg[kMfChunkWaves][64][9];
     RecWave<R, 8> rw;
     rec_issue<R, 8>(Yr, b, b + n - 1, rw, lane);
     const double *tq = a.tc + 3 * (size_t)pq;
     const double t0 = tq[0], t1 = tq[1], t2 = tq[2];
     R Y[28];
-    rec_transpose<R, 8>(rw, n, Y, stg, lane);
+    rec_transpose<R, 8>(rw, n, Y, stg[wv], lane);
     double v[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) v[r] = 0.;
@@ -1157,7 +1186,7 @@ __global__ __launch_bounds__(64) void k_mf_chunks(PcgArgs a)
     for (int r = 0; r < 9; ++r) share -= dk[r] * v[r];
     if (ch == a.ch_off[k]) {
         // (S_kk + D_k) d_k, row lane
-        __shared__ double ed[9];
+        __shared__ double ed[kMfChunkWaves][9];
         if (lane < 9) {
             const double *S = a.Sd + 81 * (size_t)k + 9 * lane;
             const double *D = a.Dk + 81 * (size_t)k + 9 * lane;
@@ -1165,12 +1194,12 @@ __global__ __launch_bounds__(64) void k_mf_chunks(PcgArgs a)
 #pragma unroll
             for (int c = 0; c < 9; ++c) y += (S[c] + D[c]) * dk[c];
             a.Ad[9 * (size_t)k + lane] = y;
-            ed[lane] = y;
+            ed[wv][lane] = y;
         }
-        __syncthreads();
+        wave_lds_sync();
         double e = 0.;
 #pragma unroll
-        for (int r = 0; r < 9; ++r) e += dk[r] * ed[r];
+        for (int r = 0; r < 9; ++r) e += dk[r] * ed[wv][r];
         share += e;
     }
     if (lane == 0) a.dpart[ch] = share;
@@ -1199,12 +1228,12 @@ __device__ __forceinline__ void pcg_a_body(const PcgArgs &a, const PcgScalars &S
         for (int c = 0; c < 9; ++c) Crow[c] = C[c];
     }
     if (a.implicit) {
-        // (S d)_k = (S_kk + D_k) d_k - the chunk sums of image k, in chunk order
+        // (S d)_k = (S_kk + D_k) d_k - the chunk sums of image k: group g
+        // takes chunks g, g + 28, ... in order, the groups are added in order
         if (g < 28) {
             double acc = 0.;
-            if (g == 0)
-                for (int ch = a.ch_off[k]; ch < a.ch_off[k + 1]; ++ch) acc += a.cpart[9 * (size_t)ch + r];
-            red[g][r] = (g == 0) ? a.Ad[9 * (size_t)k + r] - acc : 0.;
+            for (int ch = a.ch_off[k] + g; ch < a.ch_off[k + 1]; ch += 28) acc += a.cpart[9 * (size_t)ch + r];
+            red[g][r] = (g == 0) ? a.Ad[9 * (size_t)k + r] - acc : -acc;
         }
     } else if (g < 28) {
     // row k: the row partials of its strip's chunks, in chunk order
@@ -2027,19 +2056,18 @@ void launch_spmv(const PcgArgs &a, hipStream_t st)
     if (a.npgrp <= 0 || a.nch <= 0) return;
     if (a.rec32) {
         k_mf_points<float><<<a.npgrp, 256, 0, st>>>(a);
-        k_mf_chunks<float><<<a.nch, 64, 0, st>>>(a);
+        k_mf_chunks<float><<<(a.nch + kMfChunkWaves - 1) / kMfChunkWaves, 64 * kMfChunkWaves, 0, st>>>(a);
     } else {
         k_mf_points<double><<<a.npgrp, 256, 0, st>>>(a);
-        k_mf_chunks<double><<<a.nch, 64, 0, st>>>(a);
+        k_mf_chunks<double><<<(a.nch + kMfChunkWaves - 1) / kMfChunkWaves, 64 * kMfChunkWaves, 0, st>>>(a);
     }
     k_mf_dsum<<<kMfDsumGroups, 256, 0, st>>>(a);
 }
 int pcg_groups(int nc) { return (nc + 27) / 28; }
-void launch_mf_index(int ntr, int nch, const int *camq, const int *tpt, const int *tcam, const int *ch_start,
-                     int *cq_pt, int *ch_cam, hipStream_t st)
+void launch_cm_table(const LinArgs &a, const CmTable &t, hipStream_t st)
 {
-    const int n = std::max(ntr, nch);
-    if (n > 0) k_mf_index<<<nblk(n, 256), 256, 0, st>>>(ntr, nch, camq, tpt, tcam, ch_start, cq_pt, ch_cam);
+    const int n = std::max(a.ntr, a.nch);
+    if (n > 0) k_cm_table<<<nblk(n, 256), 256, 0, st>>>(a, t);
 }
 void launch_pcg_init_w(const PcgArgs &a, hipStream_t st)
 {

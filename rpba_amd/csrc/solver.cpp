@@ -77,7 +77,11 @@ struct rpba_ctx {
     DBuf<double> d_Su;
     DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_partial, d_scal, d_Dd, d_ppart, d_rowpart, d_colpart, d_dpart;
     DBuf<double> d_t, d_cpart, d_Dk;  // matrix-free form
-    DBuf<int> d_cqpt, d_chcam;         // matrix-free form: camera-major entry -> point, chunk -> image
+    // camera-major entry table (point, observation, weights; chunk -> image),
+    // rebuilt when the structure or the weights change (cm_dirty)
+    DBuf<int> d_cmpt, d_chcam;
+    DBuf<float> d_cmox, d_cmoy, d_cmwx, d_cmwy, d_cmwxy;
+    bool cm_dirty = true;
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -249,10 +253,16 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_rowpart.alloc(63 * (size_t)std::max(c->ds.nchunk, 1));
     rc |= c->d_colpart.alloc(9 * (size_t)std::max(c->ds.nslab, 1));
     rc |= c->d_dpart.alloc((size_t)std::max(spmv_grid(c->ds.nblk), c->ds.nch + kMfDsumGroups));
+    rc |= c->d_cmpt.alloc(std::max<size_t>(ntr, 1));
+    rc |= c->d_cmox.alloc(std::max<size_t>(ntr, 1));
+    rc |= c->d_cmoy.alloc(std::max<size_t>(ntr, 1));
+    rc |= c->d_cmwx.alloc(std::max<size_t>(ntr, 1));
+    rc |= c->d_cmwy.alloc(std::max<size_t>(ntr, 1));
+    rc |= c->d_cmwxy.alloc(std::max<size_t>(ntr, 1));
+    rc |= c->d_chcam.alloc((size_t)std::max(c->ds.nch, 1));
+    c->cm_dirty = true;
     if (c->implicit) {
         rc |= c->d_t.alloc(3 * std::max<size_t>(np, 1));
-        rc |= c->d_cqpt.alloc(std::max<size_t>(ntr, 1));
-        rc |= c->d_chcam.alloc((size_t)std::max(c->ds.nch, 1));
         rc |= c->d_cpart.alloc(9 * (size_t)std::max(c->ds.nch, 1));
         rc |= c->d_Dk.alloc(81 * (size_t)c->nc);
     }
@@ -375,6 +385,10 @@ static LinArgs lin_args(rpba_ctx *c)
     a.Dd = c->d_Dd.p; a.nch = c->ds.nch; a.ch_start = c->ds.ch_start; a.ch_off = c->ds.ch_off;
     a.Dk = c->implicit ? c->d_Dk.p : nullptr;
     a.yrank = c->implicit ? c->ds.camrank : nullptr;
+    // the camera-major entry table (bundle_step refreshes it before the
+    // camera kernels run when the structure or the weights changed)
+    a.cm_pt = c->d_cmpt.p; a.cm_ox = c->d_cmox.p; a.cm_oy = c->d_cmoy.p;
+    a.cm_wx = c->d_cmwx.p; a.cm_wy = c->d_cmwy.p; a.cm_wxy = c->d_cmwxy.p;
     return a;
 }
 
@@ -404,7 +418,7 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.pgrp = c->ds.pgrp; a.npgrp = c->ds.npgrp;
     a.ch_start = c->ds.ch_start; a.ch_off = c->ds.ch_off;
     a.Wrec = c->d_Wrec.p; a.Yrec = c->d_Yrec.p; a.Dk = c->d_Dk.p; a.tc = c->d_t.p; a.cpart = c->d_cpart.p;
-    a.cq_pt = c->implicit ? c->d_cqpt.p : nullptr;
+    a.cq_pt = c->implicit ? c->d_cmpt.p : nullptr;
     a.ch_cam = c->implicit ? c->d_chcam.p : nullptr;
     a.omega = c->fl.preconditioner >= 1 ? c->d_omega.p : nullptr;
     return a;
@@ -464,6 +478,11 @@ static int bundle_step(rpba_ctx *c)
     SchurArgs sa;
     sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
     sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.sm = c->ds.sumap(); sa.sperm = c->ds.sperm; sa.nslab = c->ds.nslab; sa.slab_bs = c->ds.slab_bs; sa.npairs = c->ds.npairs; sa.rec32 = c->rec32;
+    if (c->cm_dirty) {
+        CmTable ct{c->d_cmpt.p, c->d_cmox.p, c->d_cmoy.p, c->d_cmwx.p, c->d_cmwy.p, c->d_cmwxy.p, c->d_chcam.p};
+        c->timed("cm_table", [&] { launch_cm_table(la, ct, c->st); });
+        c->cm_dirty = false;
+    }
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
     // the off-diagonal blocks depend only on the records: without kernel
     // timing they run on a second stream, beside the diagonal terms, the
@@ -491,9 +510,6 @@ static int bundle_step(rpba_ctx *c)
         return finish_bundle_step(c, hs);
     }
     c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
-    if (c->implicit)
-        launch_mf_index(c->ds.ntr, c->ds.nch, c->ds.camq, c->ds.tpt, c->ds.tcam, c->ds.ch_start, c->d_cqpt.p,
-                        c->d_chcam.p, c->st);
     PcgArgs pa = pcg_args(c);
     if (pa.omega) {
         // the corrected preconditioner multiplies by the off-diagonal blocks,
@@ -710,6 +726,7 @@ static int pass_setup(rpba_ctx *c)
         HIPCHK(hipStreamSynchronize(c->st));
         c->wsum = c->hb->ws;
         c->ittflag = false;
+        c->cm_dirty = true;
     }
     c->wsum = (double)(2 * c->imagepointnr) / c->wsum;
     c->wsumg = c->wsum;
@@ -791,6 +808,7 @@ static int lm_iteration(rpba_ctx *c)
         ra.s00 = c->d_s00.p; ra.wx = c->d_wx.p; ra.wy = c->d_wy.p; ra.wxy = c->d_wxy.p;
         ra.flag = c->d_flag.p; ra.partial = c->d_partial.p; ra.ipartial = c->d_ipartial.p;
         c->timed("reweight", [&] { launch_reweight(ra, c->d_scal.p + 1, c->d_iscal.p, c->st); });
+        c->cm_dirty = true;
     }
     int rc = bundle_step(c);
     if (rc) return rc;
@@ -1328,8 +1346,8 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_colpart.release();
     c->d_dpart.release();
     c->d_t.release();
-    c->d_cqpt.release();
-    c->d_chcam.release();
+    c->d_cmpt.release(); c->d_chcam.release();
+    c->d_cmox.release(); c->d_cmoy.release(); c->d_cmwx.release(); c->d_cmwy.release(); c->d_cmwxy.release();
     c->d_cpart.release();
     c->d_Dk.release();
     c->d_ppart.release();
