@@ -60,8 +60,8 @@ def algorithmic_bytes(kernel, sz):
     """Compulsory HBM bytes of one launch: every array the kernel must touch,
     once.  sz: cams, pts, obs, pairs, blocks."""
     Nc, Np, No, P, B = sz["cams"], sz["pts"], sz["obs"], sz["pairs"], sz["blocks"]
-    if kernel == "spmv" and B == 0:  # matrix-free: W and Y records once, t per entry out and in, S_kk and D_k
-        return 2 * 256 * No + 48 * No + 1296 * Nc + 144 * Nc
+    if kernel == "spmv" and B == 0:  # matrix-free: W and Y (27 doubles each) and two indices per entry, t out
+        return 2 * 216 * No + 8 * No + 48 * Np + 1296 * Nc + 144 * Nc  # and in per point, S_kk + D_k, d and Ad
     if kernel == "spmv":  # S blocks once (upper + diagonal), d in, Ad out, row partials
         return 648 * (B + Nc) + 72 * Nc + 72 * Nc + 8 * Nc
     if kernel == "schur_pairs":  # Y and W of every entry, pair lists, block offsets, S blocks out

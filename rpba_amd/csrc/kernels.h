@@ -5,8 +5,10 @@
 
 namespace rpba {
 
-// doubles per entry record of W and Y (27 used; 256 bytes keeps records line aligned)
-constexpr int kRec = 32;
+// elements per entry record of W and Y (27 used): 224 bytes in FP64, 112 in
+// FP32, whole 16-byte pieces; the kernels that read a wavefront's records
+// read them as one contiguous span
+constexpr int kRec = 28;
 
 // Upper off-diagonal blocks of S (9 x 9, block list blk_k/blk_l, row-major
 // order) are stored in strip slabs.  Strip s holds rows kSlab s .. kSlab s +

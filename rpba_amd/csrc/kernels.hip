@@ -231,72 +231,65 @@ __device__ __forceinline__ uint4 rec_piece(const float *v, int j)
     return __builtin_bit_cast(uint4, f);
 }
 
-// The 27 values of the records of a wavefront's entries q0 .. q0 + nq - 1
-// (nq <= 64, record of entry q at recs + q kRec) into lane q - q0's v, read as
-// whole lines: lane l loads piece l % PP of record (64 / PP) p + l / PP, so
-// one load instruction covers 64 / PP records' consecutive 16-byte pieces (a
-// lane loading its own record touches 64 lines per instruction), and the
-// pieces reach their record's lane through the wavefront's LDS, PP pieces of
-// every record per pass.  Two halves: rec_issue issues every load of the
-// wavefront's records into raw (one round trip to memory; the caller may
-// issue loads of its own behind them), rec_transpose hands the pieces on.
-// Every lane of the wavefront calls both; lanes past the entries get nothing.
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kMfChunkWaves = 4;  // chunks (wavefronts) per workgroup of k_mf_chunks
-template <typename R, int PP>
-struct RecWave {
-    static constexpr int NV = 16 / (int)sizeof(R);  // values per piece
-    static constexpr int NP = (27 + NV - 1) / NV;   // pieces holding the 27 values
-    static constexpr int RPI = 64 / PP;             // records per load instruction
-    static constexpr int NH = (NP + PP - 1) / PP;   // passes through LDS
-    u32x4 raw[NH * PP];
-};
 
-template <typename R, int PP>
-__device__ __forceinline__ void rec_issue(const R *recs, int q0, int qlast, RecWave<R, PP> &w, int lane)
+// The records q0 .. q0 + 63 of a wavefront as one contiguous span of 16-byte
+// pieces: load instruction i, lane l takes piece 64 i + l of the span (one
+// contiguous kilobyte per instruction); pieces past record qlast (a valid
+// record) are clamped onto its last piece and dropped.  span_issue issues every
+// load (one round trip; the caller may issue loads of its own behind them),
+// span_transpose hands each record to its lane through the wavefront's LDS:
+// the span's image, RP records (RP NP pieces) per pass, lanes of the pass's
+// records read theirs.  Every lane of the wavefront calls both; lanes past nq
+// get nothing.
+template <typename R>
+struct RecSpan {
+    static constexpr int NP = kRec * (int)sizeof(R) / 16;  // pieces per record: 14 (FP64) or 7 (FP32)
+    static constexpr int NPASS = sizeof(R) == 8 ? 2 : 1;
+    static constexpr int RP = 64 / NPASS;                  // records per pass
+    static constexpr int IPP = NP / NPASS;                 // load instructions per pass (RP NP / 64)
+    static constexpr int LDS = RP * NP;                    // pieces of LDS per wavefront
+    u32x4 raw[NP];                                         // 64 records x NP pieces / 64 lanes
+};
+static_assert(kRec * 4 % 16 == 0 && (kRec * 8 / 16) % 2 == 0, "records are whole pieces, two passes of whole loads");
+
+template <typename R>
+__device__ __forceinline__ void span_issue(const R *recs, int q0, int qlast, RecSpan<R> &w, int lane)
 {
-    // qlast: a valid record index (q0 .. q0 + nq - 1 when nq > 0); loads past
-    // the entries or the 27 values are clamped onto valid pieces and dropped
-    // later, so that no load sits under a branch (a branch makes the compiler
-    // wait for the load at its end)
-    using W = RecWave<R, PP>;
+    using S = RecSpan<R>;
+    // piece 64 i + lane of the span, clamped onto the last piece of record
+    // qlast (the span's pieces past it are dropped)
+    const u32x4 *base = (const u32x4 *)recs + (size_t)q0 * S::NP;
+    const int last = (qlast - q0 + 1) * S::NP - 1;
 #pragma unroll
-    for (int h = 0; h < W::NH; ++h) {
-#pragma unroll
-        for (int p = 0; p < PP; ++p) {
-            const int rr = W::RPI * p + lane / PP, pp = lane % PP;
-            const int q = min(q0 + rr, qlast), pc = min(PP * h + pp, W::NP - 1);
-            w.raw[h * PP + p] = ((const u32x4 *)(recs + (size_t)q * kRec))[pc];
-        }
-    }
+    for (int i = 0; i < S::NP; ++i) w.raw[i] = base[min(64 * i + lane, last)];
 }
 
-template <typename R, int PP>
-__device__ __forceinline__ void rec_transpose(const RecWave<R, PP> &w, int nq, R *v, uint4 (*stg)[PP + 1], int lane)
+template <typename R>
+__device__ __forceinline__ void span_transpose(const RecSpan<R> &w, int nq, R *v, u32x4 *stg, int lane)
 {
-    using W = RecWave<R, PP>;
+    using S = RecSpan<R>;
 #pragma unroll
-    for (int h = 0; h < W::NH; ++h) {
+    for (int h = 0; h < S::NPASS; ++h) {
 #pragma unroll
-        for (int p = 0; p < PP; ++p) *(u32x4 *)&stg[W::RPI * p + lane / PP][lane % PP] = w.raw[h * PP + p];
+        for (int i = 0; i < S::IPP; ++i) stg[64 * i + lane] = w.raw[S::IPP * h + i];
         wave_lds_sync();
-        if (lane < nq) {
+        const int rl = lane - S::RP * h;
+        if (rl >= 0 && rl < S::RP && lane < nq) {
 #pragma unroll
-            for (int j = 0; j < PP; ++j) {
-                const int pc = PP * h + j;
-                if (pc < W::NP) {
-                    const uint4 u = stg[lane][j];
-                    if constexpr (sizeof(R) == 8) {
-                        const double2 d = __builtin_bit_cast(double2, u);
-                        if (2 * pc < 27) v[2 * pc] = d.x;
-                        if (2 * pc + 1 < 27) v[2 * pc + 1] = d.y;
-                    } else {
-                        const float4 f = __builtin_bit_cast(float4, u);
-                        if (4 * pc < 27) v[4 * pc] = f.x;
-                        if (4 * pc + 1 < 27) v[4 * pc + 1] = f.y;
-                        if (4 * pc + 2 < 27) v[4 * pc + 2] = f.z;
-                        if (4 * pc + 3 < 27) v[4 * pc + 3] = f.w;
-                    }
+            for (int pc = 0; pc < S::NP; ++pc) {
+                const u32x4 u = stg[S::NP * rl + pc];
+                if constexpr (sizeof(R) == 8) {
+                    const double2 d = __builtin_bit_cast(double2, u);
+                    if (2 * pc < 27) v[2 * pc] = d.x;
+                    if (2 * pc + 1 < 27) v[2 * pc + 1] = d.y;
+                } else {
+                    const float4 f = __builtin_bit_cast(float4, u);
+                    if (4 * pc < 27) v[4 * pc] = f.x;
+                    if (4 * pc + 1 < 27) v[4 * pc + 1] = f.y;
+                    if (4 * pc + 2 < 27) v[4 * pc + 2] = f.z;
+                    if (4 * pc + 3 < 27) v[4 * pc + 3] = f.w;
                 }
             }
         }
@@ -315,18 +308,19 @@ template <typename R>
 __device__ __forceinline__ void store_records_wave(R *base, const int *rank, int q0, int nq, const R *v, uint4 (*stg)[9],
                                                    int lane)
 {
-    constexpr int NP = kRec * (int)sizeof(R) / 16;  // pieces per record: 16 (FP64) or 8 (FP32)
+    constexpr int NP = kRec * (int)sizeof(R) / 16;  // pieces per record: 14 (FP64) or 7 (FP32)
 #pragma unroll
-    for (int h = 0; h < NP / 8; ++h) {
+    for (int h = 0; h < (NP + 7) / 8; ++h) {
         if (lane < nq) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) stg[lane][j] = rec_piece(v, 8 * h + j);
+            for (int j = 0; j < 8; ++j)
+                if (8 * h + j < NP) stg[lane][j] = rec_piece(v, 8 * h + j);
         }
         wave_lds_sync();
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
             const int rr = 8 * p + (lane >> 3), pp = lane & 7;
-            if (rr < nq) {
+            if (rr < nq && 8 * h + pp < NP) {
                 const size_t rec = rank ? (size_t)rank[q0 + rr] : (size_t)(q0 + rr);
                 ((uint4 *)(base + rec * kRec))[8 * h + pp] = stg[rr][pp];
             }
@@ -422,7 +416,7 @@ __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
 
 // One thread per track entry: recomputes the observation's Jacobian (cheaper
 // than storing it) and writes W = Bc^T W Bp and Y = W V^-1 into the entry's
-// 32-element records (R = double: 256 bytes; R = float, the FP32 form: 128
+// kRec-element records (R = double: 224 bytes; R = float, the FP32 form: 112
 // bytes).  Consecutive lanes write consecutive records, so a wavefront fills
 // whole cache lines.
 template <typename R>
@@ -1061,20 +1055,20 @@ __global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
     }
     double t0 = 0., t1 = 0., t2 = 0.;
     const R *Wr = (const R *)a.Wrec;
-    __shared__ uint4 stg[4][64][9];  // per wavefront: eight 16-byte pieces (one line) of 64 records
+    __shared__ u32x4 stg[4][RecSpan<R>::LDS];  // per wavefront
     const int lane = t & 63, wv = t >> 6;
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
         const int cam = a.tcam[min(q, e1 - 1)];
         const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
-        RecWave<R, 8> rw;
-        rec_issue<R, 8>(Wr, q0, e1 - 1, rw, lane);
+        RecSpan<R> rw;
+        span_issue<R>(Wr, q0, e1 - 1, rw, lane);
         double dv[9];
         const double *dp = a.d + 9 * (size_t)cam;
 #pragma unroll
         for (int r = 0; r < 9; ++r) dv[r] = dp[r];
         R W[28];
-        rec_transpose<R, 8>(rw, nq, W, stg[wv], lane);
+        span_transpose<R>(rw, nq, W, stg[wv], lane);
         if (q < e1) {
             double u0 = 0., u1 = 0., u2 = 0.;
 #pragma unroll
@@ -1157,13 +1151,13 @@ __global__ __launch_bounds__(64 * kMfChunkWaves) void k_mf_chunks(PcgArgs a)
     // the entry's point, then the chunk's Y records (camera-major,
     // consecutive, read as whole lines) and the point's t behind them
     const int pq = a.cq_pt[b + min(lane, n - 1)];  // a chunk has at least one entry
-    __shared__ uint4 stg[kMfChunkWaves][64][9];
-    RecWave<R, 8> rw;
-    rec_issue<R, 8>(Yr, b, b + n - 1, rw, lane);
+    __shared__ u32x4 stg[kMfChunkWaves][RecSpan<R>::LDS];
+    RecSpan<R> rw;
+    span_issue<R>(Yr, b, b + n - 1, rw, lane);
     const double *tq = a.tc + 3 * (size_t)pq;
     const double t0 = tq[0], t1 = tq[1], t2 = tq[2];
     R Y[28];
-    rec_transpose<R, 8>(rw, n, Y, stg[wv], lane);
+    span_transpose<R>(rw, n, Y, stg[wv], lane);
     double v[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) v[r] = 0.;
@@ -1517,20 +1511,20 @@ __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
         t2 = a.gp[3 * (size_t)i + 2];
     }
     const R *Wr = (const R *)a.Wrec;
-    __shared__ uint4 stg[4][64][5];  // per wavefront: four 16-byte pieces of 64 records
+    __shared__ u32x4 stg[4][RecSpan<R>::LDS];  // per wavefront
     const int lane = t & 63, wv = t >> 6;
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
         const int cam = a.tcam[min(q, e1 - 1)];
         const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
-        RecWave<R, 4> rw;
-        rec_issue<R, 4>(Wr, q0, e1 - 1, rw, lane);
+        RecSpan<R> rw;
+        span_issue<R>(Wr, q0, e1 - 1, rw, lane);
         double xc[9];
         const double *xp = a.sol + 9 * (size_t)cam;
 #pragma unroll
         for (int r = 0; r < 9; ++r) xc[r] = xp[r];
         R W[28];
-        rec_transpose<R, 4>(rw, nq, W, stg[wv], lane);
+        span_transpose<R>(rw, nq, W, stg[wv], lane);
         if (q < e1) {
             double u0 = 0., u1 = 0., u2 = 0.;
 #pragma unroll

@@ -986,8 +986,10 @@ static int ctx_init(rpba_ctx *c)
         }
         const double bup = std::min(0.5 * (double)nc * (double)(nc - 1), pairs);
         const double formed = 648. * (bup + (double)nc);
+        // matrix-free: both records, the image and point index of every
+        // entry, and t written and read per point
         const double rec = (double)kRec * (c->rec32 ? 4. : 8.);
-        const double free_ = (2. * rec + 48.) * (double)c->ds.ntr;
+        const double free_ = (2. * rec + 8.) * (double)c->ds.ntr + 48. * (double)c->ds.npts;
         c->implicit = c->fl.solver == 0 && c->fl.preconditioner == 0 && free_ < formed;
         c->ds.with_pairs = !c->implicit;
     }
