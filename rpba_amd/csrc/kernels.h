@@ -191,14 +191,16 @@ void launch_lin_cameras(const LinArgs &a, hipStream_t st);
 int pcg_groups(int nc);
 // The camera-major entry table: for entry j of camq its point (pt), its
 // observation (ox, oy) and current weights (wx, wy, wxy), copied from the
-// point-major arrays once per change of structure or weights, so that the
-// camera kernels stream them; ch_cam[c]: the image of chunk c
+// point-major arrays (entry q to slot camrank[q]) when the structure (full)
+// or the weights change, so that the camera kernels stream them; ch_cam[c]:
+// the image of chunk c
 struct CmTable {
+    const int *camrank;
     int *pt;
     float *ox, *oy, *wx, *wy, *wxy;
     int *ch_cam;
 };
-void launch_cm_table(const LinArgs &a, const CmTable &t, hipStream_t st);
+void launch_cm_table(const LinArgs &a, const CmTable &t, bool full, hipStream_t st);
 int spmv_grid(int nblk);  // workgroups of the SpMV launch (PcgArgs::ndpart)
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st);
 void launch_pcg_init(const PcgArgs &a, hipStream_t st);

@@ -1119,20 +1119,24 @@ __global__ __launch_bounds__(256) void k_mf_dsum(PcgArgs a)
     if (threadIdx.x == 0) a.dpart[a.nch + blockIdx.x] = s;
 }
 
-// camera-major entry j: its point, observation and weights; chunk -> its image
-__global__ void k_cm_table(LinArgs a, CmTable t)
+// the camera-major entry table, from the point-major arrays read in order
+// (entry q goes to its camera-major slot camrank[q]): point and observation
+// when full, the current weights always; chunk -> its image when full
+__global__ void k_cm_table(LinArgs a, CmTable t, int full)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < a.ntr) {
-        const int q = a.camq[j];
-        t.pt[j] = a.tpt[q];
-        t.ox[j] = a.ox[q];
-        t.oy[j] = a.oy[q];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < a.ntr) {
+        const int j = t.camrank[q];
+        if (full) {
+            t.pt[j] = a.tpt[q];
+            t.ox[j] = a.ox[q];
+            t.oy[j] = a.oy[q];
+        }
         t.wx[j] = a.wx[q];
         t.wy[j] = a.wy[q];
         t.wxy[j] = a.wxy[q];
     }
-    if (j < a.nch) t.ch_cam[j] = a.tcam[a.camq[a.ch_start[j]]];
+    if (full && q < a.nch) t.ch_cam[q] = a.tcam[a.camq[a.ch_start[q]]];
 }
 
 template <typename R>
@@ -2058,10 +2062,10 @@ void launch_spmv(const PcgArgs &a, hipStream_t st)
     k_mf_dsum<<<kMfDsumGroups, 256, 0, st>>>(a);
 }
 int pcg_groups(int nc) { return (nc + 27) / 28; }
-void launch_cm_table(const LinArgs &a, const CmTable &t, hipStream_t st)
+void launch_cm_table(const LinArgs &a, const CmTable &t, bool full, hipStream_t st)
 {
     const int n = std::max(a.ntr, a.nch);
-    if (n > 0) k_cm_table<<<nblk(n, 256), 256, 0, st>>>(a, t);
+    if (n > 0) k_cm_table<<<nblk(n, 256), 256, 0, st>>>(a, t, full ? 1 : 0);
 }
 void launch_pcg_init_w(const PcgArgs &a, hipStream_t st)
 {

@@ -78,10 +78,10 @@ struct rpba_ctx {
     DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_partial, d_scal, d_Dd, d_ppart, d_rowpart, d_colpart, d_dpart;
     DBuf<double> d_t, d_cpart, d_Dk;  // matrix-free form
     // camera-major entry table (point, observation, weights; chunk -> image),
-    // rebuilt when the structure or the weights change (cm_dirty)
+    // rebuilt when the structure (cm_full) or the weights (cm_w) change
     DBuf<int> d_cmpt, d_chcam;
     DBuf<float> d_cmox, d_cmoy, d_cmwx, d_cmwy, d_cmwxy;
-    bool cm_dirty = true;
+    bool cm_full = true, cm_w = true;
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -260,7 +260,7 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_cmwy.alloc(std::max<size_t>(ntr, 1));
     rc |= c->d_cmwxy.alloc(std::max<size_t>(ntr, 1));
     rc |= c->d_chcam.alloc((size_t)std::max(c->ds.nch, 1));
-    c->cm_dirty = true;
+    c->cm_full = true;
     if (c->implicit) {
         rc |= c->d_t.alloc(3 * std::max<size_t>(np, 1));
         rc |= c->d_cpart.alloc(9 * (size_t)std::max(c->ds.nch, 1));
@@ -478,10 +478,10 @@ static int bundle_step(rpba_ctx *c)
     SchurArgs sa;
     sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
     sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.sm = c->ds.sumap(); sa.sperm = c->ds.sperm; sa.nslab = c->ds.nslab; sa.slab_bs = c->ds.slab_bs; sa.npairs = c->ds.npairs; sa.rec32 = c->rec32;
-    if (c->cm_dirty) {
-        CmTable ct{c->d_cmpt.p, c->d_cmox.p, c->d_cmoy.p, c->d_cmwx.p, c->d_cmwy.p, c->d_cmwxy.p, c->d_chcam.p};
-        c->timed("cm_table", [&] { launch_cm_table(la, ct, c->st); });
-        c->cm_dirty = false;
+    if (c->cm_full || c->cm_w) {
+        CmTable ct{c->ds.camrank, c->d_cmpt.p, c->d_cmox.p, c->d_cmoy.p, c->d_cmwx.p, c->d_cmwy.p, c->d_cmwxy.p, c->d_chcam.p};
+        c->timed("cm_table", [&] { launch_cm_table(la, ct, c->cm_full, c->st); });
+        c->cm_full = c->cm_w = false;
     }
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
     // the off-diagonal blocks depend only on the records: without kernel
@@ -542,7 +542,6 @@ static int pcg_solve(rpba_ctx *c, PcgArgs &pa, bool wait_pairs, PcgScalars &hs)
     if (wait_pairs) HIPCHK(hipStreamWaitEvent(c->st, c->ev_pairs, 0));
     // launch j reads the scalars of slot j & 1 and writes slot (j + 1) & 1;
     // once the stopping rule fired, later launches only pass the state on.
-    // The first chunk is sized from the previous solve's iteration count.
     // Chunks are pipelined: chunk n + 1 is queued before the host reads the
     // scalars after chunk n, so the GPU never waits for the host; at most one
     // short chunk runs after the stop.
@@ -601,10 +600,16 @@ static int pcg_solve(rpba_ctx *c, PcgArgs &pa, bool wait_pairs, PcgScalars &hs)
         ++n;
         return RPBA_OK;
     };
-    int rc = queue_chunk(std::max(2, c->pcg_last));
+    // an iteration that streams more than ~256 MB (S d) outlasts the host's
+    // turnaround by far: chunks of one iteration (after a first chunk of
+    // two) stop within a launch of the rule; smaller ones take the previous
+    // solve's count first, then chunks of three
+    const double it_bytes = c->implicit ? 450. * (double)c->ds.ntr : 648. * ((double)c->ds.nblk + (double)c->nc);
+    const bool long_it = it_bytes > 2.56e8;
+    int rc = queue_chunk(long_it ? 2 : std::max(2, c->pcg_last));
     if (rc) return rc;
     for (int m = 0;; ++m) {
-        rc = queue_chunk(use_graph ? 2 : 3);
+        rc = queue_chunk(use_graph ? 2 : (long_it ? 1 : 3));
         if (rc) return rc;
         HIPCHK(hipEventSynchronize(c->pcg_ev[m & 1]));
         hs = c->hb->pcg[m & 1];
@@ -726,7 +731,7 @@ static int pass_setup(rpba_ctx *c)
         HIPCHK(hipStreamSynchronize(c->st));
         c->wsum = c->hb->ws;
         c->ittflag = false;
-        c->cm_dirty = true;
+        c->cm_w = true;
     }
     c->wsum = (double)(2 * c->imagepointnr) / c->wsum;
     c->wsumg = c->wsum;
@@ -808,7 +813,7 @@ static int lm_iteration(rpba_ctx *c)
         ra.s00 = c->d_s00.p; ra.wx = c->d_wx.p; ra.wy = c->d_wy.p; ra.wxy = c->d_wxy.p;
         ra.flag = c->d_flag.p; ra.partial = c->d_partial.p; ra.ipartial = c->d_ipartial.p;
         c->timed("reweight", [&] { launch_reweight(ra, c->d_scal.p + 1, c->d_iscal.p, c->st); });
-        c->cm_dirty = true;
+        c->cm_w = true;
     }
     int rc = bundle_step(c);
     if (rc) return rc;
