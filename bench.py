@@ -14,16 +14,22 @@ pass boundary (RobustDelete + set-up of pass 1); if the solve finishes inside
 the timed region it restarts from the initial state (rpba_ctx_reset) and the
 restart is timed too.  value = K / elapsed (LM iterations per second).
 
+The plain form takes schur = 2: whichever PCG product streams fewer bytes
+per iteration (the matrix-free product through the W and Y records on this
+scene, S formed block by block on the 961-camera one).
+
 Also reported: time to converged (one full solve: both passes, from the
 uploaded block to the stored state on the device) and the whole drop-in call
 from host buffers (rpba_calc_plsa: PointXXvIn, uploads, solve, download); the
-final sigma0 and RMS; the other forms (FP32 Jacobians and W, Y = W V^-1
-records, the matrix-free Schur product, the learned correction of the
-block-Jacobi preconditioner, and combinations), each under a PCG-iteration and
-wall-time budget, with its time to reach the plain form's final sigma0 and its
-errors against plain FP64, the fastest that reaches it as "chosen"; the
-per-kernel HIP-event roofline of the dominant kernel; and the CPU oracle timed
-on a bounded sample of one LM iteration of the same scene (cpu_baseline).
+final sigma0 and RMS; the other forms (S formed, FP32 Jacobians and W, Y =
+W V^-1 records, the learned correction of the block-Jacobi preconditioner,
+and combinations), each under a PCG-iteration and wall-time budget, with its
+time to reach the plain form's final sigma0 and its errors against plain
+FP64, the fastest that reaches it as "chosen"; the per-kernel HIP-event
+roofline of the dominant kernel (S d); the CPU oracle timed on a bounded
+sample of one LM iteration of the same scene (cpu_baseline); and the
+partitioned solve (Threads = 0 dispatch) of the 961-camera scene as a whole
+rpba_calc_plsa call beside the oracle's OpenMP restatement of it.
 """
 import argparse
 import ctypes as C
