@@ -85,7 +85,14 @@ typedef struct rpba_flags {
     int32_t est_f;           /* f  -> addflags[2] */
     int32_t k24out;          /* k24out (I/O scaling only, rpba.cpp:255-262)     */
     int32_t precision;       /* 0 = plain FP64; 1 = FP32 Jacobians and W V^-1 W^T
-                                products with FP64 sums, S, PCG, residuals and cost */
+                                products with FP64 sums, S, PCG, residuals and cost;
+                                2 = whitened Jacobians L^T Bc, L^T Bp (observation
+                                weight W = L L^T) rounded to FP32 once, and U, V,
+                                W, g, the Schur terms and the PCG product formed
+                                from them in FP64 (S stays positive definite at
+                                every rounding); 3 = the same records in FP64.
+                                2 and 3 take the matrix-free product (schur 1 or
+                                2), block-Jacobi and solver 0 only            */
     int32_t preconditioner;  /* 0 = block-Jacobi C = S_kk^-1 (rbacore.cpp:1838-1846);
                                 1 = block-Jacobi with the first-order Neumann
                                 correction C - w C N C (N: off-diagonal part of S),

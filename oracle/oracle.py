@@ -203,6 +203,23 @@ class order:
         lib().oracle_set_order(0)
 
 
+class arithmetic:
+    """Context manager selecting how the oracle forms its normal equations
+    (oracle_set_diag): 0 = the reference's order (default); 1 = the FP32
+    W / Y records emulated; 2 = the whitened Jacobians L^T Bc, L^T Bp (with
+    the observation weight W = L L^T) rounded to float once and U, V, W, g
+    and S formed from them in FP64; 3 = the same whitened Jacobians in FP64."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().oracle_set_diag(self.m, 0, 0, -1, None, 0)
+
+    def __exit__(self, *a):
+        lib().oracle_set_diag(0, 0, 0, -1, None, 0)
+
+
 class Handle:
     """One oracle solver state, for probing single steps."""
 

@@ -61,14 +61,17 @@ static int g_order = 0;
 void oracle_set_mutation(int m) { g_mut = m; }
 void oracle_set_order(int m) { g_order = m; }
 
-/* Diagnostics of the FP32 forms (tests/golden/fp32_diagnostic.py only):
+/* Arithmetic modes of the normal equations (the FP32 diagnostic
+ * tests/golden/fp32_diagnostic.py, and the parity reference of the GPU's
+ * whitened-Jacobian forms):
  *   prec 1: the GPU's FP32 records emulated -- the Jacobians Bc, Bp rounded
  *           to float, Bc^T W and W = Bc^T W Bp formed and rounded in float,
  *           Y = W V^-1 from a float V^-1 rounded in float; S sums the
  *           products of those floats in double;
  *   prec 2: consistently rounded factors -- with the weight W = L L^T, the
  *           whitened Jacobians L^T Bc and L^T Bp rounded to float once, and
- *           U, V, W, g, Y and S all formed from them in double.
+ *           U, V, W, g, Y and S all formed from them in double;
+ *   prec 3: the same whitened Jacobians, not rounded (FP64 throughout).
  * pcg_cap > 0 lowers the PCG cap; the step-norm ratios m / m0 of PCG solve
  * number pcg_trace_call (0-based, counted from the setting) go to
  * pcg_trace[0..cap); stop_after > 0 ends Adjust after that many LM
@@ -809,17 +812,20 @@ static void corrections(orc *o, int mode, double mult1, const double *XM, int sr
                 if (g_prec == 1) {
                     for (int j = 0; j < n; ++j) { Bc[j] = (float)Bc[j]; Bc[MAXB + j] = (float)Bc[MAXB + j]; }
                     for (int j = 0; j < 3; ++j) { Bp[j] = (float)Bp[j]; Bp[3 + j] = (float)Bp[3 + j]; }
-                } else if (g_prec == 2) {
-                    /* W = L L^T; whitened columns L^T B rounded once; residual L^T r */
+                } else if (g_prec == 2 || g_prec == 3) {
+                    /* W = L L^T; whitened columns L^T B (rounded once to
+                     * float in mode 2); residual L^T r */
                     const double l11 = sqrt((double)W[0]), l21 = (double)W[2] / l11;
                     const double l22 = sqrt((double)W[1] - l21 * l21);
                     for (int j = 0; j < n; ++j) {
                         const double a0 = l11 * Bc[j] + l21 * Bc[MAXB + j], a1 = l22 * Bc[MAXB + j];
-                        Bc[j] = (float)a0; Bc[MAXB + j] = (float)a1;
+                        Bc[j] = g_prec == 2 ? (double)(float)a0 : a0;
+                        Bc[MAXB + j] = g_prec == 2 ? (double)(float)a1 : a1;
                     }
                     for (int j = 0; j < 3; ++j) {
                         const double a0 = l11 * Bp[j] + l21 * Bp[3 + j], a1 = l22 * Bp[3 + j];
-                        Bp[j] = (float)a0; Bp[3 + j] = (float)a1;
+                        Bp[j] = g_prec == 2 ? (double)(float)a0 : a0;
+                        Bp[3 + j] = g_prec == 2 ? (double)(float)a1 : a1;
                     }
                     const double r0 = l11 * e.diff1 + l21 * e.diff2, r1 = l22 * e.diff2;
                     e.diff1 = r0;

@@ -82,6 +82,12 @@ struct LinArgs {
     // they gather through camq
     const int *cm_pt = nullptr;
     const float *cm_ox = nullptr, *cm_oy = nullptr, *cm_wx = nullptr, *cm_wy = nullptr, *cm_wxy = nullptr;
+    // whitened Jacobian records (jform, precision 2): point-major and
+    // camera-major copies, kJRec elements each (float when rec32), and the
+    // damping of S_kk's diagonal per camera (9)
+    int jform = 0;
+    void *Jpm = nullptr, *Jcm = nullptr;
+    double *dmp = nullptr;
 };
 
 struct SchurArgs {
@@ -131,6 +137,11 @@ struct PcgArgs {
     // corrected preconditioner (S formed only): h = C r - w C N C r with the
     // weight w in device memory (null: block-Jacobi)
     float *omega = nullptr;
+    // matrix-free product through the whitened Jacobian records (jform):
+    // tc holds s_i = V_i^-1 t_i, Ad_k = dmp_k d_k
+    int jform = 0;
+    const void *Jpm = nullptr, *Jcm = nullptr;
+    const double *dmp = nullptr, *Vinv = nullptr;
 };
 
 struct StepArgs {
@@ -147,6 +158,9 @@ struct StepArgs {
     // step (decode skipped otherwise), bit 1 = first camera of its block
     // (pose fixed, never decoded; image 0 when null)
     const unsigned char *camflag = nullptr;
+    // whitened Jacobian records (point-major), or null
+    int jform = 0;
+    const void *Jpm = nullptr;
 };
 
 struct CostArgs {
@@ -188,6 +202,12 @@ void launch_lin_points(const LinArgs &a, hipStream_t st);
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st);
 void launch_schur_diag(const LinArgs &a, hipStream_t st);
 void launch_lin_cameras(const LinArgs &a, hipStream_t st);
+// whitened Jacobian records (precision 2): V, g_p and the point-major records,
+// then the camera blocks S_kk, b, the damping and the camera-major records
+// (no separate Schur terms)
+void launch_lin_points_j(const LinArgs &a, hipStream_t st);
+void launch_lin_cameras_j(const LinArgs &a, hipStream_t st);
+constexpr int kJRecElems = 24;  // elements of a whitened Jacobian record
 int pcg_groups(int nc);
 // The camera-major entry table: for entry j of camq its point (pt), its
 // observation (ox, oy) and current weights (wx, wy, wxy), copied from the

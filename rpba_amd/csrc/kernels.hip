@@ -1971,6 +1971,545 @@ __global__ __launch_bounds__(256) void k_out_points(int npts, const int *nrpv, u
 }
 
 // ---------------------------------------------------------------------------
+// Whitened Jacobian records (precision 2).  With the observation weight
+// W = L L^T (L lower: l11 = sqrt(wx), l21 = wxy / l11, l22 = sqrt(wy - l21^2))
+// the rows of L^T Bc and L^T Bp are rounded once to R and stored per entry
+// (kJRec values: 96 bytes in FP32).  Everything else is formed from them in
+// FP64 (a product of two floats is exact in FP64):
+//   V_i = sum Bp~^T Bp~, g_p = sum Bp~^T r~ (r~ = L^T r, FP64),
+//   S_kk = sum_q Bc~^T K_q Bc~ with K_q = I - Bp~ V^-1 Bp~^T (2 x 2),
+//   b_k = sum_q Bc~^T (r~ - Bp~ V^-1 g_p),
+//   (S d)_k = sum_q Bc~^T (Bc~ d_k - Bp~ s_pt(q)) + delta_k d_k with
+//   s_i = V_i^-1 sum_(q of i) Bp~^T Bc~ d_cam(q) and delta_k the damping of
+//   S_kk's diagonal.
+// The reduced camera system is then the Schur complement of the positive
+// semi-definite sum J~^T J~ at every rounding of the records (plus the
+// damping), so it stays positive definite and the PCG product is symmetric
+// to FP64 rounding.  Records are point-major (Jpm, entry q at q) and
+// camera-major (Jcm, camera-major entry j at j).
+// ---------------------------------------------------------------------------
+
+constexpr int kJRec = kJRecElems;  // Bc~ row 0 (9), row 1 (9), Bp~ row 0 (3), row 1 (3)
+
+struct Whitener {
+    double l11, l21, l22;
+};
+__device__ __forceinline__ Whitener whitener(float wx, float wy, float wxy)
+{
+    Whitener w;
+    w.l11 = sqrt((double)wx);
+    w.l21 = (double)wxy / w.l11;
+    w.l22 = sqrt((double)wy - w.l21 * w.l21);
+    return w;
+}
+
+template <typename R>
+__device__ __forceinline__ void whiten_jac(const double Bc[2][9], const double Bp[2][3], const Whitener &w,
+                                           R J[kJRec])
+{
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        J[j] = (R)(w.l11 * Bc[0][j] + w.l21 * Bc[1][j]);
+        J[9 + j] = (R)(w.l22 * Bc[1][j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        J[18 + j] = (R)(w.l11 * Bp[0][j] + w.l21 * Bp[1][j]);
+        J[21 + j] = (R)(w.l22 * Bp[1][j]);
+    }
+}
+
+template <typename R>
+struct JSpan {
+    static constexpr int NP = kJRec * (int)sizeof(R) / 16;  // pieces per record: 6 (FP32) or 12 (FP64)
+    static constexpr int NPASS = sizeof(R) == 8 ? 2 : 1;   // LDS passes of the load transpose
+    static constexpr int LDS = 64 * NP / NPASS;            // pieces of LDS per wavefront (load)
+};
+
+template <typename R>
+__device__ __forceinline__ void jrec_pack(const R J[kJRec], u32x4 pc[JSpan<R>::NP])
+{
+#pragma unroll
+    for (int p = 0; p < JSpan<R>::NP; ++p) {
+        if constexpr (sizeof(R) == 8) {
+            double2 d;
+            d.x = J[2 * p];
+            d.y = J[2 * p + 1];
+            pc[p] = __builtin_bit_cast(u32x4, d);
+        } else {
+            float4 f;
+            f.x = J[4 * p];
+            f.y = J[4 * p + 1];
+            f.z = J[4 * p + 2];
+            f.w = J[4 * p + 3];
+            pc[p] = __builtin_bit_cast(u32x4, f);
+        }
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void jrec_unpack(const u32x4 pc[JSpan<R>::NP], R J[kJRec])
+{
+#pragma unroll
+    for (int p = 0; p < JSpan<R>::NP; ++p) {
+        if constexpr (sizeof(R) == 8) {
+            const double2 d = __builtin_bit_cast(double2, pc[p]);
+            J[2 * p] = d.x;
+            J[2 * p + 1] = d.y;
+        } else {
+            const float4 f = __builtin_bit_cast(float4, pc[p]);
+            J[4 * p] = f.x;
+            J[4 * p + 1] = f.y;
+            J[4 * p + 2] = f.z;
+            J[4 * p + 3] = f.w;
+        }
+    }
+}
+
+// The wavefront's records q0 .. q0 + nq - 1 (consecutive from dst, NP pieces
+// each) written as one contiguous span: each lane's record goes to the
+// wavefront's LDS, then store instruction i writes piece 64 i + l of the
+// span (whole lines).  Every lane of the wavefront calls it.
+template <int NP>
+__device__ __forceinline__ void jspan_store(u32x4 *dst, int nq, const u32x4 pc[NP], u32x4 *stg, int lane)
+{
+    if (lane < nq) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) stg[NP * lane + p] = pc[p];
+    }
+    wave_lds_sync();
+    const int n = nq * NP;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        const int x = 64 * i + lane;
+        if (x < n) dst[x] = stg[x];
+    }
+    wave_lds_sync();
+}
+
+// The records q0 .. q0 + 63 as one contiguous span of pieces (load
+// instruction i, lane l: piece 64 i + l, clamped onto the last piece of
+// record qlast, a valid record); jspan_transpose hands each record to its
+// lane through the wavefront's LDS (NPASS passes of 64 / NPASS records).
+// Every lane of the wavefront calls both; lanes past nq get nothing.
+template <typename R>
+__device__ __forceinline__ void jspan_issue(const void *recs, int q0, int qlast, u32x4 raw[JSpan<R>::NP], int lane)
+{
+    constexpr int NP = JSpan<R>::NP;
+    const u32x4 *base = (const u32x4 *)recs + (size_t)q0 * NP;
+    const int last = (qlast - q0 + 1) * NP - 1;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) raw[i] = base[min(64 * i + lane, last)];
+}
+
+template <typename R>
+__device__ __forceinline__ void jspan_transpose(const u32x4 raw[JSpan<R>::NP], int nq, R J[kJRec], u32x4 *stg, int lane)
+{
+    constexpr int NP = JSpan<R>::NP, NPASS = JSpan<R>::NPASS, IPP = NP / NPASS, RP = 64 / NPASS;
+    u32x4 pc[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) pc[p] = raw[p];  // lanes past nq keep their own (unused) pieces
+#pragma unroll
+    for (int h = 0; h < NPASS; ++h) {
+#pragma unroll
+        for (int i = 0; i < IPP; ++i) stg[64 * i + lane] = raw[IPP * h + i];
+        wave_lds_sync();
+        const int rl = lane - RP * h;
+        if (rl >= 0 && rl < RP && lane < nq) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p) pc[p] = stg[NP * rl + p];
+        }
+        wave_lds_sync();
+    }
+    jrec_unpack<R>(pc, J);
+}
+
+// V, g_p and the point-major records: a workgroup per point group, lane per
+// entry (Jacobians in FP64, whitened, rounded once to R, written as the
+// wavefront's span; its V and g_p terms in FP64 from the rounded values to
+// LDS), then each point's lane adds its entries in order; damping and V^-1
+// as k_lin_points.
+template <typename R>
+__global__ __launch_bounds__(256) void k_lin_points_j(LinArgs a)
+{
+    constexpr int NP = JSpan<R>::NP;
+    constexpr int SV = 256 * 12 / 2, ST = 4 * 64 * NP;  // pieces: per-entry terms; record staging
+    __shared__ u32x4 lds[SV > ST ? SV : ST];
+    double(*sv)[12] = reinterpret_cast<double(*)[12]>(lds);
+    const int t = threadIdx.x, c = blockIdx.x, lane = t & 63, wv = t >> 6;
+    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    if (p0 >= p1) return;
+    const int e0 = a.toff[p0], e1 = a.toff[p1];
+    const int np = p1 - p0;
+    const int i = p0 + t;
+    int pb = 0, pe = 0;
+    if (t < np) {
+        pb = a.toff[i];
+        pe = a.toff[i + 1];
+    }
+    double V[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+    u32x4 *Jpm = (u32x4 *)a.Jpm;
+    for (int base = e0; base < e1; base += 256) {
+        const int q = base + t;
+        double term[12];
+        u32x4 pc[NP];
+        if (q < e1) {
+            const int k = a.tcam[q], ip = a.tpt[q];
+            CamV cv;
+            load_cam(a.cam, k, cv);
+            Ev ev;
+            eval_obs(cv, a.X[3 * (size_t)ip], a.X[3 * (size_t)ip + 1], a.X[3 * (size_t)ip + 2], a.ox[q], a.oy[q], a.defdiff, ev);
+            double Bc[2][9], Bp[2][3];
+            jac_obs<double>(cv, ev, a.cam.mask[k], Bc, Bp);
+            const Whitener w = whitener(a.wx[q], a.wy[q], a.wxy[q]);
+            R J[kJRec];
+            whiten_jac<R>(Bc, Bp, w, J);
+            const double r0 = w.l11 * ev.d1 + w.l21 * ev.d2, r1 = w.l22 * ev.d2;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const double b0 = (double)J[18 + r], b1 = (double)J[21 + r];
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) term[r * 3 + cc] = b0 * (double)J[18 + cc] + b1 * (double)J[21 + cc];
+                term[9 + r] = b0 * r0 + b1 * r1;
+            }
+            jrec_pack<R>(J, pc);
+        }
+        const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
+        if (nq > 0) jspan_store<NP>(Jpm + (size_t)q0 * NP, nq, pc, lds + 64 * NP * wv, lane);
+        __syncthreads();
+        if (q < e1) {
+#pragma unroll
+            for (int j = 0; j < 12; ++j) sv[t][j] = term[j];
+        }
+        __syncthreads();
+        if (t < np) {
+            const int lo = max(pb, base), hi = min(pe, base + 256);
+            for (int x = lo; x < hi; ++x) {
+#pragma unroll
+                for (int j = 0; j < 9; ++j) V[j] += sv[x - base][j];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) g[j] += sv[x - base][9 + j];
+            }
+        }
+        __syncthreads();
+    }
+    if (t >= np) return;
+    V[0] *= a.mult1;
+    V[4] *= a.mult1;
+    V[8] *= a.mult1;
+    double Vi[9];
+    inv3(V, Vi);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) a.Vinv[9 * (size_t)i + j] = Vi[j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) a.gp[3 * (size_t)i + j] = g[j];
+}
+
+// Camera blocks and the camera-major records: a workgroup per image, lane per
+// camera-major entry (from the entry table; the point's X, V^-1 and g_p
+// gathered): the same rounded whitened Jacobian as the point side, written as
+// the wavefront's span, and the entry's terms Bc~^T K Bc~ (upper 45) and
+// Bc~^T rho in FP64, added over the image (lanes, then the four wavefronts);
+// S_kk = that sum with the diagonal damping x(1 + lambda), +1e-2
+// (rbacore.cpp:1164-1170), the damping itself in dmp (1 on fixed slots),
+// fixed slots an identity row / column with zero right-hand side.
+template <typename R>
+__global__ __launch_bounds__(256) void k_lin_cameras_j(LinArgs a)
+{
+    constexpr int NP = JSpan<R>::NP;
+    const int k = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    __shared__ double red[4][54];
+    __shared__ double Ssh[81];
+    __shared__ double bsh[9];
+    __shared__ u32x4 stg[4 * 64 * NP];
+    const unsigned mask = a.cam.mask[k];
+    CamV c;
+    load_cam(a.cam, k, c);
+    const int b = a.camoff[k], e = a.camoff[k + 1];
+    double acc[54];
+#pragma unroll
+    for (int j = 0; j < 54; ++j) acc[j] = 0.;
+    u32x4 *Jcm = (u32x4 *)a.Jcm;
+    for (int j0 = b; j0 < e; j0 += 256) {
+        const int j = j0 + t;
+        u32x4 pc[NP];
+        if (j < e) {
+            const int p = a.cm_pt[j];
+            const double *Xp = a.X + 3 * (size_t)p;
+            const double *Vp = a.Vinv + 9 * (size_t)p;
+            const double *gq = a.gp + 3 * (size_t)p;
+            double Vi[9], gpv[3];
+#pragma unroll
+            for (int m = 0; m < 9; ++m) Vi[m] = Vp[m];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) gpv[m] = gq[m];
+            Ev ev;
+            eval_obs(c, Xp[0], Xp[1], Xp[2], a.cm_ox[j], a.cm_oy[j], a.defdiff, ev);
+            double Bc[2][9], Bp[2][3];
+            jac_obs<double>(c, ev, mask, Bc, Bp);
+            const Whitener w = whitener(a.cm_wx[j], a.cm_wy[j], a.cm_wxy[j]);
+            R J[kJRec];
+            whiten_jac<R>(Bc, Bp, w, J);
+            const double r0 = w.l11 * ev.d1 + w.l21 * ev.d2, r1 = w.l22 * ev.d2;
+            // P = Bp~ V^-1 (2 x 3), K = I - P Bp~^T, rho = r~ - P g_p
+            double P[2][3];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+                    P[s][cc] = (double)J[18 + 3 * s] * Vi[cc] + (double)J[19 + 3 * s] * Vi[3 + cc] +
+                               (double)J[20 + 3 * s] * Vi[6 + cc];
+            const double k00 = 1. - (P[0][0] * (double)J[18] + P[0][1] * (double)J[19] + P[0][2] * (double)J[20]);
+            const double k01 = -(P[0][0] * (double)J[21] + P[0][1] * (double)J[22] + P[0][2] * (double)J[23]);
+            const double k11 = 1. - (P[1][0] * (double)J[21] + P[1][1] * (double)J[22] + P[1][2] * (double)J[23]);
+            const double rho0 = r0 - (P[0][0] * gpv[0] + P[0][1] * gpv[1] + P[0][2] * gpv[2]);
+            const double rho1 = r1 - (P[1][0] * gpv[0] + P[1][1] * gpv[1] + P[1][2] * gpv[2]);
+            int n = 0;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const double a0 = (double)J[r], a1 = (double)J[9 + r];
+                const double t0 = a0 * k00 + a1 * k01, t1 = a0 * k01 + a1 * k11;
+#pragma unroll
+                for (int cc = r; cc < 9; ++cc) acc[n++] += t0 * (double)J[cc] + t1 * (double)J[9 + cc];
+                acc[45 + r] += a0 * rho0 + a1 * rho1;
+            }
+            jrec_pack<R>(J, pc);
+        }
+        const int q0 = j0 + 64 * wv, nq = min(64, max(0, e - q0));
+        if (nq > 0) jspan_store<NP>(Jcm + (size_t)q0 * NP, nq, pc, stg + 64 * NP * wv, lane);
+    }
+#pragma unroll
+    for (int j = 0; j < 54; ++j) {
+        const double s = wave_sum(acc[j]);
+        if (lane == 0) red[wv][j] = s;
+    }
+    __syncthreads();
+    if (t < 81) {
+        const int r = t / 9, cc = t % 9;
+        const int rr = r < cc ? r : cc, c2 = r < cc ? cc : r;
+        const int n = rr * 9 - rr * (rr - 1) / 2 + (c2 - rr);  // index in the packed upper triangle
+        const double s0 = red[0][n] + red[1][n] + red[2][n] + red[3][n];
+        double s = s0;
+        if (r == cc) {
+            s *= a.mult1;
+            s += 1.e-2;
+        }
+        const bool fr = (mask >> r) & 1u, fc = (mask >> cc) & 1u;
+        if (!fr || !fc) s = (r == cc) ? 1. : 0.;
+        Ssh[t] = s;
+        if (r == cc) a.dmp[9 * (size_t)k + r] = fr ? s - s0 : 1.;
+    } else if (t >= 128 && t < 137) {
+        const int r = t - 128;
+        double s = red[0][45 + r] + red[1][45 + r] + red[2][45 + r] + red[3][45 + r];
+        if (!((mask >> r) & 1u)) s = 0.;
+        bsh[r] = s;
+    }
+    __syncthreads();
+    if (t < 81) a.Sd[81 * (size_t)k + t] = Ssh[t];
+    if (t < 9) a.bvec[9 * (size_t)k + t] = bsh[t];
+}
+
+// Matrix-free S d through the whitened records, point side: a workgroup per
+// point group, lane per entry u_q = Bp~^T (Bc~ d_cam(q)) (the records as the
+// wavefront's span, the image index and d behind them) to LDS; each point's
+// lane adds its entries in order and stores s_i = V_i^-1 t_i (tc).
+template <typename R>
+__global__ __launch_bounds__(256) void k_mf_points_j(PcgArgs a)
+{
+    if (a.S[a.j & 1].done) return;
+    __shared__ double su[256][3];
+    __shared__ u32x4 stg[4][JSpan<R>::LDS];
+    const int t = threadIdx.x, c = blockIdx.x, lane = t & 63, wv = t >> 6;
+    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    if (p0 >= p1) return;
+    const int e0 = a.toff[p0], e1 = a.toff[p1];
+    const int np = p1 - p0;
+    int pb = 0, pe = 0;
+    if (t < np) {
+        pb = a.toff[p0 + t];
+        pe = a.toff[p0 + t + 1];
+    }
+    double t0 = 0., t1 = 0., t2 = 0.;
+    for (int base = e0; base < e1; base += 256) {
+        const int q = base + t;
+        const int cam = a.tcam[min(q, e1 - 1)];
+        const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
+        u32x4 raw[JSpan<R>::NP];
+        jspan_issue<R>(a.Jpm, q0, e1 - 1, raw, lane);
+        double dv[9];
+        const double *dp = a.d + 9 * (size_t)cam;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) dv[r] = dp[r];
+        R J[kJRec];
+        jspan_transpose<R>(raw, nq, J, stg[wv], lane);
+        if (q < e1) {
+            double v0 = 0., v1 = 0.;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                v0 += (double)J[r] * dv[r];
+                v1 += (double)J[9 + r] * dv[r];
+            }
+            su[t][0] = (double)J[18] * v0 + (double)J[21] * v1;
+            su[t][1] = (double)J[19] * v0 + (double)J[22] * v1;
+            su[t][2] = (double)J[20] * v0 + (double)J[23] * v1;
+        }
+        __syncthreads();
+        if (t < np) {
+            const int lo = max(pb, base), hi = min(pe, base + 256);
+            for (int x = lo; x < hi; ++x) {
+                t0 += su[x - base][0];
+                t1 += su[x - base][1];
+                t2 += su[x - base][2];
+            }
+        }
+        __syncthreads();
+    }
+    if (t < np) {
+        const double *Vi = a.Vinv + 9 * (size_t)(p0 + t);
+        double *o = a.tc + 3 * (size_t)(p0 + t);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) o[r] = Vi[3 * r] * t0 + Vi[3 * r + 1] * t1 + Vi[3 * r + 2] * t2;
+    }
+}
+
+// Camera side: a wavefront per camera chunk, lane per entry
+// Bc~^T (Bc~ d_k - Bp~ s_pt(q)) (camera-major records as the span, the
+// entry's point and its s gathered behind them), summed over the chunk in
+// lane order and stored negated in cpart (k_pcg_a subtracts the chunk sums
+// from Ad); the image's first chunk stores Ad_k = dmp_k d_k.  Shares of
+// d.S d as k_mf_chunks.
+template <typename R>
+__global__ __launch_bounds__(64 * kMfChunkWaves) void k_mf_chunks_j(PcgArgs a)
+{
+    if (a.S[a.j & 1].done) return;
+    const int wv = threadIdx.x >> 6;
+    const int ch = blockIdx.x * kMfChunkWaves + wv;
+    if (ch >= a.nch) return;  // the whole wavefront; no workgroup barrier follows
+    const int lane = threadIdx.x & 63;
+    const int b = a.ch_start[ch], n = a.ch_start[ch + 1] - b;
+    const int k = a.ch_cam[ch];
+    const int pq = a.cq_pt[b + min(lane, n - 1)];  // a chunk has at least one entry
+    __shared__ u32x4 stg[kMfChunkWaves][JSpan<R>::LDS];
+    u32x4 raw[JSpan<R>::NP];
+    jspan_issue<R>(a.Jcm, b, b + n - 1, raw, lane);
+    const double *sq = a.tc + 3 * (size_t)pq;
+    const double s0 = sq[0], s1 = sq[1], s2 = sq[2];
+    const double *dk = a.d + 9 * (size_t)k;
+    double dv[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) dv[r] = dk[r];
+    R J[kJRec];
+    jspan_transpose<R>(raw, n, J, stg[wv], lane);
+    double v[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) v[r] = 0.;
+    if (lane < n) {
+        double v0 = 0., v1 = 0.;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            v0 += (double)J[r] * dv[r];
+            v1 += (double)J[9 + r] * dv[r];
+        }
+        const double w0 = v0 - ((double)J[18] * s0 + (double)J[19] * s1 + (double)J[20] * s2);
+        const double w1 = v1 - ((double)J[21] * s0 + (double)J[22] * s1 + (double)J[23] * s2);
+#pragma unroll
+        for (int r = 0; r < 9; ++r) v[r] = -((double)J[r] * w0 + (double)J[9 + r] * w1);
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) v[r] = wave_sum(v[r]);
+    if (lane < 9) {
+        double vr = v[0];
+#pragma unroll
+        for (int r = 1; r < 9; ++r)
+            if (lane == r) vr = v[r];
+        a.cpart[9 * (size_t)ch + lane] = vr;
+    }
+    double share = 0.;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) share -= dv[r] * v[r];
+    if (ch == a.ch_off[k]) {
+        double e = 0.;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const double y = a.dmp[9 * (size_t)k + r] * dv[r];
+            e += dv[r] * y;
+            if (lane == r) a.Ad[9 * (size_t)k + r] = y;
+        }
+        share += e;
+    }
+    if (lane == 0) a.dpart[ch] = share;
+}
+
+// back-substitution through the whitened records: dp_i = V_i^-1 (g_p -
+// sum_q Bp~^T (Bc~ dc_k)); Xn = X - dp (rbacore.cpp:1395-1410, 547)
+template <typename R>
+__global__ __launch_bounds__(256) void k_backsub_j(StepArgs a)
+{
+    __shared__ double su[256][3];
+    __shared__ u32x4 stg[4][JSpan<R>::LDS];
+    const int t = threadIdx.x, c = blockIdx.x, lane = t & 63, wv = t >> 6;
+    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    if (p0 >= p1) return;
+    const int e0 = a.toff[p0], e1 = a.toff[p1];
+    const int np = p1 - p0;
+    const int i = p0 + t;
+    int pb = 0, pe = 0;
+    double t0 = 0., t1 = 0., t2 = 0.;
+    if (t < np) {
+        pb = a.toff[i];
+        pe = a.toff[i + 1];
+        t0 = a.gp[3 * (size_t)i];
+        t1 = a.gp[3 * (size_t)i + 1];
+        t2 = a.gp[3 * (size_t)i + 2];
+    }
+    for (int base = e0; base < e1; base += 256) {
+        const int q = base + t;
+        const int cam = a.tcam[min(q, e1 - 1)];
+        const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
+        u32x4 raw[JSpan<R>::NP];
+        jspan_issue<R>(a.Jpm, q0, e1 - 1, raw, lane);
+        double xc[9];
+        const double *xp = a.sol + 9 * (size_t)cam;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) xc[r] = xp[r];
+        R J[kJRec];
+        jspan_transpose<R>(raw, nq, J, stg[wv], lane);
+        if (q < e1) {
+            double v0 = 0., v1 = 0.;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                v0 += (double)J[r] * xc[r];
+                v1 += (double)J[9 + r] * xc[r];
+            }
+            su[t][0] = (double)J[18] * v0 + (double)J[21] * v1;
+            su[t][1] = (double)J[19] * v0 + (double)J[22] * v1;
+            su[t][2] = (double)J[20] * v0 + (double)J[23] * v1;
+        }
+        __syncthreads();
+        if (t < np) {
+            const int lo = max(pb, base), hi = min(pe, base + 256);
+            for (int x = lo; x < hi; ++x) {
+                t0 -= su[x - base][0];
+                t1 -= su[x - base][1];
+                t2 -= su[x - base][2];
+            }
+        }
+        __syncthreads();
+    }
+    if (t >= np) return;
+    const double *Vi = a.Vinv + 9 * (size_t)i;
+    bool bad = false;
+    const double tt[3] = {t0, t1, t2};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const double dp = Vi[r * 3] * tt[0] + Vi[r * 3 + 1] * tt[1] + Vi[r * 3 + 2] * tt[2];
+        if (isnan(dp) || isinf(dp)) bad = true;
+        a.Xn[3 * (size_t)i + r] = a.X[3 * (size_t)i + r] - dp;
+    }
+    if (bad) a.S->nan_flag = 1;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 
@@ -2013,6 +2552,21 @@ void launch_lin_cameras(const LinArgs &a, hipStream_t st)
     else
         k_lin_cameras<double><<<a.nc, 256, 0, st>>>(a);
 }
+void launch_lin_points_j(const LinArgs &a, hipStream_t st)
+{
+    if (a.npts <= 0 || a.npgrp <= 0) return;
+    if (a.rec32)
+        k_lin_points_j<float><<<a.npgrp, 256, 0, st>>>(a);
+    else
+        k_lin_points_j<double><<<a.npgrp, 256, 0, st>>>(a);
+}
+void launch_lin_cameras_j(const LinArgs &a, hipStream_t st)
+{
+    if (a.rec32)
+        k_lin_cameras_j<float><<<a.nc, 256, 0, st>>>(a);
+    else
+        k_lin_cameras_j<double><<<a.nc, 256, 0, st>>>(a);
+}
 void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
 {
     k_bj_inverse<<<nblk(nc, 64), 64, 0, st>>>(nc, Sd, Cinv);
@@ -2052,7 +2606,15 @@ void launch_spmv(const PcgArgs &a, hipStream_t st)
         return;
     }
     if (a.npgrp <= 0 || a.nch <= 0) return;
-    if (a.rec32) {
+    if (a.jform) {
+        if (a.rec32) {
+            k_mf_points_j<float><<<a.npgrp, 256, 0, st>>>(a);
+            k_mf_chunks_j<float><<<(a.nch + kMfChunkWaves - 1) / kMfChunkWaves, 64 * kMfChunkWaves, 0, st>>>(a);
+        } else {
+            k_mf_points_j<double><<<a.npgrp, 256, 0, st>>>(a);
+            k_mf_chunks_j<double><<<(a.nch + kMfChunkWaves - 1) / kMfChunkWaves, 64 * kMfChunkWaves, 0, st>>>(a);
+        }
+    } else if (a.rec32) {
         k_mf_points<float><<<a.npgrp, 256, 0, st>>>(a);
         k_mf_chunks<float><<<(a.nch + kMfChunkWaves - 1) / kMfChunkWaves, 64 * kMfChunkWaves, 0, st>>>(a);
     } else {
@@ -2090,7 +2652,12 @@ void launch_pcg_update(const PcgArgs &a, hipStream_t st)
 void launch_backsub(const StepArgs &a, hipStream_t st)
 {
     if (a.npts <= 0 || a.npgrp <= 0) return;
-    if (a.rec32)
+    if (a.jform) {
+        if (a.rec32)
+            k_backsub_j<float><<<a.npgrp, 256, 0, st>>>(a);
+        else
+            k_backsub_j<double><<<a.npgrp, 256, 0, st>>>(a);
+    } else if (a.rec32)
         k_backsub<float><<<a.npgrp, 256, 0, st>>>(a);
     else
         k_backsub<double><<<a.npgrp, 256, 0, st>>>(a);

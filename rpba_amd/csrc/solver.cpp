@@ -64,7 +64,8 @@ struct rpba_ctx {
     std::vector<double> Pst, Ist;    // stored cameras (host copy at pass boundaries)
     std::vector<double> Pst_init, Ist_init, R0, C0;
     bool structure_dirty = true;     // pairs/blocks must be rebuilt
-    int rec32 = 0;                   // FP32 Jacobians and W/Y records (precision 1)
+    int rec32 = 0;                   // FP32 records (precision 1: W/Y; precision 2: whitened Jacobians)
+    int jform = 0;                   // whitened Jacobian records (precision 2 and 3; matrix-free)
     bool implicit = false;           // matrix-free S product (schur 1)
     DevStruct ds;                    // device tracks, camera-major order, pairs, blocks
     DenseSolver dense;               // dense Cholesky form (solver 1)
@@ -77,6 +78,8 @@ struct rpba_ctx {
     DBuf<double> d_Su;
     DBuf<double> d_x, d_best, d_r, d_d, d_Ad, d_h, d_h1, d_partial, d_scal, d_Dd, d_ppart, d_rowpart, d_colpart, d_dpart;
     DBuf<double> d_t, d_cpart, d_Dk;  // matrix-free form
+    DBuf<unsigned char> d_Jpm, d_Jcm;  // whitened Jacobian records (precision 2)
+    DBuf<double> d_dmp;                // damping of S_kk's diagonal (precision 2)
     // camera-major entry table (point, observation, weights; chunk -> image),
     // rebuilt when the structure (cm_full) or the weights (cm_w) change
     DBuf<int> d_cmpt, d_chcam;
@@ -244,12 +247,18 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_flag.alloc(ntr);
     rc |= c->d_flagst.alloc(ntr);
     const size_t esz = c->rec32 ? sizeof(float) : sizeof(double);
-    rc |= c->d_Wrec.alloc((size_t)kRec * esz * ntr);
-    rc |= c->d_Yrec.alloc((size_t)kRec * esz * ntr);
+    if (c->jform) {
+        rc |= c->d_Jpm.alloc((size_t)kJRecElems * esz * ntr);
+        rc |= c->d_Jcm.alloc((size_t)kJRecElems * esz * ntr);
+        rc |= c->d_dmp.alloc(9 * (size_t)c->nc);
+    } else {
+        rc |= c->d_Wrec.alloc((size_t)kRec * esz * ntr);
+        rc |= c->d_Yrec.alloc((size_t)kRec * esz * ntr);
+    }
     rc |= c->d_Vinv.alloc(9 * np);
     rc |= c->d_gp.alloc(3 * np);
     rc |= c->d_Su.alloc(81 * (size_t)std::max(c->ds.nblk, 1));
-    rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
+    if (!c->jform) rc |= c->d_Dd.alloc(90 * (size_t)std::max(c->ds.nch, 1));
     rc |= c->d_rowpart.alloc(63 * (size_t)std::max(c->ds.nchunk, 1));
     rc |= c->d_colpart.alloc(9 * (size_t)std::max(c->ds.nslab, 1));
     rc |= c->d_dpart.alloc((size_t)std::max(spmv_grid(c->ds.nblk), c->ds.nch + kMfDsumGroups));
@@ -264,7 +273,7 @@ static int build_structure(rpba_ctx *c)
     if (c->implicit) {
         rc |= c->d_t.alloc(3 * std::max<size_t>(np, 1));
         rc |= c->d_cpart.alloc(9 * (size_t)std::max(c->ds.nch, 1));
-        rc |= c->d_Dk.alloc(81 * (size_t)c->nc);
+        if (!c->jform) rc |= c->d_Dk.alloc(81 * (size_t)c->nc);
     }
     const int nb = std::max(cost_blocks(c->ds.ntr), (int)((np + 255) / 256 + 1));
     rc |= c->d_partial.alloc((size_t)nb);
@@ -389,6 +398,7 @@ static LinArgs lin_args(rpba_ctx *c)
     // camera kernels run when the structure or the weights changed)
     a.cm_pt = c->d_cmpt.p; a.cm_ox = c->d_cmox.p; a.cm_oy = c->d_cmoy.p;
     a.cm_wx = c->d_cmwx.p; a.cm_wy = c->d_cmwy.p; a.cm_wxy = c->d_cmwxy.p;
+    a.jform = c->jform; a.Jpm = c->d_Jpm.p; a.Jcm = c->d_Jcm.p; a.dmp = c->d_dmp.p;
     return a;
 }
 
@@ -421,6 +431,7 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.cq_pt = c->implicit ? c->d_cmpt.p : nullptr;
     a.ch_cam = c->implicit ? c->d_chcam.p : nullptr;
     a.omega = c->fl.preconditioner >= 1 ? c->d_omega.p : nullptr;
+    a.jform = c->jform; a.Jpm = c->d_Jpm.p; a.Jcm = c->d_Jcm.p; a.dmp = c->d_dmp.p; a.Vinv = c->d_Vinv.p;
     return a;
 }
 
@@ -432,6 +443,7 @@ static StepArgs step_args(rpba_ctx *c, const double *sol)
     a.Wrec = c->d_Wrec.p; a.rec32 = c->rec32;
     a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.sol = sol; a.X = c->d_X.p; a.Xn = c->d_Xn.p;
     a.S = c->d_pcg.p;
+    a.jform = c->jform; a.Jpm = c->d_Jpm.p;
     return a;
 }
 
@@ -482,6 +494,19 @@ static int bundle_step(rpba_ctx *c)
         CmTable ct{c->ds.camrank, c->d_cmpt.p, c->d_cmox.p, c->d_cmoy.p, c->d_cmwx.p, c->d_cmwy.p, c->d_cmwxy.p, c->d_chcam.p};
         c->timed("cm_table", [&] { launch_cm_table(la, ct, c->cm_full, c->st); });
         c->cm_full = c->cm_w = false;
+    }
+    if (c->jform) {
+        // whitened Jacobian records: V, g_p and the point-major records, then
+        // the camera blocks with their Schur terms and the camera-major records
+        c->timed("lin_points", [&] { launch_lin_points_j(la, c->st); });
+        c->timed("lin_cameras", [&] { launch_lin_cameras_j(la, c->st); });
+        c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
+        PcgArgs pa = pcg_args(c);
+        PcgScalars hs{};
+        const int rc = pcg_solve(c, pa, false, hs);
+        if (rc) return rc;
+        if (c->pcg_budget > 0 && hs.iters >= pa.msize && pa.msize < std::max(1000, c->nrioff)) return RPBA_E_BUDGET;
+        return finish_bundle_step(c, hs);
     }
     c->timed("lin_points", [&] { launch_lin_points(la, c->st); });
     // the off-diagonal blocks depend only on the records: without kernel
@@ -604,7 +629,7 @@ static int pcg_solve(rpba_ctx *c, PcgArgs &pa, bool wait_pairs, PcgScalars &hs)
     // turnaround by far: chunks of one iteration (after a first chunk of
     // two) stop within a launch of the rule; smaller ones take the previous
     // solve's count first, then chunks of three
-    const double it_bytes = c->implicit ? 450. * (double)c->ds.ntr : 648. * ((double)c->ds.nblk + (double)c->nc);
+    const double it_bytes = c->implicit ? (c->jform ? (c->rec32 ? 200. : 392.) : 450.) * (double)c->ds.ntr : 648. * ((double)c->ds.nblk + (double)c->nc);
     const bool long_it = it_bytes > 2.56e8;
     int rc = queue_chunk(long_it ? 2 : std::max(2, c->pcg_last));
     if (rc) return rc;
@@ -871,9 +896,13 @@ static int ctx_init(rpba_ctx *c)
     rpba_block *io = c->io;
     const int nc = io->n_cams;
     c->nc = nc;
-    // precision 1: FP32 Jacobians and W/Y records, FP64 sums, S, PCG and cost
-    c->rec32 = c->fl.precision >= 1;
-    c->implicit = c->fl.schur == 1;
+    // precision 1: FP32 Jacobians and W/Y records, FP64 sums, S, PCG and cost;
+    // precision 2: whitened Jacobians rounded to FP32 once, everything else
+    // formed from them in FP64, matrix-free product; precision 3: the same
+    // records in FP64
+    c->rec32 = c->fl.precision == 1 || c->fl.precision == 2;
+    c->jform = c->fl.precision >= 2;
+    c->implicit = c->fl.schur == 1 || c->jform;
     c->ds.with_pairs = !c->implicit;
     c->addf[0] = c->fl.est_k2 != 0;
     c->addf[1] = c->fl.est_k4 != 0;
@@ -995,7 +1024,7 @@ static int ctx_init(rpba_ctx *c)
         // entry, and t written and read per point
         const double rec = (double)kRec * (c->rec32 ? 4. : 8.);
         const double free_ = (2. * rec + 8.) * (double)c->ds.ntr + 48. * (double)c->ds.npts;
-        c->implicit = c->fl.solver == 0 && c->fl.preconditioner == 0 && free_ < formed;
+        c->implicit = c->jform || (c->fl.solver == 0 && c->fl.preconditioner == 0 && free_ < formed);
         c->ds.with_pairs = !c->implicit;
     }
     rc = upload_initial_cameras(c);
@@ -1040,7 +1069,10 @@ static int partition_dispatch(const rpba_flags *f, int nc)
 extern "C" int rpba_ctx_create(rpba_ctx **out, rpba_block *io, const rpba_flags *flags, rpba_log_fn log, void *user)
 {
     if (!out || !io || !flags || io->n_cams < 1 || io->n_pts < 0 || io->n_obs < 0) return RPBA_E_INVALID;
-    if (flags->precision < 0 || flags->precision > 1 || flags->schur < 0 || flags->schur > 2) return RPBA_E_INVALID;
+    if (flags->precision < 0 || flags->precision > 3 || flags->schur < 0 || flags->schur > 2) return RPBA_E_INVALID;
+    // the whitened Jacobian records are read by the matrix-free product only
+    if (flags->precision >= 2 && (flags->schur == 0 || flags->solver != 0 || flags->preconditioner != 0))
+        return RPBA_E_INVALID;
     // the dense form factors the formed S: it needs the off-diagonal blocks
     if (flags->solver < 0 || flags->solver > 1 || (flags->solver == 1 && flags->schur == 1)) return RPBA_E_INVALID;
     // the corrected preconditioner multiplies by the formed off-diagonal blocks
@@ -1357,6 +1389,9 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_cmox.release(); c->d_cmoy.release(); c->d_cmwx.release(); c->d_cmwy.release(); c->d_cmwxy.release();
     c->d_cpart.release();
     c->d_Dk.release();
+    c->d_Jpm.release();
+    c->d_Jcm.release();
+    c->d_dmp.release();
     c->d_ppart.release();
     if (c->st) (void)hipStreamDestroy(c->st);
     if (c->st2) (void)hipStreamDestroy(c->st2);

@@ -89,10 +89,15 @@ def test_invalid_arguments_are_rejected_without_gpu():
 
 
 def test_unknown_form_selectors_are_rejected_without_gpu(tiny_scene):
-    """precision outside {0, 1} and schur outside {0, 1, 2} fail before any
-    device call."""
+    """precision outside {0 .. 3} and schur outside {0, 1, 2} fail before any
+    device call, and so do the whitened-Jacobian forms (precision 2, 3) with
+    S formed, the dense solver or a corrected preconditioner: their records
+    are read by the matrix-free product only."""
     from rpba_amd import api
-    for kw in ({"precision": 2}, {"schur": 3}, {"schur": -1}):
+    for kw in ({"precision": 4}, {"precision": -1}, {"schur": 3}, {"schur": -1},
+               {"precision": 2, "schur": 0}, {"precision": 3, "schur": 0}, {"precision": 2, "schur": 2, "solver": 1},
+               {"precision": 2, "schur": 2, "preconditioner": 1, "pc_omega": 0.5},
+               {"precision": 3, "schur": 1, "preconditioner": 2}):
         rc, _, _, _ = api.calc_plsa(tiny_scene, capi.make_flags(**kw))
         assert rc == capi.RPBA_E_INVALID, kw
 

@@ -55,10 +55,13 @@ def _param_errors(fix, out):
             "pts_median_abs": float(np.median(np.abs(X - X_ref)))}
 
 
-def test_c3_plain_fp64_matches_oracle(scene_c3):
+@pytest.mark.parametrize("precision", [0, 3])
+def test_c3_plain_fp64_matches_oracle(scene_c3, precision):
+    """precision 3: the FP64 whitened-Jacobian records, held to the same bar."""
     fix = _fixture()
-    print("[c3] plain FP64 GPU solve", flush=True)
-    rc, out, st, log = api.calc_plsa(scene_c3, capi.make_flags(robust=1))
+    print(f"[c3] plain FP64 GPU solve, precision {precision}", flush=True)
+    fl = capi.make_flags(robust=1, precision=precision, schur=2 if precision else 0)
+    rc, out, st, log = api.calc_plsa(scene_c3, fl)
     assert rc == fix["status"] == 0 and st["passes_done"] == 2
     sig, ref = np.asarray(st["trace_sig0"]), np.asarray(fix["trace_sig0"])
     print("[c3] GPU sig0", sig.tolist(), "pcg", st["trace_pcg"].tolist(), flush=True)
@@ -80,6 +83,34 @@ def test_c3_plain_fp64_matches_oracle(scene_c3):
     print(f"[c3] pass-1 first sig0 rel {rel1.tolist()}, final sigma0 rel {s1:.3g}, RMS {rms:.6f} "
           f"(oracle {fix['rms_px']:.6f}, rel {rms_rel:.3g}), params {_param_errors(fix, out)}", flush=True)
     assert rel1.max() < PASS1_REL and s1 < PASS1_REL and rms_rel < PASS1_REL
+
+
+def test_c3_fp32_whitened_form_against_oracle(scene_c3):
+    """The FP32 whitened-Jacobian form (precision 2: L^T Bc, L^T Bp rounded to
+    FP32 once, everything formed from them in FP64, matrix-free product)
+    against the FP64 oracle: pass 0 within 1e-4 of it in sig0 at every
+    iteration, PCG counts within 2, the pass-0 sigma0 within 1e-5, labels of
+    all but 1e-4 of the observations alike; every PCG meets the step-norm
+    rule (none reaches the reference's cap of max(1000, n))."""
+    fix = _fixture()
+    print("[c3] FP32 whitened-Jacobian GPU solve", flush=True)
+    rc, out, st, log = api.calc_plsa(scene_c3, capi.make_flags(robust=1, precision=2, schur=2))
+    assert rc == 0 and st["passes_done"] == 2
+    sig, ref = np.asarray(st["trace_sig0"]), np.asarray(fix["trace_sig0"])
+    n0 = int(log[0].split()[-1])
+    m = min(n0, int(fix["log"][0].split()[-1]))
+    rel0 = np.abs(sig[:m] - ref[:m]) / ref[:m]
+    s0 = abs(st["sigma_out"][0] - fix["sigma_out"][0]) / fix["sigma_out"][0]
+    s1 = abs(st["sigma_out"][1] - fix["sigma_out"][1]) / fix["sigma_out"][1]
+    rms = scenes.rms_px(scene_c3, out)
+    rms_rel = abs(rms - fix["rms_px"]) / fix["rms_px"]
+    print(f"[c3 fp32 whitened] pass-0 sig0 rel {rel0.tolist()}, sigma0 rel pass 0 {s0:.3g} pass 1 {s1:.3g}, "
+          f"RMS {rms:.6f} rel {rms_rel:.3g}, itt {st['itt']} (oracle {fix['itt']}), pcg {st['trace_pcg'].tolist()} "
+          f"(oracle {list(fix['trace_pcg'])}), params {_param_errors(fix, out)}", flush=True)
+    assert rel0.max() < 1e-4 and s0 < 1e-5
+    assert np.abs(np.asarray(st["trace_pcg"][:m]) - np.asarray(fix["trace_pcg"][:m])).max() <= 2
+    assert max(st["trace_pcg"]) < max(1000, 9 * scene_c3["n_cams"] - 9)
+    assert abs(int(out["obs_kept"].sum()) - fix["n_obs_kept"]) <= 1e-4 * fix["n_obs"]
 
 
 def test_c3_fp32_form_against_oracle(scene_c3):

@@ -21,12 +21,16 @@ pytestmark = pytest.mark.gpu
 CASES = [c for c in make_golden.CASES if os.path.exists(os.path.join(GOLD, c + ".json"))]
 
 
+# precision 3: the FP64 whitened-Jacobian records (the same products in
+# another association), held to the same bar against the same fixtures
+@pytest.mark.parametrize("precision", [0, 3])
 @pytest.mark.parametrize("case", CASES)
-def test_gpu_matches_golden(case):
+def test_gpu_matches_golden(case, precision):
     fix = json.load(open(os.path.join(GOLD, case + ".json")))
     sc = make_golden.make_scene(make_golden.CASES[case]["scene"])
-    print(f"[golden {case}] GPU solve n_obs={sc['n_obs']}", flush=True)
-    rc, out, st, log = api.calc_plsa(sc, capi.make_flags(**fix["flags"]))
+    print(f"[golden {case} precision={precision}] GPU solve n_obs={sc['n_obs']}", flush=True)
+    fl = dict(fix["flags"], precision=precision, schur=2) if precision else fix["flags"]
+    rc, out, st, log = api.calc_plsa(sc, capi.make_flags(**fl))
     assert rc == fix["status"]
     assert st["itt"] == fix["itt"] and st["passes_done"] == fix["passes_done"]
     check_trajectory(st["trace_sig0"], fix["trace_sig0"], st["trace_pcg"], fix["trace_pcg"],
