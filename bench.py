@@ -21,11 +21,13 @@ scene, S formed block by block on the 961-camera one).
 Also reported: time to converged (one full solve: both passes, from the
 uploaded block to the stored state on the device) and the whole drop-in call
 from host buffers (rpba_calc_plsa: PointXXvIn, uploads, solve, download); the
-final sigma0 and RMS; the other forms (S formed, FP32 Jacobians and W, Y =
-W V^-1 records, the learned correction of the block-Jacobi preconditioner,
-and combinations), each under a PCG-iteration and wall-time budget, with its
-time to reach the plain form's final sigma0 and its errors against plain
-FP64, the fastest that reaches it as "chosen"; the per-kernel HIP-event
+final sigma0 and RMS; the other forms (the whitened Jacobian records in FP32
+and in FP64, S formed, FP32 Jacobians and W, Y = W V^-1 records, the learned
+correction of the block-Jacobi preconditioner, and combinations), each under a
+PCG-iteration and wall-time budget, with its LM iterations per second, its
+time to reach the plain form's final sigma0 and its pass-0 sigma0, and its
+errors against plain FP64, the fastest that reaches the final sigma0 as
+"chosen" (and the fastest to the pass-0 sigma0 beside it); the per-kernel HIP-event
 roofline of the dominant kernel (S d); the CPU oracle timed on a bounded
 sample of one LM iteration of the same scene (cpu_baseline); and the
 partitioned solve (Threads = 0 dispatch) of the 961-camera scene as a whole
@@ -91,7 +93,10 @@ def algorithmic_bytes(kernel, sz):
 # reported in the line)
 FORM_SKIP = {
     ("c5_13682", "fp32_records"): "FP32 W / Y records: the first pass-1 PCG does not meet the step-norm rule on "
-                                  "this scene (S is not consistently rounded; see DESIGN)",
+                                  "this scene (W, Y, U and V rounded independently: S loses positive "
+                                  "definiteness); the whitened FP32 form (precision 2) is its replacement",
+    ("c5_13682", "fp32_matrix_free"): "FP32 W / Y records, matrix-free: the same stall as fp32_records; the "
+                                      "whitened FP32 form (precision 2) is its replacement",
     ("c3_4585", "fp32_learned"): "FP32 W / Y records with the learned preconditioner: the first pass-1 PCG does "
                                  "not meet the step-norm rule on this scene (see DESIGN)",
 }
@@ -182,6 +187,11 @@ def run_bench(args):
     # FP64 (FP32 records, the matrix-free Schur product, the learned
     # correction of block-Jacobi, and their combinations)
     target = st["sig0_final"] * (1.0 + 1e-6)
+    # pass 0 (lambda = 1e-3) converges the same way in every form; pass 1
+    # opens with lambda = 1e-6 on a nearly singular S, and where a form lands
+    # there turns on its first pass-1 PCG
+    p0 = [(t, sg) for t, ps, sg in recs if ps == 0]
+    plain_pass0_s, plain_pass0_sig = p0[-1] if p0 else (None, None)
     f64 = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
     lp_model = None
     from rpba_amd import lp_train
@@ -206,9 +216,31 @@ def run_bench(args):
             progress(f"form {name}: stopped ({e})")
             return dict(res, stopped=str(e), time_to_reach_plain_sigma0_s=None)
         _, cout, cst = cctx.download()
+        # the form's LM iterations per second over the headline's step count
+        cctx.reset()
+        device_sync()
+        t0f = time.perf_counter()
+        left = args.steps
+        while left > 0:
+            rcf, donef, finf = cctx.run(left)
+            if rcf != 0:
+                break
+            left -= donef
+            if finf:
+                cctx.reset()
+        device_sync()
+        lm_its = None if left > 0 else round(args.steps / (time.perf_counter() - t0f), 3)
         cctx.close()
         reach = next((t for t, ps, sg in crecs if ps == 1 and sg <= target), None)
-        return dict(res, **{
+        c0 = [(t, sg) for t, ps, sg in crecs if ps == 0]
+        pass0 = {}
+        if c0 and plain_pass0_sig:
+            t_c0, sg_c0 = c0[-1]
+            pass0 = {"pass0_s": round(t_c0, 4), "pass0_sig0_rel_vs_plain": abs(sg_c0 - plain_pass0_sig) / plain_pass0_sig,
+                     "pass0_speedup_vs_plain": round(plain_pass0_s / t_c0, 3)
+                     if sg_c0 <= plain_pass0_sig * (1.0 + 1e-6) else None}
+        return dict(res, **pass0, **{
+            "lm_iters_per_s": lm_its,
             "time_to_converged_s": round(cttc, 4),
             "time_to_reach_plain_sigma0_s": None if reach is None else round(reach, 4),
             "speedup_vs_plain": None if reach is None else round(ttc / reach, 3),
@@ -227,12 +259,17 @@ def run_bench(args):
 
     # candidate forms, each a whole solve; the chosen one is the fastest to
     # reach the plain form's final sigma0 (x (1 + 1e-6))
-    chosen = learned = None
+    chosen = learned = chosen_pass0 = None
     forms = {}
     skipped = {}
     if not args.no_chosen:
         base = dict(robust=1, k2=1, k4=1, f=1, threads=1)
         cands = [
+            ("fp32_whitened", capi.make_flags(**base, precision=2, schur=2),
+             "precision=2: whitened Jacobians L^T Bc, L^T Bp rounded to FP32 once; V, g, S_kk, b and the "
+             "matrix-free PCG product formed from them in FP64; FP64 residuals, PCG and cost"),
+            ("fp64_whitened", capi.make_flags(**base, precision=3, schur=2),
+             "precision=3: the whitened Jacobian records in FP64 (the plain arithmetic in another association)"),
             ("fp32_records", capi.make_flags(**base, precision=1),
              "precision=1: FP32 Jacobians and W, Y = W V^-1 records; FP64 residuals, sums, S, PCG, cost"),
             ("fp32_matrix_free", capi.make_flags(**base, precision=1, schur=1),
@@ -265,6 +302,12 @@ def run_bench(args):
         if reached:
             t_best, k_best = min(reached)
             chosen = dict(forms[k_best], name=k_best)
+        reached0 = [(v["pass0_s"], k) for k, v in forms.items() if v.get("pass0_speedup_vs_plain") is not None]
+        if reached0:
+            t_best, k_best = min(reached0)
+            chosen_pass0 = {"name": k_best, "pass0_s": t_best, "plain_pass0_s": round(plain_pass0_s, 4),
+                            "speedup_vs_plain": forms[k_best]["pass0_speedup_vs_plain"],
+                            "pass0_sig0_rel_vs_plain": forms[k_best]["pass0_sig0_rel_vs_plain"]}
         learned = forms.get("fp64_learned")
 
     # roofline: the same steps again with HIP events around every launch
@@ -296,8 +339,9 @@ def run_bench(args):
             traffic = json.load(fh).get("bytes_per_launch")
 
     def form_summary(v):
-        keep = ("time_to_converged_s", "time_to_reach_plain_sigma0_s", "speedup_vs_plain", "itt", "pcg_iters",
-                "sigma0_final", "stopped")
+        keep = ("lm_iters_per_s", "time_to_converged_s", "time_to_reach_plain_sigma0_s", "speedup_vs_plain", "itt",
+                "pcg_iters", "sigma0_final", "rms_px", "pass0_s", "pass0_speedup_vs_plain", "pass0_sig0_rel_vs_plain",
+                "stopped")
         d = {kk: v[kk] for kk in keep if kk in v}
         if "err_vs_plain" in v:
             d["sigma0_rel_vs_plain"] = v["err_vs_plain"]["sigma0_rel"]
@@ -332,6 +376,7 @@ def run_bench(args):
         "sigma0_final": st["sigma_out"][1],
         "rms_px": st["rms_px"],
         "chosen": chosen,
+        "chosen_to_plain_pass0_sigma0": chosen_pass0,
         "learned_fp64": learned,
         "forms": {k: form_summary(v) for k, v in forms.items()},
         "forms_skipped": skipped,

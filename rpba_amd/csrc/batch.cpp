@@ -515,7 +515,7 @@ static int encode_cameras(rpba_batch *c)
 static LinArgs lin_args(rpba_batch *c)
 {
     LinArgs a;
-    a.nc = c->nc; a.npts = c->ds.npts; a.ntr = c->ds.ntr; a.npgrp = c->ds.npgrp; a.pgrp = c->ds.pgrp;
+    a.nc = c->nc; a.npts = c->ds.npts; a.ntr = c->ds.ntr; a.npgrp = c->ds.npgrp; a.pgrp = c->ds.pgrp; a.pg4 = c->ds.pg4;
     a.defdiff = c->defdiff; a.mult1 = 1. + c->mult;
     a.cam = c->cam();
     a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.tpt = c->ds.tpt; a.camoff = c->ds.camoff; a.camq = c->ds.camq;
@@ -533,7 +533,7 @@ static StepArgs step_args(rpba_batch *c, const double *sol)
 {
     StepArgs a;
     a.nc = c->nc; a.npts = c->ds.npts; a.cam = c->cam();
-    a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.pgrp = c->ds.pgrp; a.npgrp = c->ds.npgrp;
+    a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.pgrp = c->ds.pgrp; a.npgrp = c->ds.npgrp; a.pg4 = c->ds.pg4;
     a.Wrec = c->d_Wrec.p; a.rec32 = c->rec32;
     a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.sol = sol; a.X = c->d_X.p; a.Xn = c->d_Xn.p;
     a.S = c->d_pcg.p;

@@ -88,6 +88,7 @@ struct LinArgs {
     int jform = 0;
     void *Jpm = nullptr, *Jcm = nullptr;
     double *dmp = nullptr;
+    const int4 *pg4 = nullptr;  // per point group {p0, p1, e0, e1}
 };
 
 struct SchurArgs {
@@ -142,6 +143,8 @@ struct PcgArgs {
     int jform = 0;
     const void *Jpm = nullptr, *Jcm = nullptr;
     const double *dmp = nullptr, *Vinv = nullptr;
+    const int4 *pg4 = nullptr;  // per point group {p0, p1, e0, e1}
+    const int4 *ch4 = nullptr;  // per chunk {first entry, entries, image, image's first chunk}
 };
 
 struct StepArgs {
@@ -161,6 +164,7 @@ struct StepArgs {
     // whitened Jacobian records (point-major), or null
     int jform = 0;
     const void *Jpm = nullptr;
+    const int4 *pg4 = nullptr;  // per point group {p0, p1, e0, e1}
 };
 
 struct CostArgs {

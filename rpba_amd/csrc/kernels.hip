@@ -133,6 +133,15 @@ __device__ __forceinline__ void jac_obs(const CamV &c, const Ev &e, unsigned mas
         if (!(mask & (1u << j))) Bc[0][j] = Bc[1][j] = zero;
 }
 
+// point group c: {first point, end point, first entry, end entry}, one load
+// when the per-group table is there
+__device__ __forceinline__ int4 point_group(const int4 *pg4, const int *pgrp, const int *toff, int c)
+{
+    if (pg4) return pg4[c];
+    const int p0 = pgrp[c], p1 = pgrp[c + 1];
+    return make_int4(p0, p1, toff[p0], toff[p1]);
+}
+
 // 3x3 inverse by cofactors (adjugate / determinant)
 __device__ __forceinline__ void inv3(const double m[9], double r[9])
 {
@@ -339,9 +348,9 @@ __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
 {
     __shared__ double sv[256][12];
     const int t = threadIdx.x, c = blockIdx.x;
-    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    const int4 grp = point_group(a.pg4, a.pgrp, a.toff, c);
+    const int p0 = grp.x, p1 = grp.y, e0 = grp.z, e1 = grp.w;
     if (p0 >= p1) return;
-    const int e0 = a.toff[p0], e1 = a.toff[p1];
     const int np = p1 - p0;
     const int i = p0 + t;
     int pb = 0, pe = 0;
@@ -1044,9 +1053,9 @@ __global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
     if (a.S[a.j & 1].done) return;
     __shared__ double su[256][3];
     const int t = threadIdx.x, c = blockIdx.x;
-    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    const int4 grp = point_group(a.pg4, a.pgrp, a.toff, c);
+    const int p0 = grp.x, p1 = grp.y, e0 = grp.z, e1 = grp.w;
     if (p0 >= p1) return;
-    const int e0 = a.toff[p0], e1 = a.toff[p1];
     const int np = p1 - p0;  // at most kPtGroup / 2 + 1: a point has at least two entries
     int pb = 0, pe = 0;
     if (t < np) {
@@ -1149,8 +1158,16 @@ __global__ __launch_bounds__(64 * kMfChunkWaves) void k_mf_chunks(PcgArgs a)
     const int ch = blockIdx.x * kMfChunkWaves + wv;
     if (ch >= a.nch) return;  // the whole wavefront; no workgroup barrier follows
     const int lane = threadIdx.x & 63;
-    const int b = a.ch_start[ch], n = a.ch_start[ch + 1] - b;
-    const int k = a.ch_cam[ch];
+    int b, n, k, first;
+    if (a.ch4) {
+        const int4 m = a.ch4[ch];
+        b = m.x; n = m.y; k = m.z; first = m.w;
+    } else {
+        b = a.ch_start[ch];
+        n = a.ch_start[ch + 1] - b;
+        k = a.ch_cam[ch];
+        first = ch == a.ch_off[k];
+    }
     const R *Yr = (const R *)a.Yrec;
     // the entry's point, then the chunk's Y records (camera-major,
     // consecutive, read as whole lines) and the point's t behind them
@@ -1182,7 +1199,7 @@ __global__ __launch_bounds__(64 * kMfChunkWaves) void k_mf_chunks(PcgArgs a)
     double share = 0.;
 #pragma unroll
     for (int r = 0; r < 9; ++r) share -= dk[r] * v[r];
-    if (ch == a.ch_off[k]) {
+    if (first) {
         // (S_kk + D_k) d_k, row lane
         __shared__ double ed[kMfChunkWaves][9];
         if (lane < 9) {
@@ -1500,9 +1517,9 @@ __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
     // subtracts its entries from g_p in entry order
     __shared__ double su[256][3];
     const int t = threadIdx.x, c = blockIdx.x;
-    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    const int4 grp = point_group(a.pg4, a.pgrp, a.toff, c);
+    const int p0 = grp.x, p1 = grp.y, e0 = grp.z, e1 = grp.w;
     if (p0 >= p1) return;
-    const int e0 = a.toff[p0], e1 = a.toff[p1];
     const int np = p1 - p0;
     const int i = p0 + t;
     int pb = 0, pe = 0;
@@ -2137,9 +2154,9 @@ __global__ __launch_bounds__(256) void k_lin_points_j(LinArgs a)
     __shared__ u32x4 lds[SV > ST ? SV : ST];
     double(*sv)[12] = reinterpret_cast<double(*)[12]>(lds);
     const int t = threadIdx.x, c = blockIdx.x, lane = t & 63, wv = t >> 6;
-    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    const int4 grp = a.pg4[c];
+    const int p0 = grp.x, p1 = grp.y, e0 = grp.z, e1 = grp.w;
     if (p0 >= p1) return;
-    const int e0 = a.toff[p0], e1 = a.toff[p1];
     const int np = p1 - p0;
     const int i = p0 + t;
     int pb = 0, pe = 0;
@@ -2320,14 +2337,21 @@ __global__ __launch_bounds__(256) void k_mf_points_j(PcgArgs a)
     __shared__ double su[256][3];
     __shared__ u32x4 stg[4][JSpan<R>::LDS];
     const int t = threadIdx.x, c = blockIdx.x, lane = t & 63, wv = t >> 6;
-    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    const int4 g = a.pg4[c];
+    const int p0 = g.x, p1 = g.y, e0 = g.z, e1 = g.w;
     if (p0 >= p1) return;
-    const int e0 = a.toff[p0], e1 = a.toff[p1];
     const int np = p1 - p0;
+    // the point's entry range (and with FP32 records its V^-1, which FP64
+    // records have no registers to spare for), in flight with the records
     int pb = 0, pe = 0;
-    if (t < np) {
-        pb = a.toff[p0 + t];
-        pe = a.toff[p0 + t + 1];
+    double Vi[9];
+    const int pt = min(p0 + t, p1 - 1);
+    pb = a.toff[pt];
+    pe = a.toff[pt + 1];
+    if constexpr (sizeof(R) == 4) {
+        const double *vp = a.Vinv + 9 * (size_t)pt;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) Vi[r] = vp[r];
     }
     double t0 = 0., t1 = 0., t2 = 0.;
     for (int base = e0; base < e1; base += 256) {
@@ -2365,7 +2389,11 @@ __global__ __launch_bounds__(256) void k_mf_points_j(PcgArgs a)
         __syncthreads();
     }
     if (t < np) {
-        const double *Vi = a.Vinv + 9 * (size_t)(p0 + t);
+        if constexpr (sizeof(R) == 8) {
+            const double *vp = a.Vinv + 9 * (size_t)pt;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) Vi[r] = vp[r];
+        }
         double *o = a.tc + 3 * (size_t)(p0 + t);
 #pragma unroll
         for (int r = 0; r < 3; ++r) o[r] = Vi[3 * r] * t0 + Vi[3 * r + 1] * t1 + Vi[3 * r + 2] * t2;
@@ -2386,8 +2414,8 @@ __global__ __launch_bounds__(64 * kMfChunkWaves) void k_mf_chunks_j(PcgArgs a)
     const int ch = blockIdx.x * kMfChunkWaves + wv;
     if (ch >= a.nch) return;  // the whole wavefront; no workgroup barrier follows
     const int lane = threadIdx.x & 63;
-    const int b = a.ch_start[ch], n = a.ch_start[ch + 1] - b;
-    const int k = a.ch_cam[ch];
+    const int4 m = a.ch4[ch];
+    const int b = m.x, n = m.y, k = m.z;
     const int pq = a.cq_pt[b + min(lane, n - 1)];  // a chunk has at least one entry
     __shared__ u32x4 stg[kMfChunkWaves][JSpan<R>::LDS];
     u32x4 raw[JSpan<R>::NP];
@@ -2427,7 +2455,7 @@ __global__ __launch_bounds__(64 * kMfChunkWaves) void k_mf_chunks_j(PcgArgs a)
     double share = 0.;
 #pragma unroll
     for (int r = 0; r < 9; ++r) share -= dv[r] * v[r];
-    if (ch == a.ch_off[k]) {
+    if (m.w) {
         double e = 0.;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
@@ -2448,9 +2476,9 @@ __global__ __launch_bounds__(256) void k_backsub_j(StepArgs a)
     __shared__ double su[256][3];
     __shared__ u32x4 stg[4][JSpan<R>::LDS];
     const int t = threadIdx.x, c = blockIdx.x, lane = t & 63, wv = t >> 6;
-    const int p0 = a.pgrp[c], p1 = a.pgrp[c + 1];
+    const int4 g = a.pg4[c];
+    const int p0 = g.x, p1 = g.y, e0 = g.z, e1 = g.w;
     if (p0 >= p1) return;
-    const int e0 = a.toff[p0], e1 = a.toff[p1];
     const int np = p1 - p0;
     const int i = p0 + t;
     int pb = 0, pe = 0;

@@ -600,12 +600,12 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     // DecodeCameras(XPMatrices): the trial cameras of the linearisation
     StepArgs s0;
     s0.nc = nc; s0.npts = (int)np; s0.npgrp = ds->npgrp; s0.cam = cp; s0.toff = ds->toff; s0.tcam = ds->tcam;
-    s0.pgrp = ds->pgrp; s0.Wrec = Wrec.p; s0.rec32 = 0; s0.Vinv = Vinv.p; s0.gp = gp.p; s0.sol = nullptr; s0.X = X.p;
+    s0.pgrp = ds->pgrp; s0.pg4 = ds->pg4; s0.Wrec = Wrec.p; s0.rec32 = 0; s0.Vinv = Vinv.p; s0.gp = gp.p; s0.sol = nullptr; s0.X = X.p;
     s0.Xn = Xn.p; s0.S = pcgS.p; s0.camflag = camflag.p;
     launch_decode(s0, st);
     // CalcBundleSep with multfact 1e-4 (rbacore.cpp:371-372, 1094-1195)
     LinArgs la;
-    la.nc = nc; la.npts = (int)np; la.ntr = (int)ntr; la.npgrp = ds->npgrp; la.pgrp = ds->pgrp;
+    la.nc = nc; la.npts = (int)np; la.ntr = (int)ntr; la.npgrp = ds->npgrp; la.pgrp = ds->pgrp; la.pg4 = ds->pg4;
     la.defdiff = defdiff; la.mult1 = 1. + 1.e-4; la.cam = cp;
     la.toff = ds->toff; la.tcam = ds->tcam; la.tpt = ds->tpt; la.camoff = ds->camoff; la.camq = ds->camq;
     la.ox = ds->ox; la.oy = ds->oy; la.wx = ds->w0x; la.wy = ds->w0y; la.wxy = ds->w0xy; la.X = X.p;

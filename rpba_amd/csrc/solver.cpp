@@ -383,7 +383,7 @@ static int upload_initial_cameras(rpba_ctx *c)
 static LinArgs lin_args(rpba_ctx *c)
 {
     LinArgs a;
-    a.nc = c->nc; a.npts = c->ds.npts; a.ntr = c->ds.ntr; a.npgrp = c->ds.npgrp; a.pgrp = c->ds.pgrp;
+    a.nc = c->nc; a.npts = c->ds.npts; a.ntr = c->ds.ntr; a.npgrp = c->ds.npgrp; a.pgrp = c->ds.pgrp; a.pg4 = c->ds.pg4;
     a.defdiff = c->defdiff; a.mult1 = 1. + c->mult;
     a.cam = c->cam();
     a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.tpt = c->ds.tpt; a.camoff = c->ds.camoff; a.camq = c->ds.camq;
@@ -432,6 +432,7 @@ static PcgArgs pcg_args(rpba_ctx *c)
     a.ch_cam = c->implicit ? c->d_chcam.p : nullptr;
     a.omega = c->fl.preconditioner >= 1 ? c->d_omega.p : nullptr;
     a.jform = c->jform; a.Jpm = c->d_Jpm.p; a.Jcm = c->d_Jcm.p; a.dmp = c->d_dmp.p; a.Vinv = c->d_Vinv.p;
+    a.pg4 = c->ds.pg4; a.ch4 = c->ds.ch4;
     return a;
 }
 
@@ -439,11 +440,11 @@ static StepArgs step_args(rpba_ctx *c, const double *sol)
 {
     StepArgs a;
     a.nc = c->nc; a.npts = c->ds.npts; a.cam = c->cam();
-    a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.pgrp = c->ds.pgrp; a.npgrp = c->ds.npgrp;
+    a.toff = c->ds.toff; a.tcam = c->ds.tcam; a.pgrp = c->ds.pgrp; a.npgrp = c->ds.npgrp; a.pg4 = c->ds.pg4;
     a.Wrec = c->d_Wrec.p; a.rec32 = c->rec32;
     a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.sol = sol; a.X = c->d_X.p; a.Xn = c->d_Xn.p;
     a.S = c->d_pcg.p;
-    a.jform = c->jform; a.Jpm = c->d_Jpm.p;
+    a.jform = c->jform; a.Jpm = c->d_Jpm.p; a.pg4 = c->ds.pg4;
     return a;
 }
 

@@ -39,12 +39,19 @@ struct DevStruct {
     int *pgrp = nullptr;
     size_t pgrp_cap = 0;
     int npgrp = 0;
+    // per point group {first point, end point, first entry, end entry}: one
+    // load gives a point kernel its group
+    int4 *pg4 = nullptr;
+    size_t pg4_cap = 0;
     // chunks of at most kChunk entries of one image over camq: ch_start (nch + 1
     // entry offsets), ch_off (nc + 1 first chunk of each image)
     static constexpr int kChunk = 64;
     int nch = 0;
     int *ch_start = nullptr, *ch_off = nullptr;
     size_t ch_start_cap = 0, ch_off_cap = 0;
+    // per chunk {first entry, entries, image, 1 if the image's first chunk}
+    int4 *ch4 = nullptr;
+    size_t ch4_cap = 0;
     // pairs (a << 32 | b), cam(a) < cam(b), grouped by block in row-major
     // block order; poff: pairs of block b; bk/bl: block (k, l); roff: blocks of
     // row k; coff/cblk: blocks of column l (ids, k ascending)

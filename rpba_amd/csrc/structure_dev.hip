@@ -153,11 +153,21 @@ __global__ void k_set_last(int *a, int idx, int v)
 
 static inline unsigned nb_(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+__global__ void k_pg4(int ng, const int *pgrp, const int *toff, int4 *pg4)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ng) return;
+    const int p0 = pgrp[c], p1 = pgrp[c + 1];
+    pg4[c] = make_int4(p0, p1, toff[p0], toff[p1]);
+}
+
 int DevStruct::point_groups(hipStream_t st)
 {
     npgrp = (ntr + kPtGroup - 1) / kPtGroup;
     if (grow(pgrp, pgrp_cap, (size_t)npgrp + 1)) return RPBA_E_NOMEM;
+    if (grow(pg4, pg4_cap, (size_t)std::max(npgrp, 1))) return RPBA_E_NOMEM;
     k_lower_bound<<<nb_(npgrp + 1, 256), 256, 0, st>>>(npgrp, (unsigned)kPtGroup, (const unsigned *)toff, npts, pgrp);
+    if (npgrp > 0) k_pg4<<<nb_(npgrp, 256), 256, 0, st>>>(npgrp, pgrp, toff, pg4);
     return RPBA_OK;
 }
 
@@ -186,12 +196,22 @@ int DevStruct::chunks_from_camoff(hipStream_t st, int *first_empty)
     cs.push_back(ntr);
     rc |= grow(ch_start, ch_start_cap, cs.size());
     rc |= grow(ch_off, ch_off_cap, cof.size());
+    rc |= grow(ch4, ch4_cap, (size_t)std::max(nch, 1));
     if (rc) return RPBA_E_NOMEM;
-    if (pinned(cs.size() + cof.size())) return RPBA_E_NOMEM;
+    if (pinned(cs.size() + cof.size() + 4 * (size_t)nch)) return RPBA_E_NOMEM;
     std::copy(cs.begin(), cs.end(), hpin);
     std::copy(cof.begin(), cof.end(), hpin + cs.size());
+    int *h4 = hpin + cs.size() + cof.size();
+    for (int k = 0; k < nc; ++k)
+        for (int c = cof[k]; c < cof[(size_t)k + 1]; ++c) {
+            h4[4 * (size_t)c] = cs[c];
+            h4[4 * (size_t)c + 1] = cs[(size_t)c + 1] - cs[c];
+            h4[4 * (size_t)c + 2] = k;
+            h4[4 * (size_t)c + 3] = c == cof[k] ? 1 : 0;
+        }
     DCHK(hipMemcpyAsync(ch_start, hpin, sizeof(int) * cs.size(), hipMemcpyHostToDevice, st));
     DCHK(hipMemcpyAsync(ch_off, hpin + cs.size(), sizeof(int) * cof.size(), hipMemcpyHostToDevice, st));
+    if (nch > 0) DCHK(hipMemcpyAsync(ch4, h4, sizeof(int4) * (size_t)nch, hipMemcpyHostToDevice, st));
     DCHK(hipStreamSynchronize(st));
     return RPBA_OK;
 }
@@ -967,7 +987,7 @@ void DevStruct::release()
         p = nullptr;
         c = 0;
     };
-    fr(camoff, camoff_cap); fr(camq, camq_cap); fr(camrank, camrank_cap); fr(pgrp, pgrp_cap); fr(ch_start, ch_start_cap); fr(ch_off, ch_off_cap); fr(u32a, u32a_cap); fr(i32a, i32a_cap); fr(i32b, i32b_cap);
+    fr(camoff, camoff_cap); fr(camq, camq_cap); fr(camrank, camrank_cap); fr(pgrp, pgrp_cap); fr(pg4, pg4_cap); fr(ch4, ch4_cap); fr(ch_start, ch_start_cap); fr(ch_off, ch_off_cap); fr(u32a, u32a_cap); fr(i32a, i32a_cap); fr(i32b, i32b_cap);
     fr(ppoff, ppoff_cap); fr(key_in, key_in_cap); fr(key_out, key_out_cap); fr(val_in, val_in_cap);
     fr(sslot, sslot_cap); fr(sperm, sperm_cap); fr(slot_slab, slot_slab_cap); fr(slab_bs, slab_bs_cap);
     fr(slab_l, slab_l_cap); fr(slab_mask, slab_mask_cap); fr(strip_s0, strip_s0_cap); fr(chunk_off, chunk_off_cap);
