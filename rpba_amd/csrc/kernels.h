@@ -81,7 +81,8 @@ struct LinArgs {
     // point, observation and weights), streamed by the camera kernels; null:
     // they gather through camq
     const int *cm_pt = nullptr;
-    const float *cm_ox = nullptr, *cm_oy = nullptr, *cm_wx = nullptr, *cm_wy = nullptr, *cm_wxy = nullptr;
+    const float *cm_ox = nullptr, *cm_oy = nullptr;
+    const float4 *cm_w = nullptr;  // (wx, wy, wxy, 0)
     // whitened Jacobian records (jform, precision 2): point-major and
     // camera-major copies, kJRec elements each (float when rec32), and the
     // damping of S_kk's diagonal per camera (9)
@@ -177,6 +178,10 @@ struct CostArgs {
     float *key;
     double *partial;
     double *terms = nullptr;  // per-entry FP64 cost terms (batched blocks), or null
+    // the robust key reduced by the track-length factor of the upper median,
+    // at the entry's camera-major slot (cm_key[camrank[q]]), or null
+    float *cm_key = nullptr;
+    const int *camrank = nullptr, *toff = nullptr;
 };
 
 struct RobustArgs {
@@ -189,6 +194,11 @@ struct RobustArgs {
     int *ipartial;
     // batched blocks (else null): iflag of the camera's block
     const int *iflag_blk = nullptr, *camblk = nullptr;
+    // camera-major copies (else null): the reduced keys the median reads in
+    // order, and the new weights written at the entry's camera-major slot
+    const float *cm_key = nullptr;
+    float4 *cm_w = nullptr;
+    const int *camrank = nullptr;
 };
 
 // Exteval over tracks in point-major CSR (toff[npts + 1], entries q):
@@ -221,7 +231,8 @@ int pcg_groups(int nc);
 struct CmTable {
     const int *camrank;
     int *pt;
-    float *ox, *oy, *wx, *wy, *wxy;
+    float *ox, *oy;
+    float4 *w;  // (wx, wy, wxy, 0)
     int *ch_cam;
 };
 void launch_cm_table(const LinArgs &a, const CmTable &t, bool full, hipStream_t st);

@@ -572,9 +572,10 @@ __global__ __launch_bounds__(256) void k_lin_cameras(LinArgs a)
             p = a.cm_pt[j];
             ox = a.cm_ox[j];
             oy = a.cm_oy[j];
-            w0 = a.cm_wx[j];
-            w1 = a.cm_wy[j];
-            w2 = a.cm_wxy[j];
+            const float4 wv = a.cm_w[j];
+            w0 = wv.x;
+            w1 = wv.y;
+            w2 = wv.z;
         } else {
             const int q = a.camq[j];
             p = a.tpt[q];
@@ -1141,9 +1142,7 @@ __global__ void k_cm_table(LinArgs a, CmTable t, int full)
             t.ox[j] = a.ox[q];
             t.oy[j] = a.oy[q];
         }
-        t.wx[j] = a.wx[q];
-        t.wy[j] = a.wy[q];
-        t.wxy[j] = a.wxy[q];
+        t.w[j] = make_float4(a.wx[q], a.wy[q], a.wxy[q], 0.f);
     }
     if (full && q < a.nch) t.ch_cam[q] = a.tcam[a.camq[a.ch_start[q]]];
 }
@@ -1675,6 +1674,15 @@ __global__ __launch_bounds__(64) void k_decode(StepArgs a)
 
 // per-entry residual at the trial state: float robust key (sig0rmv) and the
 // FP64 cost term; per-workgroup partial sums
+// the upper median's track-length factor of a key (Reweight2D rbacore.cpp:831-835)
+__device__ __forceinline__ float redfac_of(int t)
+{
+    int pxs = t * 2;
+    if (pxs < 4) pxs = 4;
+    const float rf = (float)pxs / (float)(pxs - 3);
+    return rf * rf;
+}
+
 __global__ __launch_bounds__(256) void k_cost(CostArgs a)
 {
     __shared__ double sh[4];
@@ -1686,13 +1694,13 @@ __global__ __launch_bounds__(256) void k_cost(CostArgs a)
         load_cam(a.cam, k, c);
         Ev ev;
         eval_obs(c, a.X[3 * p], a.X[3 * p + 1], a.X[3 * p + 2], a.ox[q], a.oy[q], a.defdiff, ev);
+        float key = 1.e20f;
         if (ev.nan) {
-            a.key[q] = 1.e20f;
+            a.key[q] = key;
         } else {
             const float W0 = a.wx[q], W1 = a.wy[q], W2 = a.wxy[q];
             const float d1f = (float)ev.d1, d2f = (float)ev.d2;
             const float d11 = d1f * d1f, d12 = 2.f * d1f * d2f, d22 = d2f * d2f;
-            float key;
             if (a.robust)
                 key = d11 * a.w0x[q] + d12 * a.w0xy[q] + d22 * a.w0y[q];
             else
@@ -1700,6 +1708,8 @@ __global__ __launch_bounds__(256) void k_cost(CostArgs a)
             if (a.write_key) a.key[q] = key;
             cst = ev.d1 * ev.d1 * (double)W0 + 2. * ev.d1 * ev.d2 * (double)W2 + ev.d2 * ev.d2 * (double)W1;
         }
+        // the key as the upper median reads it, at the camera-major slot
+        if (a.cm_key && (ev.nan || a.write_key)) a.cm_key[a.camrank[q]] = key * redfac_of(a.toff[p + 1] - a.toff[p]);
         if (a.terms) a.terms[q] = cst;
     }
     const double s = block_sum(cst, sh);
@@ -1729,14 +1739,6 @@ __global__ __launch_bounds__(1024) void k_final_sum(const double *partial, int n
 // ---------------------------------------------------------------------------
 // robust reweighting (Reweight2D rbacore.cpp:823-887)
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ float redfac_of(int t)
-{
-    int pxs = t * 2;
-    if (pxs < 4) pxs = 4;
-    const float rf = (float)pxs / (float)(pxs - 3);
-    return rf * rf;
-}
 
 __device__ __forceinline__ unsigned ord_bits(float f)
 {
@@ -1770,9 +1772,14 @@ __global__ __launch_bounds__(256) void k_median(RobustArgs a)
             if (staged && shift != 24) {
                 u = ukey[j - b];
             } else {
-                const int q = a.camq[j];
-                const int p = a.tpt[q];
-                const float kv = a.key[q] * redfac_of(a.toff[p + 1] - a.toff[p]);
+                float kv;
+                if (a.cm_key) {
+                    kv = a.cm_key[j];
+                } else {
+                    const int q = a.camq[j];
+                    const int p = a.tpt[q];
+                    kv = a.key[q] * redfac_of(a.toff[p + 1] - a.toff[p]);
+                }
                 u = ord_bits(kv);
                 if (staged) ukey[j - b] = u;
             }
@@ -1828,6 +1835,7 @@ __global__ __launch_bounds__(256) void k_reweight(RobustArgs a)
         a.wx[q] = W0;
         a.wy[q] = W1;
         a.wxy[q] = W2;
+        if (a.cm_w) a.cm_w[a.camrank[q]] = make_float4(W0, W1, W2, 0.f);
         ws = (double)W0 + (double)W1;
     }
     const double s = block_sum(ws, sh);
@@ -2125,7 +2133,7 @@ __device__ __forceinline__ void jspan_transpose(const u32x4 raw[JSpan<R>::NP], i
     constexpr int NP = JSpan<R>::NP, NPASS = JSpan<R>::NPASS, IPP = NP / NPASS, RP = 64 / NPASS;
     u32x4 pc[NP];
 #pragma unroll
-    for (int p = 0; p < NP; ++p) pc[p] = raw[p];  // lanes past nq keep their own (unused) pieces
+    for (int p = 0; p < NP; ++p) pc[p] = u32x4{0u, 0u, 0u, 0u};  // lanes past nq get zeros (unused)
 #pragma unroll
     for (int h = 0; h < NPASS; ++h) {
 #pragma unroll
@@ -2261,10 +2269,11 @@ __global__ __launch_bounds__(256) void k_lin_cameras_j(LinArgs a)
 #pragma unroll
             for (int m = 0; m < 3; ++m) gpv[m] = gq[m];
             Ev ev;
+            const float4 wv = a.cm_w[j];
             eval_obs(c, Xp[0], Xp[1], Xp[2], a.cm_ox[j], a.cm_oy[j], a.defdiff, ev);
             double Bc[2][9], Bp[2][3];
             jac_obs<double>(c, ev, mask, Bc, Bp);
-            const Whitener w = whitener(a.cm_wx[j], a.cm_wy[j], a.cm_wxy[j]);
+            const Whitener w = whitener(wv.x, wv.y, wv.z);
             R J[kJRec];
             whiten_jac<R>(Bc, Bp, w, J);
             const double r0 = w.l11 * ev.d1 + w.l21 * ev.d2, r1 = w.l22 * ev.d2;

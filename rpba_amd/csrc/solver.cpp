@@ -83,7 +83,8 @@ struct rpba_ctx {
     // camera-major entry table (point, observation, weights; chunk -> image),
     // rebuilt when the structure (cm_full) or the weights (cm_w) change
     DBuf<int> d_cmpt, d_chcam;
-    DBuf<float> d_cmox, d_cmoy, d_cmwx, d_cmwy, d_cmwxy;
+    DBuf<float> d_cmox, d_cmoy, d_cmkey;
+    DBuf<float4> d_cmw;
     bool cm_full = true, cm_w = true;
     DBuf<int> d_ipartial, d_iscal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
@@ -265,9 +266,8 @@ static int build_structure(rpba_ctx *c)
     rc |= c->d_cmpt.alloc(std::max<size_t>(ntr, 1));
     rc |= c->d_cmox.alloc(std::max<size_t>(ntr, 1));
     rc |= c->d_cmoy.alloc(std::max<size_t>(ntr, 1));
-    rc |= c->d_cmwx.alloc(std::max<size_t>(ntr, 1));
-    rc |= c->d_cmwy.alloc(std::max<size_t>(ntr, 1));
-    rc |= c->d_cmwxy.alloc(std::max<size_t>(ntr, 1));
+    rc |= c->d_cmw.alloc(std::max<size_t>(ntr, 1));
+    rc |= c->d_cmkey.alloc(std::max<size_t>(ntr, 1));
     rc |= c->d_chcam.alloc((size_t)std::max(c->ds.nch, 1));
     c->cm_full = true;
     if (c->implicit) {
@@ -397,7 +397,7 @@ static LinArgs lin_args(rpba_ctx *c)
     // the camera-major entry table (bundle_step refreshes it before the
     // camera kernels run when the structure or the weights changed)
     a.cm_pt = c->d_cmpt.p; a.cm_ox = c->d_cmox.p; a.cm_oy = c->d_cmoy.p;
-    a.cm_wx = c->d_cmwx.p; a.cm_wy = c->d_cmwy.p; a.cm_wxy = c->d_cmwxy.p;
+    a.cm_w = c->d_cmw.p;
     a.jform = c->jform; a.Jpm = c->d_Jpm.p; a.Jcm = c->d_Jcm.p; a.dmp = c->d_dmp.p;
     return a;
 }
@@ -456,6 +456,10 @@ static CostArgs cost_args(rpba_ctx *c)
     a.ox = c->ds.ox; a.oy = c->ds.oy; a.wx = c->d_wx.p; a.wy = c->d_wy.p; a.wxy = c->d_wxy.p;
     a.w0x = c->ds.w0x; a.w0y = c->ds.w0y; a.w0xy = c->ds.w0xy;
     a.X = c->d_Xn.p; a.key = c->d_key.p; a.partial = c->d_partial.p;
+    // the reduced keys in camera-major order feed the robust reweighting of
+    // pass 0 only
+    a.cm_key = (c->it == 0 && c->fl.robust) ? c->d_cmkey.p : nullptr;
+    a.camrank = c->ds.camrank; a.toff = c->ds.toff;
     return a;
 }
 
@@ -492,7 +496,7 @@ static int bundle_step(rpba_ctx *c)
     sa.nblk = c->ds.nblk; sa.poff = c->ds.poff; sa.pab = c->ds.pab;
     sa.Yrec = c->d_Yrec.p; sa.Wrec = c->d_Wrec.p; sa.Su = c->d_Su.p; sa.sm = c->ds.sumap(); sa.sperm = c->ds.sperm; sa.nslab = c->ds.nslab; sa.slab_bs = c->ds.slab_bs; sa.npairs = c->ds.npairs; sa.rec32 = c->rec32;
     if (c->cm_full || c->cm_w) {
-        CmTable ct{c->ds.camrank, c->d_cmpt.p, c->d_cmox.p, c->d_cmoy.p, c->d_cmwx.p, c->d_cmwy.p, c->d_cmwxy.p, c->d_chcam.p};
+        CmTable ct{c->ds.camrank, c->d_cmpt.p, c->d_cmox.p, c->d_cmoy.p, c->d_cmw.p, c->d_chcam.p};
         c->timed("cm_table", [&] { launch_cm_table(la, ct, c->cm_full, c->st); });
         c->cm_full = c->cm_w = false;
     }
@@ -838,8 +842,10 @@ static int lm_iteration(rpba_ctx *c)
         ra.key = c->d_key.p; ra.w0x = c->ds.w0x; ra.w0y = c->ds.w0y; ra.w0xy = c->ds.w0xy;
         ra.s00 = c->d_s00.p; ra.wx = c->d_wx.p; ra.wy = c->d_wy.p; ra.wxy = c->d_wxy.p;
         ra.flag = c->d_flag.p; ra.partial = c->d_partial.p; ra.ipartial = c->d_ipartial.p;
+        // the median reads the keys the last cost wrote in camera-major
+        // order; the new weights go to the camera-major table as well
+        ra.cm_key = c->d_cmkey.p; ra.cm_w = c->d_cmw.p; ra.camrank = c->ds.camrank;
         c->timed("reweight", [&] { launch_reweight(ra, c->d_scal.p + 1, c->d_iscal.p, c->st); });
-        c->cm_w = true;
     }
     int rc = bundle_step(c);
     if (rc) return rc;
@@ -1387,7 +1393,7 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_dpart.release();
     c->d_t.release();
     c->d_cmpt.release(); c->d_chcam.release();
-    c->d_cmox.release(); c->d_cmoy.release(); c->d_cmwx.release(); c->d_cmwy.release(); c->d_cmwxy.release();
+    c->d_cmox.release(); c->d_cmoy.release(); c->d_cmw.release(); c->d_cmkey.release();
     c->d_cpart.release();
     c->d_Dk.release();
     c->d_Jpm.release();
