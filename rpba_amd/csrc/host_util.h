@@ -10,6 +10,10 @@
 #include <cmath>
 #include <cstdio>
 #include <string>
+#include <thread>
+#include <vector>
+
+#include <sched.h>
 
 #include "../../include/rpba.h"
 
@@ -59,6 +63,35 @@ struct Timed {
     int pcg_j;   // launch index inside a PCG solve, or -1
     bool skip;   // a PCG launch after the stopping rule fired (passes the state on only)
 };
+
+// host threads for the bookkeeping loops: the processors of the calling
+// process (its CPU affinity), at most 16
+inline int host_threads()
+{
+    cpu_set_t cs;
+    CPU_ZERO(&cs);
+    int n = 0;
+    if (sched_getaffinity(0, sizeof cs, &cs) == 0) n = CPU_COUNT(&cs);
+    if (n <= 0) n = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(n, 16));
+}
+
+// f(begin, end, r) over nr contiguous ranges r of [0, n) (range r is the r-th
+// of nr equal cuts), on nr threads; the ranges depend on n and nr only, so a
+// loop whose ranges write disjoint outputs gives the same result as one thread
+template <class F>
+inline void parallel_ranges(long long n, int nr, F f)
+{
+    if (nr <= 1 || n < 4096) {
+        for (int r = 0; r < nr; ++r) f(n * r / nr, n * (r + 1) / nr, r);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve((size_t)nr - 1);
+    for (int r = 1; r < nr; ++r) th.emplace_back([=, &f] { f(n * r / nr, n * (r + 1) / nr, r); });
+    f(0, n / nr, 0);
+    for (std::thread &t : th) t.join();
+}
 
 inline double cpu_seconds()
 {

@@ -279,12 +279,48 @@ struct PartSolver {
     int64_t pcg_total = 0;
     PhaseClock clk;
     int cur_iter = 0;
+    int nthr = host_threads();  // host threads of the bookkeeping loops (per-point ranges)
 
     void logs(const char *s)
     {
         if (log) log(s, user);
     }
     std::vector<int> ecam, eslot;  // per entry: image, slot (entry_index)
+    // the union of the parts and its device buffers, kept from one outer
+    // iteration to the next (their room only grows)
+    std::unique_ptr<DevStruct> uds;
+    struct UnionBufs {
+        DBuf<double> Wrec, Yrec, Vinv, gp, Sd, Su, bv, Dd, rowpart, colpart, dpart, Cinv, x, best, r, dv, Ad, h, h1, X, Xn,
+            Sdf, ev, c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist, gc_c;
+        DBuf<float> gc_w;
+        DBuf<unsigned> c_mask;
+        DBuf<unsigned char> camflag;
+        DBuf<int> gc_of, d_boff, d_cblk, d_ptblk, d_nan, d_ndone;
+        DBuf<SegScal> segS;
+        DBuf<PcgScalars> pcgS;
+        void release()
+        {
+            for (DBuf<double> *b : {&Wrec, &Yrec, &Vinv, &gp, &Sd, &Su, &bv, &Dd, &rowpart, &colpart, &dpart, &Cinv, &x, &best, &r,
+                                    &dv, &Ad, &h, &h1, &X, &Xn, &Sdf, &ev, &c_Rb, &c_Cb, &c_Rn, &c_Cn, &c_In, &c_th, &c_thn,
+                                    &c_Pst, &c_Ist, &gc_c})
+                b->release();
+            gc_w.release();
+            c_mask.release();
+            camflag.release();
+            for (DBuf<int> *b : {&gc_of, &d_boff, &d_cblk, &d_ptblk, &d_nan, &d_ndone}) b->release();
+            segS.release();
+            pcgS.release();
+        }
+    } ub;
+    Structure us;                                 // the union's host structure
+    std::vector<int> up, ptblk, gcp_of, scnt;     // slot -> union point; union point -> block, prior; entries per slot
+    std::vector<double> Xu, gcc;                  // union points' X; the priors' centres
+    std::vector<float> w0, gcw;                   // union entries' weights; the priors' weights
+    ~PartSolver()
+    {
+        ub.release();
+        if (uds) uds->release();
+    }
     int point_pass(int robustmode, int weightonly, int wxyflag, std::vector<double> &sg, std::vector<int> &rd,
                    std::vector<float> *keys);
     int block_step(int iter, const int addf[3], int &pcg_it);
@@ -343,17 +379,19 @@ void PartSolver::entry_index()
 {
     ecam.resize(trk.size());
     eslot.resize(trk.size());
-    for (int i = 0; i < pxs; ++i) {
-        const long long b = base[i], s0 = soff[i];
-        for (int q = 0; q < len[i]; ++q) {
-            const int c = io->obs_cam[trk[(size_t)(b + q)]];
-            ecam[(size_t)(b + q)] = c;
-            const int p = part[c];
-            long long s = s0;
-            while (spart[(size_t)s] != p) ++s;  // the slots hold every part of the initial track
-            eslot[(size_t)(b + q)] = (int)s;
+    parallel_ranges(pxs, nthr, [&](long long i0, long long i1, int) {
+        for (long long i = i0; i < i1; ++i) {
+            const long long b = base[i], s0 = soff[i];
+            for (int q = 0; q < len[i]; ++q) {
+                const int c = io->obs_cam[trk[(size_t)(b + q)]];
+                ecam[(size_t)(b + q)] = c;
+                const int p = part[c];
+                long long s = s0;
+                while (spart[(size_t)s] != p) ++s;  // the slots hold every part of the initial track
+                eslot[(size_t)(b + q)] = (int)s;
+            }
         }
-    }
+    });
 }
 
 int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
@@ -363,36 +401,54 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     const size_t NS = spart.size();
     entry_index();
     // entries of each point in each of its parts
-    std::vector<int> scnt(NS, 0);
-    for (int i = 0; i < pxs; ++i)
-        for (int q = 0; q < len[i]; ++q) scnt[(size_t)eslot[(size_t)(base[i] + q)]]++;
+    // (per-point loops over ranges of points: a point's slots and entries are
+    // its own; per-part and per-image counts are summed over the ranges)
+    const int nr = nthr;
+    scnt.assign(NS, 0);
+    parallel_ranges(pxs, nr, [&](long long i0, long long i1, int) {
+        for (long long i = i0; i < i1; ++i)
+            for (int q = 0; q < len[i]; ++q) scnt[(size_t)eslot[(size_t)(base[i] + q)]]++;
+    });
     // PointXXvIn with GCP information (rbacore.cpp:1949-1962): a point stays
     // in a part with two entries there, or one if its GCP number is > 0;
     // validity of each part (an image without entries, or a negative
     // redundancy, ends its Adjust before the step: rbacore.cpp:401-430, 466-476)
     std::vector<char> valid((size_t)P, 1), lowred((size_t)P, 0);
     std::vector<int> npk((size_t)P, 0), ntk((size_t)P, 0), ngk((size_t)P, 0);
-    for (int i = 0; i < pxs; ++i) {
-        if (!len[i]) continue;
-        const bool zok = std::fabs(XXd[3 * (size_t)i + 2]) > 1.e-8;
-        for (long long sl = soff[i]; sl < soff[(size_t)i + 1]; ++sl) {
-            const int c = scnt[(size_t)sl];
-            if (!c) continue;
-            const bool keep = zok && (c >= 2 || (gcpinfo[i] > 0 && c > 0));
-            bkept[(size_t)sl] = keep ? 1 : 0;
-            if (!keep) continue;
-            const int p = spart[(size_t)sl];
-            npk[p]++;
-            ntk[p] += c;
-            if (gcpinfo[i] > -1) ngk[p]++;
+    std::vector<int> rcnt((size_t)nr * (3 * (size_t)P + (size_t)N), 0);  // per range: npk, ntk, ngk, imgcnt
+    parallel_ranges(pxs, nr, [&](long long i0, long long i1, int r) {
+        int *rp = rcnt.data() + (size_t)r * (3 * (size_t)P + (size_t)N);
+        int *rimg = rp + 3 * (size_t)P;
+        for (long long i = i0; i < i1; ++i) {
+            if (!len[i]) continue;
+            const bool zok = std::fabs(XXd[3 * (size_t)i + 2]) > 1.e-8;
+            for (long long sl = soff[i]; sl < soff[(size_t)i + 1]; ++sl) {
+                const int c = scnt[(size_t)sl];
+                if (!c) continue;
+                const bool keep = zok && (c >= 2 || (gcpinfo[i] > 0 && c > 0));
+                bkept[(size_t)sl] = keep ? 1 : 0;
+                if (!keep) continue;
+                const int p = spart[(size_t)sl];
+                rp[p]++;
+                rp[P + p] += c;
+                if (gcpinfo[i] > -1) rp[2 * P + p]++;
+            }
+            for (int q = 0; q < len[i]; ++q) {
+                const size_t e = (size_t)(base[i] + q);
+                if (bkept[(size_t)eslot[e]]) rimg[ecam[e]]++;
+            }
         }
-    }
+    });
     std::vector<int> imgcnt((size_t)N, 0);
-    for (int i = 0; i < pxs; ++i)
-        for (int q = 0; q < len[i]; ++q) {
-            const size_t e = (size_t)(base[i] + q);
-            if (bkept[(size_t)eslot[e]]) imgcnt[ecam[e]]++;
+    for (int r = 0; r < nr; ++r) {
+        const int *rp = rcnt.data() + (size_t)r * (3 * (size_t)P + (size_t)N);
+        for (int p = 0; p < P; ++p) {
+            npk[p] += rp[p];
+            ntk[p] += rp[P + p];
+            ngk[p] += rp[2 * P + p];
         }
+        for (int k = 0; k < N; ++k) imgcnt[k] += rp[3 * P + k];
+    }
     for (int p = 0; p < P; ++p) {
         for (int k = pcnt[p]; k < pcnt[(size_t)p + 1]; ++k)
             if (imgcnt[pimg[k]] == 0) valid[p] = 0;
@@ -435,72 +491,131 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     clk.mark(st, "block lists and validity", cur_iter);
     if (nb == 0) return RPBA_OK;
     const int nup = (int)uoff[P];
-    std::vector<int> up(NS, -1);
-    Structure us;
+    up.assign(NS, -1);
     us.nc = nc;
     us.npts = nup;
-    us.nrpv.assign((size_t)nup, 0);
-    us.toff.assign((size_t)nup + 1, 0);
-    std::vector<double> Xu(3 * (size_t)std::max(nup, 1), 0.);
-    std::vector<int> ptblk((size_t)std::max(nup, 1), 0), gcp_of((size_t)std::max(nup, 1), -1);
+    // every union point's nrpv, toff, Xu and ptblk are written below
+    // (resize, no fill); gcp_of takes -1 below for the points without a prior
+    us.nrpv.resize((size_t)nup);
+    us.toff.resize((size_t)nup + 1);
+    us.toff[0] = 0;
+    Xu.resize(3 * (size_t)std::max(nup, 1));
+    ptblk.resize((size_t)std::max(nup, 1));
+    gcp_of.resize((size_t)std::max(nup, 1));
     {
-        std::vector<long long> w(uoff.begin(), uoff.end() - 1);
-        for (int i = 0; i < pxs; ++i) {
-            if (!len[i]) continue;
-            for (long long sl = soff[i]; sl < soff[(size_t)i + 1]; ++sl) {
-                const int p = spart[(size_t)sl];
-                if (!scnt[(size_t)sl] || !bkept[(size_t)sl] || !valid[p]) continue;
-                const int u = (int)w[p]++;
-                up[(size_t)sl] = u;
-                us.nrpv[u] = i;
-                us.toff[(size_t)u + 1] = scnt[(size_t)sl];
-                for (int j = 0; j < 3; ++j) Xu[3 * (size_t)u + j] = XXd[3 * (size_t)i + j];
-                ptblk[u] = pblk[p];
+        // union point u of part p: the points of p in global order, so range
+        // r of the points starts at p's offset plus the ranges before it
+        std::vector<long long> w((size_t)nr * P, 0);
+        auto take = [&](long long sl) {
+            const int p = spart[(size_t)sl];
+            return scnt[(size_t)sl] && bkept[(size_t)sl] && valid[p];
+        };
+        parallel_ranges(pxs, nr, [&](long long i0, long long i1, int r) {
+            long long *wr = w.data() + (size_t)r * P;
+            for (long long i = i0; i < i1; ++i)
+                if (len[i])
+                    for (long long sl = soff[i]; sl < soff[(size_t)i + 1]; ++sl)
+                        if (take(sl)) wr[spart[(size_t)sl]]++;
+        });
+        for (int p = 0; p < P; ++p) {
+            long long at = uoff[p];
+            for (int r = 0; r < nr; ++r) {
+                const long long n = w[(size_t)r * P + p];
+                w[(size_t)r * P + p] = at;
+                at += n;
             }
         }
+        parallel_ranges(pxs, nr, [&](long long i0, long long i1, int r) {
+            long long *wr = w.data() + (size_t)r * P;
+            for (long long i = i0; i < i1; ++i) {
+                if (!len[i]) continue;
+                for (long long sl = soff[i]; sl < soff[(size_t)i + 1]; ++sl) {
+                    if (!take(sl)) continue;
+                    const int p = spart[(size_t)sl];
+                    const int u = (int)wr[p]++;
+                    up[(size_t)sl] = u;
+                    us.nrpv[u] = (int)i;
+                    us.toff[(size_t)u + 1] = scnt[(size_t)sl];
+                    for (int j = 0; j < 3; ++j) Xu[3 * (size_t)u + j] = XXd[3 * (size_t)i + j];
+                    ptblk[u] = pblk[p];
+                }
+            }
+        });
     }
     for (int u = 0; u < nup; ++u) us.toff[(size_t)u + 1] += us.toff[u];
     us.ntr = us.toff[nup];
     clk.mark(st, "union points", cur_iter);
-    us.tcam.assign((size_t)us.ntr, 0);
-    us.tpt.assign((size_t)us.ntr, 0);
-    us.tfo.assign((size_t)us.ntr, 0);
-    std::vector<float> w0(3 * (size_t)us.ntr);
+    us.tcam.resize((size_t)us.ntr);  // every union entry is written below
+    us.tpt.resize((size_t)us.ntr);
+    us.tfo.resize((size_t)us.ntr);
+    w0.resize(3 * (size_t)us.ntr);
     {
+        // the entries of union point u all come from one point's track
         std::vector<int> pos(us.toff.begin(), us.toff.end() - 1);
-        for (int i = 0; i < pxs; ++i)
-            for (int q = 0; q < len[i]; ++q) {
-                const size_t e = (size_t)(base[i] + q);
-                const int u = up[(size_t)eslot[e]];
-                if (u < 0) continue;
-                const int at = pos[u]++, fo = trk[e];
-                us.tcam[at] = uidx[ecam[e]];
-                us.tpt[at] = u;
-                us.tfo[at] = fo;
-                w0[3 * (size_t)at] = io->obs_w ? io->obs_w[3 * (size_t)fo] : 1.f;
-                w0[3 * (size_t)at + 1] = io->obs_w ? io->obs_w[3 * (size_t)fo + 1] : 1.f;
-                w0[3 * (size_t)at + 2] = io->obs_w ? io->obs_w[3 * (size_t)fo + 2] : 0.f;
-            }
+        parallel_ranges(pxs, nr, [&](long long i0, long long i1, int) {
+            for (long long i = i0; i < i1; ++i)
+                for (int q = 0; q < len[i]; ++q) {
+                    const size_t e = (size_t)(base[i] + q);
+                    const int u = up[(size_t)eslot[e]];
+                    if (u < 0) continue;
+                    const int at = pos[u]++, fo = trk[e];
+                    us.tcam[at] = uidx[ecam[e]];
+                    us.tpt[at] = u;
+                    us.tfo[at] = fo;
+                    w0[3 * (size_t)at] = io->obs_w ? io->obs_w[3 * (size_t)fo] : 1.f;
+                    w0[3 * (size_t)at + 1] = io->obs_w ? io->obs_w[3 * (size_t)fo + 1] : 1.f;
+                    w0[3 * (size_t)at + 2] = io->obs_w ? io->obs_w[3 * (size_t)fo + 2] : 0.f;
+                }
+        });
     }
     clk.mark(st, "union entries", cur_iter);
     // the priors: the gn-th kept GCP point of a part takes entry gn of its list
-    std::vector<double> gcc;
-    std::vector<float> gcw;
     {
-        int gn = 0, cur = -1;
-        for (int u = 0; u < nup; ++u) {
-            if (ptblk[u] != cur) {
-                cur = ptblk[u];
-                gn = 0;
+        // the GCP union points in union order: gcp_of[u] = their running
+        // count; within a part the count from the part's first union point
+        std::vector<long long> rc0((size_t)nr + 1, 0);
+        parallel_ranges(nup, nr, [&](long long u0, long long u1, int r) {
+            long long n = 0;
+            for (long long u = u0; u < u1; ++u) n += gcpinfo[us.nrpv[u]] >= 0;
+            rc0[(size_t)r + 1] = n;
+        });
+        for (int r = 0; r < nr; ++r) rc0[(size_t)r + 1] += rc0[r];
+        const long long ng = rc0[nr];
+        gcc.resize(3 * (size_t)ng);
+        gcw.resize(6 * (size_t)ng);
+        parallel_ranges(nup, nr, [&](long long u0, long long u1, int r) {
+            long long x = rc0[r];
+            for (long long u = u0; u < u1; ++u) {
+                const int i = us.nrpv[u];
+                if (gcpinfo[i] < 0) {
+                    gcp_of[u] = -1;
+                    continue;
+                }
+                gcp_of[u] = (int)x;
+                ++x;
             }
-            const int i = us.nrpv[u];
-            if (gcpinfo[i] < 0) continue;
-            const std::vector<float> &L = w3list[(size_t)bpart[cur]];
-            gcp_of[u] = (int)(gcc.size() / 3);
-            for (int j = 0; j < 3; ++j) gcc.push_back(gcpc[3 * (size_t)gcpinfo[i] + j]);
-            for (int v = 0; v < 6; ++v) gcw.push_back(6 * (size_t)gn + v < L.size() ? L[6 * (size_t)gn + v] : 0.f);
-            ++gn;
+        });
+        // a part's first GCP union point: its list entry 0
+        std::vector<long long> pfirst((size_t)nb, -1);
+        for (int bb = 0; bb < nb; ++bb) {
+            const int p = bpart[bb];
+            for (long long u = uoff[p]; u < uoff[(size_t)p + 1]; ++u)
+                if (gcp_of[u] >= 0) {
+                    pfirst[bb] = gcp_of[u];
+                    break;
+                }
         }
+        parallel_ranges(nup, nr, [&](long long u0, long long u1, int) {
+            for (long long u = u0; u < u1; ++u) {
+                const int x = gcp_of[u];
+                if (x < 0) continue;
+                const int i = us.nrpv[u], bb = ptblk[u];
+                const std::vector<float> &L = w3list[(size_t)bpart[bb]];
+                const size_t gn = (size_t)(x - pfirst[bb]);
+                for (int j = 0; j < 3; ++j) gcc[3 * (size_t)x + j] = gcpc[3 * (size_t)gcpinfo[i] + j];
+                for (int v = 0; v < 6; ++v) gcw[6 * (size_t)x + v] = 6 * gn + v < L.size() ? L[6 * gn + v] : 0.f;
+            }
+        });
     }
     // camera state of the pass (EncodeCameras in GCP mode: every camera has its
     // six pose slots and the estimated intrinsics; no gauge camera)
@@ -526,7 +641,8 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     }
     clk.mark(st, "priors and cameras", cur_iter);
     // device structure of the union (pairs within parts only)
-    std::unique_ptr<DevStruct> ds(new DevStruct());
+    if (!uds) uds.reset(new DevStruct());
+    DevStruct *ds = uds.get();
     ds->with_pairs = true;
     ds->spmv_waves = 4 * spmv_grid(0);
     int rc = ds->upload_tracks(us, io->obs_xy, w0, st);
@@ -538,14 +654,18 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     if (ds->nslab <= 0 || ds->nchunk <= 0) return RPBA_E_INVALID;  // a part's S without off-diagonal blocks
     clk.mark(st, "union structure (device)", cur_iter);
     const size_t ntr = (size_t)ds->ntr, np = (size_t)ds->npts;
-    DBuf<double> Wrec, Yrec, Vinv, gp, Sd, Su, bv, Dd, rowpart, colpart, dpart, Cinv, x, best, r, dv, Ad, h, h1, X, Xn, Sdf,
-        ev, c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist, gc_c;
-    DBuf<float> gc_w;
-    DBuf<unsigned> c_mask;
-    DBuf<unsigned char> camflag;
-    DBuf<int> gc_of, d_boff, d_cblk, d_ptblk, d_nan, d_ndone;
-    DBuf<SegScal> segS;
-    DBuf<PcgScalars> pcgS;
+    UnionBufs &U = ub;
+    auto &Wrec = U.Wrec, &Yrec = U.Yrec, &Vinv = U.Vinv, &gp = U.gp, &Sd = U.Sd, &Su = U.Su, &bv = U.bv, &Dd = U.Dd;
+    auto &rowpart = U.rowpart, &colpart = U.colpart, &dpart = U.dpart, &Cinv = U.Cinv, &x = U.x, &best = U.best, &r = U.r;
+    auto &dv = U.dv, &Ad = U.Ad, &h = U.h, &h1 = U.h1, &X = U.X, &Xn = U.Xn, &Sdf = U.Sdf, &ev = U.ev;
+    auto &c_Rb = U.c_Rb, &c_Cb = U.c_Cb, &c_Rn = U.c_Rn, &c_Cn = U.c_Cn, &c_In = U.c_In, &c_th = U.c_th, &c_thn = U.c_thn;
+    auto &c_Pst = U.c_Pst, &c_Ist = U.c_Ist, &gc_c = U.gc_c;
+    auto &gc_w = U.gc_w;
+    auto &c_mask = U.c_mask;
+    auto &camflag = U.camflag;
+    auto &gc_of = U.gc_of, &d_boff = U.d_boff, &d_cblk = U.d_cblk, &d_ptblk = U.d_ptblk, &d_nan = U.d_nan, &d_ndone = U.d_ndone;
+    auto &segS = U.segS;
+    auto &pcgS = U.pcgS;
     rc = 0;
     rc |= Wrec.alloc((size_t)kRec * std::max<size_t>(ntr, 1));
     rc |= Yrec.alloc((size_t)kRec * std::max<size_t>(ntr, 1));
@@ -675,7 +795,6 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     HIPCHK(hipMemcpyAsync(nanf.data(), d_nan.p, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(hs.data(), segS.p, sizeof(SegScal) * (size_t)nb, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    ds->release();
     clk.mark(st, "step, decode, store", cur_iter);
     // OutputLSAP: the stored matrices and camerasnew (float) of every part
     // that stored; a part whose step was NaN keeps its encoded matrices and
@@ -853,51 +972,85 @@ int partitioned_solve(rpba_block *io, const rpba_flags *f, int P, rpba_log_fn lo
         if (iter > 1) {
             // pointXXv from the parts' tracks, part by part (rpbacore.cpp:276-289);
             // the slots of the entries are those block_step indexed
-            std::vector<int> tmp;
-            for (int i = 0; i < pxs; ++i) {
-                if (!S.len[i]) continue;
-                tmp.clear();
-                const long long b = S.base[i];
-                for (long long sl = S.soff[i]; sl < S.soff[(size_t)i + 1]; ++sl) {
-                    if (!S.bkept[(size_t)sl]) continue;
-                    for (int q = 0; q < S.len[i]; ++q)
-                        if (S.eslot[(size_t)(b + q)] == sl) tmp.push_back(S.trk[(size_t)(b + q)]);
+            parallel_ranges(pxs, S.nthr, [&](long long i0, long long i1, int) {
+                std::vector<int> tmp;
+                for (long long i = i0; i < i1; ++i) {
+                    if (!S.len[i]) continue;
+                    tmp.clear();
+                    const long long b = S.base[i];
+                    for (long long sl = S.soff[i]; sl < S.soff[(size_t)i + 1]; ++sl) {
+                        if (!S.bkept[(size_t)sl]) continue;
+                        for (int q = 0; q < S.len[i]; ++q)
+                            if (S.eslot[(size_t)(b + q)] == sl) tmp.push_back(S.trk[(size_t)(b + q)]);
+                    }
+                    std::copy(tmp.begin(), tmp.end(), S.trk.begin() + b);
+                    S.len[i] = tmp.size() < 2 ? 0 : (int)tmp.size();
                 }
-                std::copy(tmp.begin(), tmp.end(), S.trk.begin() + b);
-                S.len[i] = tmp.size() < 2 ? 0 : (int)tmp.size();
-            }
+            });
         }
         S.clk.mark(nullptr, "tracks regrouped", iter);
         // pitflags of the points with entries (rpbacore.cpp:301-309)
         S.entry_index();
-        for (int i = 0; i < pxs; ++i) {
-            if (!S.len[i]) continue;
-            for (long long sl = S.soff[i]; sl < S.soff[(size_t)i + 1]; ++sl) S.pit[(size_t)sl] = 0;
-            for (int q = 0; q < S.len[i]; ++q) S.pit[(size_t)S.eslot[(size_t)(S.base[i] + q)]] = 1;
-        }
+        parallel_ranges(pxs, S.nthr, [&](long long i0, long long i1, int) {
+            for (long long i = i0; i < i1; ++i) {
+                if (!S.len[i]) continue;
+                for (long long sl = S.soff[i]; sl < S.soff[(size_t)i + 1]; ++sl) S.pit[(size_t)sl] = 0;
+                for (int q = 0; q < S.len[i]; ++q) S.pit[(size_t)S.eslot[(size_t)(S.base[i] + q)]] = 1;
+            }
+        });
         S.clk.mark(nullptr, "pitflags", iter);
         if (robustit) {
             // the robust statistics pass and the per-image scales (rpbacore.cpp:311-375)
             std::vector<int> len0 = S.len;
             rc = S.point_pass(1, 0, 0, sgv, rdv, &keys);
             if (rc) return rc;
-            std::vector<std::vector<float>> per((size_t)N);
-            for (int i = 0; i < pxs; ++i)
-                for (int q = 0; q < len0[i]; ++q) {
-                    const long long e = S.base[i] + q;
-                    per[(size_t)io->obs_cam[S.trk[(size_t)e]]].push_back(keys[(size_t)e]);
+            // the keys grouped by image (a counting sort over ranges of
+            // points), then each image's upper median, which does not depend
+            // on the order inside the group
+            const int nr = S.nthr;
+            std::vector<long long> kc((size_t)nr * N + 1, 0);
+            parallel_ranges(pxs, nr, [&](long long i0, long long i1, int r) {
+                long long *c = kc.data() + (size_t)r * N;
+                for (long long i = i0; i < i1; ++i)
+                    for (int q = 0; q < len0[i]; ++q) c[io->obs_cam[S.trk[(size_t)(S.base[i] + q)]]]++;
+            });
+            std::vector<long long> koff((size_t)N + 1, 0);
+            {
+                long long at = 0;
+                for (int k = 0; k < N; ++k) {
+                    koff[k] = at;
+                    for (int r = 0; r < nr; ++r) {
+                        const long long n = kc[(size_t)r * N + k];
+                        kc[(size_t)r * N + k] = at;
+                        at += n;
+                    }
                 }
-            for (int k = 0; k < N; ++k) {
-                std::vector<float> &v = per[(size_t)k];
-                if (v.empty()) {
+                koff[N] = at;
+            }
+            std::vector<float> kv((size_t)std::max<long long>(koff[N], 1));
+            parallel_ranges(pxs, nr, [&](long long i0, long long i1, int r) {
+                long long *c = kc.data() + (size_t)r * N;
+                for (long long i = i0; i < i1; ++i)
+                    for (int q = 0; q < len0[i]; ++q) {
+                        const long long e = S.base[i] + q;
+                        kv[(size_t)c[io->obs_cam[S.trk[(size_t)e]]]++] = keys[(size_t)e];
+                    }
+            });
+            parallel_ranges(N, nr, [&](long long k0, long long k1, int) {
+                for (long long k = k0; k < k1; ++k) {
+                    float *v0 = kv.data() + koff[k], *v1 = kv.data() + koff[(size_t)k + 1];
+                    if (v1 == v0) continue;
+                    const size_t n = (size_t)(v1 - v0);
+                    std::nth_element(v0, v0 + n / 2, v1);
+                    S.sig0wv[k] = 1.f / (v0[n / 2] * 2.1981f);
+                }
+            });
+            for (int k = 0; k < N; ++k)
+                if (koff[(size_t)k + 1] == koff[k]) {
                     std::snprintf(buf, sizeof buf, "Image %d without observations!!!!!!!!!!!!!!!!!!!!!\n\n", k);
                     S.logs(buf);
                     S.sig0wv[k] = 0.f;
-                } else {
-                    std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
-                    S.sig0wv[k] = 1.f / (v[v.size() / 2] * 2.1981f);
                 }
-            }
         }
         S.clk.mark(nullptr, "robust statistics pass", iter);
         // the main point pass (rpbacore.cpp:378-392)
@@ -944,25 +1097,53 @@ int partitioned_solve(rpba_block *io, const rpba_flags *f, int P, rpba_log_fn lo
         // GCPs: points with entries in two parts or more, their centres and
         // each part's weight list (rpbacore.cpp:419-450)
         {
-            int nrgcp = 0;
-            for (int p = 0; p < P; ++p) S.w3list[(size_t)p].clear();
-            for (int i = 0; i < pxs; ++i) {
-                if (S.len[i] < 2) continue;
+            // GCP numbers in point order and each part's list in point
+            // order: counts over ranges of points, then the ranges fill
+            // their own places
+            const int nr = S.nthr;
+            auto is_gcp = [&](long long i) {
+                if (S.len[i] < 2) return false;
                 int gnum = 0;
                 for (long long s = S.soff[i]; s < S.soff[(size_t)i + 1] && gnum < 2; ++s) gnum += S.pit[(size_t)s];
-                if (gnum > 1) {
-                    S.gcpinfo[i] = nrgcp;
-                    for (int j = 0; j < 3; ++j) S.gcpc[3 * (size_t)nrgcp + j] = S.XXd[3 * (size_t)i + j];
+                return gnum > 1;
+            };
+            std::vector<long long> gc((size_t)nr * (P + 1), 0);  // per range: GCP points, then entries per part
+            parallel_ranges(pxs, nr, [&](long long i0, long long i1, int r) {
+                long long *c = gc.data() + (size_t)r * (P + 1);
+                for (long long i = i0; i < i1; ++i) {
+                    if (!is_gcp(i)) continue;
+                    c[0]++;
+                    for (long long s = S.soff[i]; s < S.soff[(size_t)i + 1]; ++s)
+                        if (S.pit[(size_t)s]) c[1 + S.spart[(size_t)s]]++;
+                }
+            });
+            for (int j = 0; j <= P; ++j) {
+                long long at = 0;
+                for (int r = 0; r < nr; ++r) {
+                    const long long n = gc[(size_t)r * (P + 1) + j];
+                    gc[(size_t)r * (P + 1) + j] = at;
+                    at += n;
+                }
+                if (j > 0) S.w3list[(size_t)j - 1].assign(6 * (size_t)at, 0.f);
+            }
+            parallel_ranges(pxs, nr, [&](long long i0, long long i1, int r) {
+                long long *c = gc.data() + (size_t)r * (P + 1);
+                for (long long i = i0; i < i1; ++i) {
+                    if (!is_gcp(i)) {
+                        S.gcpinfo[i] = -1;
+                        continue;
+                    }
+                    const long long g = c[0]++;
+                    S.gcpinfo[i] = (int)g;
+                    for (int j = 0; j < 3; ++j) S.gcpc[3 * (size_t)g + j] = S.XXd[3 * (size_t)i + j];
                     for (long long s = S.soff[i]; s < S.soff[(size_t)i + 1]; ++s) {
                         if (!S.pit[(size_t)s]) continue;
-                        std::vector<float> &L = S.w3list[(size_t)S.spart[(size_t)s]];
-                        for (int v = 0; v < 6; ++v) L.push_back(S.w3h[6 * (size_t)s + v]);
+                        const int p = S.spart[(size_t)s];
+                        float *L = S.w3list[(size_t)p].data() + 6 * (size_t)(c[1 + p]++);
+                        for (int v = 0; v < 6; ++v) L[v] = S.w3h[6 * (size_t)s + v];
                     }
-                    ++nrgcp;
-                } else {
-                    S.gcpinfo[i] = -1;
                 }
-            }
+            });
         }
         S.clk.mark(nullptr, "sums, best state, GCP lists", iter);
         // iflag (rpbacore.cpp:479-493)

@@ -100,6 +100,11 @@ struct DevStruct {
     bool updated = false;    // the next build() has nothing to do
     int updated_empty = -1;  // first image without entries after that update
     int npts0 = 0, ntr0 = 0;
+    // room of the track arrays and their snapshot (upload_tracks reallocates
+    // both when a structure needs more), and its host staging
+    size_t trk_np_alloc = 0, trk_nt_alloc = 0;
+    void *stage = nullptr;  // pinned
+    size_t stage_cap = 0;
     // pinned host staging for the structure's small readbacks and uploads
     int *hpin = nullptr;
     size_t hpin_cap = 0;

@@ -15,6 +15,10 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "structure_dev.h"
+
+#include <cstring>
+
+#include "host_util.h"
 #include "kernels.h"
 
 namespace rpba {
@@ -891,36 +895,68 @@ int DevStruct::upload_tracks(const Structure &s, const float *obs_xy, const std:
     sizes_cap_pts = (size_t)npts;
     sizes_cap_tr = (size_t)ntr;
     Tracks t;
-    if (!toff) {
+    if (!toff || (size_t)npts > trk_np_alloc || (size_t)ntr > trk_nt_alloc) {
+        // first upload, or a structure larger than the room (a solve that
+        // uploads again, as the partitioned one does per outer iteration,
+        // keeps its buffers otherwise)
+        if (toff) {
+            Tracks old;
+            old.toff = toff; old.nrpv = nrpv; old.tcam = tcam; old.tpt = tpt; old.tfo = tfo;
+            old.ox = ox; old.oy = oy; old.w0x = w0x; old.w0y = w0y; old.w0xy = w0xy;
+            old.release();
+            init.release();
+        }
         if (t.alloc(sizes_cap_pts, sizes_cap_tr)) return RPBA_E_NOMEM;
         toff = t.toff; nrpv = t.nrpv; tcam = t.tcam; tpt = t.tpt; tfo = t.tfo;
         ox = t.ox; oy = t.oy; w0x = t.w0x; w0y = t.w0y; w0xy = t.w0xy;
         (void)hipFree(t.X);
+        trk_np_alloc = sizes_cap_pts;
+        trk_nt_alloc = sizes_cap_tr;
     }
-    std::vector<float> hx((size_t)ntr), hy((size_t)ntr), a((size_t)ntr), b((size_t)ntr), c((size_t)ntr);
-    for (int q = 0; q < ntr; ++q) {
-        const int fo = s.tfo[q];
-        hx[q] = obs_xy[2 * (size_t)fo];
-        hy[q] = obs_xy[2 * (size_t)fo + 1];
-        a[q] = w0[3 * (size_t)q];
-        b[q] = w0[3 * (size_t)q + 1];
-        c[q] = w0[3 * (size_t)q + 2];
+    // every array through one pinned staging buffer (grown when needed), so
+    // the copies run at the link's rate; filled over ranges of entries
+    const size_t np1 = (size_t)npts + 1, nt = (size_t)ntr;
+    const size_t need = sizeof(int) * (2 * np1 + 8 * nt);
+    if (need > stage_cap) {
+        if (stage) (void)hipHostFree(stage);
+        stage = nullptr;
+        stage_cap = 0;
+        if (hipHostMalloc(&stage, need, hipHostMallocDefault) != hipSuccess) return RPBA_E_NOMEM;
+        stage_cap = need;
     }
-    DCHK(hipMemcpyAsync(toff, s.toff.data(), sizeof(int) * ((size_t)npts + 1), hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(nrpv, s.nrpv.data(), sizeof(int) * (size_t)npts, hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(tcam, s.tcam.data(), sizeof(int) * (size_t)ntr, hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(tpt, s.tpt.data(), sizeof(int) * (size_t)ntr, hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(tfo, s.tfo.data(), sizeof(int) * (size_t)ntr, hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(ox, hx.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(oy, hy.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(w0x, a.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(w0y, b.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
-    DCHK(hipMemcpyAsync(w0xy, c.data(), sizeof(float) * (size_t)ntr, hipMemcpyHostToDevice, st));
+    int *h_toff = (int *)stage, *h_nrpv = h_toff + np1, *h_tcam = h_nrpv + np1, *h_tpt = h_tcam + nt;
+    int *h_tfo = h_tpt + nt;
+    float *hx = (float *)(h_tfo + nt), *hy = hx + nt, *a = hy + nt, *b = a + nt, *c = b + nt;
+    std::memcpy(h_toff, s.toff.data(), sizeof(int) * np1);
+    std::memcpy(h_nrpv, s.nrpv.data(), sizeof(int) * (size_t)npts);
+    parallel_ranges(ntr, host_threads(), [&](long long q0, long long q1, int) {
+        for (long long q = q0; q < q1; ++q) {
+            const int fo = s.tfo[q];
+            h_tcam[q] = s.tcam[q];
+            h_tpt[q] = s.tpt[q];
+            h_tfo[q] = fo;
+            hx[q] = obs_xy[2 * (size_t)fo];
+            hy[q] = obs_xy[2 * (size_t)fo + 1];
+            a[q] = w0[3 * (size_t)q];
+            b[q] = w0[3 * (size_t)q + 1];
+            c[q] = w0[3 * (size_t)q + 2];
+        }
+    });
+    DCHK(hipMemcpyAsync(toff, h_toff, sizeof(int) * np1, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(nrpv, h_nrpv, sizeof(int) * (size_t)npts, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(tcam, h_tcam, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(tpt, h_tpt, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(tfo, h_tfo, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(ox, hx, sizeof(float) * nt, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(oy, hy, sizeof(float) * nt, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(w0x, a, sizeof(float) * nt, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(w0y, b, sizeof(float) * nt, hipMemcpyHostToDevice, st));
+    DCHK(hipMemcpyAsync(w0xy, c, sizeof(float) * nt, hipMemcpyHostToDevice, st));
     // snapshot for restore_tracks
     npts0 = npts;
     ntr0 = ntr;
     if (!init.toff && init.alloc(sizes_cap_pts, sizes_cap_tr)) return RPBA_E_NOMEM;
-    const size_t np = (size_t)npts, nt = (size_t)ntr;
+    const size_t np = (size_t)npts;
     DCHK(hipMemcpyAsync(init.toff, toff, sizeof(int) * (np + 1), hipMemcpyDeviceToDevice, st));
     DCHK(hipMemcpyAsync(init.nrpv, nrpv, sizeof(int) * np, hipMemcpyDeviceToDevice, st));
     DCHK(hipMemcpyAsync(init.tcam, tcam, sizeof(int) * nt, hipMemcpyDeviceToDevice, st));
@@ -973,6 +1009,10 @@ void DevStruct::release()
 {
     init.release();
     spare.release();
+    if (stage) (void)hipHostFree(stage);
+    stage = nullptr;
+    stage_cap = 0;
+    trk_np_alloc = trk_nt_alloc = 0;
     if (hpin) (void)hipHostFree(hpin);
     hpin = nullptr;
     hpin_cap = 0;
