@@ -51,23 +51,25 @@ struct Graph {
 // vertices reachable from s inside the set (mask), breadth first, ties by index
 static std::vector<int> bfs_order(const Graph &g, int s, const std::vector<char> &in)
 {
+    // the set's vertices not yet reached, as bits: a vertex's row masked by
+    // them gives exactly its new neighbours, in index order
+    std::vector<uint64_t> avail(g.words, 0);
+    for (int v = 0; v < g.n; ++v)
+        if (in[v]) avail[v >> 6] |= 1ull << (v & 63);
     std::vector<int> order;
-    std::vector<char> seen(g.n, 0);
     order.push_back(s);
-    seen[s] = 1;
+    avail[s >> 6] &= ~(1ull << (s & 63));
     for (size_t h = 0; h < order.size(); ++h) {
         const int v = order[h];
         const uint64_t *row = g.adj.data() + (size_t)v * g.words;
         for (int wd = 0; wd < g.words; ++wd) {
-            uint64_t m = row[wd];
+            uint64_t m = row[wd] & avail[wd];
+            if (!m) continue;
+            avail[wd] &= ~m;
             while (m) {
                 const int b = __builtin_ctzll(m);
                 m &= m - 1;
-                const int u = wd * 64 + b;
-                if (u < g.n && in[u] && !seen[u]) {
-                    seen[u] = 1;
-                    order.push_back(u);
-                }
+                order.push_back(wd * 64 + b);
             }
         }
     }
@@ -202,21 +204,28 @@ int partition_images(int nc, int nobs, const int *obs_cam, const int *obs_pt, in
     g.n = nc;
     g.words = (nc + 63) / 64;
     g.adj.assign((size_t)nc * g.words, 0);
+    // bits and sums from ranges of points at once: OR and integer addition
+    // give the same result in any order
     std::vector<long long> BM((size_t)nc, 0);
-    for (int i = 0; i < npts; ++i) {
-        const int b = cnt[i], e = cnt[(size_t)i + 1], t = e - b;
-        if (t < 2) continue;
-        for (int x = b; x < e; ++x) {
-            const int fi = cams[x];
-            BM[fi] += t;
-            for (int y = x + 1; y < e; ++y) {
-                const int si = cams[y];
-                if (si == fi) continue;
-                g.adj[(size_t)fi * g.words + (si >> 6)] |= 1ull << (si & 63);
-                g.adj[(size_t)si * g.words + (fi >> 6)] |= 1ull << (fi & 63);
+    uint64_t *adj = g.adj.data();
+    long long *bm = BM.data();
+    const int words = g.words;
+    parallel_ranges(npts, host_threads(), [&](long long i0, long long i1, int) {
+        for (long long i = i0; i < i1; ++i) {
+            const int b = cnt[i], e = cnt[(size_t)i + 1], t = e - b;
+            if (t < 2) continue;
+            for (int x = b; x < e; ++x) {
+                const int fi = cams[x];
+                __atomic_fetch_add(bm + fi, (long long)t, __ATOMIC_RELAXED);
+                for (int y = x + 1; y < e; ++y) {
+                    const int si = cams[y];
+                    if (si == fi) continue;
+                    __atomic_fetch_or(adj + (size_t)fi * words + (si >> 6), 1ull << (si & 63), __ATOMIC_RELAXED);
+                    __atomic_fetch_or(adj + (size_t)si * words + (fi >> 6), 1ull << (fi & 63), __ATOMIC_RELAXED);
+                }
             }
         }
-    }
+    });
     g.w.resize(nc);
     for (int i = 0; i < nc; ++i) g.w[i] = std::max(1, (int)std::pow((double)BM[i], 0.3333));
     std::vector<int> vs(nc), pv(nc, 0);
@@ -875,28 +884,42 @@ int partitioned_solve(rpba_block *io, const rpba_flags *f, int P, rpba_log_fn lo
             const int p = io->obs_pt[q];
             S.trk[(size_t)(S.base[p] + S.len[p]++)] = q;
         }
-    // a camera seen twice in one track is refused, as in the global solve
-    {
-        std::vector<int> seen((size_t)N, -1);
-        for (int i = 0; i < pxs; ++i)
-            for (int q = 0; q < S.len[i]; ++q) {
-                const int c = io->obs_cam[S.trk[(size_t)(S.base[i] + q)]];
-                if (seen[c] == i) return RPBA_E_INVALID;
-                seen[c] = i;
-            }
-    }
-    // slots: the parts of each point's initial track, ascending
+    // a camera seen twice in one track is refused, as in the global solve;
+    // slots: the parts of each point's initial track, ascending (counted over
+    // ranges of points, then filled)
     S.soff.assign((size_t)pxs + 1, 0);
     {
-        std::vector<int> tmp;
-        for (int i = 0; i < pxs; ++i) {
+        const int nr = S.nthr;
+        std::vector<char> dup((size_t)nr, 0);
+        auto parts_of = [&](long long i, std::vector<int> &tmp) {
             tmp.clear();
             for (int q = 0; q < S.len[i]; ++q) tmp.push_back(S.part[io->obs_cam[S.trk[(size_t)(S.base[i] + q)]]]);
             std::sort(tmp.begin(), tmp.end());
             tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-            S.soff[(size_t)i + 1] = S.soff[i] + (long long)tmp.size();
-            for (int p : tmp) S.spart.push_back(p);
-        }
+        };
+        parallel_ranges(pxs, nr, [&](long long i0, long long i1, int r) {
+            std::vector<int> seen((size_t)N, -1), tmp;
+            for (long long i = i0; i < i1; ++i) {
+                for (int q = 0; q < S.len[i]; ++q) {
+                    const int c = io->obs_cam[S.trk[(size_t)(S.base[i] + q)]];
+                    if (seen[c] == (int)i) dup[r] = 1;
+                    seen[c] = (int)i;
+                }
+                parts_of(i, tmp);
+                S.soff[(size_t)i + 1] = (long long)tmp.size();
+            }
+        });
+        for (int r = 0; r < nr; ++r)
+            if (dup[r]) return RPBA_E_INVALID;
+        for (int i = 0; i < pxs; ++i) S.soff[(size_t)i + 1] += S.soff[i];
+        S.spart.resize((size_t)S.soff[pxs]);
+        parallel_ranges(pxs, nr, [&](long long i0, long long i1, int) {
+            std::vector<int> tmp;
+            for (long long i = i0; i < i1; ++i) {
+                parts_of(i, tmp);
+                std::copy(tmp.begin(), tmp.end(), S.spart.begin() + S.soff[i]);
+            }
+        });
     }
     const size_t NS = (size_t)S.soff[pxs];
     S.pit.assign(std::max<size_t>(NS, 1), 0);
