@@ -30,8 +30,9 @@ errors against plain FP64, the fastest that reaches the final sigma0 as
 "chosen" (and the fastest to the pass-0 sigma0 beside it); the per-kernel HIP-event
 roofline of the dominant kernel (S d); the CPU oracle timed on a bounded
 sample of one LM iteration of the same scene (cpu_baseline); and the
-partitioned solve (Threads = 0 dispatch) of the 961-camera scene as a whole
-rpba_calc_plsa call beside the oracle's OpenMP restatement of it.
+partitioned solve (Threads = 0 dispatch) of the 961-, 4,585- and
+13,682-camera scenes, each as a whole rpba_calc_plsa call beside the oracle's
+OpenMP restatement of it.
 """
 import argparse
 import ctypes as C
@@ -398,7 +399,7 @@ def run_bench(args):
             fctx.close()
         line["cpu_baseline"] = cpu_baseline(sc, fsz, k_avg, args.config)
     if args.partition_config != "none":
-        line["partitioned"] = partitioned_leg(args.partition_config, args.no_cpu_baseline)
+        line["partitioned"] = [partitioned_leg(cfg, args.no_cpu_baseline) for cfg in args.partition_config.split(",")]
     print(json.dumps(line), flush=True)
 
 
@@ -700,8 +701,8 @@ def main():
                     help="plain FP64 form with S formed (0), the matrix-free Schur product (1) or whichever "
                          "streams fewer bytes per PCG iteration (2, default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--partition-config", default="c2_961",
-                    help="scene of the partitioned-solve leg (Threads = 0 dispatch); none to skip")
+    ap.add_argument("--partition-config", default="c2_961,c3_4585,c5_13682",
+                    help="scenes of the partitioned-solve leg (Threads = 0 dispatch), comma-separated; none to skip")
     ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (FP32, matrix-free) form")
     ap.add_argument("--no-sequential", action="store_true",
                     help="c4_batch: skip the one-block-after-another comparison (profiling runs)")

@@ -801,23 +801,42 @@ __device__ __forceinline__ void apply_cinv(const double *Cinv, const double *r, 
 
 // x = 0, r = b, h = C b, d = h, best = b (exivold starts as a copy of b),
 // rho = r.h.  Single workgroup of 1024.  Scalars live in S[0] (launch 0).
-__global__ __launch_bounds__(1024) void k_pcg_init(PcgArgs a)
+// PCG set-up, thread per camera: x = 0, r = best = b, h = d = C r, and the
+// workgroup's share of r.h (the camera's nine terms in order, then the
+// workgroup's cameras in order) into part[blockIdx.x]
+__global__ __launch_bounds__(256) void k_pcg_init(PcgArgs a)
+{
+    __shared__ double sh[4];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.;
+    if (k < a.nc) {
+        const size_t j0 = 9 * (size_t)k;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const double bj = a.b[j0 + j];
+            a.x[j0 + j] = 0.;
+            a.r[j0 + j] = bj;
+            a.best[j0 + j] = bj;
+        }
+        apply_cinv(a.Cinv, a.r, a.h, k);
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const double hj = a.h[j0 + j];
+            a.d[j0 + j] = hj;
+            s += a.r[j0 + j] * hj;
+        }
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = s;
+}
+
+// rho = the shares in workgroup order; the scalars of both slots
+__global__ __launch_bounds__(1024) void k_pcg_init_fin(PcgArgs a, int nb)
 {
     __shared__ double sh[16];
-    const int n = 9 * a.nc;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        a.x[j] = 0.;
-        a.r[j] = a.b[j];
-        a.best[j] = a.b[j];
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < a.nc; k += blockDim.x) apply_cinv(a.Cinv, a.r, a.h, k);
-    __syncthreads();
     double s = 0.;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        a.d[j] = a.h[j];
-        s += a.r[j] * a.h[j];
-    }
+    const int per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
+    for (int i = threadIdx.x * per; i < min(nb, ((int)threadIdx.x + 1) * per); ++i) s += a.part[i];
     s = block_sum(s, sh);
     if (threadIdx.x == 0) {
         PcgScalars z;
@@ -2608,7 +2627,12 @@ void launch_bj_inverse(int nc, const double *Sd, double *Cinv, hipStream_t st)
 {
     k_bj_inverse<<<nblk(nc, 64), 64, 0, st>>>(nc, Sd, Cinv);
 }
-void launch_pcg_init(const PcgArgs &a, hipStream_t st) { k_pcg_init<<<1, 1024, 0, st>>>(a); }
+void launch_pcg_init(const PcgArgs &a, hipStream_t st)
+{
+    const int nb = (int)nblk(a.nc, 256);  // part holds 3 nc doubles: room for the shares
+    k_pcg_init<<<nb, 256, 0, st>>>(a);
+    k_pcg_init_fin<<<1, 1024, 0, st>>>(a, nb);
+}
 // compute units of the current device (persistent SpMV grid)
 static int device_cus()
 {
