@@ -69,6 +69,10 @@ def algorithmic_bytes(kernel, sz):
     """Compulsory HBM bytes of one launch: every array the kernel must touch,
     once.  sz: cams, pts, obs, pairs, blocks."""
     Nc, Np, No, P, B = sz["cams"], sz["pts"], sz["obs"], sz["pairs"], sz["blocks"]
+    jrec = sz.get("jrec", 0)  # bytes of a whitened Jacobian record (precision 2: 96, 3: 192), else 0
+    if kernel == "spmv" and jrec:  # whitened records twice (point- and camera-major) and two indices per entry;
+        # V^-1 in, s out and s in per point; S_kk-diagonal damping, d and Ad per camera
+        return 2 * jrec * No + 8 * No + (72 + 24 + 24) * Np + 3 * 72 * Nc
     if kernel == "spmv" and B == 0:  # matrix-free: W and Y (27 doubles each) and two indices per entry, t out
         return 2 * 216 * No + 8 * No + 48 * Np + 1296 * Nc + 144 * Nc  # and in per point, S_kk + D_k, d and Ad
     if kernel == "spmv":  # S blocks once (upper + diagonal), d in, Ad out, row partials
@@ -102,6 +106,8 @@ FORM_SKIP = {
                                  "not meet the step-norm rule on this scene (see DESIGN)",
 }
 FORM_WALL_S = 60.0  # a form evaluation stops after this much wall time
+PRECISION_FORM = {0: "plain FP64", 1: "FP32 W / Y records (precision 1)",
+                  2: "whitened Jacobian records in FP32 (precision 2)", 3: "whitened Jacobian records in FP64 (precision 3)"}
 
 
 def run_bench(args):
@@ -115,7 +121,7 @@ def run_bench(args):
     sc = scenes.generate(args.config)
     gen_s = time.time() - t0
     progress(f"generated {args.config} ({sc['n_cams']} cameras, {sc['n_obs']} observations) in {gen_s:.1f} s")
-    flags = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, schur=args.schur)
+    flags = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, schur=args.schur, precision=args.precision)
     t0 = time.time()
     ctx = api.Context(sc, flags)
     create_s = time.time() - t0
@@ -331,10 +337,12 @@ def run_bench(args):
               key=lambda k: ktimes[k][0])
     tot_ms, nl = ktimes[dom]
     avg_ms = tot_ms / nl
+    sz0["jrec"] = {2: 96, 3: 192}.get(args.precision, 0)
     alg = algorithmic_bytes(dom, sz0)
     achieved = alg / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}{'_mf' if sz0['blocks'] == 0 else ''}_{args.config}.json")
+    tag = f"_p{args.precision}" if args.precision else ("_mf" if sz0["blocks"] == 0 else "")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}{tag}_{args.config}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -350,7 +358,8 @@ def run_bench(args):
 
     line = {
         "metric": "LM iters/s, plain FP64 robust LM Schur solve (time-to-converged reported beside)",
-        "form": "plain FP64, " + ("matrix-free Schur product" if plain_mf else "S formed block by block") +
+        "form": PRECISION_FORM[args.precision] + ", " +
+                ("matrix-free Schur product" if plain_mf else "S formed block by block") +
                 (" (the product streaming fewer bytes per PCG iteration)" if args.schur == 2 else ""),
         "value": round(args.steps / elapsed, 3),
         "unit": "LM iters/s",
@@ -361,7 +370,7 @@ def run_bench(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision in (0, 3) else "f32 records, f64 sums",
         "data": "synthetic (seeded generator, BAL camera model)",
         "config": {"workload": f"{args.config}: robust LM, {sc['n_cams']} cameras, {sc['n_pts']} points, "
                                f"{sc['n_obs']} observations, k2 k4 f estimated",
@@ -700,6 +709,9 @@ def main():
     ap.add_argument("--schur", type=int, default=2, choices=(0, 1, 2),
                     help="plain FP64 form with S formed (0), the matrix-free Schur product (1) or whichever "
                          "streams fewer bytes per PCG iteration (2, default)")
+    ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="form of the timed steps: plain FP64 (0, default), FP32 W / Y records (1), whitened "
+                         "Jacobian records in FP32 (2) or FP64 (3); the forms other than 0 are for profiling runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--partition-config", default="c2_961,c3_4585,c5_13682",
                     help="scenes of the partitioned-solve leg (Threads = 0 dispatch), comma-separated; none to skip")
