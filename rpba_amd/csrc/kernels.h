@@ -89,6 +89,9 @@ struct LinArgs {
     int jform = 0;
     void *Jpm = nullptr, *Jcm = nullptr;
     double *dmp = nullptr;
+    // per point, one 128-byte line for the camera side (jform): X (3), the
+    // upper triangle of V^-1 (6: 00 01 02 11 12 22), g_p (3), 4 unused
+    double *ptrec = nullptr;
     const int4 *pg4 = nullptr;  // per point group {p0, p1, e0, e1}
 };
 

@@ -2242,11 +2242,20 @@ __global__ __launch_bounds__(256) void k_lin_points_j(LinArgs a)
     V[4] *= a.mult1;
     V[8] *= a.mult1;
     double Vi[9];
-    inv3(V, Vi);
+    inv3(V, Vi);  // V is symmetric to the bit, and so is its cofactor inverse
 #pragma unroll
     for (int j = 0; j < 9; ++j) a.Vinv[9 * (size_t)i + j] = Vi[j];
 #pragma unroll
     for (int j = 0; j < 3; ++j) a.gp[3 * (size_t)i + j] = g[j];
+    double2 *pr = (double2 *)(a.ptrec + 16 * (size_t)i);
+    pr[0] = make_double2(a.X[3 * (size_t)i], a.X[3 * (size_t)i + 1]);
+    pr[1] = make_double2(a.X[3 * (size_t)i + 2], Vi[0]);
+    pr[2] = make_double2(Vi[1], Vi[2]);
+    pr[3] = make_double2(Vi[4], Vi[5]);
+    pr[4] = make_double2(Vi[8], g[0]);
+    pr[5] = make_double2(g[1], g[2]);
+    pr[6] = make_double2(0., 0.);
+    pr[7] = make_double2(0., 0.);
 }
 
 // Camera blocks and the camera-major records: a workgroup per image, lane per
@@ -2278,15 +2287,13 @@ __global__ __launch_bounds__(256) void k_lin_cameras_j(LinArgs a)
         const int j = j0 + t;
         u32x4 pc[NP];
         if (j < e) {
+            // the point's line: X, V^-1 (symmetric), g_p
             const int p = a.cm_pt[j];
-            const double *Xp = a.X + 3 * (size_t)p;
-            const double *Vp = a.Vinv + 9 * (size_t)p;
-            const double *gq = a.gp + 3 * (size_t)p;
-            double Vi[9], gpv[3];
-#pragma unroll
-            for (int m = 0; m < 9; ++m) Vi[m] = Vp[m];
-#pragma unroll
-            for (int m = 0; m < 3; ++m) gpv[m] = gq[m];
+            const double2 *pr = (const double2 *)(a.ptrec + 16 * (size_t)p);
+            const double2 r0v = pr[0], r1v = pr[1], r2v = pr[2], r3v = pr[3], r4v = pr[4], r5v = pr[5];
+            const double Xp[3] = {r0v.x, r0v.y, r1v.x};
+            const double Vi[9] = {r1v.y, r2v.x, r2v.y, r2v.x, r3v.x, r3v.y, r2v.y, r3v.y, r4v.x};
+            const double gpv[3] = {r4v.y, r5v.x, r5v.y};
             Ev ev;
             const float4 wv = a.cm_w[j];
             eval_obs(c, Xp[0], Xp[1], Xp[2], a.cm_ox[j], a.cm_oy[j], a.defdiff, ev);

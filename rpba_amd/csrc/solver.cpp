@@ -80,6 +80,7 @@ struct rpba_ctx {
     DBuf<double> d_t, d_cpart, d_Dk;  // matrix-free form
     DBuf<unsigned char> d_Jpm, d_Jcm;  // whitened Jacobian records (precision 2)
     DBuf<double> d_dmp;                // damping of S_kk's diagonal (precision 2)
+    DBuf<double> d_ptrec;              // per point X, V^-1, g_p as one line (precision 2)
     // camera-major entry table (point, observation, weights; chunk -> image),
     // rebuilt when the structure (cm_full) or the weights (cm_w) change
     DBuf<int> d_cmpt, d_chcam;
@@ -252,6 +253,7 @@ static int build_structure(rpba_ctx *c)
         rc |= c->d_Jpm.alloc((size_t)kJRecElems * esz * ntr);
         rc |= c->d_Jcm.alloc((size_t)kJRecElems * esz * ntr);
         rc |= c->d_dmp.alloc(9 * (size_t)c->nc);
+        rc |= c->d_ptrec.alloc(16 * std::max<size_t>(np, 1));
     } else {
         rc |= c->d_Wrec.alloc((size_t)kRec * esz * ntr);
         rc |= c->d_Yrec.alloc((size_t)kRec * esz * ntr);
@@ -398,7 +400,7 @@ static LinArgs lin_args(rpba_ctx *c)
     // camera kernels run when the structure or the weights changed)
     a.cm_pt = c->d_cmpt.p; a.cm_ox = c->d_cmox.p; a.cm_oy = c->d_cmoy.p;
     a.cm_w = c->d_cmw.p;
-    a.jform = c->jform; a.Jpm = c->d_Jpm.p; a.Jcm = c->d_Jcm.p; a.dmp = c->d_dmp.p;
+    a.jform = c->jform; a.Jpm = c->d_Jpm.p; a.Jcm = c->d_Jcm.p; a.dmp = c->d_dmp.p; a.ptrec = c->d_ptrec.p;
     return a;
 }
 
@@ -1399,6 +1401,7 @@ extern "C" void rpba_ctx_destroy(rpba_ctx *c)
     c->d_Jpm.release();
     c->d_Jcm.release();
     c->d_dmp.release();
+    c->d_ptrec.release();
     c->d_ppart.release();
     if (c->st) (void)hipStreamDestroy(c->st);
     if (c->st2) (void)hipStreamDestroy(c->st2);
