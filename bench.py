@@ -515,6 +515,32 @@ def run_batch_bench(args):
     n_fact = kt["chol_solve"][1]  # factorisations in the timed solve (one backward sweep each)
     achieved_tf = upd_flops * n_fact / (ms_upd * 1e-3) / 1e12 if ms_upd > 0 else None
 
+    # the same batch with the reference's camera solve: block-Jacobi PCG
+    # segmented by block (one union SpMV per PCG iteration)
+    fpcg = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, solver=0)
+    bp = api.Batch(blocks, fpcg)
+    rc, _, _ = bp.run(-1)
+    if rc != 0:
+        raise RuntimeError(f"rpba_batch_run (PCG) failed with status {rc}")
+    bp.reset()
+    device_sync()
+    t0 = time.perf_counter()
+    rc, n_lock_pcg, _ = bp.run(-1)
+    device_sync()
+    ttc_pcg = time.perf_counter() - t0
+    _, res_pcg = bp.download()
+    itts_pcg = [r[2]["itt"] for r in res_pcg]
+    sig_bpcg = np.asarray([r[2]["sigma_out"][1] for r in res_pcg])
+    pcg_its = [int(r[2]["pcg_iters"]) for r in res_pcg]
+    bp.reset()
+    bp.set_timing(True)
+    bp.run(-1)
+    kt_pcg = {k: bp.kernel_time(k) for k in ["lin_points", "schur_pairs", "schur_diag", "lin_cameras", "bj_inverse",
+                                              "pcg_init", "pcg_iter", "backsub", "decode", "cost", "reweight",
+                                              "block_sums"]}
+    bp.close()
+    progress(f"batched PCG: {ttc_pcg:.3f} s, {n_lock_pcg} lockstep iterations, {sum(itts_pcg)} block iterations")
+
     # the same blocks one after another through the global solve
     def seq(solver):
         tot = 0.0
@@ -563,6 +589,20 @@ def run_batch_bench(args):
         "speedup_vs_sequential_dense": round(seq_dense / ttc, 3),
         "sigma0_rel_vs_sequential_pcg_max": float(np.max(np.abs(sig - np.asarray(sig_pcg)) / np.asarray(sig_pcg))),
         "sigma0_rel_vs_sequential_dense_max": float(np.max(np.abs(sig - np.asarray(sig_dense)) / np.asarray(sig_dense))),
+        "batched_pcg": {
+            "form": "batched blocks with the reference's camera solve: block-Jacobi PCG segmented by block "
+                    "(solver 0), one union SpMV per PCG iteration, hipGraph replay",
+            "time_to_converged_s": round(ttc_pcg, 4),
+            "block_lm_iters_per_s": round(sum(itts_pcg) / ttc_pcg, 2),
+            "lockstep_iterations": n_lock_pcg,
+            "block_itt_total": int(sum(itts_pcg)),
+            "pcg_iterations_total": int(sum(pcg_its)),
+            "speedup_vs_sequential_pcg": round(seq_pcg / ttc_pcg, 3),
+            "sigma0_rel_vs_sequential_pcg_max": float(np.max(np.abs(sig_bpcg - np.asarray(sig_pcg))
+                                                             / np.asarray(sig_pcg))),
+            "kernel_ms_per_solve": {k: round(v[0], 3) for k, v in kt_pcg.items() if v[1] > 0},
+            "pcg_iter_launches": kt_pcg["pcg_iter"][1],
+        },
         "kernel_ms_per_solve": {k: round(v[0], 3) for k, v in kt.items() if v[1] > 0},
         "roofline": {"bound": "mfma", "kernel": "chol_update", "achieved": None if achieved_tf is None else round(achieved_tf, 2),
                      "peak": 78.6, "unit": "TFLOP/s",

@@ -231,11 +231,15 @@ int rpba_dense_solve_batch(int32_t nsys, const int32_t *ncam, const double *S, c
  * GPU, each by the global robust LM of LSA::Adjust (rbacore.cpp:333-616) with
  * its own LM state, as the partitioned mode solves its METIS blocks one LSA
  * per OpenMP thread (rpbacore.cpp:219-235).  Every LM iteration of all blocks
- * runs as one launch sequence over the union of the blocks (replayed from a
- * hipGraph), and the blocks' reduced camera systems are factored together by
- * the batched dense Cholesky (solver = 1 is implied; schur must be 0).  Passes
- * run in lockstep: a block that ends a pass early waits, unchanged.  Each
- * block needs at least 3 images, and all blocks the same image sizes.
+ * runs as one launch sequence over the union of the blocks (replayed from
+ * hipGraphs).  The blocks' reduced camera systems are solved by the
+ * reference's block-Jacobi SparsePCG segmented by block (solver = 0: one SpMV
+ * over the union per PCG iteration, each block with its own scalars, best
+ * iterate, stopping rule and cap max(1000, nrioff)) or factored together by
+ * the batched dense Cholesky (solver = 1).  schur must be 0, precision 0 or 1,
+ * preconditioner 0.  Passes run in lockstep: a block that ends a pass early
+ * waits, unchanged.  Each block needs at least 3 images, and all blocks the
+ * same image sizes.
  * Block results are written back into the blocks by rpba_batch_download;
  * stats and status are arrays of nblocks entries (either may be NULL). */
 typedef struct rpba_batch rpba_batch;

@@ -17,9 +17,19 @@
 // A block's result is therefore exactly what a solve of that block alone
 // computes with the same kernels.
 //
+// The camera systems are solved either by the reference's block-Jacobi PCG
+// (solver 0, SparsePCG of rbacore.cpp:1807-1914), segmented by block: one
+// SpMV over the union S per iteration, each block with its own alpha, beta,
+// best iterate, stopping rule and cap max(1000, nrioff), a block that stops
+// leaving the rest running; or by the batched dense Cholesky (solver 1).
+//
 // The LM iteration's launch sequence is static between pass boundaries (the
-// per-block inputs, iflag and the decode mask, live in device memory), so it
-// is captured once per pass in a hipGraph and replayed.
+// per-block inputs, iflag, the decode mask and the PCG caps, live in device
+// memory), so it is captured once per pass in hipGraphs and replayed: one
+// graph for the whole iteration with the dense solve; with the PCG one for
+// the linearisation through the PCG set-up, one for four PCG iterations
+// (replayed until every block has stopped; launches after a block stopped
+// return at once) and one for the step, decode and trial cost.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +44,7 @@
 #include "dense.h"
 #include "host_util.h"
 #include "kernels.h"
+#include "partition.h"
 #include "structure.h"
 #include "structure_dev.h"
 
@@ -51,8 +62,11 @@ struct BlkState {
     int nc = 0, npts = 0, ntr = 0, nrioff = 0, normcamera = -1, esvmaxm = 0, lflflag = 0;
     rpba_stats stats{};
     std::vector<double> tr_sig0;
-    std::vector<int> tr_acc, tr_nrobs;
+    std::vector<int> tr_acc, tr_nrobs, tr_pcg;
+    int64_t pcg_total = 0;
 };
+
+constexpr int kSegChunk = 4;  // PCG iterations per replayed graph
 
 }  // namespace
 
@@ -98,10 +112,21 @@ struct rpba_batch {
     double *h_bsum = nullptr;  // pinned: 3 per block
     int *h_nanb = nullptr;     // pinned: 1 per block
 
-    // graph replay of the LM iteration
+    // segmented PCG of the blocks' camera systems (solver 0)
+    bool pcg = false;
+    DBuf<double> d_Cinv, d_px, d_best, d_r, d_dv, d_Ad, d_h, d_h1, d_Sdf, d_ev, d_rowpart, d_colpart, d_dpart;
+    DBuf<SegScal> d_segS;
+    DBuf<int> d_ndone;
+    int *h_ndone = nullptr;    // pinned
+    SegScal *h_seg = nullptr;  // pinned: 1 per block
+    int pcg_expect = 0;        // iterations the last lockstep PCG took (its slowest block)
+
+    // graph replay of the LM iteration: [0] the whole iteration (dense solve)
+    // or the part before the PCG, [1] kSegChunk PCG iterations, [2] the part
+    // after the PCG
     bool graph_on = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     bool graph_valid = false;
     int64_t graph_launches = 0;
 
@@ -161,10 +186,12 @@ struct rpba_batch {
     }
     void drop_graph()
     {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        gexec = nullptr;
-        graph = nullptr;
+        for (int i = 0; i < 3; ++i) {
+            if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
+            if (graph[i]) (void)hipGraphDestroy(graph[i]);
+            gexec[i] = nullptr;
+            graph[i] = nullptr;
+        }
         graph_valid = false;
     }
 };
@@ -312,7 +339,9 @@ static int batch_init(rpba_batch *c)
     HIPCHK(hipGetDevice(&c->dev));
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     if (hipHostMalloc((void **)&c->h_bsum, sizeof(double) * 3 * (size_t)nb, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_nanb, sizeof(int) * (size_t)nb, hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void **)&c->h_nanb, sizeof(int) * (size_t)nb, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_ndone, sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_seg, sizeof(SegScal) * (size_t)nb, hipHostMallocDefault) != hipSuccess)
         return RPBA_E_NOMEM;
     const size_t n9 = 9 * (size_t)nc;
     rc = 0;
@@ -337,16 +366,31 @@ static int batch_init(rpba_batch *c)
     rc |= c->d_camblk.alloc((size_t)nc);
     rc |= c->c_Rb.alloc(n9);
     rc |= c->c_Cb.alloc(3 * (size_t)nc);
+    if (c->pcg) {
+        rc |= c->d_Cinv.alloc(81 * (size_t)nc);
+        for (DBuf<double> *v : {&c->d_px, &c->d_best, &c->d_r, &c->d_dv, &c->d_Ad, &c->d_h, &c->d_h1, &c->d_Sdf})
+            rc |= v->alloc(n9);
+        rc |= c->d_ev.alloc((size_t)nc);
+        rc |= c->d_segS.alloc((size_t)nb);
+        rc |= c->d_ndone.alloc(1);
+    }
     if (rc) return RPBA_E_NOMEM;
     rc |= c->d_coff.put(c->coff.data(), c->coff.size(), c->st);
     rc |= c->d_camblk.put(c->camblk_h.data(), c->camblk_h.size(), c->st);
     if (rc) return RPBA_E_NOMEM;
-    std::vector<int> ncam(nb);
-    for (int b = 0; b < nb; ++b) ncam[b] = c->B[b].nc;
-    rc = c->dense.plan(nb, c->coff.data(), ncam.data(), nc);
+    // the PCG's two scalar slots: slot 0's done flag gates the union SpMV
+    PcgScalars z2[2];
+    std::memset(z2, 0, sizeof z2);
+    rc = c->d_pcg.put(z2, 2, c->st);
     if (rc) return rc;
-    rc = c->dense.autotune(c->st);
-    if (rc) return rc;
+    if (!c->pcg) {
+        std::vector<int> ncam(nb);
+        for (int b = 0; b < nb; ++b) ncam[b] = c->B[b].nc;
+        rc = c->dense.plan(nb, c->coff.data(), ncam.data(), nc);
+        if (rc) return rc;
+        rc = c->dense.autotune(c->st);
+        if (rc) return rc;
+    }
     rc = c->ds.upload_tracks(c->s_init, c->U.obs_xy, c->w0_init, c->st);
     if (rc) return rc;
     rc |= c->d_X0.put(c->X_init.data(), c->X_init.size(), c->st);
@@ -376,6 +420,7 @@ static int restore_initial(rpba_batch *c)
     c->stage = 0;
     c->mult = 1.e-3;
     c->iters_total = 0;
+    c->pcg_expect = 0;
     const size_t n = c->X_init.size();
     if (n) {
         HIPCHK(hipMemcpyAsync(c->d_X.p, c->d_X0.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
@@ -455,6 +500,11 @@ static int build_structure(rpba_batch *c)
     const int nbk = std::max(cost_blocks(c->ds.ntr), (int)((np + 255) / 256 + 1));
     rc |= c->d_partial.alloc((size_t)nbk);
     rc |= c->d_ipartial.alloc((size_t)nbk);
+    if (c->pcg) {
+        rc |= c->d_rowpart.alloc(63 * (size_t)std::max(c->ds.nchunk, 1));
+        rc |= c->d_colpart.alloc(9 * (size_t)std::max(c->ds.nslab, 1));
+        rc |= c->d_dpart.alloc((size_t)std::max(spmv_grid(c->ds.nblk), 1));
+    }
     if (rc) return RPBA_E_NOMEM;
     HIPCHK(hipStreamSynchronize(c->st));
     c->structure_dirty = false;
@@ -553,10 +603,38 @@ static CostArgs cost_args(rpba_batch *c)
     return a;
 }
 
-// the launches of one LM iteration (reweighting in pass 0 of a robust solve,
-// CalcBundleSep with the dense solve, back-substitution, decode, trial cost,
-// the per-block sums and NaN flags); recorded into the graph or run eagerly
-static void iteration_launches(rpba_batch *c)
+// the union SpMV of the PCG (S formed by strip slabs) and its per-block
+// segmentation
+static PcgArgs seg_pcg_args(rpba_batch *c)
+{
+    PcgArgs pa;
+    std::memset(&pa, 0, sizeof pa);
+    const DevStruct &d = c->ds;
+    pa.nc = c->nc; pa.msize = 1000; pa.Sd = c->d_Sd.p; pa.Su = c->d_Su.p; pa.Cinv = c->d_Cinv.p; pa.b = c->d_b.p;
+    pa.row_off = d.roff; pa.col_off = d.coff; pa.col_blk = d.cblk; pa.blk_k = d.bk; pa.blk_l = d.bl;
+    pa.x = c->d_px.p; pa.best = c->d_best.p; pa.r = c->d_r.p; pa.d = c->d_dv.p; pa.Ad = c->d_Ad.p;
+    pa.h = c->d_h.p; pa.h1 = c->d_h1.p; pa.part = nullptr;
+    pa.sm = d.sumap(); pa.nslab = d.nslab; pa.nchunk = d.nchunk; pa.nstrip = d.nstrip; pa.slab_chunk = d.slab_chunk;
+    pa.slab_bs = d.slab_bs; pa.slab_l = d.slab_l; pa.slab_mask = d.slab_mask; pa.strip_s0 = d.strip_s0;
+    pa.chunk_off = d.chunk_off; pa.chunk_strip = d.chunk_strip; pa.cs_off = d.cs_off; pa.cs_list = d.cs_list;
+    pa.rowpart = c->d_rowpart.p; pa.colpart = c->d_colpart.p; pa.dpart = c->d_dpart.p; pa.nblk = d.nblk;
+    pa.ndpart = spmv_grid(d.nblk); pa.dsum = c->d_dpart.p; pa.ndsum = pa.ndpart; pa.S = c->d_pcg.p; pa.j = 0;
+    pa.implicit = 0; pa.npts = d.npts; pa.ntr = d.ntr; pa.nch = d.nch;
+    return pa;
+}
+
+static SegArgs seg_args(rpba_batch *c)
+{
+    SegArgs sg;
+    sg.nc = c->nc; sg.nb = c->nb; sg.boff = c->d_coff.p; sg.cblk = c->d_camblk.p; sg.S = c->d_segS.p;
+    sg.ndone = c->d_ndone.p; sg.Sdf = c->d_Sdf.p; sg.e = c->d_ev.p;
+    return sg;
+}
+
+// the launches of one LM iteration before the camera solve's iterations
+// (reweighting in pass 0 of a robust solve, CalcBundleSep), then the dense
+// solve, or the block-Jacobi inverses and the PCG set-up
+static void launches_pre(rpba_batch *c)
 {
     const bool rw = c->pass == 0 && c->fl.robust;
     if (rw) {
@@ -578,11 +656,34 @@ static void iteration_launches(rpba_batch *c)
     c->timed("schur_pairs", [&] { launch_schur_pairs(sa, c->st); });
     c->timed("schur_diag", [&] { launch_schur_diag(la, c->st); });
     c->timed("lin_cameras", [&] { launch_lin_cameras(la, c->st); });
+    if (c->pcg) {
+        c->timed("bj_inverse", [&] { launch_bj_inverse(c->nc, c->d_Sd.p, c->d_Cinv.p, c->st); });
+        const PcgArgs pa = seg_pcg_args(c);
+        const SegArgs sg = seg_args(c);
+        c->timed("pcg_init", [&] { launch_seg_init(pa, sg, c->st); });
+        return;
+    }
     c->timed("dense_assemble", [&] {
         c->dense.assemble(c->d_Sd.p, c->d_Su.p, c->ds.sumap(), c->ds.bk, c->ds.bl, c->ds.nblk, c->d_b.p, c->st);
     });
     c->dense.factor_solve(c->d_sol.p, c->st, [&](const char *nm, auto f) { c->timed(nm, f); });
-    StepArgs sa2 = step_args(c, c->d_sol.p);
+}
+
+// n iterations of the segmented PCG
+static void launches_pcg(rpba_batch *c, int n)
+{
+    const PcgArgs pa = seg_pcg_args(c);
+    const SegArgs sg = seg_args(c);
+    for (int i = 0; i < n; ++i) c->timed("pcg_iter", [&] { launch_seg_iter(pa, sg, c->st); });
+}
+
+// the rest of the iteration: back-substitution, decode, trial cost, the
+// per-block sums and NaN flags
+static void launches_post(rpba_batch *c)
+{
+    const bool rw = c->pass == 0 && c->fl.robust;
+    const double *sol = c->pcg ? c->d_best.p : c->d_sol.p;
+    StepArgs sa2 = step_args(c, sol);
     c->timed("backsub", [&] { launch_backsub(sa2, c->st); });
     launch_copy(c->d_In_bak.p, c->c_In.p, 3 * (size_t)c->nc, c->st);
     c->timed("decode", [&] { launch_decode(sa2, c->st); });
@@ -592,21 +693,62 @@ static void iteration_launches(rpba_batch *c)
     c->timed("block_sums", [&] {
         launch_block_sums(ba, c->d_terms.p, rw ? c->d_wx.p : nullptr, rw ? c->d_wy.p : nullptr,
                           rw ? c->d_flag.p : nullptr, c->d_bsum.p, c->d_bpart.p, c->st);
-        launch_block_nan(ba, c->d_sol.p, c->d_Xn.p, c->d_nanb.p, c->d_npart.p, c->st);
+        launch_block_nan(ba, sol, c->d_Xn.p, c->d_nanb.p, c->d_npart.p, c->st);
     });
+}
+
+static int capture_one(rpba_batch *c, int i, void (*f)(rpba_batch *))
+{
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    f(c);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(c->st, &g));
+    c->graph[i] = g;
+    HIPCHK(hipGraphInstantiate(&c->gexec[i], g, nullptr, nullptr, 0));
+    return RPBA_OK;
 }
 
 static int capture_graph(rpba_batch *c)
 {
     c->drop_graph();
-    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    iteration_launches(c);
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamEndCapture(c->st, &g));
-    c->graph = g;
-    HIPCHK(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    int rc;
+    if (!c->pcg) {
+        rc = capture_one(c, 0, [](rpba_batch *cc) {
+            launches_pre(cc);
+            launches_post(cc);
+        });
+    } else {
+        rc = capture_one(c, 0, launches_pre);
+        if (!rc) rc = capture_one(c, 1, [](rpba_batch *cc) { launches_pcg(cc, kSegChunk); });
+        if (!rc) rc = capture_one(c, 2, launches_post);
+    }
+    if (rc) return rc;
     c->graph_valid = true;
     return RPBA_OK;
+}
+
+// the segmented PCG until every block has stopped: as many iterations as the
+// last lockstep solve took before the first look at the count, then
+// kSegChunk at a time
+static int run_pcg(rpba_batch *c, bool replay)
+{
+    int cap = 0;
+    for (int b = 0; b < c->nb; ++b) cap = std::max(cap, c->h_seg[b].msize);
+    int n = std::max(1, (c->pcg_expect + kSegChunk - 1) / kSegChunk), launched = 0;
+    for (;;) {
+        for (int i = 0; i < n; ++i) {
+            if (replay)
+                HIPCHK(hipGraphLaunch(c->gexec[1], c->st));
+            else
+                launches_pcg(c, kSegChunk);
+        }
+        launched += n * kSegChunk;
+        HIPCHK(hipMemcpyAsync(c->h_ndone, c->d_ndone.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (*c->h_ndone >= c->nb) return RPBA_OK;
+        if (launched > cap + kSegChunk) return RPBA_E_HIP;  // cannot happen: every block stops at its cap
+        n = 1;
+    }
 }
 
 // per-block sums of the trial cost, readback; returns 0
@@ -752,20 +894,46 @@ static int lm_iteration(rpba_batch *c)
     int rc = 0;
     rc |= c->d_iflag.put(ifl.data(), ifl.size(), c->st);
     rc |= c->d_camflag.put(cf.data(), cf.size(), c->st);
-    if (rc) return rc;
-    if (c->graph_on && !c->timing) {
-        if (!c->graph_valid) {
-            rc = capture_graph(c);
-            if (rc) return rc;
+    if (c->pcg) {
+        // the PCG cap of every running block (rbacore.cpp:1848-1849); 0 keeps
+        // a block that sits out from iterating
+        for (int b = 0; b < nb; ++b) {
+            std::memset(&c->h_seg[b], 0, sizeof(SegScal));
+            c->h_seg[b].msize = c->B[b].active ? std::max(1000, c->B[b].nrioff) : 0;
         }
-        HIPCHK(hipGraphLaunch(c->gexec, c->st));
+        rc |= c->d_segS.put(c->h_seg, (size_t)nb, c->st);
+    }
+    if (rc) return rc;
+    const bool replay = c->graph_on && !c->timing;
+    if (replay && !c->graph_valid) {
+        rc = capture_graph(c);
+        if (rc) return rc;
+    }
+    if (replay) {
+        HIPCHK(hipGraphLaunch(c->gexec[0], c->st));
         c->graph_launches++;
     } else {
-        iteration_launches(c);
+        launches_pre(c);
+    }
+    if (c->pcg) {
+        rc = run_pcg(c, replay);
+        if (rc) return rc;
+        if (replay)
+            HIPCHK(hipGraphLaunch(c->gexec[2], c->st));
+        else
+            launches_post(c);
+        HIPCHK(hipMemcpyAsync(c->h_seg, c->d_segS.p, sizeof(SegScal) * (size_t)nb, hipMemcpyDeviceToHost, c->st));
+    } else if (!replay) {
+        launches_post(c);
     }
     HIPCHK(hipMemcpyAsync(c->h_bsum, c->d_bsum.p, sizeof(double) * 3 * (size_t)nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(c->h_nanb, c->d_nanb.p, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    if (c->pcg) {
+        c->pcg_expect = 0;
+        for (int b = 0; b < nb; ++b)
+            if (c->B[b].active) c->pcg_expect = std::max(c->pcg_expect, c->h_seg[b].iters);
+    }
     if (c->timing) c->resolve_timing();
     std::vector<unsigned char> acc(nb, 0);
     bool any_nan = false;
@@ -799,9 +967,12 @@ static int lm_iteration(rpba_batch *c)
             S.sig0old = S.sig0;
             a = 1;
         }
+        const int pit = c->pcg ? c->h_seg[b].iters : 0;
         S.tr_sig0.push_back(S.sig0);
         S.tr_acc.push_back(a);
         S.tr_nrobs.push_back(S.nrobs);
+        S.tr_pcg.push_back(pit);
+        S.pcg_total += pit;
         if (S.itt > 200) {  // rbacore.cpp:575-576
             S.active = false;
             continue;
@@ -848,8 +1019,15 @@ extern "C" void rpba_batch_destroy(rpba_batch *c)
     c->c_Rb.release(); c->c_Cb.release(); c->c_Rn.release(); c->c_Cn.release(); c->c_In.release();
     c->c_th.release(); c->c_thn.release(); c->c_Pst.release(); c->c_Ist.release(); c->c_mask.release();
     c->d_pcg.release();
+    for (DBuf<double> *v : {&c->d_Cinv, &c->d_px, &c->d_best, &c->d_r, &c->d_dv, &c->d_Ad, &c->d_h, &c->d_h1, &c->d_Sdf,
+                            &c->d_ev, &c->d_rowpart, &c->d_colpart, &c->d_dpart})
+        v->release();
+    c->d_segS.release();
+    c->d_ndone.release();
     if (c->h_bsum) (void)hipHostFree(c->h_bsum);
     if (c->h_nanb) (void)hipHostFree(c->h_nanb);
+    if (c->h_ndone) (void)hipHostFree(c->h_ndone);
+    if (c->h_seg) (void)hipHostFree(c->h_seg);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -858,14 +1036,16 @@ extern "C" int rpba_batch_create(rpba_batch **out, int32_t nblocks, rpba_block *
                                  rpba_batch_log_fn log, void *user)
 {
     if (!out || nblocks < 1 || !blocks || !flags) return RPBA_E_INVALID;
-    if (flags->precision < 0 || flags->precision > 1 || flags->schur != 0) return RPBA_E_INVALID;
+    if (flags->precision < 0 || flags->precision > 1 || flags->schur != 0 || flags->solver < 0 || flags->solver > 1 ||
+        flags->preconditioner != 0)
+        return RPBA_E_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return RPBA_E_HIP;
     rpba_batch *c = new rpba_batch();
     c->nb = nblocks;
     c->io = blocks;
     c->fl = *flags;
-    c->fl.solver = 1;  // the blocks' systems are solved together by the batched dense Cholesky
+    c->pcg = flags->solver == 0;
     c->log = log;
     c->user = user;
     int rc = batch_init(c);
@@ -991,7 +1171,7 @@ extern "C" int rpba_batch_download(rpba_batch *c, rpba_stats *stats, int32_t *st
             rpba_stats keep = st;
             st = S.stats;
             st.itt = S.itt;
-            st.pcg_iters = 0;
+            st.pcg_iters = S.pcg_total;
             st.sig0_final = S.sig0old;
             st.rms_px = sn[bb] ? std::sqrt(ss[bb] / (double)(2 * sn[bb])) : 0.;
             st.n_obs_kept = kept_obs[bb];
@@ -1006,7 +1186,7 @@ extern "C" int rpba_batch_download(rpba_batch *c, rpba_stats *stats, int32_t *st
             for (int j = 0; j < st.trace_len; ++j) {
                 st.trace_sig0[j] = S.tr_sig0[j];
                 if (st.trace_accept) st.trace_accept[j] = S.tr_acc[j];
-                if (st.trace_pcg) st.trace_pcg[j] = 0;
+                if (st.trace_pcg) st.trace_pcg[j] = S.tr_pcg[j];
                 if (st.trace_nrobs) st.trace_nrobs[j] = S.tr_nrobs[j];
             }
         }
@@ -1043,6 +1223,10 @@ extern "C" int rpba_batch_tuning(rpba_batch *c, char *buf, int32_t cap)
 {
     if (!c || !buf || cap < 1) return RPBA_E_INVALID;
     char extra[96];
+    if (c->pcg) {
+        std::snprintf(buf, (size_t)cap, "segmented block-Jacobi PCG; graph launches %lld", (long long)c->graph_launches);
+        return RPBA_OK;
+    }
     std::snprintf(extra, sizeof extra, "; graph launches %lld; flops/factorisation %.3g",
                   (long long)c->graph_launches, c->dense.flops());
     std::snprintf(buf, (size_t)cap, "%s%s", c->dense.tuning().c_str(), extra);

@@ -336,13 +336,43 @@ void launch_lsax(const LsaxArgs &a, hipStream_t st)
 // block-segmented PCG
 // ---------------------------------------------------------------------------
 
-// per block: x = 0, r = b, best = b, h = C r, d = h, rho = r.h (rbacore.cpp:1829-1847)
+// the done counter and the all-done flag of the union SpMV cleared (a kernel,
+// so the set-up can be recorded into a graph)
+__global__ void k_seg_reset(PcgArgs a, SegArgs s)
+{
+    *s.ndone = 0;
+    a.S[0].done = 0;
+}
+
+// per block: x = 0, r = b, best = b, h = C r, d = h, rho = r.h (rbacore.cpp:1829-1847).
+// A block whose msize the host set to 0 does not run: it enters done with a
+// zero step (x = best = d = 0) and counts as stopped.  The block that makes the
+// count reach nb raises a.S[0].done, which ends the union SpMV launches that
+// follow.
 __global__ __launch_bounds__(256) void k_seg_init(PcgArgs a, SegArgs s)
 {
     __shared__ double sh[4];
     const int bb = blockIdx.x;
     const int c0 = s.boff[bb], c1 = s.boff[bb + 1];
     const size_t j0 = 9 * (size_t)c0, j1 = 9 * (size_t)c1;
+    if (s.S[bb].msize <= 0) {
+        for (size_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+            a.x[j] = 0.;
+            a.best[j] = 0.;
+            a.d[j] = 0.;
+        }
+        if (threadIdx.x == 0) {
+            SegScal z = s.S[bb];
+            z.rho = 0.;
+            z.m0 = z.mold = 1.e50;
+            z.k = 0;
+            z.done = 1;
+            z.iters = 0;
+            s.S[bb] = z;
+            if (atomicAdd(s.ndone, 1) + 1 == s.nb) a.S[0].done = 1;
+        }
+        return;
+    }
     for (size_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
         a.x[j] = 0.;
         a.r[j] = a.b[j];
@@ -477,13 +507,14 @@ __global__ __launch_bounds__(1024) void k_seg_step(PcgArgs a, SegArgs s)
     __syncthreads();
     if (threadIdx.x == 0) {
         s.S[bb] = z;
-        if (z.done) atomicAdd(s.ndone, 1);
+        if (z.done && atomicAdd(s.ndone, 1) + 1 == s.nb) a.S[0].done = 1;
     }
 }
 
 void launch_seg_init(const PcgArgs &a, const SegArgs &s, hipStream_t st)
 {
     if (s.nb <= 0) return;
+    k_seg_reset<<<1, 1, 0, st>>>(a, s);
     k_seg_init<<<s.nb, 256, 0, st>>>(a, s);
 }
 

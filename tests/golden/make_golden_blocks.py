@@ -2,10 +2,12 @@
 solve, solver = 1) of each of the 64 independent blocks of the batched-blocks
 workload (scenes.BATCH_CONFIG: 20-400 cameras log-uniform, 8.5M observations
 in all), one block at a time, as the reference solves its partition blocks one
-LSA per thread (rpbacore.cpp:219-235).  The GPU's batched solve of all blocks
-at once is held to these records (tests/test_gpu_c4_blocks.py).
+LSA per thread (rpbacore.cpp:219-235); with the argument pcg, c4_blocks_pcg.json:
+the same with the reference's camera solve, block-Jacobi SparsePCG
+(rbacore.cpp:1807-1914, solver = 0).  The GPU's batched solve of all blocks at
+once is held to these records (tests/test_gpu_c4_blocks.py).
 
-    python tests/golden/make_golden_blocks.py [workers]
+    python tests/golden/make_golden_blocks.py [workers] [pcg]
 """
 import json
 import os
@@ -25,6 +27,9 @@ from oracle import oracle  # noqa: E402
 from rpba_amd import capi, scenes  # noqa: E402
 
 FLAGS = dict(robust=1, k2=1, k4=1, f=1, threads=1, solver=1)
+if len(sys.argv) > 2 and sys.argv[2] == "pcg":
+    FLAGS["solver"] = 0
+OUT = "c4_blocks.json" if FLAGS["solver"] == 1 else "c4_blocks_pcg.json"
 
 
 def one(b):
@@ -60,5 +65,5 @@ if __name__ == "__main__":
     with Pool(workers) as pool:
         recs = pool.map(one, order, chunksize=1)
     recs.sort(key=lambda r: r["block"])
-    with open(os.path.join(HERE, "c4_blocks.json"), "w") as fh:
+    with open(os.path.join(HERE, OUT), "w") as fh:
         json.dump({"config": scenes.BATCH_CONFIG, "flags": FLAGS, "blocks": recs}, fh)

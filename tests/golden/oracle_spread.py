@@ -6,6 +6,9 @@ rounding spread two correct FP64 implementations of the reference can show,
 the basis of the trajectory bar in tests/tolerance.py.
 
     python tests/golden/oracle_spread.py [case ...]     # writes spread_<case>.json
+
+A case c4_block<b> is block b of the batched-blocks workload
+(scenes.BATCH_CONFIG) with the reference's camera solve, SparsePCG.
 """
 import json
 import os
@@ -30,6 +33,14 @@ def case_spec(case):
     return make_golden.CASES[case] if case in make_golden.CASES else make_golden.LARGE[case]
 
 
+def case_scene_flags(case):
+    if case.startswith("c4_block"):
+        sc = scenes.batch_block(int(case[len("c4_block"):]), **scenes.BATCH_CONFIG)
+        return sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, solver=0)
+    c = case_spec(case)
+    return make_golden.make_scene(c["scene"]), capi.make_flags(**c["flags"])
+
+
 def solve(sc, flags, order):
     with oracle.order(order):
         t = time.time()
@@ -38,9 +49,7 @@ def solve(sc, flags, order):
 
 
 def measure(case, variants=(1, 2, 3)):
-    c = case_spec(case)
-    sc = make_golden.make_scene(c["scene"])
-    flags = capi.make_flags(**c["flags"])
+    sc, flags = case_scene_flags(case)
     rc0, out0, st0, dt0 = solve(sc, flags, 0)
     res = {"case": case, "base": {"itt": st0["itt"], "trace_sig0": [float(x) for x in st0["trace_sig0"]],
                                   "trace_pcg": [int(x) for x in st0["trace_pcg"]],
@@ -58,6 +67,10 @@ def measure(case, variants=(1, 2, 3)):
             "same_labels": bool(np.array_equal(out["obs_kept"], out0["obs_kept"])),
             "sigma_out_rel": [abs(x - y) / y for x, y in zip(st["sigma_out"], st0["sigma_out"])],
             "rms_rel": abs(scenes.rms_px(sc, out) - scenes.rms_px(sc, out0)) / scenes.rms_px(sc, out0),
+            "cam_f_rel_max": float(np.max(np.abs(out["cam_f"].astype(np.float64) - out0["cam_f"])
+                                          / np.abs(out0["cam_f"].astype(np.float64)))),
+            "pts_diff_max_over_extent": float(np.abs(out["pts"].astype(np.float64) - out0["pts"]).max()
+                                              / np.abs(out0["pts"].astype(np.float64)).max()),
             "seconds": round(dt, 1)}
         print(case, "variant", v, "max rel sig0 %.3g" % max(rel), "pcg", res["variants"][str(v)]["trace_pcg"],
               flush=True)

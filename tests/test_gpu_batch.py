@@ -1,7 +1,8 @@
 """Batched independent blocks (rpba_batch_*): every block's result against
-the oracle's solve of that block alone with the same exact camera solve
-(solver = 1), under the trajectory rule of the parity tests; hipGraph replay
-against eager launches (bitwise); reset and re-run (bitwise)."""
+the oracle's solve of that block alone with the same camera solve (solver 1,
+exact; solver 0, the reference's block-Jacobi PCG segmented by block), under
+the trajectory rule of the parity tests; hipGraph replay against eager
+launches (bitwise); reset and re-run (bitwise)."""
 import numpy as np
 import pytest
 
@@ -106,6 +107,91 @@ def test_batch_fp32_records_match_single_block_solves():
     for i, (sc, (rc1, out1, st1, log1)) in enumerate(zip(blocks, res)):
         rc0, out0, st0, log0 = api.calc_plsa(sc, flags)
         print(f"[batch fp32 block {i}] itt {st0['itt']} {st1['itt']}", flush=True)
+        assert rc1 == rc0 == 0 and st1["itt"] == st0["itt"]
+        check_trajectory(st1["trace_sig0"], st0["trace_sig0"], st1["trace_pcg"], st0["trace_pcg"],
+                         st1["trace_accept"], st0["trace_accept"], st1["trace_nrobs"], st0["trace_nrobs"])
+        assert np.array_equal(out1["obs_kept"], out0["obs_kept"])
+
+
+def _compare_block_pcg(sc, flags, got, tag):
+    """The PCG form of a block against the oracle's SparsePCG solve of it: the
+    bars of the single-block golden tests (trajectory rule, converged sigma
+    within 1e-6, labels bit for bit, point mean and focal lengths)."""
+    rc1, out1, st1, log1 = got
+    rc0, out0, st0, log0 = oracle.solve(sc, flags, fsites=0)
+    print(f"[{tag}] n_cams={sc['n_cams']} itt oracle {st0['itt']} batch {st1['itt']} pcg oracle "
+          f"{list(st0['trace_pcg'])} batch {list(st1['trace_pcg'])}", flush=True)
+    assert rc1 == rc0
+    assert st1["passes_done"] == st0["passes_done"] and st1["itt"] == st0["itt"]
+    check_trajectory(st1["trace_sig0"], st0["trace_sig0"], st1["trace_pcg"], st0["trace_pcg"],
+                     st1["trace_accept"], st0["trace_accept"], st1["trace_nrobs"], st0["trace_nrobs"])
+    assert st1["pcg_iters"] == int(np.sum(st1["trace_pcg"])) and st1["pcg_iters"] > 0
+    for p in range(st0["passes_done"]):
+        assert abs(st1["sigma_out"][p] - st0["sigma_out"][p]) <= 1e-6 * abs(st0["sigma_out"][p])
+        for key in ("redundancy", "nobs", "nunknowns"):
+            assert st1[key][p] == st0[key][p]
+    assert np.array_equal(out1["obs_kept"], out0["obs_kept"])
+    assert np.array_equal(out1["pt_kept"], out0["pt_kept"])
+    assert np.allclose(out1["pts"].astype(np.float64).mean(axis=0), out0["pts"].astype(np.float64).mean(axis=0),
+                       rtol=1e-5, atol=1e-6)
+    assert np.allclose(out1["cam_f"], out0["cam_f"], rtol=1e-6)
+    assert len(log1) == len(log0)
+
+
+@pytest.mark.parametrize("robust", [1, 0])
+def test_batch_pcg_blocks_match_oracle(robust):
+    """solver 0: one SpMV over the union per PCG iteration, each block with its
+    own alpha, beta, best iterate, stopping rule and cap; blocks that stop
+    early (and blocks that sit out the rest of a pass) leave the others
+    running.  Each block against the oracle's PCG solve of it alone."""
+    sizes = [(8, 300), (20, 900), (5, 200), (33, 1500), (12, 500), (3, 120)]
+    blocks = _blocks(600 + 10 * robust, sizes)
+    flags = capi.make_flags(robust=robust, solver=0)
+    print(f"[batch pcg robust={robust}] {len(blocks)} blocks", flush=True)
+    res = api.calc_plsa_batch(blocks, flags)
+    for i, (sc, got) in enumerate(zip(blocks, res)):
+        _compare_block_pcg(sc, flags, got, f"pcg block {i}")
+
+
+def test_batch_pcg_graph_replay_equals_eager_and_reset():
+    """The three graphs of the PCG form (set-up, four PCG iterations, step)
+    replayed against eager launches, bitwise, and a reset re-run."""
+    blocks = _blocks(760, [(10, 400), (25, 1200), (6, 250), (40, 2000)])
+    flags = capi.make_flags(robust=1, solver=0)
+    b = api.Batch(blocks, flags)
+    print("[batch pcg graph] tuning:", b.tuning(), flush=True)
+    rc, n, fin = b.run(-1)
+    assert rc == 0 and fin
+    rd, g1 = b.download()
+    assert "segmented" in b.tuning() and "graph launches 0" not in b.tuning()
+    b.reset()
+    b.set_graph(False)
+    rc, n2, fin = b.run(-1)
+    assert rc == 0 and fin and n2 == n
+    rd, e1 = b.download()
+    b.reset()
+    b.set_graph(True)
+    rc, n3, fin = b.run(-1)
+    rd, g2 = b.download()
+    b.close()
+    for (s1, o1, t1, _), (s2, o2, t2, _), (s3, o3, t3, _) in zip(g1, e1, g2):
+        assert s1 == s2 == s3 == 0
+        for key in ("cam_P", "cam_f", "cam_d", "pts", "obs_kept", "pt_kept"):
+            assert np.array_equal(o1[key], o2[key]) and np.array_equal(o1[key], o3[key]), key
+        for key in ("trace_sig0", "trace_pcg"):
+            assert np.array_equal(t1[key], t2[key]) and np.array_equal(t1[key], t3[key]), key
+
+
+def test_batch_pcg_matches_single_block_solves():
+    """Each block of the segmented PCG against the single-block GPU solve of
+    the same block (solver 0), under the trajectory rule."""
+    blocks = _blocks(1300, [(10, 400), (22, 900), (6, 250), (30, 1400)])
+    flags = capi.make_flags(robust=1, solver=0)
+    res = api.calc_plsa_batch(blocks, flags)
+    for i, (sc, (rc1, out1, st1, log1)) in enumerate(zip(blocks, res)):
+        rc0, out0, st0, log0 = api.calc_plsa(sc, flags)
+        print(f"[batch pcg vs single {i}] itt {st0['itt']} {st1['itt']} pcg {list(st0['trace_pcg'])} "
+              f"{list(st1['trace_pcg'])}", flush=True)
         assert rc1 == rc0 == 0 and st1["itt"] == st0["itt"]
         check_trajectory(st1["trace_sig0"], st0["trace_sig0"], st1["trace_pcg"], st0["trace_pcg"],
                          st1["trace_accept"], st0["trace_accept"], st1["trace_nrobs"], st0["trace_nrobs"])

@@ -68,3 +68,44 @@ def test_bar_matches_the_oracles_own_rounding_spread(scene_c1):
     assert np.array_equal(o1["obs_kept"], o0["obs_kept"])
     for p in range(2):
         assert abs(s1["sigma_out"][p] - s0["sigma_out"][p]) <= 1e-6 * s0["sigma_out"][p]
+
+
+def test_converged_tail_lifts_only_the_count_bar():
+    sig = np.array([4.0, 2.0, 1.0, 0.5, 0.25, 0.25000001])  # last step moves sig0 by 4e-8
+    p = PCG.copy()
+    p[-1] += 7
+    with pytest.raises(AssertionError):
+        check_trajectory(sig, sig, p, PCG)
+    check_trajectory(sig, sig, p, PCG, converged_rel=1e-6)
+    q = PCG.copy()
+    q[3] += 2  # an iteration that still moves sig0
+    with pytest.raises(AssertionError):
+        check_trajectory(sig, sig, q, PCG, converged_rel=1e-6)
+    s = sig.copy()
+    s[-1] *= 1 + 2e-5  # the sig0 bar still holds on the converged iteration
+    with pytest.raises(AssertionError):
+        check_trajectory(s, sig, p, PCG, converged_rel=1e-6)
+
+
+def test_converged_tail_rule_basis():
+    """The oracle against itself on block 6 of the batched-blocks workload
+    with the reference's SparsePCG, only the summation order changed: the
+    last LM iteration moves sig0 by less than 1e-6, and there the PCG stops 7
+    iterations apart; every other count is equal and sig0 agrees within 1e-8."""
+    from oracle import oracle
+    from rpba_amd import capi, scenes
+    sc = scenes.batch_block(6, **scenes.BATCH_CONFIG)
+    fl = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, solver=0)
+    _, o0, s0, _ = oracle.solve(sc, fl)
+    with oracle.order(1):
+        _, o1, s1, _ = oracle.solve(sc, fl)
+    d = np.abs(np.asarray(s1["trace_pcg"]) - np.asarray(s0["trace_pcg"]))
+    assert d[-1] > 1 and d[:-1].max() == 0
+    sig = np.asarray(s0["trace_sig0"])
+    assert abs(sig[-1] - sig[-2]) < 1e-6 * sig[-1]
+    with pytest.raises(AssertionError):
+        check_trajectory(s1["trace_sig0"], s0["trace_sig0"], s1["trace_pcg"], s0["trace_pcg"])
+    assert check_trajectory(s1["trace_sig0"], s0["trace_sig0"], s1["trace_pcg"], s0["trace_pcg"],
+                            s1["trace_accept"], s0["trace_accept"], s1["trace_nrobs"], s0["trace_nrobs"],
+                            converged_rel=1e-6) < 1e-8
+    assert np.array_equal(o1["obs_kept"], o0["obs_kept"])

@@ -14,7 +14,18 @@ decisions, PCG counts, labels and a converged sigma0 within 1e-9.  The rule:
   * the first LM iteration (one PCG from identical inputs) within 1e-8,
   * every iteration within 1e-5,
   * PCG iteration counts within one, accept decisions and nrobs exactly.
-Converged sigmas are checked separately at the FP64 bar of 1e-6."""
+Converged sigmas are checked separately at the FP64 bar of 1e-6.
+
+On a converged LM iteration (one that moves sig0 by less than about 1e-6) the
+PCG's step-norm ratio can hover near the 1e-5 threshold for several
+iterations, and rounding alone then moves the stopping point by more than one.
+Measured on the oracle against itself (tests/golden/spread_c4_block6.json,
+block 6 of the batched-blocks workload, SparsePCG): only the summation order
+changed, the last LM iteration (sig0 change 6.1e-7) takes 40 PCG iterations
+instead of 33, every other count equal, sig0 within 1.8e-9, labels equal.
+With converged_rel set, the count bar is lifted on iterations whose
+reference sig0 changed by less than converged_rel from the one before; the
+sig0 bars still hold there."""
 import numpy as np
 
 FIRST = 1e-8
@@ -22,13 +33,18 @@ TRAJECTORY = 1e-5
 
 
 def check_trajectory(sig, sig_ref, pcg, pcg_ref, accept=None, accept_ref=None, nrobs=None, nrobs_ref=None,
-                     pcg_slack=1):
+                     pcg_slack=1, converged_rel=None):
     """pcg_slack: the PCG count difference allowed per iteration (1 per PCG
-    solve; the partitioned solve sums one solve per part)"""
+    solve; the partitioned solve sums one solve per part).  converged_rel:
+    no count bar on iterations whose reference sig0 changed by less than this
+    (relative) from the iteration before."""
     sig, sig_ref = np.asarray(sig, dtype=np.float64), np.asarray(sig_ref, dtype=np.float64)
     pcg, pcg_ref = np.asarray(pcg), np.asarray(pcg_ref)
     assert sig.size == sig_ref.size and pcg.size == pcg_ref.size == sig.size
-    assert np.abs(pcg - pcg_ref).max(initial=0) <= pcg_slack, (pcg, pcg_ref)
+    held = np.ones(sig.size, dtype=bool)
+    if converged_rel is not None and sig.size > 1:
+        held[1:] = np.abs(np.diff(sig_ref)) >= converged_rel * np.abs(sig_ref[1:])
+    assert np.abs(pcg - pcg_ref)[held].max(initial=0) <= pcg_slack, (pcg, pcg_ref)
     rel = np.abs(sig - sig_ref) / np.abs(sig_ref)
     if rel.size:
         assert rel[0] < FIRST, (rel, pcg, pcg_ref)
