@@ -32,7 +32,9 @@ roofline of the dominant kernel (S d); the CPU oracle timed on a bounded
 sample of one LM iteration of the same scene (cpu_baseline); and the
 partitioned solve (Threads = 0 dispatch) of the 961-, 4,585- and
 13,682-camera scenes, each as a whole rpba_calc_plsa call beside the oracle's
-OpenMP restatement of it.
+OpenMP restatement of it; and the batched blocks (64 blocks of 20-400
+cameras in one batched solve with the block-segmented PCG, against the blocks
+one after another; --config c4_batch gives that leg's full line).
 """
 import argparse
 import ctypes as C
@@ -409,6 +411,13 @@ def run_bench(args):
         line["cpu_baseline"] = cpu_baseline(sc, fsz, k_avg, args.config)
     if args.partition_config != "none":
         line["partitioned"] = [partitioned_leg(cfg, args.no_cpu_baseline) for cfg in args.partition_config.split(",")]
+    if not args.no_batch:
+        b = batch_leg(sequential_dense=False, no_cpu=True)
+        line["batched_blocks"] = {k: b[k] for k in ("metric", "form", "value", "unit", "time_to_converged_s",
+                                                    "sequential_pcg_s", "speedup_vs_sequential_pcg",
+                                                    "sigma0_rel_vs_sequential_pcg_max", "roofline")}
+        line["batched_blocks"]["workload"] = b["config"]["workload"]
+        line["batched_blocks"]["dense_form_time_to_converged_s"] = b["batched_dense"]["time_to_converged_s"]
     print(json.dumps(line), flush=True)
 
 
@@ -460,12 +469,14 @@ def partitioned_leg(config, no_cpu):
     return res
 
 
-def run_batch_bench(args):
+def batch_leg(no_sequential=False, no_cpu=False, sequential_dense=True, only=None):
     """Batched blocks: 64 independent blocks (20-400 cameras, log-uniform) in
     one batched solve (one launch sequence per LM iteration over all blocks,
-    hipGraph replay, batched FP64-MFMA dense Cholesky), against the same
-    blocks solved one after another by the global solve (block-Jacobi PCG,
-    the reference's per-block algorithm, and the dense form)."""
+    hipGraph replay), with the reference's camera solve, block-Jacobi PCG
+    segmented by block (solver 0), and with the batched FP64-MFMA dense
+    Cholesky (solver 1); against the same blocks solved one after another by
+    the global solve (the same two camera solves).  only = 0 or 1: that
+    camera solve alone (profiling runs).  Returns the line."""
     import numpy as np
 
     import rpba_amd
@@ -475,79 +486,63 @@ def run_batch_bench(args):
     t0 = time.time()
     blocks = scenes.generate_batch(**{k: v for k, v in scenes.BATCH_CONFIG.items()})
     gen_s = time.time() - t0
-    flags = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, solver=1)
-    t0 = time.time()
-    b = api.Batch(blocks, flags)
-    create_s = time.time() - t0
-    rc, n_lock, fin = b.run(-1)  # warm-up: one whole solve (code loading, graph capture)
-    if rc != 0:
-        raise RuntimeError(f"rpba_batch_run failed with status {rc}")
-    b.reset()
-    device_sync()
-    t0 = time.perf_counter()
-    rc, n_lock, fin = b.run(-1)
-    device_sync()
-    ttc = time.perf_counter() - t0
-    rd, res = b.download()
-    itts = [r[2]["itt"] for r in res]
-    sig = [r[2]["sigma_out"][1] for r in res]
-    # per-kernel times (eager launches with HIP events)
-    b.reset()
-    b.set_timing(True)
-    b.run(-1)
-    kern = ["lin_points", "schur_pairs", "schur_diag", "lin_cameras", "dense_assemble", "chol_potrf", "chol_trsm",
-            "chol_update", "chol_solve", "backsub", "decode", "cost", "reweight", "block_sums"]
-    kt = {k: b.kernel_time(k) for k in kern}
-    b.set_timing(False)
-    tuning = b.tuning()
-    sizes = b.sizes()
-    b.close()
-    # FP64 MFMA rate of the trailing updates: flops of one factorisation's
-    # updates (sum over systems of the tile products) / their time
-    T = 64
-    upd_flops = 0.0
-    for sc in blocks:
-        nt = (9 * sc["n_cams"] + T - 1) // T
-        for k in range(nt):
-            m = nt - k - 1  # tile columns right of the panel
-            upd_flops += 2.0 * T ** 3 * (m * (m + 1) / 2 + m)  # lower triangle + the rhs tile row
-    ms_upd, n_upd = kt["chol_update"]
-    n_fact = kt["chol_solve"][1]  # factorisations in the timed solve (one backward sweep each)
-    achieved_tf = upd_flops * n_fact / (ms_upd * 1e-3) / 1e12 if ms_upd > 0 else None
+    base = dict(robust=1, k2=1, k4=1, f=1, threads=1)
 
-    # the same batch with the reference's camera solve: block-Jacobi PCG
-    # segmented by block (one union SpMV per PCG iteration)
-    fpcg = capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, solver=0)
-    bp = api.Batch(blocks, fpcg)
-    rc, _, _ = bp.run(-1)
-    if rc != 0:
-        raise RuntimeError(f"rpba_batch_run (PCG) failed with status {rc}")
-    bp.reset()
-    device_sync()
-    t0 = time.perf_counter()
-    rc, n_lock_pcg, _ = bp.run(-1)
-    device_sync()
-    ttc_pcg = time.perf_counter() - t0
-    _, res_pcg = bp.download()
-    itts_pcg = [r[2]["itt"] for r in res_pcg]
-    sig_bpcg = np.asarray([r[2]["sigma_out"][1] for r in res_pcg])
-    pcg_its = [int(r[2]["pcg_iters"]) for r in res_pcg]
-    bp.reset()
-    bp.set_timing(True)
-    bp.run(-1)
-    kt_pcg = {k: bp.kernel_time(k) for k in ["lin_points", "schur_pairs", "schur_diag", "lin_cameras", "bj_inverse",
-                                              "pcg_init", "pcg_iter", "backsub", "decode", "cost", "reweight",
-                                              "block_sums"]}
-    bp.close()
-    progress(f"batched PCG: {ttc_pcg:.3f} s, {n_lock_pcg} lockstep iterations, {sum(itts_pcg)} block iterations")
+    def batched(solver, kern):
+        t0 = time.time()
+        b = api.Batch(blocks, capi.make_flags(**base, solver=solver))
+        create_s = time.time() - t0
+        rc, _, _ = b.run(-1)  # warm-up: one whole solve (code loading, graph capture)
+        if rc != 0:
+            raise RuntimeError(f"rpba_batch_run (solver {solver}) failed with status {rc}")
+        b.reset()
+        device_sync()
+        t0 = time.perf_counter()
+        rc, n_lock, _ = b.run(-1)
+        device_sync()
+        ttc = time.perf_counter() - t0
+        _, res = b.download()
+        # per-kernel times: a solve with HIP events around every launch
+        b.reset()
+        b.set_timing(True)
+        b.run(-1)
+        kt = {k: b.kernel_time(k) for k in kern}
+        b.set_timing(False)
+        tuning = b.tuning()
+        # the structure of pass 0 (the one most launches ran on)
+        b.reset()
+        b.run(1)
+        sizes = b.sizes()
+        b.close()
+        progress(f"batched solver {solver}: {ttc:.3f} s, {n_lock} lockstep iterations")
+        return {"ttc": ttc, "n_lock": n_lock, "res": res, "kt": kt, "tuning": tuning, "sizes": sizes,
+                "create_s": create_s}
+
+    common = ["lin_points", "schur_pairs", "schur_diag", "lin_cameras", "backsub", "decode", "cost", "reweight",
+              "block_sums"]
+    P = batched(0, common + ["bj_inverse", "pcg_init", "spmv", "seg_update"]) if only != 1 else None
+    D = batched(1, common + ["dense_assemble", "chol_potrf", "chol_trsm", "chol_update", "chol_solve"]) \
+        if only != 0 else None
+    if only is not None:
+        R = P if only == 0 else D
+        return {"profiling_run": f"batched blocks, solver {only} alone", "time_to_converged_s": round(R["ttc"], 4),
+                "lockstep_iterations": R["n_lock"],
+                "kernel_ms_per_solve": {k: round(v[0], 3) for k, v in R["kt"].items() if v[1] > 0},
+                "kernel_launches": {k: v[1] for k, v in R["kt"].items() if v[1] > 0}, "sizes": R["sizes"]}
+
+    def summary(R):
+        itts = [r[2]["itt"] for r in R["res"]]
+        return itts, np.asarray([r[2]["sigma_out"][1] for r in R["res"]]), sum(itts)
+
+    itts_p, sig_p, it_p = summary(P)
+    itts_d, sig_d, it_d = summary(D)
+    pcg_its = int(sum(int(r[2]["pcg_iters"]) for r in P["res"]))
 
     # the same blocks one after another through the global solve
     def seq(solver):
-        tot = 0.0
-        its = 0
-        sigs = []
+        tot, sigs = 0.0, []
         for sc in blocks:
-            ctx = api.Context(sc, capi.make_flags(robust=1, k2=1, k4=1, f=1, threads=1, solver=solver))
+            ctx = api.Context(sc, capi.make_flags(**base, solver=solver))
             ctx.run(-1)
             ctx.reset()
             device_sync()
@@ -556,64 +551,97 @@ def run_batch_bench(args):
             device_sync()
             tot += time.perf_counter() - t1
             _, _, st = ctx.download()
-            its += st["itt"]
             sigs.append(st["sigma_out"][1])
             ctx.close()
-        return tot, its, sigs
+        progress(f"sequential solver {solver}: {tot:.3f} s")
+        return tot, np.asarray(sigs)
 
-    if args.no_sequential:
-        seq_pcg = seq_dense = float("nan")
-        sig_pcg = sig_dense = sig
-    else:
-        seq_pcg, its_pcg, sig_pcg = seq(0)
-        seq_dense, its_dense, sig_dense = seq(1)
-    sig = np.asarray(sig)
+    nan = float("nan")
+    seq_pcg, sig_sp = (nan, sig_p) if no_sequential else seq(0)
+    seq_dense, sig_sd = (nan, sig_d) if (no_sequential or not sequential_dense) else seq(1)
+
+    # roofline of the batched PCG's dominant kernel, the union SpMV: S blocks
+    # once, d in, Ad out, row partials (HIP-event time of its working launches)
+    ms_spmv, n_spmv = P["kt"]["spmv"]
+    alg = algorithmic_bytes("spmv", P["sizes"])
+    achieved = alg / (ms_spmv / n_spmv * 1e-3) / 1e9 if n_spmv else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_spmv_c4_batch.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    # FP64 MFMA rate of the dense form's trailing updates: flops of one
+    # factorisation's updates (sum over systems of the tile products) / time
+    T = 64
+    upd_flops = 0.0
+    for sc in blocks:
+        nt = (9 * sc["n_cams"] + T - 1) // T
+        for k in range(nt):
+            m = nt - k - 1  # tile columns right of the panel
+            upd_flops += 2.0 * T ** 3 * (m * (m + 1) / 2 + m)  # lower triangle + the rhs tile row
+    ms_upd, n_upd = D["kt"]["chol_update"]
+    n_fact = D["kt"]["chol_solve"][1]  # factorisations in the timed solve (one backward sweep each)
+    upd_tf = upd_flops * n_fact / (ms_upd * 1e-3) / 1e12 if ms_upd > 0 else None
+    sz = P["sizes"]
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b) / b))
+
     line = {
         "metric": "time to converged, 64 independent blocks (batched form vs one block after another)",
-        "form": "batched blocks: union launch sequence per LM iteration, hipGraph replay, batched FP64-MFMA dense "
-                "Cholesky of every block's reduced camera system",
-        "value": round(sum(itts) / ttc, 2),
+        "form": "batched blocks with the reference's camera solve: union launch sequence per LM iteration, "
+                "block-Jacobi PCG segmented by block (one union SpMV per PCG iteration, per-block scalars, "
+                "stopping rule and cap), hipGraph replay",
+        "value": round(it_p / P["ttc"], 2),
         "unit": "block LM iters/s",
         "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded generator, BAL camera model)",
         "config": {"workload": "c4_batch: 64 blocks, 20-400 cameras log-uniform, 150 points per camera "
                                "(2k-60k), track 2 + geometric mean 6, robust, k2 k4 f estimated",
-                   "blocks": len(blocks), "cameras": sizes["cams"], "points": sizes["pts"],
-                   "observations": sizes["obs"], "dense_doubles": sizes["dense_doubles"]},
-        "time_to_converged_s": round(ttc, 4),
-        "lockstep_iterations": n_lock,
-        "block_itt_total": int(sum(itts)),
+                   "blocks": len(blocks), "cameras": sz["cams"], "points": sz["pts"], "observations": sz["obs"],
+                   "s_blocks_pass0": sz["blocks"]},
+        "time_to_converged_s": round(P["ttc"], 4),
+        "lockstep_iterations": P["n_lock"],
+        "block_itt_total": int(it_p),
+        "pcg_iterations_total": pcg_its,
         "sequential_pcg_s": round(seq_pcg, 4),
-        "sequential_dense_s": round(seq_dense, 4),
-        "speedup_vs_sequential_pcg": round(seq_pcg / ttc, 3),
-        "speedup_vs_sequential_dense": round(seq_dense / ttc, 3),
-        "sigma0_rel_vs_sequential_pcg_max": float(np.max(np.abs(sig - np.asarray(sig_pcg)) / np.asarray(sig_pcg))),
-        "sigma0_rel_vs_sequential_dense_max": float(np.max(np.abs(sig - np.asarray(sig_dense)) / np.asarray(sig_dense))),
-        "batched_pcg": {
-            "form": "batched blocks with the reference's camera solve: block-Jacobi PCG segmented by block "
-                    "(solver 0), one union SpMV per PCG iteration, hipGraph replay",
-            "time_to_converged_s": round(ttc_pcg, 4),
-            "block_lm_iters_per_s": round(sum(itts_pcg) / ttc_pcg, 2),
-            "lockstep_iterations": n_lock_pcg,
-            "block_itt_total": int(sum(itts_pcg)),
-            "pcg_iterations_total": int(sum(pcg_its)),
-            "speedup_vs_sequential_pcg": round(seq_pcg / ttc_pcg, 3),
-            "sigma0_rel_vs_sequential_pcg_max": float(np.max(np.abs(sig_bpcg - np.asarray(sig_pcg))
-                                                             / np.asarray(sig_pcg))),
-            "kernel_ms_per_solve": {k: round(v[0], 3) for k, v in kt_pcg.items() if v[1] > 0},
-            "pcg_iter_launches": kt_pcg["pcg_iter"][1],
+        "speedup_vs_sequential_pcg": round(seq_pcg / P["ttc"], 3),
+        "sigma0_rel_vs_sequential_pcg_max": rel(sig_p, sig_sp),
+        "kernel_ms_per_solve": {k: round(v[0], 3) for k, v in P["kt"].items() if v[1] > 0},
+        "spmv_launches": n_spmv,
+        "roofline": {"bound": "hbm", "kernel": "spmv", "achieved": None if achieved is None else round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": alg, "launches": n_spmv},
+        "batched_dense": {
+            "form": "the same batch with the batched FP64-MFMA dense Cholesky of every block's reduced camera "
+                    "system (solver 1)",
+            "time_to_converged_s": round(D["ttc"], 4),
+            "block_lm_iters_per_s": round(it_d / D["ttc"], 2),
+            "lockstep_iterations": D["n_lock"],
+            "block_itt_total": int(it_d),
+            "sequential_dense_s": round(seq_dense, 4),
+            "speedup_vs_sequential_dense": round(seq_dense / D["ttc"], 3),
+            "speedup_vs_sequential_pcg": round(seq_pcg / D["ttc"], 3),
+            "sigma0_rel_vs_sequential_dense_max": rel(sig_d, sig_sd),
+            "sigma0_rel_vs_sequential_pcg_max": rel(sig_d, sig_sp),
+            "kernel_ms_per_solve": {k: round(v[0], 3) for k, v in D["kt"].items() if v[1] > 0},
+            "roofline": {"bound": "mfma", "kernel": "chol_update",
+                         "achieved": None if upd_tf is None else round(upd_tf, 2), "peak": 78.6, "unit": "TFLOP/s",
+                         "frac": None if upd_tf is None else round(upd_tf / 78.6, 4), "launches": n_upd},
+            "tuning": D["tuning"],
         },
-        "kernel_ms_per_solve": {k: round(v[0], 3) for k, v in kt.items() if v[1] > 0},
-        "roofline": {"bound": "mfma", "kernel": "chol_update", "achieved": None if achieved_tf is None else round(achieved_tf, 2),
-                     "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": None if achieved_tf is None else round(achieved_tf / 78.6, 4),
-                     "traffic": None, "launches": n_upd},
-        "tuning": tuning,
-        "setup_s": {"generate": round(gen_s, 2), "create_incl_autotune": round(create_s, 2)},
+        "setup_s": {"generate": round(gen_s, 2), "create_pcg": round(P["create_s"], 2),
+                    "create_dense_incl_autotune": round(D["create_s"], 2)},
     }
-    if not args.no_cpu_baseline:
+    if not no_cpu:
         line["cpu_baseline"] = cpu_baseline_blocks(blocks)
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def run_batch_bench(args):
+    only = None if args.batch_solver == "both" else int(args.batch_solver)
+    print(json.dumps(batch_leg(args.no_sequential, args.no_cpu_baseline, only=only)), flush=True)
 
 
 class FormStopped(Exception):
@@ -756,8 +784,11 @@ def main():
     ap.add_argument("--partition-config", default="c2_961,c3_4585,c5_13682",
                     help="scenes of the partitioned-solve leg (Threads = 0 dispatch), comma-separated; none to skip")
     ap.add_argument("--no-chosen", action="store_true", help="skip the chosen (FP32, matrix-free) form")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-blocks leg (c4_batch) of the line")
     ap.add_argument("--no-sequential", action="store_true",
                     help="c4_batch: skip the one-block-after-another comparison (profiling runs)")
+    ap.add_argument("--batch-solver", default="both", choices=("both", "0", "1"),
+                    help="c4_batch: run one batched camera solve alone (profiling runs; no line of the metric)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run device-to-device copies of this many bytes (counter calibration)")
     args = ap.parse_args()

@@ -104,7 +104,7 @@ struct rpba_batch {
     DBuf<unsigned char> d_flag, d_flagst, d_camflag, d_accept, d_Wrec, d_Yrec;
     DBuf<double> d_X, d_Xn, d_X0, d_Vinv, d_gp, d_Sd, d_b, d_sol, d_In_bak, d_Su, d_Dd, d_terms, d_bsum, d_partial;
     DBuf<double> d_bpart;
-    DBuf<int> d_coff, d_poff, d_eoff, d_camblk, d_ptblk, d_iflag, d_nanb, d_ipartial, d_iscal, d_npart;
+    DBuf<int> d_coff, d_uptoff, d_poff, d_eoff, d_camblk, d_ptblk, d_iflag, d_nanb, d_ipartial, d_iscal, d_npart;
     DBuf<double> d_scal;
     DBuf<double> c_Rb, c_Cb, c_Rn, c_Cn, c_In, c_th, c_thn, c_Pst, c_Ist;
     DBuf<unsigned> c_mask;
@@ -361,6 +361,7 @@ static int batch_init(rpba_batch *c)
     rc |= c->d_bpart.alloc(3 * (size_t)block_slices() * nb);
     rc |= c->d_npart.alloc((size_t)block_slices() * nb);
     rc |= c->d_coff.alloc((size_t)nb + 1);
+    rc |= c->d_uptoff.alloc((size_t)nb + 1);
     rc |= c->d_poff.alloc((size_t)nb + 1);
     rc |= c->d_eoff.alloc((size_t)nb + 1);
     rc |= c->d_camblk.alloc((size_t)nc);
@@ -376,6 +377,7 @@ static int batch_init(rpba_batch *c)
     }
     if (rc) return RPBA_E_NOMEM;
     rc |= c->d_coff.put(c->coff.data(), c->coff.size(), c->st);
+    rc |= c->d_uptoff.put(c->uptoff.data(), c->uptoff.size(), c->st);
     rc |= c->d_camblk.put(c->camblk_h.data(), c->camblk_h.size(), c->st);
     if (rc) return RPBA_E_NOMEM;
     // the PCG's two scalar slots: slot 0's done flag gates the union SpMV
@@ -460,29 +462,23 @@ static int build_structure(rpba_batch *c)
         }
         return RPBA_E_NO_OBS_IN_IMAGE;
     }
-    std::vector<int> toff, nrpv, tfo, tcam;
-    rc = c->ds.download_tracks(toff, nrpv, tfo, tcam, c->st);
-    if (rc) return rc;
+    // kept points are in union point order, so each block's are contiguous:
+    // its point and entry ranges and each point's block, on the device
     const int nb = c->nb, np = c->ds.npts;
-    std::vector<int> ptblk(std::max(np, 1));
+    rc |= c->d_ptblk.alloc((size_t)std::max(np, 1));
+    if (rc) return RPBA_E_NOMEM;
+    launch_batch_ranges(np, nb, c->ds.nrpv, c->d_uptoff.p, c->ds.toff, c->d_ptblk.p, c->d_poff.p, c->d_eoff.p, c->st);
+    HIPCHK(hipGetLastError());
     c->poff_h.assign(nb + 1, 0);
     c->eoff_h.assign(nb + 1, 0);
-    // kept points are in union point order, so each block's are contiguous
-    int b = 0;
-    for (int i = 0; i < np; ++i) {
-        while (nrpv[i] >= c->uptoff[b + 1]) c->poff_h[++b] = i;
-        ptblk[i] = b;
-    }
-    while (b < nb) c->poff_h[++b] = np;
-    for (int j = 0; j <= nb; ++j) c->eoff_h[j] = toff[c->poff_h[j]];
+    HIPCHK(hipMemcpyAsync(c->poff_h.data(), c->d_poff.p, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->eoff_h.data(), c->d_eoff.p, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
     for (int j = 0; j < nb; ++j) {
         c->B[j].npts = c->poff_h[j + 1] - c->poff_h[j];
         c->B[j].ntr = c->eoff_h[j + 1] - c->eoff_h[j];
     }
     const size_t ntr = (size_t)c->ds.ntr;
-    rc |= c->d_ptblk.put(ptblk.data(), ptblk.size(), c->st);
-    rc |= c->d_poff.put(c->poff_h.data(), c->poff_h.size(), c->st);
-    rc |= c->d_eoff.put(c->eoff_h.data(), c->eoff_h.size(), c->st);
     rc |= c->d_wx.alloc(ntr);
     rc |= c->d_wy.alloc(ntr);
     rc |= c->d_wxy.alloc(ntr);
@@ -674,7 +670,10 @@ static void launches_pcg(rpba_batch *c, int n)
 {
     const PcgArgs pa = seg_pcg_args(c);
     const SegArgs sg = seg_args(c);
-    for (int i = 0; i < n; ++i) c->timed("pcg_iter", [&] { launch_seg_iter(pa, sg, c->st); });
+    for (int i = 0; i < n; ++i) {
+        c->timed("spmv", [&] { launch_spmv(pa, c->st); });
+        c->timed("seg_update", [&] { launch_seg_update(pa, sg, c->st); });
+    }
 }
 
 // the rest of the iteration: back-substitution, decode, trial cost, the
@@ -729,20 +728,22 @@ static int capture_graph(rpba_batch *c)
 
 // the segmented PCG until every block has stopped: as many iterations as the
 // last lockstep solve took before the first look at the count, then
-// kSegChunk at a time
+// kSegChunk at a time.  With kernel timing on, one iteration at a time, so
+// that no launch after the last block stopped enters the timings.
 static int run_pcg(rpba_batch *c, bool replay)
 {
     int cap = 0;
     for (int b = 0; b < c->nb; ++b) cap = std::max(cap, c->h_seg[b].msize);
-    int n = std::max(1, (c->pcg_expect + kSegChunk - 1) / kSegChunk), launched = 0;
+    const int per = c->timing ? 1 : kSegChunk;
+    int n = c->timing ? 1 : std::max(1, (c->pcg_expect + kSegChunk - 1) / kSegChunk), launched = 0;
     for (;;) {
         for (int i = 0; i < n; ++i) {
             if (replay)
                 HIPCHK(hipGraphLaunch(c->gexec[1], c->st));
             else
-                launches_pcg(c, kSegChunk);
+                launches_pcg(c, per);
         }
-        launched += n * kSegChunk;
+        launched += n * per;
         HIPCHK(hipMemcpyAsync(c->h_ndone, c->d_ndone.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         if (*c->h_ndone >= c->nb) return RPBA_OK;
@@ -1013,7 +1014,7 @@ extern "C" void rpba_batch_destroy(rpba_batch *c)
     c->d_X.release(); c->d_Xn.release(); c->d_X0.release(); c->d_Vinv.release(); c->d_gp.release();
     c->d_Sd.release(); c->d_b.release(); c->d_sol.release(); c->d_In_bak.release(); c->d_Su.release();
     c->d_Dd.release(); c->d_terms.release(); c->d_bsum.release(); c->d_partial.release();
-    c->d_coff.release(); c->d_poff.release(); c->d_eoff.release(); c->d_camblk.release(); c->d_ptblk.release();
+    c->d_coff.release(); c->d_uptoff.release(); c->d_poff.release(); c->d_eoff.release(); c->d_camblk.release(); c->d_ptblk.release();
     c->d_iflag.release(); c->d_nanb.release(); c->d_ipartial.release(); c->d_iscal.release(); c->d_scal.release();
     c->d_bpart.release(); c->d_npart.release();
     c->c_Rb.release(); c->c_Cb.release(); c->c_Rn.release(); c->c_Cn.release(); c->c_In.release();

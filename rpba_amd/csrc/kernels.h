@@ -294,5 +294,10 @@ void launch_batch_store(const BatchArgs &b, const unsigned char *accept, const C
 // trial intrinsics back from the copy taken before the step, for blocks with nanb[b]
 void launch_batch_restore_in(const BatchArgs &b, const int *nanb, const double *In_bak, double *In, hipStream_t st);
 void launch_copy(double *dst, const double *src, size_t n, hipStream_t st);
+// the blocks' kept-point and entry ranges after a structure build: ptblk per
+// kept point, poff / eoff (nb + 1 each) from the kept points' union indices
+// nrpv (ascending) and the blocks' union point offsets uptoff
+void launch_batch_ranges(int npts, int nb, const int *nrpv, const int *uptoff, const int *toff, int *ptblk, int *poff,
+                         int *eoff, hipStream_t st);
 
 }  // namespace rpba

@@ -2906,6 +2906,52 @@ void launch_batch_restore_in(const BatchArgs &b, const int *nanb, const double *
 {
     k_batch_restore_in<<<nblk(std::max(b.nc, 1), 64), 64, 0, st>>>(b, nanb, In_bak, In);
 }
+// point -> block: the block whose union point range holds the point's union
+// index (uptoff[lo] <= u < uptoff[lo + 1]; an empty block's equal offsets
+// resolve to the block after it)
+__global__ void k_batch_ptblk(int npts, int nb, const int *nrpv, const int *uptoff, int *ptblk)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const int u = nrpv[i];
+    int lo = 0, hi = nb;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (uptoff[mid] <= u)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    ptblk[i] = lo;
+}
+
+// block b's first kept point (the first with union index >= uptoff[b]; kept
+// points stay in union order, so nrpv ascends) and first entry
+__global__ void k_batch_ranges(int npts, int nb, const int *nrpv, const int *uptoff, const int *toff, int *poff,
+                               int *eoff)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nb) return;
+    const int key = uptoff[b];
+    int lo = 0, hi = npts;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (nrpv[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    poff[b] = lo;
+    eoff[b] = toff[lo];
+}
+
+void launch_batch_ranges(int npts, int nb, const int *nrpv, const int *uptoff, const int *toff, int *ptblk, int *poff,
+                         int *eoff, hipStream_t st)
+{
+    if (npts > 0) k_batch_ptblk<<<nblk(npts, 256), 256, 0, st>>>(npts, nb, nrpv, uptoff, ptblk);
+    k_batch_ranges<<<nblk(nb + 1, 64), 64, 0, st>>>(npts, nb, nrpv, uptoff, toff, poff, eoff);
+}
+
 void launch_copy(double *dst, const double *src, size_t n, hipStream_t st)
 {
     if (!n) return;

@@ -68,6 +68,8 @@ struct SegArgs {
 };
 void launch_seg_init(const PcgArgs &a, const SegArgs &s, hipStream_t st);
 void launch_seg_iter(const PcgArgs &a, const SegArgs &s, hipStream_t st);
+// the iteration without its SpMV: the gather of (S d)_k and the block updates
+void launch_seg_update(const PcgArgs &a, const SegArgs &s, hipStream_t st);
 // per block: 1 when the float camera step or a point step of the block is
 // NaN or infinite (the reference's check, rbacore.cpp:537-546); flag[nb]
 // cleared by the launcher

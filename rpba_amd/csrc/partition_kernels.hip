@@ -518,12 +518,18 @@ void launch_seg_init(const PcgArgs &a, const SegArgs &s, hipStream_t st)
     k_seg_init<<<s.nb, 256, 0, st>>>(a, s);
 }
 
+void launch_seg_update(const PcgArgs &a, const SegArgs &s, hipStream_t st)
+{
+    if (s.nb <= 0) return;
+    k_seg_gather<<<s.nc, 256, 0, st>>>(a, s);
+    k_seg_step<<<s.nb, 1024, 0, st>>>(a, s);
+}
+
 void launch_seg_iter(const PcgArgs &a, const SegArgs &s, hipStream_t st)
 {
     if (s.nb <= 0) return;
     launch_spmv(a, st);
-    k_seg_gather<<<s.nc, 256, 0, st>>>(a, s);
-    k_seg_step<<<s.nb, 1024, 0, st>>>(a, s);
+    launch_seg_update(a, s, st);
 }
 
 __global__ void k_seg_nan(int nc, int npts, const int *cblk, const int *ptblk, const double *sol, const double *Xn,
