@@ -507,6 +507,7 @@ def batch_leg(no_sequential=False, no_cpu=False, sequential_dense=True, only=Non
         b.set_timing(True)
         b.run(-1)
         kt = {k: b.kernel_time(k) for k in kern}
+        spmv_bytes = b.spmv_bytes() if solver == 0 else None
         b.set_timing(False)
         tuning = b.tuning()
         # the structure of pass 0 (the one most launches ran on)
@@ -516,7 +517,7 @@ def batch_leg(no_sequential=False, no_cpu=False, sequential_dense=True, only=Non
         b.close()
         progress(f"batched solver {solver}: {ttc:.3f} s, {n_lock} lockstep iterations")
         return {"ttc": ttc, "n_lock": n_lock, "res": res, "kt": kt, "tuning": tuning, "sizes": sizes,
-                "create_s": create_s}
+                "create_s": create_s, "spmv_bytes": spmv_bytes}
 
     common = ["lin_points", "schur_pairs", "schur_diag", "lin_cameras", "backsub", "decode", "cost", "reweight",
               "block_sums"]
@@ -560,11 +561,14 @@ def batch_leg(no_sequential=False, no_cpu=False, sequential_dense=True, only=Non
     seq_pcg, sig_sp = (nan, sig_p) if no_sequential else seq(0)
     seq_dense, sig_sd = (nan, sig_d) if (no_sequential or not sequential_dense) else seq(1)
 
-    # roofline of the batched PCG's dominant kernel, the union SpMV: S blocks
-    # once, d in, Ad out, row partials (HIP-event time of its working launches)
+    # roofline of the batched PCG's dominant kernel, the union SpMV: per
+    # launch the upper S blocks of the blocks whose PCG still runs and every
+    # diagonal block once, d in, Ad out, row partials (the library's count over
+    # the timed solve, whose launches are exactly the PCG iterations), over its
+    # HIP-event time
     ms_spmv, n_spmv = P["kt"]["spmv"]
-    alg = algorithmic_bytes("spmv", P["sizes"])
-    achieved = alg / (ms_spmv / n_spmv * 1e-3) / 1e9 if n_spmv else None
+    alg = P["spmv_bytes"] / n_spmv if n_spmv else None
+    achieved = P["spmv_bytes"] / (ms_spmv * 1e-3) / 1e9 if n_spmv else None
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_spmv_c4_batch.json")
     if os.path.exists(tfile):
@@ -612,7 +616,8 @@ def batch_leg(no_sequential=False, no_cpu=False, sequential_dense=True, only=Non
         "roofline": {"bound": "hbm", "kernel": "spmv", "achieved": None if achieved is None else round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": alg, "launches": n_spmv},
+                     "algorithmic_bytes_per_launch": None if alg is None else int(alg),
+                     "algorithmic_bytes_whole_union_s": algorithmic_bytes("spmv", P["sizes"]), "launches": n_spmv},
         "batched_dense": {
             "form": "the same batch with the batched FP64-MFMA dense Cholesky of every block's reduced camera "
                     "system (solver 1)",

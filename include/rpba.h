@@ -254,6 +254,11 @@ int rpba_batch_download(rpba_batch *batch, rpba_stats *stats, int32_t *status);
 int rpba_batch_set_graph(rpba_batch *batch, int32_t on);
 int rpba_batch_set_timing(rpba_batch *batch, int32_t on);
 int rpba_batch_kernel_time(rpba_batch *batch, const char *kernel, double *total_ms, int64_t *launches);
+/* solver 0: the algorithmic HBM bytes of the SpMV launches timed since
+ * rpba_batch_set_timing(1): per launch, 648 bytes for each upper block of S of
+ * the blocks whose PCG still runs and for every diagonal block, 72 + 72 + 8
+ * bytes per camera (d in, the diagonal product out, the row partials) */
+int rpba_batch_spmv_bytes(rpba_batch *batch, double *bytes);
 int rpba_batch_tuning(rpba_batch *batch, char *buf, int32_t cap);
 /* out[0..5] = cameras, points, observations, pairs, off-diagonal S blocks,
  * doubles of dense storage, over the union */

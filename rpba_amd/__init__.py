@@ -18,7 +18,7 @@ EXPORTS = [
     "rpba_ctx_set_timing", "rpba_ctx_kernel_time", "rpba_ctx_progress", "rpba_ctx_sizes", "rpba_ctx_destroy",
     "rpba_ctx_tuning", "rpba_dense_solve_batch", "rpba_ctx_set_graph",
     "rpba_batch_create", "rpba_batch_run", "rpba_batch_reset", "rpba_batch_download", "rpba_batch_set_graph",
-    "rpba_batch_set_timing", "rpba_batch_kernel_time", "rpba_batch_tuning", "rpba_batch_sizes", "rpba_batch_destroy",
+    "rpba_batch_set_timing", "rpba_batch_kernel_time", "rpba_batch_spmv_bytes", "rpba_batch_tuning", "rpba_batch_sizes", "rpba_batch_destroy",
     "rpba_calc_plsa_batch",
     "rpba_lp_create", "rpba_lp_destroy", "rpba_lp_n_params", "rpba_lp_get_params", "rpba_lp_predict",
     "rpba_lp_gradient", "rpba_lp_train", "rpba_ctx_set_lp", "rpba_ctx_last_omega", "rpba_ctx_lp_sample",
@@ -72,6 +72,7 @@ def lib():
     L.rpba_batch_set_graph.argtypes = [C.c_void_p, C.c_int32]
     L.rpba_batch_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.rpba_batch_kernel_time.argtypes = [C.c_void_p, C.c_char_p, P(C.c_double), P(C.c_int64)]
+    L.rpba_batch_spmv_bytes.argtypes = [C.c_void_p, P(C.c_double)]
     L.rpba_batch_tuning.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
     L.rpba_batch_sizes.argtypes = [C.c_void_p, P(C.c_int64)]
     L.rpba_batch_destroy.argtypes = [C.c_void_p]

@@ -198,6 +198,12 @@ class Batch:
         lib().rpba_batch_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n))
         return ms.value, n.value
 
+    def spmv_bytes(self):
+        """solver 0: algorithmic bytes of the SpMV launches timed so far"""
+        v = C.c_double()
+        lib().rpba_batch_spmv_bytes(self.h, C.byref(v))
+        return v.value
+
     def tuning(self):
         buf = C.create_string_buffer(1024)
         lib().rpba_batch_tuning(self.h, buf, 1024)

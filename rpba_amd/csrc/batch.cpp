@@ -116,10 +116,13 @@ struct rpba_batch {
     bool pcg = false;
     DBuf<double> d_Cinv, d_px, d_best, d_r, d_dv, d_Ad, d_h, d_h1, d_Sdf, d_ev, d_rowpart, d_colpart, d_dpart;
     DBuf<SegScal> d_segS;
-    DBuf<int> d_ndone;
+    DBuf<int> d_ndone, d_live;
     int *h_ndone = nullptr;    // pinned
     SegScal *h_seg = nullptr;  // pinned: 1 per block
     int pcg_expect = 0;        // iterations the last lockstep PCG took (its slowest block)
+    std::vector<int> roff_h;
+    std::vector<long long> sblk_h;  // per block: its upper blocks of S (current structure)
+    double spmv_bytes = 0.;    // algorithmic bytes of the SpMV launches timed (kernel timing on)
 
     // graph replay of the LM iteration: [0] the whole iteration (dense solve)
     // or the part before the PCG, [1] kSegChunk PCG iterations, [2] the part
@@ -500,8 +503,19 @@ static int build_structure(rpba_batch *c)
         rc |= c->d_rowpart.alloc(63 * (size_t)std::max(c->ds.nchunk, 1));
         rc |= c->d_colpart.alloc(9 * (size_t)std::max(c->ds.nslab, 1));
         rc |= c->d_dpart.alloc((size_t)std::max(spmv_grid(c->ds.nblk), 1));
+        rc |= c->d_live.alloc((size_t)std::max(c->ds.nstrip, 1));
     }
     if (rc) return RPBA_E_NOMEM;
+    if (c->pcg) {
+        // the upper blocks are in row order: block b holds those of rows
+        // coff[b] .. coff[b+1] - 1
+        c->roff_h.assign((size_t)c->nc + 1, 0);
+        HIPCHK(hipMemcpyAsync(c->roff_h.data(), c->ds.roff, sizeof(int) * c->roff_h.size(), hipMemcpyDeviceToHost,
+                              c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->sblk_h.assign(c->nb, 0);
+        for (int b = 0; b < c->nb; ++b) c->sblk_h[b] = c->roff_h[c->coff[b + 1]] - c->roff_h[c->coff[b]];
+    }
     HIPCHK(hipStreamSynchronize(c->st));
     c->structure_dirty = false;
     return RPBA_OK;
@@ -616,6 +630,7 @@ static PcgArgs seg_pcg_args(rpba_batch *c)
     pa.rowpart = c->d_rowpart.p; pa.colpart = c->d_colpart.p; pa.dpart = c->d_dpart.p; pa.nblk = d.nblk;
     pa.ndpart = spmv_grid(d.nblk); pa.dsum = c->d_dpart.p; pa.ndsum = pa.ndpart; pa.S = c->d_pcg.p; pa.j = 0;
     pa.implicit = 0; pa.npts = d.npts; pa.ntr = d.ntr; pa.nch = d.nch;
+    pa.strip_live = c->d_live.p;
     return pa;
 }
 
@@ -624,6 +639,7 @@ static SegArgs seg_args(rpba_batch *c)
     SegArgs sg;
     sg.nc = c->nc; sg.nb = c->nb; sg.boff = c->d_coff.p; sg.cblk = c->d_camblk.p; sg.S = c->d_segS.p;
     sg.ndone = c->d_ndone.p; sg.Sdf = c->d_Sdf.p; sg.e = c->d_ev.p;
+    sg.strip_live = c->d_live.p; sg.nstrip = c->ds.nstrip;
     return sg;
 }
 
@@ -932,8 +948,17 @@ static int lm_iteration(rpba_batch *c)
     HIPCHK(hipStreamSynchronize(c->st));
     if (c->pcg) {
         c->pcg_expect = 0;
+        double sb = 0.;
         for (int b = 0; b < nb; ++b)
-            if (c->B[b].active) c->pcg_expect = std::max(c->pcg_expect, c->h_seg[b].iters);
+            if (c->B[b].active) {
+                c->pcg_expect = std::max(c->pcg_expect, c->h_seg[b].iters);
+                sb += (double)c->h_seg[b].iters * (double)c->sblk_h[b];
+            }
+        // the SpMV launches of this PCG (one per iteration of its slowest
+        // block, exactly so with timing on): each reads the upper blocks of
+        // the running blocks' cameras and every diagonal block (648 bytes a
+        // block), d (72 per camera), and writes Ad (72) and row partials (8)
+        if (c->timing) c->spmv_bytes += 648. * sb + 800. * (double)c->nc * (double)c->pcg_expect;
     }
     if (c->timing) c->resolve_timing();
     std::vector<unsigned char> acc(nb, 0);
@@ -1025,6 +1050,7 @@ extern "C" void rpba_batch_destroy(rpba_batch *c)
         v->release();
     c->d_segS.release();
     c->d_ndone.release();
+    c->d_live.release();
     if (c->h_bsum) (void)hipHostFree(c->h_bsum);
     if (c->h_nanb) (void)hipHostFree(c->h_nanb);
     if (c->h_ndone) (void)hipHostFree(c->h_ndone);
@@ -1208,6 +1234,14 @@ extern "C" int rpba_batch_set_timing(rpba_batch *c, int32_t on)
     if (!c) return RPBA_E_INVALID;
     c->timing = on != 0;
     c->ktime.clear();
+    c->spmv_bytes = 0.;
+    return RPBA_OK;
+}
+
+extern "C" int rpba_batch_spmv_bytes(rpba_batch *c, double *bytes)
+{
+    if (!c || !bytes) return RPBA_E_INVALID;
+    *bytes = c->spmv_bytes;
     return RPBA_OK;
 }
 

@@ -149,6 +149,9 @@ struct PcgArgs {
     const double *dmp = nullptr, *Vinv = nullptr;
     const int4 *pg4 = nullptr;  // per point group {p0, p1, e0, e1}
     const int4 *ch4 = nullptr;  // per chunk {first entry, entries, image, image's first chunk}
+    // block-segmented PCG: per strip, the running blocks with cameras in it;
+    // the strip-slab SpMV skips the chunks of strips at 0 (null: none skipped)
+    const int *strip_live = nullptr;
 };
 
 struct StepArgs {

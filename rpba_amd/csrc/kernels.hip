@@ -882,16 +882,24 @@ constexpr int kSpmvWaves = 4;  // wavefronts per workgroup (256 threads)
 // A wavefront's walk over its slabs: chunk ci of its sequence (ci = w, w + W,
 // ...), chunk id c of strip st, slab q, up to e.  ok = 0 past the last slab;
 // q is then slab 0, so that the loads of that position stay in bounds.
+// With a.strip_live set, the chunks of strips at 0 are passed over (CI moves
+// on to the next chunk of the sequence that is live).
 #define SPMV_CHUNK_START(CI, C, Q, E, ST, OK)                                         \
     {                                                                                 \
-        const int ci_ = (CI);                                                         \
-        OK = ci_ < a.nchunk;                                                          \
-        C = 0; Q = 0; E = 0; ST = 0;                                                  \
-        if (OK) {                                                                     \
+        for (;;) {                                                                    \
+            const int ci_ = (CI);                                                     \
+            OK = ci_ < a.nchunk;                                                      \
+            C = 0; Q = 0; E = 0; ST = 0;                                              \
+            if (!OK) break;                                                           \
             C = rev ? a.nchunk - 1 - ci_ : ci_;                                       \
             ST = chunk_strip[C];                                                      \
+            if (a.strip_live && a.strip_live[ST] == 0) {                              \
+                CI = ci_ + W;                                                         \
+                continue;                                                             \
+            }                                                                         \
             Q = strip_s0[ST] + a.slab_chunk * (C - chunk_off[ST]);                    \
             E = min(Q + a.slab_chunk, strip_s0[ST + 1]);                              \
+            break;                                                                    \
         }                                                                             \
     }
 // slab Q's first slot B, block count M, rows present K and column L, loaded
@@ -979,8 +987,8 @@ __device__ __forceinline__ void spmv_slabs_body(const PcgArgs &a, int j, const i
     // two slabs in flight: register sets A and B alternate
     int ci0, c0, q0, e0, st0, ok0, ci1, c1, q1, e1, st1, ok1;
     int b0, m0, k0, l0, b1, m1, k1, l1;  // the two slabs' metadata
-    SPMV_CHUNK_START(blockIdx.x * kSpmvWaves + w, c0, q0, e0, st0, ok0);
     ci0 = blockIdx.x * kSpmvWaves + w;
+    SPMV_CHUNK_START(ci0, c0, q0, e0, st0, ok0);
     SPMV_META(q0, b0, m0, k0, l0);
     SPMV_ADVANCE(ci0, c0, q0, e0, st0, ok0, ci1, c1, q1, e1, st1, ok1, b1, m1, k1, l1);
     double vA[9], vB[9], dkA, dkB;

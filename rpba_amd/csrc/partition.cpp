@@ -304,7 +304,7 @@ struct PartSolver {
         DBuf<float> gc_w;
         DBuf<unsigned> c_mask;
         DBuf<unsigned char> camflag;
-        DBuf<int> gc_of, d_boff, d_cblk, d_ptblk, d_nan, d_ndone;
+        DBuf<int> gc_of, d_boff, d_cblk, d_ptblk, d_nan, d_ndone, d_live;
         DBuf<SegScal> segS;
         DBuf<PcgScalars> pcgS;
         void release()
@@ -316,7 +316,7 @@ struct PartSolver {
             gc_w.release();
             c_mask.release();
             camflag.release();
-            for (DBuf<int> *b : {&gc_of, &d_boff, &d_cblk, &d_ptblk, &d_nan, &d_ndone}) b->release();
+            for (DBuf<int> *b : {&gc_of, &d_boff, &d_cblk, &d_ptblk, &d_nan, &d_ndone, &d_live}) b->release();
             segS.release();
             pcgS.release();
         }
@@ -767,6 +767,11 @@ int PartSolver::block_step(int iter, const int addf[3], int &pcg_it)
     SegArgs sg;
     sg.nc = nc; sg.nb = nb; sg.boff = d_boff.p; sg.cblk = d_cblk.p; sg.S = segS.p; sg.ndone = d_ndone.p; sg.Sdf = Sdf.p;
     sg.e = ev.p;
+    rc = U.d_live.alloc((size_t)std::max(ds->nstrip, 1));
+    if (rc) return RPBA_E_NOMEM;
+    sg.strip_live = U.d_live.p;
+    sg.nstrip = ds->nstrip;
+    pa.strip_live = U.d_live.p;
     launch_seg_init(pa, sg, st);
     int *hd = nullptr;
     if (hipHostMalloc((void **)&hd, sizeof(int), hipHostMallocDefault) != hipSuccess) return RPBA_E_NOMEM;

@@ -65,6 +65,10 @@ struct SegArgs {
     int *ndone;     // blocks that stopped (integer counter)
     double *Sdf;    // (S d)_k gathered per camera (9 each)
     double *e;      // d_k . (S d)_k per camera
+    // per strip of kSlab cameras, the running blocks with cameras in it (the
+    // SpMV skips strips at 0; PcgArgs::strip_live points here), or null
+    int *strip_live = nullptr;
+    int nstrip = 0;
 };
 void launch_seg_init(const PcgArgs &a, const SegArgs &s, hipStream_t st);
 void launch_seg_iter(const PcgArgs &a, const SegArgs &s, hipStream_t st);

@@ -336,12 +336,25 @@ void launch_lsax(const LsaxArgs &a, hipStream_t st)
 // block-segmented PCG
 // ---------------------------------------------------------------------------
 
-// the done counter and the all-done flag of the union SpMV cleared (a kernel,
-// so the set-up can be recorded into a graph)
+// the done counter, the all-done flag of the union SpMV and the strips' live
+// counts cleared (a kernel, so the set-up can be recorded into a graph)
 __global__ void k_seg_reset(PcgArgs a, SegArgs s)
 {
-    *s.ndone = 0;
-    a.S[0].done = 0;
+    if (threadIdx.x == 0) {
+        *s.ndone = 0;
+        a.S[0].done = 0;
+    }
+    if (s.strip_live)
+        for (int i = threadIdx.x; i < s.nstrip; i += blockDim.x) s.strip_live[i] = 0;
+}
+
+// the strips holding cameras c0 .. c1 - 1 of a block: add v to their live
+// counts (threads of the workgroup share the strips)
+__device__ __forceinline__ void seg_strips_add(const SegArgs &s, int c0, int c1, int v)
+{
+    if (!s.strip_live || c1 <= c0) return;
+    for (int st = c0 / kSlab + (int)threadIdx.x; st <= (c1 - 1) / kSlab; st += blockDim.x)
+        atomicAdd(s.strip_live + st, v);
 }
 
 // per block: x = 0, r = b, best = b, h = C r, d = h, rho = r.h (rbacore.cpp:1829-1847).
@@ -373,6 +386,7 @@ __global__ __launch_bounds__(256) void k_seg_init(PcgArgs a, SegArgs s)
         }
         return;
     }
+    seg_strips_add(s, c0, c1, 1);
     for (size_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
         a.x[j] = 0.;
         a.r[j] = a.b[j];
@@ -505,6 +519,7 @@ __global__ __launch_bounds__(1024) void k_seg_step(PcgArgs a, SegArgs s)
         }
     }
     __syncthreads();
+    if (z.done) seg_strips_add(s, c0, c1, -1);
     if (threadIdx.x == 0) {
         s.S[bb] = z;
         if (z.done && atomicAdd(s.ndone, 1) + 1 == s.nb) a.S[0].done = 1;
@@ -514,7 +529,7 @@ __global__ __launch_bounds__(1024) void k_seg_step(PcgArgs a, SegArgs s)
 void launch_seg_init(const PcgArgs &a, const SegArgs &s, hipStream_t st)
 {
     if (s.nb <= 0) return;
-    k_seg_reset<<<1, 1, 0, st>>>(a, s);
+    k_seg_reset<<<1, 256, 0, st>>>(a, s);
     k_seg_init<<<s.nb, 256, 0, st>>>(a, s);
 }
 
