@@ -89,10 +89,16 @@ struct LinArgs {
     int jform = 0;
     void *Jpm = nullptr, *Jcm = nullptr;
     double *dmp = nullptr;
-    // per point, one 128-byte line for the camera side (jform): X (3), the
-    // upper triangle of V^-1 (6: 00 01 02 11 12 22), g_p (3), 4 unused
+    // per point, one 128-byte line for the camera side: with jform X (3),
+    // the upper triangle of V^-1 (6: 00 01 02 11 12 22), g_p (3), 4 unused;
+    // with ycm X (3), V^-1 (9, row-major), g_p (3), 1 unused
     double *ptrec = nullptr;
     const int4 *pg4 = nullptr;  // per point group {p0, p1, e0, e1}
+    // matrix-free form with W / Y records: the point side writes W only and
+    // the point lines; k_lin_chunks forms Y in camera-major order and the
+    // chunks' diagonal Schur terms (needs the entry table, ptrec and ch4)
+    int ycm = 0;
+    const int4 *ch4 = nullptr;  // per chunk {first entry, entries, image, first chunk of the image}
 };
 
 struct SchurArgs {

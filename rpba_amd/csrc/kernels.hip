@@ -421,35 +421,28 @@ __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
     for (int j = 0; j < 9; ++j) a.Vinv[9 * (size_t)i + j] = Vi[j];
 #pragma unroll
     for (int j = 0; j < 3; ++j) a.gp[3 * (size_t)i + j] = g[j];
+    if (a.ycm) {
+        // the point's line for the camera side: X, V^-1, g_p, one unused
+        double2 *L = (double2 *)(a.ptrec + 16 * (size_t)i);
+        L[0] = make_double2(a.X[3 * (size_t)i], a.X[3 * (size_t)i + 1]);
+        L[1] = make_double2(a.X[3 * (size_t)i + 2], Vi[0]);
+        L[2] = make_double2(Vi[1], Vi[2]);
+        L[3] = make_double2(Vi[3], Vi[4]);
+        L[4] = make_double2(Vi[5], Vi[6]);
+        L[5] = make_double2(Vi[7], Vi[8]);
+        L[6] = make_double2(g[0], g[1]);
+        L[7] = make_double2(g[2], 0.);
+    }
 }
 
-// One thread per track entry: recomputes the observation's Jacobian (cheaper
-// than storing it) and writes W = Bc^T W Bp and Y = W V^-1 into the entry's
-// kRec-element records (R = double: 224 bytes; R = float, the FP32 form: 112
-// bytes).  Consecutive lanes write consecutive records, so a wavefront fills
-// whole cache lines.
-template <typename R>
-__global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
+// W = Bc^T W Bp (9 x 3) of one entry and, with WithY, Y = W V^-1, in R: the
+// values of its records.  The point side (k_lin_records) and the camera side
+// of the matrix-free form (k_lin_chunks) both form them here from the same
+// inputs, so the two agree bit for bit.
+template <typename R, bool WithY>
+__device__ __forceinline__ void wy_values(const R (&Bc)[2][9], const R (&Bp)[2][3], R W0, R W1, R W2, const R (&Vr)[9],
+                                          R (&Wv)[27], R (&Yv)[27])
 {
-    __shared__ uint4 stg[2][64][9];  // per wavefront: 64 records' staged pieces
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int q0 = q - lane, nq = min(64, a.ntr - q0);
-    if (nq <= 0) return;  // the whole wavefront is past the entries
-    const int qc = min(q, a.ntr - 1);  // lanes past the entries compute entry ntr - 1 and store nothing
-    const int i = a.tpt[qc], k = a.tcam[qc];
-    const double X0 = a.X[3 * i], X1 = a.X[3 * i + 1], X2 = a.X[3 * i + 2];
-    R Vr[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) Vr[j] = (R)a.Vinv[9 * (size_t)i + j];
-    CamV c;
-    load_cam(a.cam, k, c);
-    Ev ev;
-    eval_obs(c, X0, X1, X2, a.ox[qc], a.oy[qc], a.defdiff, ev);
-    R Bc[2][9], Bp[2][3];
-    jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
-    const R W0 = (R)a.wx[qc], W1 = (R)a.wy[qc], W2 = (R)a.wxy[qc];
-    R Wv[27], Yv[27];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const R t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
@@ -460,9 +453,47 @@ __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
         Wv[r * 3] = w0;
         Wv[r * 3 + 1] = w1;
         Wv[r * 3 + 2] = w2;
+        if constexpr (WithY) {
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) Yv[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
+            for (int cc = 0; cc < 3; ++cc)
+                Yv[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
+        }
     }
+}
+
+// One thread per track entry: recomputes the observation's Jacobian (cheaper
+// than storing it) and writes W = Bc^T W Bp and Y = W V^-1 into the entry's
+// kRec-element records (R = double: 224 bytes; R = float, the FP32 form: 112
+// bytes).  Consecutive lanes write consecutive records, so a wavefront fills
+// whole cache lines.  WithY = false (the matrix-free form, a.ycm): only W;
+// the camera side forms Y in camera-major order (k_lin_chunks).
+template <typename R, bool WithY>
+__global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
+{
+    __shared__ uint4 stg[2][64][9];  // per wavefront: 64 records' staged pieces
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q0 = q - lane, nq = min(64, a.ntr - q0);
+    if (nq <= 0) return;  // the whole wavefront is past the entries
+    const int qc = min(q, a.ntr - 1);  // lanes past the entries compute entry ntr - 1 and store nothing
+    const int i = a.tpt[qc], k = a.tcam[qc];
+    const double X0 = a.X[3 * i], X1 = a.X[3 * i + 1], X2 = a.X[3 * i + 2];
+    CamV c;
+    load_cam(a.cam, k, c);
+    Ev ev;
+    eval_obs(c, X0, X1, X2, a.ox[qc], a.oy[qc], a.defdiff, ev);
+    R Bc[2][9], Bp[2][3];
+    jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
+    const R W0 = (R)a.wx[qc], W1 = (R)a.wy[qc], W2 = (R)a.wxy[qc];
+    R Vr[9], Wv[27], Yv[27];
+    if constexpr (!WithY) {
+        wy_values<R, false>(Bc, Bp, W0, W1, W2, Vr, Wv, Yv);
+        store_records_wave<R>((R *)a.Wrec, nullptr, q0, nq, Wv, stg[w], lane);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) Vr[j] = (R)a.Vinv[9 * (size_t)i + j];
+    wy_values<R, true>(Bc, Bp, W0, W1, W2, Vr, Wv, Yv);
     // whole records, padding included: every cache line is fully written
     store_records_wave<R>((R *)a.Wrec, nullptr, q0, nq, Wv, stg[w], lane);
     store_records_wave<R>((R *)a.Yrec, a.yrank, q0, nq, Yv, stg[w], lane);
@@ -528,6 +559,103 @@ __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
         }
 #pragma unroll
         for (int u = 0; u < kSchurU; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+    }
+    // D[i][j] in lane j + 16 (i % 4), element i / 4
+    double *D = a.Dd + (size_t)ch * 90;
+    if (i16 < 10) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = q + 4 * v;
+            if (i < 9) D[(i16 < 9) ? i * 9 + i16 : 81 + i] = acc[v];
+        }
+    }
+}
+
+// Matrix-free form, camera side of the records: one wavefront per chunk (at
+// most 64 entries of one image).  Lane e takes camera-major entry b + e: its
+// point, observation and weights from the camera-major entry table, its
+// point's 128-byte line (X, V^-1, g_p) and the image's parameters.  It
+// evaluates the observation and its Jacobian again and forms W and Y = W V^-1
+// with the point side's operations (wy_values: W equals the point-major
+// record, Y the record k_lin_records would write, bit for bit), and Y goes to
+// the entry's camera-major record in whole lines.  D = sum Y W^T and
+// e = sum Y g_p of the chunk then follow on the matrix cores exactly as in
+// k_schur_diag (the same K order, so the same bits), with the operands staged
+// in LDS 32 entries at a time instead of read from the records.  Reads per
+// entry: the table (28 bytes) and one point line (128); writes: Y.
+template <typename R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_lin_chunks(LinArgs a)
+{
+    constexpr int kOp = 57;                   // per staged entry: Y (27), W (27), g_p (3)
+    __shared__ __align__(16) double opnd[32 * kOp];
+    uint4 (*stg)[9] = (uint4 (*)[9])opnd;     // the record stores' staging (64 x 9 pieces) shares it
+    static_assert(sizeof(opnd) >= 64 * 9 * sizeof(uint4), "staging fits");
+    const int ch = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int i16 = lane & 15, q = lane >> 4;
+    const int4 c4 = a.ch4[ch];  // {first entry, entries, image, first chunk of the image}
+    const int b = c4.x, n = c4.y, k = c4.z;
+    const int j = b + min(lane, n - 1);  // lanes past the chunk's entries take its last entry and store nothing
+    const int p = a.cm_pt[j];
+    const float4 wv = a.cm_w[j];
+    const double2 *L = (const double2 *)(a.ptrec + 16 * (size_t)p);
+    double ln[16];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+        const double2 d = L[h];
+        ln[2 * h] = d.x;
+        ln[2 * h + 1] = d.y;
+    }
+    CamV c;
+    load_cam(a.cam, k, c);
+    Ev ev;
+    eval_obs(c, ln[0], ln[1], ln[2], a.cm_ox[j], a.cm_oy[j], a.defdiff, ev);
+    R Bc[2][9], Bp[2][3];
+    jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
+    R Vr[9], Wv[27], Yv[27];
+#pragma unroll
+    for (int x = 0; x < 9; ++x) Vr[x] = (R)ln[3 + x];
+    wy_values<R, true>(Bc, Bp, (R)wv.x, (R)wv.y, (R)wv.z, Vr, Wv, Yv);
+    store_records_wave<R>((R *)a.Yrec, nullptr, b, n, Yv, stg, lane);
+    f64x4 acc = {0., 0., 0., 0.};
+    for (int base = 0; base < n; base += 32) {
+        const int nr = min(32, n - base);
+        if (lane >= base && lane < base + nr) {
+            double *o = opnd + kOp * (lane - base);
+#pragma unroll
+            for (int x = 0; x < 27; ++x) {
+                o[x] = (double)Yv[x];
+                o[27 + x] = (double)Wv[x];
+            }
+            o[54] = ln[12];
+            o[55] = ln[13];
+            o[56] = ln[14];
+        }
+        wave_lds_sync();
+        const int kend = 3 * nr;
+        for (int k0 = 0; k0 < kend; k0 += 4 * kSchurU) {
+            double av[kSchurU], bv[kSchurU];
+#pragma unroll
+            for (int u = 0; u < kSchurU; ++u) {
+                const int kk = k0 + 4 * u + q;
+                double x = 0., y = 0.;
+                if (kk < kend) {
+                    const int ent = kk / 3, cc = kk - 3 * ent;
+                    const double *o = opnd + kOp * ent;
+                    if (i16 < 9) {
+                        x = o[3 * i16 + cc];
+                        y = o[27 + 3 * i16 + cc];
+                    } else if (i16 == 9) {
+                        y = o[54 + cc];
+                    }
+                }
+                av[u] = x;
+                bv[u] = y;
+            }
+#pragma unroll
+            for (int u = 0; u < kSchurU; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+        }
+        wave_lds_sync();
     }
     // D[i][j] in lane j + 16 (i % 4), element i / 4
     double *D = a.Dd + (size_t)ch * 90;
@@ -2589,12 +2717,19 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
 void launch_lin_points(const LinArgs &a, hipStream_t st)
 {
     if (a.npts <= 0 || a.npgrp <= 0) return;
+    const unsigned g = nblk(a.ntr, 128);
     if (a.rec32) {
         k_lin_points<float><<<a.npgrp, 256, 0, st>>>(a);
-        k_lin_records<float><<<nblk(a.ntr, 128), 128, 0, st>>>(a);
+        if (a.ycm)
+            k_lin_records<float, false><<<g, 128, 0, st>>>(a);
+        else
+            k_lin_records<float, true><<<g, 128, 0, st>>>(a);
     } else {
         k_lin_points<double><<<a.npgrp, 256, 0, st>>>(a);
-        k_lin_records<double><<<nblk(a.ntr, 128), 128, 0, st>>>(a);
+        if (a.ycm)
+            k_lin_records<double, false><<<g, 128, 0, st>>>(a);
+        else
+            k_lin_records<double, true><<<g, 128, 0, st>>>(a);
     }
 }
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
@@ -2611,6 +2746,13 @@ void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
 void launch_schur_diag(const LinArgs &a, hipStream_t st)
 {
     if (a.nch <= 0) return;
+    if (a.ycm) {
+        if (a.rec32)
+            k_lin_chunks<float><<<a.nch, 64, 0, st>>>(a);
+        else
+            k_lin_chunks<double><<<a.nch, 64, 0, st>>>(a);
+        return;
+    }
     if (a.rec32)
         k_schur_diag<float><<<a.nch, 64, 0, st>>>(a);
     else

@@ -80,7 +80,7 @@ struct rpba_ctx {
     DBuf<double> d_t, d_cpart, d_Dk;  // matrix-free form
     DBuf<unsigned char> d_Jpm, d_Jcm;  // whitened Jacobian records (precision 2)
     DBuf<double> d_dmp;                // damping of S_kk's diagonal (precision 2)
-    DBuf<double> d_ptrec;              // per point X, V^-1, g_p as one line (precision 2)
+    DBuf<double> d_ptrec;              // per point X, V^-1, g_p as one line (whitened and matrix-free forms)
     // camera-major entry table (point, observation, weights; chunk -> image),
     // rebuilt when the structure (cm_full) or the weights (cm_w) change
     DBuf<int> d_cmpt, d_chcam;
@@ -257,6 +257,7 @@ static int build_structure(rpba_ctx *c)
     } else {
         rc |= c->d_Wrec.alloc((size_t)kRec * esz * ntr);
         rc |= c->d_Yrec.alloc((size_t)kRec * esz * ntr);
+        if (c->implicit) rc |= c->d_ptrec.alloc(16 * std::max<size_t>(np, 1));
     }
     rc |= c->d_Vinv.alloc(9 * np);
     rc |= c->d_gp.alloc(3 * np);
@@ -401,6 +402,9 @@ static LinArgs lin_args(rpba_ctx *c)
     a.cm_pt = c->d_cmpt.p; a.cm_ox = c->d_cmox.p; a.cm_oy = c->d_cmoy.p;
     a.cm_w = c->d_cmw.p;
     a.jform = c->jform; a.Jpm = c->d_Jpm.p; a.Jcm = c->d_Jcm.p; a.dmp = c->d_dmp.p; a.ptrec = c->d_ptrec.p;
+    // matrix-free W / Y records: Y is formed on the camera side (k_lin_chunks)
+    a.ycm = c->implicit && !c->jform;
+    a.ch4 = c->ds.ch4;
     return a;
 }
 
