@@ -9,8 +9,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include <sched.h>
@@ -91,6 +93,20 @@ inline void parallel_ranges(long long n, int nr, F f)
     for (int r = 1; r < nr; ++r) th.emplace_back([=, &f] { f(n * r / nr, n * (r + 1) / nr, r); });
     f(0, n / nr, 0);
     for (std::thread &t : th) t.join();
+}
+
+// dst = src for large vectors of trivially copyable elements, the bytes
+// copied in nr ranges
+template <class D, class S>
+inline void parallel_assign(D &dst, const S &src, int nr)
+{
+    using T = typename S::value_type;
+    static_assert(std::is_same<T, typename D::value_type>::value, "same element type");
+    dst.resize(src.size());
+    const long long n = (long long)src.size();
+    parallel_ranges(n, nr, [&](long long i0, long long i1, int) {
+        if (i1 > i0) std::memcpy(dst.data() + i0, src.data() + i0, sizeof(T) * (size_t)(i1 - i0));
+    });
 }
 
 inline double cpu_seconds()

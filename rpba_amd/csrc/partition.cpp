@@ -1108,10 +1108,10 @@ int partitioned_solve(rpba_block *io, const rpba_flags *f, int P, rpba_log_fn lo
         S.logs(buf);
         const bool saved = sig0g <= sig0opt;
         if (saved) {
-            XXdold = S.XXd;
+            parallel_assign(XXdold, S.XXd, S.nthr);
             PMdold = S.PMd;
-            trkold = S.trk;
-            lenold = S.len;
+            parallel_assign(trkold, S.trk, S.nthr);
+            parallel_assign(lenold, S.len, S.nthr);
             cfold = S.cf;
             cdold = S.cd;
         }
