@@ -196,3 +196,26 @@ def test_batch_pcg_matches_single_block_solves():
         check_trajectory(st1["trace_sig0"], st0["trace_sig0"], st1["trace_pcg"], st0["trace_pcg"],
                          st1["trace_accept"], st0["trace_accept"], st1["trace_nrobs"], st0["trace_nrobs"])
         assert np.array_equal(out1["obs_kept"], out0["obs_kept"])
+
+
+def test_batch_pcg_spmv_bytes_accounting():
+    """rpba_batch_spmv_bytes over a timed solve: with one SpMV launch per PCG
+    iteration of the slowest running block, its bytes lie between the
+    per-camera terms alone (152 + 648 bytes a camera per launch) and the
+    whole union S read on every launch; with every block still running on
+    the first launch, more than the per-camera terms."""
+    blocks = _blocks(1400, [(10, 400), (22, 900), (6, 250), (30, 1400)])
+    b = api.Batch(blocks, capi.make_flags(robust=1, solver=0))
+    b.set_timing(True)
+    rc, n, fin = b.run(1)
+    sz = b.sizes()  # the union S of pass 0 (RobustDelete keeps every block)
+    rc, n, fin = b.run(-1)
+    assert rc == 0 and fin
+    spmv_bytes = b.spmv_bytes()
+    ms, launches = b.kernel_time("spmv")
+    b.close()
+    print(f"[batch spmv bytes] {spmv_bytes:.4g} bytes over {launches} launches, {ms:.3f} ms", flush=True)
+    assert launches > 0 and ms > 0
+    per_cam = 800.0 * sz["cams"]
+    assert spmv_bytes > per_cam * launches
+    assert spmv_bytes <= (648.0 * sz["blocks"] + per_cam) * launches
