@@ -109,7 +109,7 @@ typedef struct rpba_flags {
                                 2 = whichever streams fewer bytes per PCG
                                 iteration: 648 (B + N_c) with B bounded by
                                 min(N_c (N_c - 1) / 2, point pairs), against
-                                (2 record bytes + 8) N_o + 48 N_p (formed with
+                                (W and Y record bytes + 8) N_o + 48 N_p (formed with
                                 solver 1 or a corrected preconditioner)      */
     int32_t solver;          /* 0 = block-Jacobi PCG (SparsePCG, rbacore.cpp:1807-1914);
                                 1 = dense Cholesky of the reduced camera system on

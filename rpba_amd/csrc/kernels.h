@@ -5,10 +5,15 @@
 
 namespace rpba {
 
-// elements per entry record of W and Y (27 used): 224 bytes in FP64, 112 in
-// FP32, whole 16-byte pieces; the kernels that read a wavefront's records
+// elements per entry record of Y = W V^-1 (27 used): 224 bytes in FP64, 112
+// in FP32, whole 16-byte pieces; the kernels that read a wavefront's records
 // read them as one contiguous span
 constexpr int kRec = 28;
+// elements per entry record of W = Bc^T W Bp, kept as its two factors: the
+// rows of Bt = W Bc as pairs (t0_r, t1_r), r = 0..8 (18), then the two rows
+// of Bp (6); 192 bytes in FP64.  W[r][c] = t0_r Bp[0][c] + t1_r Bp[1][c],
+// the operations that form it, so every reader rebuilds it bit for bit.
+constexpr int kWRec = 24;
 
 // Upper off-diagonal blocks of S (9 x 9, block list blk_k/blk_l, row-major
 // order) are stored in strip slabs.  Strip s holds rows kSlab s .. kSlab s +
@@ -94,9 +99,10 @@ struct LinArgs {
     // with ycm X (3), V^-1 (9, row-major), g_p (3), 1 unused
     double *ptrec = nullptr;
     const int4 *pg4 = nullptr;  // per point group {p0, p1, e0, e1}
-    // matrix-free form with W / Y records: the point side writes W only and
-    // the point lines; k_lin_chunks forms Y in camera-major order and the
-    // chunks' diagonal Schur terms (needs the entry table, ptrec and ch4)
+    // matrix-free form with W / Y records: the point side writes W only, as
+    // its factors (kWRec elements), and the point lines; k_lin_chunks forms Y
+    // in camera-major order and the chunks' diagonal Schur terms (needs the
+    // entry table, ptrec and ch4)
     int ycm = 0;
     const int4 *ch4 = nullptr;  // per chunk {first entry, entries, image, first chunk of the image}
 };
@@ -178,6 +184,9 @@ struct StepArgs {
     int jform = 0;
     const void *Jpm = nullptr;
     const int4 *pg4 = nullptr;  // per point group {p0, p1, e0, e1}
+    // 1: the W records are their factors (kWRec elements: the matrix-free
+    // form with the camera side forming Y); 0: kRec-element records
+    int wfac = 0;
 };
 
 struct CostArgs {

@@ -222,21 +222,23 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// 16-byte piece j of a 27-value record (zero padding after value 26)
+// 16-byte piece j of a U-value record (zero padding after value U - 1)
+template <int U>
 __device__ __forceinline__ uint4 rec_piece(const double *v, int j)
 {
     double2 d;
-    d.x = (2 * j < 27) ? v[2 * j] : 0.;
-    d.y = (2 * j + 1 < 27) ? v[2 * j + 1] : 0.;
+    d.x = (2 * j < U) ? v[2 * j] : 0.;
+    d.y = (2 * j + 1 < U) ? v[2 * j + 1] : 0.;
     return __builtin_bit_cast(uint4, d);
 }
+template <int U>
 __device__ __forceinline__ uint4 rec_piece(const float *v, int j)
 {
     float4 f;
-    f.x = (4 * j < 27) ? v[4 * j] : 0.f;
-    f.y = (4 * j + 1 < 27) ? v[4 * j + 1] : 0.f;
-    f.z = (4 * j + 2 < 27) ? v[4 * j + 2] : 0.f;
-    f.w = (4 * j + 3 < 27) ? v[4 * j + 3] : 0.f;
+    f.x = (4 * j < U) ? v[4 * j] : 0.f;
+    f.y = (4 * j + 1 < U) ? v[4 * j + 1] : 0.f;
+    f.z = (4 * j + 2 < U) ? v[4 * j + 2] : 0.f;
+    f.w = (4 * j + 3 < U) ? v[4 * j + 3] : 0.f;
     return __builtin_bit_cast(uint4, f);
 }
 
@@ -252,9 +254,12 @@ constexpr int kMfChunkWaves = 4;  // chunks (wavefronts) per workgroup of k_mf_c
 // the span's image, RP records (RP NP pieces) per pass, lanes of the pass's
 // records read theirs.  Every lane of the wavefront calls both; lanes past nq
 // get nothing.
-template <typename R>
+// E: elements stored per record (kRec for Y and W V^-1 style records, kWRec
+// for the factored W records); U: the values used (the rest is padding)
+template <typename R, int E = kRec>
 struct RecSpan {
-    static constexpr int NP = kRec * (int)sizeof(R) / 16;  // pieces per record: 14 (FP64) or 7 (FP32)
+    static constexpr int U = E == kRec ? 27 : E;
+    static constexpr int NP = E * (int)sizeof(R) / 16;     // pieces per record: 14 / 12 (FP64) or 7 / 6 (FP32)
     static constexpr int NPASS = sizeof(R) == 8 ? 2 : 1;
     static constexpr int RP = 64 / NPASS;                  // records per pass
     static constexpr int IPP = NP / NPASS;                 // load instructions per pass (RP NP / 64)
@@ -262,11 +267,12 @@ struct RecSpan {
     u32x4 raw[NP];                                         // 64 records x NP pieces / 64 lanes
 };
 static_assert(kRec * 4 % 16 == 0 && (kRec * 8 / 16) % 2 == 0, "records are whole pieces, two passes of whole loads");
+static_assert(kWRec * 4 % 16 == 0 && (kWRec * 8 / 16) % 2 == 0, "records are whole pieces, two passes of whole loads");
 
-template <typename R>
-__device__ __forceinline__ void span_issue(const R *recs, int q0, int qlast, RecSpan<R> &w, int lane)
+template <typename R, int E = kRec>
+__device__ __forceinline__ void span_issue(const R *recs, int q0, int qlast, RecSpan<R, E> &w, int lane)
 {
-    using S = RecSpan<R>;
+    using S = RecSpan<R, E>;
     // piece 64 i + lane of the span, clamped onto the last piece of record
     // qlast (the span's pieces past it are dropped)
     const u32x4 *base = (const u32x4 *)recs + (size_t)q0 * S::NP;
@@ -275,10 +281,11 @@ __device__ __forceinline__ void span_issue(const R *recs, int q0, int qlast, Rec
     for (int i = 0; i < S::NP; ++i) w.raw[i] = base[min(64 * i + lane, last)];
 }
 
-template <typename R>
-__device__ __forceinline__ void span_transpose(const RecSpan<R> &w, int nq, R *v, u32x4 *stg, int lane)
+template <typename R, int E = kRec>
+__device__ __forceinline__ void span_transpose(const RecSpan<R, E> &w, int nq, R *v, u32x4 *stg, int lane)
 {
-    using S = RecSpan<R>;
+    using S = RecSpan<R, E>;
+    constexpr int U = S::U;
 #pragma unroll
     for (int h = 0; h < S::NPASS; ++h) {
 #pragma unroll
@@ -291,14 +298,14 @@ __device__ __forceinline__ void span_transpose(const RecSpan<R> &w, int nq, R *v
                 const u32x4 u = stg[S::NP * rl + pc];
                 if constexpr (sizeof(R) == 8) {
                     const double2 d = __builtin_bit_cast(double2, u);
-                    if (2 * pc < 27) v[2 * pc] = d.x;
-                    if (2 * pc + 1 < 27) v[2 * pc + 1] = d.y;
+                    if (2 * pc < U) v[2 * pc] = d.x;
+                    if (2 * pc + 1 < U) v[2 * pc + 1] = d.y;
                 } else {
                     const float4 f = __builtin_bit_cast(float4, u);
-                    if (4 * pc < 27) v[4 * pc] = f.x;
-                    if (4 * pc + 1 < 27) v[4 * pc + 1] = f.y;
-                    if (4 * pc + 2 < 27) v[4 * pc + 2] = f.z;
-                    if (4 * pc + 3 < 27) v[4 * pc + 3] = f.w;
+                    if (4 * pc < U) v[4 * pc] = f.x;
+                    if (4 * pc + 1 < U) v[4 * pc + 1] = f.y;
+                    if (4 * pc + 2 < U) v[4 * pc + 2] = f.z;
+                    if (4 * pc + 3 < U) v[4 * pc + 3] = f.w;
                 }
             }
         }
@@ -313,17 +320,18 @@ __device__ __forceinline__ void span_transpose(const RecSpan<R> &w, int nq, R *v
 // storing its own record would touch 64 lines per instruction).  Record of
 // entry q at rank[q] (or q).  Every lane of the wavefront calls this; lanes
 // past the entries (lane >= nq) hold no record.
-template <typename R>
+template <typename R, int E = kRec>
 __device__ __forceinline__ void store_records_wave(R *base, const int *rank, int q0, int nq, const R *v, uint4 (*stg)[9],
                                                    int lane)
 {
-    constexpr int NP = kRec * (int)sizeof(R) / 16;  // pieces per record: 14 (FP64) or 7 (FP32)
+    constexpr int NP = E * (int)sizeof(R) / 16;  // pieces per record: 14 / 12 (FP64) or 7 / 6 (FP32)
+    constexpr int U = E == kRec ? 27 : E;
 #pragma unroll
     for (int h = 0; h < (NP + 7) / 8; ++h) {
         if (lane < nq) {
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                if (8 * h + j < NP) stg[lane][j] = rec_piece(v, 8 * h + j);
+                if (8 * h + j < NP) stg[lane][j] = rec_piece<U>(v, 8 * h + j);
         }
         wave_lds_sync();
 #pragma unroll
@@ -331,7 +339,7 @@ __device__ __forceinline__ void store_records_wave(R *base, const int *rank, int
             const int rr = 8 * p + (lane >> 3), pp = lane & 7;
             if (rr < nq && 8 * h + pp < NP) {
                 const size_t rec = rank ? (size_t)rank[q0 + rr] : (size_t)(q0 + rr);
-                ((uint4 *)(base + rec * kRec))[8 * h + pp] = stg[rr][pp];
+                ((uint4 *)(base + rec * E))[8 * h + pp] = stg[rr][pp];
             }
         }
         wave_lds_sync();
@@ -461,12 +469,38 @@ __device__ __forceinline__ void wy_values(const R (&Bc)[2][9], const R (&Bp)[2][
     }
 }
 
+// W of one entry as its factors (the kWRec-element record): Bt = W Bc as
+// pairs (t0_r, t1_r), formed as in wy_values, then the rows of Bp
+template <typename R>
+__device__ __forceinline__ void w_factors(const R (&Bc)[2][9], const R (&Bp)[2][3], R W0, R W1, R W2, R (&Wf)[kWRec])
+{
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        Wf[2 * r] = Bc[0][r] * W0 + Bc[1][r] * W2;
+        Wf[2 * r + 1] = Bc[0][r] * W2 + Bc[1][r] * W1;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Wf[18 + c] = Bp[0][c];
+        Wf[21 + c] = Bp[1][c];
+    }
+}
+
+// element (r, c) of W from its factors: the operations of wy_values, so the
+// same bits
+template <typename R>
+__device__ __forceinline__ R w_elem(const R *Wf, int r, int c)
+{
+    return Wf[2 * r] * Wf[18 + c] + Wf[2 * r + 1] * Wf[21 + c];
+}
+
 // One thread per track entry: recomputes the observation's Jacobian (cheaper
 // than storing it) and writes W = Bc^T W Bp and Y = W V^-1 into the entry's
 // kRec-element records (R = double: 224 bytes; R = float, the FP32 form: 112
 // bytes).  Consecutive lanes write consecutive records, so a wavefront fills
-// whole cache lines.  WithY = false (the matrix-free form, a.ycm): only W;
-// the camera side forms Y in camera-major order (k_lin_chunks).
+// whole cache lines.  WithY = false (the matrix-free form, a.ycm): only W,
+// as its factors (kWRec elements, 192 bytes in FP64); the camera side forms
+// Y in camera-major order (k_lin_chunks).
 template <typename R, bool WithY>
 __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
 {
@@ -487,8 +521,9 @@ __global__ __launch_bounds__(128) void k_lin_records(LinArgs a)
     const R W0 = (R)a.wx[qc], W1 = (R)a.wy[qc], W2 = (R)a.wxy[qc];
     R Vr[9], Wv[27], Yv[27];
     if constexpr (!WithY) {
-        wy_values<R, false>(Bc, Bp, W0, W1, W2, Vr, Wv, Yv);
-        store_records_wave<R>((R *)a.Wrec, nullptr, q0, nq, Wv, stg[w], lane);
+        R Wf[kWRec];
+        w_factors<R>(Bc, Bp, W0, W1, W2, Wf);
+        store_records_wave<R, kWRec>((R *)a.Wrec, nullptr, q0, nq, Wf, stg[w], lane);
         return;
     }
 #pragma unroll
@@ -1219,29 +1254,29 @@ __global__ __launch_bounds__(256) void k_mf_points(PcgArgs a)
         pe = a.toff[p0 + t + 1];
     }
     double t0 = 0., t1 = 0., t2 = 0.;
-    const R *Wr = (const R *)a.Wrec;
-    __shared__ u32x4 stg[4][RecSpan<R>::LDS];  // per wavefront
+    const R *Wr = (const R *)a.Wrec;  // W as its factors (kWRec elements)
+    __shared__ u32x4 stg[4][RecSpan<R, kWRec>::LDS];  // per wavefront
     const int lane = t & 63, wv = t >> 6;
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
         const int cam = a.tcam[min(q, e1 - 1)];
         const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
-        RecSpan<R> rw;
-        span_issue<R>(Wr, q0, e1 - 1, rw, lane);
+        RecSpan<R, kWRec> rw;
+        span_issue<R, kWRec>(Wr, q0, e1 - 1, rw, lane);
         double dv[9];
         const double *dp = a.d + 9 * (size_t)cam;
 #pragma unroll
         for (int r = 0; r < 9; ++r) dv[r] = dp[r];
-        R W[28];
-        span_transpose<R>(rw, nq, W, stg[wv], lane);
+        R Wf[kWRec];
+        span_transpose<R, kWRec>(rw, nq, Wf, stg[wv], lane);
         if (q < e1) {
             double u0 = 0., u1 = 0., u2 = 0.;
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
                 const double dr = dv[r];
-                u0 += (double)W[3 * r] * dr;
-                u1 += (double)W[3 * r + 1] * dr;
-                u2 += (double)W[3 * r + 2] * dr;
+                u0 += (double)w_elem(Wf, r, 0) * dr;
+                u1 += (double)w_elem(Wf, r, 1) * dr;
+                u2 += (double)w_elem(Wf, r, 2) * dr;
             }
             su[t][0] = u0;
             su[t][1] = u1;
@@ -1663,7 +1698,7 @@ __global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
 
 // dp_i = V_i^-1 (g_i - sum_q W_q^T dc_k) with the double camera solution
 // (rbacore.cpp:1395-1410); Xn = X - dp (rbacore.cpp:547)
-template <typename R>
+template <typename R, bool WF>
 __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
 {
     // a workgroup per point group (first entry in a kPtGroup-entry window);
@@ -1685,29 +1720,38 @@ __global__ __launch_bounds__(256) void k_backsub(StepArgs a)
         t1 = a.gp[3 * (size_t)i + 1];
         t2 = a.gp[3 * (size_t)i + 2];
     }
+    // WF: W as its factors (kWRec elements, the matrix-free form), else the
+    // kRec-element record
+    constexpr int E = WF ? kWRec : kRec;
     const R *Wr = (const R *)a.Wrec;
-    __shared__ u32x4 stg[4][RecSpan<R>::LDS];  // per wavefront
+    __shared__ u32x4 stg[4][RecSpan<R, E>::LDS];  // per wavefront
     const int lane = t & 63, wv = t >> 6;
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
         const int cam = a.tcam[min(q, e1 - 1)];
         const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
-        RecSpan<R> rw;
-        span_issue<R>(Wr, q0, e1 - 1, rw, lane);
+        RecSpan<R, E> rw;
+        span_issue<R, E>(Wr, q0, e1 - 1, rw, lane);
         double xc[9];
         const double *xp = a.sol + 9 * (size_t)cam;
 #pragma unroll
         for (int r = 0; r < 9; ++r) xc[r] = xp[r];
-        R W[28];
-        span_transpose<R>(rw, nq, W, stg[wv], lane);
+        R W[E];
+        span_transpose<R, E>(rw, nq, W, stg[wv], lane);
         if (q < e1) {
             double u0 = 0., u1 = 0., u2 = 0.;
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
                 const double xr = xc[r];
-                u0 += (double)W[r * 3] * xr;
-                u1 += (double)W[r * 3 + 1] * xr;
-                u2 += (double)W[r * 3 + 2] * xr;
+                if constexpr (WF) {
+                    u0 += (double)w_elem(W, r, 0) * xr;
+                    u1 += (double)w_elem(W, r, 1) * xr;
+                    u2 += (double)w_elem(W, r, 2) * xr;
+                } else {
+                    u0 += (double)W[r * 3] * xr;
+                    u1 += (double)W[r * 3 + 1] * xr;
+                    u2 += (double)W[r * 3 + 2] * xr;
+                }
             }
             su[t][0] = u0;
             su[t][1] = u1;
@@ -2875,10 +2919,16 @@ void launch_backsub(const StepArgs &a, hipStream_t st)
             k_backsub_j<float><<<a.npgrp, 256, 0, st>>>(a);
         else
             k_backsub_j<double><<<a.npgrp, 256, 0, st>>>(a);
-    } else if (a.rec32)
-        k_backsub<float><<<a.npgrp, 256, 0, st>>>(a);
-    else
-        k_backsub<double><<<a.npgrp, 256, 0, st>>>(a);
+    } else if (a.wfac) {
+        if (a.rec32)
+            k_backsub<float, true><<<a.npgrp, 256, 0, st>>>(a);
+        else
+            k_backsub<double, true><<<a.npgrp, 256, 0, st>>>(a);
+    } else if (a.rec32) {
+        k_backsub<float, false><<<a.npgrp, 256, 0, st>>>(a);
+    } else {
+        k_backsub<double, false><<<a.npgrp, 256, 0, st>>>(a);
+    }
 }
 void launch_decode(const StepArgs &a, hipStream_t st) { k_decode<<<nblk(a.nc, 64), 64, 0, st>>>(a); }
 int cost_blocks(int ntr) { return (int)nblk(ntr > 0 ? ntr : 1, 256); }

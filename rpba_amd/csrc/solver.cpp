@@ -451,6 +451,7 @@ static StepArgs step_args(rpba_ctx *c, const double *sol)
     a.Vinv = c->d_Vinv.p; a.gp = c->d_gp.p; a.sol = sol; a.X = c->d_X.p; a.Xn = c->d_Xn.p;
     a.S = c->d_pcg.p;
     a.jform = c->jform; a.Jpm = c->d_Jpm.p; a.pg4 = c->ds.pg4;
+    a.wfac = c->implicit && !c->jform;  // the matrix-free form keeps W as its factors
     return a;
 }
 
@@ -1022,7 +1023,8 @@ static int ctx_init(rpba_ctx *c)
         // the product of the PCG by bytes per iteration: S formed streams
         // 648 (B + N_c) bytes (B <= min(N_c (N_c - 1) / 2, point pairs)), the
         // matrix-free product the W and Y records and the point shares,
-        // (2 record bytes + 48) N_o; the sums are the same in another order
+        // (W and Y record bytes + 8) N_o + 48 N_p; the sums are the same in
+        // another order
         std::vector<int> toff((size_t)c->ds.npts + 1);
         HIPCHK(hipMemcpyAsync(toff.data(), c->ds.toff, sizeof(int) * toff.size(), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
@@ -1033,10 +1035,10 @@ static int ctx_init(rpba_ctx *c)
         }
         const double bup = std::min(0.5 * (double)nc * (double)(nc - 1), pairs);
         const double formed = 648. * (bup + (double)nc);
-        // matrix-free: both records, the image and point index of every
-        // entry, and t written and read per point
-        const double rec = (double)kRec * (c->rec32 ? 4. : 8.);
-        const double free_ = (2. * rec + 8.) * (double)c->ds.ntr + 48. * (double)c->ds.npts;
+        // matrix-free: both records (W as its factors), the image and point
+        // index of every entry, and t written and read per point
+        const double esz = c->rec32 ? 4. : 8.;
+        const double free_ = ((double)(kRec + kWRec) * esz + 8.) * (double)c->ds.ntr + 48. * (double)c->ds.npts;
         c->implicit = c->jform || (c->fl.solver == 0 && c->fl.preconditioner == 0 && free_ < formed);
         c->ds.with_pairs = !c->implicit;
     }
