@@ -347,14 +347,70 @@ __device__ __forceinline__ void store_records_wave(R *base, const int *rank, int
 }
 
 
+// W = Bc^T W Bp (9 x 3) of one entry and, with WithY, Y = W V^-1, in R: the
+// values of its records.  The point side (k_lin_records) and the camera side
+// of the matrix-free form (k_lin_chunks) both form them here from the same
+// inputs, so the two agree bit for bit.
+template <typename R, bool WithY>
+__device__ __forceinline__ void wy_values(const R (&Bc)[2][9], const R (&Bp)[2][3], R W0, R W1, R W2, const R (&Vr)[9],
+                                          R (&Wv)[27], R (&Yv)[27])
+{
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const R t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
+        const R t1 = Bc[0][r] * W2 + Bc[1][r] * W1;
+        const R w0 = t0 * Bp[0][0] + t1 * Bp[1][0];
+        const R w1 = t0 * Bp[0][1] + t1 * Bp[1][1];
+        const R w2 = t0 * Bp[0][2] + t1 * Bp[1][2];
+        Wv[r * 3] = w0;
+        Wv[r * 3 + 1] = w1;
+        Wv[r * 3 + 2] = w2;
+        if constexpr (WithY) {
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+                Yv[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
+        }
+    }
+}
+
+// W of one entry as its factors (the kWRec-element record): Bt = W Bc as
+// pairs (t0_r, t1_r), formed as in wy_values, then the rows of Bp
+template <typename R>
+__device__ __forceinline__ void w_factors(const R (&Bc)[2][9], const R (&Bp)[2][3], R W0, R W1, R W2, R (&Wf)[kWRec])
+{
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        Wf[2 * r] = Bc[0][r] * W0 + Bc[1][r] * W2;
+        Wf[2 * r + 1] = Bc[0][r] * W2 + Bc[1][r] * W1;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Wf[18 + c] = Bp[0][c];
+        Wf[21 + c] = Bp[1][c];
+    }
+}
+
+// element (r, c) of W from its factors: the operations of wy_values, so the
+// same bits
+template <typename R>
+__device__ __forceinline__ R w_elem(const R *Wf, int r, int c)
+{
+    return Wf[2 * r] * Wf[18 + c] + Wf[2 * r + 1] * Wf[21 + c];
+}
+
 // V = sum Bp^T W Bp and g_p = sum Bp^T W r per point, then damping and the
 // 3x3 inverse (rbacore.cpp:1341-1344).  A workgroup per point group (first
 // entry in a kPtGroup-entry window): lane per entry computes its terms
 // (to LDS), then each point's lane adds its entries in order.  Sums are FP64.
-template <typename R>
+// WithW (the matrix-free form): each lane also writes its entry's W record,
+// as its factors (w_factors from jac_obs, whose Bp is jac_point's to the bit),
+// in whole lines; the records' staging shares the LDS of the entry terms.
+template <typename R, bool WithW>
 __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
 {
-    __shared__ double sv[256][12];
+    constexpr size_t kSv = sizeof(double) * 256 * 12, kStg = WithW ? sizeof(uint4) * 4 * 64 * 9 : 0;
+    __shared__ __align__(16) unsigned char lds[kSv > kStg ? kSv : kStg];
+    double(*sv)[12] = (double(*)[12])lds;
     const int t = threadIdx.x, c = blockIdx.x;
     const int4 grp = point_group(a.pg4, a.pgrp, a.toff, c);
     const int p0 = grp.x, p1 = grp.y, e0 = grp.z, e1 = grp.w;
@@ -369,6 +425,8 @@ __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
     double V[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
     for (int base = e0; base < e1; base += 256) {
         const int q = base + t;
+        double tv[12];
+        R Wf[kWRec];
         if (q < e1) {
             const int k = a.tcam[q], ip = a.tpt[q];
             CamV cv;
@@ -376,16 +434,34 @@ __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
             Ev ev;
             eval_obs(cv, a.X[3 * (size_t)ip], a.X[3 * (size_t)ip + 1], a.X[3 * (size_t)ip + 2], a.ox[q], a.oy[q], a.defdiff, ev);
             R Bp[2][3];
-            jac_point<R>(cv, ev, Bp);
             const R W0 = (R)a.wx[q], W1 = (R)a.wy[q], W2 = (R)a.wxy[q];
+            if constexpr (WithW) {
+                R Bc[2][9];
+                jac_obs<R>(cv, ev, a.cam.mask[k], Bc, Bp);
+                w_factors<R>(Bc, Bp, W0, W1, W2, Wf);
+            } else {
+                jac_point<R>(cv, ev, Bp);
+            }
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 const R t0 = Bp[0][r] * W0 + Bp[1][r] * W2;
                 const R t1 = Bp[0][r] * W2 + Bp[1][r] * W1;
 #pragma unroll
-                for (int cc = 0; cc < 3; ++cc) sv[t][r * 3 + cc] = (double)(t0 * Bp[0][cc] + t1 * Bp[1][cc]);
-                sv[t][9 + r] = (double)t0 * ev.d1 + (double)t1 * ev.d2;
+                for (int cc = 0; cc < 3; ++cc) tv[r * 3 + cc] = (double)(t0 * Bp[0][cc] + t1 * Bp[1][cc]);
+                tv[9 + r] = (double)t0 * ev.d1 + (double)t1 * ev.d2;
             }
+        }
+        if constexpr (WithW) {
+            // the wavefront's 64 entries' records, then the LDS is the terms'
+            const int lane = t & 63, wv = t >> 6;
+            const int q0 = base + 64 * wv, nq = min(64, max(0, e1 - q0));
+            store_records_wave<R, kWRec>((R *)a.Wrec, nullptr, q0, nq, Wf,
+                                         (uint4(*)[9])(lds + sizeof(uint4) * 64 * 9 * wv), lane);
+            __syncthreads();
+        }
+        if (q < e1) {
+#pragma unroll
+            for (int j = 0; j < 12; ++j) sv[t][j] = tv[j];
         }
         __syncthreads();
         if (t < np) {
@@ -441,57 +517,6 @@ __global__ __launch_bounds__(256) void k_lin_points(LinArgs a)
         L[6] = make_double2(g[0], g[1]);
         L[7] = make_double2(g[2], 0.);
     }
-}
-
-// W = Bc^T W Bp (9 x 3) of one entry and, with WithY, Y = W V^-1, in R: the
-// values of its records.  The point side (k_lin_records) and the camera side
-// of the matrix-free form (k_lin_chunks) both form them here from the same
-// inputs, so the two agree bit for bit.
-template <typename R, bool WithY>
-__device__ __forceinline__ void wy_values(const R (&Bc)[2][9], const R (&Bp)[2][3], R W0, R W1, R W2, const R (&Vr)[9],
-                                          R (&Wv)[27], R (&Yv)[27])
-{
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        const R t0 = Bc[0][r] * W0 + Bc[1][r] * W2;
-        const R t1 = Bc[0][r] * W2 + Bc[1][r] * W1;
-        const R w0 = t0 * Bp[0][0] + t1 * Bp[1][0];
-        const R w1 = t0 * Bp[0][1] + t1 * Bp[1][1];
-        const R w2 = t0 * Bp[0][2] + t1 * Bp[1][2];
-        Wv[r * 3] = w0;
-        Wv[r * 3 + 1] = w1;
-        Wv[r * 3 + 2] = w2;
-        if constexpr (WithY) {
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc)
-                Yv[r * 3 + cc] = w0 * Vr[0 * 3 + cc] + w1 * Vr[1 * 3 + cc] + w2 * Vr[2 * 3 + cc];
-        }
-    }
-}
-
-// W of one entry as its factors (the kWRec-element record): Bt = W Bc as
-// pairs (t0_r, t1_r), formed as in wy_values, then the rows of Bp
-template <typename R>
-__device__ __forceinline__ void w_factors(const R (&Bc)[2][9], const R (&Bp)[2][3], R W0, R W1, R W2, R (&Wf)[kWRec])
-{
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        Wf[2 * r] = Bc[0][r] * W0 + Bc[1][r] * W2;
-        Wf[2 * r + 1] = Bc[0][r] * W2 + Bc[1][r] * W1;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        Wf[18 + c] = Bp[0][c];
-        Wf[21 + c] = Bp[1][c];
-    }
-}
-
-// element (r, c) of W from its factors: the operations of wy_values, so the
-// same bits
-template <typename R>
-__device__ __forceinline__ R w_elem(const R *Wf, int r, int c)
-{
-    return Wf[2 * r] * Wf[18 + c] + Wf[2 * r + 1] * Wf[21 + c];
 }
 
 // One thread per track entry: recomputes the observation's Jacobian (cheaper
@@ -2761,19 +2786,23 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
 void launch_lin_points(const LinArgs &a, hipStream_t st)
 {
     if (a.npts <= 0 || a.npgrp <= 0) return;
+    // the matrix-free form: W (its factors) written by the point kernel;
+    // else W and Y by k_lin_records, which needs V^-1
     const unsigned g = nblk(a.ntr, 128);
     if (a.rec32) {
-        k_lin_points<float><<<a.npgrp, 256, 0, st>>>(a);
-        if (a.ycm)
-            k_lin_records<float, false><<<g, 128, 0, st>>>(a);
-        else
+        if (a.ycm) {
+            k_lin_points<float, true><<<a.npgrp, 256, 0, st>>>(a);
+        } else {
+            k_lin_points<float, false><<<a.npgrp, 256, 0, st>>>(a);
             k_lin_records<float, true><<<g, 128, 0, st>>>(a);
+        }
     } else {
-        k_lin_points<double><<<a.npgrp, 256, 0, st>>>(a);
-        if (a.ycm)
-            k_lin_records<double, false><<<g, 128, 0, st>>>(a);
-        else
+        if (a.ycm) {
+            k_lin_points<double, true><<<a.npgrp, 256, 0, st>>>(a);
+        } else {
+            k_lin_points<double, false><<<a.npgrp, 256, 0, st>>>(a);
             k_lin_records<double, true><<<g, 128, 0, st>>>(a);
+        }
     }
 }
 void launch_schur_pairs(const SchurArgs &s, hipStream_t st)
