@@ -75,9 +75,8 @@ def algorithmic_bytes(kernel, sz):
     if kernel == "spmv" and jrec:  # whitened records twice (point- and camera-major) and two indices per entry;
         # V^-1 in, s out and s in per point; S_kk-diagonal damping, d and Ad per camera
         return 2 * jrec * No + 8 * No + (72 + 24 + 24) * Np + 3 * 72 * Nc
-    if kernel == "spmv" and B == 0:  # matrix-free: W (its 24 factors) and Y (27 doubles) and two indices per
-        return (192 + 216) * No + 8 * No + 48 * Np + 1296 * Nc + 144 * Nc  # entry, t out and in per point,
-        # S_kk + D_k, d and Ad
+    if kernel == "spmv" and B == 0:  # matrix-free: W and Y (24 factors each) and two indices per entry, t
+        return 2 * 192 * No + 8 * No + 48 * Np + 1296 * Nc + 144 * Nc  # out and in per point, S_kk + D_k, d, Ad
     if kernel == "spmv":  # S blocks once (upper + diagonal), d in, Ad out, row partials
         return 648 * (B + Nc) + 72 * Nc + 72 * Nc + 8 * Nc
     if kernel == "schur_pairs":  # Y and W of every entry, pair lists, block offsets, S blocks out
@@ -87,7 +86,7 @@ def algorithmic_bytes(kernel, sz):
     if kernel == "lin_points":  # entry inputs, points, W and Y out, V^-1 and g_p out
         return (4 + 8 + 12) * No + 432 * No + (24 + 8) * Np + (72 + 24) * Np
     if kernel == "schur_diag" and B == 0:  # matrix-free (k_lin_chunks): the entry table, one point line per
-        return (28 + 216) * No + 128 * Np + 720 * Nc  # point, Y out, the chunks' diagonal terms out
+        return (28 + 192) * No + 128 * Np + 720 * Nc  # point, Y (its 24 factors) out, the chunks' diagonal terms
     if kernel == "schur_diag":  # Y, W and g_p of every entry, diagonal terms out
         return (432 + 4 + 4) * No + 24 * Np + 720 * Nc
     if kernel == "lin_cameras":  # entry inputs and points in; S_kk and b out

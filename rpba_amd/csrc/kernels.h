@@ -9,10 +9,12 @@ namespace rpba {
 // in FP32, whole 16-byte pieces; the kernels that read a wavefront's records
 // read them as one contiguous span
 constexpr int kRec = 28;
-// elements per entry record of W = Bc^T W Bp, kept as its two factors: the
-// rows of Bt = W Bc as pairs (t0_r, t1_r), r = 0..8 (18), then the two rows
-// of Bp (6); 192 bytes in FP64.  W[r][c] = t0_r Bp[0][c] + t1_r Bp[1][c],
-// the operations that form it, so every reader rebuilds it bit for bit.
+// elements per entry record of W = Bc^T W Bp in the matrix-free form, kept as
+// its two factors: the rows of Bt = W Bc as pairs (t0_r, t1_r), r = 0..8
+// (18), then the two rows of Bp (6); 192 bytes in FP64.  W[r][c] =
+// t0_r Bp[0][c] + t1_r Bp[1][c], the operations that form it, so every reader
+// rebuilds it bit for bit.  The matrix-free form's Y = W V^-1 is kept the
+// same way as Bt^T P with P = Bp V^-1 (the Bt pairs, then P).
 constexpr int kWRec = 24;
 
 // Upper off-diagonal blocks of S (9 x 9, block list blk_k/blk_l, row-major
@@ -101,8 +103,8 @@ struct LinArgs {
     const int4 *pg4 = nullptr;  // per point group {p0, p1, e0, e1}
     // matrix-free form with W / Y records: the point side writes W only, as
     // its factors (kWRec elements), and the point lines; k_lin_chunks forms Y
-    // in camera-major order and the chunks' diagonal Schur terms (needs the
-    // entry table, ptrec and ch4)
+    // (as Bt^T P, kWRec elements) in camera-major order and the chunks'
+    // diagonal Schur terms (needs the entry table, ptrec and ch4)
     int ycm = 0;
     const int4 *ch4 = nullptr;  // per chunk {first entry, entries, image, first chunk of the image}
 };

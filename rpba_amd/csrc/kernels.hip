@@ -635,14 +635,13 @@ __global__ __launch_bounds__(64) void k_schur_diag(LinArgs a)
 // most 64 entries of one image).  Lane e takes camera-major entry b + e: its
 // point, observation and weights from the camera-major entry table, its
 // point's 128-byte line (X, V^-1, g_p) and the image's parameters.  It
-// evaluates the observation and its Jacobian again and forms W and Y = W V^-1
-// with the point side's operations (wy_values: W equals the point-major
-// record, Y the record k_lin_records would write, bit for bit), and Y goes to
-// the entry's camera-major record in whole lines.  D = sum Y W^T and
-// e = sum Y g_p of the chunk then follow on the matrix cores exactly as in
-// k_schur_diag (the same K order, so the same bits), with the operands staged
-// in LDS 32 entries at a time instead of read from the records.  Reads per
-// entry: the table (28 bytes) and one point line (128); writes: Y.
+// evaluates the observation and its Jacobian again and forms W with the point
+// side's operations (w_factors: W equals the point-major record bit for bit)
+// and Y = W V^-1 as Bt^T (Bp V^-1), and Y goes to the entry's camera-major
+// record (its factors, kWRec elements) in whole lines.  D = sum Y W^T and
+// e = sum Y g_p of the chunk then follow on the matrix cores in the K order of
+// k_schur_diag, with the operands staged in LDS 32 entries at a time.  Reads
+// per entry: the table (28 bytes) and one point line (128); writes: Y.
 template <typename R>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_lin_chunks(LinArgs a)
 {
@@ -672,11 +671,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     eval_obs(c, ln[0], ln[1], ln[2], a.cm_ox[j], a.cm_oy[j], a.defdiff, ev);
     R Bc[2][9], Bp[2][3];
     jac_obs<R>(c, ev, a.cam.mask[k], Bc, Bp);
-    R Vr[9], Wv[27], Yv[27];
+    // W as its factors (Bt pairs, Bp: the point-major record's values) and
+    // Y = W V^-1 as Bt^T P with P = Bp V^-1 (2 x 3): the Y record holds the
+    // Bt pairs and P (kWRec elements), and D below is formed from these same
+    // Y values, so the product's own-camera terms cancel D_k exactly
+    R Wf[kWRec], Yf[kWRec], Wv[27], Yv[27];
+    w_factors<R>(Bc, Bp, (R)wv.x, (R)wv.y, (R)wv.z, Wf);
 #pragma unroll
-    for (int x = 0; x < 9; ++x) Vr[x] = (R)ln[3 + x];
-    wy_values<R, true>(Bc, Bp, (R)wv.x, (R)wv.y, (R)wv.z, Vr, Wv, Yv);
-    store_records_wave<R>((R *)a.Yrec, nullptr, b, n, Yv, stg, lane);
+    for (int x = 0; x < 18; ++x) Yf[x] = Wf[x];
+#pragma unroll
+    for (int sr = 0; sr < 2; ++sr)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            Yf[18 + 3 * sr + cc] = Bp[sr][0] * (R)ln[3 + cc] + Bp[sr][1] * (R)ln[6 + cc] + Bp[sr][2] * (R)ln[9 + cc];
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            Wv[3 * r + cc] = w_elem(Wf, r, cc);
+            Yv[3 * r + cc] = w_elem(Yf, r, cc);
+        }
+    store_records_wave<R, kWRec>((R *)a.Yrec, nullptr, b, n, Yf, stg, lane);
     f64x4 acc = {0., 0., 0., 0.};
     for (int base = 0; base < n; base += 32) {
         const int nr = min(32, n - base);
@@ -1386,19 +1401,21 @@ __global__ __launch_bounds__(64 * kMfChunkWaves) void k_mf_chunks(PcgArgs a)
     // the entry's point, then the chunk's Y records (camera-major,
     // consecutive, read as whole lines) and the point's t behind them
     const int pq = a.cq_pt[b + min(lane, n - 1)];  // a chunk has at least one entry
-    __shared__ u32x4 stg[kMfChunkWaves][RecSpan<R>::LDS];
-    RecSpan<R> rw;
-    span_issue<R>(Yr, b, b + n - 1, rw, lane);
+    __shared__ u32x4 stg[kMfChunkWaves][RecSpan<R, kWRec>::LDS];
+    RecSpan<R, kWRec> rw;
+    span_issue<R, kWRec>(Yr, b, b + n - 1, rw, lane);
     const double *tq = a.tc + 3 * (size_t)pq;
     const double t0 = tq[0], t1 = tq[1], t2 = tq[2];
-    R Y[28];
-    span_transpose<R>(rw, n, Y, stg[wv], lane);
+    R Yf[kWRec];  // Y as Bt^T P: the Bt pairs, then P = Bp V^-1 (2 x 3)
+    span_transpose<R, kWRec>(rw, n, Yf, stg[wv], lane);
     double v[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) v[r] = 0.;
     if (lane < n) {
+        const double p0 = (double)Yf[18] * t0 + (double)Yf[19] * t1 + (double)Yf[20] * t2;
+        const double p1 = (double)Yf[21] * t0 + (double)Yf[22] * t1 + (double)Yf[23] * t2;
 #pragma unroll
-        for (int r = 0; r < 9; ++r) v[r] = (double)Y[3 * r] * t0 + (double)Y[3 * r + 1] * t1 + (double)Y[3 * r + 2] * t2;
+        for (int r = 0; r < 9; ++r) v[r] = (double)Yf[2 * r] * p0 + (double)Yf[2 * r + 1] * p1;
     }
 #pragma unroll
     for (int r = 0; r < 9; ++r) v[r] = wave_sum(v[r]);

@@ -1035,10 +1035,10 @@ static int ctx_init(rpba_ctx *c)
         }
         const double bup = std::min(0.5 * (double)nc * (double)(nc - 1), pairs);
         const double formed = 648. * (bup + (double)nc);
-        // matrix-free: both records (W as its factors), the image and point
-        // index of every entry, and t written and read per point
+        // matrix-free: both records (W and Y as their factors), the image and
+        // point index of every entry, and t written and read per point
         const double esz = c->rec32 ? 4. : 8.;
-        const double free_ = ((double)(kRec + kWRec) * esz + 8.) * (double)c->ds.ntr + 48. * (double)c->ds.npts;
+        const double free_ = ((double)(2 * kWRec) * esz + 8.) * (double)c->ds.ntr + 48. * (double)c->ds.npts;
         c->implicit = c->jform || (c->fl.solver == 0 && c->fl.preconditioner == 0 && free_ < formed);
         c->ds.with_pairs = !c->implicit;
     }
