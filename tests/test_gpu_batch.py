@@ -97,12 +97,14 @@ def test_batch_block_error_is_isolated():
     _compare_block(bad, flags, res[1], "failing block")
 
 
-def test_batch_fp32_records_match_single_block_solves():
+@pytest.mark.parametrize("solver", [1, 0])
+def test_batch_fp32_records_match_single_block_solves(solver):
     """precision = 1 in the batch: each block equals the single-block GPU
-    solve of the same form (FP32 records, dense solve) under the trajectory
-    rule (the two differ only in the order of the cost sums)."""
+    solve of the same form (FP32 records; dense solve, or the PCG segmented
+    by block) under the trajectory rule (the two differ only in the order of
+    the cost sums, and with the PCG in the order of its dot products)."""
     blocks = _blocks(1200, [(10, 400), (22, 900), (6, 250)])
-    flags = capi.make_flags(robust=1, solver=1, precision=1)
+    flags = capi.make_flags(robust=1, solver=solver, precision=1)
     res = api.calc_plsa_batch(blocks, flags)
     for i, (sc, (rc1, out1, st1, log1)) in enumerate(zip(blocks, res)):
         rc0, out0, st0, log0 = api.calc_plsa(sc, flags)
