@@ -114,3 +114,18 @@ def test_window_tracks_are_banded_and_distinct():
         # on the circle of 60 cameras, all of them fit in 10 consecutive ones
         gaps = np.diff(np.r_[c, c[0] + 60])
         assert 60 - gaps.max() <= 9
+
+
+def test_batch_form_selectors_are_rejected_without_gpu(tiny_scene):
+    """The batched blocks take the reference's PCG or the dense solve with S
+    formed, FP64 or FP32 records and block-Jacobi only: any other camera solve,
+    product, record form or preconditioner fails before any device call."""
+    from rpba_amd import api
+    L = rpba_amd.lib()
+    for kw in ({"solver": 2}, {"solver": -1}, {"schur": 1}, {"schur": 2}, {"precision": 2}, {"precision": 3},
+               {"precision": -1}, {"preconditioner": 1, "pc_omega": 0.5}, {"preconditioner": 2}):
+        with pytest.raises(RuntimeError, match=f"status {capi.RPBA_E_INVALID}"):
+            api.Batch([tiny_scene, tiny_scene], capi.make_flags(**kw))
+    h = C.c_void_p()
+    fl = capi.make_flags()
+    assert L.rpba_batch_create(C.byref(h), 0, None, C.byref(fl), capi.BATCH_LOG_FN(0), None) == capi.RPBA_E_INVALID
