@@ -7,7 +7,7 @@
  *
  * A from-scratch C restatement of the reference's algorithm (the reference is
  * C++ on Eigen 3, Boost and METIS, none of which exist in this image, so the
- * reference cannot be built).  It keeps the reference's data
+ * reference cannot be built here).  It keeps the reference's data
  * model: compact camera blocks of the sizes camparv gives (intrinsics only for
  * image 0, 8 for the gauge camera `normcamera`, 6+a for the others), the
  * camera-major observation numbering `nro`, the profile-style upper block
